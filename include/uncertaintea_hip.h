@@ -1,0 +1,197 @@
+/* uncertaintea_hip.h -- C ABI of libuncertaintea_hip, the MI355X (gfx950) device
+ * backend of UncertainTea's batched density and batched MCMC path.
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, and returns an
+ * int status (UT_OK on success).  On failure ut_last_error() returns a message for
+ * the calling thread.  Matrices follow the reference's column-major Julia layout:
+ * a "P x C" parameter matrix stores chain c's P unconstrained parameters
+ * contiguously at offset c*P.  All calls are synchronous: they return after the
+ * device work they launched has finished and results are in the caller's buffers.
+ *
+ * Reference interfaces replaced (paths are relative to the reference repository):
+ *   plans       <- DeviceExecutionPlan lowering + observation staging
+ *                  (src/device/plan/steps.jl:532-539, src/device/plan/lowering.jl:1360-1478,
+ *                   src/device/staging.jl:499-550)
+ *   workspace   <- DeviceBatchedWorkspace (src/device/workspace.jl:71-162)
+ *   density     <- device_batched_logjoint[!] / device_batched_logjoint_gradient[!]
+ *                  (src/device/api.jl:34-124)
+ *   samplers    <- batched_hmc / batched_nuts(tree_strategy=:persistent) device branches
+ *                  (src/inference/api_batched_hmc.jl:16-75, src/device/hmc_kernels.jl:306-723;
+ *                   src/inference/api_batched_nuts.jl:22-144,761-980,
+ *                   src/device/persistent_nuts.jl:226-786)
+ *   RNG         <- philox4x32 / device_rand_* (src/device/rng.jl:175-343)
+ */
+#ifndef UNCERTAINTEA_HIP_H
+#define UNCERTAINTEA_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------------
+ * UT_E_UNSUPPORTED   model/feature not lowerable (reference: ArgumentError naming
+ *                    device_lowering_report, src/device/workspace.jl:24-35)
+ * UT_E_DIM           wrong parameter count or chain count (reference:
+ *                    _signature_length_error / DimensionMismatch, src/device/api.jl:99-109)
+ * UT_E_PRECISION     unsupported precision (reference: src/device/workspace.jl:12-22)
+ * UT_E_HIP           a HIP runtime call failed (message carries hipGetErrorString)
+ * UT_E_NONFINITE_INIT initial positions non-finite after init_max_retries re-draws
+ *                    (reference: src/inference/api_batched_nuts.jl:184-200)
+ * UT_E_ARG           invalid argument value (reference: ArgumentError from the
+ *                    _validate_batched_* helpers)                                   */
+enum {
+  UT_OK = 0,
+  UT_E_UNSUPPORTED = 1,
+  UT_E_DIM = 2,
+  UT_E_PRECISION = 3,
+  UT_E_HIP = 4,
+  UT_E_NONFINITE_INIT = 5,
+  UT_E_ARG = 6
+};
+
+/* precision codes (reference: the `precision::Type` kwarg) */
+enum { UT_FLOAT32 = 0, UT_FLOAT64 = 1 };
+
+/* init strategies (reference: init_strategy = :prior | :uniform,
+ * src/inference/hmc_support.jl:266-290) */
+enum { UT_INIT_PRIOR = 0, UT_INIT_UNIFORM = 1 };
+
+typedef struct ut_plan ut_plan;
+typedef struct ut_workspace ut_workspace;
+
+/* Thread-local message for the last failing call on this thread. */
+const char* ut_last_error(void);
+/* ABI version: bumps on any incompatible change. */
+int32_t ut_abi_version(void);
+/* 1 when a HIP device is visible, 0 otherwise (no error). */
+int32_t ut_device_available(void);
+
+/* ---- plans: static model + staged observations ---------------------------------
+ * A plan is the device analogue of the reference's lowered DeviceExecutionPlan
+ * plus its staged observation stream: the model structure is fixed at creation
+ * and the data are copied to HBM once, immutable for the plan's lifetime.       */
+
+/* Bayesian logistic regression without intercept:
+ *   beta_d ~ normal(prior_loc, prior_scale), d = 1..D
+ *   y_i ~ bernoullilogit(x_i . beta), i = 1..N
+ * x is N x D row-major; y holds 0/1.  P = D.
+ * Reference: GLM step lowering src/device/plan/lowering.jl:664-675, analytic GLM
+ * gradient src/device/gradient_kernel.jl:578-657, bernoullilogit
+ * src/device/math.jl:202-219.                                                   */
+int ut_plan_logistic_glm(const float* x, const float* y, int64_t n, int32_t d,
+                         double prior_loc, double prior_scale, ut_plan** out);
+
+/* Non-centred hierarchical normal (eight-schools form):
+ *   mu ~ normal(mu_loc, mu_scale); tau ~ halfcauchy(tau_scale) (log transform);
+ *   theta_raw_j ~ normal(0, 1); theta_j = mu + tau * theta_raw_j;
+ *   y_ij ~ normal(theta_j, sigma_j).
+ * Observations of group j are y[group_offsets[j] .. group_offsets[j+1]).
+ * Parameter order: (mu, log tau, theta_raw_1..J); P = J + 2.  J <= 64.
+ * Reference families: normal src/device/math.jl:88-91, halfcauchy :126-130,
+ * log transform :1144-1159.                                                     */
+int ut_plan_hier_normal_nc(const float* y, const int32_t* group_offsets,
+                           const double* sigma, int32_t num_groups, double mu_loc,
+                           double mu_scale, double tau_scale, ut_plan** out);
+
+/* Neal's funnel: v ~ normal(0, v_scale); x_i ~ normal(0, exp(v/2)), i = 1..dim.
+ * Parameter order: (v, x_1..x_dim); P = dim + 1.                                */
+int ut_plan_funnel(int32_t dim, double v_scale, ut_plan** out);
+
+int ut_plan_destroy(ut_plan* plan);
+int ut_plan_num_params(const ut_plan* plan, int32_t* num_params);
+
+/* ---- workspace -----------------------------------------------------------------
+ * Owns the device buffers for num_chains chains of one plan, one HIP stream and
+ * the sampler state.  Not thread-safe: one caller at a time per workspace.      */
+int ut_workspace_create(const ut_plan* plan, int32_t num_chains, int32_t precision,
+                        ut_workspace** out);
+int ut_workspace_destroy(ut_workspace* ws);
+
+/* ---- density -------------------------------------------------------------------
+ * params: P x C unconstrained parameters in the workspace precision (float or
+ * double per the precision code).  Outputs the unconstrained logjoint (log density
+ * plus transform log-abs-det) and its gradient, in the same precision.
+ * Reference: device_batched_logjoint!/device_batched_logjoint_gradient!
+ * (src/device/api.jl:34-51, 98-124).                                            */
+int ut_logjoint(ut_workspace* ws, const void* params, void* values);
+int ut_logjoint_gradient(ut_workspace* ws, const void* params, void* values,
+                         void* gradients);
+
+/* ---- samplers ------------------------------------------------------------------
+ * Options mirror the keyword arguments of batched_nuts / batched_hmc
+ * (src/inference/api_batched_nuts.jl:22-51, api_batched_hmc.jl:16-39).          */
+typedef struct ut_sampler_opts {
+  int32_t num_samples;
+  int32_t num_warmup;
+  int32_t max_tree_depth;         /* NUTS (default 10) */
+  int32_t num_leapfrog_steps;     /* HMC (default 10) */
+  double step_size;               /* initial step size (NUTS/HMC default 0.1) */
+  double target_accept;           /* 0.8 */
+  double divergence_threshold;    /* 1000 */
+  double mass_matrix_regularization; /* 1e-3 */
+  int32_t mass_matrix_min_samples;   /* 10 */
+  int32_t adapt_step_size;        /* bool, default 1 */
+  int32_t adapt_mass_matrix;      /* bool, default 1 */
+  int32_t find_reasonable_step_size; /* bool, default 0 */
+  int32_t init_strategy;          /* UT_INIT_PRIOR / UT_INIT_UNIFORM */
+  int32_t init_max_retries;       /* 100 */
+  const double* initial_params;   /* P x C or NULL */
+  uint64_t seed;                  /* run seed; draws are a pure function of it */
+} ut_sampler_opts;
+
+/* Fills opts with the reference defaults (num_samples/num_warmup zero). */
+void ut_sampler_opts_default(ut_sampler_opts* opts);
+
+/* Per-chain results, laid out like the reference's HMCChain fields stacked over
+ * chains (src/inference/data_types.jl:55-80).  samples is P x S x C in the
+ * workspace precision (float for UT_FLOAT32); S x C matrices are column-major.
+ * Any pointer except samples may be NULL to skip that output.                   */
+typedef struct ut_chain_out {
+  void* samples;               /* P x S x C unconstrained draws */
+  double* logjoint;            /* S x C */
+  double* accept_stat;         /* S x C */
+  double* energy;              /* S x C */
+  double* energy_error;        /* S x C */
+  uint8_t* accepted;           /* S x C */
+  uint8_t* divergent;          /* S x C */
+  int32_t* tree_depth;         /* S x C (NUTS) */
+  int32_t* integration_steps;  /* S x C */
+  double* step_size;           /* C: final adapted step per chain */
+  double* inverse_mass;        /* P: final shared diagonal inverse mass */
+  double warmup_seconds;       /* wall time of initialisation + warmup */
+  double sampling_seconds;     /* wall time of sampling incl. copy-out */
+  int64_t sampling_gradient_evals; /* leapfrog gradient evaluations while sampling */
+  double sampling_kernel_seconds;  /* device time of the sampling kernels (HIP events) */
+  int32_t sampling_launches;       /* number of sampling kernel launches */
+} ut_chain_out;
+
+/* Persistent-kernel NUTS: every chain builds its whole multinomial NUTS tree in
+ * one kernel; warmup uses per-chain dual averaging and a pooled diagonal mass
+ * (reference _batched_nuts_device_per_chain!, api_batched_nuts.jl:761-980).    */
+int ut_nuts_run(ut_workspace* ws, const ut_sampler_opts* opts, ut_chain_out* out);
+
+/* Fixed-length HMC with per-chain step and pooled diagonal mass
+ * (reference _run_device_batched_hmc, src/device/hmc_kernels.jl:306-723).      */
+int ut_hmc_run(ut_workspace* ws, const ut_sampler_opts* opts, ut_chain_out* out);
+
+/* ---- RNG -----------------------------------------------------------------------
+ * Host evaluation of one Philox4x32-10 block (reference philox4x32, rng.jl:219). */
+void ut_philox4x32(const uint32_t key[2], const uint32_t ctr[4], uint32_t out[4]);
+
+/* Device fill for the determinism checks: for i in [0, n):
+ *   blocks[4i..4i+3] = philox4x32((chain, iteration), (stream, first + i, 0, 0))
+ *   uniform_f32[i]  = device_rand_uniform(Float32, ...) of the same coordinate
+ *   normal_f32[i]   = device_rand_normal(Float32, ...) of the same coordinate
+ *   uniform_f64[i]  = device_rand_uniform(Float64, ...) of the same coordinate
+ * Output arrays are host pointers; any may be NULL.                             */
+int ut_philox_fill(uint32_t chain, uint32_t iteration, uint32_t stream, uint32_t first,
+                   int64_t n, uint32_t* blocks, float* uniform_f32, float* normal_f32,
+                   double* uniform_f64);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* UNCERTAINTEA_HIP_H */

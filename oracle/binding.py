@@ -1,0 +1,223 @@
+"""ctypes binding of the CPU oracle (oracle/_build/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() as the
+checker and by bench.py's cpu_baseline leg.  The product never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "_build" / "liboracle.so"
+
+_lib = None
+
+
+def _load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(f"oracle not built: {LIB_PATH} (run make -C oracle)")
+    L = ct.CDLL(str(LIB_PATH))
+    vp, i32, i64, u32, dbl = ct.c_void_p, ct.c_int32, ct.c_int64, ct.c_uint32, ct.c_double
+    sig = {
+        "orc_philox4x32": (None, [vp, vp, vp]),
+        "orc_philox_fill": (None, [u32, u32, u32, u32, i64, vp, vp, vp, vp]),
+        "orc_math_f32": (None, [ct.c_int, vp, vp, i64]),
+        "orc_math_f64": (None, [ct.c_int, vp, vp, i64]),
+        "orc_model_logistic": (vp, [vp, vp, i64, i32, dbl, dbl]),
+        "orc_model_hier": (vp, [vp, vp, vp, i32, dbl, dbl, dbl]),
+        "orc_model_funnel": (vp, [i32, dbl]),
+        "orc_model_free": (None, [vp]),
+        "orc_model_num_params": (i32, [vp]),
+        "orc_logjoint_grad_f64": (None, [vp, vp, i32, vp, vp]),
+        "orc_logjoint_grad_twin": (None, [vp, i32, vp, i32, vp, vp]),
+        "orc_sampler_run": (ct.c_int, [vp, i32, i32, vp, i32, i32, vp, vp, i32]),
+        "orc_warmup_schedule": (i32, [i32, vp, vp, vp, i32]),
+        "orc_nuts_sample_from": (i64, [vp, i32, i32, i32, vp, vp, vp, i32, i32, ct.c_uint64, i32]),
+        "orc_dual_averaging": (None, [dbl, dbl, vp, i32, vp, vp]),
+        "orc_running_variance": (dbl, [vp, vp, vp, i32, i32, i32, i32, i32, dbl, vp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ct.c_void_p)
+
+
+def philox4x32(key, ctr):
+    k = np.ascontiguousarray(key, dtype=np.uint32)
+    c = np.ascontiguousarray(ctr, dtype=np.uint32)
+    out = np.zeros(4, dtype=np.uint32)
+    _load().orc_philox4x32(_p(k), _p(c), _p(out))
+    return out
+
+
+def philox_fill(chain, iteration, stream, first, n):
+    blocks = np.zeros((n, 4), dtype=np.uint32)
+    uf = np.zeros(n, dtype=np.float32)
+    nf = np.zeros(n, dtype=np.float32)
+    ud = np.zeros(n, dtype=np.float64)
+    _load().orc_philox_fill(chain, iteration, stream, first, n, _p(blocks), _p(uf), _p(nf), _p(ud))
+    return blocks, uf, nf, ud
+
+
+MATH = {"exp": 0, "log": 1, "log1p": 2, "cos": 3, "log1p_exp": 4, "logistic": 5, "pow_m075": 6}
+
+
+def math_f32(fn: str, x):
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    y = np.empty_like(x)
+    _load().orc_math_f32(MATH[fn], _p(x), _p(y), x.size)
+    return y
+
+
+def math_f64(fn: str, x):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.empty_like(x)
+    _load().orc_math_f64(MATH[fn], _p(x), _p(y), x.size)
+    return y
+
+
+class OracleModel:
+    """Oracle twin of a uncertaintea_amd model (same data, same layout)."""
+
+    def __init__(self, model):
+        L = _load()
+        self.model = model
+        kind = type(model).__name__
+        if kind == "LogisticRegression":
+            self._keep = (model.x, model.y)
+            self.h = L.orc_model_logistic(_p(model.x), _p(model.y), model.x.shape[0], model.x.shape[1],
+                                          model.prior_loc, model.prior_scale)
+        elif kind == "HierarchicalNormalNC":
+            self._keep = (model.y, model.offsets, model.sigma)
+            self.h = L.orc_model_hier(_p(model.y), _p(model.offsets), _p(model.sigma), model.num_groups,
+                                      model.mu_loc, model.mu_scale, model.tau_scale)
+        elif kind == "NealsFunnel":
+            self.h = L.orc_model_funnel(model.dim, model.v_scale)
+        else:
+            raise TypeError(kind)
+        self.P = L.orc_model_num_params(self.h)
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.orc_model_free(self.h)
+            self.h = None
+
+    def logjoint_grad_f64(self, params):
+        """Reference-formula binary64 values/gradients for P x C params."""
+        p = np.ascontiguousarray(np.asarray(params, dtype=np.float64).T)
+        C = p.shape[0]
+        lj = np.empty(C)
+        g = np.empty((C, self.P))
+        _load().orc_logjoint_grad_f64(self.h, _p(p), C, _p(lj), _p(g))
+        return lj, g.T
+
+    def logjoint_grad_twin(self, params, precision=np.float32):
+        """Kernel-exact values/gradients (float32 or float64) for P x C params."""
+        dt = np.dtype(precision)
+        p = np.ascontiguousarray(np.asarray(params, dtype=dt).T)
+        C = p.shape[0]
+        lj = np.empty(C, dtype=dt)
+        g = np.empty((C, self.P), dtype=dt)
+        _load().orc_logjoint_grad_twin(self.h, 0 if dt == np.float32 else 1, _p(p), C, _p(lj), _p(g))
+        return lj, g.T
+
+    def run(self, kind: str, mode: str, num_chains: int, *, num_samples: int, num_warmup: int = 0,
+            step_size: float = 0.1, max_tree_depth: int = 10, num_leapfrog_steps: int = 10,
+            initial_params=None, init_strategy: str = "prior", target_accept: float = 0.8, seed: int = 0,
+            adapt_step_size=True, adapt_mass_matrix=True, find_reasonable_step_size=False, threads: int = 0,
+            stop_after: int = -1, divergence_threshold=1000.0):
+        """Run the oracle sampler.  kind 'nuts'|'hmc'; mode 'twin' (float32,
+        kernel-exact) or 'f64' (binary64 reference formulas).  Returns a dict."""
+        from uncertaintea_amd import _lib as UL  # struct layouts only
+        o = UL.SamplerOpts()
+        o.num_samples, o.num_warmup = num_samples, num_warmup
+        o.max_tree_depth, o.num_leapfrog_steps = max_tree_depth, num_leapfrog_steps
+        o.step_size, o.target_accept, o.divergence_threshold = step_size, target_accept, divergence_threshold
+        o.mass_matrix_regularization, o.mass_matrix_min_samples = 1e-3, 10
+        o.adapt_step_size, o.adapt_mass_matrix = int(adapt_step_size), int(adapt_mass_matrix)
+        o.find_reasonable_step_size = int(find_reasonable_step_size)
+        o.init_strategy = 0 if init_strategy == "prior" else 1
+        o.init_max_retries = 100
+        o.seed = seed
+        C, P, S = num_chains, self.P, num_samples
+        init = None
+        if initial_params is not None:
+            init = np.ascontiguousarray(np.asarray(initial_params, dtype=np.float64).T)
+            o.initial_params = init.ctypes.data_as(ct.POINTER(ct.c_double))
+        dt = np.float32 if mode == "twin" else np.float64
+        samples = np.zeros((C, S, P), dtype=dt)
+        res = {k: np.zeros((C, S)) for k in ("logjoint", "accept_stat", "energy", "energy_error")}
+        acc = np.zeros((C, S), dtype=np.uint8)
+        div = np.zeros((C, S), dtype=np.uint8)
+        depth = np.zeros((C, S), dtype=np.int32)
+        steps = np.zeros((C, S), dtype=np.int32)
+        step = np.zeros(C)
+        im = np.zeros(P)
+        out = UL.ChainOut()
+        out.samples = samples.ctypes.data_as(ct.c_void_p)
+        for k in res:
+            setattr(out, k, res[k].ctypes.data_as(ct.POINTER(ct.c_double)))
+        out.accepted = acc.ctypes.data_as(ct.POINTER(ct.c_uint8))
+        out.divergent = div.ctypes.data_as(ct.POINTER(ct.c_uint8))
+        out.tree_depth = depth.ctypes.data_as(ct.POINTER(ct.c_int32))
+        out.integration_steps = steps.ctypes.data_as(ct.POINTER(ct.c_int32))
+        out.step_size = step.ctypes.data_as(ct.POINTER(ct.c_double))
+        out.inverse_mass = im.ctypes.data_as(ct.POINTER(ct.c_double))
+        rc = _load().orc_sampler_run(self.h, 0 if kind == "nuts" else 1, 0 if mode == "twin" else 1,
+                                     ct.byref(o), C, threads, _p(samples), ct.byref(out), stop_after)
+        if rc != 0:
+            raise RuntimeError(f"oracle sampler failed with status {rc}")
+        res.update(samples=samples, accepted=acc.astype(bool), divergent=div.astype(bool), tree_depth=depth,
+                   integration_steps=steps, step_size=step, inverse_mass=im)
+        return res
+
+
+def nuts_sample_from(om: OracleModel, mode: str, positions, steps, inverse_mass, first_iteration: int,
+                     iters: int, seed: int = 0, max_depth: int = 10, threads: int = 0):
+    """Run `iters` NUTS sampling iterations (no adaptation) from `positions` (P x C,
+    updated in place); returns the number of leapfrog gradient evaluations."""
+    pos = np.ascontiguousarray(np.asarray(positions, dtype=np.float64).T)
+    st = np.ascontiguousarray(steps, dtype=np.float64)
+    im = np.ascontiguousarray(inverse_mass, dtype=np.float64)
+    C = pos.shape[0]
+    n = _load().orc_nuts_sample_from(om.h, 0 if mode == "twin" else 1, C, threads, _p(pos), _p(st), _p(im),
+                                     first_iteration, iters, seed, max_depth)
+    return int(n), pos.T
+
+
+def warmup_schedule(n: int):
+    ib = ct.c_int32()
+    tb = ct.c_int32()
+    ends = np.zeros(64, dtype=np.int32)
+    k = _load().orc_warmup_schedule(n, ct.byref(ib), ct.byref(tb), _p(ends), 64)
+    return ib.value, [int(e) for e in ends[:k]], tb.value
+
+
+def dual_averaging(step: float, target: float, accepts):
+    a = np.ascontiguousarray(accepts, dtype=np.float64)
+    steps = np.empty_like(a)
+    fin = ct.c_double()
+    _load().orc_dual_averaging(step, target, _p(a), a.size, _p(steps), ct.byref(fin))
+    return steps, fin.value
+
+
+def running_variance(hist, acc, flags, window_length, use_accepted=False, reg=1e-3):
+    h = np.ascontiguousarray(hist, dtype=np.float64)
+    it, C, P = h.shape
+    a = np.ascontiguousarray(acc, dtype=np.float64)
+    f = np.ascontiguousarray(flags, dtype=np.uint8)
+    im = np.empty(P)
+    eff = _load().orc_running_variance(_p(h), _p(a), _p(f), it, C, P, window_length, int(use_accepted), reg, _p(im))
+    return im, eff

@@ -1,0 +1,1221 @@
+// oracle.cpp -- CPU restatement of the reference's batched density and batched MCMC
+// algorithms.  TEST INFRASTRUCTURE ONLY: the tests, the smoke check and the
+// benchmark's CPU-baseline leg load this library as the checker; the product
+// (libuncertaintea_hip) never calls it.
+//
+// Two evaluation policies share one sampler restatement:
+//   * RefF64  -- binary64 throughout, models written with the reference's own
+//                per-observation formulas (src/device/math.jl:88-130,202-224) and
+//                plain sequential sums.  This is the accuracy oracle and the
+//                multi-threaded "reference CPU path" timed by the benchmark.
+//   * TwinF32 -- binary32 positions/momenta/gradients with binary64 energies,
+//                evaluated in exactly the arithmetic order of the gfx950 kernels
+//                (per-lane partial sums over register slots, then the xor
+//                butterfly over 64 lanes).  Driven by the same Philox coordinates,
+//                it must reproduce the kernels bit for bit, including every NUTS
+//                tree-depth and divergence decision.
+//
+// Reference algorithms restated (paths relative to the reference repository):
+//   NUTS transition       src/device/persistent_nuts.jl:226-492
+//                          (= src/inference/nuts/tree_dynamics.jl:1-175,
+//                             tree_expand.jl:64-133, state_init.jl:284-330)
+//   HMC transition        src/device/hmc_kernels.jl:218-280 (leapfrog),
+//                          :517-577 (Hamiltonian, accept, divergence)
+//   step-size search      src/inference/nuts/adaptation.jl:437-520
+//   dual averaging        src/inference/nuts/adaptation.jl:1-28
+//   warmup schedule       src/inference/nuts/adaptation.jl:103-138
+//   pooled mass           src/inference/warmup.jl:262-472,
+//                          src/inference/nuts/adaptation.jl:30-41,140-260,
+//                          src/inference/data_types.jl:189-193
+//   sampler drivers       src/inference/api_batched_nuts.jl:761-980,
+//                          src/device/hmc_kernels.jl:306-723
+#include <omp.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "../include/uncertaintea_hip.h"
+#include "../uncertaintea_amd/csrc/ut_math.h"
+
+using namespace ut;
+
+namespace {
+
+constexpr double HALF_LOG_2PI = 0.9189385332046728;
+constexpr double LOG_PI = 1.1447298858494002;
+constexpr double LOG_2 = 0.6931471805599453;
+enum { KIND_LOGISTIC = 1, KIND_HIER = 2, KIND_FUNNEL = 3 };
+enum { S_MOM = 0, S_DIR = 1, S_LEAF = 2, S_MERGE = 3, S_SEARCH = 4, S_INIT = 5, S_ACCEPT = 6 };
+
+// ------------------------------------------------------------------ model data
+struct Model {
+  int kind = 0;
+  int P = 0;
+  // logistic
+  std::vector<float> x;  // N x D row-major
+  std::vector<float> y;
+  long long n = 0;
+  int D = 0;
+  double loc = 0, scale = 1;
+  // hierarchical
+  std::vector<float> hy;
+  std::vector<int> offsets;
+  std::vector<double> sigma;
+  int J = 0;
+  double mu_loc = 0, mu_scale = 1, tau_scale = 1;
+  // funnel
+  int dim = 0;
+  double v_scale = 1;
+  // twin layout
+  int K = 1;
+  double total_const = 0;
+  std::vector<float> ty_f;      // hierarchical y[i*64 + j]
+  std::vector<int> tnobs;
+  std::vector<double> tlane;    // [3][64], binary64 (binary32 kernels round each)
+  int max_n = 0;
+
+  int api(int lane, int slot) const {
+    const int i = slot * WAVE + lane;
+    switch (kind) {
+      case KIND_LOGISTIC:
+        return i < D ? i : -1;
+      case KIND_HIER:
+        if (slot == 0) return lane < J ? 2 + lane : -1;
+        if (slot == 1) return lane == 0 ? 0 : (lane == 1 ? 1 : -1);
+        return -1;
+      case KIND_FUNNEL:
+        if (i < dim) return 1 + i;
+        return i == dim ? 0 : -1;
+    }
+    return -1;
+  }
+};
+
+int slots_for(int P) {
+  int k = (P + WAVE - 1) / WAVE;
+  if (k <= 1) return 1;
+  if (k <= 2) return 2;
+  return 4;
+}
+
+// ------------------------------------------------------------------ RefF64 models
+// log N(x; mu, sigma) as the reference writes it (math.jl:88-91)
+inline double normal_lpdf(double mu, double sigma, double x) {
+  const double z = (x - mu) / sigma;
+  return -std::log(sigma) - HALF_LOG_2PI - z * z / 2.0;
+}
+inline double log1p_exp_ref(double eta) {
+  return eta > 0.0 ? eta + std::log1p(std::exp(-eta)) : std::log1p(std::exp(eta));
+}
+
+// logjoint + gradient in api order (binary64, reference formulas)
+double ref_eval(const Model& m, const double* q, double* g, bool& ok) {
+  double lp = 0.0;
+  const int P = m.P;
+  for (int a = 0; a < P; ++a) g[a] = 0.0;
+  if (m.kind == KIND_LOGISTIC) {
+    for (int d = 0; d < m.D; ++d) {
+      lp += normal_lpdf(m.loc, m.scale, q[d]);
+      g[d] += -(q[d] - m.loc) / (m.scale * m.scale);
+    }
+    for (long long i = 0; i < m.n; ++i) {
+      double eta = 0.0;
+      for (int d = 0; d < m.D; ++d) eta += (double)m.x[i * m.D + d] * q[d];
+      const double yi = m.y[i];
+      lp += yi * eta - log1p_exp_ref(eta);
+      const double r = yi - 1.0 / (1.0 + std::exp(-eta));
+      for (int d = 0; d < m.D; ++d) g[d] += (double)m.x[i * m.D + d] * r;
+    }
+  } else if (m.kind == KIND_HIER) {
+    const double mu = q[0], ltau = q[1], tau = std::exp(ltau);
+    lp += normal_lpdf(m.mu_loc, m.mu_scale, mu);
+    g[0] += -(mu - m.mu_loc) / (m.mu_scale * m.mu_scale);
+    const double zt = tau / m.tau_scale;
+    lp += LOG_2 - LOG_PI - std::log(m.tau_scale) - std::log1p(zt * zt) + ltau;
+    g[1] += -2.0 * zt * zt / (1.0 + zt * zt) + 1.0;
+    for (int j = 0; j < m.J; ++j) {
+      const double r = q[2 + j];
+      lp += normal_lpdf(0.0, 1.0, r);
+      g[2 + j] += -r;
+      const double theta = mu + tau * r;
+      double gth = 0.0;
+      for (int i = m.offsets[j]; i < m.offsets[j + 1]; ++i) {
+        lp += normal_lpdf(theta, m.sigma[j], m.hy[i]);
+        gth += (m.hy[i] - theta) / (m.sigma[j] * m.sigma[j]);
+      }
+      g[0] += gth;
+      g[1] += gth * tau * r;
+      g[2 + j] += gth * tau;
+    }
+  } else if (m.kind == KIND_FUNNEL) {
+    const double v = q[0];
+    lp += normal_lpdf(0.0, m.v_scale, v);
+    g[0] += -v / (m.v_scale * m.v_scale);
+    const double s = std::exp(v / 2.0);
+    for (int i = 0; i < m.dim; ++i) {
+      const double x = q[1 + i];
+      lp += normal_lpdf(0.0, s, x);
+      g[1 + i] += -x / (s * s);
+      g[0] += 0.5 * x * x / (s * s) - 0.5;
+    }
+  }
+  ok = std::isfinite(lp);
+  for (int a = 0; a < P && ok; ++a) ok = std::isfinite(g[a]);
+  return lp;
+}
+
+// ------------------------------------------------------------------ TwinF32 models
+// Vectors are K x 64 (slot-major); bfly() is the kernels' xor butterfly.
+template <class T>
+T bfly(const T* in) {
+  T v[WAVE], w[WAVE];
+  for (int l = 0; l < WAVE; ++l) v[l] = in[l];
+  for (int mm = 1; mm < WAVE; mm <<= 1) {
+    for (int l = 0; l < WAVE; ++l) w[l] = v[l] + v[l ^ mm];
+    for (int l = 0; l < WAVE; ++l) v[l] = w[l];
+  }
+  return v[0];
+}
+
+template <class T> T texp(T x);
+template <> float texp<float>(float x) { return exp_f(x); }
+template <> double texp<double>(double x) { return exp_d(x); }
+template <class T> T tlog1p(T x);
+template <> float tlog1p<float>(float x) { return log1p_f(x); }
+template <> double tlog1p<double>(double x) { return log1p_d(x); }
+template <class T> T tl1pe(T x);
+template <> float tl1pe<float>(float x) { return log1p_exp_f(x); }
+template <> double tl1pe<double>(double x) { return log1p_exp_d(x); }
+template <class T> T tlogistic(T x);
+template <> float tlogistic<float>(float x) { return logistic_f(x); }
+template <> double tlogistic<double>(double x) { return logistic_d(x); }
+template <class T> bool tfin(T x) { return std::isfinite(x); }
+
+template <class T>
+T twin_eval(const Model& m, const T* q, T* g, bool& ok) {
+  const int K = m.K;
+  bool fin = true;
+  if (m.kind == KIND_HIER) {
+    const T mu = q[1 * WAVE + 0], ltau = q[1 * WAVE + 1];
+    const T tau = texp<T>(ltau);
+    T gth[WAVE], gq[WAVE], lpl[WAVE];
+    for (int l = 0; l < WAVE; ++l) {
+      const T qr = q[l];
+      const T theta = mu + tau * qr;
+      const int n = m.tnobs[l];
+      T sd = T(0), ss = T(0);
+      for (int i = 0; i < m.max_n; ++i) {
+        if (i < n) {
+          const T d = T(m.ty_f[i * WAVE + l]) - theta;
+          sd = sd + d;
+          ss = ss + d * d;
+        }
+      }
+      const T cj = T(m.tlane[l]), hinv = T(m.tlane[WAVE + l]), inv = T(m.tlane[2 * WAVE + l]);
+      T lp = cj - ss * hinv;
+      gth[l] = sd * inv;
+      lp = lp - T(0.5) * qr * qr;
+      g[l] = tau * gth[l] - qr;
+      gq[l] = gth[l] * qr;
+      lpl[l] = lp;
+    }
+    const T s1 = bfly(gth), s2 = bfly(gq), sl = bfly(lpl);
+    const T zmu = (mu - T(m.mu_loc)) / T(m.mu_scale);
+    const T lp_mu = -(zmu * zmu) / T(2);
+    const T g_mu = s1 - zmu / T(m.mu_scale);
+    const T zt = tau / T(m.tau_scale);
+    const T z2 = zt * zt;
+    const T lp_tau = ltau - tlog1p<T>(z2);
+    const T g_ltau = tau * s2 - (T(2) * z2) / (T(1) + z2) + T(1);
+    const T total = ((sl + lp_mu) + lp_tau) + T(m.total_const);
+    for (int l = 0; l < WAVE; ++l) g[WAVE + l] = l == 0 ? g_mu : (l == 1 ? g_ltau : T(0));
+    for (int i = 0; i < 2 * WAVE; ++i) fin = fin && tfin(g[i]);
+    ok = fin && tfin(total);
+    return total;
+  }
+  if (m.kind == KIND_LOGISTIC) {
+    const int DMAX = K * WAVE;
+    std::vector<T> beta(DMAX), gp((size_t)DMAX * WAVE, T(0)), lpl(WAVE, T(0));
+    for (int d = 0; d < DMAX; ++d) beta[d] = q[(d / WAVE) * WAVE + d % WAVE];
+    for (int l = 0; l < WAVE; ++l) {
+      T lp = T(0);
+      T* gpl = &gp[(size_t)l * DMAX];
+      for (long long i = l; i < m.n; i += WAVE) {
+        T eta = T(0);
+        for (int d = 0; d < m.D; ++d) eta = eta + T(m.x[i * m.D + d]) * beta[d];
+        const T yi = T(m.y[i]);
+        lp = lp + (yi * eta - tl1pe<T>(eta));
+        const T r = yi - tlogistic<T>(eta);
+        for (int d = 0; d < m.D; ++d) gpl[d] = gpl[d] + T(m.x[i * m.D + d]) * r;
+      }
+      for (int k = 0; k < K; ++k) {
+        if (m.api(l, k) >= 0) {
+          const T z = (q[k * WAVE + l] - T(m.loc)) / T(m.scale);
+          lp = lp - (z * z) / T(2);
+        }
+      }
+      lpl[l] = lp;
+    }
+    std::vector<T> gl((size_t)K * WAVE, T(0));
+    T col[WAVE];
+    for (int d = 0; d < m.D; ++d) {
+      for (int l = 0; l < WAVE; ++l) col[l] = gp[(size_t)l * DMAX + d];
+      gl[(d / WAVE) * WAVE + d % WAVE] = bfly(col);
+    }
+    for (int k = 0; k < K; ++k)
+      for (int l = 0; l < WAVE; ++l) {
+        const int i = k * WAVE + l;
+        if (m.api(l, k) >= 0) {
+          const T z = (q[i] - T(m.loc)) / T(m.scale);
+          g[i] = gl[i] - z / T(m.scale);
+        } else {
+          g[i] = T(0);
+        }
+        fin = fin && tfin(g[i]);
+      }
+    const T total = bfly(lpl.data()) + T(m.total_const);
+    ok = fin && tfin(total);
+    return total;
+  }
+  // funnel
+  const T v = q[(m.dim / WAVE) * WAVE + m.dim % WAVE];
+  const T ev = texp<T>(-v);
+  T spl[WAVE];
+  for (int l = 0; l < WAVE; ++l) {
+    T sp = T(0);
+    for (int k = 0; k < K; ++k) {
+      const int i = k * WAVE + l;
+      if (i < m.dim) {
+        const T s = (q[i] * q[i]) * ev;
+        sp = sp + s;
+        g[i] = -(q[i] * ev);
+      }
+    }
+    spl[l] = sp;
+  }
+  const T S = bfly(spl);
+  const T zv = v / T(m.v_scale);
+  const T gv = (S / T(2) - T(m.dim) / T(2)) - zv / T(m.v_scale);
+  const T total = ((-(S / T(2)) - T(m.dim) * (v / T(2))) - (zv * zv) / T(2)) + T(m.total_const);
+  for (int k = 0; k < K; ++k)
+    for (int l = 0; l < WAVE; ++l) {
+      const int i = k * WAVE + l;
+      if (i == m.dim) g[i] = gv;
+      else if (i > m.dim) g[i] = T(0);
+      fin = fin && tfin(g[i]);
+    }
+  ok = fin && tfin(total);
+  return total;
+}
+
+// ------------------------------------------------------------------ policies
+// A policy owns the vector layout, the reductions and the model evaluation.
+struct TwinF32 {
+  using T = float;
+  const Model* m;
+  int n() const { return m->K * WAVE; }
+  int api(int i) const { return m->api(i % WAVE, i / WAVE); }
+  T eval(const T* q, T* g, bool& ok) const { return twin_eval<float>(*m, q, g, ok); }
+  double kinetic(const T* p, const T* im) const {
+    double lane[WAVE];
+    for (int l = 0; l < WAVE; ++l) {
+      double acc = 0.0;
+      for (int k = 0; k < m->K; ++k) {
+        const int i = k * WAVE + l;
+        acc = acc + ((double)p[i] * (double)p[i]) * (double)im[i];
+      }
+      lane[l] = acc;
+    }
+    return bfly(lane) * 0.5;
+  }
+  // sum_i (a_i * im_i) * b_i and (a_i * im_i) * c_i with a = x - y, per-lane then butterfly
+  bool turning(const T* x, const T* y, const T* pl, const T* pr, const T* im) const {
+    float la[WAVE], ra[WAVE];
+    for (int l = 0; l < WAVE; ++l) {
+      float a = 0.0f, b = 0.0f;
+      for (int k = 0; k < m->K; ++k) {
+        const int i = k * WAVE + l;
+        const float delta = x[i] - y[i];
+        a = a + (delta * im[i]) * pl[i];
+        b = b + (delta * im[i]) * pr[i];
+      }
+      la[l] = a;
+      ra[l] = b;
+    }
+    const float ld = bfly(la), rd = bfly(ra);
+    return ld <= 0.0f || rd <= 0.0f;
+  }
+};
+
+struct RefF64 {
+  using T = double;
+  const Model* m;
+  int n() const { return m->P; }
+  int api(int i) const { return i; }
+  T eval(const T* q, T* g, bool& ok) const { return ref_eval(*m, q, g, ok); }
+  double kinetic(const T* p, const T* im) const {
+    double acc = 0.0;
+    for (int i = 0; i < m->P; ++i) acc += p[i] * p[i] * im[i];
+    return acc * 0.5;
+  }
+  bool turning(const T* x, const T* y, const T* pl, const T* pr, const T* im) const {
+    double a = 0.0, b = 0.0;
+    for (int i = 0; i < m->P; ++i) {
+      const double delta = x[i] - y[i];
+      a += delta * im[i] * pl[i];
+      b += delta * im[i] * pr[i];
+    }
+    return a <= 0.0 || b <= 0.0;
+  }
+};
+
+template <class T> T sqrt_t(T x);
+template <> float sqrt_t<float>(float x) { return sqrt_f(x); }
+template <> double sqrt_t<double>(double x) { return sqrt_d(x); }
+template <class T> T draw_normal(uint32_t c, uint32_t itc, uint32_t s, uint32_t a) {
+  return (T)rand_normal_f(c, itc, s, a);
+}
+
+struct TransOut {
+  double accept_prob = 0, energy = 0, error = 0;
+  int depth = 0, steps = 0;
+  bool divergent = false, moved = false;
+};
+
+// ------------------------------------------------------------------ NUTS
+// One chain's multinomial NUTS transition; vectors have pol.n() entries.
+template <class Pol>
+void nuts_transition(const Pol& pol, const typename Pol::T* im, typename Pol::T h, uint32_t chain,
+                     uint32_t itc, int max_depth, double div_thr, typename Pol::T* q,
+                     typename Pol::T* g, typename Pol::T& lj, TransOut& out) {
+  using T = typename Pol::T;
+  const int n = pol.n();
+  std::vector<T> buf((size_t)n * 16);
+  T* p = &buf[0 * n];
+  T* q0 = &buf[1 * n];
+  T* Lq = &buf[2 * n]; T* Lp = &buf[3 * n]; T* Lg = &buf[4 * n];
+  T* Rq = &buf[5 * n]; T* Rp = &buf[6 * n]; T* Rg = &buf[7 * n];
+  T* sq = &buf[8 * n]; T* sg = &buf[9 * n];
+  T* pq = &buf[10 * n]; T* pg = &buf[11 * n];
+  T* wg = &buf[12 * n];
+  std::vector<T> ck((size_t)(max_depth + 1) * 2 * n);
+  auto cp = [n](T* d, const T* s) { std::memcpy(d, s, sizeof(T) * n); };
+
+  for (int i = 0; i < n; ++i) {
+    const int a = pol.api(i);
+    p[i] = a >= 0 ? draw_normal<T>(chain, itc, S_MOM, (uint32_t)a) / sqrt_t<T>(im[i]) : T(0);
+  }
+  const double H0 = pol.kinetic(p, im) - (double)lj;
+  cp(q0, q);
+  cp(wg, g);
+  cp(Lq, q); cp(Lp, p); cp(Lg, g);
+  cp(Rq, q); cp(Rp, p); cp(Rg, g);
+  cp(pq, q); cp(pg, g);
+  cp(sq, q); cp(sg, g);
+  T ljL = lj, ljR = lj, ljP = lj, ljS = lj;
+  double cont_w = -H0, accept_sum = 0.0;
+  int accept_count = 0, depth = 1, steps = 0;
+  bool divergent = false, turning = false;
+  double prop_energy = H0, prop_error = 0.0;
+  uint32_t leaf_counter = 0;
+  T* cq = q;  // working position
+
+  {
+    const float u = rand_uniform_f(chain, itc, S_DIR, 0u);
+    const bool fwd = !(u < 0.5f);
+    const T ss = fwd ? h : -h;
+    const T half = ss / T(2);
+    for (int i = 0; i < n; ++i) {
+      p[i] = p[i] + half * wg[i];
+      cq[i] = cq[i] + ss * (im[i] * p[i]);
+    }
+    bool ok;
+    const T lj1 = pol.eval(cq, wg, ok);
+    if (!ok) {
+      divergent = true;
+    } else {
+      for (int i = 0; i < n; ++i) p[i] = p[i] + half * wg[i];
+      if (fwd) { cp(Rq, cq); cp(Rp, p); cp(Rg, wg); ljR = lj1; }
+      else { cp(Lq, cq); cp(Lp, p); cp(Lg, wg); ljL = lj1; }
+      const double pe = pol.kinetic(p, im) - (double)lj1;
+      const double delta = pe - H0;
+      steps = 1;
+      if (!std::isfinite(delta) || delta > div_thr) {
+        divergent = true;
+      } else {
+        const double e = exp_d(std::fmin(0.0, -delta));
+        accept_sum += e < 1.0 ? e : 1.0;
+        accept_count += 1;
+        const double cand = -pe;
+        const double comb = logaddexp_d(cont_w, cand);
+        const double ul = rand_uniform_d(chain, itc, S_LEAF, leaf_counter);
+        if (!std::isfinite(cont_w) || log_d(ul) < cand - comb) {
+          cp(pq, cq); cp(pg, wg); ljP = lj1;
+          prop_energy = pe;
+          prop_error = delta;
+        }
+        cont_w = comb;
+        turning = pol.turning(Rq, Lq, Lp, Rp, im);
+      }
+      leaf_counter += 1;
+    }
+  }
+
+  while (depth < max_depth && !divergent && !turning) {
+    const uint32_t round = (uint32_t)depth;
+    const float ud = rand_uniform_f(chain, itc, S_DIR, round);
+    const bool fwd = !(ud < 0.5f);
+    const T ss = fwd ? h : -h;
+    const T half = ss / T(2);
+    if (fwd) { cp(cq, Rq); cp(p, Rp); cp(wg, Rg); }
+    else { cp(cq, Lq); cp(p, Lp); cp(wg, Lg); }
+    T cur_lj = fwd ? ljR : ljL;
+    double sub_w = -inf_d(), sub_asum = 0.0, sub_energy = 0.0, sub_error = 0.0;
+    int sub_int = 0, sub_acount = 0;
+    bool sub_div = false, sub_turn = false;
+    const int nleaves = 1 << depth;
+    for (int leaf = 0; leaf < nleaves; ++leaf) {
+      for (int i = 0; i < n; ++i) {
+        p[i] = p[i] + half * wg[i];
+        cq[i] = cq[i] + ss * (im[i] * p[i]);
+      }
+      bool ok;
+      const T ljc = pol.eval(cq, wg, ok);
+      if (!ok) { sub_div = true; break; }
+      for (int i = 0; i < n; ++i) p[i] = p[i] + half * wg[i];
+      cur_lj = ljc;
+      sub_int += 1;
+      const double pe = pol.kinetic(p, im) - (double)ljc;
+      const double delta = pe - H0;
+      if (!std::isfinite(delta) || delta > div_thr) { sub_div = true; break; }
+      const double e = exp_d(std::fmin(0.0, -delta));
+      sub_asum += e < 1.0 ? e : 1.0;
+      sub_acount += 1;
+      const double cand = -pe;
+      const double comb = logaddexp_d(sub_w, cand);
+      const double ul = rand_uniform_d(chain, itc, S_LEAF, leaf_counter);
+      if (!std::isfinite(sub_w) || log_d(ul) < cand - comb) {
+        cp(sq, cq); cp(sg, wg); ljS = ljc;
+        sub_energy = pe;
+        sub_error = delta;
+      }
+      sub_w = comb;
+      leaf_counter += 1;
+      const int li = sub_int - 1;
+      if ((li & 1) == 0) {
+        const int slot = __builtin_popcount((unsigned)li);
+        cp(&ck[(size_t)(slot * 2 + 0) * n], cq);
+        cp(&ck[(size_t)(slot * 2 + 1) * n], p);
+      } else {
+        const int nk = __builtin_ctz(~(unsigned)li);
+        bool turned = false;
+        for (int k = 1; k <= nk && !turned; ++k) {
+          const int bs = li - (1 << k) + 1;
+          const int slot = __builtin_popcount((unsigned)bs);
+          const T* cpos = &ck[(size_t)(slot * 2 + 0) * n];
+          const T* cmom = &ck[(size_t)(slot * 2 + 1) * n];
+          if (fwd) turned = pol.turning(cq, cpos, cmom, p, im);
+          else turned = pol.turning(cpos, cq, p, cmom, im);
+        }
+        if (turned) { sub_turn = true; break; }
+      }
+    }
+    depth += 1;
+    if (sub_int == 0) { divergent = sub_div; break; }
+    if (sub_turn || sub_div) {
+      steps += sub_int;
+      accept_sum += sub_asum;
+      accept_count += sub_acount;
+      turning = sub_turn;
+      divergent = sub_div;
+      break;
+    }
+    if (fwd) { cp(Rq, cq); cp(Rp, p); cp(Rg, wg); ljR = cur_lj; }
+    else { cp(Lq, cq); cp(Lp, p); cp(Lg, wg); ljL = cur_lj; }
+    double comb_cw = cont_w;
+    if (std::isfinite(sub_w)) {
+      comb_cw = logaddexp_d(cont_w, sub_w);
+      const double um = rand_uniform_d(chain, itc, S_MERGE, round);
+      if (log_d(um) < sub_w - cont_w) {
+        cp(pq, sq); cp(pg, sg); ljP = ljS;
+        prop_energy = sub_energy;
+        prop_error = sub_error;
+      }
+    }
+    steps += sub_int;
+    accept_sum += sub_asum;
+    accept_count += sub_acount;
+    if (std::isfinite(sub_w)) cont_w = comb_cw;
+    turning = sub_turn || pol.turning(Rq, Lq, Lp, Rp, im);
+  }
+  bool moved = false;
+  for (int i = 0; i < n; ++i) moved = moved || (pq[i] != q0[i]);
+  out.moved = moved;
+  out.accept_prob = accept_count == 0 ? 0.0 : accept_sum / (double)accept_count;
+  out.energy = prop_energy;
+  out.error = prop_error;
+  out.divergent = divergent;
+  out.depth = depth;
+  out.steps = steps;
+  if (moved) {
+    cp(q, pq); cp(g, pg); lj = ljP;
+  } else {
+    cp(q, q0);
+  }
+}
+
+// ------------------------------------------------------------------ HMC
+template <class Pol>
+void hmc_transition(const Pol& pol, const typename Pol::T* im, typename Pol::T h, uint32_t chain,
+                    uint32_t itc, int L, double div_thr, uint32_t mom_stream, uint32_t acc_stream,
+                    typename Pol::T* q, typename Pol::T* g, typename Pol::T& lj, TransOut& out,
+                    double* lar_out) {
+  using T = typename Pol::T;
+  const int n = pol.n();
+  std::vector<T> p(n), nq(q, q + n), ng(g, g + n);
+  for (int i = 0; i < n; ++i) {
+    const int a = pol.api(i);
+    p[i] = a >= 0 ? draw_normal<T>(chain, itc, mom_stream, (uint32_t)a) / sqrt_t<T>(im[i]) : T(0);
+  }
+  const double H0 = pol.kinetic(p.data(), im) - (double)lj;
+  T nlj = lj;
+  bool valid = true;
+  for (int i = 0; i < n; ++i) valid = valid && std::isfinite(g[i]);
+  const T half = T(0.5) * h;
+  if (valid)
+    for (int i = 0; i < n; ++i) p[i] = p[i] + half * ng[i];
+  for (int s = 1; s <= L; ++s) {
+    if (!valid) break;
+    for (int i = 0; i < n; ++i) nq[i] = nq[i] + (h * im[i]) * p[i];
+    bool ok;
+    nlj = pol.eval(nq.data(), ng.data(), ok);
+    if (s < L) {
+      bool gfin = true;
+      for (int i = 0; i < n; ++i) gfin = gfin && std::isfinite(ng[i]);
+      valid = valid && gfin;
+      if (valid)
+        for (int i = 0; i < n; ++i) p[i] = p[i] + (T(1) * h) * ng[i];
+    } else {
+      valid = valid && ok;
+    }
+  }
+  if (valid)
+    for (int i = 0; i < n; ++i) p[i] = -(p[i] + half * ng[i]);
+  out.depth = 0;
+  out.steps = L;
+  out.moved = false;
+  if (valid) {
+    const double H1 = pol.kinetic(p.data(), im) - (double)nlj;
+    const double lar = H0 - H1;
+    const double err = H1 - H0;
+    out.error = err;
+    out.divergent = !std::isfinite(err) || err > div_thr;
+    out.accept_prob = !std::isfinite(lar) ? 0.0 : (lar >= 0.0 ? 1.0 : exp_d(lar));
+    if (lar_out) *lar_out = lar;
+    if (acc_stream != 0xffffffffu) {
+      const double u = rand_uniform_d(chain, itc, acc_stream, 0u);
+      out.moved = log_d(u) < std::fmin(0.0, lar);
+    }
+    out.energy = out.moved ? H1 : H0;
+  } else {
+    out.error = inf_d();
+    out.divergent = true;
+    out.accept_prob = 0.0;
+    out.energy = H0;
+    if (lar_out) *lar_out = -inf_d();
+  }
+  if (out.moved) {
+    std::copy(nq.begin(), nq.end(), q);
+    std::copy(ng.begin(), ng.end(), g);
+    lj = nlj;
+  }
+}
+
+// ------------------------------------------------------------------ adaptation
+struct Dual {
+  double mu, log_step, log_step_avg, hbar, it;
+};
+Dual dual_init(double step) {
+  const double ls = log_d(step);
+  return Dual{log_d(10.0 * step), ls, ls, 0.0, 0.0};
+}
+double dual_update(Dual& d, double accept_prob, double target) {
+  const double gamma = 0.05, t0 = 10.0, kappa = 0.75;
+  d.it = d.it + 1.0;
+  const double it = d.it;
+  const double eta = 1.0 / (it + t0);
+  d.hbar = (1.0 - eta) * d.hbar + eta * (target - accept_prob);
+  d.log_step = d.mu - sqrt_d(it) / gamma * d.hbar;
+  const double eta_bar = pow_d(it, -kappa);
+  d.log_step_avg = eta_bar * d.log_step + (1.0 - eta_bar) * d.log_step_avg;
+  return exp_d(d.log_step);
+}
+
+struct Schedule {
+  int initial_buffer = 0;
+  std::vector<int> ends;
+  int terminal_buffer = 0;
+};
+Schedule warmup_schedule(int n) {
+  Schedule s;
+  if (n <= 0) return s;
+  if (n < 20) {
+    const int ib = std::min(5, std::max(n - 1, 0));
+    s.initial_buffer = ib;
+    if (ib == n) return s;
+    s.ends.push_back(n);
+    return s;
+  }
+  const int ib = std::min(std::max(5, n / 10), n);
+  const int tb = std::min(std::max(5, n / 10), std::max(n - ib, 0));
+  const int slow = n - ib - tb;
+  if (slow <= 0) { s.initial_buffer = n; return s; }
+  s.initial_buffer = ib;
+  s.terminal_buffer = tb;
+  int next = std::min(25, slow), start = ib + 1, remaining = slow;
+  while (remaining > 0) {
+    const int size = remaining <= (3 * next) / 2 ? remaining : next;
+    const int end = start + size - 1;
+    s.ends.push_back(end);
+    remaining -= size;
+    start = end + 1;
+    next *= 2;
+  }
+  return s;
+}
+
+// clipped weighted running variance (reference adaptation.jl:30-41,140-260)
+struct RunVar {
+  std::vector<double> mean, m2, clipped;
+  int window_length = 20;
+  long long count = 0;
+  double wsum = 0, wsq = 0;
+  void reset(int P, int wl) {
+    mean.assign(P, 0.0); m2.assign(P, 0.0); clipped.assign(P, 0.0);
+    window_length = wl; count = 0; wsum = 0; wsq = 0;
+  }
+  static double progress(long long count, int wl) {
+    if (wl <= 4) return 1.0;
+    const double r = (double)std::max(count - 4, 0ll) / (double)(wl - 4);
+    return r < 1.0 ? r : 1.0;
+  }
+  double clip_scale() const {
+    if (count <= 4) return 8.0;
+    return 8.0 + (5.0 - 8.0) * progress(count, window_length);
+  }
+  void update(const double* x, double w) {
+    if (w == 0.0) return;
+    const int P = (int)mean.size();
+    if (count < 4) {
+      for (int i = 0; i < P; ++i) clipped[i] = x[i];
+    } else {
+      const double cs = clip_scale();
+      const double dc = (double)std::max(count - 1, 1ll);
+      for (int i = 0; i < P; ++i) {
+        const double var = m2[i] / dc;
+        const double bound = cs * sqrt_d(var > 1e-3 ? var : 1e-3);
+        double delta = x[i] - mean[i];
+        delta = delta > bound ? bound : (delta < -bound ? -bound : delta);
+        clipped[i] = mean[i] + delta;
+      }
+    }
+    count += 1;
+    const double nw = wsum + w;
+    const double frac = w / nw;
+    for (int i = 0; i < P; ++i) {
+      const double delta = clipped[i] - mean[i];
+      mean[i] += frac * delta;
+      const double delta2 = clipped[i] - mean[i];
+      m2[i] += (w * delta) * delta2;
+    }
+    wsum = nw;
+    wsq += w * w;
+  }
+  double eff() const { return wsq <= 0.0 ? 0.0 : (wsum * wsum) / wsq; }
+  void inverse_mass(double reg, std::vector<double>& im) const {
+    const int P = (int)mean.size();
+    const double e = eff();
+    const double denom = wsum - wsq / wsum;
+    const double shrink = e / (e + 5.0);
+    for (int i = 0; i < P; ++i) {
+      if (e < 2.0 || denom <= 0.0) { im[i] = 1.0; continue; }
+      const double v = shrink * (m2[i] / denom) + (1.0 - shrink);
+      im[i] = v > reg ? v : reg;
+    }
+  }
+};
+
+double mass_weight(const RunVar& st, bool accepted, double ap, bool divergent) {
+  if (divergent) return 0.0;
+  if (accepted) return 1.0;
+  if (!std::isfinite(ap)) return 0.0;
+  const double prog = RunVar::progress(st.count, st.window_length);
+  const double rej = ap < 0.0 ? 0.0 : (ap > 1.0 ? 1.0 : ap);
+  return 1.0 + (rej - 1.0) * prog;
+}
+
+uint32_t run_seed_of(uint64_t seed) {
+  return philox4x32((uint32_t)seed, (uint32_t)(seed >> 32), 0x5EEDu, 0u, 0u, 0u).w[0];
+}
+
+void draw_initial_column(const Model& m, int strategy, uint32_t chain, uint32_t key, double* col) {
+  const int P = m.P;
+  if (strategy == UT_INIT_UNIFORM) {
+    for (int a = 0; a < P; ++a) col[a] = 4.0 * rand_uniform_d(chain, key, S_INIT, (uint32_t)a) - 2.0;
+    return;
+  }
+  auto z = [&](int idx) { return (double)rand_normal_f(chain, key, S_INIT, (uint32_t)idx); };
+  if (m.kind == KIND_LOGISTIC) {
+    for (int a = 0; a < P; ++a) col[a] = m.loc + m.scale * z(a);
+  } else if (m.kind == KIND_HIER) {
+    col[0] = m.mu_loc + m.mu_scale * z(0);
+    const double tau = m.tau_scale * std::fabs(z(1) / z(P + 1));
+    col[1] = log_d(tau);
+    for (int j = 0; j < m.J; ++j) col[2 + j] = z(2 + j);
+  } else {
+    const double v = m.v_scale * z(0);
+    col[0] = v;
+    const double s = exp_d(v / 2.0);
+    for (int i = 0; i < m.dim; ++i) col[1 + i] = s * z(1 + i);
+  }
+}
+
+// ------------------------------------------------------------------ sampler driver
+// Mirrors the device driver: per-chain step with dual averaging, pooled diagonal
+// mass fed per iteration in chain order, window-end re-search, sampling.
+template <class Pol>
+int run_sampler(const Pol& pol, const ut_sampler_opts* o, int C, bool nuts, int threads,
+                void* samples, ut_chain_out* out, const int* stop_after) {
+  using T = typename Pol::T;
+  const Model& m = *pol.m;
+  const int P = m.P, n = pol.n(), S = o->num_samples, W = o->num_warmup;
+  const uint32_t rs = run_seed_of(o->seed);
+  if (threads > 0) omp_set_num_threads(threads);
+  std::vector<double> pos((size_t)P * C);
+  if (o->initial_params) std::memcpy(pos.data(), o->initial_params, pos.size() * 8);
+  else
+    for (int c = 0; c < C; ++c) draw_initial_column(m, o->init_strategy, (uint32_t)c, rs, &pos[(size_t)c * P]);
+  std::vector<T> q((size_t)n * C, T(0)), g((size_t)n * C, T(0)), lj(C);
+  auto load = [&](int c) {
+    for (int i = 0; i < n; ++i) {
+      const int a = pol.api(i);
+      q[(size_t)c * n + i] = a >= 0 ? (T)(float)pos[(size_t)c * P + a] : T(0);
+    }
+    if (std::is_same<T, double>::value)
+      for (int i = 0; i < n; ++i) {
+        const int a = pol.api(i);
+        q[(size_t)c * n + i] = a >= 0 ? (T)pos[(size_t)c * P + a] : T(0);
+      }
+  };
+  for (int attempt = 0;; ++attempt) {
+    std::vector<int> bad;
+    for (int c = 0; c < C; ++c) {
+      load(c);
+      bool ok;
+      lj[c] = pol.eval(&q[(size_t)c * n], &g[(size_t)c * n], ok);
+      if (!ok) bad.push_back(c);
+    }
+    if (bad.empty()) break;
+    if (o->initial_params || attempt >= o->init_max_retries) return UT_E_NONFINITE_INIT;
+    for (int c : bad)
+      draw_initial_column(m, o->init_strategy, (uint32_t)c, rs + (uint32_t)(attempt + 1), &pos[(size_t)c * P]);
+  }
+  std::vector<T> im(n, T(0));
+  std::vector<double> im_d(P, 1.0);
+  auto set_im = [&]() {
+    for (int i = 0; i < n; ++i) {
+      const int a = pol.api(i);
+      im[i] = a >= 0 ? (T)(float)im_d[a] : T(0);
+      if (std::is_same<T, double>::value && a >= 0) im[i] = (T)im_d[a];
+    }
+  };
+  set_im();
+  std::vector<double> step(C, o->step_size);
+  std::vector<Dual> dual(C);
+  auto search = [&](uint32_t itc, bool do_search) {
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int c = 0; c < C; ++c) {
+      double s = step[c];
+      if (do_search) {
+        const double log_target = -0.6931471805599453;
+        auto trial = [&](double st) {
+          std::vector<T> tq(&q[(size_t)c * n], &q[(size_t)c * n] + n), tg(&g[(size_t)c * n], &g[(size_t)c * n] + n);
+          T tlj = lj[c];
+          TransOut to;
+          double lar;
+          hmc_transition(pol, im.data(), (T)st, (uint32_t)c, itc, 1, 0.0, S_SEARCH, 0xffffffffu,
+                         tq.data(), tg.data(), tlj, to, &lar);
+          return lar;
+        };
+        const double lar0 = trial(s);
+        const double dir = lar0 > log_target ? 1.0 : -1.0;
+        for (int t = 0; t < 20; ++t) {
+          const double next = s * (dir > 0.0 ? 2.0 : 0.5);
+          if (next < 1e-8 || next > 1e3) break;
+          const double lar = trial(next);
+          s = next;
+          if ((dir > 0.0 && lar <= log_target) || (dir < 0.0 && lar >= log_target)) break;
+        }
+        s = s < 1e-8 ? 1e-8 : (s > 1e3 ? 1e3 : s);
+      }
+      step[c] = s;
+      dual[c] = dual_init(s);
+    }
+  };
+  const bool do_search = o->find_reasonable_step_size || (W > 0 && o->adapt_step_size);
+  search(rs, do_search);
+
+  const Schedule sch = warmup_schedule(W);
+  RunVar rv;
+  size_t win = 0;
+  rv.reset(P, sch.ends.empty() ? 20 : sch.ends[0] - sch.initial_buffer);
+  std::vector<TransOut> outs(C);
+  std::vector<double> colbuf(P);
+  const int last = stop_after ? std::min(*stop_after, W + S) : W + S;
+  for (int it = 1; it <= last; ++it) {
+    const uint32_t itc = rs + (uint32_t)it;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int c = 0; c < C; ++c) {
+      const T h = (T)(float)step[c];
+      const T hh = std::is_same<T, double>::value ? (T)step[c] : h;
+      T* qc = &q[(size_t)c * n];
+      T* gc = &g[(size_t)c * n];
+      std::vector<T> gk(gc, gc + n);
+      const T ljk = lj[c];
+      if (nuts) {
+        nuts_transition(pol, im.data(), hh, (uint32_t)c, itc, o->max_tree_depth, o->divergence_threshold, qc, gc, lj[c], outs[c]);
+        if (!outs[c].moved) { std::copy(gk.begin(), gk.end(), gc); lj[c] = ljk; }
+      } else {
+        hmc_transition(pol, im.data(), hh, (uint32_t)c, itc, o->num_leapfrog_steps, o->divergence_threshold,
+                       S_MOM, S_ACCEPT, qc, gc, lj[c], outs[c], nullptr);
+      }
+      if (it <= W && o->adapt_step_size) {
+        step[c] = dual_update(dual[c], outs[c].divergent ? 0.0 : outs[c].accept_prob, o->target_accept);
+        if (it == W) step[c] = exp_d(dual[c].log_step_avg);
+      }
+    }
+    if (it <= W) {
+      // pooled mass update, chain order, weights from the state at iteration start
+      const bool in_window = o->adapt_mass_matrix && win < sch.ends.size() && it > sch.initial_buffer;
+      if (in_window) {
+        std::vector<double> w(C);
+        for (int c = 0; c < C; ++c) w[c] = mass_weight(rv, !nuts && outs[c].moved, outs[c].accept_prob, outs[c].divergent);
+        for (int c = 0; c < C; ++c) {
+          for (int i = 0; i < n; ++i) {
+            const int a = pol.api(i);
+            if (a >= 0) colbuf[a] = (double)q[(size_t)c * n + i];
+          }
+          rv.update(colbuf.data(), w[c]);
+        }
+        if (it == sch.ends[win]) {
+          if (rv.eff() >= (double)o->mass_matrix_min_samples) {
+            rv.inverse_mass(o->mass_matrix_regularization, im_d);
+            set_im();
+          }
+          ++win;
+          if (win < sch.ends.size()) rv.reset(P, sch.ends[win] - sch.ends[win - 1]);
+          else rv.reset(P, 20);
+          if (o->adapt_step_size && it < W) search(rs + (uint32_t)it, true);
+        }
+      }
+    } else if (samples) {
+      const int s = it - W - 1;
+      for (int c = 0; c < C; ++c) {
+        const size_t row = (size_t)c * S + s;
+        for (int i = 0; i < n; ++i) {
+          const int a = pol.api(i);
+          if (a < 0) continue;
+          if (std::is_same<T, double>::value) ((double*)samples)[row * P + a] = (double)q[(size_t)c * n + i];
+          else ((float*)samples)[row * P + a] = (float)q[(size_t)c * n + i];
+        }
+        if (out) {
+          if (out->logjoint) out->logjoint[row] = (double)lj[c];
+          if (out->accept_stat) out->accept_stat[row] = outs[c].accept_prob;
+          if (out->energy) out->energy[row] = outs[c].energy;
+          if (out->energy_error) out->energy_error[row] = outs[c].error;
+          if (out->accepted) out->accepted[row] = outs[c].moved ? 1 : 0;
+          if (out->divergent) out->divergent[row] = outs[c].divergent ? 1 : 0;
+          if (out->tree_depth) out->tree_depth[row] = outs[c].depth;
+          if (out->integration_steps) out->integration_steps[row] = outs[c].steps;
+        }
+      }
+    }
+  }
+  if (out) {
+    if (out->step_size) for (int c = 0; c < C; ++c) out->step_size[c] = step[c];
+    if (out->inverse_mass) for (int a = 0; a < P; ++a) out->inverse_mass[a] = im_d[a];
+  }
+  return UT_OK;
+}
+
+// Sampling-phase NUTS iterations from a given state, without adaptation; returns
+// the number of leapfrog gradient evaluations.  Used to time the CPU baseline.
+template <class Pol>
+int64_t sample_from(const Pol& pol, int C, int threads, double* positions, const double* steps,
+                    const double* im_api, uint32_t rs, int first_it, int iters, int max_depth) {
+  using T = typename Pol::T;
+  const Model& m = *pol.m;
+  const int P = m.P, n = pol.n();
+  if (threads > 0) omp_set_num_threads(threads);
+  std::vector<T> im(n, T(0));
+  for (int i = 0; i < n; ++i) {
+    const int a = pol.api(i);
+    im[i] = a >= 0 ? (T)im_api[a] : T(0);
+  }
+  int64_t total = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : total)
+  for (int c = 0; c < C; ++c) {
+    std::vector<T> q(n, T(0)), g(n, T(0)), gk(n);
+    for (int i = 0; i < n; ++i) {
+      const int a = pol.api(i);
+      if (a >= 0) q[i] = (T)positions[(size_t)c * P + a];
+    }
+    bool ok;
+    T lj = pol.eval(q.data(), g.data(), ok);
+    for (int it = first_it; it < first_it + iters; ++it) {
+      TransOut o;
+      const T ljk = lj;
+      gk = g;
+      nuts_transition(pol, im.data(), (T)steps[c], (uint32_t)c, rs + (uint32_t)it, max_depth, 1000.0,
+                      q.data(), g.data(), lj, o);
+      if (!o.moved) { g = gk; lj = ljk; }
+      total += o.steps;
+    }
+    for (int i = 0; i < n; ++i) {
+      const int a = pol.api(i);
+      if (a >= 0) positions[(size_t)c * P + a] = (double)q[i];
+    }
+  }
+  return total;
+}
+
+void finish_twin_layout(Model& m) {
+  m.K = m.kind == KIND_HIER ? 2 : slots_for(m.P);
+  if (m.kind == KIND_HIER) {
+    m.max_n = 0;
+    m.tnobs.assign(WAVE, 0);
+    m.tlane.assign(3 * WAVE, 0.0);
+    for (int j = 0; j < m.J; ++j) m.max_n = std::max(m.max_n, m.offsets[j + 1] - m.offsets[j]);
+    m.ty_f.assign((size_t)std::max(m.max_n, 1) * WAVE, 0.0f);
+    for (int j = 0; j < m.J; ++j) {
+      const int nj = m.offsets[j + 1] - m.offsets[j];
+      m.tnobs[j] = nj;
+      for (int i = 0; i < nj; ++i) m.ty_f[(size_t)i * WAVE + j] = m.hy[m.offsets[j] + i];
+      const double s = m.sigma[j];
+      m.tlane[j] = -(double)nj * log_d(s) - (double)nj * HALF_LOG_2PI;
+      m.tlane[WAVE + j] = 1.0 / (2.0 * s * s);
+      m.tlane[2 * WAVE + j] = 1.0 / (s * s);
+    }
+    m.total_const = (double)m.J * (-HALF_LOG_2PI) + (-log_d(m.mu_scale) - HALF_LOG_2PI) +
+                    (LOG_2 - LOG_PI - log_d(m.tau_scale));
+  } else if (m.kind == KIND_LOGISTIC) {
+    m.total_const = (double)m.D * (-log_d(m.scale) - HALF_LOG_2PI);
+  } else {
+    m.total_const = -(double)m.dim * HALF_LOG_2PI + (-log_d(m.v_scale) - HALF_LOG_2PI);
+  }
+}
+
+}  // namespace
+
+// ====================================================================== C API
+extern "C" {
+
+void orc_philox4x32(const uint32_t key[2], const uint32_t ctr[4], uint32_t out[4]) {
+  philox_block b = philox4x32(key[0], key[1], ctr[0], ctr[1], ctr[2], ctr[3]);
+  for (int i = 0; i < 4; ++i) out[i] = b.w[i];
+}
+
+void orc_philox_fill(uint32_t chain, uint32_t iteration, uint32_t stream, uint32_t first, int64_t n,
+                     uint32_t* blocks, float* uf, float* nf, double* ud) {
+  for (int64_t i = 0; i < n; ++i) {
+    const philox_block b = draw_block(chain, iteration, stream, first + (uint32_t)i);
+    if (blocks) for (int w = 0; w < 4; ++w) blocks[4 * i + w] = b.w[w];
+    if (uf) uf[i] = u01_f(b.w[0]);
+    if (nf) nf[i] = box_muller_f(b);
+    if (ud) ud[i] = u01_d(b.w[0], b.w[1]);
+  }
+}
+
+// elementwise shared math, for accuracy tests: fn 0 exp 1 log 2 log1p 3 cos 4 log1p_exp 5 logistic
+void orc_math_f32(int fn, const float* x, float* y, int64_t n) {
+  for (int64_t i = 0; i < n; ++i) {
+    switch (fn) {
+      case 0: y[i] = exp_f(x[i]); break;
+      case 1: y[i] = log_f(x[i]); break;
+      case 2: y[i] = log1p_f(x[i]); break;
+      case 3: y[i] = cos_f(x[i]); break;
+      case 4: y[i] = log1p_exp_f(x[i]); break;
+      case 5: y[i] = logistic_f(x[i]); break;
+    }
+  }
+}
+void orc_math_f64(int fn, const double* x, double* y, int64_t n) {
+  for (int64_t i = 0; i < n; ++i) {
+    switch (fn) {
+      case 0: y[i] = exp_d(x[i]); break;
+      case 1: y[i] = log_d(x[i]); break;
+      case 2: y[i] = log1p_d(x[i]); break;
+      case 4: y[i] = log1p_exp_d(x[i]); break;
+      case 5: y[i] = logistic_d(x[i]); break;
+      case 6: y[i] = pow_d(x[i], -0.75); break;
+      default: y[i] = 0.0;
+    }
+  }
+}
+
+void* orc_model_logistic(const float* x, const float* y, int64_t n, int32_t d, double loc, double scale) {
+  auto* m = new Model();
+  m->kind = KIND_LOGISTIC;
+  m->x.assign(x, x + n * d);
+  m->y.assign(y, y + n);
+  m->n = n;
+  m->D = d;
+  m->P = d;
+  m->loc = loc;
+  m->scale = scale;
+  finish_twin_layout(*m);
+  return m;
+}
+
+void* orc_model_hier(const float* y, const int32_t* offsets, const double* sigma, int32_t J,
+                     double mu_loc, double mu_scale, double tau_scale) {
+  auto* m = new Model();
+  m->kind = KIND_HIER;
+  m->J = J;
+  m->offsets.assign(offsets, offsets + J + 1);
+  m->hy.assign(y, y + offsets[J]);
+  m->sigma.assign(sigma, sigma + J);
+  m->mu_loc = mu_loc;
+  m->mu_scale = mu_scale;
+  m->tau_scale = tau_scale;
+  m->P = J + 2;
+  finish_twin_layout(*m);
+  return m;
+}
+
+void* orc_model_funnel(int32_t dim, double v_scale) {
+  auto* m = new Model();
+  m->kind = KIND_FUNNEL;
+  m->dim = dim;
+  m->v_scale = v_scale;
+  m->P = dim + 1;
+  finish_twin_layout(*m);
+  return m;
+}
+
+void orc_model_free(void* h) { delete (Model*)h; }
+int32_t orc_model_num_params(void* h) { return ((Model*)h)->P; }
+
+// binary64 reference-formula logjoint/gradient, params P x C (api order)
+void orc_logjoint_grad_f64(void* h, const double* params, int32_t C, double* lj, double* grad) {
+  const Model& m = *(Model*)h;
+#pragma omp parallel for
+  for (int c = 0; c < C; ++c) {
+    bool ok;
+    lj[c] = ref_eval(m, params + (size_t)c * m.P, grad + (size_t)c * m.P, ok);
+  }
+}
+
+// kernel-exact logjoint/gradient in binary32 (twin=1) or binary64 (twin=2, the
+// precision=Float64 density kernels), params P x C (api order)
+void orc_logjoint_grad_twin(void* h, int32_t precision, const void* params, int32_t C, void* lj, void* grad) {
+  const Model& m = *(Model*)h;
+  const int n = m.K * WAVE;
+  for (int c = 0; c < C; ++c) {
+    if (precision == UT_FLOAT32) {
+      std::vector<float> q(n, 0.0f), g(n, 0.0f);
+      for (int i = 0; i < n; ++i) {
+        const int a = m.api(i % WAVE, i / WAVE);
+        if (a >= 0) q[i] = ((const float*)params)[(size_t)c * m.P + a];
+      }
+      bool ok;
+      ((float*)lj)[c] = twin_eval<float>(m, q.data(), g.data(), ok);
+      for (int i = 0; i < n; ++i) {
+        const int a = m.api(i % WAVE, i / WAVE);
+        if (a >= 0) ((float*)grad)[(size_t)c * m.P + a] = g[i];
+      }
+    } else {
+      std::vector<double> q(n, 0.0), g(n, 0.0);
+      for (int i = 0; i < n; ++i) {
+        const int a = m.api(i % WAVE, i / WAVE);
+        if (a >= 0) q[i] = ((const double*)params)[(size_t)c * m.P + a];
+      }
+      bool ok;
+      ((double*)lj)[c] = twin_eval<double>(m, q.data(), g.data(), ok);
+      for (int i = 0; i < n; ++i) {
+        const int a = m.api(i % WAVE, i / WAVE);
+        if (a >= 0) ((double*)grad)[(size_t)c * m.P + a] = g[i];
+      }
+    }
+  }
+}
+
+// Full sampler run.  mode 0: TwinF32 (kernel-exact, samples are float);
+// mode 1: RefF64 (samples are double).  kind 0 NUTS, 1 HMC.  stop_after < 0 runs
+// every iteration; otherwise stops after that many iterations (timing samples).
+int orc_sampler_run(void* h, int32_t kind, int32_t mode, const ut_sampler_opts* o, int32_t C,
+                    int32_t threads, void* samples, ut_chain_out* out, int32_t stop_after) {
+  const Model* m = (const Model*)h;
+  const int* sa = stop_after >= 0 ? &stop_after : nullptr;
+  if (mode == 0) {
+    TwinF32 pol{m};
+    return run_sampler(pol, o, C, kind == 0, threads, samples, out, sa);
+  }
+  RefF64 pol{m};
+  return run_sampler(pol, o, C, kind == 0, threads, samples, out, sa);
+}
+
+int64_t orc_nuts_sample_from(void* h, int32_t mode, int32_t C, int32_t threads, double* positions,
+                             const double* steps, const double* inverse_mass, int32_t first_iteration,
+                             int32_t iters, uint64_t seed, int32_t max_depth) {
+  const Model* m = (const Model*)h;
+  const uint32_t rs = run_seed_of(seed);
+  if (mode == 0) return sample_from(TwinF32{m}, C, threads, positions, steps, inverse_mass, rs, first_iteration, iters, max_depth);
+  return sample_from(RefF64{m}, C, threads, positions, steps, inverse_mass, rs, first_iteration, iters, max_depth);
+}
+
+// warmup schedule: writes initial buffer, terminal buffer and up to cap window ends
+int32_t orc_warmup_schedule(int32_t n, int32_t* initial_buffer, int32_t* terminal_buffer,
+                            int32_t* ends, int32_t cap) {
+  Schedule s = warmup_schedule(n);
+  *initial_buffer = s.initial_buffer;
+  *terminal_buffer = s.terminal_buffer;
+  for (int i = 0; i < (int)s.ends.size() && i < cap; ++i) ends[i] = s.ends[i];
+  return (int32_t)s.ends.size();
+}
+
+// dual-averaging sequence for KATs: runs `n` updates from `step` with the given
+// accept statistics, writing the step after each update and the final averaged step
+void orc_dual_averaging(double step, double target, const double* accepts, int32_t n, double* steps,
+                        double* final_step) {
+  Dual d = dual_init(step);
+  for (int i = 0; i < n; ++i) steps[i] = dual_update(d, accepts[i], target);
+  *final_step = exp_d(d.log_step_avg);
+}
+
+// pooled running variance over a (iterations x C x P) history with per-entry
+// accept probabilities and flags (bit0 divergent, bit1 accepted), as the mass
+// adaptation of one window consumes it; writes the inverse mass and returns the
+// effective count.
+double orc_running_variance(const double* hist, const double* acc, const uint8_t* flags, int32_t iters,
+                            int32_t C, int32_t P, int32_t window_length, int32_t use_accepted,
+                            double reg, double* inverse_mass) {
+  RunVar rv;
+  rv.reset(P, window_length);
+  for (int it = 0; it < iters; ++it) {
+    std::vector<double> w(C);
+    for (int c = 0; c < C; ++c) {
+      const uint8_t f = flags[(size_t)it * C + c];
+      w[c] = mass_weight(rv, use_accepted && (f & 2), acc[(size_t)it * C + c], f & 1);
+    }
+    for (int c = 0; c < C; ++c) rv.update(&hist[((size_t)it * C + c) * P], w[c]);
+  }
+  std::vector<double> im(P);
+  rv.inverse_mass(reg, im);
+  for (int i = 0; i < P; ++i) inverse_mass[i] = im[i];
+  return rv.eff();
+}
+
+}  // extern "C"

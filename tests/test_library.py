@@ -1,0 +1,59 @@
+"""CPU checks of the C ABI: the library loads, exports every symbol the header
+declares, and validates arguments before touching a device."""
+import ctypes as ct
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from uncertaintea_amd import _lib as L
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def header_symbols():
+    text = (ROOT / "include" / "uncertaintea_hip.h").read_text()
+    return sorted(set(re.findall(r"\b(ut_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_header_symbols():
+    lib = L.lib()
+    syms = header_symbols()
+    assert len(syms) >= 15
+    for s in syms:
+        assert hasattr(lib, s), s
+    assert set(L.EXPORTED) == set(syms)
+
+
+def test_abi_version_and_host_philox(golden):
+    assert L.lib().ut_abi_version() == 1
+    for key, ctr, expected in golden["philox_kat"]:
+        assert L.philox4x32(key, ctr).tolist() == expected
+
+
+def test_plan_argument_validation_without_device():
+    h = ct.c_void_p()
+    x = np.zeros((4, 2), np.float32)
+    y = np.array([0, 1, 2, 0], np.float32)  # 2 is not a Bernoulli outcome
+    rc = L.lib().ut_plan_logistic_glm(L.ptr(x), L.ptr(y), 4, 2, 0.0, 1.0, ct.byref(h))
+    assert rc == L.UT_E_ARG and b"0/1" in L.lib().ut_last_error()
+    rc = L.lib().ut_plan_logistic_glm(L.ptr(x), L.ptr(y), 4, 300, 0.0, 1.0, ct.byref(h))
+    assert rc == L.UT_E_UNSUPPORTED
+    rc = L.lib().ut_plan_funnel(0, 3.0, ct.byref(h))
+    assert rc == L.UT_E_DIM
+    off = np.array([0, 2], np.int32)
+    sig = np.array([-1.0])
+    rc = L.lib().ut_plan_hier_normal_nc(L.ptr(np.zeros(2, np.float32)), L.ptr(off), L.ptr(sig), 1, 0.0, 5.0, 5.0,
+                                        ct.byref(h))
+    assert rc == L.UT_E_ARG
+    with pytest.raises(ValueError):
+        L.check(rc)
+
+
+def test_sampler_defaults_match_reference():
+    o = L.SamplerOpts()
+    L.lib().ut_sampler_opts_default(ct.byref(o))
+    assert (o.max_tree_depth, o.num_leapfrog_steps, o.step_size, o.target_accept) == (10, 10, 0.1, 0.8)
+    assert (o.divergence_threshold, o.mass_matrix_regularization, o.mass_matrix_min_samples) == (1000.0, 1e-3, 10)
+    assert (o.adapt_step_size, o.adapt_mass_matrix, o.init_max_retries) == (1, 1, 100)

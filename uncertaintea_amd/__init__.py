@@ -1,0 +1,15 @@
+"""uncertaintea_amd: the MI355X (gfx950) device backend of UncertainTea's batched
+density and batched MCMC path, behind the C ABI of libuncertaintea_hip."""
+from . import diagnostics, synthetic
+from ._lib import DeviceError, NonFiniteInitError, device_available, philox4x32, philox_fill
+from .device import (DeviceBatchedWorkspace, device_batched_logjoint, device_batched_logjoint_gradient,
+                     device_batched_logjoint_gradient_inplace, device_batched_logjoint_inplace)
+from .inference import HMCChains, batched_hmc, batched_nuts
+from .models import HierarchicalNormalNC, LogisticRegression, NealsFunnel, TeaModel
+
+__all__ = [
+    "DeviceBatchedWorkspace", "DeviceError", "HMCChains", "HierarchicalNormalNC", "LogisticRegression",
+    "NealsFunnel", "NonFiniteInitError", "TeaModel", "batched_hmc", "batched_nuts", "device_available",
+    "device_batched_logjoint", "device_batched_logjoint_gradient", "device_batched_logjoint_gradient_inplace",
+    "device_batched_logjoint_inplace", "diagnostics", "philox4x32", "philox_fill", "synthetic",
+]

@@ -1,0 +1,172 @@
+"""ctypes binding of libuncertaintea_hip (the C ABI in include/uncertaintea_hip.h).
+
+The library is loaded from the package directory only.  There is no fallback: if
+the shared object is missing, every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "libuncertaintea_hip.so"
+
+UT_OK = 0
+UT_E_UNSUPPORTED = 1
+UT_E_DIM = 2
+UT_E_PRECISION = 3
+UT_E_HIP = 4
+UT_E_NONFINITE_INIT = 5
+UT_E_ARG = 6
+UT_FLOAT32 = 0
+UT_FLOAT64 = 1
+UT_INIT_PRIOR = 0
+UT_INIT_UNIFORM = 1
+
+# names exported by the C ABI (checked by the load test)
+EXPORTED = (
+    "ut_last_error", "ut_abi_version", "ut_device_available",
+    "ut_plan_logistic_glm", "ut_plan_hier_normal_nc", "ut_plan_funnel",
+    "ut_plan_destroy", "ut_plan_num_params",
+    "ut_workspace_create", "ut_workspace_destroy",
+    "ut_logjoint", "ut_logjoint_gradient",
+    "ut_sampler_opts_default", "ut_nuts_run", "ut_hmc_run",
+    "ut_philox4x32", "ut_philox_fill",
+)
+
+
+class SamplerOpts(ct.Structure):
+    _fields_ = [
+        ("num_samples", ct.c_int32),
+        ("num_warmup", ct.c_int32),
+        ("max_tree_depth", ct.c_int32),
+        ("num_leapfrog_steps", ct.c_int32),
+        ("step_size", ct.c_double),
+        ("target_accept", ct.c_double),
+        ("divergence_threshold", ct.c_double),
+        ("mass_matrix_regularization", ct.c_double),
+        ("mass_matrix_min_samples", ct.c_int32),
+        ("adapt_step_size", ct.c_int32),
+        ("adapt_mass_matrix", ct.c_int32),
+        ("find_reasonable_step_size", ct.c_int32),
+        ("init_strategy", ct.c_int32),
+        ("init_max_retries", ct.c_int32),
+        ("initial_params", ct.POINTER(ct.c_double)),
+        ("seed", ct.c_uint64),
+    ]
+
+
+class ChainOut(ct.Structure):
+    _fields_ = [
+        ("samples", ct.c_void_p),
+        ("logjoint", ct.POINTER(ct.c_double)),
+        ("accept_stat", ct.POINTER(ct.c_double)),
+        ("energy", ct.POINTER(ct.c_double)),
+        ("energy_error", ct.POINTER(ct.c_double)),
+        ("accepted", ct.POINTER(ct.c_uint8)),
+        ("divergent", ct.POINTER(ct.c_uint8)),
+        ("tree_depth", ct.POINTER(ct.c_int32)),
+        ("integration_steps", ct.POINTER(ct.c_int32)),
+        ("step_size", ct.POINTER(ct.c_double)),
+        ("inverse_mass", ct.POINTER(ct.c_double)),
+        ("warmup_seconds", ct.c_double),
+        ("sampling_seconds", ct.c_double),
+        ("sampling_gradient_evals", ct.c_int64),
+        ("sampling_kernel_seconds", ct.c_double),
+        ("sampling_launches", ct.c_int32),
+    ]
+
+
+class DeviceError(RuntimeError):
+    """A HIP runtime failure reported by libuncertaintea_hip."""
+
+
+class NonFiniteInitError(ValueError):
+    """Initial positions stayed non-finite after the allowed re-draws."""
+
+
+_lib = None
+
+
+def lib() -> ct.CDLL:
+    """Load (once) and return the library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(
+            f"libuncertaintea_hip is not built ({LIB_PATH} missing); run "
+            "`python -c 'import __graft_entry__ as g; g.build()'` first")
+    L = ct.CDLL(str(LIB_PATH))
+    vp, i32, i64, u32, dbl = ct.c_void_p, ct.c_int32, ct.c_int64, ct.c_uint32, ct.c_double
+    pp = ct.POINTER(ct.c_void_p)
+    sig = {
+        "ut_last_error": (ct.c_char_p, []),
+        "ut_abi_version": (i32, []),
+        "ut_device_available": (i32, []),
+        "ut_plan_logistic_glm": (ct.c_int, [vp, vp, i64, i32, dbl, dbl, pp]),
+        "ut_plan_hier_normal_nc": (ct.c_int, [vp, vp, vp, i32, dbl, dbl, dbl, pp]),
+        "ut_plan_funnel": (ct.c_int, [i32, dbl, pp]),
+        "ut_plan_destroy": (ct.c_int, [vp]),
+        "ut_plan_num_params": (ct.c_int, [vp, ct.POINTER(i32)]),
+        "ut_workspace_create": (ct.c_int, [vp, i32, i32, pp]),
+        "ut_workspace_destroy": (ct.c_int, [vp]),
+        "ut_logjoint": (ct.c_int, [vp, vp, vp]),
+        "ut_logjoint_gradient": (ct.c_int, [vp, vp, vp, vp]),
+        "ut_sampler_opts_default": (None, [ct.POINTER(SamplerOpts)]),
+        "ut_nuts_run": (ct.c_int, [vp, ct.POINTER(SamplerOpts), ct.POINTER(ChainOut)]),
+        "ut_hmc_run": (ct.c_int, [vp, ct.POINTER(SamplerOpts), ct.POINTER(ChainOut)]),
+        "ut_philox4x32": (None, [vp, vp, vp]),
+        "ut_philox_fill": (ct.c_int, [u32, u32, u32, u32, i64, vp, vp, vp, vp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def check(rc: int) -> None:
+    """Map a status code to the reference's exception classes."""
+    if rc == UT_OK:
+        return
+    msg = lib().ut_last_error().decode(errors="replace")
+    if rc == UT_E_DIM:
+        raise ValueError(f"DimensionMismatch: {msg}")
+    if rc == UT_E_HIP:
+        raise DeviceError(msg)
+    if rc == UT_E_NONFINITE_INIT:
+        raise NonFiniteInitError(msg)
+    raise ValueError(msg)  # UT_E_ARG / UT_E_UNSUPPORTED / UT_E_PRECISION: ArgumentError
+
+
+def ptr(a: np.ndarray | None):
+    return None if a is None else a.ctypes.data_as(ct.c_void_p)
+
+
+def dptr(a: np.ndarray | None, ctype):
+    return None if a is None else a.ctypes.data_as(ct.POINTER(ctype))
+
+
+def device_available() -> bool:
+    return bool(lib().ut_device_available())
+
+
+def philox4x32(key, ctr) -> np.ndarray:
+    k = np.ascontiguousarray(key, dtype=np.uint32)
+    c = np.ascontiguousarray(ctr, dtype=np.uint32)
+    out = np.zeros(4, dtype=np.uint32)
+    lib().ut_philox4x32(ptr(k), ptr(c), ptr(out))
+    return out
+
+
+def philox_fill(chain: int, iteration: int, stream: int, first: int, n: int):
+    """Device draws of n consecutive Philox coordinates: (blocks, u32f, normal32, u64f)."""
+    blocks = np.zeros((n, 4), dtype=np.uint32)
+    uf = np.zeros(n, dtype=np.float32)
+    nf = np.zeros(n, dtype=np.float32)
+    ud = np.zeros(n, dtype=np.float64)
+    check(lib().ut_philox_fill(chain, iteration, stream, first, n, ptr(blocks), ptr(uf), ptr(nf), ptr(ud)))
+    return blocks, uf, nf, ud

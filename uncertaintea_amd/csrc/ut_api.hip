@@ -1,0 +1,659 @@
+// ut_api.hip -- the C ABI of libuncertaintea_hip (declared in include/uncertaintea_hip.h).
+//
+// Host responsibilities: staging a plan's data into HBM once, owning per-workspace
+// device buffers and one HIP stream, drawing initial positions, and sequencing the
+// sampler kernels: warmup runs as one launch range per mass-adaptation window
+// (per-chain dual averaging happens inside the kernel), each window end replays
+// the pooled running variance on the device and re-searches every chain's step
+// size, and sampling runs in fixed launch ranges that write draws straight into
+// device result buffers, copied out once at the end.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/uncertaintea_hip.h"
+#include "ut_kernels.h"
+#include "ut_math.h"
+
+using namespace ut;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+#define UT_HIP(expr)                                                                   \
+  do {                                                                                 \
+    hipError_t _e = (expr);                                                            \
+    if (_e != hipSuccess)                                                              \
+      return fail(UT_E_HIP, std::string(#expr) + ": " + hipGetErrorString(_e));       \
+  } while (0)
+
+template <class T>
+hipError_t dalloc(T** p, size_t count) {
+  *p = nullptr;
+  if (count == 0) return hipSuccess;
+  return hipMalloc((void**)p, count * sizeof(T));
+}
+
+template <class T>
+hipError_t upload(T** p, const std::vector<T>& v) {
+  hipError_t e = dalloc(p, v.size());
+  if (e != hipSuccess || v.empty()) return e;
+  return hipMemcpy(*p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+}
+
+double now_seconds() {
+  using namespace std::chrono;
+  return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+int k_slots(int P) {
+  int k = (P + WAVE - 1) / WAVE;
+  if (k <= 1) return 1;
+  if (k <= 2) return 2;
+  if (k <= 4) return 4;
+  return -1;
+}
+
+constexpr double HALF_LOG_2PI = 0.9189385332046728;
+constexpr double LOG_PI = 1.1447298858494002;
+constexpr double LOG_2 = 0.6931471805599453;
+
+}  // namespace
+
+// ---------------------------------------------------------------------------- plan
+struct ut_plan {
+  DevModelDesc dev;
+  // host copies for initial draws and validation
+  int P = 0;
+  std::vector<void*> owned;  // device allocations
+  ~ut_plan() {
+    for (void* p : owned) (void)hipFree(p);
+  }
+  template <class T>
+  int stage(const T** dst, const std::vector<T>& v) {
+    T* p = nullptr;
+    hipError_t e = upload(&p, v);
+    if (e != hipSuccess) return fail(UT_E_HIP, std::string("plan staging: ") + hipGetErrorString(e));
+    owned.push_back(p);
+    *dst = p;
+    return UT_OK;
+  }
+};
+
+struct ut_workspace {
+  const ut_plan* plan = nullptr;
+  int C = 0;
+  int P = 0;
+  int precision = UT_FLOAT32;
+  hipStream_t stream = nullptr;
+  // density scratch
+  void* d_params = nullptr;
+  void* d_values = nullptr;
+  void* d_grads = nullptr;
+  // sampler state
+  ChainState st;
+  double* d_im = nullptr;
+  double* d_summary = nullptr;
+  ~ut_workspace() {
+    void* ptrs[] = {d_params, d_values, d_grads, st.pos, st.grad, st.lj,
+                    st.step, st.dual, st.im_f, d_im, d_summary};
+    for (void* p : ptrs) (void)hipFree(p);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+extern "C" {
+
+const char* ut_last_error(void) { return g_last_error.c_str(); }
+int32_t ut_abi_version(void) { return 1; }
+
+int32_t ut_device_available(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return n > 0 ? 1 : 0;
+}
+
+void ut_philox4x32(const uint32_t key[2], const uint32_t ctr[4], uint32_t out[4]) {
+  philox_block b = philox4x32(key[0], key[1], ctr[0], ctr[1], ctr[2], ctr[3]);
+  for (int i = 0; i < 4; ++i) out[i] = b.w[i];
+}
+
+int ut_plan_logistic_glm(const float* x, const float* y, int64_t n, int32_t d, double prior_loc,
+                         double prior_scale, ut_plan** out) {
+  if (!out || !x || !y) return fail(UT_E_ARG, "ut_plan_logistic_glm: null pointer");
+  *out = nullptr;
+  if (n <= 0 || d <= 0) return fail(UT_E_DIM, "ut_plan_logistic_glm: n and d must be positive");
+  if (k_slots(d) < 0)
+    return fail(UT_E_UNSUPPORTED, "ut_plan_logistic_glm: d > 256 is not lowerable to the wavefront-per-chain kernels");
+  if (!(prior_scale > 0)) return fail(UT_E_ARG, "ut_plan_logistic_glm: prior_scale must be > 0");
+  if ((long long)n * d >= (1ll << 31)) return fail(UT_E_DIM, "ut_plan_logistic_glm: n*d too large");
+  for (int64_t i = 0; i < n; ++i)
+    if (!(y[i] == 0.0f || y[i] == 1.0f))
+      return fail(UT_E_ARG, "ut_plan_logistic_glm: y must hold 0/1 outcomes");
+  auto* plan = new ut_plan();
+  std::vector<float> xT_f((size_t)n * d);
+  std::vector<double> xT_d((size_t)n * d);
+  for (int64_t i = 0; i < n; ++i)
+    for (int j = 0; j < d; ++j) {
+      xT_f[(size_t)j * n + i] = x[(size_t)i * d + j];
+      xT_d[(size_t)j * n + i] = (double)x[(size_t)i * d + j];
+    }
+  std::vector<float> y_f(y, y + n);
+  std::vector<double> y_d(y, y + n);
+  DevModelDesc& m = plan->dev;
+  m.kind = MODEL_LOGISTIC;
+  m.P = d;
+  m.K = k_slots(d);
+  m.n = n;
+  m.D = d;
+  m.loc = prior_loc;
+  m.scale = prior_scale;
+  m.total_const = (double)d * (-log_d(prior_scale) - HALF_LOG_2PI);
+  int rc;
+  if ((rc = plan->stage(&m.xT_f, xT_f)) || (rc = plan->stage(&m.xT_d, xT_d)) ||
+      (rc = plan->stage(&m.yv_f, y_f)) || (rc = plan->stage(&m.yv_d, y_d))) {
+    delete plan;
+    return rc;
+  }
+  plan->P = d;
+  *out = plan;
+  return UT_OK;
+}
+
+int ut_plan_hier_normal_nc(const float* y, const int32_t* group_offsets, const double* sigma,
+                           int32_t num_groups, double mu_loc, double mu_scale, double tau_scale,
+                           ut_plan** out) {
+  if (!out || !y || !group_offsets || !sigma) return fail(UT_E_ARG, "ut_plan_hier_normal_nc: null pointer");
+  *out = nullptr;
+  if (num_groups <= 0 || num_groups > WAVE)
+    return fail(UT_E_UNSUPPORTED, "ut_plan_hier_normal_nc: 1 <= num_groups <= 64 required");
+  if (!(mu_scale > 0) || !(tau_scale > 0)) return fail(UT_E_ARG, "ut_plan_hier_normal_nc: scales must be > 0");
+  int max_n = 0;
+  for (int j = 0; j < num_groups; ++j) {
+    int nj = group_offsets[j + 1] - group_offsets[j];
+    if (nj < 0) return fail(UT_E_ARG, "ut_plan_hier_normal_nc: group_offsets must be non-decreasing");
+    if (!(sigma[j] > 0)) return fail(UT_E_ARG, "ut_plan_hier_normal_nc: sigma_j must be > 0");
+    max_n = nj > max_n ? nj : max_n;
+  }
+  auto* plan = new ut_plan();
+  std::vector<float> y_f((size_t)(max_n > 0 ? max_n : 1) * WAVE, 0.0f);
+  std::vector<double> y_d(y_f.size(), 0.0);
+  std::vector<int> nobs(WAVE, 0);
+  std::vector<float> lane_f(3 * WAVE, 0.0f);
+  std::vector<double> lane_d(3 * WAVE, 0.0);
+  for (int j = 0; j < num_groups; ++j) {
+    int nj = group_offsets[j + 1] - group_offsets[j];
+    nobs[j] = nj;
+    for (int i = 0; i < nj; ++i) {
+      float v = y[group_offsets[j] + i];
+      y_f[(size_t)i * WAVE + j] = v;
+      y_d[(size_t)i * WAVE + j] = (double)v;
+    }
+    const double s = sigma[j];
+    const double cj = -(double)nj * log_d(s) - (double)nj * HALF_LOG_2PI;
+    const double hinv = 1.0 / (2.0 * s * s);
+    const double inv = 1.0 / (s * s);
+    lane_d[j] = cj; lane_d[WAVE + j] = hinv; lane_d[2 * WAVE + j] = inv;
+    lane_f[j] = (float)cj; lane_f[WAVE + j] = (float)hinv; lane_f[2 * WAVE + j] = (float)inv;
+  }
+  DevModelDesc& m = plan->dev;
+  m.kind = MODEL_HIER;
+  m.P = num_groups + 2;
+  m.K = 2;
+  m.J = num_groups;
+  m.max_n = max_n;
+  m.mu_loc = mu_loc;
+  m.mu_scale = mu_scale;
+  m.tau_scale = tau_scale;
+  m.total_const = (double)num_groups * (-HALF_LOG_2PI) + (-log_d(mu_scale) - HALF_LOG_2PI) +
+                  (LOG_2 - LOG_PI - log_d(tau_scale));
+  int rc;
+  if ((rc = plan->stage(&m.y_f, y_f)) || (rc = plan->stage(&m.y_d, y_d)) ||
+      (rc = plan->stage(&m.nobs, nobs)) || (rc = plan->stage(&m.lane_f, lane_f)) ||
+      (rc = plan->stage(&m.lane_d, lane_d))) {
+    delete plan;
+    return rc;
+  }
+  plan->P = m.P;
+  *out = plan;
+  return UT_OK;
+}
+
+int ut_plan_funnel(int32_t dim, double v_scale, ut_plan** out) {
+  if (!out) return fail(UT_E_ARG, "ut_plan_funnel: null pointer");
+  *out = nullptr;
+  if (dim <= 0) return fail(UT_E_DIM, "ut_plan_funnel: dim must be positive");
+  if (k_slots(dim + 1) < 0) return fail(UT_E_UNSUPPORTED, "ut_plan_funnel: dim + 1 > 256 unsupported");
+  if (!(v_scale > 0)) return fail(UT_E_ARG, "ut_plan_funnel: v_scale must be > 0");
+  auto* plan = new ut_plan();
+  DevModelDesc& m = plan->dev;
+  m.kind = MODEL_FUNNEL;
+  m.P = dim + 1;
+  m.K = k_slots(dim + 1);
+  m.dim = dim;
+  m.v_scale = v_scale;
+  m.total_const = -(double)dim * HALF_LOG_2PI + (-log_d(v_scale) - HALF_LOG_2PI);
+  plan->P = m.P;
+  *out = plan;
+  return UT_OK;
+}
+
+int ut_plan_destroy(ut_plan* plan) {
+  delete plan;
+  return UT_OK;
+}
+
+int ut_plan_num_params(const ut_plan* plan, int32_t* num_params) {
+  if (!plan || !num_params) return fail(UT_E_ARG, "ut_plan_num_params: null pointer");
+  *num_params = plan->P;
+  return UT_OK;
+}
+
+int ut_workspace_create(const ut_plan* plan, int32_t num_chains, int32_t precision,
+                        ut_workspace** out) {
+  if (!plan || !out) return fail(UT_E_ARG, "ut_workspace_create: null pointer");
+  *out = nullptr;
+  if (num_chains <= 0) return fail(UT_E_DIM, "ut_workspace_create: num_chains must be positive");
+  if (precision != UT_FLOAT32 && precision != UT_FLOAT64)
+    return fail(UT_E_PRECISION, "ut_workspace_create: precision must be UT_FLOAT32 or UT_FLOAT64");
+  if (!ut_device_available()) return fail(UT_E_HIP, "ut_workspace_create: no HIP device visible");
+  auto* ws = new ut_workspace();
+  ws->plan = plan;
+  ws->C = num_chains;
+  ws->P = plan->P;
+  ws->precision = precision;
+  const size_t PC = (size_t)ws->P * num_chains;
+  const size_t es = precision == UT_FLOAT32 ? 4 : 8;
+  hipError_t e = hipStreamCreateWithFlags(&ws->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipMalloc(&ws->d_params, PC * es);
+  if (e == hipSuccess) e = hipMalloc(&ws->d_values, (size_t)num_chains * es);
+  if (e == hipSuccess) e = hipMalloc(&ws->d_grads, PC * es);
+  if (e == hipSuccess) e = dalloc(&ws->st.pos, PC);
+  if (e == hipSuccess) e = dalloc(&ws->st.grad, PC);
+  if (e == hipSuccess) e = dalloc(&ws->st.lj, (size_t)num_chains);
+  if (e == hipSuccess) e = dalloc(&ws->st.step, (size_t)num_chains);
+  if (e == hipSuccess) e = dalloc(&ws->st.dual, (size_t)num_chains * 5);
+  if (e == hipSuccess) e = dalloc(&ws->st.im_f, (size_t)ws->P);
+  if (e == hipSuccess) e = dalloc(&ws->d_im, (size_t)ws->P);
+  if (e == hipSuccess) e = dalloc(&ws->d_summary, 8);
+  if (e != hipSuccess) {
+    delete ws;
+    return fail(UT_E_HIP, std::string("ut_workspace_create: ") + hipGetErrorString(e));
+  }
+  *out = ws;
+  return UT_OK;
+}
+
+int ut_workspace_destroy(ut_workspace* ws) {
+  delete ws;
+  return UT_OK;
+}
+
+static int density_call(ut_workspace* ws, const void* params, void* values, void* gradients) {
+  if (!ws || !params || !values) return fail(UT_E_ARG, "density: null pointer");
+  const size_t PC = (size_t)ws->P * ws->C;
+  const size_t es = ws->precision == UT_FLOAT32 ? 4 : 8;
+  UT_HIP(hipMemcpyAsync(ws->d_params, params, PC * es, hipMemcpyHostToDevice, ws->stream));
+  hipError_t e;
+  if (ws->precision == UT_FLOAT32)
+    e = launch_density_f32(ws->plan->dev, (const float*)ws->d_params, (float*)ws->d_values,
+                           gradients ? (float*)ws->d_grads : nullptr, ws->C, ws->stream);
+  else
+    e = launch_density_f64(ws->plan->dev, (const double*)ws->d_params, (double*)ws->d_values,
+                           gradients ? (double*)ws->d_grads : nullptr, ws->C, ws->stream);
+  UT_HIP(e);
+  UT_HIP(hipMemcpyAsync(values, ws->d_values, (size_t)ws->C * es, hipMemcpyDeviceToHost, ws->stream));
+  if (gradients)
+    UT_HIP(hipMemcpyAsync(gradients, ws->d_grads, PC * es, hipMemcpyDeviceToHost, ws->stream));
+  UT_HIP(hipStreamSynchronize(ws->stream));
+  return UT_OK;
+}
+
+int ut_logjoint(ut_workspace* ws, const void* params, void* values) {
+  return density_call(ws, params, values, nullptr);
+}
+
+int ut_logjoint_gradient(ut_workspace* ws, const void* params, void* values, void* gradients) {
+  if (!gradients) return fail(UT_E_ARG, "ut_logjoint_gradient: null gradients");
+  return density_call(ws, params, values, gradients);
+}
+
+void ut_sampler_opts_default(ut_sampler_opts* o) {
+  if (!o) return;
+  std::memset(o, 0, sizeof(*o));
+  o->max_tree_depth = 10;
+  o->num_leapfrog_steps = 10;
+  o->step_size = 0.1;
+  o->target_accept = 0.8;
+  o->divergence_threshold = 1000.0;
+  o->mass_matrix_regularization = 1e-3;
+  o->mass_matrix_min_samples = 10;
+  o->adapt_step_size = 1;
+  o->adapt_mass_matrix = 1;
+  o->find_reasonable_step_size = 0;
+  o->init_strategy = UT_INIT_PRIOR;
+  o->init_max_retries = 100;
+  o->initial_params = nullptr;
+  o->seed = 0;
+}
+
+int ut_philox_fill(uint32_t chain, uint32_t iteration, uint32_t stream, uint32_t first, int64_t n,
+                   uint32_t* blocks, float* uniform_f32, float* normal_f32, double* uniform_f64) {
+  if (n < 0) return fail(UT_E_ARG, "ut_philox_fill: n < 0");
+  if (n == 0) return UT_OK;
+  if (!ut_device_available()) return fail(UT_E_HIP, "ut_philox_fill: no HIP device visible");
+  uint32_t* db = nullptr;
+  float *du = nullptr, *dn = nullptr;
+  double* dd = nullptr;
+  hipError_t e = hipSuccess;
+  if (blocks) e = dalloc(&db, (size_t)n * 4);
+  if (e == hipSuccess && uniform_f32) e = dalloc(&du, (size_t)n);
+  if (e == hipSuccess && normal_f32) e = dalloc(&dn, (size_t)n);
+  if (e == hipSuccess && uniform_f64) e = dalloc(&dd, (size_t)n);
+  if (e == hipSuccess) e = launch_philox_fill(chain, iteration, stream, first, n, db, du, dn, dd, nullptr);
+  if (e == hipSuccess && db) e = hipMemcpy(blocks, db, (size_t)n * 16, hipMemcpyDeviceToHost);
+  if (e == hipSuccess && du) e = hipMemcpy(uniform_f32, du, (size_t)n * 4, hipMemcpyDeviceToHost);
+  if (e == hipSuccess && dn) e = hipMemcpy(normal_f32, dn, (size_t)n * 4, hipMemcpyDeviceToHost);
+  if (e == hipSuccess && dd) e = hipMemcpy(uniform_f64, dd, (size_t)n * 8, hipMemcpyDeviceToHost);
+  (void)hipFree(db); (void)hipFree(du); (void)hipFree(dn); (void)hipFree(dd);
+  if (e != hipSuccess) return fail(UT_E_HIP, std::string("ut_philox_fill: ") + hipGetErrorString(e));
+  return UT_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- sampler driver
+namespace {
+
+struct Schedule {
+  int initial_buffer = 0;
+  std::vector<int> ends;
+  int terminal_buffer = 0;
+};
+
+// Stan-style warmup windows (reference nuts/adaptation.jl:103-138).
+Schedule warmup_schedule(int n) {
+  Schedule s;
+  if (n <= 0) return s;
+  if (n < 20) {
+    int ib = std::min(5, std::max(n - 1, 0));
+    s.initial_buffer = ib;
+    if (ib == n) { s.initial_buffer = n; return s; }
+    s.ends.push_back(n);
+    return s;
+  }
+  int ib = std::min(std::max(5, n / 10), n);
+  int tb = std::min(std::max(5, n / 10), std::max(n - ib, 0));
+  int slow = n - ib - tb;
+  if (slow <= 0) { s.initial_buffer = n; return s; }
+  s.initial_buffer = ib;
+  s.terminal_buffer = tb;
+  int next = std::min(25, slow);
+  int start = ib + 1;
+  int remaining = slow;
+  while (remaining > 0) {
+    int size = remaining <= (3 * next) / 2 ? remaining : next;
+    int end = start + size - 1;
+    s.ends.push_back(end);
+    remaining -= size;
+    start = end + 1;
+    next *= 2;
+  }
+  return s;
+}
+
+uint32_t run_seed_of(uint64_t seed) {
+  return philox4x32((uint32_t)seed, (uint32_t)(seed >> 32), 0x5EEDu, 0u, 0u, 0u).w[0];
+}
+
+// One column's initial unconstrained position (reference hmc_support.jl:266-290):
+// uniform(-2, 2) per coordinate, or a draw from the model prior mapped to the
+// unconstrained space.  Draws are Philox coordinates (chain, run_seed + attempt,
+// STREAM_INIT, index).
+void draw_initial_column(const DevModelDesc& m, int strategy, uint32_t chain, uint32_t key,
+                         double* col) {
+  const int P = m.P;
+  if (strategy == UT_INIT_UNIFORM) {
+    for (int a = 0; a < P; ++a) col[a] = 4.0 * rand_uniform_d(chain, key, STREAM_INIT, (uint32_t)a) - 2.0;
+    return;
+  }
+  auto z = [&](int idx) { return (double)rand_normal_f(chain, key, STREAM_INIT, (uint32_t)idx); };
+  if (m.kind == MODEL_LOGISTIC) {
+    for (int a = 0; a < P; ++a) col[a] = m.loc + m.scale * z(a);
+  } else if (m.kind == MODEL_HIER) {
+    col[0] = m.mu_loc + m.mu_scale * z(0);
+    const double tau = m.tau_scale * std::fabs(z(1) / z(P + 1));
+    col[1] = log_d(tau);
+    for (int j = 0; j < m.J; ++j) col[2 + j] = z(2 + j);
+  } else if (m.kind == MODEL_FUNNEL) {
+    const double v = m.v_scale * z(0);
+    col[0] = v;
+    const double s = exp_d(v / 2.0);
+    for (int i = 0; i < m.dim; ++i) col[1 + i] = s * z(1 + i);
+  }
+}
+
+struct DeviceBuffers {
+  std::vector<void*> ptrs;
+  ~DeviceBuffers() {
+    for (void* p : ptrs) (void)hipFree(p);
+  }
+  template <class T>
+  hipError_t alloc(T** p, size_t n) {
+    hipError_t e = dalloc(p, n);
+    if (e == hipSuccess && *p) ptrs.push_back(*p);
+    return e;
+  }
+};
+
+int validate_opts(const ut_workspace* ws, const ut_sampler_opts* o, const ut_chain_out* out,
+                  bool nuts) {
+  if (!ws || !o || !out) return fail(UT_E_ARG, "sampler: null pointer");
+  if (ws->precision != UT_FLOAT32)
+    return fail(UT_E_PRECISION, "sampler: the device samplers run in UT_FLOAT32 (binary64 energies)");
+  if (!out->samples) return fail(UT_E_ARG, "sampler: out->samples is required");
+  if (o->num_samples < 0 || o->num_warmup < 0) return fail(UT_E_ARG, "sampler: num_samples and num_warmup must be >= 0");
+  if (!(o->step_size > 0) || !std::isfinite(o->step_size)) return fail(UT_E_ARG, "sampler: step_size must be finite and > 0");
+  if (!(o->target_accept > 0 && o->target_accept < 1)) return fail(UT_E_ARG, "sampler: target_accept must lie in (0, 1)");
+  if (!(o->divergence_threshold > 0)) return fail(UT_E_ARG, "sampler: divergence_threshold must be > 0");
+  if (!(o->mass_matrix_regularization > 0)) return fail(UT_E_ARG, "sampler: mass_matrix_regularization must be > 0");
+  if (o->mass_matrix_min_samples < 1) return fail(UT_E_ARG, "sampler: mass_matrix_min_samples must be >= 1");
+  if (o->init_strategy != UT_INIT_PRIOR && o->init_strategy != UT_INIT_UNIFORM)
+    return fail(UT_E_ARG, "sampler: init_strategy must be UT_INIT_PRIOR or UT_INIT_UNIFORM");
+  if (o->init_max_retries < 0) return fail(UT_E_ARG, "sampler: init_max_retries must be >= 0");
+  if (nuts) {
+    if (o->max_tree_depth < 1 || o->max_tree_depth > 20) return fail(UT_E_ARG, "batched_nuts: max_tree_depth must lie in 1..20");
+  } else {
+    if (o->num_leapfrog_steps < 1) return fail(UT_E_ARG, "batched_hmc: num_leapfrog_steps must be >= 1");
+  }
+  if (ws->P > 1024) return fail(UT_E_UNSUPPORTED, "sampler: more than 1024 parameters");
+  return UT_OK;
+}
+
+// Iterations per sampler launch: long enough to amortise the launch, short enough
+// that one launch stays in the millisecond range.
+constexpr int kChunk = 50;
+
+int run_sampler(ut_workspace* ws, const ut_sampler_opts* o, ut_chain_out* out, bool nuts) {
+  int rc = validate_opts(ws, o, out, nuts);
+  if (rc) return rc;
+  const double t_start = now_seconds();
+  const int P = ws->P, C = ws->C, S = o->num_samples, W = o->num_warmup;
+  const DevModelDesc& m = ws->plan->dev;
+  hipStream_t stream = ws->stream;
+  const uint32_t rs = run_seed_of(o->seed);
+
+  // ---- initial positions, logjoint and gradient (retry non-finite columns)
+  std::vector<double> pos((size_t)P * C);
+  std::vector<float> pos_f((size_t)P * C);
+  std::vector<float> lj_h(C), grad_h((size_t)P * C);
+  if (o->initial_params) {
+    std::memcpy(pos.data(), o->initial_params, pos.size() * sizeof(double));
+  } else {
+    for (int c = 0; c < C; ++c) draw_initial_column(m, o->init_strategy, (uint32_t)c, rs, &pos[(size_t)c * P]);
+  }
+  for (int attempt = 0;; ++attempt) {
+    for (size_t i = 0; i < pos.size(); ++i) pos_f[i] = (float)pos[i];
+    UT_HIP(hipMemcpyAsync(ws->st.pos, pos_f.data(), pos_f.size() * 4, hipMemcpyHostToDevice, stream));
+    UT_HIP(launch_density_f32(m, ws->st.pos, ws->st.lj, ws->st.grad, C, stream));
+    UT_HIP(hipMemcpyAsync(lj_h.data(), ws->st.lj, (size_t)C * 4, hipMemcpyDeviceToHost, stream));
+    UT_HIP(hipMemcpyAsync(grad_h.data(), ws->st.grad, grad_h.size() * 4, hipMemcpyDeviceToHost, stream));
+    UT_HIP(hipStreamSynchronize(stream));
+    std::vector<int> bad;
+    for (int c = 0; c < C; ++c) {
+      bool ok = std::isfinite(lj_h[c]);
+      for (int a = 0; a < P && ok; ++a) ok = std::isfinite(grad_h[(size_t)c * P + a]);
+      if (!ok) bad.push_back(c);
+    }
+    if (bad.empty()) break;
+    if (o->initial_params || attempt >= o->init_max_retries) {
+      char buf[256];
+      std::snprintf(buf, sizeof(buf),
+                    "initial batched parameters produced a non-finite unconstrained logjoint or gradient in %zu of %d chain(s)%s",
+                    bad.size(), C, o->initial_params ? "" : " after the allowed re-draws");
+      return fail(UT_E_NONFINITE_INIT, buf);
+    }
+    for (int c : bad)
+      draw_initial_column(m, o->init_strategy, (uint32_t)c, rs + (uint32_t)(attempt + 1), &pos[(size_t)c * P]);
+  }
+
+  // ---- adaptation state
+  std::vector<double> im_h(P, 1.0);
+  std::vector<float> im_f(P, 1.0f);
+  UT_HIP(hipMemcpyAsync(ws->d_im, im_h.data(), P * 8, hipMemcpyHostToDevice, stream));
+  UT_HIP(hipMemcpyAsync(ws->st.im_f, im_f.data(), P * 4, hipMemcpyHostToDevice, stream));
+  std::vector<double> step_h(C, o->step_size);
+  UT_HIP(hipMemcpyAsync(ws->st.step, step_h.data(), (size_t)C * 8, hipMemcpyHostToDevice, stream));
+  const int do_search = (o->find_reasonable_step_size || (W > 0 && o->adapt_step_size)) ? 1 : 0;
+  UT_HIP(launch_step_search(m, ws->st, C, rs, do_search, o->target_accept, stream));
+
+  DeviceBuffers buf;
+  SamplerParams sp;
+  sp.run_seed = rs;
+  sp.P = P;
+  sp.C = C;
+  sp.max_depth = o->max_tree_depth;
+  sp.num_leapfrog = o->num_leapfrog_steps;
+  sp.div_thr = o->divergence_threshold;
+  sp.target_accept = o->target_accept;
+  sp.adapt_step = o->adapt_step_size ? 1 : 0;
+  sp.num_warmup = W;
+  sp.num_samples = S;
+  UT_HIP(buf.alloc(&sp.hist_pos, (size_t)W * C * P));
+  UT_HIP(buf.alloc(&sp.hist_acc, (size_t)W * C));
+  UT_HIP(buf.alloc(&sp.hist_flags, (size_t)W * C));
+  UT_HIP(buf.alloc(&sp.draws, (size_t)S * C * P));
+  UT_HIP(buf.alloc(&sp.o_lj, (size_t)S * C));
+  UT_HIP(buf.alloc(&sp.o_acc, (size_t)S * C));
+  UT_HIP(buf.alloc(&sp.o_energy, (size_t)S * C));
+  UT_HIP(buf.alloc(&sp.o_err, (size_t)S * C));
+  UT_HIP(buf.alloc(&sp.o_flags, (size_t)S * C));
+  UT_HIP(buf.alloc(&sp.o_depth, (size_t)S * C));
+  UT_HIP(buf.alloc(&sp.o_steps, (size_t)S * C));
+
+  auto run_range = [&](int a, int b) -> hipError_t {
+    const int chunk = kChunk;
+    for (int s = a; s <= b; s += chunk) {
+      const int e = std::min(b, s + chunk - 1);
+      hipError_t err = nuts ? launch_nuts(m, ws->st, sp, s, e, stream) : launch_hmc(m, ws->st, sp, s, e, stream);
+      if (err != hipSuccess) return err;
+    }
+    return hipSuccess;
+  };
+
+  // ---- warmup: one launch range per mass window, device replay at each window end
+  const Schedule sch = warmup_schedule(W);
+  std::vector<int> bounds;
+  if (o->adapt_mass_matrix)
+    for (int e : sch.ends) bounds.push_back(e);
+  if (W > 0 && (bounds.empty() || bounds.back() != W)) bounds.push_back(W);
+  int it = 1;
+  int window = 0;
+  for (int b : bounds) {
+    UT_HIP(run_range(it, b));
+    const bool is_window_end = o->adapt_mass_matrix && window < (int)sch.ends.size() && sch.ends[window] == b;
+    if (is_window_end) {
+      const int start = window == 0 ? sch.initial_buffer + 1 : sch.ends[window - 1] + 1;
+      const int wl = b - start + 1;
+      UT_HIP(launch_mass_replay(sp.hist_pos, sp.hist_acc, sp.hist_flags, C, P, start, b, wl, nuts ? 0 : 1,
+                                o->mass_matrix_regularization, o->mass_matrix_min_samples, ws->d_im,
+                                ws->st.im_f, ws->d_summary, stream));
+      if (o->adapt_step_size && b < W)
+        UT_HIP(launch_step_search(m, ws->st, C, rs + (uint32_t)b, 1, o->target_accept, stream));
+      ++window;
+    }
+    it = b + 1;
+  }
+  UT_HIP(hipStreamSynchronize(stream));
+  const double t_warm = now_seconds();
+
+  // ---- sampling (device time bracketed by events on the workspace stream)
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  UT_HIP(hipEventCreate(&ev0));
+  UT_HIP(hipEventCreate(&ev1));
+  UT_HIP(hipEventRecord(ev0, stream));
+  if (S > 0) UT_HIP(run_range(W + 1, W + S));
+  UT_HIP(hipEventRecord(ev1, stream));
+  const size_t SC = (size_t)S * C;
+  std::vector<uint8_t> flags(SC);
+  std::vector<int> steps(SC);
+  if (S > 0) {
+    UT_HIP(hipMemcpyAsync(out->samples, sp.draws, SC * P * 4, hipMemcpyDeviceToHost, stream));
+    if (out->logjoint) UT_HIP(hipMemcpyAsync(out->logjoint, sp.o_lj, SC * 8, hipMemcpyDeviceToHost, stream));
+    if (out->accept_stat) UT_HIP(hipMemcpyAsync(out->accept_stat, sp.o_acc, SC * 8, hipMemcpyDeviceToHost, stream));
+    if (out->energy) UT_HIP(hipMemcpyAsync(out->energy, sp.o_energy, SC * 8, hipMemcpyDeviceToHost, stream));
+    if (out->energy_error) UT_HIP(hipMemcpyAsync(out->energy_error, sp.o_err, SC * 8, hipMemcpyDeviceToHost, stream));
+    if (out->tree_depth) UT_HIP(hipMemcpyAsync(out->tree_depth, sp.o_depth, SC * 4, hipMemcpyDeviceToHost, stream));
+    UT_HIP(hipMemcpyAsync(flags.data(), sp.o_flags, SC, hipMemcpyDeviceToHost, stream));
+    UT_HIP(hipMemcpyAsync(steps.data(), sp.o_steps, SC * 4, hipMemcpyDeviceToHost, stream));
+  }
+  UT_HIP(hipStreamSynchronize(stream));
+  const double t_end = now_seconds();
+  float kernel_ms = 0.0f;
+  UT_HIP(hipEventElapsedTime(&kernel_ms, ev0, ev1));
+  (void)hipEventDestroy(ev0);
+  (void)hipEventDestroy(ev1);
+  out->sampling_kernel_seconds = 1e-3 * (double)kernel_ms;
+  out->sampling_launches = S > 0 ? (S + kChunk - 1) / kChunk : 0;
+  int64_t evals = 0;
+  for (size_t i = 0; i < SC; ++i) {
+    if (out->accepted) out->accepted[i] = (flags[i] & FLAG_ACCEPTED) ? 1 : 0;
+    if (out->divergent) out->divergent[i] = (flags[i] & FLAG_DIVERGENT) ? 1 : 0;
+    if (out->integration_steps) out->integration_steps[i] = steps[i];
+    evals += steps[i];
+  }
+  if (out->step_size) UT_HIP(hipMemcpy(out->step_size, ws->st.step, (size_t)C * 8, hipMemcpyDeviceToHost));
+  if (out->inverse_mass) UT_HIP(hipMemcpy(out->inverse_mass, ws->d_im, (size_t)P * 8, hipMemcpyDeviceToHost));
+  out->warmup_seconds = t_warm - t_start;
+  out->sampling_seconds = t_end - t_warm;
+  out->sampling_gradient_evals = evals;
+  return UT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ut_nuts_run(ut_workspace* ws, const ut_sampler_opts* opts, ut_chain_out* out) {
+  return run_sampler(ws, opts, out, true);
+}
+
+int ut_hmc_run(ut_workspace* ws, const ut_sampler_opts* opts, ut_chain_out* out) {
+  return run_sampler(ws, opts, out, false);
+}
+
+}  // extern "C"

@@ -1,0 +1,114 @@
+// ut_kernels.h -- host-side interface between the C ABI layer (ut_api.hip) and the
+// device kernels (ut_kernels.hip).  Internal to libuncertaintea_hip.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ut {
+
+enum ModelKind { MODEL_LOGISTIC = 1, MODEL_HIER = 2, MODEL_FUNNEL = 3 };
+
+// Philox stream ids.  NUTS uses 0..3 exactly as the reference's persistent kernel
+// (src/device/persistent_nuts.jl:70-73); the remaining purposes get their own ids.
+enum Stream : uint32_t {
+  STREAM_MOMENTUM = 0,
+  STREAM_DIRECTION = 1,
+  STREAM_LEAF = 2,
+  STREAM_MERGE = 3,
+  STREAM_STEP_SEARCH = 4,
+  STREAM_INIT = 5,
+  STREAM_HMC_ACCEPT = 6
+};
+
+// Per-iteration flag bits stored next to warmup history and draws.
+enum : uint8_t { FLAG_DIVERGENT = 1, FLAG_ACCEPTED = 2 };
+
+// Device description of a model: pointers into plan-owned HBM plus scalars.  Each
+// kernel launch builds the matching device model struct from it.
+struct DevModelDesc {
+  int kind = 0;
+  int P = 0;
+  int K = 0;
+  // hierarchical
+  const float* y_f = nullptr;
+  const double* y_d = nullptr;
+  const int* nobs = nullptr;
+  const float* lane_f = nullptr;
+  const double* lane_d = nullptr;
+  int J = 0;
+  int max_n = 0;
+  double mu_loc = 0, mu_scale = 1, tau_scale = 1;
+  // logistic
+  const float* xT_f = nullptr;
+  const double* xT_d = nullptr;
+  const float* yv_f = nullptr;
+  const double* yv_d = nullptr;
+  long long n = 0;
+  int D = 0;
+  double loc = 0, scale = 1;
+  // funnel
+  int dim = 0;
+  double v_scale = 1;
+  double total_const = 0;
+};
+
+// Device pointers of the chain state (api order, P x C column-major).
+struct ChainState {
+  float* pos = nullptr;
+  float* grad = nullptr;
+  float* lj = nullptr;
+  double* step = nullptr;   // C
+  double* dual = nullptr;   // C x 5: mu, log_step, log_step_avg, hbar, iteration
+  float* im_f = nullptr;    // P
+};
+
+struct SamplerParams {
+  uint32_t run_seed = 0;
+  int P = 0, C = 0;
+  int max_depth = 10;
+  int num_leapfrog = 10;
+  double div_thr = 1000.0;
+  double target_accept = 0.8;
+  int adapt_step = 1;
+  int num_warmup = 0;
+  int num_samples = 0;
+  float* hist_pos = nullptr;      // num_warmup x C x P
+  double* hist_acc = nullptr;     // num_warmup x C
+  uint8_t* hist_flags = nullptr;  // num_warmup x C
+  float* draws = nullptr;         // C x S x P
+  double* o_lj = nullptr;         // C x S
+  double* o_acc = nullptr;
+  double* o_energy = nullptr;
+  double* o_err = nullptr;
+  uint8_t* o_flags = nullptr;
+  int* o_depth = nullptr;
+  int* o_steps = nullptr;
+};
+
+// Launchers (all asynchronous on `stream`).  They return hipSuccess or the launch
+// error.
+hipError_t launch_density_f32(const DevModelDesc& m, const float* params, float* values,
+                              float* grads, int C, hipStream_t stream);
+hipError_t launch_density_f64(const DevModelDesc& m, const double* params, double* values,
+                              double* grads, int C, hipStream_t stream);
+hipError_t launch_nuts(const DevModelDesc& m, const ChainState& st, const SamplerParams& sp,
+                       int it_begin, int it_end, hipStream_t stream);
+hipError_t launch_hmc(const DevModelDesc& m, const ChainState& st, const SamplerParams& sp,
+                      int it_begin, int it_end, hipStream_t stream);
+// Per-chain reasonable-step-size search (do_search) and dual-averaging reset.
+hipError_t launch_step_search(const DevModelDesc& m, const ChainState& st, int C,
+                              uint32_t iteration_coord, int do_search, double target_accept,
+                              hipStream_t stream);
+// Replays the pooled running-variance updates of warmup iterations [it_lo, it_hi]
+// (1-based) and writes the window's inverse mass.  summary[0..4] = count, weight
+// sum, weight-square sum, effective count, updated flag.
+hipError_t launch_mass_replay(const float* hist_pos, const double* hist_acc,
+                              const uint8_t* hist_flags, int C, int P, int it_lo, int it_hi,
+                              int window_length, int use_accepted, double regularization,
+                              int min_samples, double* im_d, float* im_f, double* summary,
+                              hipStream_t stream);
+hipError_t launch_philox_fill(uint32_t chain, uint32_t iteration, uint32_t stream_id,
+                              uint32_t first, int64_t n, uint32_t* blocks, float* uf,
+                              float* nf, double* ud, hipStream_t stream);
+
+}  // namespace ut

@@ -1,0 +1,93 @@
+"""`UncertainTea.Device` API on the gfx950 backend.
+
+Mirrors the reference's device density entry points (src/device/api.jl:34-124):
+parameters are the P x C unconstrained matrix (column c = chain c), the output is
+the unconstrained logjoint (log density plus transform log-abs-det) per chain, and
+its gradient as a P x C matrix.  Every call is synchronous.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+
+import numpy as np
+
+from . import _lib as L
+from .models import TeaModel
+
+_PREC = {np.dtype(np.float32): L.UT_FLOAT32, np.dtype(np.float64): L.UT_FLOAT64}
+
+
+def _precision(precision) -> np.dtype:
+    dt = np.dtype(precision)
+    if dt not in _PREC:
+        raise ValueError(f"precision must be float32 or float64, got {dt}")
+    return dt
+
+
+class DeviceBatchedWorkspace:
+    """Device buffers + one HIP stream for `num_chains` chains of one model
+    (reference DeviceBatchedWorkspace, src/device/workspace.jl:71-162)."""
+
+    def __init__(self, model: TeaModel, num_chains: int, precision=np.float32):
+        self.model = model
+        self.num_chains = int(num_chains)
+        self.dtype = _precision(precision)
+        self.num_params = model.num_params
+        h = ct.c_void_p()
+        L.check(L.lib().ut_workspace_create(model.plan, self.num_chains, _PREC[self.dtype], ct.byref(h)))
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def __del__(self):
+        if getattr(self, "_h", None) is not None and L._lib is not None:
+            L._lib.ut_workspace_destroy(self._h)
+            self._h = None
+
+    def _params(self, params) -> np.ndarray:
+        a = np.asarray(params)
+        if a.ndim != 2:
+            raise ValueError("params must be a P x C matrix")
+        if a.shape[0] != self.num_params:
+            raise ValueError(f"expected {self.num_params} parameters per chain, got {a.shape[0]}")
+        if a.shape[1] != self.num_chains:
+            raise ValueError(f"DimensionMismatch: workspace has {self.num_chains} chains, got {a.shape[1]}")
+        # column-major P x C == row-major C x P: chain c's parameters are contiguous
+        return np.ascontiguousarray(a.T, dtype=self.dtype)
+
+
+def device_batched_logjoint_inplace(ws: DeviceBatchedWorkspace, params) -> np.ndarray:
+    p = ws._params(params)
+    out = np.empty(ws.num_chains, dtype=ws.dtype)
+    L.check(L.lib().ut_logjoint(ws.handle, L.ptr(p), L.ptr(out)))
+    return out
+
+
+def device_batched_logjoint_gradient_inplace(ws: DeviceBatchedWorkspace, params):
+    p = ws._params(params)
+    values = np.empty(ws.num_chains, dtype=ws.dtype)
+    grads = np.empty((ws.num_chains, ws.num_params), dtype=ws.dtype)
+    L.check(L.lib().ut_logjoint_gradient(ws.handle, L.ptr(p), L.ptr(values), L.ptr(grads)))
+    return values, grads.T
+
+
+def device_batched_logjoint(model: TeaModel, params, args=(), constraints=None, *, backend="hip",
+                            precision=np.float32) -> np.ndarray:
+    """Per-chain unconstrained logjoint of `params` (P x C)."""
+    if backend != "hip":
+        raise ValueError(f"backend must be 'hip', got {backend!r}")
+    params = np.asarray(params)
+    ws = DeviceBatchedWorkspace(model, params.shape[1], precision)
+    return device_batched_logjoint_inplace(ws, params)
+
+
+def device_batched_logjoint_gradient(model: TeaModel, params, args=(), constraints=None, *, backend="hip",
+                                     precision=np.float32):
+    """(values[C], gradients[P x C]) of the unconstrained logjoint."""
+    if backend != "hip":
+        raise ValueError(f"backend must be 'hip', got {backend!r}")
+    params = np.asarray(params)
+    ws = DeviceBatchedWorkspace(model, params.shape[1], precision)
+    return device_batched_logjoint_gradient_inplace(ws, params)

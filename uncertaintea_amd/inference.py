@@ -1,0 +1,192 @@
+"""Batched samplers on the gfx950 backend: `batched_nuts` and `batched_hmc`.
+
+Signatures and defaults follow the reference (src/inference/api_batched_nuts.jl:22-51,
+src/inference/api_batched_hmc.jl:16-39).  The device path is always the pooled-mass /
+per-chain-step adaptation the reference uses on devices
+(api_batched_nuts.jl:761-980, src/device/hmc_kernels.jl:306-723); all randomness is
+drawn on the device from Philox coordinates derived from `seed`, so a run is a pure
+function of its inputs.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib as L
+from . import diagnostics as D
+from .device import DeviceBatchedWorkspace
+from .models import TeaModel
+
+
+@dataclass
+class HMCChains:
+    """Stacked per-chain results (reference HMCChain fields, data_types.jl:55-97).
+
+    Arrays are chain-major: `unconstrained_samples[c, s, p]`, `logjoint_values[c, s]`.
+    """
+
+    sampler: str
+    model: TeaModel
+    unconstrained_samples: np.ndarray
+    logjoint_values: np.ndarray
+    acceptance_stats: np.ndarray
+    energies: np.ndarray
+    energy_errors: np.ndarray
+    accepted: np.ndarray
+    divergent: np.ndarray
+    tree_depths: np.ndarray
+    integration_steps: np.ndarray
+    step_size: np.ndarray
+    mass_matrix: np.ndarray
+    max_tree_depth: int
+    num_leapfrog_steps: int
+    target_accept: float
+    warmup_seconds: float
+    sampling_seconds: float
+    sampling_gradient_evals: int
+    sampling_kernel_seconds: float = 0.0
+    sampling_launches: int = 0
+    _constrained: np.ndarray | None = field(default=None, repr=False)
+
+    @property
+    def num_chains(self) -> int:
+        return self.unconstrained_samples.shape[0]
+
+    @property
+    def num_samples(self) -> int:
+        return self.unconstrained_samples.shape[1]
+
+    @property
+    def constrained_samples(self) -> np.ndarray:
+        if self._constrained is None:
+            self._constrained = self.model.constrain(self.unconstrained_samples)
+        return self._constrained
+
+    def draws(self, space: str = "constrained") -> np.ndarray:
+        if space == "constrained":
+            return self.constrained_samples
+        if space == "unconstrained":
+            return self.unconstrained_samples
+        raise ValueError(f"space must be 'constrained' or 'unconstrained', got {space!r}")
+
+    def rhat(self, space: str = "constrained", method: str = "rank") -> np.ndarray:
+        return D.rhat(self.draws(space), method=method)
+
+    def ess(self, space: str = "constrained") -> np.ndarray:
+        return D.ess(self.draws(space))
+
+    def ess_bulk(self, space: str = "constrained") -> np.ndarray:
+        return D.ess_bulk(self.draws(space))
+
+    def divergence_rate(self) -> float:
+        return float(self.divergent.mean()) if self.divergent.size else 0.0
+
+
+def _opts(num_samples, num_warmup, step_size, target_accept, divergence_threshold, mass_matrix_regularization,
+          mass_matrix_min_samples, adapt_step_size, adapt_mass_matrix, find_reasonable_step_size, init_strategy,
+          init_max_retries, seed, max_tree_depth=10, num_leapfrog_steps=10) -> L.SamplerOpts:
+    o = L.SamplerOpts()
+    L.lib().ut_sampler_opts_default(ct.byref(o))
+    if init_strategy not in ("prior", "uniform"):
+        raise ValueError(f"init_strategy must be 'prior' or 'uniform', got {init_strategy!r}")
+    o.num_samples = int(num_samples)
+    o.num_warmup = int(num_warmup)
+    o.max_tree_depth = int(max_tree_depth)
+    o.num_leapfrog_steps = int(num_leapfrog_steps)
+    o.step_size = float(step_size)
+    o.target_accept = float(target_accept)
+    o.divergence_threshold = float(divergence_threshold)
+    o.mass_matrix_regularization = float(mass_matrix_regularization)
+    o.mass_matrix_min_samples = int(mass_matrix_min_samples)
+    o.adapt_step_size = int(bool(adapt_step_size))
+    o.adapt_mass_matrix = int(bool(adapt_mass_matrix))
+    o.find_reasonable_step_size = int(bool(find_reasonable_step_size))
+    o.init_strategy = L.UT_INIT_PRIOR if init_strategy == "prior" else L.UT_INIT_UNIFORM
+    o.init_max_retries = int(init_max_retries)
+    o.seed = int(seed) & 0xFFFFFFFFFFFFFFFF
+    return o
+
+
+def _run(kind: str, model: TeaModel, num_chains: int, opts: L.SamplerOpts, initial_params,
+         max_tree_depth: int, num_leapfrog_steps: int, target_accept: float) -> HMCChains:
+    if num_chains <= 0:
+        raise ValueError("num_chains must be positive")
+    ws = DeviceBatchedWorkspace(model, num_chains, np.float32)
+    P, C, S = model.num_params, int(num_chains), int(opts.num_samples)
+    init = None
+    if initial_params is not None:
+        init = np.asarray(initial_params, dtype=np.float64)
+        if init.shape == (P,):
+            init = np.repeat(init[:, None], C, axis=1)
+        if init.shape != (P, C):
+            raise ValueError(f"initial_params must be P x C = {P} x {C}")
+        init = np.ascontiguousarray(init.T)  # chain-major
+        opts.initial_params = L.dptr(init, ct.c_double)
+    samples = np.empty((C, S, P), dtype=np.float32)
+    lj = np.empty((C, S)); acc = np.empty((C, S)); en = np.empty((C, S)); err = np.empty((C, S))
+    accepted = np.empty((C, S), dtype=np.uint8); div = np.empty((C, S), dtype=np.uint8)
+    depth = np.empty((C, S), dtype=np.int32); steps = np.empty((C, S), dtype=np.int32)
+    step = np.empty(C); im = np.empty(P)
+    out = L.ChainOut()
+    out.samples = samples.ctypes.data_as(ct.c_void_p)
+    out.logjoint = L.dptr(lj, ct.c_double)
+    out.accept_stat = L.dptr(acc, ct.c_double)
+    out.energy = L.dptr(en, ct.c_double)
+    out.energy_error = L.dptr(err, ct.c_double)
+    out.accepted = L.dptr(accepted, ct.c_uint8)
+    out.divergent = L.dptr(div, ct.c_uint8)
+    out.tree_depth = L.dptr(depth, ct.c_int32)
+    out.integration_steps = L.dptr(steps, ct.c_int32)
+    out.step_size = L.dptr(step, ct.c_double)
+    out.inverse_mass = L.dptr(im, ct.c_double)
+    fn = L.lib().ut_nuts_run if kind == "nuts" else L.lib().ut_hmc_run
+    L.check(fn(ws.handle, ct.byref(opts), ct.byref(out)))
+    return HMCChains(kind, model, samples, lj, acc, en, err, accepted.astype(bool), div.astype(bool), depth,
+                     steps, step, im, max_tree_depth, num_leapfrog_steps, target_accept, out.warmup_seconds,
+                     out.sampling_seconds, int(out.sampling_gradient_evals), out.sampling_kernel_seconds,
+                     int(out.sampling_launches))
+
+
+def batched_nuts(model: TeaModel, args=(), constraints=None, *, num_chains: int, num_samples: int,
+                 num_warmup: int = 0, step_size: float = 0.1, max_tree_depth: int = 10, initial_params=None,
+                 init_strategy: str = "prior", init_max_retries: int = 100, target_accept: float = 0.8,
+                 adapt_step_size: bool = True, adapt_mass_matrix: bool = True, per_chain_adaptation=None,
+                 find_reasonable_step_size: bool = False, divergence_threshold: float = 1000.0,
+                 mass_matrix_regularization: float = 1e-3, mass_matrix_min_samples: int = 10,
+                 tree_strategy: str = "persistent", backend: str = "hip", precision=np.float32,
+                 seed: int = 0) -> HMCChains:
+    """Many NUTS chains, each building its whole tree in one persistent kernel."""
+    if tree_strategy != "persistent":
+        raise ValueError(f"the gfx950 backend implements tree_strategy='persistent', got {tree_strategy!r}")
+    if backend != "hip":
+        raise ValueError(f"backend must be 'hip', got {backend!r}")
+    if per_chain_adaptation is False:
+        raise ValueError("the device path adapts a per-chain step with a pooled mass (per_chain_adaptation=True)")
+    if np.dtype(precision) != np.float32:
+        raise ValueError("the device samplers run in float32 with float64 energies")
+    o = _opts(num_samples, num_warmup, step_size, target_accept, divergence_threshold, mass_matrix_regularization,
+              mass_matrix_min_samples, adapt_step_size, adapt_mass_matrix, find_reasonable_step_size, init_strategy,
+              init_max_retries, seed, max_tree_depth=max_tree_depth)
+    return _run("nuts", model, num_chains, o, initial_params, max_tree_depth, 0, target_accept)
+
+
+def batched_hmc(model: TeaModel, args=(), constraints=None, *, num_chains: int, num_samples: int,
+                num_warmup: int = 0, step_size: float = 0.1, num_leapfrog_steps: int = 10, initial_params=None,
+                init_strategy: str = "prior", init_max_retries: int = 100, target_accept: float = 0.8,
+                adapt_step_size: bool = True, adapt_mass_matrix: bool = True, per_chain_adaptation=None,
+                find_reasonable_step_size: bool = False, divergence_threshold: float = 1000.0,
+                mass_matrix_regularization: float = 1e-3, mass_matrix_min_samples: int = 10,
+                backend: str = "hip", precision=np.float32, seed: int = 0) -> HMCChains:
+    """Many fixed-length HMC chains, device-resident (momentum, accept and adaptation on the GPU)."""
+    if backend != "hip":
+        raise ValueError(f"backend must be 'hip', got {backend!r}")
+    if per_chain_adaptation is False:
+        raise ValueError("the device path adapts a per-chain step with a pooled mass (per_chain_adaptation=True)")
+    if np.dtype(precision) != np.float32:
+        raise ValueError("the device samplers run in float32 with float64 energies")
+    o = _opts(num_samples, num_warmup, step_size, target_accept, divergence_threshold, mass_matrix_regularization,
+              mass_matrix_min_samples, adapt_step_size, adapt_mass_matrix, find_reasonable_step_size, init_strategy,
+              init_max_retries, seed, num_leapfrog_steps=num_leapfrog_steps)
+    return _run("hmc", model, num_chains, o, initial_params, 0, num_leapfrog_steps, target_accept)

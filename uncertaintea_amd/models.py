@@ -1,0 +1,146 @@
+"""Static models lowered to device plans.
+
+The reference lowers a `@tea (static)` model to an isbits DeviceExecutionPlan and
+stages its observations (src/device/plan/lowering.jl:1360-1478,
+src/device/staging.jl:499-550).  Here each supported model family is a class that
+holds its data and creates the matching `ut_plan` through the C ABI; the plan
+keeps the data resident in HBM for its lifetime.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+
+import numpy as np
+
+from . import _lib as L
+
+
+class TeaModel:
+    """A static model with a fixed parameter layout (unconstrained order)."""
+
+    name = "model"
+
+    def __init__(self):
+        self._plan = None
+
+    # layout ---------------------------------------------------------------
+    @property
+    def num_params(self) -> int:
+        raise NotImplementedError
+
+    @property
+    def param_names(self) -> list[str]:
+        raise NotImplementedError
+
+    def constrain(self, unconstrained: np.ndarray) -> np.ndarray:
+        """Map unconstrained draws (..., P) to constrained values (..., P)."""
+        return np.asarray(unconstrained, dtype=np.float64)
+
+    # device plan ------------------------------------------------------------
+    def _create_plan(self, out: ct.c_void_p) -> int:
+        raise NotImplementedError
+
+    @property
+    def plan(self) -> ct.c_void_p:
+        if self._plan is None:
+            h = ct.c_void_p()
+            L.check(self._create_plan(ct.byref(h)))
+            self._plan = h
+        return self._plan
+
+    def __del__(self):
+        if getattr(self, "_plan", None) is not None and L._lib is not None:
+            L._lib.ut_plan_destroy(self._plan)
+            self._plan = None
+
+
+class LogisticRegression(TeaModel):
+    """beta_d ~ normal(prior_loc, prior_scale); y_i ~ bernoullilogit(x_i . beta)."""
+
+    name = "logistic_regression"
+
+    def __init__(self, x: np.ndarray, y: np.ndarray, prior_loc: float = 0.0, prior_scale: float = 1.0):
+        super().__init__()
+        self.x = np.ascontiguousarray(x, dtype=np.float32)
+        self.y = np.ascontiguousarray(y, dtype=np.float32).reshape(-1)
+        if self.x.ndim != 2 or self.x.shape[0] != self.y.shape[0]:
+            raise ValueError("DimensionMismatch: x must be N x D with N = len(y)")
+        self.prior_loc = float(prior_loc)
+        self.prior_scale = float(prior_scale)
+
+    @property
+    def num_params(self):
+        return self.x.shape[1]
+
+    @property
+    def param_names(self):
+        return [f"beta[{d + 1}]" for d in range(self.num_params)]
+
+    def _create_plan(self, out):
+        n, d = self.x.shape
+        return L.lib().ut_plan_logistic_glm(L.ptr(self.x), L.ptr(self.y), n, d, self.prior_loc,
+                                            self.prior_scale, out)
+
+
+class HierarchicalNormalNC(TeaModel):
+    """Non-centred hierarchical normal.
+
+    mu ~ normal(mu_loc, mu_scale); tau ~ halfcauchy(tau_scale);
+    theta_raw_j ~ normal(0, 1); theta_j = mu + tau theta_raw_j; y_ij ~ normal(theta_j, sigma_j).
+    Unconstrained layout: (mu, log tau, theta_raw_1..J).
+    """
+
+    name = "hierarchical_normal_nc"
+
+    def __init__(self, y: np.ndarray, group_offsets: np.ndarray, sigma: np.ndarray, mu_loc: float = 0.0,
+                 mu_scale: float = 5.0, tau_scale: float = 5.0):
+        super().__init__()
+        self.y = np.ascontiguousarray(y, dtype=np.float32).reshape(-1)
+        self.offsets = np.ascontiguousarray(group_offsets, dtype=np.int32)
+        self.sigma = np.ascontiguousarray(sigma, dtype=np.float64)
+        if self.offsets.shape[0] != self.sigma.shape[0] + 1 or self.offsets[-1] != self.y.shape[0]:
+            raise ValueError("DimensionMismatch: group_offsets must have J+1 entries ending at len(y)")
+        self.mu_loc, self.mu_scale, self.tau_scale = float(mu_loc), float(mu_scale), float(tau_scale)
+
+    @property
+    def num_groups(self):
+        return self.sigma.shape[0]
+
+    @property
+    def num_params(self):
+        return self.num_groups + 2
+
+    @property
+    def param_names(self):
+        return ["mu", "tau"] + [f"theta_raw[{j + 1}]" for j in range(self.num_groups)]
+
+    def constrain(self, u):
+        c = np.array(u, dtype=np.float64, copy=True)
+        c[..., 1] = np.exp(c[..., 1])
+        return c
+
+    def _create_plan(self, out):
+        return L.lib().ut_plan_hier_normal_nc(L.ptr(self.y), L.ptr(self.offsets), L.ptr(self.sigma),
+                                              self.num_groups, self.mu_loc, self.mu_scale, self.tau_scale, out)
+
+
+class NealsFunnel(TeaModel):
+    """v ~ normal(0, v_scale); x_i ~ normal(0, exp(v/2)), i = 1..dim.  Layout (v, x_1..x_dim)."""
+
+    name = "neals_funnel"
+
+    def __init__(self, dim: int = 100, v_scale: float = 3.0):
+        super().__init__()
+        self.dim = int(dim)
+        self.v_scale = float(v_scale)
+
+    @property
+    def num_params(self):
+        return self.dim + 1
+
+    @property
+    def param_names(self):
+        return ["v"] + [f"x[{i + 1}]" for i in range(self.dim)]
+
+    def _create_plan(self, out):
+        return L.lib().ut_plan_funnel(self.dim, self.v_scale, out)
