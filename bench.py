@@ -50,7 +50,7 @@ def cpu_cores() -> int:
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000, help="sampling iterations")
+    ap.add_argument("--steps", type=int, default=3000, help="sampling iterations")
     ap.add_argument("--warmup", type=int, default=1000, help="warmup iterations")
     ap.add_argument("--chains", type=int, default=1024)
     ap.add_argument("--seed", type=int, default=20261203)
@@ -87,8 +87,7 @@ def main() -> int:
     wall = time.perf_counter() - t0
     log(f"bench: sampler done warmup {res.warmup_seconds:.3f}s sampling {res.sampling_seconds:.3f}s wall {wall:.3f}s")
     draws = res.constrained_samples
-    rh = D.rhat(draws)
-    eb = D.ess_bulk(draws)
+    rh, eb = D.rhat_and_ess_bulk(draws)
     min_ess = float(np.nanmin(eb))
     rhat_max = float(np.nanmax(rh))
     value = min_ess / res.sampling_seconds
@@ -112,7 +111,7 @@ def main() -> int:
         pos = res.unconstrained_samples[:, -1, :].astype(np.float64).T.copy()
         it = args.warmup + args.steps + 1
         n_iter, n_evals, t_cpu = 0, 0, 0.0
-        while t_cpu < args.cpu_seconds and n_iter < 50:
+        while t_cpu < args.cpu_seconds and n_iter < 5000:
             ts = time.perf_counter()
             ev, pos = B.nuts_sample_from(om, "f64", pos, res.step_size, res.mass_matrix, it + n_iter, 1,
                                          seed=args.seed, threads=cores)

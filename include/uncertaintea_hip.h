@@ -176,6 +176,14 @@ int ut_nuts_run(ut_workspace* ws, const ut_sampler_opts* opts, ut_chain_out* out
  * (reference _run_device_batched_hmc, src/device/hmc_kernels.jl:306-723).      */
 int ut_hmc_run(ut_workspace* ws, const ut_sampler_opts* opts, ut_chain_out* out);
 
+/* ---- pinned host memory ----------------------------------------------------------
+ * Page-locked host buffers for sampler outputs: the samplers copy each launch
+ * range's draws out while the next range computes, at full link bandwidth when the
+ * destination is pinned.  No reference counterpart (the reference returns Julia
+ * arrays); used by the host layer to allocate ut_chain_out buffers.              */
+int ut_host_alloc(int64_t bytes, void** out);
+int ut_host_free(void* ptr);
+
 /* ---- RNG -----------------------------------------------------------------------
  * Host evaluation of one Philox4x32-10 block (reference philox4x32, rng.jl:219). */
 void ut_philox4x32(const uint32_t key[2], const uint32_t ctr[4], uint32_t out[4]);

@@ -167,16 +167,20 @@ double ref_eval(const Model& m, const double* q, double* g, bool& ok) {
 }
 
 // ------------------------------------------------------------------ TwinF32 models
-// Vectors are K x 64 (slot-major); bfly() is the kernels' xor butterfly.
+// Vectors are K x 64 (slot-major).  bfly() is the kernels' wavefront reduction:
+// lane l adds lane l^1, l^2, l^7 and l^15 (DPP quad permutes, half-row mirror and
+// row mirror inside each 16-lane row), then the four row totals combine as
+// (R0 + R1) + (R2 + R3).
 template <class T>
 T bfly(const T* in) {
   T v[WAVE], w[WAVE];
   for (int l = 0; l < WAVE; ++l) v[l] = in[l];
-  for (int mm = 1; mm < WAVE; mm <<= 1) {
-    for (int l = 0; l < WAVE; ++l) w[l] = v[l] + v[l ^ mm];
+  const int partner_mask[4] = {1, 2, 7, 15};
+  for (int s = 0; s < 4; ++s) {
+    for (int l = 0; l < WAVE; ++l) w[l] = v[l] + v[l ^ partner_mask[s]];
     for (int l = 0; l < WAVE; ++l) v[l] = w[l];
   }
-  return v[0];
+  return (v[0] + v[16]) + (v[32] + v[48]);
 }
 
 template <class T> T texp(T x);

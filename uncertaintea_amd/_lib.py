@@ -32,7 +32,7 @@ EXPORTED = (
     "ut_workspace_create", "ut_workspace_destroy",
     "ut_logjoint", "ut_logjoint_gradient",
     "ut_sampler_opts_default", "ut_nuts_run", "ut_hmc_run",
-    "ut_philox4x32", "ut_philox_fill",
+    "ut_philox4x32", "ut_philox_fill", "ut_host_alloc", "ut_host_free",
 )
 
 
@@ -119,6 +119,8 @@ def lib() -> ct.CDLL:
         "ut_hmc_run": (ct.c_int, [vp, ct.POINTER(SamplerOpts), ct.POINTER(ChainOut)]),
         "ut_philox4x32": (None, [vp, vp, vp]),
         "ut_philox_fill": (ct.c_int, [u32, u32, u32, u32, i64, vp, vp, vp, vp]),
+        "ut_host_alloc": (ct.c_int, [i64, pp]),
+        "ut_host_free": (ct.c_int, [vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -148,6 +150,34 @@ def ptr(a: np.ndarray | None):
 
 def dptr(a: np.ndarray | None, ctype):
     return None if a is None else a.ctypes.data_as(ct.POINTER(ctype))
+
+
+class _PinnedBlock:
+    """Owner of one page-locked allocation, exposed through the numpy array
+    interface; numpy arrays made from it keep it alive and free it last."""
+
+    def __init__(self, shape, dtype):
+        self.ptr = ct.c_void_p()
+        self.shape = tuple(int(s) for s in shape)
+        self.dtype = np.dtype(dtype)
+        nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        check(lib().ut_host_alloc(nbytes, ct.byref(self.ptr)))
+
+    @property
+    def __array_interface__(self):
+        return {"data": (self.ptr.value, False), "shape": self.shape, "typestr": self.dtype.str, "version": 3}
+
+    def __del__(self):
+        if self.ptr and _lib is not None:
+            _lib.ut_host_free(self.ptr)
+            self.ptr = None
+
+
+def pinned_empty(shape, dtype) -> np.ndarray:
+    """numpy array backed by page-locked host memory (allocated by the library)."""
+    if int(np.prod(shape)) == 0:
+        return np.empty(shape, dtype=dtype)
+    return np.asarray(_PinnedBlock(shape, dtype))
 
 
 def device_available() -> bool:

@@ -97,6 +97,7 @@ struct ut_workspace {
   int P = 0;
   int precision = UT_FLOAT32;
   hipStream_t stream = nullptr;
+  hipStream_t copy_stream = nullptr;  // result copy-out overlapping compute
   // density scratch
   void* d_params = nullptr;
   void* d_values = nullptr;
@@ -110,6 +111,7 @@ struct ut_workspace {
                     st.step, st.dual, st.im_f, d_im, d_summary};
     for (void* p : ptrs) (void)hipFree(p);
     if (stream) (void)hipStreamDestroy(stream);
+    if (copy_stream) (void)hipStreamDestroy(copy_stream);
   }
 };
 
@@ -279,6 +281,7 @@ int ut_workspace_create(const ut_plan* plan, int32_t num_chains, int32_t precisi
   const size_t PC = (size_t)ws->P * num_chains;
   const size_t es = precision == UT_FLOAT32 ? 4 : 8;
   hipError_t e = hipStreamCreateWithFlags(&ws->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&ws->copy_stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipMalloc(&ws->d_params, PC * es);
   if (e == hipSuccess) e = hipMalloc(&ws->d_values, (size_t)num_chains * es);
   if (e == hipSuccess) e = hipMalloc(&ws->d_grads, PC * es);
@@ -349,6 +352,22 @@ void ut_sampler_opts_default(ut_sampler_opts* o) {
   o->init_max_retries = 100;
   o->initial_params = nullptr;
   o->seed = 0;
+}
+
+int ut_host_alloc(int64_t bytes, void** out) {
+  if (!out || bytes < 0) return fail(UT_E_ARG, "ut_host_alloc: invalid argument");
+  *out = nullptr;
+  if (bytes == 0) return UT_OK;
+  hipError_t e = hipHostMalloc(out, (size_t)bytes, hipHostMallocDefault);
+  if (e != hipSuccess) return fail(UT_E_HIP, std::string("ut_host_alloc: ") + hipGetErrorString(e));
+  return UT_OK;
+}
+
+int ut_host_free(void* p) {
+  if (!p) return UT_OK;
+  hipError_t e = hipHostFree(p);
+  if (e != hipSuccess) return fail(UT_E_HIP, std::string("ut_host_free: ") + hipGetErrorString(e));
+  return UT_OK;
 }
 
 int ut_philox_fill(uint32_t chain, uint32_t iteration, uint32_t stream, uint32_t first, int64_t n,
@@ -561,7 +580,10 @@ int run_sampler(ut_workspace* ws, const ut_sampler_opts* o, ut_chain_out* out, b
   UT_HIP(buf.alloc(&sp.o_acc, (size_t)S * C));
   UT_HIP(buf.alloc(&sp.o_energy, (size_t)S * C));
   UT_HIP(buf.alloc(&sp.o_err, (size_t)S * C));
-  UT_HIP(buf.alloc(&sp.o_flags, (size_t)S * C));
+  UT_HIP(buf.alloc(&sp.o_accepted, (size_t)S * C));
+  UT_HIP(buf.alloc(&sp.o_divergent, (size_t)S * C));
+  UT_HIP(buf.alloc(&sp.eval_counter, 1));
+  UT_HIP(hipMemsetAsync(sp.eval_counter, 0, sizeof(unsigned long long), stream));
   UT_HIP(buf.alloc(&sp.o_depth, (size_t)S * C));
   UT_HIP(buf.alloc(&sp.o_steps, (size_t)S * C));
 
@@ -601,46 +623,59 @@ int run_sampler(ut_workspace* ws, const ut_sampler_opts* o, ut_chain_out* out, b
   UT_HIP(hipStreamSynchronize(stream));
   const double t_warm = now_seconds();
 
-  // ---- sampling (device time bracketed by events on the workspace stream)
+  // ---- sampling: device time bracketed by events on the workspace stream; each
+  // launch range's results are copied out on the copy stream while the next range
+  // computes (strided 2-D copies: the outputs are chain-major).
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   UT_HIP(hipEventCreate(&ev0));
   UT_HIP(hipEventCreate(&ev1));
+  std::vector<hipEvent_t> done;
   UT_HIP(hipEventRecord(ev0, stream));
-  if (S > 0) UT_HIP(run_range(W + 1, W + S));
-  UT_HIP(hipEventRecord(ev1, stream));
-  const size_t SC = (size_t)S * C;
-  std::vector<uint8_t> flags(SC);
-  std::vector<int> steps(SC);
-  if (S > 0) {
-    UT_HIP(hipMemcpyAsync(out->samples, sp.draws, SC * P * 4, hipMemcpyDeviceToHost, stream));
-    if (out->logjoint) UT_HIP(hipMemcpyAsync(out->logjoint, sp.o_lj, SC * 8, hipMemcpyDeviceToHost, stream));
-    if (out->accept_stat) UT_HIP(hipMemcpyAsync(out->accept_stat, sp.o_acc, SC * 8, hipMemcpyDeviceToHost, stream));
-    if (out->energy) UT_HIP(hipMemcpyAsync(out->energy, sp.o_energy, SC * 8, hipMemcpyDeviceToHost, stream));
-    if (out->energy_error) UT_HIP(hipMemcpyAsync(out->energy_error, sp.o_err, SC * 8, hipMemcpyDeviceToHost, stream));
-    if (out->tree_depth) UT_HIP(hipMemcpyAsync(out->tree_depth, sp.o_depth, SC * 4, hipMemcpyDeviceToHost, stream));
-    UT_HIP(hipMemcpyAsync(flags.data(), sp.o_flags, SC, hipMemcpyDeviceToHost, stream));
-    UT_HIP(hipMemcpyAsync(steps.data(), sp.o_steps, SC * 4, hipMemcpyDeviceToHost, stream));
+  for (int s0 = 0; s0 < S; s0 += kChunk) {
+    const int s1 = std::min(S, s0 + kChunk);
+    hipError_t e = nuts ? launch_nuts(m, ws->st, sp, W + 1 + s0, W + s1, stream)
+                        : launch_hmc(m, ws->st, sp, W + 1 + s0, W + s1, stream);
+    UT_HIP(e);
+    hipEvent_t ev;
+    UT_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    done.push_back(ev);
+    UT_HIP(hipEventRecord(ev, stream));
+    UT_HIP(hipStreamWaitEvent(ws->copy_stream, ev, 0));
+    auto copy2d = [&](void* dst, const void* src, size_t elem) -> hipError_t {
+      if (!dst) return hipSuccess;
+      const size_t pitch = (size_t)S * elem;
+      return hipMemcpy2DAsync((char*)dst + (size_t)s0 * elem, pitch, (const char*)src + (size_t)s0 * elem,
+                              pitch, (size_t)(s1 - s0) * elem, (size_t)C, hipMemcpyDeviceToHost,
+                              ws->copy_stream);
+    };
+    UT_HIP(copy2d(out->samples, sp.draws, (size_t)P * 4));
+    UT_HIP(copy2d(out->logjoint, sp.o_lj, 8));
+    UT_HIP(copy2d(out->accept_stat, sp.o_acc, 8));
+    UT_HIP(copy2d(out->energy, sp.o_energy, 8));
+    UT_HIP(copy2d(out->energy_error, sp.o_err, 8));
+    UT_HIP(copy2d(out->accepted, sp.o_accepted, 1));
+    UT_HIP(copy2d(out->divergent, sp.o_divergent, 1));
+    UT_HIP(copy2d(out->tree_depth, sp.o_depth, 4));
+    UT_HIP(copy2d(out->integration_steps, sp.o_steps, 4));
   }
+  UT_HIP(hipEventRecord(ev1, stream));
+  unsigned long long evals = 0;
+  UT_HIP(hipMemcpyAsync(&evals, sp.eval_counter, sizeof(evals), hipMemcpyDeviceToHost, stream));
   UT_HIP(hipStreamSynchronize(stream));
+  UT_HIP(hipStreamSynchronize(ws->copy_stream));
   const double t_end = now_seconds();
   float kernel_ms = 0.0f;
   UT_HIP(hipEventElapsedTime(&kernel_ms, ev0, ev1));
   (void)hipEventDestroy(ev0);
   (void)hipEventDestroy(ev1);
+  for (hipEvent_t ev : done) (void)hipEventDestroy(ev);
   out->sampling_kernel_seconds = 1e-3 * (double)kernel_ms;
-  out->sampling_launches = S > 0 ? (S + kChunk - 1) / kChunk : 0;
-  int64_t evals = 0;
-  for (size_t i = 0; i < SC; ++i) {
-    if (out->accepted) out->accepted[i] = (flags[i] & FLAG_ACCEPTED) ? 1 : 0;
-    if (out->divergent) out->divergent[i] = (flags[i] & FLAG_DIVERGENT) ? 1 : 0;
-    if (out->integration_steps) out->integration_steps[i] = steps[i];
-    evals += steps[i];
-  }
+  out->sampling_launches = (int32_t)done.size();
   if (out->step_size) UT_HIP(hipMemcpy(out->step_size, ws->st.step, (size_t)C * 8, hipMemcpyDeviceToHost));
   if (out->inverse_mass) UT_HIP(hipMemcpy(out->inverse_mass, ws->d_im, (size_t)P * 8, hipMemcpyDeviceToHost));
   out->warmup_seconds = t_warm - t_start;
   out->sampling_seconds = t_end - t_warm;
-  out->sampling_gradient_evals = evals;
+  out->sampling_gradient_evals = (int64_t)evals;
   return UT_OK;
 }
 

@@ -80,9 +80,11 @@ struct SamplerParams {
   double* o_acc = nullptr;
   double* o_energy = nullptr;
   double* o_err = nullptr;
-  uint8_t* o_flags = nullptr;
+  uint8_t* o_accepted = nullptr;
+  uint8_t* o_divergent = nullptr;
   int* o_depth = nullptr;
   int* o_steps = nullptr;
+  unsigned long long* eval_counter = nullptr;  // leapfrog steps while sampling
 };
 
 // Launchers (all asynchronous on `stream`).  They return hipSuccess or the launch

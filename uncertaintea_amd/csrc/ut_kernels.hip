@@ -26,8 +26,9 @@ static FunnelModel<K> make_funnel(const DevModelDesc& d) {
   m.dim = d.dim; m.v_scale = d.v_scale; m.total_const = d.total_const;
   return m;
 }
-static HierModel make_hier(const DevModelDesc& d) {
-  HierModel m;
+template <int NMAX>
+static HierModel<NMAX> make_hier(const DevModelDesc& d) {
+  HierModel<NMAX> m;
   m.y_f = d.y_f; m.y_d = d.y_d; m.nobs = d.nobs; m.lane_f = d.lane_f; m.lane_d = d.lane_d;
   m.J = d.J; m.max_n = d.max_n; m.mu_loc = d.mu_loc; m.mu_scale = d.mu_scale;
   m.tau_scale = d.tau_scale; m.total_const = d.total_const;
@@ -38,7 +39,12 @@ template <class F>
 static hipError_t dispatch(const DevModelDesc& d, F&& f) {
   switch (d.kind) {
     case MODEL_HIER:
-      return f(make_hier(d));
+      // observations per group held in registers when they fit
+      if (d.max_n <= 8) return f(make_hier<8>(d));
+      if (d.max_n <= 16) return f(make_hier<16>(d));
+      if (d.max_n <= 32) return f(make_hier<32>(d));
+      if (d.max_n <= 64) return f(make_hier<64>(d));
+      return f(make_hier<0>(d));
     case MODEL_LOGISTIC:
       if (d.K == 1) return f(make_logistic<1>(d));
       if (d.K == 2) return f(make_logistic<2>(d));
@@ -53,6 +59,16 @@ static hipError_t dispatch(const DevModelDesc& d, F&& f) {
       return hipErrorInvalidValue;
   }
 }
+
+// A model bound to its per-lane data (prepared once per launch).
+template <class M, class T>
+struct Bound {
+  static constexpr int K = M::K;
+  const M& m;
+  const typename M::template Lane<T>& ld;
+  __device__ int api(int lane, int slot) const { return m.api(lane, slot); }
+  __device__ T eval(const T (&q)[K], T (&g)[K], bool& ok) const { return m.eval(ld, q, g, ok); }
+};
 
 // ------------------------------------------------------------------ wave helpers
 template <int K>
@@ -438,10 +454,13 @@ __device__ void hmc_transition(const M& m, const float (&im)[M::K], float h, uin
 
 // ------------------------------------------------------------------ kernels
 template <class M, class T>
-__global__ __launch_bounds__(64) void k_density(M m, const T* __restrict__ params,
+__global__ __launch_bounds__(64) void k_density(M model, const T* __restrict__ params,
                                                 T* __restrict__ values, T* __restrict__ grads,
                                                 int P) {
   constexpr int K = M::K;
+  typename M::template Lane<T> ld;
+  model.prepare(ld);
+  const Bound<M, T> m{model, ld};
   const int lane = threadIdx.x;
   const long long c = blockIdx.x;
   T q[K], g[K];
@@ -489,10 +508,13 @@ __device__ inline void store_vec(const M& m, float* dst, const float (&v)[M::K])
 // chain per workgroup: transition, accept, and -- during warmup -- the chain's
 // dual-averaging update and its warmup-history record for the pooled mass.
 template <class M>
-__global__ __launch_bounds__(64) void k_nuts(M m, ChainState st, SamplerParams sp, int it_begin,
+__global__ __launch_bounds__(64) void k_nuts(M model, ChainState st, SamplerParams sp, int it_begin,
                                              int it_end) {
   extern __shared__ float ck[];
   constexpr int K = M::K;
+  typename M::template Lane<float> ld;
+  model.prepare(ld);
+  const Bound<M, float> m{model, ld};
   const int lane = threadIdx.x;
   const long long c = blockIdx.x;
   const int P = sp.P;
@@ -504,6 +526,7 @@ __global__ __launch_bounds__(64) void k_nuts(M m, ChainState st, SamplerParams s
 #pragma unroll
   for (int i = 0; i < 5; ++i) dual[i] = st.dual[c * 5 + i];
   const uint32_t chain = (uint32_t)c;
+  unsigned long long sample_steps = 0;
   for (int it = it_begin; it <= it_end; ++it) {
     const uint32_t itc = sp.run_seed + (uint32_t)it;
     const float h = (float)step;
@@ -534,12 +557,15 @@ __global__ __launch_bounds__(64) void k_nuts(M m, ChainState st, SamplerParams s
         sp.o_acc[row] = o.accept_prob;
         sp.o_energy[row] = o.energy;
         sp.o_err[row] = o.error;
-        sp.o_flags[row] = flags;
+        sp.o_accepted[row] = o.moved ? 1 : 0;
+        sp.o_divergent[row] = o.divergent ? 1 : 0;
         sp.o_depth[row] = o.depth;
         sp.o_steps[row] = o.steps;
       }
+      sample_steps += o.steps;
     }
   }
+  if (lane == 0 && sample_steps > 0) atomicAdd(sp.eval_counter, sample_steps);
   store_vec(m, st.pos + c * P, q);
   store_vec(m, st.grad + c * P, g);
   if (lane == 0) {
@@ -551,9 +577,12 @@ __global__ __launch_bounds__(64) void k_nuts(M m, ChainState st, SamplerParams s
 }
 
 template <class M>
-__global__ __launch_bounds__(64) void k_hmc(M m, ChainState st, SamplerParams sp, int it_begin,
+__global__ __launch_bounds__(64) void k_hmc(M model, ChainState st, SamplerParams sp, int it_begin,
                                             int it_end) {
   constexpr int K = M::K;
+  typename M::template Lane<float> ld;
+  model.prepare(ld);
+  const Bound<M, float> m{model, ld};
   const int lane = threadIdx.x;
   const long long c = blockIdx.x;
   const int P = sp.P;
@@ -565,6 +594,7 @@ __global__ __launch_bounds__(64) void k_hmc(M m, ChainState st, SamplerParams sp
 #pragma unroll
   for (int i = 0; i < 5; ++i) dual[i] = st.dual[c * 5 + i];
   const uint32_t chain = (uint32_t)c;
+  unsigned long long sample_steps = 0;
   for (int it = it_begin; it <= it_end; ++it) {
     const uint32_t itc = sp.run_seed + (uint32_t)it;
     const float h = (float)step;
@@ -592,12 +622,15 @@ __global__ __launch_bounds__(64) void k_hmc(M m, ChainState st, SamplerParams sp
         sp.o_acc[row] = o.accept_prob;
         sp.o_energy[row] = o.energy;
         sp.o_err[row] = o.error;
-        sp.o_flags[row] = flags;
+        sp.o_accepted[row] = o.moved ? 1 : 0;
+        sp.o_divergent[row] = o.divergent ? 1 : 0;
         sp.o_depth[row] = 0;
         sp.o_steps[row] = sp.num_leapfrog;
       }
+      sample_steps += (unsigned long long)sp.num_leapfrog;
     }
   }
+  if (lane == 0 && sample_steps > 0) atomicAdd(sp.eval_counter, sample_steps);
   store_vec(m, st.pos + c * P, q);
   store_vec(m, st.grad + c * P, g);
   if (lane == 0) {
@@ -615,8 +648,11 @@ __global__ __launch_bounds__(64) void k_hmc(M m, ChainState st, SamplerParams sp
 // Afterwards the chain's dual-averaging state restarts from its step
 // (adaptation.jl:1-13).
 template <class M>
-__global__ __launch_bounds__(64) void k_step_search(M m, ChainState st, int P, uint32_t itc,
+__global__ __launch_bounds__(64) void k_step_search(M model, ChainState st, int P, uint32_t itc,
                                                     int do_search, double target) {
+  typename M::template Lane<float> ld;
+  model.prepare(ld);
+  const Bound<M, float> m{model, ld};
   constexpr int K = M::K;
   const int lane = threadIdx.x;
   const long long c = blockIdx.x;

@@ -5,27 +5,62 @@
 // api(lane, slot) maps it back to the reference's parameter order (the row of the
 // P x C parameter matrix), or -1 for a padding slot (value 0, inverse mass 0).
 //
-// Every reduction over lanes is the xor butterfly of wave_sum(): lane l adds lane
-// l^m for m = 1, 2, 4, 8, 16, 32, so every lane ends with the same bit pattern.
-// Per-lane partial sums run over slots in increasing order starting from zero.
-// A CPU restatement that reproduces this order reproduces the kernels bit for bit.
+// Every reduction over lanes is wave_sum(): four DPP butterfly steps inside each
+// 16-lane row (lane l adds lane l^1, l^2, then the half-row mirror l^7 and the row
+// mirror l^15), after which every lane of row r holds the row total R_r; the
+// wavefront total is (R_0 + R_1) + (R_2 + R_3), read with v_readlane.  Every lane
+// ends with the same bit pattern.  DPP moves and readlane stay in the VALU
+// pipeline (no LDS round trip per step).  Per-lane partial sums run over slots in
+// increasing order starting from zero.  A CPU restatement that reproduces this
+// order reproduces the kernels bit for bit.
 #pragma once
 #include "ut_math.h"
 
 namespace ut {
 
-__device__ inline float wave_sum(float v) {
-#pragma unroll
-  for (int m = 1; m < WAVE; m <<= 1) v = v + __shfl_xor(v, m);
-  return v;
+// DPP controls (GCN/CDNA encoding): quad_perm [1,0,3,2] and [2,3,0,1],
+// row_half_mirror, row_mirror.
+constexpr int DPP_XOR1 = 0xB1;
+constexpr int DPP_XOR2 = 0x4E;
+constexpr int DPP_HALF_MIRROR = 0x141;
+constexpr int DPP_MIRROR = 0x140;
+
+template <int CTRL>
+__device__ inline int dpp_i(int v) {
+  return __builtin_amdgcn_mov_dpp(v, CTRL, 0xF, 0xF, true);
 }
-__device__ inline double wave_sum(double v) {
-#pragma unroll
-  for (int m = 1; m < WAVE; m <<= 1) v = v + __shfl_xor(v, m);
-  return v;
+template <int CTRL>
+__device__ inline float dpp(float v) {
+  return __builtin_bit_cast(float, dpp_i<CTRL>(__builtin_bit_cast(int, v)));
+}
+template <int CTRL>
+__device__ inline double dpp(double v) {
+  const uint64_t b = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = (uint32_t)dpp_i<CTRL>((int)(uint32_t)b);
+  const uint32_t hi = (uint32_t)dpp_i<CTRL>((int)(uint32_t)(b >> 32));
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+__device__ inline float readlane(float v, int lane) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), lane));
+}
+__device__ inline double readlane(double v, int lane) {
+  const uint64_t b = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, lane);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), lane);
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+template <class T>
+__device__ inline T wave_sum(T v) {
+  v = v + dpp<DPP_XOR1>(v);
+  v = v + dpp<DPP_XOR2>(v);
+  v = v + dpp<DPP_HALF_MIRROR>(v);
+  v = v + dpp<DPP_MIRROR>(v);
+  const T r0 = readlane(v, 0), r1 = readlane(v, 16), r2 = readlane(v, 32), r3 = readlane(v, 48);
+  return (r0 + r1) + (r2 + r3);
 }
 template <class T>
-__device__ inline T wave_bcast(T v, int src) { return __shfl(v, src); }
+__device__ inline T wave_bcast(T v, int src) { return readlane(v, src); }
 __device__ inline bool wave_all(bool b) { return __all(b ? 1 : 0) != 0; }
 __device__ inline bool wave_any(bool b) { return __any(b ? 1 : 0) != 0; }
 
@@ -63,7 +98,10 @@ template <> struct Pick<double> {
 //   with c_j = -n_j log sigma_j - n_j log(2 pi)/2 and hinv_j = 1 / (2 sigma_j^2),
 //   d/dtheta_j = inv_j * sum_i (y_ij - theta_j), inv_j = 1 / sigma_j^2.
 // Observations are stored y[i * 64 + j] (zero padded to max_n rows) so that the lanes
-// of one row read consecutive words.
+// of one row read consecutive words.  With NMAX > 0 every lane keeps its group's
+// observations in registers for the whole launch (prepare()); NMAX == 0 streams
+// them from memory on every evaluation.  The arithmetic order is the same.
+template <int NMAX>
 struct HierModel {
   static constexpr int K = 2;
   const float* y_f;
@@ -75,33 +113,62 @@ struct HierModel {
   int max_n;
   double mu_loc, mu_scale, tau_scale, total_const;
 
+  template <class T>
+  struct Lane {
+    T y[NMAX > 0 ? NMAX : 1];
+    T cj, hinv, inv;
+    int n;
+  };
+
   __device__ int api(int lane, int slot) const {
     if (slot == 0) return lane < J ? 2 + lane : -1;
     return lane == 0 ? 0 : (lane == 1 ? 1 : -1);
   }
 
   template <class T>
-  __device__ T eval(const T (&q)[K], T (&g)[K], bool& ok) const {
+  __device__ void prepare(Lane<T>& ld) const {
+    const int lane = threadIdx.x & (WAVE - 1);
+    ld.n = nobs[lane];
+    if (NMAX > 0) {
+#pragma unroll
+      for (int i = 0; i < (NMAX > 0 ? NMAX : 1); ++i)
+        ld.y[i] = i < ld.n ? Pick<T>::get(y_f, y_d, i * WAVE + lane) : T(0);
+    }
+    ld.cj = Pick<T>::get(lane_f, lane_d, lane);
+    ld.hinv = Pick<T>::get(lane_f, lane_d, WAVE + lane);
+    ld.inv = Pick<T>::get(lane_f, lane_d, 2 * WAVE + lane);
+  }
+
+  template <class T>
+  __device__ T eval(const Lane<T>& ld, const T (&q)[K], T (&g)[K], bool& ok) const {
     const int lane = threadIdx.x & (WAVE - 1);
     const T mu = wave_bcast(q[1], 0);
     const T ltau = wave_bcast(q[1], 1);
     const T tau = t_exp<T>(ltau);
     const T qr = q[0];
     const T theta = mu + tau * qr;
-    const int n = nobs[lane];
+    const int n = ld.n;
     T sd = T(0), ss = T(0);
-    for (int i = 0; i < max_n; ++i) {
-      if (i < n) {
-        T d = Pick<T>::get(y_f, y_d, i * WAVE + lane) - theta;
-        sd = sd + d;
-        ss = ss + d * d;
+    if (NMAX > 0) {
+#pragma unroll
+      for (int i = 0; i < (NMAX > 0 ? NMAX : 1); ++i) {
+        if (i < n) {
+          const T d = ld.y[i] - theta;
+          sd = sd + d;
+          ss = ss + d * d;
+        }
+      }
+    } else {
+      for (int i = 0; i < max_n; ++i) {
+        if (i < n) {
+          const T d = Pick<T>::get(y_f, y_d, i * WAVE + lane) - theta;
+          sd = sd + d;
+          ss = ss + d * d;
+        }
       }
     }
-    const T cj = Pick<T>::get(lane_f, lane_d, lane);
-    const T hinv = Pick<T>::get(lane_f, lane_d, WAVE + lane);
-    const T inv = Pick<T>::get(lane_f, lane_d, 2 * WAVE + lane);
-    T lp = cj - ss * hinv;
-    const T gth = sd * inv;
+    T lp = ld.cj - ss * ld.hinv;
+    const T gth = sd * ld.inv;
     lp = lp - T(0.5) * qr * qr;
     g[0] = tau * gth - qr;
     const T s1 = wave_sum(gth);
@@ -141,13 +208,18 @@ struct LogisticModel {
   int D;
   double loc, scale, total_const;
 
+  template <class T>
+  struct Lane {};
+  template <class T>
+  __device__ void prepare(Lane<T>&) const {}
+
   __device__ int api(int lane, int slot) const {
     int i = slot * WAVE + lane;
     return i < D ? i : -1;
   }
 
   template <class T>
-  __device__ T eval(const T (&q)[K], T (&g)[K], bool& ok) const {
+  __device__ T eval(const Lane<T>&, const T (&q)[K], T (&g)[K], bool& ok) const {
     const int lane = threadIdx.x & (WAVE - 1);
     T beta[DMAX];
     T gp[DMAX];
@@ -216,6 +288,11 @@ struct FunnelModel {
   int dim;
   double v_scale, total_const;
 
+  template <class T>
+  struct Lane {};
+  template <class T>
+  __device__ void prepare(Lane<T>&) const {}
+
   __device__ int api(int lane, int slot) const {
     int i = slot * WAVE + lane;
     if (i < dim) return 1 + i;
@@ -223,7 +300,7 @@ struct FunnelModel {
   }
 
   template <class T>
-  __device__ T eval(const T (&q)[K], T (&g)[K], bool& ok) const {
+  __device__ T eval(const Lane<T>&, const T (&q)[K], T (&g)[K], bool& ok) const {
     const int lane = threadIdx.x & (WAVE - 1);
     const T v = wave_bcast(q[dim / WAVE], dim % WAVE);
     const T ev = t_exp<T>(-v);

@@ -124,10 +124,11 @@ def _run(kind: str, model: TeaModel, num_chains: int, opts: L.SamplerOpts, initi
             raise ValueError(f"initial_params must be P x C = {P} x {C}")
         init = np.ascontiguousarray(init.T)  # chain-major
         opts.initial_params = L.dptr(init, ct.c_double)
-    samples = np.empty((C, S, P), dtype=np.float32)
-    lj = np.empty((C, S)); acc = np.empty((C, S)); en = np.empty((C, S)); err = np.empty((C, S))
-    accepted = np.empty((C, S), dtype=np.uint8); div = np.empty((C, S), dtype=np.uint8)
-    depth = np.empty((C, S), dtype=np.int32); steps = np.empty((C, S), dtype=np.int32)
+    pe = L.pinned_empty  # page-locked: the sampler copies results out while it computes
+    samples = pe((C, S, P), np.float32)
+    lj, acc, en, err = pe((C, S), np.float64), pe((C, S), np.float64), pe((C, S), np.float64), pe((C, S), np.float64)
+    accepted, div = pe((C, S), np.uint8), pe((C, S), np.uint8)
+    depth, steps = pe((C, S), np.int32), pe((C, S), np.int32)
     step = np.empty(C); im = np.empty(P)
     out = L.ChainOut()
     out.samples = samples.ctypes.data_as(ct.c_void_p)
@@ -143,7 +144,7 @@ def _run(kind: str, model: TeaModel, num_chains: int, opts: L.SamplerOpts, initi
     out.inverse_mass = L.dptr(im, ct.c_double)
     fn = L.lib().ut_nuts_run if kind == "nuts" else L.lib().ut_hmc_run
     L.check(fn(ws.handle, ct.byref(opts), ct.byref(out)))
-    return HMCChains(kind, model, samples, lj, acc, en, err, accepted.astype(bool), div.astype(bool), depth,
+    return HMCChains(kind, model, samples, lj, acc, en, err, accepted.view(bool), div.view(bool), depth,
                      steps, step, im, max_tree_depth, num_leapfrog_steps, target_accept, out.warmup_seconds,
                      out.sampling_seconds, int(out.sampling_gradient_evals), out.sampling_kernel_seconds,
                      int(out.sampling_launches))
