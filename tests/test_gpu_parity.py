@@ -164,10 +164,12 @@ def test_dimension_errors():
 
 def test_hier_cfg2_posterior_statistics():
     """Full 1024-chain config-2 run against the binary64 oracle (different seed):
-    posterior means within 4 combined MCSE, R-hat < 1.01, no divergences."""
+    posterior means within 4 combined MCSE, R-hat < 1.01, no divergences.  The
+    non-centred posterior couples tau with every theta_raw, so mu and tau mix slowly
+    (bulk ESS ~ 0.05 per draw); 2000 draws per chain bring R-hat under 1.01."""
     say("hier cfg2 statistics")
     model = _models()["hier_cfg2"]
-    dev = ut.batched_nuts(model, num_chains=1024, num_samples=300, num_warmup=300, seed=2026)
+    dev = ut.batched_nuts(model, num_chains=1024, num_samples=2000, num_warmup=1000, seed=2026)
     d = dev.constrained_samples
     rh = ut.diagnostics.rhat(d)
     assert np.nanmax(rh) < 1.01
