@@ -98,8 +98,12 @@ def main() -> int:
     ksec = res.sampling_kernel_seconds
     launches = max(res.sampling_launches, 1)
     achieved = flops / ksec / 1e12 if ksec > 0 else 0.0
-    roofline = {"bound": "mfma", "compute": "fp32 vector ALU (no HBM traffic in the tree)", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+    traffic = None
+    pmc = ROOT / "profiles" / "k_nuts_hier_cfg2_pmc.json"
+    if pmc.exists():  # HBM bytes per launch from the committed FETCH_SIZE/WRITE_SIZE runs
+        traffic = json.loads(pmc.read_text()).get("traffic_bytes_per_launch")
+    roofline = {"bound": "mfma", "compute": "fp32 vector ALU (no HBM traffic in the tree)", "achieved": achieved,
+                "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
                 "kernel": "k_nuts<HierModel>", "launches": res.sampling_launches,
                 "avg_launch_ms": 1e3 * ksec / launches, "flops_per_launch": flops / launches}
 
