@@ -196,6 +196,9 @@ template <class T> T tlogistic(T x);
 template <> float tlogistic<float>(float x) { return logistic_f(x); }
 template <> double tlogistic<double>(double x) { return logistic_d(x); }
 template <class T> bool tfin(T x) { return std::isfinite(x); }
+template <class T> T tfma(T a, T b, T c);
+template <> float tfma<float>(float a, float b, float c) { return fma_f(a, b, c); }
+template <> double tfma<double>(double a, double b, double c) { return fma_d(a, b, c); }
 
 template <class T>
 T twin_eval(const Model& m, const T* q, T* g, bool& ok) {
@@ -214,22 +217,26 @@ T twin_eval(const Model& m, const T* q, T* g, bool& ok) {
         if (i < n) {
           const T d = T(m.ty_f[i * WAVE + l]) - theta;
           sd = sd + d;
-          ss = ss + d * d;
+          ss = tfma<T>(d, d, ss);
         }
       }
       const T cj = T(m.tlane[l]), hinv = T(m.tlane[WAVE + l]), inv = T(m.tlane[2 * WAVE + l]);
       T lp = cj - ss * hinv;
       gth[l] = sd * inv;
+      if (l >= m.J) {
+        lp = T(0);
+        gth[l] = T(0);
+      }
       lp = lp - T(0.5) * qr * qr;
       g[l] = tau * gth[l] - qr;
       gq[l] = gth[l] * qr;
       lpl[l] = lp;
     }
     const T s1 = bfly(gth), s2 = bfly(gq), sl = bfly(lpl);
-    const T zmu = (mu - T(m.mu_loc)) / T(m.mu_scale);
-    const T lp_mu = -(zmu * zmu) / T(2);
-    const T g_mu = s1 - zmu / T(m.mu_scale);
-    const T zt = tau / T(m.tau_scale);
+    const T zmu = (mu - T(m.mu_loc)) * T(1.0 / m.mu_scale);
+    const T lp_mu = -(zmu * zmu) * T(0.5);
+    const T g_mu = s1 - zmu * T(1.0 / m.mu_scale);
+    const T zt = tau * T(1.0 / m.tau_scale);
     const T z2 = zt * zt;
     const T lp_tau = ltau - tlog1p<T>(z2);
     const T g_ltau = tau * s2 - (T(2) * z2) / (T(1) + z2) + T(1);

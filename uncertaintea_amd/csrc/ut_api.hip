@@ -221,6 +221,11 @@ int ut_plan_hier_normal_nc(const float* y, const int32_t* group_offsets, const d
   m.mu_loc = mu_loc;
   m.mu_scale = mu_scale;
   m.tau_scale = tau_scale;
+  m.inv_mu_scale = 1.0 / mu_scale;
+  m.inv_tau_scale = 1.0 / tau_scale;
+  m.dense = 1;
+  for (int j = 0; j < num_groups; ++j)
+    if (group_offsets[j + 1] - group_offsets[j] != max_n) m.dense = 0;
   m.total_const = (double)num_groups * (-HALF_LOG_2PI) + (-log_d(mu_scale) - HALF_LOG_2PI) +
                   (LOG_2 - LOG_PI - log_d(tau_scale));
   int rc;

@@ -32,6 +32,8 @@ static HierModel<NMAX> make_hier(const DevModelDesc& d) {
   m.y_f = d.y_f; m.y_d = d.y_d; m.nobs = d.nobs; m.lane_f = d.lane_f; m.lane_d = d.lane_d;
   m.J = d.J; m.max_n = d.max_n; m.mu_loc = d.mu_loc; m.mu_scale = d.mu_scale;
   m.tau_scale = d.tau_scale; m.total_const = d.total_const;
+  m.inv_mu_scale = d.inv_mu_scale; m.inv_tau_scale = d.inv_tau_scale;
+  m.dense = (d.dense && d.max_n == NMAX) ? 1 : 0;
   return m;
 }
 
@@ -196,6 +198,14 @@ __device__ void nuts_transition(const M& m, const float (&im)[M::K], float h, ui
     }
   }
   const double H0 = kinetic(p, im) - (double)lj;
+  // Lane-parallel draws for this iteration: lane r holds the direction uniform and
+  // the log merge uniform of doubling round r; leaf uniforms come in blocks of 64
+  // (lane l: leaf 64 b + l).  Each value is the same function of its Philox
+  // coordinate as a per-use draw; only the evaluation is batched across lanes.
+  const float dir_u = rand_uniform_f(chain, itc, STREAM_DIRECTION, (uint32_t)lane);
+  const double merge_lu = log_d(rand_uniform_d(chain, itc, STREAM_MERGE, (uint32_t)lane));
+  double leaf_lu = 0.0;
+  int leaf_block = -1;
   float Lq[K], Lp[K], Lg[K], Rq[K], Rp[K], Rg[K], sq[K], sg[K], pq[K], pg[K];
   copyv(q0, q);
   copyv(Lq, q); copyv(Lp, p); copyv(Lg, g);
@@ -217,7 +227,7 @@ __device__ void nuts_transition(const M& m, const float (&im)[M::K], float h, ui
 
   // first single-leaf trajectory
   {
-    const float u = rand_uniform_f(chain, itc, STREAM_DIRECTION, 0u);
+    const float u = readlane(dir_u, 0);
     const bool forward = !(u < 0.5f);
     const float ss = forward ? h : -h;
     const float half = ss / 2.0f;
@@ -241,13 +251,17 @@ __device__ void nuts_transition(const M& m, const float (&im)[M::K], float h, ui
       if (!isfinite_d(delta) || delta > div_thr) {
         divergent = true;
       } else {
-        const double e = exp_d(fmin(0.0, -delta));
+        const double cand = -pe;
+        double hi, arg;
+        lae_split(cont_w, cand, hi, arg);
+        const double ex = exp_d(lane == 0 ? fmin(0.0, -delta) : arg);
+        const double e = readlane(ex, 0);
         accept_sum += e < 1.0 ? e : 1.0;
         accept_count += 1;
-        const double cand = -pe;
-        const double comb = logaddexp_d(cont_w, cand);
-        const double ul = rand_uniform_d(chain, itc, STREAM_LEAF, leaf_counter);
-        if (!isfinite_d(cont_w) || log_d(ul) < cand - comb) {
+        const double comb = lae_join(hi, readlane(ex, 1));
+        leaf_lu = log_d(rand_uniform_d(chain, itc, STREAM_LEAF, (uint32_t)lane));
+        leaf_block = 0;
+        if (!isfinite_d(cont_w) || readlane(leaf_lu, 0) < cand - comb) {
           copyv(pq, q); copyv(pg, g); ljP = lj1;
           prop_energy = pe;
           prop_error = delta;
@@ -262,7 +276,7 @@ __device__ void nuts_transition(const M& m, const float (&im)[M::K], float h, ui
   // doubling loop (reference _continue_nuts_proposal!)
   while (depth < max_depth && !divergent && !turning) {
     const uint32_t round = (uint32_t)depth;
-    const float ud = rand_uniform_f(chain, itc, STREAM_DIRECTION, round);
+    const float ud = readlane(dir_u, (int)round);
     const bool forward = !(ud < 0.5f);
     const float ss = forward ? h : -h;
     const float half = ss / 2.0f;
@@ -294,13 +308,20 @@ __device__ void nuts_transition(const M& m, const float (&im)[M::K], float h, ui
       const double pe = kinetic(p, im) - (double)ljc;
       const double delta = pe - H0;
       if (!isfinite_d(delta) || delta > div_thr) { sub_div = true; break; }
-      const double e = exp_d(fmin(0.0, -delta));
+      const double cand = -pe;
+      double hi, arg;
+      lae_split(sub_w, cand, hi, arg);
+      const double ex = exp_d(lane == 0 ? fmin(0.0, -delta) : arg);
+      const double e = readlane(ex, 0);
       sub_asum += e < 1.0 ? e : 1.0;
       sub_acount += 1;
-      const double cand = -pe;
-      const double comb = logaddexp_d(sub_w, cand);
-      const double ul = rand_uniform_d(chain, itc, STREAM_LEAF, leaf_counter);
-      if (!isfinite_d(sub_w) || log_d(ul) < cand - comb) {
+      const double comb = lae_join(hi, readlane(ex, 1));
+      const int blk = (int)(leaf_counter >> 6);
+      if (blk != leaf_block) {
+        leaf_lu = log_d(rand_uniform_d(chain, itc, STREAM_LEAF, ((uint32_t)blk << 6) + (uint32_t)lane));
+        leaf_block = blk;
+      }
+      if (!isfinite_d(sub_w) || readlane(leaf_lu, (int)(leaf_counter & 63u)) < cand - comb) {
         copyv(sq, q); copyv(sg, g); ljS = ljc;
         sub_energy = pe;
         sub_error = delta;
@@ -336,8 +357,7 @@ __device__ void nuts_transition(const M& m, const float (&im)[M::K], float h, ui
     double comb_cw = cont_w;
     if (isfinite_d(sub_w)) {
       comb_cw = logaddexp_d(cont_w, sub_w);
-      const double um = rand_uniform_d(chain, itc, STREAM_MERGE, round);
-      if (log_d(um) < sub_w - cont_w) {
+      if (readlane(merge_lu, (int)round) < sub_w - cont_w) {
         copyv(pq, sq); copyv(pg, sg); ljP = ljS;
         prop_energy = sub_energy;
         prop_error = sub_error;

@@ -321,15 +321,23 @@ UT_HD double log1p_d(double x) {
 // x^y for x > 0 (dual-averaging schedule only).
 UT_HD double pow_d(double x, double y) { return exp_d(y * log_d(x)); }
 
-// log(e^x + e^y) with -inf handling (reference state_init.jl:284-293,
-// persistent_nuts.jl:83-92).
-UT_HD double logaddexp_d(double x, double y) {
+// log(e^x + e^y) = hi + log1p(e^(lo - hi)) with -inf handling (reference
+// state_init.jl:284-293, persistent_nuts.jl:83-92), in two halves so that a
+// caller can evaluate the exponential together with another one: lae_split gives
+// hi and the exponent (-inf when either input is -inf, so the sum is the other
+// input), lae_join adds log1p of the exponential.
+UT_HD void lae_split(double x, double y, double& hi, double& arg) {
   const double ninf = -inf_d();
-  if (x == ninf) return y;
-  if (y == ninf) return x;
-  double hi = x > y ? x : y;
-  double lo = x > y ? y : x;
-  return hi + log1p_d(exp_d(lo - hi));
+  if (x == ninf) { hi = y; arg = ninf; return; }
+  if (y == ninf) { hi = x; arg = ninf; return; }
+  hi = x > y ? x : y;
+  arg = (x > y ? y : x) - hi;
+}
+UT_HD double lae_join(double hi, double e) { return hi + log1p_d(e); }
+UT_HD double logaddexp_d(double x, double y) {
+  double hi, arg;
+  lae_split(x, y, hi, arg);
+  return lae_join(hi, exp_d(arg));
 }
 
 // log(1 + e^eta) without overflow (reference math.jl:202-208).

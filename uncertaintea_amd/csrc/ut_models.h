@@ -76,6 +76,9 @@ template <> __device__ inline double t_log1p_exp<double>(double x) { return log1
 template <class T> __device__ inline T t_logistic(T x);
 template <> __device__ inline float t_logistic<float>(float x) { return logistic_f(x); }
 template <> __device__ inline double t_logistic<double>(double x) { return logistic_d(x); }
+template <class T> __device__ inline T t_fma(T a, T b, T c);
+template <> __device__ inline float t_fma<float>(float a, float b, float c) { return fma_f(a, b, c); }
+template <> __device__ inline double t_fma<double>(double a, double b, double c) { return fma_d(a, b, c); }
 template <class T> __device__ inline bool t_isfinite(T x);
 template <> __device__ inline bool t_isfinite<float>(float x) { return isfinite_f(x); }
 template <> __device__ inline bool t_isfinite<double>(double x) { return isfinite_d(x); }
@@ -111,7 +114,9 @@ struct HierModel {
   const double* lane_d;
   int J;
   int max_n;
+  int dense;  // 1 when every group has exactly max_n == NMAX observations
   double mu_loc, mu_scale, tau_scale, total_const;
+  double inv_mu_scale, inv_tau_scale;
 
   template <class T>
   struct Lane {
@@ -149,13 +154,22 @@ struct HierModel {
     const T theta = mu + tau * qr;
     const int n = ld.n;
     T sd = T(0), ss = T(0);
-    if (NMAX > 0) {
+    if (NMAX > 0 && dense) {
+      // every group has exactly NMAX observations: no per-observation mask; lanes
+      // past the last group are zeroed after the loop
+#pragma unroll
+      for (int i = 0; i < (NMAX > 0 ? NMAX : 1); ++i) {
+        const T d = ld.y[i] - theta;
+        sd = sd + d;
+        ss = t_fma<T>(d, d, ss);
+      }
+    } else if (NMAX > 0) {
 #pragma unroll
       for (int i = 0; i < (NMAX > 0 ? NMAX : 1); ++i) {
         if (i < n) {
           const T d = ld.y[i] - theta;
           sd = sd + d;
-          ss = ss + d * d;
+          ss = t_fma<T>(d, d, ss);
         }
       }
     } else {
@@ -163,23 +177,27 @@ struct HierModel {
         if (i < n) {
           const T d = Pick<T>::get(y_f, y_d, i * WAVE + lane) - theta;
           sd = sd + d;
-          ss = ss + d * d;
+          ss = t_fma<T>(d, d, ss);
         }
       }
     }
     T lp = ld.cj - ss * ld.hinv;
-    const T gth = sd * ld.inv;
+    T gth = sd * ld.inv;
+    if (lane >= J) {
+      lp = T(0);
+      gth = T(0);
+    }
     lp = lp - T(0.5) * qr * qr;
     g[0] = tau * gth - qr;
     const T s1 = wave_sum(gth);
     const T s2 = wave_sum(gth * qr);
     const T sl = wave_sum(lp);
     // mu ~ normal(mu_loc, mu_scale)
-    const T zmu = (mu - T(mu_loc)) / T(mu_scale);
-    const T lp_mu = -(zmu * zmu) / T(2);
-    const T g_mu = s1 - zmu / T(mu_scale);
+    const T zmu = (mu - T(mu_loc)) * T(inv_mu_scale);
+    const T lp_mu = -(zmu * zmu) * T(0.5);
+    const T g_mu = s1 - zmu * T(inv_mu_scale);
     // tau ~ halfcauchy(tau_scale), tau = exp(log tau), log-abs-det = log tau
-    const T zt = tau / T(tau_scale);
+    const T zt = tau * T(inv_tau_scale);
     const T z2 = zt * zt;
     const T lp_tau = ltau - t_log1p<T>(z2);
     const T g_ltau = tau * s2 - (T(2) * z2) / (T(1) + z2) + T(1);
