@@ -184,6 +184,15 @@ int ut_hmc_run(ut_workspace* ws, const ut_sampler_opts* opts, ut_chain_out* out)
 int ut_host_alloc(int64_t bytes, void** out);
 int ut_host_free(void* ptr);
 
+/* ---- matrix cores ----------------------------------------------------------------
+ * C[m x n] = A[m x k] * B[k x n] on the gfx950 MFMA units, bfloat16 inputs (A row-major,
+ * B given transposed as n x k row-major), fp32 accumulation, split-K over `splits`.
+ * Host pointers; m, n multiples of 128, k of 32.  The same kernel computes the
+ * dense GLM contractions and the flow layers; exposed for verification.  No
+ * reference counterpart.                                                          */
+int ut_gemm_bf16(const uint16_t* a, const uint16_t* bt, float* c, int32_t m, int32_t n, int32_t k,
+                 int32_t splits);
+
 /* ---- RNG -----------------------------------------------------------------------
  * Host evaluation of one Philox4x32-10 block (reference philox4x32, rng.jl:219). */
 void ut_philox4x32(const uint32_t key[2], const uint32_t ctr[4], uint32_t out[4]);

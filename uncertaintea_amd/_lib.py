@@ -32,7 +32,7 @@ EXPORTED = (
     "ut_workspace_create", "ut_workspace_destroy",
     "ut_logjoint", "ut_logjoint_gradient",
     "ut_sampler_opts_default", "ut_nuts_run", "ut_hmc_run",
-    "ut_philox4x32", "ut_philox_fill", "ut_host_alloc", "ut_host_free",
+    "ut_philox4x32", "ut_philox_fill", "ut_host_alloc", "ut_host_free", "ut_gemm_bf16",
 )
 
 
@@ -121,6 +121,7 @@ def lib() -> ct.CDLL:
         "ut_philox_fill": (ct.c_int, [u32, u32, u32, u32, i64, vp, vp, vp, vp]),
         "ut_host_alloc": (ct.c_int, [i64, pp]),
         "ut_host_free": (ct.c_int, [vp]),
+        "ut_gemm_bf16": (ct.c_int, [vp, vp, vp, i32, i32, i32, i32]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -178,6 +179,28 @@ def pinned_empty(shape, dtype) -> np.ndarray:
     if int(np.prod(shape)) == 0:
         return np.empty(shape, dtype=dtype)
     return np.asarray(_PinnedBlock(shape, dtype))
+
+
+def gemm_bf16(a_bf16: np.ndarray, bt_bf16: np.ndarray, splits: int = 1) -> np.ndarray:
+    """C = A @ Bt.T on the matrix cores; A (m x k), Bt (n x k) as uint16 bfloat16 bits."""
+    a = np.ascontiguousarray(a_bf16, dtype=np.uint16)
+    b = np.ascontiguousarray(bt_bf16, dtype=np.uint16)
+    m, k = a.shape
+    n = b.shape[0]
+    c = np.empty((m, n), dtype=np.float32)
+    check(lib().ut_gemm_bf16(ptr(a), ptr(b), ptr(c), m, n, k, splits))
+    return c
+
+
+def to_bf16(x: np.ndarray) -> np.ndarray:
+    """Round-to-nearest-even float32 -> bfloat16 bit patterns (uint16)."""
+    u = np.ascontiguousarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
+    return u.astype(np.uint16)
+
+
+def from_bf16(h: np.ndarray) -> np.ndarray:
+    return (np.asarray(h, dtype=np.uint32) << 16).view(np.float32)
 
 
 def device_available() -> bool:

@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "../../include/uncertaintea_hip.h"
+#include "ut_gemm.h"
 #include "ut_kernels.h"
 #include "ut_math.h"
 
@@ -694,6 +695,36 @@ int ut_nuts_run(ut_workspace* ws, const ut_sampler_opts* opts, ut_chain_out* out
 
 int ut_hmc_run(ut_workspace* ws, const ut_sampler_opts* opts, ut_chain_out* out) {
   return run_sampler(ws, opts, out, false);
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int ut_gemm_bf16(const uint16_t* a, const uint16_t* bt, float* c, int32_t m, int32_t n, int32_t k,
+                 int32_t splits) {
+  if (!a || !bt || !c) return fail(UT_E_ARG, "ut_gemm_bf16: null pointer");
+  if (m <= 0 || n <= 0 || k <= 0 || m % 128 || n % 128 || k % 32 || splits < 1)
+    return fail(UT_E_DIM, "ut_gemm_bf16: m, n must be multiples of 128, k of 32, splits >= 1");
+  if (!ut_device_available()) return fail(UT_E_HIP, "ut_gemm_bf16: no HIP device visible");
+  DeviceBuffers buf;
+  uint16_t *da = nullptr, *db = nullptr;
+  float* dc = nullptr;
+  const size_t mn = (size_t)m * n;
+  UT_HIP(buf.alloc(&da, (size_t)m * k));
+  UT_HIP(buf.alloc(&db, (size_t)n * k));
+  UT_HIP(buf.alloc(&dc, mn * splits));
+  UT_HIP(hipMemcpy(da, a, (size_t)m * k * 2, hipMemcpyHostToDevice));
+  UT_HIP(hipMemcpy(db, bt, (size_t)n * k * 2, hipMemcpyHostToDevice));
+  UT_HIP(launch_gemm_nt(da, k, db, k, dc, n, m, n, k, splits, (long long)mn, nullptr));
+  std::vector<float> part(mn * splits);
+  UT_HIP(hipMemcpy(part.data(), dc, part.size() * 4, hipMemcpyDeviceToHost));
+  for (size_t i = 0; i < mn; ++i) {
+    float s = 0.0f;
+    for (int z = 0; z < splits; ++z) s += part[(size_t)z * mn + i];
+    c[i] = s;
+  }
+  return UT_OK;
 }
 
 }  // extern "C"

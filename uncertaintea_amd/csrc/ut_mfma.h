@@ -1,0 +1,38 @@
+// ut_mfma.h -- BF16 matrix-core building blocks for gfx950 (v_mfma_f32_32x32x16_bf16).
+//
+// Operand layouts of one 32x32x16 MFMA (per 64-lane wavefront):
+//   A (32 x 16, M x K):  lane l holds A[l % 32][8 (l / 32) .. 8 (l / 32) + 7]
+//   B (16 x 32, K x N):  lane l holds B[8 (l / 32) .. + 7][l % 32]
+//   D (32 x 32, fp32):   lane l, register r holds D[8 (r / 4) + 4 (l / 32) + r % 4][l % 32]
+// Both operands are therefore read as 8 consecutive K elements (16 bytes), so every
+// GEMM here takes its operands "K-contiguous": A as M x K row-major and B as the
+// transposed N x K row-major matrix.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ut {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef uint16_t bf16_t;  // storage type of bfloat16 values
+
+__device__ inline f32x16 mfma32(const bf16x8& a, const bf16x8& b, const f32x16& c) {
+  return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+}
+
+// Round-to-nearest-even float -> bfloat16 (NaN kept quiet).
+__host__ __device__ inline bf16_t f2bf(float f) {
+  uint32_t u = __builtin_bit_cast(uint32_t, f);
+  if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x007fffffu)) return (bf16_t)((u >> 16) | 0x40);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (bf16_t)(u >> 16);
+}
+__host__ __device__ inline float bf2f(bf16_t h) {
+  return __builtin_bit_cast(float, (uint32_t)h << 16);
+}
+
+// Row index in a 32x32 MFMA result held by lane `lane`, register `r`.
+__device__ inline int mfma32_row(int lane, int r) { return 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3); }
+
+}  // namespace ut
