@@ -18,7 +18,9 @@
  *   samplers    <- batched_hmc / batched_nuts(tree_strategy=:persistent) device branches
  *                  (src/inference/api_batched_hmc.jl:16-75, src/device/hmc_kernels.jl:306-723;
  *                   src/inference/api_batched_nuts.jl:22-144,761-980,
- *                   src/device/persistent_nuts.jl:226-786)
+ *                   src/device/persistent_nuts.jl:226-786); batched_chees
+ *                  (src/inference/api_batched_chees.jl:277-700, device variant
+ *                   src/device/chees_kernels.jl)
  *   RNG         <- philox4x32 / device_rand_* (src/device/rng.jl:175-343)
  */
 #ifndef UNCERTAINTEA_HIP_H
@@ -91,6 +93,17 @@ int ut_plan_logistic_glm(const float* x, const float* y, int64_t n, int32_t d,
  * Parameter order: (mu, log tau, theta_raw_1..J); P = J + 2.  J <= 64.
  * Reference families: normal src/device/math.jl:88-91, halfcauchy :126-130,
  * log transform :1144-1159.                                                     */
+/* The same model for the matrix cores: x is N x D row-major bfloat16 bit patterns
+ * (the design matrix is bf16 exactly), D == 256, N a multiple of 32.  Density and
+ * gradient of all chains run as one fused two-stage MFMA contraction (eta = X B,
+ * then X^T r) with fp32 accumulation; positions and residuals enter the matrix
+ * cores as bf16 hi + lo pairs.  Workspaces of this plan need num_chains a multiple
+ * of 128 and UT_FLOAT32; it is sampled by ut_chees_run (all chains in lock step).
+ * Reference: the same GLM lowering and gradient as ut_plan_logistic_glm
+ * (lowering.jl:664-675, gradient_kernel.jl:578-657).                            */
+int ut_plan_logistic_glm_bf16(const uint16_t* x_bf16, const float* y, int64_t n, int32_t d,
+                              double prior_loc, double prior_scale, ut_plan** out);
+
 int ut_plan_hier_normal_nc(const float* y, const int32_t* group_offsets,
                            const double* sigma, int32_t num_groups, double mu_loc,
                            double mu_scale, double tau_scale, ut_plan** out);
@@ -175,6 +188,37 @@ int ut_nuts_run(ut_workspace* ws, const ut_sampler_opts* opts, ut_chain_out* out
 /* Fixed-length HMC with per-chain step and pooled diagonal mass
  * (reference _run_device_batched_hmc, src/device/hmc_kernels.jl:306-723).      */
 int ut_hmc_run(ut_workspace* ws, const ut_sampler_opts* opts, ut_chain_out* out);
+
+/* ChEES-HMC: jittered lock-step HMC with a shared step size (dual averaging to
+ * the harmonic-mean acceptance), pooled diagonal mass, and warmup adaptation of
+ * the trajectory time T by Adam ascent on the cross-chain ChEES criterion
+ * (reference batched_chees, src/inference/api_batched_chees.jl:277-700; its
+ * keyword arguments map to the fields below, defaults from the reference).
+ * base.num_leapfrog_steps sets the initial T = num_leapfrog_steps * step.
+ * trajectory_trace, if not NULL, receives T after each warmup iteration
+ * (num_warmup entries; reference `_trajectory_trace`).                          */
+typedef struct ut_chees_opts {
+  ut_sampler_opts base;              /* target_accept default 0.651 */
+  double jitter_amount;              /* 1.0, in [0, 1] */
+  int32_t adapt_trajectory_length;   /* bool, default 1 */
+  int32_t max_leapfrog_steps;        /* 1000 */
+  double trajectory_decay_rate;      /* 0.5 */
+  double trajectory_learning_rate;   /* 0.5 */
+  double trajectory_adam_beta1;      /* 0.0 */
+  double trajectory_adam_beta2;      /* 0.95 */
+  double trajectory_adam_epsilon;    /* 1e-8 */
+  /* 0 (reference): the criterion's end velocity is the proposal momentum as the
+   * reference passes it, i.e. after the final momentum flip; the Adam ascent on
+   * that estimate then shortens T towards one step.  1: the velocity in the
+   * direction of integration (the ChEES paper's d q'/dT), under which the same
+   * ascent lengthens T where longer trajectories decorrelate more.  Not a
+   * reference option; default 0 keeps parity.                                */
+  int32_t forward_velocity;
+} ut_chees_opts;
+
+void ut_chees_opts_default(ut_chees_opts* opts);
+int ut_chees_run(ut_workspace* ws, const ut_chees_opts* opts, ut_chain_out* out,
+                 double* trajectory_trace);
 
 /* ---- pinned host memory ----------------------------------------------------------
  * Page-locked host buffers for sampler outputs: the samplers copy each launch

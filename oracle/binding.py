@@ -41,6 +41,10 @@ def _load():
         "orc_nuts_sample_from": (i64, [vp, i32, i32, i32, vp, vp, vp, i32, i32, ct.c_uint64, i32]),
         "orc_dual_averaging": (None, [dbl, dbl, vp, i32, vp, vp]),
         "orc_running_variance": (dbl, [vp, vp, vp, i32, i32, i32, i32, i32, dbl, vp]),
+        "orc_chees_run": (ct.c_int, [vp, i32, vp, i32, i32, vp, vp, vp]),
+        "orc_chees_halton": (None, [i32, vp]),
+        "orc_chees_steps": (i32, [dbl, dbl, dbl]),
+        "orc_chees_trajectory": (None, [dbl, dbl, dbl, dbl, dbl, dbl, i32, vp, vp, vp, vp, i32, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -95,7 +99,7 @@ class OracleModel:
         L = _load()
         self.model = model
         kind = type(model).__name__
-        if kind == "LogisticRegression":
+        if kind in ("LogisticRegression", "LogisticRegressionBF16"):
             self._keep = (model.x, model.y)
             self.h = L.orc_model_logistic(_p(model.x), _p(model.y), model.x.shape[0], model.x.shape[1],
                                           model.prior_loc, model.prior_scale)
@@ -135,12 +139,15 @@ class OracleModel:
 
     def run(self, kind: str, mode: str, num_chains: int, *, num_samples: int, num_warmup: int = 0,
             step_size: float = 0.1, max_tree_depth: int = 10, num_leapfrog_steps: int = 10,
-            initial_params=None, init_strategy: str = "prior", target_accept: float = 0.8, seed: int = 0,
+            initial_params=None, init_strategy: str = "prior", target_accept=None, seed: int = 0,
             adapt_step_size=True, adapt_mass_matrix=True, find_reasonable_step_size=False, threads: int = 0,
-            stop_after: int = -1, divergence_threshold=1000.0):
-        """Run the oracle sampler.  kind 'nuts'|'hmc'; mode 'twin' (float32,
-        kernel-exact) or 'f64' (binary64 reference formulas).  Returns a dict."""
+            stop_after: int = -1, divergence_threshold=1000.0, chees=None):
+        """Run the oracle sampler.  kind 'nuts'|'hmc'|'chees'; mode 'twin' (float32,
+        kernel-exact) or 'f64' (binary64 reference formulas).  `chees` holds the
+        ChEES keyword arguments (reference names).  Returns a dict."""
         from uncertaintea_amd import _lib as UL  # struct layouts only
+        if target_accept is None:
+            target_accept = 0.651 if kind == "chees" else 0.8  # the reference's defaults
         o = UL.SamplerOpts()
         o.num_samples, o.num_warmup = num_samples, num_warmup
         o.max_tree_depth, o.num_leapfrog_steps = max_tree_depth, num_leapfrog_steps
@@ -175,12 +182,26 @@ class OracleModel:
         out.integration_steps = steps.ctypes.data_as(ct.POINTER(ct.c_int32))
         out.step_size = step.ctypes.data_as(ct.POINTER(ct.c_double))
         out.inverse_mass = im.ctypes.data_as(ct.POINTER(ct.c_double))
-        rc = _load().orc_sampler_run(self.h, 0 if kind == "nuts" else 1, 0 if mode == "twin" else 1,
-                                     ct.byref(o), C, threads, _p(samples), ct.byref(out), stop_after)
+        trace = None
+        if kind == "chees":
+            co = UL.CheesOpts()
+            co.base = o  # ChEES defaults restated from the reference's keyword defaults
+            kw = dict(jitter_amount=1.0, adapt_trajectory_length=True, max_leapfrog_steps=1000,
+                      trajectory_decay_rate=0.5, trajectory_learning_rate=0.5, trajectory_adam_beta1=0.0,
+                      trajectory_adam_beta2=0.95, trajectory_adam_epsilon=1e-8, forward_velocity=False)
+            kw.update(chees or {})
+            for k, v in kw.items():
+                setattr(co, k, int(v) if isinstance(getattr(co, k), int) else float(v))
+            trace = np.zeros(num_warmup)
+            rc = _load().orc_chees_run(self.h, 0 if mode == "twin" else 1, ct.byref(co), C, threads, _p(samples),
+                                       ct.byref(out), _p(trace))
+        else:
+            rc = _load().orc_sampler_run(self.h, 0 if kind == "nuts" else 1, 0 if mode == "twin" else 1,
+                                         ct.byref(o), C, threads, _p(samples), ct.byref(out), stop_after)
         if rc != 0:
             raise RuntimeError(f"oracle sampler failed with status {rc}")
         res.update(samples=samples, accepted=acc.astype(bool), divergent=div.astype(bool), tree_depth=depth,
-                   integration_steps=steps, step_size=step, inverse_mass=im)
+                   integration_steps=steps, step_size=step, inverse_mass=im, trajectory_trace=trace)
         return res
 
 
@@ -221,3 +242,23 @@ def running_variance(hist, acc, flags, window_length, use_accepted=False, reg=1e
     im = np.empty(P)
     eff = _load().orc_running_variance(_p(h), _p(a), _p(f), it, C, P, window_length, int(use_accepted), reg, _p(im))
     return im, eff
+
+
+def chees_halton(n: int):
+    out = np.zeros(n)
+    _load().orc_chees_halton(n, _p(out))
+    return out
+
+
+def chees_steps(jitter: float, T: float, eps: float) -> int:
+    return int(_load().orc_chees_steps(jitter, T, eps))
+
+
+def chees_trajectory(T0, num, den, jitter, eps, lr=0.5, beta1=0.0, beta2=0.95, adam_eps=1e-8, decay=0.5,
+                     max_steps=1000):
+    """Trajectory times after successive ChEES updates from the given reduced terms."""
+    num, den, jitter, eps = (np.ascontiguousarray(v, dtype=np.float64) for v in (num, den, jitter, eps))
+    out = np.zeros(num.size)
+    _load().orc_chees_trajectory(T0, lr, beta1, beta2, adam_eps, decay, max_steps, _p(num), _p(den), _p(jitter),
+                                 _p(eps), num.size, _p(out))
+    return out

@@ -358,6 +358,31 @@ struct TwinF32 {
     const float ld = bfly(la), rd = bfly(ra);
     return ld <= 0.0f || rd <= 0.0f;
   }
+  // lock-step kernels keep parameter a (reference order) in lane a % 64, slot a / 64
+  double ls_kinetic(const T* p, const T* im) const {
+    double lane[WAVE];
+    for (int l = 0; l < WAVE; ++l) {
+      double acc = 0.0;
+      for (int a = l; a < m->P; a += WAVE) acc = acc + ((double)p[a] * (double)p[a]) * (double)im[a];
+      lane[l] = acc;
+    }
+    return bfly(lane) * 0.5;
+  }
+  // the density kernel on reference-order vectors (P entries)
+  T ls_eval(const T* q_ref, T* g_ref) const {
+    std::vector<T> q(n(), T(0)), g(n(), T(0));
+    for (int i = 0; i < n(); ++i) {
+      const int a = api(i);
+      if (a >= 0) q[i] = q_ref[a];
+    }
+    bool ok;
+    const T v = eval(q.data(), g.data(), ok);
+    for (int i = 0; i < n(); ++i) {
+      const int a = api(i);
+      if (a >= 0) g_ref[a] = g[i];
+    }
+    return v;
+  }
 };
 
 struct RefF64 {
@@ -379,6 +404,11 @@ struct RefF64 {
       b += delta * im[i] * pr[i];
     }
     return a <= 0.0 || b <= 0.0;
+  }
+  double ls_kinetic(const T* p, const T* im) const { return kinetic(p, im); }
+  T ls_eval(const T* q, T* g) const {
+    bool ok;
+    return eval(q, g, ok);
   }
 };
 
@@ -1002,6 +1032,368 @@ int64_t sample_from(const Pol& pol, int C, int threads, double* positions, const
   return total;
 }
 
+// ------------------------------------------------------------------ ChEES
+// ChEES-HMC restated from src/inference/api_batched_chees.jl: Halton jitter
+// (:29-51), jittered step count with the integer snap (:53-70), scalar Adam ascent
+// (:86-110), trajectory update (:157-275), harmonic-mean accept (:277-295), the host
+// loop (:435-700) and the shared batched step-size search
+// (src/inference/nuts/adaptation.jl:303-375).  Trajectories run in lock step in
+// reference parameter order, as the device kernels do.
+double chees_halton(long long index) {
+  double r = 0.0, f = 0.5;
+  for (long long k = index; k > 0; k >>= 1) {
+    if (k & 1) r += f;
+    f *= 0.5;
+  }
+  return r;
+}
+
+int chees_leapfrog_steps(double jitter, double T, double eps) {
+  const double raw = jitter * T / eps;
+  const double nearest = std::nearbyint(raw);
+  double steps;
+  if (std::fabs(raw - nearest) <= 1e-9 * std::max(1.0, std::fabs(raw))) steps = nearest;
+  else steps = std::ceil(raw);
+  if (steps < 1.0) steps = 1.0;
+  return (int)std::min(steps, 1e9);
+}
+
+struct CheesState {
+  double T, log_T_ma;
+  double b1, b2, aeps, lr, m = 0.0, v = 0.0;
+  long long t = 0, step = 1;
+  double decay;
+  int max_steps;
+  double adam(double g) {
+    t += 1;
+    m = b1 * m + (1.0 - b1) * g;
+    v = b2 * v + (1.0 - b2) * g * g;
+    double p1 = 1.0, p2 = 1.0;
+    for (long long i = 0; i < t; ++i) {
+      p1 *= b1;
+      p2 *= b2;
+    }
+    return lr * (m / (1.0 - p1)) / (sqrt_d(v / (1.0 - p2)) + aeps);
+  }
+  void update(double numerator, double denominator, double jitter, double eps) {
+    const double grad = denominator > 0.0 ? jitter * T * numerator / denominator : 0.0;
+    const double lt = log_d(T);
+    const double sm = m, sv = v;
+    const long long st = t;
+    const double u = std::min(std::max(adam(grad), -0.35), 0.35);
+    double ltn = lt + u;
+    if (!std::isfinite(ltn)) {
+      ltn = lt;
+      m = sm;
+      v = sv;
+      t = st;
+    }
+    const double w = pow_d((double)step, -decay);
+    log_T_ma = (1.0 - w) * log_T_ma + w * ltn;
+    T = std::min(std::max(exp_d(log_T_ma), eps), (double)max_steps * eps);
+    step += 1;
+  }
+};
+
+template <class Pol>
+int run_chees(const Pol& pol, const ut_chees_opts* co, int C, int threads, void* samples, ut_chain_out* out,
+              double* trace) {
+  using T = typename Pol::T;
+  constexpr bool F32 = std::is_same<T, float>::value;
+  const ut_sampler_opts* o = &co->base;
+  const Model& m = *pol.m;
+  const int P = m.P, S = o->num_samples, W = o->num_warmup;
+  const uint32_t rs = run_seed_of(o->seed);
+  if (threads > 0) omp_set_num_threads(threads);
+  const size_t PC = (size_t)P * C;
+  std::vector<double> pos(PC);
+  if (o->initial_params) std::memcpy(pos.data(), o->initial_params, PC * 8);
+  else
+    for (int c = 0; c < C; ++c) draw_initial_column(m, o->init_strategy, (uint32_t)c, rs, &pos[(size_t)c * P]);
+  std::vector<T> q(PC), g(PC), lj(C);
+  for (int attempt = 0;; ++attempt) {
+    std::vector<int> bad;
+    for (int c = 0; c < C; ++c) {
+      for (int a = 0; a < P; ++a) q[(size_t)c * P + a] = F32 ? (T)(float)pos[(size_t)c * P + a] : (T)pos[(size_t)c * P + a];
+      lj[c] = pol.ls_eval(&q[(size_t)c * P], &g[(size_t)c * P]);
+      bool ok = std::isfinite(lj[c]);
+      for (int a = 0; a < P; ++a) ok = ok && std::isfinite(g[(size_t)c * P + a]);
+      if (!ok) bad.push_back(c);
+    }
+    if (bad.empty()) break;
+    if (o->initial_params || attempt >= o->init_max_retries) return UT_E_NONFINITE_INIT;
+    for (int c : bad)
+      draw_initial_column(m, o->init_strategy, (uint32_t)c, rs + (uint32_t)(attempt + 1), &pos[(size_t)c * P]);
+  }
+  std::vector<double> im_d(P, 1.0);
+  std::vector<T> im(P, T(1));
+  // trajectory buffers and per-chain records
+  std::vector<T> wq(PC), wp(PC), wg(PC), wlj(C), init(PC);
+  std::vector<double> h0(C), ap(C), energy(C), err(C);
+  std::vector<char> valid(C), div(C), acc(C), pfin(C);
+
+  auto begin = [&](uint32_t itc, uint32_t stream, T h, bool save) {
+#pragma omp parallel for schedule(static)
+    for (int c = 0; c < C; ++c) {
+      T* qc = &wq[(size_t)c * P];
+      T* pc = &wp[(size_t)c * P];
+      T* gc = &wg[(size_t)c * P];
+      bool fin = true;
+      for (int a = 0; a < P; ++a) {
+        qc[a] = q[(size_t)c * P + a];
+        gc[a] = g[(size_t)c * P + a];
+        pc[a] = draw_normal<T>((uint32_t)c, itc, stream, (uint32_t)a) / sqrt_t<T>(im[a]);
+        fin = fin && std::isfinite(gc[a]);
+        if (save) init[(size_t)c * P + a] = qc[a];
+      }
+      h0[c] = pol.ls_kinetic(pc, im.data()) - (double)lj[c];
+      valid[c] = fin;
+      if (fin) {
+        const T half = T(0.5) * h;
+        for (int a = 0; a < P; ++a) {
+          pc[a] = pc[a] + half * gc[a];
+          qc[a] = qc[a] + (h * im[a]) * pc[a];
+        }
+      }
+    }
+  };
+  auto gradient = [&]() {
+#pragma omp parallel for schedule(static)
+    for (int c = 0; c < C; ++c) wlj[c] = pol.ls_eval(&wq[(size_t)c * P], &wg[(size_t)c * P]);
+  };
+  auto inner = [&](T h) {
+#pragma omp parallel for schedule(static)
+    for (int c = 0; c < C; ++c) {
+      if (!valid[c]) continue;
+      T* qc = &wq[(size_t)c * P];
+      T* pc = &wp[(size_t)c * P];
+      const T* gc = &wg[(size_t)c * P];
+      bool fin = true;
+      for (int a = 0; a < P; ++a) fin = fin && std::isfinite(gc[a]);
+      if (!fin) {
+        valid[c] = 0;
+        continue;
+      }
+      for (int a = 0; a < P; ++a) {
+        pc[a] = pc[a] + (T(1) * h) * gc[a];
+        qc[a] = qc[a] + (h * im[a]) * pc[a];
+      }
+    }
+  };
+  // mode 0: accept / reject; mode 1: acceptance probability only
+  auto finish = [&](uint32_t itc, T h, int mode) {
+#pragma omp parallel for schedule(static)
+    for (int c = 0; c < C; ++c) {
+      T* qc = &wq[(size_t)c * P];
+      T* pc = &wp[(size_t)c * P];
+      const T* gc = &wg[(size_t)c * P];
+      bool fin = true;
+      for (int a = 0; a < P; ++a) fin = fin && std::isfinite(gc[a]);
+      const bool ok = valid[c] && fin && std::isfinite(wlj[c]);
+      if (ok) {
+        const T half = T(0.5) * h;
+        for (int a = 0; a < P; ++a) pc[a] = -(pc[a] + half * gc[a]);
+      }
+      double a_p = 0.0, e = inf_d(), en = h0[c];
+      bool d = true, moved = false;
+      if (ok) {
+        const double H1 = pol.ls_kinetic(pc, im.data()) - (double)wlj[c];
+        const double lar = h0[c] - H1;
+        e = H1 - h0[c];
+        d = !std::isfinite(e) || e > o->divergence_threshold;
+        a_p = !std::isfinite(lar) ? 0.0 : (lar >= 0.0 ? 1.0 : exp_d(lar));
+        if (mode == 0) moved = log_d(rand_uniform_d((uint32_t)c, itc, S_ACCEPT, 0u)) < std::fmin(0.0, lar);
+        en = moved ? H1 : h0[c];
+      }
+      bool pf = true;
+      for (int a = 0; a < P; ++a) pf = pf && std::isfinite(qc[a]) && std::isfinite(pc[a]);
+      ap[c] = a_p;
+      err[c] = e;
+      energy[c] = en;
+      div[c] = d;
+      acc[c] = moved;
+      pfin[c] = pf;
+      if (mode == 0 && moved) {
+        std::copy(qc, qc + P, &q[(size_t)c * P]);
+        std::copy(gc, gc + P, &g[(size_t)c * P]);
+        lj[c] = wlj[c];
+      }
+    }
+  };
+  auto mean_accept = [&]() {
+    double s = 0.0;
+    for (int c = 0; c < C; ++c) s += div[c] ? 0.0 : ap[c];
+    return s / (double)C;
+  };
+  auto harmonic = [&]() {
+    double s = 0.0;
+    long long n = 0;
+    for (int c = 0; c < C; ++c) {
+      if (div[c]) continue;
+      s += 1.0 / ap[c];
+      n += 1;
+    }
+    if (n == 0) return 0.0;
+    const double hm = (double)n / s;
+    return std::isfinite(hm) ? hm : 0.0;
+  };
+  auto cols_finite = [&](const std::vector<T>& v, int c) {
+    for (int a = 0; a < P; ++a)
+      if (!std::isfinite(v[(size_t)c * P + a])) return false;
+    return true;
+  };
+  // ChEES numerator and denominator, with the iteration-start inverse mass
+  const double vsign = co->forward_velocity ? -1.0 : 1.0;
+  auto criterion = [&](const std::vector<double>& sigma, double* num, double* den) {
+    std::vector<double> imean(P, 0.0), pmean(P, 0.0), isq(P);
+    double wsum = 0.0;
+    long long icount = 0;
+    for (int a = 0; a < P; ++a) {
+      double is = 0.0, ps = 0.0;
+      long long ic = 0;
+      double ws = 0.0;
+      for (int c = 0; c < C; ++c) {
+        if (cols_finite(init, c)) {
+          is += (double)init[(size_t)c * P + a];
+          ic += 1;
+        }
+        if (!div[c] && cols_finite(wq, c)) {
+          ws += ap[c];
+          ps += ap[c] * (double)wq[(size_t)c * P + a];
+        }
+      }
+      imean[a] = ic > 0 ? is / (double)ic : 0.0;
+      pmean[a] = ps * (1.0 / (ws + 1e-20));
+      isq[a] = 1.0 / sqrt_d(sigma[a]);
+      wsum = ws;
+      icount = ic;
+    }
+    (void)wsum;
+    (void)icount;
+    double n = 0.0, d = 0.0;
+    for (int c = 0; c < C; ++c) {
+      if (div[c] || !pfin[c]) continue;
+      double se = 0.0, ss = 0.0, ev = 0.0;
+      for (int a = 0; a < P; ++a) {
+        const double we = ((double)wq[(size_t)c * P + a] - pmean[a]) * isq[a];
+        const double ws = ((double)init[(size_t)c * P + a] - imean[a]) * isq[a];
+        const double wv = vsign * ((double)wp[(size_t)c * P + a] * sigma[a] * isq[a]);
+        se += we * we;
+        ss += ws * ws;
+        ev += we * wv;
+      }
+      n += ap[c] * (se - ss) * ev;
+      d += ap[c] + 1e-20;
+    }
+    *num = n;
+    *den = d;
+  };
+  auto search = [&](double step, uint32_t itc) {
+    double direction = 0.0;
+    for (int trial = 0; trial <= 20; ++trial) {
+      const T h = F32 ? (T)(float)step : (T)step;
+      begin(itc, S_SEARCH, h, false);
+      gradient();
+      finish(itc, h, 1);
+      const double ma = mean_accept();
+      if (trial == 0) direction = ma > 0.5 ? 1.0 : -1.0;
+      else if ((direction > 0 && ma <= 0.5) || (direction < 0 && ma >= 0.5)) break;
+      const double next = step * (direction > 0 ? 2.0 : 0.5);
+      if (next < 1e-8 || next > 1e3) break;
+      step = next;
+    }
+    return std::min(std::max(step, 1e-8), 1e3);
+  };
+  auto set_im = [&]() {
+    for (int a = 0; a < P; ++a) im[a] = F32 ? (T)(float)im_d[a] : (T)im_d[a];
+  };
+
+  double eps = o->step_size;
+  if (o->find_reasonable_step_size || (W > 0 && o->adapt_step_size)) eps = search(eps, rs);
+  Dual dual = dual_init(eps);
+  const double T0 = (double)o->num_leapfrog_steps * eps;
+  CheesState cs{T0, log_d(T0), co->trajectory_adam_beta1, co->trajectory_adam_beta2, co->trajectory_adam_epsilon,
+                co->trajectory_learning_rate};
+  cs.decay = co->trajectory_decay_rate;
+  cs.max_steps = co->max_leapfrog_steps;
+  const bool adapt_traj = W > 0 && co->adapt_trajectory_length;
+  const Schedule sch = warmup_schedule(W);
+  RunVar rv;
+  size_t win = 0;
+  rv.reset(P, sch.ends.empty() ? 20 : sch.ends[0] - sch.initial_buffer);
+  for (int it = 1; it <= W + S; ++it) {
+    const double jitter = chees_halton(it) * co->jitter_amount + (1.0 - co->jitter_amount);
+    const int L = chees_leapfrog_steps(jitter, cs.T, eps);
+    const uint32_t itc = rs + (uint32_t)it;
+    const T h = F32 ? (T)(float)eps : (T)eps;
+    const bool warm = it <= W;
+    begin(itc, S_MOM, h, warm && adapt_traj);
+    for (int l = 1; l < L; ++l) {
+      gradient();
+      inner(h);
+    }
+    gradient();
+    finish(itc, h, 0);
+    if (warm) {
+      double num = 0.0, den = 0.0;
+      if (adapt_traj) criterion(im_d, &num, &den);
+      const double hm = harmonic();
+      if (o->adapt_step_size) eps = dual_update(dual, hm, o->target_accept);
+      if (o->adapt_mass_matrix && win < sch.ends.size() && it > sch.initial_buffer) {
+        std::vector<double> w(C), col(P);
+        for (int c = 0; c < C; ++c) w[c] = mass_weight(rv, acc[c], ap[c], div[c]);
+        for (int c = 0; c < C; ++c) {
+          for (int a = 0; a < P; ++a) col[a] = (double)q[(size_t)c * P + a];
+          rv.update(col.data(), w[c]);
+        }
+        if (it == sch.ends[win]) {
+          if (rv.eff() >= (double)o->mass_matrix_min_samples) {
+            rv.inverse_mass(o->mass_matrix_regularization, im_d);
+            set_im();
+          }
+          ++win;
+          if (win < sch.ends.size()) rv.reset(P, sch.ends[win] - sch.ends[win - 1]);
+          else rv.reset(P, 20);
+          if (o->adapt_step_size && it < W) {
+            eps = search(eps, rs + (uint32_t)it);
+            dual = dual_init(eps);
+          }
+        }
+      }
+      if (adapt_traj) cs.update(num, den, jitter, eps);
+      if (trace) trace[it - 1] = cs.T;
+      if (it == W && o->adapt_step_size) eps = exp_d(dual.log_step_avg);
+    } else {
+      const int s = it - W - 1;
+      for (int c = 0; c < C; ++c) {
+        const size_t row = (size_t)c * S + s;
+        if (samples)
+          for (int a = 0; a < P; ++a) {
+            if (F32) ((float*)samples)[row * P + a] = (float)q[(size_t)c * P + a];
+            else ((double*)samples)[row * P + a] = (double)q[(size_t)c * P + a];
+          }
+        if (out) {
+          if (out->logjoint) out->logjoint[row] = (double)lj[c];
+          if (out->accept_stat) out->accept_stat[row] = ap[c];
+          if (out->energy) out->energy[row] = energy[c];
+          if (out->energy_error) out->energy_error[row] = err[c];
+          if (out->accepted) out->accepted[row] = acc[c] ? 1 : 0;
+          if (out->divergent) out->divergent[row] = div[c] ? 1 : 0;
+          if (out->tree_depth) out->tree_depth[row] = 0;
+          if (out->integration_steps) out->integration_steps[row] = L;
+        }
+      }
+    }
+  }
+  if (out) {
+    if (out->step_size)
+      for (int c = 0; c < C; ++c) out->step_size[c] = eps;
+    if (out->inverse_mass)
+      for (int a = 0; a < P; ++a) out->inverse_mass[a] = im_d[a];
+  }
+  return UT_OK;
+}
+
 void finish_twin_layout(Model& m) {
   m.K = m.kind == KIND_HIER ? 2 : slots_for(m.P);
   if (m.kind == KIND_HIER) {
@@ -1176,6 +1568,32 @@ int orc_sampler_run(void* h, int32_t kind, int32_t mode, const ut_sampler_opts* 
   }
   RefF64 pol{m};
   return run_sampler(pol, o, C, kind == 0, threads, samples, out, sa);
+}
+
+// ChEES run.  mode 0: TwinF32 (kernel-exact, samples float); mode 1: RefF64.
+int orc_chees_run(void* h, int32_t mode, const ut_chees_opts* co, int32_t C, int32_t threads, void* samples,
+                  ut_chain_out* out, double* trace) {
+  const Model* m = (const Model*)h;
+  if (mode == 0) return run_chees(TwinF32{m}, co, C, threads, samples, out, trace);
+  return run_chees(RefF64{m}, co, C, threads, samples, out, trace);
+}
+
+// ChEES scalar pieces for known-answer tests: Halton values, step counts and a
+// sequence of trajectory updates from given (numerator, denominator, jitter, eps)
+void orc_chees_halton(int32_t n, double* out) {
+  for (int i = 0; i < n; ++i) out[i] = chees_halton(i + 1);
+}
+int32_t orc_chees_steps(double jitter, double T, double eps) { return chees_leapfrog_steps(jitter, T, eps); }
+void orc_chees_trajectory(double T0, double lr, double b1, double b2, double aeps, double decay, int32_t max_steps,
+                          const double* num, const double* den, const double* jitter, const double* eps, int32_t n,
+                          double* T_out) {
+  CheesState cs{T0, log_d(T0), b1, b2, aeps, lr};
+  cs.decay = decay;
+  cs.max_steps = max_steps;
+  for (int i = 0; i < n; ++i) {
+    cs.update(num[i], den[i], jitter[i], eps[i]);
+    T_out[i] = cs.T;
+  }
 }
 
 int64_t orc_nuts_sample_from(void* h, int32_t mode, int32_t C, int32_t threads, double* positions,

@@ -27,11 +27,11 @@ UT_INIT_UNIFORM = 1
 # names exported by the C ABI (checked by the load test)
 EXPORTED = (
     "ut_last_error", "ut_abi_version", "ut_device_available",
-    "ut_plan_logistic_glm", "ut_plan_hier_normal_nc", "ut_plan_funnel",
+    "ut_plan_logistic_glm", "ut_plan_logistic_glm_bf16", "ut_plan_hier_normal_nc", "ut_plan_funnel",
     "ut_plan_destroy", "ut_plan_num_params",
     "ut_workspace_create", "ut_workspace_destroy",
     "ut_logjoint", "ut_logjoint_gradient",
-    "ut_sampler_opts_default", "ut_nuts_run", "ut_hmc_run",
+    "ut_sampler_opts_default", "ut_nuts_run", "ut_hmc_run", "ut_chees_opts_default", "ut_chees_run",
     "ut_philox4x32", "ut_philox_fill", "ut_host_alloc", "ut_host_free", "ut_gemm_bf16",
 )
 
@@ -54,6 +54,21 @@ class SamplerOpts(ct.Structure):
         ("init_max_retries", ct.c_int32),
         ("initial_params", ct.POINTER(ct.c_double)),
         ("seed", ct.c_uint64),
+    ]
+
+
+class CheesOpts(ct.Structure):
+    _fields_ = [
+        ("base", SamplerOpts),
+        ("jitter_amount", ct.c_double),
+        ("adapt_trajectory_length", ct.c_int32),
+        ("max_leapfrog_steps", ct.c_int32),
+        ("trajectory_decay_rate", ct.c_double),
+        ("trajectory_learning_rate", ct.c_double),
+        ("trajectory_adam_beta1", ct.c_double),
+        ("trajectory_adam_beta2", ct.c_double),
+        ("trajectory_adam_epsilon", ct.c_double),
+        ("forward_velocity", ct.c_int32),
     ]
 
 
@@ -106,6 +121,7 @@ def lib() -> ct.CDLL:
         "ut_abi_version": (i32, []),
         "ut_device_available": (i32, []),
         "ut_plan_logistic_glm": (ct.c_int, [vp, vp, i64, i32, dbl, dbl, pp]),
+        "ut_plan_logistic_glm_bf16": (ct.c_int, [vp, vp, i64, i32, dbl, dbl, pp]),
         "ut_plan_hier_normal_nc": (ct.c_int, [vp, vp, vp, i32, dbl, dbl, dbl, pp]),
         "ut_plan_funnel": (ct.c_int, [i32, dbl, pp]),
         "ut_plan_destroy": (ct.c_int, [vp]),
@@ -117,6 +133,8 @@ def lib() -> ct.CDLL:
         "ut_sampler_opts_default": (None, [ct.POINTER(SamplerOpts)]),
         "ut_nuts_run": (ct.c_int, [vp, ct.POINTER(SamplerOpts), ct.POINTER(ChainOut)]),
         "ut_hmc_run": (ct.c_int, [vp, ct.POINTER(SamplerOpts), ct.POINTER(ChainOut)]),
+        "ut_chees_opts_default": (None, [ct.POINTER(CheesOpts)]),
+        "ut_chees_run": (ct.c_int, [vp, ct.POINTER(CheesOpts), ct.POINTER(ChainOut), vp]),
         "ut_philox4x32": (None, [vp, vp, vp]),
         "ut_philox_fill": (ct.c_int, [u32, u32, u32, u32, i64, vp, vp, vp, vp]),
         "ut_host_alloc": (ct.c_int, [i64, pp]),
@@ -163,14 +181,15 @@ class _PinnedBlock:
         self.dtype = np.dtype(dtype)
         nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
         check(lib().ut_host_alloc(nbytes, ct.byref(self.ptr)))
+        self._free = lib().ut_host_free  # kept: module globals may be gone at exit
 
     @property
     def __array_interface__(self):
         return {"data": (self.ptr.value, False), "shape": self.shape, "typestr": self.dtype.str, "version": 3}
 
     def __del__(self):
-        if self.ptr and _lib is not None:
-            _lib.ut_host_free(self.ptr)
+        if self.ptr and getattr(self, "_free", None) is not None:
+            self._free(self.ptr)
             self.ptr = None
 
 

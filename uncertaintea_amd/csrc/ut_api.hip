@@ -17,8 +17,11 @@
 #include <vector>
 
 #include "../../include/uncertaintea_hip.h"
+#include "ut_chees.h"
 #include "ut_gemm.h"
+#include "ut_glm.h"
 #include "ut_kernels.h"
+#include "ut_lockstep.h"
 #include "ut_math.h"
 
 using namespace ut;
@@ -75,6 +78,8 @@ constexpr double LOG_2 = 0.6931471805599453;
 // ---------------------------------------------------------------------------- plan
 struct ut_plan {
   DevModelDesc dev;
+  // matrix-core GLM (MODEL_GLM_MFMA): evaluated for all chains at once
+  GlmPlan glm;
   // host copies for initial draws and validation
   int P = 0;
   std::vector<void*> owned;  // device allocations
@@ -107,9 +112,11 @@ struct ut_workspace {
   ChainState st;
   double* d_im = nullptr;
   double* d_summary = nullptr;
+  GlmScratch glm;  // MODEL_GLM_MFMA only
   ~ut_workspace() {
     void* ptrs[] = {d_params, d_values, d_grads, st.pos, st.grad, st.lj,
-                    st.step, st.dual, st.im_f, d_im, d_summary};
+                    st.step, st.dual, st.im_f, d_im, d_summary,
+                    glm.bhi, glm.blo, glm.gpart, glm.llpart};
     for (void* p : ptrs) (void)hipFree(p);
     if (stream) (void)hipStreamDestroy(stream);
     if (copy_stream) (void)hipStreamDestroy(copy_stream);
@@ -169,6 +176,48 @@ int ut_plan_logistic_glm(const float* x, const float* y, int64_t n, int32_t d, d
   int rc;
   if ((rc = plan->stage(&m.xT_f, xT_f)) || (rc = plan->stage(&m.xT_d, xT_d)) ||
       (rc = plan->stage(&m.yv_f, y_f)) || (rc = plan->stage(&m.yv_d, y_d))) {
+    delete plan;
+    return rc;
+  }
+  plan->P = d;
+  *out = plan;
+  return UT_OK;
+}
+
+int ut_plan_logistic_glm_bf16(const uint16_t* x_bf16, const float* y, int64_t n, int32_t d, double prior_loc,
+                              double prior_scale, ut_plan** out) {
+  if (!out || !x_bf16 || !y) return fail(UT_E_ARG, "ut_plan_logistic_glm_bf16: null pointer");
+  *out = nullptr;
+  if (n <= 0 || d <= 0) return fail(UT_E_DIM, "ut_plan_logistic_glm_bf16: n and d must be positive");
+  if (d != 256 || n % 32 != 0)
+    return fail(UT_E_UNSUPPORTED, "ut_plan_logistic_glm_bf16: the matrix-core GLM needs d == 256 and n a multiple of 32");
+  if (n > (1ll << 30)) return fail(UT_E_DIM, "ut_plan_logistic_glm_bf16: n too large");
+  if (!(prior_scale > 0)) return fail(UT_E_ARG, "ut_plan_logistic_glm_bf16: prior_scale must be > 0");
+  for (int64_t i = 0; i < n; ++i)
+    if (!(y[i] == 0.0f || y[i] == 1.0f))
+      return fail(UT_E_ARG, "ut_plan_logistic_glm_bf16: y must hold 0/1 outcomes");
+  auto* plan = new ut_plan();
+  std::vector<uint16_t> X(x_bf16, x_bf16 + (size_t)n * d);
+  std::vector<uint16_t> Xt((size_t)n * d);
+  for (int64_t i = 0; i < n; ++i)
+    for (int j = 0; j < d; ++j) Xt[(size_t)j * n + i] = X[(size_t)i * d + j];
+  std::vector<float> y_f(y, y + n);
+  DevModelDesc& m = plan->dev;
+  m.kind = MODEL_GLM_MFMA;
+  m.P = d;
+  m.n = n;
+  m.D = d;
+  m.loc = prior_loc;
+  m.scale = prior_scale;
+  m.total_const = (double)d * (-log_d(prior_scale) - HALF_LOG_2PI);
+  GlmPlan& g = plan->glm;
+  g.N = (int)n;
+  g.D = d;
+  g.loc = prior_loc;
+  g.scale = prior_scale;
+  g.total_const = m.total_const;
+  int rc;
+  if ((rc = plan->stage(&g.X, X)) || (rc = plan->stage(&g.Xt, Xt)) || (rc = plan->stage(&g.y, y_f))) {
     delete plan;
     return rc;
   }
@@ -278,6 +327,11 @@ int ut_workspace_create(const ut_plan* plan, int32_t num_chains, int32_t precisi
   if (num_chains <= 0) return fail(UT_E_DIM, "ut_workspace_create: num_chains must be positive");
   if (precision != UT_FLOAT32 && precision != UT_FLOAT64)
     return fail(UT_E_PRECISION, "ut_workspace_create: precision must be UT_FLOAT32 or UT_FLOAT64");
+  const bool mfma = plan->dev.kind == MODEL_GLM_MFMA;
+  if (mfma && precision != UT_FLOAT32)
+    return fail(UT_E_PRECISION, "ut_workspace_create: the matrix-core GLM plan evaluates in UT_FLOAT32");
+  if (mfma && num_chains % 128 != 0)
+    return fail(UT_E_DIM, "ut_workspace_create: the matrix-core GLM plan needs num_chains a multiple of 128");
   if (!ut_device_available()) return fail(UT_E_HIP, "ut_workspace_create: no HIP device visible");
   auto* ws = new ut_workspace();
   ws->plan = plan;
@@ -299,6 +353,14 @@ int ut_workspace_create(const ut_plan* plan, int32_t num_chains, int32_t precisi
   if (e == hipSuccess) e = dalloc(&ws->st.im_f, (size_t)ws->P);
   if (e == hipSuccess) e = dalloc(&ws->d_im, (size_t)ws->P);
   if (e == hipSuccess) e = dalloc(&ws->d_summary, 8);
+  if (mfma) {
+    GlmScratch& g = ws->glm;
+    g.splits = glm_splits(plan->glm.N, num_chains);
+    if (e == hipSuccess) e = dalloc(&g.bhi, PC);
+    if (e == hipSuccess) e = dalloc(&g.blo, PC);
+    if (e == hipSuccess) e = dalloc(&g.gpart, PC * (size_t)g.splits);
+    if (e == hipSuccess) e = dalloc(&g.llpart, (size_t)num_chains * g.splits);
+  }
   if (e != hipSuccess) {
     delete ws;
     return fail(UT_E_HIP, std::string("ut_workspace_create: ") + hipGetErrorString(e));
@@ -312,13 +374,23 @@ int ut_workspace_destroy(ut_workspace* ws) {
   return UT_OK;
 }
 
+// Logjoint (and gradient) of all chains at fp32 positions q (C x P, device), with
+// the plan's gradient engine: the matrix-core GLM, or one wavefront per chain.
+static hipError_t batched_grad(ut_workspace* ws, const float* q, float* lj, float* grad, bool need_lj) {
+  if (ws->plan->dev.kind == MODEL_GLM_MFMA)
+    return launch_glm_grad(ws->plan->glm, q, grad, lj, ws->C, need_lj, ws->glm, ws->stream);
+  return launch_density_f32(ws->plan->dev, q, lj, grad, ws->C, ws->stream);
+}
+
 static int density_call(ut_workspace* ws, const void* params, void* values, void* gradients) {
   if (!ws || !params || !values) return fail(UT_E_ARG, "density: null pointer");
   const size_t PC = (size_t)ws->P * ws->C;
   const size_t es = ws->precision == UT_FLOAT32 ? 4 : 8;
   UT_HIP(hipMemcpyAsync(ws->d_params, params, PC * es, hipMemcpyHostToDevice, ws->stream));
   hipError_t e;
-  if (ws->precision == UT_FLOAT32)
+  if (ws->plan->dev.kind == MODEL_GLM_MFMA)
+    e = batched_grad(ws, (const float*)ws->d_params, (float*)ws->d_values, (float*)ws->d_grads, true);
+  else if (ws->precision == UT_FLOAT32)
     e = launch_density_f32(ws->plan->dev, (const float*)ws->d_params, (float*)ws->d_values,
                            gradients ? (float*)ws->d_grads : nullptr, ws->C, ws->stream);
   else
@@ -457,7 +529,7 @@ void draw_initial_column(const DevModelDesc& m, int strategy, uint32_t chain, ui
     return;
   }
   auto z = [&](int idx) { return (double)rand_normal_f(chain, key, STREAM_INIT, (uint32_t)idx); };
-  if (m.kind == MODEL_LOGISTIC) {
+  if (m.kind == MODEL_LOGISTIC || m.kind == MODEL_GLM_MFMA) {
     for (int a = 0; a < P; ++a) col[a] = m.loc + m.scale * z(a);
   } else if (m.kind == MODEL_HIER) {
     col[0] = m.mu_loc + m.mu_scale * z(0);
@@ -513,16 +585,14 @@ int validate_opts(const ut_workspace* ws, const ut_sampler_opts* o, const ut_cha
 // that one launch stays in the millisecond range.
 constexpr int kChunk = 50;
 
-int run_sampler(ut_workspace* ws, const ut_sampler_opts* o, ut_chain_out* out, bool nuts) {
-  int rc = validate_opts(ws, o, out, nuts);
-  if (rc) return rc;
-  const double t_start = now_seconds();
-  const int P = ws->P, C = ws->C, S = o->num_samples, W = o->num_warmup;
+// Initial positions, logjoint and gradient of every chain (reference
+// _initial_batched_hmc_positions + the non-finite retry loop,
+// api_batched_nuts.jl:184-200): columns whose logjoint or gradient is not finite
+// are re-drawn with the next attempt key, up to init_max_retries times.
+int init_chains(ut_workspace* ws, const ut_sampler_opts* o, uint32_t rs) {
+  const int P = ws->P, C = ws->C;
   const DevModelDesc& m = ws->plan->dev;
   hipStream_t stream = ws->stream;
-  const uint32_t rs = run_seed_of(o->seed);
-
-  // ---- initial positions, logjoint and gradient (retry non-finite columns)
   std::vector<double> pos((size_t)P * C);
   std::vector<float> pos_f((size_t)P * C);
   std::vector<float> lj_h(C), grad_h((size_t)P * C);
@@ -534,7 +604,7 @@ int run_sampler(ut_workspace* ws, const ut_sampler_opts* o, ut_chain_out* out, b
   for (int attempt = 0;; ++attempt) {
     for (size_t i = 0; i < pos.size(); ++i) pos_f[i] = (float)pos[i];
     UT_HIP(hipMemcpyAsync(ws->st.pos, pos_f.data(), pos_f.size() * 4, hipMemcpyHostToDevice, stream));
-    UT_HIP(launch_density_f32(m, ws->st.pos, ws->st.lj, ws->st.grad, C, stream));
+    UT_HIP(batched_grad(ws, ws->st.pos, ws->st.lj, ws->st.grad, true));
     UT_HIP(hipMemcpyAsync(lj_h.data(), ws->st.lj, (size_t)C * 4, hipMemcpyDeviceToHost, stream));
     UT_HIP(hipMemcpyAsync(grad_h.data(), ws->st.grad, grad_h.size() * 4, hipMemcpyDeviceToHost, stream));
     UT_HIP(hipStreamSynchronize(stream));
@@ -544,7 +614,7 @@ int run_sampler(ut_workspace* ws, const ut_sampler_opts* o, ut_chain_out* out, b
       for (int a = 0; a < P && ok; ++a) ok = std::isfinite(grad_h[(size_t)c * P + a]);
       if (!ok) bad.push_back(c);
     }
-    if (bad.empty()) break;
+    if (bad.empty()) return UT_OK;
     if (o->initial_params || attempt >= o->init_max_retries) {
       char buf[256];
       std::snprintf(buf, sizeof(buf),
@@ -555,6 +625,20 @@ int run_sampler(ut_workspace* ws, const ut_sampler_opts* o, ut_chain_out* out, b
     for (int c : bad)
       draw_initial_column(m, o->init_strategy, (uint32_t)c, rs + (uint32_t)(attempt + 1), &pos[(size_t)c * P]);
   }
+}
+
+int run_sampler(ut_workspace* ws, const ut_sampler_opts* o, ut_chain_out* out, bool nuts) {
+  int rc = validate_opts(ws, o, out, nuts);
+  if (rc) return rc;
+  if (ws->plan->dev.kind == MODEL_GLM_MFMA)
+    return fail(UT_E_UNSUPPORTED,
+                "the matrix-core GLM plan evaluates all chains at once; sample it with the lock-step samplers (ut_chees_run)");
+  const double t_start = now_seconds();
+  const int P = ws->P, C = ws->C, S = o->num_samples, W = o->num_warmup;
+  const DevModelDesc& m = ws->plan->dev;
+  hipStream_t stream = ws->stream;
+  const uint32_t rs = run_seed_of(o->seed);
+  if ((rc = init_chains(ws, o, rs))) return rc;
 
   // ---- adaptation state
   std::vector<double> im_h(P, 1.0);
@@ -685,9 +769,293 @@ int run_sampler(ut_workspace* ws, const ut_sampler_opts* o, ut_chain_out* out, b
   return UT_OK;
 }
 
+// ---------------------------------------------------------------- ChEES driver
+// Shared dual averaging on the host (reference _dual_averaging_state /
+// _update_step_size! / _final_step_size, nuts/adaptation.jl:1-28), the same
+// arithmetic as the kernels' per-chain dual_update.
+struct HostDual {
+  double mu = 0, log_step = 0, log_avg = 0, hbar = 0, it = 0, target = 0.8;
+  void reset(double step, double tgt) {
+    target = tgt;
+    mu = log_d(10.0 * step);
+    log_step = log_d(step);
+    log_avg = log_step;
+    hbar = 0.0;
+    it = 0.0;
+  }
+  double update(double accept) {
+    const double gamma = 0.05, t0 = 10.0, kappa = 0.75;
+    it = it + 1.0;
+    const double eta = 1.0 / (it + t0);
+    hbar = (1.0 - eta) * hbar + eta * (target - accept);
+    log_step = mu - sqrt_d(it) / gamma * hbar;
+    const double eta_bar = pow_d(it, -kappa);
+    log_avg = eta_bar * log_step + (1.0 - eta_bar) * log_avg;
+    return exp_d(log_step);
+  }
+  double final_step() const { return exp_d(log_avg); }
+};
+
+struct PinnedStats {
+  double* p = nullptr;
+  ~PinnedStats() {
+    if (p) (void)hipHostFree(p);
+  }
+};
+
+int validate_chees(const ut_workspace* ws, const ut_chees_opts* co, const ut_chain_out* out) {
+  if (!co) return fail(UT_E_ARG, "batched_chees: null options");
+  int rc = validate_opts(ws, &co->base, out, false);
+  if (rc) return rc;
+  if (!(co->jitter_amount >= 0.0 && co->jitter_amount <= 1.0))
+    return fail(UT_E_ARG, "batched_chees jitter_amount must be in [0, 1]");
+  if (co->max_leapfrog_steps < 1) return fail(UT_E_ARG, "batched_chees max_leapfrog_steps must be >= 1");
+  if (!(co->trajectory_decay_rate > 0.0 && co->trajectory_decay_rate <= 1.0))
+    return fail(UT_E_ARG, "batched_chees trajectory_decay_rate must be in (0, 1]");
+  if (!(co->trajectory_learning_rate > 0.0))
+    return fail(UT_E_ARG, "batched_chees trajectory_learning_rate must be > 0");
+  if (!(co->trajectory_adam_beta1 >= 0.0 && co->trajectory_adam_beta1 < 1.0))
+    return fail(UT_E_ARG, "batched_chees trajectory_adam_beta1 must be in [0, 1)");
+  if (!(co->trajectory_adam_beta2 >= 0.0 && co->trajectory_adam_beta2 < 1.0))
+    return fail(UT_E_ARG, "batched_chees trajectory_adam_beta2 must be in [0, 1)");
+  if (!(co->trajectory_adam_epsilon > 0.0))
+    return fail(UT_E_ARG, "batched_chees trajectory_adam_epsilon must be > 0");
+  if (ws->P > 256) return fail(UT_E_UNSUPPORTED, "batched_chees: more than 256 parameters");
+  return UT_OK;
+}
+
+// Lock-step ChEES-HMC (reference batched_chees host loop,
+// api_batched_chees.jl:435-700, and its shared WarmupDriver, warmup.jl:262-327).
+// Every chain integrates the same jittered number of leapfrog steps with the shared
+// step size; the gradient between the trajectory kernels comes from the plan's
+// engine for all chains at once.  Warmup reads four binary64 scalars back per
+// iteration (the ChEES numerator and denominator, the harmonic-mean and the mean
+// acceptance) and runs the scalar state machines on the host; sampling launches
+// without any host synchronisation.
+int run_chees(ut_workspace* ws, const ut_chees_opts* co, ut_chain_out* out, double* trace) {
+  int rc = validate_chees(ws, co, out);
+  if (rc) return rc;
+  const ut_sampler_opts* o = &co->base;
+  const double t_start = now_seconds();
+  const int P = ws->P, C = ws->C, S = o->num_samples, W = o->num_warmup;
+  hipStream_t stream = ws->stream;
+  const uint32_t rs = run_seed_of(o->seed);
+  if ((rc = init_chains(ws, o, rs))) return rc;
+
+  std::vector<double> im_h(P, 1.0);
+  std::vector<float> im_f(P, 1.0f);
+  UT_HIP(hipMemcpyAsync(ws->d_im, im_h.data(), P * 8, hipMemcpyHostToDevice, stream));
+  UT_HIP(hipMemcpyAsync(ws->st.im_f, im_f.data(), P * 4, hipMemcpyHostToDevice, stream));
+
+  DeviceBuffers buf;
+  LockstepState ls;
+  ls.pos = ws->st.pos;
+  ls.grad = ws->st.grad;
+  ls.lj = ws->st.lj;
+  ls.im_f = ws->st.im_f;
+  const size_t PC = (size_t)P * C;
+  UT_HIP(buf.alloc(&ls.wq, PC));
+  UT_HIP(buf.alloc(&ls.wp, PC));
+  UT_HIP(buf.alloc(&ls.wg, PC));
+  UT_HIP(buf.alloc(&ls.wlj, (size_t)C));
+  UT_HIP(buf.alloc(&ls.h0, (size_t)C));
+  UT_HIP(buf.alloc(&ls.valid, (size_t)C));
+  UT_HIP(buf.alloc(&ls.initials, PC));
+  LockstepRecord rec;
+  UT_HIP(buf.alloc(&rec.accept_prob, (size_t)C));
+  UT_HIP(buf.alloc(&rec.flags, (size_t)C));
+  UT_HIP(buf.alloc(&rec.pfinite, (size_t)C));
+  UT_HIP(buf.alloc(&rec.qfinite, (size_t)C));
+  UT_HIP(buf.alloc(&rec.ifinite, (size_t)C));
+  UT_HIP(buf.alloc(&rec.energy, (size_t)C));
+  UT_HIP(buf.alloc(&rec.error, (size_t)C));
+  UT_HIP(buf.alloc(&rec.imean, (size_t)P));
+  UT_HIP(buf.alloc(&rec.pmean, (size_t)P));
+  UT_HIP(buf.alloc(&rec.inv_sqrt, (size_t)P));
+  UT_HIP(buf.alloc(&rec.term, (size_t)C));
+  UT_HIP(buf.alloc(&rec.include, (size_t)C));
+  double* d_stats = nullptr;
+  UT_HIP(buf.alloc(&d_stats, 4));
+  PinnedStats hs;
+  UT_HIP(hipHostMalloc((void**)&hs.p, 4 * sizeof(double), hipHostMallocDefault));
+
+  SamplerParams sp;
+  sp.run_seed = rs;
+  sp.P = P;
+  sp.C = C;
+  sp.num_warmup = W;
+  sp.num_samples = S;
+  UT_HIP(buf.alloc(&sp.hist_pos, (size_t)W * C * P));
+  UT_HIP(buf.alloc(&sp.hist_acc, (size_t)W * C));
+  UT_HIP(buf.alloc(&sp.hist_flags, (size_t)W * C));
+  UT_HIP(buf.alloc(&sp.draws, (size_t)S * C * P));
+  UT_HIP(buf.alloc(&sp.o_lj, (size_t)S * C));
+  UT_HIP(buf.alloc(&sp.o_acc, (size_t)S * C));
+  UT_HIP(buf.alloc(&sp.o_energy, (size_t)S * C));
+  UT_HIP(buf.alloc(&sp.o_err, (size_t)S * C));
+  UT_HIP(buf.alloc(&sp.o_accepted, (size_t)S * C));
+  UT_HIP(buf.alloc(&sp.o_divergent, (size_t)S * C));
+  UT_HIP(buf.alloc(&sp.o_depth, (size_t)S * C));
+  UT_HIP(buf.alloc(&sp.o_steps, (size_t)S * C));
+
+  auto read_stats = [&]() -> hipError_t {
+    hipError_t e = hipMemcpyAsync(hs.p, d_stats, 4 * sizeof(double), hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    return e;
+  };
+
+  // shared reasonable step size (reference _find_reasonable_batched_step_size,
+  // nuts/adaptation.jl:303-375): one momentum draw, a one-step trajectory per trial,
+  // doubling or halving until the mean acceptance (divergent chains count 0)
+  // crosses 1/2, at most 20 doublings
+  auto search = [&](double step, uint32_t itc, double* result) -> int {
+    const double min_step = 1e-8, max_step = 1e3, target = 0.5;
+    double direction = 0.0;
+    for (int trial = 0; trial <= 20; ++trial) {
+      const float eps = (float)step;
+      UT_HIP(launch_ls_begin(ls, P, C, itc, STREAM_STEP_SEARCH, eps, 0, stream));
+      UT_HIP(batched_grad(ws, ls.wq, ls.wlj, ls.wg, true));
+      UT_HIP(launch_ls_end(ls, rec, P, C, itc, eps, o->divergence_threshold, 1, stream));
+      UT_HIP(launch_chees_stats(ls, rec, ws->d_im, P, C, false, 1.0, d_stats, stream));
+      UT_HIP(read_stats());
+      const double mean_accept = hs.p[3];
+      if (trial == 0) {
+        direction = mean_accept > target ? 1.0 : -1.0;
+      } else if ((direction > 0 && mean_accept <= target) || (direction < 0 && mean_accept >= target)) {
+        break;
+      }
+      const double next = step * (direction > 0 ? 2.0 : 0.5);
+      if (next < min_step || next > max_step) break;
+      step = next;
+    }
+    *result = step < min_step ? min_step : (step > max_step ? max_step : step);
+    return UT_OK;
+  };
+
+  double eps = o->step_size;
+  if (o->find_reasonable_step_size || (W > 0 && o->adapt_step_size))
+    if ((rc = search(eps, rs, &eps))) return rc;
+  HostDual dual;
+  dual.reset(eps, o->target_accept);
+  const double T0 = (double)o->num_leapfrog_steps * eps;
+  CheesTrajectory traj{T0, log_d(T0),
+                       CheesAdam{co->trajectory_adam_beta1, co->trajectory_adam_beta2, co->trajectory_adam_epsilon,
+                                 co->trajectory_learning_rate},
+                       1, co->trajectory_decay_rate, co->max_leapfrog_steps};
+  const Schedule sch = warmup_schedule(W);
+  int window = 0;
+  const bool adapt_traj = W > 0 && co->adapt_trajectory_length;
+
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  UT_HIP(hipEventCreate(&ev0));
+  UT_HIP(hipEventCreate(&ev1));
+  double t_warm = t_start;
+  long long evals = 0;
+  int launches = 0;
+  for (int it = 1; it <= W + S; ++it) {
+    if (it == W + 1) {
+      UT_HIP(hipStreamSynchronize(stream));
+      t_warm = now_seconds();
+      UT_HIP(hipEventRecord(ev0, stream));
+    }
+    const double jitter = chees_jitter(co->jitter_amount, it);
+    const int L = chees_steps(jitter, traj.T, eps);
+    const uint32_t itc = rs + (uint32_t)it;
+    const bool warm = it <= W;
+    const float eps_f = (float)eps;
+    UT_HIP(launch_ls_begin(ls, P, C, itc, STREAM_MOMENTUM, eps_f, (warm && adapt_traj) ? 1 : 0, stream));
+    for (int l = 1; l < L; ++l) {
+      UT_HIP(batched_grad(ws, ls.wq, ls.wlj, ls.wg, false));
+      UT_HIP(launch_ls_step(ls, P, C, eps_f, stream));
+    }
+    UT_HIP(batched_grad(ws, ls.wq, ls.wlj, ls.wg, true));
+    UT_HIP(launch_ls_end(ls, rec, P, C, itc, eps_f, o->divergence_threshold, 0, stream));
+    UT_HIP(launch_ls_record(ls, rec, sp, it, L, stream));
+    if (!warm) {
+      evals += (long long)L * C;
+      launches += 2 * L + 2;
+      continue;
+    }
+    UT_HIP(launch_chees_stats(ls, rec, ws->d_im, P, C, adapt_traj, co->forward_velocity ? -1.0 : 1.0, d_stats,
+                              stream));
+    UT_HIP(read_stats());
+    const double num = hs.p[0], den = hs.p[1], harmonic = hs.p[2];
+    if (o->adapt_step_size) eps = dual.update(harmonic);
+    if (o->adapt_mass_matrix && window < (int)sch.ends.size() && sch.ends[window] == it) {
+      const int start = window == 0 ? sch.initial_buffer + 1 : sch.ends[window - 1] + 1;
+      UT_HIP(launch_mass_replay(sp.hist_pos, sp.hist_acc, sp.hist_flags, C, P, start, it, it - start + 1, 1,
+                                o->mass_matrix_regularization, o->mass_matrix_min_samples, ws->d_im,
+                                ws->st.im_f, ws->d_summary, stream));
+      ++window;
+      if (o->adapt_step_size && it < W) {
+        if ((rc = search(eps, rs + (uint32_t)it, &eps))) return rc;
+        dual.reset(eps, o->target_accept);
+      }
+    }
+    if (adapt_traj) traj.apply(num, den, jitter, eps);
+    if (trace) trace[it - 1] = traj.T;
+    if (it == W && o->adapt_step_size) eps = dual.final_step();
+  }
+  if (S == 0) {
+    UT_HIP(hipStreamSynchronize(stream));
+    t_warm = now_seconds();
+    UT_HIP(hipEventRecord(ev0, stream));
+  }
+  UT_HIP(hipEventRecord(ev1, stream));
+  auto copy = [&](void* dst, const void* src, size_t bytes) -> hipError_t {
+    if (!dst || bytes == 0) return hipSuccess;
+    return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream);
+  };
+  const size_t SC = (size_t)S * C;
+  UT_HIP(copy(out->samples, sp.draws, SC * P * 4));
+  UT_HIP(copy(out->logjoint, sp.o_lj, SC * 8));
+  UT_HIP(copy(out->accept_stat, sp.o_acc, SC * 8));
+  UT_HIP(copy(out->energy, sp.o_energy, SC * 8));
+  UT_HIP(copy(out->energy_error, sp.o_err, SC * 8));
+  UT_HIP(copy(out->accepted, sp.o_accepted, SC));
+  UT_HIP(copy(out->divergent, sp.o_divergent, SC));
+  UT_HIP(copy(out->tree_depth, sp.o_depth, SC * 4));
+  UT_HIP(copy(out->integration_steps, sp.o_steps, SC * 4));
+  if (out->inverse_mass) UT_HIP(copy(out->inverse_mass, ws->d_im, (size_t)P * 8));
+  UT_HIP(hipStreamSynchronize(stream));
+  const double t_end = now_seconds();
+  float kernel_ms = 0.0f;
+  UT_HIP(hipEventElapsedTime(&kernel_ms, ev0, ev1));
+  (void)hipEventDestroy(ev0);
+  (void)hipEventDestroy(ev1);
+  if (out->step_size)
+    for (int c = 0; c < C; ++c) out->step_size[c] = eps;
+  out->warmup_seconds = t_warm - t_start;
+  out->sampling_seconds = t_end - t_warm;
+  out->sampling_gradient_evals = evals;
+  out->sampling_kernel_seconds = 1e-3 * (double)kernel_ms;
+  out->sampling_launches = launches;
+  return UT_OK;
+}
+
 }  // namespace
 
 extern "C" {
+
+void ut_chees_opts_default(ut_chees_opts* co) {
+  if (!co) return;
+  std::memset(co, 0, sizeof(*co));
+  ut_sampler_opts_default(&co->base);
+  co->base.target_accept = 0.651;
+  co->jitter_amount = 1.0;
+  co->adapt_trajectory_length = 1;
+  co->max_leapfrog_steps = 1000;
+  co->trajectory_decay_rate = 0.5;
+  co->trajectory_learning_rate = 0.5;
+  co->trajectory_adam_beta1 = 0.0;
+  co->trajectory_adam_beta2 = 0.95;
+  co->trajectory_adam_epsilon = 1e-8;
+  co->forward_velocity = 0;
+}
+
+int ut_chees_run(ut_workspace* ws, const ut_chees_opts* opts, ut_chain_out* out, double* trajectory_trace) {
+  return run_chees(ws, opts, out, trajectory_trace);
+}
 
 int ut_nuts_run(ut_workspace* ws, const ut_sampler_opts* opts, ut_chain_out* out) {
   return run_sampler(ws, opts, out, true);

@@ -6,7 +6,9 @@
 
 namespace ut {
 
-enum ModelKind { MODEL_LOGISTIC = 1, MODEL_HIER = 2, MODEL_FUNNEL = 3 };
+// MODEL_GLM_MFMA has no wavefront-per-chain evaluator: its density and gradient run
+// for all chains at once on the matrix cores (ut_glm.h).
+enum ModelKind { MODEL_LOGISTIC = 1, MODEL_HIER = 2, MODEL_FUNNEL = 3, MODEL_GLM_MFMA = 4 };
 
 // Philox stream ids.  NUTS uses 0..3 exactly as the reference's persistent kernel
 // (src/device/persistent_nuts.jl:70-73); the remaining purposes get their own ids.

@@ -36,14 +36,15 @@ class DeviceBatchedWorkspace:
         h = ct.c_void_p()
         L.check(L.lib().ut_workspace_create(model.plan, self.num_chains, _PREC[self.dtype], ct.byref(h)))
         self._h = h
+        self._destroy = L.lib().ut_workspace_destroy  # kept: module globals may be gone at exit
 
     @property
     def handle(self):
         return self._h
 
     def __del__(self):
-        if getattr(self, "_h", None) is not None and L._lib is not None:
-            L._lib.ut_workspace_destroy(self._h)
+        if getattr(self, "_h", None) is not None:
+            self._destroy(self._h)
             self._h = None
 
     def _params(self, params) -> np.ndarray:

@@ -1,7 +1,8 @@
-"""Batched samplers on the gfx950 backend: `batched_nuts` and `batched_hmc`.
+"""Batched samplers on the gfx950 backend: `batched_nuts`, `batched_hmc` and
+`batched_chees`.
 
 Signatures and defaults follow the reference (src/inference/api_batched_nuts.jl:22-51,
-src/inference/api_batched_hmc.jl:16-39).  The device path is always the pooled-mass /
+src/inference/api_batched_hmc.jl:16-39, src/inference/api_batched_chees.jl:277-306).  The device path is always the pooled-mass /
 per-chain-step adaptation the reference uses on devices
 (api_batched_nuts.jl:761-980, src/device/hmc_kernels.jl:306-723); all randomness is
 drawn on the device from Philox coordinates derived from `seed`, so a run is a pure
@@ -48,6 +49,7 @@ class HMCChains:
     sampling_gradient_evals: int
     sampling_kernel_seconds: float = 0.0
     sampling_launches: int = 0
+    trajectory_trace: np.ndarray | None = None
     _constrained: np.ndarray | None = field(default=None, repr=False)
 
     @property
@@ -109,12 +111,13 @@ def _opts(num_samples, num_warmup, step_size, target_accept, divergence_threshol
     return o
 
 
-def _run(kind: str, model: TeaModel, num_chains: int, opts: L.SamplerOpts, initial_params,
+def _run(kind: str, model: TeaModel, num_chains: int, opts, initial_params,
          max_tree_depth: int, num_leapfrog_steps: int, target_accept: float) -> HMCChains:
     if num_chains <= 0:
         raise ValueError("num_chains must be positive")
     ws = DeviceBatchedWorkspace(model, num_chains, np.float32)
-    P, C, S = model.num_params, int(num_chains), int(opts.num_samples)
+    base = opts.base if kind == "chees" else opts
+    P, C, S = model.num_params, int(num_chains), int(base.num_samples)
     init = None
     if initial_params is not None:
         init = np.asarray(initial_params, dtype=np.float64)
@@ -123,7 +126,7 @@ def _run(kind: str, model: TeaModel, num_chains: int, opts: L.SamplerOpts, initi
         if init.shape != (P, C):
             raise ValueError(f"initial_params must be P x C = {P} x {C}")
         init = np.ascontiguousarray(init.T)  # chain-major
-        opts.initial_params = L.dptr(init, ct.c_double)
+        base.initial_params = L.dptr(init, ct.c_double)
     pe = L.pinned_empty  # page-locked: the sampler copies results out while it computes
     samples = pe((C, S, P), np.float32)
     lj, acc, en, err = pe((C, S), np.float64), pe((C, S), np.float64), pe((C, S), np.float64), pe((C, S), np.float64)
@@ -142,12 +145,17 @@ def _run(kind: str, model: TeaModel, num_chains: int, opts: L.SamplerOpts, initi
     out.integration_steps = L.dptr(steps, ct.c_int32)
     out.step_size = L.dptr(step, ct.c_double)
     out.inverse_mass = L.dptr(im, ct.c_double)
-    fn = L.lib().ut_nuts_run if kind == "nuts" else L.lib().ut_hmc_run
-    L.check(fn(ws.handle, ct.byref(opts), ct.byref(out)))
+    trace = None
+    if kind == "chees":
+        trace = np.zeros(int(base.num_warmup), dtype=np.float64)
+        L.check(L.lib().ut_chees_run(ws.handle, ct.byref(opts), ct.byref(out), L.ptr(trace)))
+    else:
+        fn = L.lib().ut_nuts_run if kind == "nuts" else L.lib().ut_hmc_run
+        L.check(fn(ws.handle, ct.byref(opts), ct.byref(out)))
     return HMCChains(kind, model, samples, lj, acc, en, err, accepted.view(bool), div.view(bool), depth,
                      steps, step, im, max_tree_depth, num_leapfrog_steps, target_accept, out.warmup_seconds,
                      out.sampling_seconds, int(out.sampling_gradient_evals), out.sampling_kernel_seconds,
-                     int(out.sampling_launches))
+                     int(out.sampling_launches), trace)
 
 
 def batched_nuts(model: TeaModel, args=(), constraints=None, *, num_chains: int, num_samples: int,
@@ -191,3 +199,46 @@ def batched_hmc(model: TeaModel, args=(), constraints=None, *, num_chains: int, 
               mass_matrix_min_samples, adapt_step_size, adapt_mass_matrix, find_reasonable_step_size, init_strategy,
               init_max_retries, seed, num_leapfrog_steps=num_leapfrog_steps)
     return _run("hmc", model, num_chains, o, initial_params, 0, num_leapfrog_steps, target_accept)
+
+
+def batched_chees(model: TeaModel, args=(), constraints=None, *, num_chains: int, num_samples: int,
+                  num_warmup: int = 0, step_size: float = 0.1, num_leapfrog_steps: int = 10,
+                  jitter_amount: float = 1.0, adapt_trajectory_length: bool = True, max_leapfrog_steps: int = 1000,
+                  trajectory_decay_rate: float = 0.5, trajectory_learning_rate: float = 0.5,
+                  trajectory_adam_beta1: float = 0.0, trajectory_adam_beta2: float = 0.95,
+                  trajectory_adam_epsilon: float = 1e-8, initial_params=None, init_strategy: str = "prior",
+                  init_max_retries: int = 100, target_accept: float = 0.651, adapt_step_size: bool = True,
+                  adapt_mass_matrix: bool = True, find_reasonable_step_size: bool = False,
+                  divergence_threshold: float = 1000.0, mass_matrix_regularization: float = 1e-3,
+                  mass_matrix_min_samples: int = 10, backend: str = "hip", precision=np.float32,
+                  seed: int = 0, forward_velocity: bool = False) -> HMCChains:
+    """ChEES-HMC: every chain integrates the same Halton-jittered number of leapfrog
+    steps with a shared step size; warmup adapts the step (dual averaging to the
+    harmonic-mean acceptance), a pooled diagonal mass and the trajectory time T
+    (Adam ascent on the cross-chain ChEES criterion).  `trajectory_trace` holds T
+    after each warmup iteration.
+
+    `forward_velocity` (not a reference option, default off for parity): the
+    reference feeds the criterion the proposal momentum after its final flip, so
+    its Adam ascent shortens T; True uses the velocity in the direction of
+    integration, under which T grows where longer trajectories pay off."""
+    if backend != "hip":
+        raise ValueError(f"backend must be 'hip', got {backend!r}")
+    if np.dtype(precision) != np.float32:
+        raise ValueError("the device samplers run in float32 with float64 energies")
+    co = L.CheesOpts()
+    L.lib().ut_chees_opts_default(ct.byref(co))
+    co.base = _opts(num_samples, num_warmup, step_size, target_accept, divergence_threshold,
+                    mass_matrix_regularization, mass_matrix_min_samples, adapt_step_size, adapt_mass_matrix,
+                    find_reasonable_step_size, init_strategy, init_max_retries, seed,
+                    num_leapfrog_steps=num_leapfrog_steps)
+    co.jitter_amount = float(jitter_amount)
+    co.adapt_trajectory_length = int(bool(adapt_trajectory_length))
+    co.max_leapfrog_steps = int(max_leapfrog_steps)
+    co.trajectory_decay_rate = float(trajectory_decay_rate)
+    co.trajectory_learning_rate = float(trajectory_learning_rate)
+    co.trajectory_adam_beta1 = float(trajectory_adam_beta1)
+    co.trajectory_adam_beta2 = float(trajectory_adam_beta2)
+    co.trajectory_adam_epsilon = float(trajectory_adam_epsilon)
+    co.forward_velocity = int(bool(forward_velocity))
+    return _run("chees", model, num_chains, co, initial_params, 0, num_leapfrog_steps, target_accept)

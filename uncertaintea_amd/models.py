@@ -46,11 +46,12 @@ class TeaModel:
             h = ct.c_void_p()
             L.check(self._create_plan(ct.byref(h)))
             self._plan = h
+            self._destroy = L.lib().ut_plan_destroy  # kept: module globals may be gone at exit
         return self._plan
 
     def __del__(self):
-        if getattr(self, "_plan", None) is not None and L._lib is not None:
-            L._lib.ut_plan_destroy(self._plan)
+        if getattr(self, "_plan", None) is not None:
+            self._destroy(self._plan)
             self._plan = None
 
 
@@ -80,6 +81,29 @@ class LogisticRegression(TeaModel):
         n, d = self.x.shape
         return L.lib().ut_plan_logistic_glm(L.ptr(self.x), L.ptr(self.y), n, d, self.prior_loc,
                                             self.prior_scale, out)
+
+
+class LogisticRegressionBF16(LogisticRegression):
+    """The logistic regression with a bfloat16 design matrix, evaluated for all
+    chains at once on the matrix cores (D = 256, N a multiple of 32; chains in
+    multiples of 128).  `x` is rounded to bfloat16 (round to nearest even) once, and
+    that rounded matrix is the model's data: `x_model` holds it as float32."""
+
+    name = "logistic_regression_bf16"
+
+    def __init__(self, x: np.ndarray, y: np.ndarray, prior_loc: float = 0.0, prior_scale: float = 1.0):
+        xb = np.asarray(x)
+        self.x_bf16 = np.ascontiguousarray(xb if xb.dtype == np.uint16 else L.to_bf16(xb), dtype=np.uint16)
+        super().__init__(L.from_bf16(self.x_bf16), y, prior_loc, prior_scale)
+
+    @property
+    def x_model(self) -> np.ndarray:
+        return self.x
+
+    def _create_plan(self, out):
+        n, d = self.x_bf16.shape
+        return L.lib().ut_plan_logistic_glm_bf16(L.ptr(self.x_bf16), L.ptr(self.y), n, d, self.prior_loc,
+                                                 self.prior_scale, out)
 
 
 class HierarchicalNormalNC(TeaModel):
