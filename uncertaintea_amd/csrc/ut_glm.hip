@@ -15,6 +15,13 @@
 // accumulation.  Partials per split are reduced by k_glm_reduce, which adds the
 // normal prior.
 //
+// Latency: every wavefront runs alone on its SIMD (the position tiles fill the
+// LDS), so the kernel hides global latency itself: the X fragments and outcomes of
+// row block k + 1 are requested right after stage 1 of block k and the X^T
+// fragments right after its stage 2, the position fragments are read from LDS one
+// k-step ahead of the multiply, and the loop has no workgroup barrier (each
+// wavefront's residual tile is its own).
+//
 // Work mapping is XCD-aware: workgroup g handles split g % S and chain block g / S;
 // workgroups are dispatched round-robin over the 8 XCDs, so with S a multiple of 8
 // every XCD streams the same rows of X through its own L2 for all its chain blocks.
@@ -41,7 +48,20 @@ __device__ inline float fast_log1p_exp(float eta) {
   return fmaxf(eta, 0.0f) + __builtin_amdgcn_logf(1.0f + e) * 0.6931471805599453f;
 }
 
-typedef uint16_t bf16x4_t __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x2_hw __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+
+// two floats -> packed bf16 pair, round to nearest even (one v_cvt_pk_bf16_f32)
+__device__ inline uint32_t pack_bf16(float a, float b) {
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2){a, b}, bf16x2_hw));
+}
+// hi and lo bf16 pairs of two floats: v = hi + lo to ~16 mantissa bits
+__device__ inline void split_pair(float a, float b, uint32_t& hi, uint32_t& lo) {
+  hi = pack_bf16(a, b);
+  const float ha = __builtin_bit_cast(float, hi << 16);
+  const float hb = __builtin_bit_cast(float, hi & 0xffff0000u);
+  lo = pack_bf16(a - ha, b - hb);
+}
 
 template <bool NEED_LJ>
 __global__ __launch_bounds__(256) void k_glm_grad(const bf16_t* __restrict__ X, const bf16_t* __restrict__ Xt,
@@ -85,10 +105,13 @@ __global__ __launch_bounds__(256) void k_glm_grad(const bf16_t* __restrict__ X, 
   // operand fragments of the current row block, loaded one block ahead
   bf16x8 xa[GD / 16];     // stage 1 A: X[rows][covariates]
   bf16x8 xb[8][RB / 16];  // stage 2 B: Xt[covariates][rows]
+  float4 yv[4];           // outcomes of rows 8 q + 4 half .. + 3 (the epilogue's rows)
   auto load_x = [&](long long rb0) {
     const bf16_t* xrow = X + (rb0 + l32) * GD + 8 * half;
 #pragma unroll
     for (int t = 0; t < GD / 16; ++t) xa[t] = *(const bf16x8*)(xrow + 16 * t);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) yv[q] = *(const float4*)(y + rb0 + 8 * q + 4 * half);
   };
   auto load_xt = [&](long long rb0) {
 #pragma unroll
@@ -103,43 +126,63 @@ __global__ __launch_bounds__(256) void k_glm_grad(const bf16_t* __restrict__ X, 
   }
 
   for (long long rb0 = r_begin; rb0 < r_end; rb0 += RB) {
-    const bool more = rb0 + RB < r_end;
+    // the prefetch is unconditional (the last block re-reads itself), so the
+    // compiler's in-order load counters stay exact across the loop
+    const long long next = rb0 + RB < r_end ? rb0 + RB : rb0;
     // ---- stage 1: eta tile (rows x chains of this wavefront)
     f32x16 e1;
 #pragma unroll
     for (int r = 0; r < 16; ++r) e1[r] = 0.0f;
+    bf16x8 bh = *(const bf16x8*)(bh_row);
+    bf16x8 bl = *(const bf16x8*)(bl_row);
 #pragma unroll
     for (int t = 0; t < GD / 16; ++t) {
-      const bf16x8 bh = *(const bf16x8*)(bh_row + 16 * t);
-      const bf16x8 bl = *(const bf16x8*)(bl_row + 16 * t);
+      bf16x8 nh = bh, nl = bl;
+      if (t + 1 < GD / 16) {
+        nh = *(const bf16x8*)(bh_row + 16 * (t + 1));
+        nl = *(const bf16x8*)(bl_row + 16 * (t + 1));
+      }
       e1 = mfma32(xa[t], bh, e1);
       e1 = mfma32(xa[t], bl, e1);
+      bh = nh;
+      bl = nl;
     }
-    if (more) load_x(rb0 + RB);
+    float ycur[16];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      ycur[4 * q + 0] = yv[q].x;
+      ycur[4 * q + 1] = yv[q].y;
+      ycur[4 * q + 2] = yv[q].z;
+      ycur[4 * q + 3] = yv[q].w;
+    }
+    load_x(next);
     // ---- epilogue: residual hi/lo to LDS (four consecutive rows per store),
     // log-likelihood partials
     float lsum = 0.0f;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      bf16x4_t vh, vl;
+      float res[4];
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const int r = 4 * q + i;
-        const int row = mfma32_row(lane, r);
         const float eta = e1[r];
-        const float yv = y[rb0 + row];
-        const float res = yv - fast_logistic(eta);
-        const bf16_t h = f2bf(res);
-        vh[i] = h;
-        vl[i] = f2bf(res - bf2f(h));
-        if (NEED_LJ) lsum += yv * eta - fast_log1p_exp(eta);
+        const float yr = ycur[r];
+        res[i] = yr - fast_logistic(eta);
+        if (NEED_LJ) lsum += yr * eta - fast_log1p_exp(eta);
       }
+      uint2 vh, vl;
+      split_pair(res[0], res[1], vh.x, vl.x);
+      split_pair(res[2], res[3], vh.y, vl.y);
       const int row0 = 8 * q + 4 * half;
-      *(bf16x4_t*)(rh_row + row0) = vh;
-      *(bf16x4_t*)(rl_row + row0) = vl;
+      *(uint2*)(rh_row + row0) = vh;
+      *(uint2*)(rl_row + row0) = vl;
     }
     if (NEED_LJ) ll += (double)lsum;
-    __syncthreads();
+    // the residual tile is private to this wavefront, whose LDS operations
+    // complete in program order: a wavefront-scope fence orders them
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     // ---- stage 2: Gt[32 chains x 256] += R^T X
 #pragma unroll
     for (int t = 0; t < RB / 16; ++t) {
@@ -151,8 +194,10 @@ __global__ __launch_bounds__(256) void k_glm_grad(const bf16_t* __restrict__ X, 
         g2[jn] = mfma32(al, xb[jn][t], g2[jn]);
       }
     }
-    if (more) load_xt(rb0 + RB);
-    __syncthreads();
+    load_xt(next);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
   float* G = Gpart + (long long)split * C * GD;
 #pragma unroll
@@ -169,15 +214,16 @@ __global__ __launch_bounds__(256) void k_glm_grad(const bf16_t* __restrict__ X, 
   }
 }
 
-// positions (fp32, C x D) -> bf16 hi and lo parts
+// positions (fp32, C x D) -> bf16 hi and lo parts, two per thread (n even)
 __global__ void k_split_bf16(const float* __restrict__ q, bf16_t* __restrict__ hi, bf16_t* __restrict__ lo,
                              long long n) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long i = 2 * ((long long)blockIdx.x * blockDim.x + threadIdx.x);
   if (i >= n) return;
-  const float v = q[i];
-  const bf16_t h = f2bf(v);
-  hi[i] = h;
-  lo[i] = f2bf(v - bf2f(h));
+  const float2 v = *(const float2*)(q + i);
+  uint32_t h, l;
+  split_pair(v.x, v.y, h, l);
+  *(uint32_t*)(hi + i) = h;
+  *(uint32_t*)(lo + i) = l;
 }
 
 // gradient = sum over splits (fixed order) + prior; logjoint likewise in binary64
@@ -224,7 +270,7 @@ hipError_t launch_glm_grad(const GlmPlan& p, const float* q, float* grad, float*
                            GlmScratch& s, hipStream_t stream) {
   if (C % CB != 0 || p.D != GD || p.N % RB != 0 || s.splits < 1) return hipErrorInvalidValue;
   const long long n = (long long)C * GD;
-  hipLaunchKernelGGL(k_split_bf16, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, q, s.bhi, s.blo, n);
+  hipLaunchKernelGGL(k_split_bf16, dim3((unsigned)((n / 2 + 255) / 256)), dim3(256), 0, stream, q, s.bhi, s.blo, n);
   const int S = s.splits;
   const int rows = ((p.N / S + RB - 1) / RB) * RB;
   const dim3 grid((unsigned)(S * (C / CB)));
