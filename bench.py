@@ -1,5 +1,7 @@
 """Benchmark: min bulk-ESS/s at R-hat < 1.01 of batched NUTS on one MI355X.
 
+Default workload (`--workload hier_nuts`, the line the driver records):
+
 Workload: the non-centred hierarchical normal (J = 64 groups x 32 observations,
 sigma_j ~ U(0.5, 2)), batched_nuts with 1024 chains, max tree depth 10, binary32
 state with binary64 energies; W warmup + K sampling iterations.  One "step" is one
@@ -12,6 +14,12 @@ persistent NUTS kernel, FP32 VALU bound) from HIP-event device time; and the CPU
 baseline -- the oracle's binary64 NUTS on the host cores, timed on a bounded number
 of sampling iterations from the GPU run's adapted state, giving CPU s/iteration and
 a projected ESS/s (GPU ESS per iteration / CPU s per iteration).
+
+`--workload glm_chees`: the large logistic regression (N = 131072 rows, D = 256,
+X ~ N(0, 1/D) stored in bf16, 4096 chains) under batched_chees with the fused
+matrix-core gradient; roofline against the bf16 MFMA peak with the algorithmic
+4 N D flops per chain-gradient, CPU baseline = the binary64 oracle gradient on the
+host cores (projected: CPU s per batch gradient x mean leapfrog steps).
 """
 from __future__ import annotations
 
@@ -29,6 +37,7 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "min bulk-ESS/sec at R̂<1.01, plain vs flow-preconditioned HMC, 1 MI355X"
 PEAK_FP32_TFLOPS = 157.3
+PEAK_BF16_TFLOPS = 2500.0
 # Algorithmic work per NUTS leaf of the J=64 x 32 model: one chain-gradient
 # (6 N + 10 J + 20 = 12,948 flops) plus kick/drift/U-turn/energy (12 P = 792 flops).
 FLOPS_PER_GRADIENT = 6 * 2048 + 10 * 64 + 20
@@ -50,9 +59,12 @@ def cpu_cores() -> int:
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3000, help="sampling iterations")
-    ap.add_argument("--warmup", type=int, default=1000, help="warmup iterations")
-    ap.add_argument("--chains", type=int, default=1024)
+    ap.add_argument("--workload", choices=("hier_nuts", "glm_chees"), default="hier_nuts")
+    ap.add_argument("--steps", type=int, default=None, help="sampling iterations")
+    ap.add_argument("--warmup", type=int, default=None, help="warmup iterations")
+    ap.add_argument("--chains", type=int, default=None)
+    ap.add_argument("--forward-velocity", action="store_true",
+                    help="glm_chees: ChEES criterion with the forward end velocity (not the reference's sign)")
     ap.add_argument("--seed", type=int, default=20261203)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -73,6 +85,12 @@ def main() -> int:
         print(f"bench.py runs on exactly one GPU; --gpus {args.gpus} is not supported "
               "(chains shard across GPUs without communication; see the design notes)", file=sys.stderr)
         return 2
+
+    if args.workload == "glm_chees":
+        return bench_glm_chees(args)
+    args.steps = 3000 if args.steps is None else args.steps
+    args.warmup = 1000 if args.warmup is None else args.warmup
+    args.chains = 1024 if args.chains is None else args.chains
 
     import uncertaintea_amd as ut
     from uncertaintea_amd import diagnostics as D
@@ -145,6 +163,76 @@ def main() -> int:
         "mean_tree_depth": float(res.tree_depths.mean()), "divergence_rate": res.divergence_rate(),
         "roofline": roofline, "cpu_baseline": cpu,
         "vs_cpu": (value / cpu["value"]) if cpu else None,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def bench_glm_chees(args) -> int:
+    import uncertaintea_amd as ut
+    from uncertaintea_amd import diagnostics as D
+    from uncertaintea_amd import synthetic as S
+
+    steps = 500 if args.steps is None else args.steps
+    warmup = 500 if args.warmup is None else args.warmup
+    chains = 4096 if args.chains is None else args.chains
+    N, Dm = 131072, 256
+    x, y, _ = S.logistic_data(N, Dm, seed=args.seed, cfg=3, x_scale=1.0 / 16.0)
+    model = ut.LogisticRegressionBF16(x, y)
+    log(f"bench: batched_chees logistic N={N} D={Dm} bf16 chains={chains} warmup={warmup} steps={steps}")
+    res = ut.batched_chees(model, num_chains=chains, num_samples=steps, num_warmup=warmup, seed=args.seed,
+                           forward_velocity=args.forward_velocity)
+    log(f"bench: sampler done warmup {res.warmup_seconds:.3f}s sampling {res.sampling_seconds:.3f}s "
+        f"T {res.trajectory_trace[-1] if warmup else 0:.4f} eps {res.step_size[0]:.5f}")
+    rh, eb = D.rhat_and_ess_bulk(res.constrained_samples)
+    min_ess, rhat_max = float(np.nanmin(eb)), float(np.nanmax(rh))
+    value = min_ess / res.sampling_seconds
+    log(f"bench: rhat max {rhat_max:.4f} min bulk ESS {min_ess:.1f} -> {value:.1f} ESS/s")
+    evals = res.sampling_gradient_evals
+    flops = evals * 4.0 * N * Dm
+    ksec = res.sampling_kernel_seconds
+    achieved = flops / ksec / 1e12 if ksec > 0 else 0.0
+    mean_steps = float(res.integration_steps.mean())
+    roofline = {"bound": "mfma", "compute": "bf16 MFMA, fp32 accumulation (hi+lo operand pairs: 2x issued)",
+                "achieved": achieved, "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
+                "frac": achieved / PEAK_BF16_TFLOPS, "traffic": None, "kernel": "k_glm_grad",
+                "timing": "HIP events around all sampling launches (gradient + trajectory kernels)",
+                "flops_per_batch_gradient": 4.0 * N * Dm * chains}
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import binding as B
+        om = B.OracleModel(model)
+        # the oracle gradient runs an OpenMP loop over chains with the default team
+        cores = min(int(os.environ.get("OMP_NUM_THREADS") or cpu_cores()), os.cpu_count() or 1)
+        sub = 64
+        params = res.unconstrained_samples[:sub, -1, :].astype(np.float64).T.copy()
+        n, t_cpu = 0, 0.0
+        while t_cpu < args.cpu_seconds and n < 50:
+            ts = time.perf_counter()
+            om.logjoint_grad_f64(params)
+            t_cpu += time.perf_counter() - ts
+            n += 1
+        s_batch = t_cpu / n * chains / sub
+        s_iter = s_batch * mean_steps
+        cpu = {"value": (min_ess / steps) / s_iter, "unit": "ESS/s (projected)", "cores": cores, "kind": "port",
+               "sample": f"{n} binary64 oracle gradients of {sub} chains ({t_cpu:.1f} s), scaled to {chains} "
+                         f"chains x {mean_steps:.2f} leapfrog steps per iteration; projected = GPU bulk ESS per "
+                         f"iteration / CPU s per iteration",
+               "cpu_s_per_batch_gradient": s_batch}
+        log(f"bench: cpu {s_batch:.2f} s per batch gradient -> projected {cpu['value']:.3f} ESS/s")
+    line = {
+        "metric": METRIC, "value": value, "unit": "ESS/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
+        "ms_per_step": 1e3 * res.sampling_seconds / steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "bf16", "data": "synthetic (Philox4x32-10, seed %d)" % args.seed,
+        "config": {"workload": "logistic regression N=131072 D=256 bf16 X, batched_chees lock-step, "
+                               f"{chains} chains, fused MFMA gradient",
+                   "chains": chains, "forward_velocity": bool(args.forward_velocity)},
+        "rhat_max": rhat_max, "rhat_ok": rhat_max < 1.01, "min_bulk_ess": min_ess,
+        "sampling_seconds": res.sampling_seconds, "warmup_seconds": res.warmup_seconds,
+        "gradient_evals_per_s": evals / res.sampling_seconds, "mean_leapfrog_steps": mean_steps,
+        "trajectory_time": float(res.trajectory_trace[-1]) if warmup else None,
+        "step_size": float(res.step_size[0]), "divergence_rate": res.divergence_rate(),
+        "roofline": roofline, "cpu_baseline": cpu, "vs_cpu": (value / cpu["value"]) if cpu else None,
     }
     print(json.dumps(line), flush=True)
     return 0
