@@ -62,6 +62,7 @@ enum { UT_INIT_PRIOR = 0, UT_INIT_UNIFORM = 1 };
 
 typedef struct ut_plan ut_plan;
 typedef struct ut_workspace ut_workspace;
+typedef struct ut_flow ut_flow;
 
 /* Thread-local message for the last failing call on this thread. */
 const char* ut_last_error(void);
@@ -219,6 +220,58 @@ typedef struct ut_chees_opts {
 void ut_chees_opts_default(ut_chees_opts* opts);
 int ut_chees_run(ut_workspace* ws, const ut_chees_opts* opts, ut_chain_out* out,
                  double* trajectory_trace);
+
+/* Fixed-length HMC with SHARED adaptation, all chains in lock step: one step
+ * size dual-averaged to the mean acceptance (divergent chains count 0), pooled
+ * diagonal mass, shared reasonable-step search (reference batched_hmc with
+ * per_chain_adaptation = false, src/inference/api_batched_hmc.jl:184-330).  The
+ * gradient comes from the workspace's engine for all chains at once, so it runs
+ * the matrix-core GLM plan and flow-preconditioned workspaces.                  */
+int ut_hmc_shared_run(ut_workspace* ws, const ut_sampler_opts* opts, ut_chain_out* out);
+
+/* ---- flow preconditioner ----------------------------------------------------------
+ * An affine-coupling normalising flow q = f(z) (the reference's FlowGuide
+ * structure, src/inference/vi/guides.jl:1-21,282-369: elementwise affine base
+ * x0 = mu + exp(log_scale) z, then num_layers coupling layers with the alternating
+ * parity masks, active x <- x exp(s) + t), with MLP conditioners
+ * [s_raw, t] = W2 tanh(W1 x_passive + b1) + b2 of width `hidden` and
+ * s = c tanh(s_raw / c) (c = scale_clamp).  Not a reference sampler feature: the
+ * reference trains linear conditioners host-side for ADVI; here the network is
+ * trained on the GPU by maximum likelihood on draws (Adam) and then preconditions
+ * HMC (NeuTra): attached to a workspace, the lock-step samplers run in z space on
+ * log p(f(z)) + log|det df/dz| and return draws mapped back through f.
+ * Conditioner GEMMs run on the matrix cores with bf16 operands and fp32
+ * accumulation; weights are fp32.  Row counts of the host-array calls below must
+ * be multiples of 128.                                                           */
+int ut_flow_create(int32_t num_params, int32_t num_layers, int32_t hidden, double scale_clamp,
+                   uint64_t seed, ut_flow** out);
+int ut_flow_destroy(ut_flow* flow);
+/* flat fp32 weight vector: mu[P], log_scale[P], then per layer W1[hidden x kp],
+ * b1[hidden], W2[op x hidden], b2[op] with kp = np rounded up to 128 and
+ * op = 2 na rounded up to 128 (padding entries are zero); offsets per layer:
+ * {W1, b1, W2, b2}                                                               */
+int ut_flow_num_weights(const ut_flow* flow, int64_t* n);
+int ut_flow_layer_layout(const ut_flow* flow, int32_t layer, int32_t* num_passive, int32_t* num_active,
+                         int64_t* offsets);
+int ut_flow_get_weights(const ut_flow* flow, float* out);
+int ut_flow_set_weights(ut_flow* flow, const float* in);
+int ut_flow_set_base(ut_flow* flow, const double* mu, const double* log_scale);
+/* `steps` Adam steps (lr, beta 0.9 / 0.999) on minibatches of `batch` rows drawn
+ * with replacement (Philox, `seed`) from draws (n x P row-major host array);
+ * loss_trace[step] = minibatch mean negative log-likelihood, or NULL.            */
+int ut_flow_train(ut_flow* flow, const float* draws, int64_t n, int32_t steps, int32_t batch,
+                  double learning_rate, uint64_t seed, double* loss_trace);
+/* q = f(z) with log|det df/dz| per row; z = f^-1(q) with log|det df^-1/dq|      */
+int ut_flow_forward(ut_flow* flow, const float* z, int64_t n, float* q, float* logdet);
+int ut_flow_inverse(ut_flow* flow, const float* q, int64_t n, float* z, float* logdet_inv);
+/* g_z = (dq/dz)^T g_q + d log|det df/dz| / dz (the sampler's gradient path)       */
+int ut_flow_vjp(ut_flow* flow, const float* z, int64_t n, const float* g_q, float* g_z);
+/* mean over rows of 0.5|f^-1(q)|^2 - log|det df^-1/dq| + P/2 log 2 pi and its
+ * gradient over the flat weights (the training objective)                       */
+int ut_flow_loss_grad(ut_flow* flow, const float* q, int64_t n, double* loss, float* grad);
+/* attach (or detach with NULL) a flow: the lock-step samplers then precondition
+ * with it.  The workspace must be UT_FLOAT32 with num_chains a multiple of 128.   */
+int ut_workspace_set_flow(ut_workspace* ws, ut_flow* flow);
 
 /* ---- pinned host memory ----------------------------------------------------------
  * Page-locked host buffers for sampler outputs: the samplers copy each launch

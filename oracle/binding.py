@@ -41,7 +41,11 @@ def _load():
         "orc_nuts_sample_from": (i64, [vp, i32, i32, i32, vp, vp, vp, i32, i32, ct.c_uint64, i32]),
         "orc_dual_averaging": (None, [dbl, dbl, vp, i32, vp, vp]),
         "orc_running_variance": (dbl, [vp, vp, vp, i32, i32, i32, i32, i32, dbl, vp]),
-        "orc_chees_run": (ct.c_int, [vp, i32, vp, i32, i32, vp, vp, vp]),
+        "orc_chees_run": (ct.c_int, [vp, i32, vp, i32, i32, vp, vp, vp, i32]),
+        "orc_flow_forward": (None, [i32, i32, i32, dbl, i32, vp, vp, i64, vp, vp, vp]),
+        "orc_flow_inverse": (None, [i32, i32, i32, dbl, i32, vp, vp, i64, vp, vp]),
+        "orc_flow_loss_grad": (dbl, [i32, i32, i32, dbl, i32, vp, vp, i64, vp]),
+        "orc_flow_num_weights": (i64, [i32, i32, i32]),
         "orc_chees_halton": (None, [i32, vp]),
         "orc_chees_steps": (i32, [dbl, dbl, dbl]),
         "orc_chees_trajectory": (None, [dbl, dbl, dbl, dbl, dbl, dbl, i32, vp, vp, vp, vp, i32, vp]),
@@ -142,12 +146,13 @@ class OracleModel:
             initial_params=None, init_strategy: str = "prior", target_accept=None, seed: int = 0,
             adapt_step_size=True, adapt_mass_matrix=True, find_reasonable_step_size=False, threads: int = 0,
             stop_after: int = -1, divergence_threshold=1000.0, chees=None):
-        """Run the oracle sampler.  kind 'nuts'|'hmc'|'chees'; mode 'twin' (float32,
+        """Run the oracle sampler.  kind 'nuts'|'hmc'|'hmc_shared'|'chees'; mode 'twin' (float32,
         kernel-exact) or 'f64' (binary64 reference formulas).  `chees` holds the
         ChEES keyword arguments (reference names).  Returns a dict."""
         from uncertaintea_amd import _lib as UL  # struct layouts only
         if target_accept is None:
             target_accept = 0.651 if kind == "chees" else 0.8  # the reference's defaults
+        lockstep = kind in ("chees", "hmc_shared")
         o = UL.SamplerOpts()
         o.num_samples, o.num_warmup = num_samples, num_warmup
         o.max_tree_depth, o.num_leapfrog_steps = max_tree_depth, num_leapfrog_steps
@@ -183,7 +188,7 @@ class OracleModel:
         out.step_size = step.ctypes.data_as(ct.POINTER(ct.c_double))
         out.inverse_mass = im.ctypes.data_as(ct.POINTER(ct.c_double))
         trace = None
-        if kind == "chees":
+        if lockstep:
             co = UL.CheesOpts()
             co.base = o  # ChEES defaults restated from the reference's keyword defaults
             kw = dict(jitter_amount=1.0, adapt_trajectory_length=True, max_leapfrog_steps=1000,
@@ -194,7 +199,7 @@ class OracleModel:
                 setattr(co, k, int(v) if isinstance(getattr(co, k), int) else float(v))
             trace = np.zeros(num_warmup)
             rc = _load().orc_chees_run(self.h, 0 if mode == "twin" else 1, ct.byref(co), C, threads, _p(samples),
-                                       ct.byref(out), _p(trace))
+                                       ct.byref(out), _p(trace), int(kind == "hmc_shared"))
         else:
             rc = _load().orc_sampler_run(self.h, 0 if kind == "nuts" else 1, 0 if mode == "twin" else 1,
                                          ct.byref(o), C, threads, _p(samples), ct.byref(out), stop_after)
@@ -262,3 +267,48 @@ def chees_trajectory(T0, num, den, jitter, eps, lr=0.5, beta1=0.0, beta2=0.95, a
     _load().orc_chees_trajectory(T0, lr, beta1, beta2, adam_eps, decay, max_steps, _p(num), _p(den), _p(jitter),
                                  _p(eps), num.size, _p(out))
     return out
+
+
+class OracleFlow:
+    """Binary64 restatement of the device flow with the same flat weights.  With
+    bf16=True the matrix-core operands are rounded to bfloat16 as on the device."""
+
+    def __init__(self, P: int, L: int, H: int, clamp: float, weights, bf16: bool = False):
+        self.P, self.L, self.H, self.clamp, self.bf16 = int(P), int(L), int(H), float(clamp), bool(bf16)
+        self.w = np.ascontiguousarray(weights, dtype=np.float32)
+        n = int(_load().orc_flow_num_weights(self.P, self.L, self.H))
+        if self.w.shape != (n,):
+            raise ValueError(f"expected {n} weights, got {self.w.shape}")
+
+    @classmethod
+    def of(cls, flow, bf16: bool = False):
+        return cls(flow.num_params, flow.num_layers, flow.hidden, flow.scale_clamp, flow.weights, bf16)
+
+    @staticmethod
+    def num_weights(P, L, H):
+        return int(_load().orc_flow_num_weights(P, L, H))
+
+    def forward(self, z, g_q=None):
+        z = np.ascontiguousarray(np.asarray(z, dtype=np.float64).reshape(-1, self.P))
+        n = z.shape[0]
+        q = np.empty_like(z)
+        ld = np.empty(n)
+        g = None if g_q is None else np.ascontiguousarray(np.asarray(g_q, dtype=np.float64).reshape(-1, self.P)).copy()
+        _load().orc_flow_forward(self.P, self.L, self.H, self.clamp, int(self.bf16), _p(self.w), _p(z), n, _p(q),
+                                 _p(ld), _p(g))
+        return (q, ld) if g is None else (q, ld, g)
+
+    def inverse(self, q):
+        q = np.ascontiguousarray(np.asarray(q, dtype=np.float64).reshape(-1, self.P))
+        z = np.empty_like(q)
+        ld = np.empty(q.shape[0])
+        _load().orc_flow_inverse(self.P, self.L, self.H, self.clamp, int(self.bf16), _p(self.w), _p(q),
+                                 q.shape[0], _p(z), _p(ld))
+        return z, ld
+
+    def loss_grad(self, q):
+        q = np.ascontiguousarray(np.asarray(q, dtype=np.float64).reshape(-1, self.P))
+        g = np.empty(self.w.shape[0])
+        loss = _load().orc_flow_loss_grad(self.P, self.L, self.H, self.clamp, int(self.bf16), _p(self.w), _p(q),
+                                          q.shape[0], _p(g))
+        return float(loss), g

@@ -1095,9 +1095,12 @@ struct CheesState {
   }
 };
 
+// hmc = true: fixed-length HMC with shared adaptation (reference batched_hmc,
+// per_chain_adaptation = false, api_batched_hmc.jl:184-330): L = num_leapfrog_steps,
+// dual averaging to the mean acceptance, no trajectory adaptation.
 template <class Pol>
 int run_chees(const Pol& pol, const ut_chees_opts* co, int C, int threads, void* samples, ut_chain_out* out,
-              double* trace) {
+              double* trace, bool hmc) {
   using T = typename Pol::T;
   constexpr bool F32 = std::is_same<T, float>::value;
   const ut_sampler_opts* o = &co->base;
@@ -1316,14 +1319,14 @@ int run_chees(const Pol& pol, const ut_chees_opts* co, int C, int threads, void*
                 co->trajectory_learning_rate};
   cs.decay = co->trajectory_decay_rate;
   cs.max_steps = co->max_leapfrog_steps;
-  const bool adapt_traj = W > 0 && co->adapt_trajectory_length;
+  const bool adapt_traj = !hmc && W > 0 && co->adapt_trajectory_length;
   const Schedule sch = warmup_schedule(W);
   RunVar rv;
   size_t win = 0;
   rv.reset(P, sch.ends.empty() ? 20 : sch.ends[0] - sch.initial_buffer);
   for (int it = 1; it <= W + S; ++it) {
     const double jitter = chees_halton(it) * co->jitter_amount + (1.0 - co->jitter_amount);
-    const int L = chees_leapfrog_steps(jitter, cs.T, eps);
+    const int L = hmc ? o->num_leapfrog_steps : chees_leapfrog_steps(jitter, cs.T, eps);
     const uint32_t itc = rs + (uint32_t)it;
     const T h = F32 ? (T)(float)eps : (T)eps;
     const bool warm = it <= W;
@@ -1337,8 +1340,8 @@ int run_chees(const Pol& pol, const ut_chees_opts* co, int C, int threads, void*
     if (warm) {
       double num = 0.0, den = 0.0;
       if (adapt_traj) criterion(im_d, &num, &den);
-      const double hm = harmonic();
-      if (o->adapt_step_size) eps = dual_update(dual, hm, o->target_accept);
+      const double stat = hmc ? mean_accept() : harmonic();
+      if (o->adapt_step_size) eps = dual_update(dual, stat, o->target_accept);
       if (o->adapt_mass_matrix && win < sch.ends.size() && it > sch.initial_buffer) {
         std::vector<double> w(C), col(P);
         for (int c = 0; c < C; ++c) w[c] = mass_weight(rv, acc[c], ap[c], div[c]);
@@ -1361,7 +1364,7 @@ int run_chees(const Pol& pol, const ut_chees_opts* co, int C, int threads, void*
         }
       }
       if (adapt_traj) cs.update(num, den, jitter, eps);
-      if (trace) trace[it - 1] = cs.T;
+      if (trace && !hmc) trace[it - 1] = cs.T;
       if (it == W && o->adapt_step_size) eps = exp_d(dual.log_step_avg);
     } else {
       const int s = it - W - 1;
@@ -1393,6 +1396,202 @@ int run_chees(const Pol& pol, const ut_chees_opts* co, int C, int threads, void*
   }
   return UT_OK;
 }
+
+// ------------------------------------------------------------------ flow
+// Affine-coupling flow, binary64 restatement of the device flow (structure of the
+// reference FlowGuide, src/inference/vi/guides.jl:282-369, with MLP conditioners):
+// x0 = mu + e^ls z; layer l: passive i with (i + l) even; [s_raw, t] =
+// W2 tanh(W1 x_p + b1) + b2; s = c tanh(s_raw / c); x_a <- x_a e^s + t.  With
+// `bf16` the GEMM operands (passive inputs, weights, hidden activations and the
+// backward's output and hidden gradients) are rounded to bfloat16 as the device's
+// matrix-core operands are; accumulation stays binary64.
+struct FlowSpec {
+  int P, L, H;
+  double c;
+  bool bf16;
+  const float* w;  // flat weights, device layout
+  struct Layer {
+    std::vector<int> p, a;
+    int kp, op;
+    long long oW1, ob1, oW2, ob2;
+  };
+  std::vector<Layer> layers;
+  FlowSpec(int P_, int L_, int H_, double c_, bool bf, const float* w_) : P(P_), L(L_), H(H_), c(c_), bf16(bf), w(w_) {
+    long long off = 2LL * P;
+    for (int l = 0; l < L; ++l) {
+      Layer ly;
+      for (int i = 0; i < P; ++i) ((i + l) % 2 == 0 ? ly.p : ly.a).push_back(i);
+      ly.kp = ((int)ly.p.size() + 127) / 128 * 128;
+      ly.op = (2 * (int)ly.a.size() + 127) / 128 * 128;
+      ly.oW1 = off;
+      off += (long long)H * ly.kp;
+      ly.ob1 = off;
+      off += H;
+      ly.oW2 = off;
+      off += (long long)ly.op * H;
+      ly.ob2 = off;
+      off += ly.op;
+      layers.push_back(ly);
+    }
+  }
+  double r(double v) const {  // operand rounding
+    if (!bf16) return v;
+    const float f = (float)v;
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    u &= 0xffff0000u;
+    float o;
+    std::memcpy(&o, &u, 4);
+    return (double)o;
+  }
+  double W1(const Layer& ly, int j, int k) const { return r(w[ly.oW1 + (long long)j * ly.kp + k]); }
+  double W2(const Layer& ly, int o, int j) const { return r(w[ly.oW2 + (long long)o * H + j]); }
+  // conditioner: h (H), s, t (na) from the passive coordinates of x
+  void cond(const Layer& ly, const double* x, std::vector<double>& h, std::vector<double>& s,
+            std::vector<double>& t) const {
+    const int np = (int)ly.p.size(), na = (int)ly.a.size();
+    h.assign(H, 0.0);
+    for (int j = 0; j < H; ++j) {
+      double u = 0.0;
+      for (int k = 0; k < np; ++k) u += r(x[ly.p[k]]) * W1(ly, j, k);
+      h[j] = std::tanh(u + (double)w[ly.ob1 + j]);
+    }
+    s.assign(na, 0.0);
+    t.assign(na, 0.0);
+    for (int i = 0; i < na; ++i) {
+      double os = 0.0, ot = 0.0;
+      for (int j = 0; j < H; ++j) {
+        os += r(h[j]) * W2(ly, i, j);
+        ot += r(h[j]) * W2(ly, na + i, j);
+      }
+      s[i] = c * std::tanh((os + (double)w[ly.ob2 + i]) / c);
+      t[i] = ot + (double)w[ly.ob2 + na + i];
+    }
+  }
+  // backward through the conditioner: given dS (d/ds) and dT (d/dt), add d/dx_p into g;
+  // optionally accumulate weight gradients
+  void cond_bwd(const Layer& ly, const double* x, const std::vector<double>& h, const std::vector<double>& s,
+                const std::vector<double>& dS, const std::vector<double>& dT, double* g, double* gw) const {
+    const int np = (int)ly.p.size(), na = (int)ly.a.size();
+    std::vector<double> dO(2 * na), dU(H);
+    for (int i = 0; i < na; ++i) {
+      const double q = s[i] / c;
+      dO[i] = dS[i] * (1.0 - q * q);
+      dO[na + i] = dT[i];
+    }
+    for (int j = 0; j < H; ++j) {
+      double gh = 0.0;
+      for (int o = 0; o < 2 * na; ++o) gh += r(dO[o]) * W2(ly, o, j);
+      dU[j] = gh * (1.0 - h[j] * h[j]);
+    }
+    for (int k = 0; k < np; ++k) {
+      double gx = 0.0;
+      for (int j = 0; j < H; ++j) gx += r(dU[j]) * W1(ly, j, k);
+      g[ly.p[k]] += gx;
+    }
+    if (!gw) return;
+    for (int o = 0; o < 2 * na; ++o) {
+      for (int j = 0; j < H; ++j) gw[ly.oW2 + (long long)o * H + j] += r(dO[o]) * r(h[j]);
+      gw[ly.ob2 + o] += dO[o];
+    }
+    for (int j = 0; j < H; ++j) {
+      for (int k = 0; k < np; ++k) gw[ly.oW1 + (long long)j * ly.kp + k] += r(dU[j]) * r(x[ly.p[k]]);
+      gw[ly.ob1 + j] += dU[j];
+    }
+  }
+  // q = f(z); returns log|det df/dz|.  With g_q, also overwrites g_q by
+  // d/dz [g_q . f(z) + logdet].
+  double forward(const double* z, double* q, double* g_q) const {
+    std::vector<double> x(P);
+    double ld = 0.0;
+    for (int i = 0; i < P; ++i) {
+      x[i] = (double)w[i] + std::exp((double)w[P + i]) * z[i];
+      ld += (double)w[P + i];
+    }
+    std::vector<std::vector<double>> X(L), Hs(L), S(L), XS(L);
+    for (int l = 0; l < L; ++l) {
+      const Layer& ly = layers[l];
+      X[l] = x;
+      std::vector<double> t;
+      cond(ly, x.data(), Hs[l], S[l], t);
+      XS[l].resize(ly.a.size());
+      for (size_t i = 0; i < ly.a.size(); ++i) {
+        const double xs = x[ly.a[i]] * std::exp(S[l][i]);
+        XS[l][i] = xs;
+        x[ly.a[i]] = xs + t[i];
+        ld += S[l][i];
+      }
+    }
+    for (int i = 0; i < P; ++i) q[i] = x[i];
+    if (!g_q) return ld;
+    std::vector<double> g(g_q, g_q + P);
+    for (int l = L - 1; l >= 0; --l) {
+      const Layer& ly = layers[l];
+      const int na = (int)ly.a.size();
+      std::vector<double> dS(na), dT(na);
+      for (int i = 0; i < na; ++i) {
+        const double ga = g[ly.a[i]];
+        dS[i] = ga * XS[l][i] + 1.0;
+        dT[i] = ga;
+        g[ly.a[i]] = ga * std::exp(S[l][i]);
+      }
+      cond_bwd(ly, X[l].data(), Hs[l], S[l], dS, dT, g.data(), nullptr);
+    }
+    for (int i = 0; i < P; ++i) g_q[i] = g[i] * std::exp((double)w[P + i]);
+    return ld;
+  }
+  // z = f^-1(q); returns log|det df^-1/dq|.  With gw, accumulates the gradient of
+  // 0.5|z|^2 - logdet_inv over the weights and returns that loss in *loss.
+  double inverse(const double* q, double* z, double* gw, double* loss) const {
+    std::vector<double> x(q, q + P);
+    double ld = 0.0;
+    std::vector<std::vector<double>> X(L), Hs(L), S(L), XO(L);
+    for (int l = L - 1; l >= 0; --l) {
+      const Layer& ly = layers[l];
+      std::vector<double> t;
+      X[l] = x;  // conditioner input (passive coordinates are unchanged by the layer)
+      cond(ly, x.data(), Hs[l], S[l], t);
+      XO[l].resize(ly.a.size());
+      for (size_t i = 0; i < ly.a.size(); ++i) {
+        const double xo = (x[ly.a[i]] - t[i]) * std::exp(-S[l][i]);
+        x[ly.a[i]] = xo;
+        XO[l][i] = xo;
+        ld -= S[l][i];
+      }
+    }
+    double zz = 0.0;
+    for (int i = 0; i < P; ++i) {
+      z[i] = (x[i] - (double)w[i]) * std::exp(-(double)w[P + i]);
+      ld -= (double)w[P + i];
+      zz += z[i] * z[i];
+    }
+    if (loss) *loss = 0.5 * zz - ld + 0.5 * P * 1.8378770664093453;
+    if (!gw) return ld;
+    std::vector<double> g(P);
+    for (int i = 0; i < P; ++i) {
+      const double e = std::exp(-(double)w[P + i]);
+      g[i] = z[i] * e;
+      gw[i] += -z[i] * e;
+      gw[P + i] += 1.0 - z[i] * z[i];
+    }
+    for (int l = 0; l < L; ++l) {
+      const Layer& ly = layers[l];
+      const int na = (int)ly.a.size();
+      std::vector<double> dS(na), dT(na);
+      for (int i = 0; i < na; ++i) {
+        const double go = g[ly.a[i]];
+        const double e = std::exp(-S[l][i]);
+        dT[i] = -go * e;
+        dS[i] = -go * XO[l][i] + 1.0;
+        g[ly.a[i]] = go * e;
+      }
+      cond_bwd(ly, X[l].data(), Hs[l], S[l], dS, dT, g.data(), gw);
+    }
+    return ld;
+  }
+  long long nparams() const { return layers.back().ob2 + layers.back().op; }
+};
 
 void finish_twin_layout(Model& m) {
   m.K = m.kind == KIND_HIER ? 2 : slots_for(m.P);
@@ -1572,10 +1771,10 @@ int orc_sampler_run(void* h, int32_t kind, int32_t mode, const ut_sampler_opts* 
 
 // ChEES run.  mode 0: TwinF32 (kernel-exact, samples float); mode 1: RefF64.
 int orc_chees_run(void* h, int32_t mode, const ut_chees_opts* co, int32_t C, int32_t threads, void* samples,
-                  ut_chain_out* out, double* trace) {
+                  ut_chain_out* out, double* trace, int32_t hmc) {
   const Model* m = (const Model*)h;
-  if (mode == 0) return run_chees(TwinF32{m}, co, C, threads, samples, out, trace);
-  return run_chees(RefF64{m}, co, C, threads, samples, out, trace);
+  if (mode == 0) return run_chees(TwinF32{m}, co, C, threads, samples, out, trace, hmc != 0);
+  return run_chees(RefF64{m}, co, C, threads, samples, out, trace, hmc != 0);
 }
 
 // ChEES scalar pieces for known-answer tests: Halton values, step counts and a
@@ -1594,6 +1793,52 @@ void orc_chees_trajectory(double T0, double lr, double b1, double b2, double aep
     cs.update(num[i], den[i], jitter[i], eps[i]);
     T_out[i] = cs.T;
   }
+}
+
+
+// ---- flow (binary64; bf16 = 1 rounds the matrix-core operands as the device does)
+void orc_flow_forward(int32_t P, int32_t L, int32_t H, double clamp, int32_t bf16, const float* weights,
+                      const double* z, int64_t n, double* q, double* logdet, double* g) {
+  FlowSpec fs(P, L, H, clamp, bf16 != 0, weights);
+#pragma omp parallel for schedule(static)
+  for (int64_t r = 0; r < n; ++r)
+    logdet[r] = fs.forward(z + r * P, q + r * P, g ? g + r * P : nullptr);
+}
+void orc_flow_inverse(int32_t P, int32_t L, int32_t H, double clamp, int32_t bf16, const float* weights,
+                      const double* q, int64_t n, double* z, double* logdet_inv) {
+  FlowSpec fs(P, L, H, clamp, bf16 != 0, weights);
+#pragma omp parallel for schedule(static)
+  for (int64_t r = 0; r < n; ++r) logdet_inv[r] = fs.inverse(q + r * P, z + r * P, nullptr, nullptr);
+}
+// mean loss and mean weight gradient over n rows
+double orc_flow_loss_grad(int32_t P, int32_t L, int32_t H, double clamp, int32_t bf16, const float* weights,
+                          const double* q, int64_t n, double* grad) {
+  FlowSpec fs(P, L, H, clamp, bf16 != 0, weights);
+  const long long np = fs.nparams();
+  std::vector<double> total((size_t)np, 0.0);
+  double loss_total = 0.0;
+#pragma omp parallel
+  {
+    std::vector<double> gw((size_t)np, 0.0), z(P);
+    double lt = 0.0;
+#pragma omp for schedule(static)
+    for (int64_t r = 0; r < n; ++r) {
+      double loss;
+      fs.inverse(q + r * P, z.data(), gw.data(), &loss);
+      lt += loss;
+    }
+#pragma omp critical
+    {
+      for (long long i = 0; i < np; ++i) total[i] += gw[i];
+      loss_total += lt;
+    }
+  }
+  for (long long i = 0; i < np; ++i) grad[i] = total[i] / (double)n;
+  return loss_total / (double)n;
+}
+int64_t orc_flow_num_weights(int32_t P, int32_t L, int32_t H) {
+  FlowSpec fs(P, L, H, 1.0, false, nullptr);
+  return fs.nparams();
 }
 
 int64_t orc_nuts_sample_from(void* h, int32_t mode, int32_t C, int32_t threads, double* positions,

@@ -1,0 +1,165 @@
+"""GPU: the affine-coupling flow preconditioner and the shared-adaptation HMC.
+
+Tolerance: device flow forward / inverse / vector-Jacobian product / training
+gradient against the oracle's binary64 flow evaluated with the same bf16-rounded
+matrix-core operands.  The device's fp32 intermediates differ from binary64 by
+~1e-7, which now and then moves an operand across a bf16 rounding boundary; such
+flips change that operand by 2^-9 relative and propagate through later layers.
+So positions and log-determinants are bounded at the 99th percentile
+(|d| <= 1e-5 (1 + |ref|), the arithmetic) and at the maximum (2e-3, the flips);
+gradients normwise (|d|_2 <= 2e-3 |ref|_2).  Errors against the unrounded
+binary64 flow are printed.  Bit-exact: shared-adaptation lock-step HMC against
+the oracle's binary32 twin.  Statistical: a flow trained on exact funnel draws maps
+held-out draws to a near-standard normal, and flow-preconditioned HMC on the
+funnel matches the exact moments of v where plain HMC on the same budget mixes
+far worse."""
+import sys
+
+import numpy as np
+import pytest
+
+import uncertaintea_amd as ut
+from oracle import binding as B
+from uncertaintea_amd import diagnostics as D
+from uncertaintea_amd import models as M
+from uncertaintea_amd import synthetic as S
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _random_weights(flow, seed, scale=0.5):
+    """Non-trivial weights in the flow's flat layout (padding entries stay zero)."""
+    rng = np.random.default_rng(seed)
+    w = flow.weights.copy()
+    P, H = flow.num_params, flow.hidden
+    w[:P] = rng.normal(size=P) * 0.5
+    w[P:2 * P] = rng.normal(size=P) * 0.3
+    for npass, nact, off in flow.layout():
+        kp = -(-npass // 128) * 128
+        op = -(-2 * nact // 128) * 128
+        W2 = np.zeros((op, H), np.float32)
+        W2[:2 * nact] = rng.normal(size=(2 * nact, H)) * scale / np.sqrt(H)
+        w[off["W2"]:off["W2"] + op * H] = W2.ravel()
+        b2 = np.zeros(op, np.float32)
+        b2[:2 * nact] = rng.normal(size=2 * nact) * 0.1
+        w[off["b2"]:off["b2"] + op] = b2
+        w[off["b1"]:off["b1"] + H] = rng.normal(size=H) * 0.1
+    return w
+
+
+def _funnel_draws(n, dim, seed=1):
+    rng = np.random.default_rng(seed)
+    v = rng.normal(size=n) * 3.0
+    x = rng.normal(size=(n, dim)) * np.exp(v / 2.0)[:, None]
+    return np.concatenate([v[:, None], x], axis=1).astype(np.float32)
+
+
+def test_flow_kernels_against_oracle():
+    say("flow kernels vs oracle")
+    P, Lyr, H = 101, 6, 256
+    flow = ut.AffineCouplingFlow(P, Lyr, H, scale_clamp=5.0, seed=3)
+    flow.weights = _random_weights(flow, 4, scale=0.2)
+    o16 = B.OracleFlow.of(flow, bf16=True)
+    o64 = B.OracleFlow.of(flow, bf16=False)
+    rng = np.random.default_rng(5)
+    z = rng.normal(size=(256, P)).astype(np.float32)
+    q, ld = flow.forward(z)
+    rq, rld = o16.forward(z)
+    fq, fld = o64.forward(z)
+    d_q = np.abs(q - rq) / (1 + np.abs(rq))
+    d_ld = np.abs(ld - rld) / (1 + np.abs(rld))
+    say(f"  forward: vs bf16-operand oracle q p99 {np.quantile(d_q, 0.99):.2e} max {d_q.max():.2e}, logdet p99 "
+        f"{np.quantile(d_ld, 0.99):.2e} max {d_ld.max():.2e}; vs binary64 q max "
+        f"{np.max(np.abs(q - fq) / (1 + np.abs(fq))):.2e} logdet max {np.max(np.abs(ld - fld)):.2e}")
+    assert np.median(d_q) <= 1e-6 and np.quantile(d_q, 0.99) <= 1e-5 and d_q.max() <= 2e-3
+    assert np.quantile(d_ld, 0.99) <= 1e-5 and d_ld.max() <= 2e-3
+    zi, ldi = flow.inverse(q)
+    rz, rldi = o16.inverse(q)
+    d_z = np.abs(zi - rz) / (1 + np.abs(rz))
+    d_ldi = np.abs(ldi - rldi) / (1 + np.abs(rldi))
+    say(f"  inverse: z p99 {np.quantile(d_z, 0.99):.2e} max {d_z.max():.2e}, logdet max {d_ldi.max():.2e}; "
+        f"round trip |z - f^-1(f(z))| max {np.max(np.abs(zi - z)):.2e}")
+    assert np.quantile(d_z, 0.99) <= 1e-5 and d_z.max() <= 2e-3 and d_ldi.max() <= 2e-3
+    gq = rng.normal(size=(256, P)).astype(np.float32)
+    gz = flow.vjp(z, gq)
+    _, _, rgz = o16.forward(z, gq)
+    e_g = np.linalg.norm(gz - rgz) / np.linalg.norm(rgz)
+    _, _, fgz = o64.forward(z, gq)
+    say(f"  vjp: vs bf16-operand oracle {e_g:.2e}; vs binary64 {np.linalg.norm(gz - fgz) / np.linalg.norm(fgz):.2e}")
+    assert e_g <= 2e-3
+    qd = (rng.normal(size=(256, P)) * 1.3).astype(np.float32)
+    loss, g = flow.loss_grad(qd)
+    rloss, rg = o16.loss_grad(qd)
+    floss, fg = o64.loss_grad(qd)
+    e_w = np.linalg.norm(g - rg) / np.linalg.norm(rg)
+    say(f"  loss {loss:.6f} vs {rloss:.6f} (binary64 {floss:.6f}); weight gradient vs bf16-operand oracle {e_w:.2e}, "
+        f"vs binary64 {np.linalg.norm(g - fg) / np.linalg.norm(fg):.2e}")
+    assert abs(loss - rloss) <= 1e-4 * abs(rloss)
+    assert e_w <= 2e-3
+
+
+def test_flow_training_learns_funnel():
+    say("flow training on exact funnel draws")
+    dim = 10
+    draws = _funnel_draws(65536, dim)
+    flow = ut.AffineCouplingFlow(dim + 1, 6, 256, seed=7)
+    trace = flow.fit(draws, steps=1500, batch=1024, learning_rate=3e-4, seed=8)
+    entropy = 0.5 * np.log(2 * np.pi * np.e * 9.0) + dim * 0.5 * np.log(2 * np.pi * np.e)
+    first, last = trace[:20].mean(), trace[-100:].mean()
+    say(f"  loss {first:.3f} -> {last:.3f} (true entropy {entropy:.3f})")
+    assert np.all(np.isfinite(trace))
+    assert last < entropy + 0.5 * (first - entropy)
+    # maximum likelihood makes the inverse map of held-out exact draws standard normal
+    zi, _ = flow.inverse(_funnel_draws(16384, dim, seed=2))
+    m, sd = zi.mean(axis=0), zi.std(axis=0)
+    z = np.random.default_rng(9).normal(size=(16384, dim + 1)).astype(np.float32)
+    q, _ = flow.forward(z)
+    say(f"  inverse of held-out draws: |mean| max {np.abs(m).max():.3f}, sd range [{sd.min():.3f}, {sd.max():.3f}]; "
+        f"pushed-forward v: mean {q[:, 0].mean():.3f} sd {q[:, 0].std():.3f} (exact 0, 3)")
+    assert np.abs(m).max() < 1.0 and sd.min() > 0.5 and sd.max() < 2.0
+
+
+def test_hmc_shared_bitexact_against_twin():
+    say("shared-adaptation HMC bitexact")
+    x, y, _ = S.logistic_data(1000, 25)
+    model = M.LogisticRegression(x, y)
+    dev = ut.batched_hmc(model, num_chains=64, num_samples=30, num_warmup=80, num_leapfrog_steps=8, step_size=0.05,
+                         per_chain_adaptation=False, seed=13)
+    orc = B.OracleModel(model).run("hmc_shared", "twin", 64, num_samples=30, num_warmup=80, num_leapfrog_steps=8,
+                                   step_size=0.05, seed=13, threads=8)
+    assert np.array_equal(dev.unconstrained_samples.view(np.uint32), orc["samples"].view(np.uint32))
+    for k_dev, k_orc in (("step_size", "step_size"), ("mass_matrix", "inverse_mass"), ("accepted", "accepted"),
+                         ("divergent", "divergent"), ("logjoint_values", "logjoint"),
+                         ("acceptance_stats", "accept_stat"), ("integration_steps", "integration_steps")):
+        assert np.array_equal(getattr(dev, k_dev), orc[k_orc]), k_dev
+
+
+def test_flow_hmc_funnel_gains_over_plain():
+    """NeuTra on Neal's funnel (dim 10): the same shared-adaptation HMC budget with
+    and without a flow trained on exact funnel draws."""
+    say("flow hmc on the funnel")
+    dim, C, W, Sn = 10, 512, 300, 400
+    model = M.NealsFunnel(dim, 3.0)
+    flow = ut.AffineCouplingFlow(dim + 1, 6, 256, seed=17)
+    flow.fit(_funnel_draws(65536, dim, seed=18), steps=1500, batch=1024, learning_rate=3e-4, seed=19)
+    res = {}
+    for name, fl in (("plain", None), ("flow", flow)):
+        r = ut.batched_hmc(model, num_chains=C, num_samples=Sn, num_warmup=W, num_leapfrog_steps=10,
+                           per_chain_adaptation=False, flow=fl, seed=21)
+        v = r.unconstrained_samples[:, :, 0].astype(np.float64)
+        ess_v = float(np.asarray(D.ess_bulk(r.unconstrained_samples[:, :, :1]))[0])
+        res[name] = (v.mean(), v.std(), ess_v, float(np.max(np.asarray(D.rhat(r.unconstrained_samples)))),
+                     r.divergence_rate(), float(r.step_size[0]))
+        say(f"  {name}: v mean {res[name][0]:.3f} sd {res[name][1]:.3f} ess(v) {ess_v:.0f} rhat {res[name][3]:.3f} "
+            f"div {res[name][4]:.4f} eps {res[name][5]:.4f}")
+    m, sd, ess, rh, div, _ = res["flow"]
+    assert abs(m) < 4 * 3.0 / np.sqrt(ess)
+    assert abs(sd - 3.0) < 0.25
+    # what the network gains: more effective draws of v, fewer divergences, better R-hat
+    assert ess > 2.0 * res["plain"][2]
+    assert div < res["plain"][4] and rh < res["plain"][3]

@@ -32,6 +32,9 @@ EXPORTED = (
     "ut_workspace_create", "ut_workspace_destroy",
     "ut_logjoint", "ut_logjoint_gradient",
     "ut_sampler_opts_default", "ut_nuts_run", "ut_hmc_run", "ut_chees_opts_default", "ut_chees_run",
+    "ut_hmc_shared_run", "ut_flow_create", "ut_flow_destroy", "ut_flow_num_weights", "ut_flow_layer_layout",
+    "ut_flow_get_weights", "ut_flow_set_weights", "ut_flow_set_base", "ut_flow_train", "ut_flow_forward",
+    "ut_flow_inverse", "ut_flow_vjp", "ut_flow_loss_grad", "ut_workspace_set_flow",
     "ut_philox4x32", "ut_philox_fill", "ut_host_alloc", "ut_host_free", "ut_gemm_bf16",
 )
 
@@ -135,6 +138,20 @@ def lib() -> ct.CDLL:
         "ut_hmc_run": (ct.c_int, [vp, ct.POINTER(SamplerOpts), ct.POINTER(ChainOut)]),
         "ut_chees_opts_default": (None, [ct.POINTER(CheesOpts)]),
         "ut_chees_run": (ct.c_int, [vp, ct.POINTER(CheesOpts), ct.POINTER(ChainOut), vp]),
+        "ut_hmc_shared_run": (ct.c_int, [vp, ct.POINTER(SamplerOpts), ct.POINTER(ChainOut)]),
+        "ut_flow_create": (ct.c_int, [i32, i32, i32, dbl, ct.c_uint64, pp]),
+        "ut_flow_destroy": (ct.c_int, [vp]),
+        "ut_flow_num_weights": (ct.c_int, [vp, ct.POINTER(i64)]),
+        "ut_flow_layer_layout": (ct.c_int, [vp, i32, ct.POINTER(i32), ct.POINTER(i32), vp]),
+        "ut_flow_get_weights": (ct.c_int, [vp, vp]),
+        "ut_flow_set_weights": (ct.c_int, [vp, vp]),
+        "ut_flow_set_base": (ct.c_int, [vp, vp, vp]),
+        "ut_flow_train": (ct.c_int, [vp, vp, i64, i32, i32, dbl, ct.c_uint64, vp]),
+        "ut_flow_forward": (ct.c_int, [vp, vp, i64, vp, vp]),
+        "ut_flow_inverse": (ct.c_int, [vp, vp, i64, vp, vp]),
+        "ut_flow_vjp": (ct.c_int, [vp, vp, i64, vp, vp]),
+        "ut_flow_loss_grad": (ct.c_int, [vp, vp, i64, ct.POINTER(dbl), vp]),
+        "ut_workspace_set_flow": (ct.c_int, [vp, vp]),
         "ut_philox4x32": (None, [vp, vp, vp]),
         "ut_philox_fill": (ct.c_int, [u32, u32, u32, u32, i64, vp, vp, vp, vp]),
         "ut_host_alloc": (ct.c_int, [i64, pp]),

@@ -18,6 +18,7 @@
 
 #include "../../include/uncertaintea_hip.h"
 #include "ut_chees.h"
+#include "ut_flow.h"
 #include "ut_gemm.h"
 #include "ut_glm.h"
 #include "ut_kernels.h"
@@ -97,8 +98,16 @@ struct ut_plan {
   }
 };
 
+// A trained (or training) affine-coupling flow; attached to a workspace it turns
+// the lock-step samplers into samplers of the flow's base space.
+struct ut_flow {
+  Flow f;
+  ~ut_flow() { flow_free(f); }
+};
+
 struct ut_workspace {
   const ut_plan* plan = nullptr;
+  ut_flow* flow = nullptr;  // not owned
   int C = 0;
   int P = 0;
   int precision = UT_FLOAT32;
@@ -376,10 +385,24 @@ int ut_workspace_destroy(ut_workspace* ws) {
 
 // Logjoint (and gradient) of all chains at fp32 positions q (C x P, device), with
 // the plan's gradient engine: the matrix-core GLM, or one wavefront per chain.
-static hipError_t batched_grad(ut_workspace* ws, const float* q, float* lj, float* grad, bool need_lj) {
+static hipError_t model_grad(ut_workspace* ws, const float* q, float* lj, float* grad, bool need_lj) {
   if (ws->plan->dev.kind == MODEL_GLM_MFMA)
     return launch_glm_grad(ws->plan->glm, q, grad, lj, ws->C, need_lj, ws->glm, ws->stream);
   return launch_density_f32(ws->plan->dev, q, lj, grad, ws->C, ws->stream);
+}
+
+// The samplers' gradient engine.  With a flow attached, positions are base-space
+// z: q = f(z), the model's logjoint and gradient at q, then
+//   log p_z(z) = log p(q) + log |det df/dz|,  grad_z = (dq/dz)^T grad_q + grad_z logdet
+// (the NeuTra reparameterisation).
+static hipError_t batched_grad(ut_workspace* ws, const float* q, float* lj, float* grad, bool need_lj) {
+  if (!ws->flow) return model_grad(ws, q, lj, grad, need_lj);
+  Flow& f = ws->flow->f;
+  hipError_t e = flow_forward(f, q, nullptr, nullptr, ws->C, ws->stream);
+  if (e == hipSuccess) e = model_grad(ws, f.sc.x, lj, f.sc.g, need_lj);
+  if (e == hipSuccess) e = flow_backward(f, f.sc.g, grad, ws->C, ws->stream);
+  if (e == hipSuccess && need_lj) e = flow_add_logdet(lj, f.sc.logdet, ws->C, ws->stream);
+  return e;
 }
 
 static int density_call(ut_workspace* ws, const void* params, void* values, void* gradients) {
@@ -389,7 +412,7 @@ static int density_call(ut_workspace* ws, const void* params, void* values, void
   UT_HIP(hipMemcpyAsync(ws->d_params, params, PC * es, hipMemcpyHostToDevice, ws->stream));
   hipError_t e;
   if (ws->plan->dev.kind == MODEL_GLM_MFMA)
-    e = batched_grad(ws, (const float*)ws->d_params, (float*)ws->d_values, (float*)ws->d_grads, true);
+    e = model_grad(ws, (const float*)ws->d_params, (float*)ws->d_values, (float*)ws->d_grads, true);
   else if (ws->precision == UT_FLOAT32)
     e = launch_density_f32(ws->plan->dev, (const float*)ws->d_params, (float*)ws->d_values,
                            gradients ? (float*)ws->d_grads : nullptr, ws->C, ws->stream);
@@ -604,6 +627,8 @@ int init_chains(ut_workspace* ws, const ut_sampler_opts* o, uint32_t rs) {
   for (int attempt = 0;; ++attempt) {
     for (size_t i = 0; i < pos.size(); ++i) pos_f[i] = (float)pos[i];
     UT_HIP(hipMemcpyAsync(ws->st.pos, pos_f.data(), pos_f.size() * 4, hipMemcpyHostToDevice, stream));
+    // with a flow the chains start at the base-space image of the drawn positions
+    if (ws->flow) UT_HIP(flow_inverse(ws->flow->f, ws->st.pos, ws->st.pos, nullptr, C, stream));
     UT_HIP(batched_grad(ws, ws->st.pos, ws->st.lj, ws->st.grad, true));
     UT_HIP(hipMemcpyAsync(lj_h.data(), ws->st.lj, (size_t)C * 4, hipMemcpyDeviceToHost, stream));
     UT_HIP(hipMemcpyAsync(grad_h.data(), ws->st.grad, grad_h.size() * 4, hipMemcpyDeviceToHost, stream));
@@ -633,6 +658,9 @@ int run_sampler(ut_workspace* ws, const ut_sampler_opts* o, ut_chain_out* out, b
   if (ws->plan->dev.kind == MODEL_GLM_MFMA)
     return fail(UT_E_UNSUPPORTED,
                 "the matrix-core GLM plan evaluates all chains at once; sample it with the lock-step samplers (ut_chees_run)");
+  if (ws->flow)
+    return fail(UT_E_UNSUPPORTED,
+                "a flow preconditions the lock-step samplers (ut_chees_run, ut_hmc_shared_run); detach it first");
   const double t_start = now_seconds();
   const int P = ws->P, C = ws->C, S = o->num_samples, W = o->num_warmup;
   const DevModelDesc& m = ws->plan->dev;
@@ -824,17 +852,23 @@ int validate_chees(const ut_workspace* ws, const ut_chees_opts* co, const ut_cha
   return UT_OK;
 }
 
-// Lock-step ChEES-HMC (reference batched_chees host loop,
-// api_batched_chees.jl:435-700, and its shared WarmupDriver, warmup.jl:262-327).
-// Every chain integrates the same jittered number of leapfrog steps with the shared
-// step size; the gradient between the trajectory kernels comes from the plan's
-// engine for all chains at once.  Warmup reads four binary64 scalars back per
-// iteration (the ChEES numerator and denominator, the harmonic-mean and the mean
-// acceptance) and runs the scalar state machines on the host; sampling launches
-// without any host synchronisation.
-int run_chees(ut_workspace* ws, const ut_chees_opts* co, ut_chain_out* out, double* trace) {
+// Lock-step samplers: ChEES-HMC (reference batched_chees host loop,
+// api_batched_chees.jl:435-700) and, with hmc = true, fixed-length HMC with shared
+// adaptation (reference batched_hmc with per_chain_adaptation = false,
+// api_batched_hmc.jl:184-330: L = num_leapfrog_steps, dual averaging to the mean
+// acceptance with divergent chains counted as 0), both on the shared WarmupDriver
+// (warmup.jl:262-327).  Every chain integrates the same number of leapfrog steps
+// with the shared step size; the gradient between the trajectory kernels comes
+// from the workspace's engine for all chains at once (through the flow when one is
+// attached).  Warmup reads four binary64 scalars back per iteration (the ChEES
+// numerator and denominator, the harmonic-mean and the mean acceptance) and runs
+// the scalar state machines on the host; sampling launches without any host
+// synchronisation.
+int run_lockstep(ut_workspace* ws, const ut_chees_opts* co, ut_chain_out* out, double* trace, bool hmc) {
   int rc = validate_chees(ws, co, out);
   if (rc) return rc;
+  if (ws->flow && (ws->C % FLOW_PAD != 0))
+    return fail(UT_E_DIM, "a flow-preconditioned workspace needs num_chains a multiple of 128");
   const ut_sampler_opts* o = &co->base;
   const double t_start = now_seconds();
   const int P = ws->P, C = ws->C, S = o->num_samples, W = o->num_warmup;
@@ -944,7 +978,7 @@ int run_chees(ut_workspace* ws, const ut_chees_opts* co, ut_chain_out* out, doub
                        1, co->trajectory_decay_rate, co->max_leapfrog_steps};
   const Schedule sch = warmup_schedule(W);
   int window = 0;
-  const bool adapt_traj = W > 0 && co->adapt_trajectory_length;
+  const bool adapt_traj = !hmc && W > 0 && co->adapt_trajectory_length;
 
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   UT_HIP(hipEventCreate(&ev0));
@@ -959,7 +993,7 @@ int run_chees(ut_workspace* ws, const ut_chees_opts* co, ut_chain_out* out, doub
       UT_HIP(hipEventRecord(ev0, stream));
     }
     const double jitter = chees_jitter(co->jitter_amount, it);
-    const int L = chees_steps(jitter, traj.T, eps);
+    const int L = hmc ? o->num_leapfrog_steps : chees_steps(jitter, traj.T, eps);
     const uint32_t itc = rs + (uint32_t)it;
     const bool warm = it <= W;
     const float eps_f = (float)eps;
@@ -979,8 +1013,8 @@ int run_chees(ut_workspace* ws, const ut_chees_opts* co, ut_chain_out* out, doub
     UT_HIP(launch_chees_stats(ls, rec, ws->d_im, P, C, adapt_traj, co->forward_velocity ? -1.0 : 1.0, d_stats,
                               stream));
     UT_HIP(read_stats());
-    const double num = hs.p[0], den = hs.p[1], harmonic = hs.p[2];
-    if (o->adapt_step_size) eps = dual.update(harmonic);
+    const double num = hs.p[0], den = hs.p[1], harmonic = hs.p[2], mean_accept = hs.p[3];
+    if (o->adapt_step_size) eps = dual.update(hmc ? mean_accept : harmonic);
     if (o->adapt_mass_matrix && window < (int)sch.ends.size() && sch.ends[window] == it) {
       const int start = window == 0 ? sch.initial_buffer + 1 : sch.ends[window - 1] + 1;
       UT_HIP(launch_mass_replay(sp.hist_pos, sp.hist_acc, sp.hist_flags, C, P, start, it, it - start + 1, 1,
@@ -993,7 +1027,7 @@ int run_chees(ut_workspace* ws, const ut_chees_opts* co, ut_chain_out* out, doub
       }
     }
     if (adapt_traj) traj.apply(num, den, jitter, eps);
-    if (trace) trace[it - 1] = traj.T;
+    if (trace && !hmc) trace[it - 1] = traj.T;
     if (it == W && o->adapt_step_size) eps = dual.final_step();
   }
   if (S == 0) {
@@ -1002,11 +1036,21 @@ int run_chees(ut_workspace* ws, const ut_chees_opts* co, ut_chain_out* out, doub
     UT_HIP(hipEventRecord(ev0, stream));
   }
   UT_HIP(hipEventRecord(ev1, stream));
+  const size_t SC = (size_t)S * C;
+  if (ws->flow) {
+    // draws back to the model's space, C rows at a time (rows are independent);
+    // logjoint values lose the log-determinant
+    Flow& f = ws->flow->f;
+    for (size_t r = 0; r < SC; r += (size_t)C) {
+      float* rows = sp.draws + r * P;
+      UT_HIP(flow_forward(f, rows, rows, nullptr, C, stream));
+      UT_HIP(flow_sub_logdet(sp.o_lj + r, f.sc.logdet, C, stream));
+    }
+  }
   auto copy = [&](void* dst, const void* src, size_t bytes) -> hipError_t {
     if (!dst || bytes == 0) return hipSuccess;
     return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream);
   };
-  const size_t SC = (size_t)S * C;
   UT_HIP(copy(out->samples, sp.draws, SC * P * 4));
   UT_HIP(copy(out->logjoint, sp.o_lj, SC * 8));
   UT_HIP(copy(out->accept_stat, sp.o_acc, SC * 8));
@@ -1054,7 +1098,17 @@ void ut_chees_opts_default(ut_chees_opts* co) {
 }
 
 int ut_chees_run(ut_workspace* ws, const ut_chees_opts* opts, ut_chain_out* out, double* trajectory_trace) {
-  return run_chees(ws, opts, out, trajectory_trace);
+  return run_lockstep(ws, opts, out, trajectory_trace, false);
+}
+
+int ut_hmc_shared_run(ut_workspace* ws, const ut_sampler_opts* opts, ut_chain_out* out) {
+  if (!opts) return fail(UT_E_ARG, "batched_hmc: null options");
+  ut_chees_opts co;
+  ut_chees_opts_default(&co);
+  co.base = *opts;
+  co.adapt_trajectory_length = 0;
+  co.jitter_amount = 0.0;
+  return run_lockstep(ws, &co, out, nullptr, true);
 }
 
 int ut_nuts_run(ut_workspace* ws, const ut_sampler_opts* opts, ut_chain_out* out) {
@@ -1092,6 +1146,248 @@ int ut_gemm_bf16(const uint16_t* a, const uint16_t* bt, float* c, int32_t m, int
     for (int z = 0; z < splits; ++z) s += part[(size_t)z * mn + i];
     c[i] = s;
   }
+  return UT_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- flow preconditioner
+namespace {
+
+int flow_rows_check(const ut_flow* fl, int64_t n, const char* who) {
+  if (!fl) return fail(UT_E_ARG, std::string(who) + ": null flow");
+  if (n <= 0 || n % FLOW_PAD != 0)
+    return fail(UT_E_DIM, std::string(who) + ": the row count must be a positive multiple of 128");
+  if (n > (1 << 20)) return fail(UT_E_DIM, std::string(who) + ": at most 1048576 rows per call");
+  return UT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ut_flow_create(int32_t num_params, int32_t num_layers, int32_t hidden, double scale_clamp, uint64_t seed,
+                   ut_flow** out) {
+  if (!out) return fail(UT_E_ARG, "ut_flow_create: null pointer");
+  *out = nullptr;
+  if (num_params < 2) return fail(UT_E_DIM, "ut_flow_create: at least 2 parameters");
+  if (num_layers < 1) return fail(UT_E_ARG, "ut_flow_create: num_layers must be >= 1");
+  if (hidden < 128 || hidden % 128 != 0) return fail(UT_E_ARG, "ut_flow_create: hidden must be a positive multiple of 128");
+  if (!(scale_clamp > 0)) return fail(UT_E_ARG, "ut_flow_create: scale_clamp must be > 0");
+  if (!ut_device_available()) return fail(UT_E_HIP, "ut_flow_create: no HIP device visible");
+  auto* fl = new ut_flow();
+  Flow& f = fl->f;
+  f.clamp = (float)scale_clamp;
+  hipError_t e = flow_alloc(f, num_params, num_layers, hidden);
+  if (e != hipSuccess) {
+    delete fl;
+    return fail(UT_E_HIP, std::string("ut_flow_create: ") + hipGetErrorString(e));
+  }
+  // identity start: W2 = 0, b = 0, mu = 0, ls = 0; W1 ~ N(0, 1/np) from Philox
+  std::vector<float> prm((size_t)f.nparams, 0.0f);
+  for (int l = 0; l < num_layers; ++l) {
+    const FlowLayer& ly = f.layers[l];
+    const int kp = (ly.np + FLOW_PAD - 1) / FLOW_PAD * FLOW_PAD;
+    const float sd = 1.0f / sqrt_f((float)ly.np);
+    for (int j = 0; j < hidden; ++j)
+      for (int k = 0; k < ly.np; ++k)
+        prm[(size_t)ly.oW1 + (size_t)j * kp + k] =
+            sd * rand_normal_f((uint32_t)seed, (uint32_t)(seed >> 32) + (uint32_t)l, 10u, (uint32_t)(j * kp + k));
+  }
+  e = hipMemcpy(f.params, prm.data(), prm.size() * 4, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = flow_sync_weights(f, f.stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(f.stream);
+  if (e != hipSuccess) {
+    delete fl;
+    return fail(UT_E_HIP, std::string("ut_flow_create: ") + hipGetErrorString(e));
+  }
+  *out = fl;
+  return UT_OK;
+}
+
+int ut_flow_destroy(ut_flow* flow) {
+  delete flow;
+  return UT_OK;
+}
+
+int ut_flow_num_weights(const ut_flow* flow, int64_t* n) {
+  if (!flow || !n) return fail(UT_E_ARG, "ut_flow_num_weights: null pointer");
+  *n = flow->f.nparams;
+  return UT_OK;
+}
+
+int ut_flow_layer_layout(const ut_flow* flow, int32_t layer, int32_t* np, int32_t* na, int64_t* offsets) {
+  if (!flow || !np || !na || !offsets) return fail(UT_E_ARG, "ut_flow_layer_layout: null pointer");
+  if (layer < 0 || layer >= flow->f.L) return fail(UT_E_ARG, "ut_flow_layer_layout: layer out of range");
+  const FlowLayer& ly = flow->f.layers[layer];
+  *np = ly.np;
+  *na = ly.na;
+  offsets[0] = ly.oW1;
+  offsets[1] = ly.ob1;
+  offsets[2] = ly.oW2;
+  offsets[3] = ly.ob2;
+  return UT_OK;
+}
+
+int ut_flow_get_weights(const ut_flow* flow, float* out) {
+  if (!flow || !out) return fail(UT_E_ARG, "ut_flow_get_weights: null pointer");
+  UT_HIP(hipMemcpy(out, flow->f.params, (size_t)flow->f.nparams * 4, hipMemcpyDeviceToHost));
+  return UT_OK;
+}
+
+int ut_flow_set_weights(ut_flow* flow, const float* in) {
+  if (!flow || !in) return fail(UT_E_ARG, "ut_flow_set_weights: null pointer");
+  Flow& f = flow->f;
+  UT_HIP(hipMemcpy(f.params, in, (size_t)f.nparams * 4, hipMemcpyHostToDevice));
+  UT_HIP(flow_sync_weights(f, f.stream));
+  UT_HIP(hipStreamSynchronize(f.stream));
+  return UT_OK;
+}
+
+int ut_flow_set_base(ut_flow* flow, const double* mu, const double* log_scale) {
+  if (!flow || !mu || !log_scale) return fail(UT_E_ARG, "ut_flow_set_base: null pointer");
+  Flow& f = flow->f;
+  std::vector<float> v(2 * (size_t)f.P);
+  for (int i = 0; i < f.P; ++i) {
+    v[i] = (float)mu[i];
+    v[f.P + i] = (float)log_scale[i];
+  }
+  UT_HIP(hipMemcpy(f.params, v.data(), v.size() * 4, hipMemcpyHostToDevice));
+  return UT_OK;
+}
+
+int ut_flow_train(ut_flow* flow, const float* draws, int64_t n, int32_t steps, int32_t batch, double learning_rate,
+                  uint64_t seed, double* loss_trace) {
+  if (!flow || !draws) return fail(UT_E_ARG, "ut_flow_train: null pointer");
+  if (n <= 0 || steps < 0) return fail(UT_E_ARG, "ut_flow_train: n must be > 0 and steps >= 0");
+  int rc = flow_rows_check(flow, batch, "ut_flow_train (batch)");
+  if (rc) return rc;
+  if (!(learning_rate > 0)) return fail(UT_E_ARG, "ut_flow_train: learning_rate must be > 0");
+  Flow& f = flow->f;
+  hipStream_t st = f.stream;
+  UT_HIP(flow_scratch(f, batch));
+  DeviceBuffers buf;
+  float *d_draws = nullptr, *d_batch = nullptr, *d_z = nullptr;
+  double* d_rows = nullptr;
+  UT_HIP(buf.alloc(&d_draws, (size_t)n * f.P));
+  UT_HIP(buf.alloc(&d_batch, (size_t)batch * f.P));
+  UT_HIP(buf.alloc(&d_z, (size_t)batch * f.P));
+  UT_HIP(buf.alloc(&d_rows, (size_t)batch));
+  UT_HIP(hipMemcpyAsync(d_draws, draws, (size_t)n * f.P * 4, hipMemcpyHostToDevice, st));
+  std::vector<double> rows(batch);
+  for (int s = 0; s < steps; ++s) {
+    UT_HIP(flow_gather_rows(d_draws, n, f.P, d_batch, batch, (uint32_t)seed, (uint32_t)s, st));
+    UT_HIP(flow_inverse(f, d_batch, d_z, nullptr, batch, st));
+    if (loss_trace) UT_HIP(flow_loss_value(d_z, f.sc.logdet, batch, f.P, d_rows, st));
+    UT_HIP(flow_loss_backward(f, d_z, batch, st));
+    UT_HIP(flow_adam(f, batch, learning_rate, 0.9, 0.999, 1e-8, st));
+    UT_HIP(flow_sync_weights(f, st));
+    if (loss_trace) {
+      UT_HIP(hipMemcpyAsync(rows.data(), d_rows, (size_t)batch * 8, hipMemcpyDeviceToHost, st));
+      UT_HIP(hipStreamSynchronize(st));
+      double acc = 0.0;
+      for (int b = 0; b < batch; ++b) acc += rows[b];
+      loss_trace[s] = acc / batch;
+    }
+  }
+  UT_HIP(hipStreamSynchronize(st));
+  return UT_OK;
+}
+
+int ut_flow_forward(ut_flow* flow, const float* z, int64_t n, float* q, float* logdet) {
+  int rc = flow_rows_check(flow, n, "ut_flow_forward");
+  if (rc) return rc;
+  if (!z || !q) return fail(UT_E_ARG, "ut_flow_forward: null pointer");
+  Flow& f = flow->f;
+  UT_HIP(flow_scratch(f, (int)n));
+  DeviceBuffers buf;
+  float *dz = nullptr, *dq = nullptr, *dl = nullptr;
+  UT_HIP(buf.alloc(&dz, (size_t)n * f.P));
+  UT_HIP(buf.alloc(&dq, (size_t)n * f.P));
+  UT_HIP(buf.alloc(&dl, (size_t)n));
+  UT_HIP(hipMemcpyAsync(dz, z, (size_t)n * f.P * 4, hipMemcpyHostToDevice, f.stream));
+  UT_HIP(flow_forward(f, dz, dq, dl, (int)n, f.stream));
+  UT_HIP(hipMemcpyAsync(q, dq, (size_t)n * f.P * 4, hipMemcpyDeviceToHost, f.stream));
+  if (logdet) UT_HIP(hipMemcpyAsync(logdet, dl, (size_t)n * 4, hipMemcpyDeviceToHost, f.stream));
+  UT_HIP(hipStreamSynchronize(f.stream));
+  return UT_OK;
+}
+
+int ut_flow_inverse(ut_flow* flow, const float* q, int64_t n, float* z, float* logdet_inv) {
+  int rc = flow_rows_check(flow, n, "ut_flow_inverse");
+  if (rc) return rc;
+  if (!z || !q) return fail(UT_E_ARG, "ut_flow_inverse: null pointer");
+  Flow& f = flow->f;
+  UT_HIP(flow_scratch(f, (int)n));
+  DeviceBuffers buf;
+  float *dz = nullptr, *dq = nullptr, *dl = nullptr;
+  UT_HIP(buf.alloc(&dz, (size_t)n * f.P));
+  UT_HIP(buf.alloc(&dq, (size_t)n * f.P));
+  UT_HIP(buf.alloc(&dl, (size_t)n));
+  UT_HIP(hipMemcpyAsync(dq, q, (size_t)n * f.P * 4, hipMemcpyHostToDevice, f.stream));
+  UT_HIP(flow_inverse(f, dq, dz, dl, (int)n, f.stream));
+  UT_HIP(hipMemcpyAsync(z, dz, (size_t)n * f.P * 4, hipMemcpyDeviceToHost, f.stream));
+  if (logdet_inv) UT_HIP(hipMemcpyAsync(logdet_inv, dl, (size_t)n * 4, hipMemcpyDeviceToHost, f.stream));
+  UT_HIP(hipStreamSynchronize(f.stream));
+  return UT_OK;
+}
+
+int ut_flow_vjp(ut_flow* flow, const float* z, int64_t n, const float* g_q, float* g_z) {
+  int rc = flow_rows_check(flow, n, "ut_flow_vjp");
+  if (rc) return rc;
+  if (!z || !g_q || !g_z) return fail(UT_E_ARG, "ut_flow_vjp: null pointer");
+  Flow& f = flow->f;
+  UT_HIP(flow_scratch(f, (int)n));
+  DeviceBuffers buf;
+  float *dz = nullptr, *dg = nullptr;
+  UT_HIP(buf.alloc(&dz, (size_t)n * f.P));
+  UT_HIP(buf.alloc(&dg, (size_t)n * f.P));
+  UT_HIP(hipMemcpyAsync(dz, z, (size_t)n * f.P * 4, hipMemcpyHostToDevice, f.stream));
+  UT_HIP(hipMemcpyAsync(dg, g_q, (size_t)n * f.P * 4, hipMemcpyHostToDevice, f.stream));
+  UT_HIP(flow_forward(f, dz, nullptr, nullptr, (int)n, f.stream));
+  UT_HIP(flow_backward(f, dg, dg, (int)n, f.stream));
+  UT_HIP(hipMemcpyAsync(g_z, dg, (size_t)n * f.P * 4, hipMemcpyDeviceToHost, f.stream));
+  UT_HIP(hipStreamSynchronize(f.stream));
+  return UT_OK;
+}
+
+int ut_flow_loss_grad(ut_flow* flow, const float* q, int64_t n, double* loss, float* grad) {
+  int rc = flow_rows_check(flow, n, "ut_flow_loss_grad");
+  if (rc) return rc;
+  if (!q || !grad) return fail(UT_E_ARG, "ut_flow_loss_grad: null pointer");
+  Flow& f = flow->f;
+  UT_HIP(flow_scratch(f, (int)n));
+  DeviceBuffers buf;
+  float *dq = nullptr, *dz = nullptr;
+  double* drows = nullptr;
+  UT_HIP(buf.alloc(&dq, (size_t)n * f.P));
+  UT_HIP(buf.alloc(&dz, (size_t)n * f.P));
+  UT_HIP(buf.alloc(&drows, (size_t)n));
+  UT_HIP(hipMemcpyAsync(dq, q, (size_t)n * f.P * 4, hipMemcpyHostToDevice, f.stream));
+  UT_HIP(flow_inverse(f, dq, dz, nullptr, (int)n, f.stream));
+  UT_HIP(flow_loss_value(dz, f.sc.logdet, (int)n, f.P, drows, f.stream));
+  UT_HIP(flow_loss_backward(f, dz, (int)n, f.stream));
+  std::vector<double> rows((size_t)n);
+  std::vector<float> g((size_t)f.nparams);
+  UT_HIP(hipMemcpyAsync(rows.data(), drows, (size_t)n * 8, hipMemcpyDeviceToHost, f.stream));
+  UT_HIP(hipMemcpyAsync(g.data(), f.grads, g.size() * 4, hipMemcpyDeviceToHost, f.stream));
+  UT_HIP(hipStreamSynchronize(f.stream));
+  double acc = 0.0;
+  for (double r : rows) acc += r;
+  if (loss) *loss = acc / (double)n;
+  for (size_t i = 0; i < g.size(); ++i) grad[i] = g[i] / (float)n;
+  return UT_OK;
+}
+
+int ut_workspace_set_flow(ut_workspace* ws, ut_flow* flow) {
+  if (!ws) return fail(UT_E_ARG, "ut_workspace_set_flow: null workspace");
+  if (flow) {
+    if (flow->f.P != ws->P) return fail(UT_E_DIM, "ut_workspace_set_flow: flow and workspace parameter counts differ");
+    if (ws->C % FLOW_PAD != 0 || ws->precision != UT_FLOAT32)
+      return fail(UT_E_DIM, "ut_workspace_set_flow: the workspace needs UT_FLOAT32 and num_chains a multiple of 128");
+    UT_HIP(flow_scratch(flow->f, ws->C));
+  }
+  ws->flow = flow;
   return UT_OK;
 }
 

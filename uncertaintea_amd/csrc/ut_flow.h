@@ -1,0 +1,98 @@
+// ut_flow.h -- affine-coupling flow preconditioner (internal to libuncertaintea_hip).
+//
+// q = f(z): x0 = mu + exp(ls) * z, then for layers l = 0 .. L-1 the passive
+// coordinates (parity of index + l even, the reference's alternating masks,
+// src/inference/vi/guides.jl:282-300) stay and the active ones become
+// x[a] = x[a] * exp(s) + t, where [s_raw, t] = W2 tanh(W1 x[p] + b1) + b2 and
+// s = c tanh(s_raw / c).  log |det df/dz| = sum(ls) + sum over layers of sum(s).
+// Weights are fp32 masters; the two GEMMs of every conditioner run on the matrix
+// cores with bf16 operands (inputs, hidden activations and bf16 weight copies) and
+// fp32 accumulation.  Everything else is fp32.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <vector>
+
+namespace ut {
+
+constexpr int FLOW_PAD = 128;  // passive inputs and conditioner outputs are padded to 128
+
+struct FlowLayer {
+  int np = 0, na = 0;      // passive / active counts (2 na <= 128, np <= 128)
+  int* pidx = nullptr;     // device, np passive indices
+  int* aidx = nullptr;     // device, na active indices
+  // offsets (floats) into the flat parameter vector
+  long long oW1 = 0, ob1 = 0, oW2 = 0, ob2 = 0;
+  uint16_t* W1b = nullptr;   // H x 128 bf16
+  uint16_t* W1Tb = nullptr;  // 128 x H
+  uint16_t* W2b = nullptr;   // 128 x H
+  uint16_t* W2Tb = nullptr;  // H x 128
+};
+
+// Per-batch activations kept for the backward pass.
+struct FlowScratch {
+  int B = 0;
+  std::vector<uint16_t*> xp;  // per layer B x 128 bf16 passive inputs
+  std::vector<float*> h;      // per layer B x H tanh activations
+  std::vector<uint16_t*> hb;  // per layer B x H bf16
+  std::vector<float*> s;      // per layer B x na
+  std::vector<float*> xa;     // per layer B x na: x_a e^s (forward) / the inverted x_a (inverse)
+  float* U = nullptr;         // B x H
+  float* O = nullptr;         // B x 128
+  uint16_t* gO = nullptr;     // B x 128
+  float* gOf = nullptr;       // B x 128
+  float* gH = nullptr;        // B x H
+  uint16_t* gU = nullptr;     // B x H
+  float* gUf = nullptr;       // B x H
+  float* gXp = nullptr;       // B x 128
+  uint16_t* tA = nullptr;     // transposes for the weight gradients (128 or H) x B
+  uint16_t* tB = nullptr;
+  float* dW = nullptr;        // H x 128 weight-gradient tile
+  float* x = nullptr;         // B x P work positions
+  float* g = nullptr;         // B x P work gradient
+  float* logdet = nullptr;    // B
+  float* colsum = nullptr;    // 2 x P base-parameter gradient sums
+};
+
+struct Flow {
+  int P = 0, L = 0, H = 0;
+  float clamp = 5.0f;
+  long long nparams = 0;
+  float* params = nullptr;  // device flat: mu[P], ls[P], per layer W1[H x 128], b1[H], W2[128 x H], b2[128]
+  float* grads = nullptr;
+  float* adam_m = nullptr;
+  float* adam_v = nullptr;
+  long long adam_t = 0;
+  std::vector<FlowLayer> layers;
+  FlowScratch sc;           // sized for sc.B rows
+  std::vector<void*> owned;
+  hipStream_t stream = nullptr;
+};
+
+hipError_t flow_alloc(Flow& f, int P, int L, int H);
+void flow_free(Flow& f);
+hipError_t flow_scratch(Flow& f, int B);
+// refresh the bf16 weight copies after the fp32 masters changed
+hipError_t flow_sync_weights(Flow& f, hipStream_t st);
+// q = f(z) for B rows (B a multiple of 128, B <= scratch rows); keeps activations
+hipError_t flow_forward(Flow& f, const float* z, float* q, float* logdet, int B, hipStream_t st);
+// g_z = (dq/dz)^T g_q + d logdet / dz, from the activations of the last forward
+hipError_t flow_backward(Flow& f, const float* g_q, float* g_z, int B, hipStream_t st);
+// z = f^-1(q), logdet_inv = log |det df^-1/dq| (keeps activations for flow_loss_backward)
+hipError_t flow_inverse(Flow& f, const float* q, float* z, float* logdet_inv, int B, hipStream_t st);
+// gradient of the mean negative log-likelihood  0.5|z|^2 - logdet_inv  (+ const) of
+// the last inverse over the parameters, into f.grads
+hipError_t flow_loss_backward(Flow& f, const float* z, int B, hipStream_t st);
+// Adam step on the mean over B rows of the last flow_loss_backward gradients
+hipError_t flow_adam(Flow& f, int B, double lr, double b1, double b2, double eps, hipStream_t st);
+// lj += logdet (z-space log density) / lj -= logdet (back to the model's log density)
+hipError_t flow_add_logdet(float* lj, const float* logdet, int B, hipStream_t st);
+hipError_t flow_sub_logdet(double* lj, const float* logdet, int B, hipStream_t st);
+// rows of src picked by Philox draws (key = (seed, step)) -> dst (B x P)
+hipError_t flow_gather_rows(const float* src, long long n, int P, float* dst, int B, uint32_t seed, uint32_t step,
+                            hipStream_t st);
+// per-row loss 0.5 |z|^2 - logdet_inv + P/2 log(2 pi), summed into *out (device double)
+hipError_t flow_loss_value(const float* z, const float* logdet_inv, int B, int P, double* out, hipStream_t st);
+
+}  // namespace ut

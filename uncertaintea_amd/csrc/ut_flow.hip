@@ -1,0 +1,571 @@
+// ut_flow.hip -- forward, inverse, both backward passes and Adam training of the
+// affine-coupling flow preconditioner (structure and conventions in ut_flow.h).
+//
+// Rows are chains (sampling) or draws (training); every per-row quantity is
+// chain-major like the samplers' state.  Per coupling layer the forward pass is
+//   gather passive x -> bf16 [B x kp]          k_gather
+//   U = X_p W1^T                               MFMA GEMM (K = kp)
+//   h = tanh(U + b1) (fp32 and bf16)           k_bias_tanh
+//   O = h W2^T                                 MFMA GEMM (K = H)
+//   s, t from O + b2; x_a <- x_a e^s + t       k_coupling_fwd   (k_coupling_inv: inverse)
+// and the backward pass mirrors it with the transposed bf16 weight copies
+//   dO (coupling), dH = dO W2, dU = dH (1 - h^2), dX_p = dU W1, scatter-add;
+// training adds dW2 = dO^T h and dW1 = dU^T X_p (transposed operands through LDS
+// tiles) and the bias column sums.
+#include <math.h>
+
+#include "ut_flow.h"
+#include "ut_gemm.h"
+#include "ut_math.h"
+#include "ut_mfma.h"
+
+namespace ut {
+
+namespace {
+
+constexpr int T256 = 256;
+inline unsigned nblk(long long n) { return (unsigned)((n + T256 - 1) / T256); }
+inline int round_up(int v, int m) { return (v + m - 1) / m * m; }
+
+__global__ void k_logdet_init(const float* __restrict__ ls, int P, float sign, float* __restrict__ logdet, int B) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  float s = 0.0f;
+  for (int i = 0; i < P; ++i) s += ls[i];
+  logdet[b] = sign * s;
+}
+
+// x = mu + e^ls z
+__global__ void k_base_fwd(const float* __restrict__ z, const float* __restrict__ prm, int P, float* __restrict__ x,
+                           long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int c = (int)(i % P);
+  x[i] = prm[c] + expf(prm[P + c]) * z[i];
+}
+
+// z = (x - mu) e^-ls
+__global__ void k_base_inv(const float* __restrict__ x, const float* __restrict__ prm, int P, float* __restrict__ z,
+                           long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int c = (int)(i % P);
+  z[i] = (x[i] - prm[c]) * expf(-prm[P + c]);
+}
+
+// g <- g e^ls (forward base, sampling backward)
+__global__ void k_base_fwd_bwd(float* __restrict__ g, const float* __restrict__ prm, int P, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  g[i] *= expf(prm[P + (int)(i % P)]);
+}
+
+// loss 0.5 |z|^2 + sum ls + ...: g_x0 = z e^-ls; parameter sums per column
+__global__ void k_base_inv_bwd(const float* __restrict__ z, const float* __restrict__ prm, int P,
+                               float* __restrict__ g, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  g[i] = z[i] * expf(-prm[P + (int)(i % P)]);
+}
+__global__ void k_base_param_grads(const float* __restrict__ z, const float* __restrict__ prm, int P, int B,
+                                   float* __restrict__ grads) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= P) return;
+  const float e = expf(-prm[P + c]);
+  float gm = 0.0f, gl = 0.0f;
+  for (int b = 0; b < B; ++b) {
+    const float v = z[(long long)b * P + c];
+    gm -= v * e;
+    gl += 1.0f - v * v;
+  }
+  grads[c] = gm;
+  grads[P + c] = gl;
+}
+
+__global__ void k_gather(const float* __restrict__ x, int P, const int* __restrict__ pidx, int np, int kp,
+                         bf16_t* __restrict__ xp, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const long long b = i / kp;
+  const int k = (int)(i % kp);
+  xp[i] = k < np ? f2bf(x[b * P + pidx[k]]) : (bf16_t)0;
+}
+
+__global__ void k_bias_tanh(const float* __restrict__ U, const float* __restrict__ b1, int H, float* __restrict__ h,
+                            bf16_t* __restrict__ hb, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float v = tanhf(U[i] + b1[i % H]);
+  h[i] = v;
+  hb[i] = f2bf(v);
+}
+
+// forward coupling, one thread per row
+__global__ void k_coupling_fwd(float* __restrict__ x, int P, const float* __restrict__ O, int op,
+                               const float* __restrict__ b2, const int* __restrict__ aidx, int na, float c,
+                               float* __restrict__ s_out, float* __restrict__ xs_out, float* __restrict__ logdet,
+                               int B) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  float sum = 0.0f;
+  for (int j = 0; j < na; ++j) {
+    const float s = c * tanhf((O[(long long)b * op + j] + b2[j]) / c);
+    const float t = O[(long long)b * op + na + j] + b2[na + j];
+    const long long ia = (long long)b * P + aidx[j];
+    const float xs = x[ia] * expf(s);
+    x[ia] = xs + t;
+    s_out[(long long)b * na + j] = s;
+    xs_out[(long long)b * na + j] = xs;
+    sum += s;
+  }
+  logdet[b] += sum;
+}
+
+// inverse coupling: x_a <- (x_a - t) e^-s, logdet_inv -= sum s
+__global__ void k_coupling_inv(float* __restrict__ x, int P, const float* __restrict__ O, int op,
+                               const float* __restrict__ b2, const int* __restrict__ aidx, int na, float c,
+                               float* __restrict__ s_out, float* __restrict__ xo_out, float* __restrict__ logdet,
+                               int B) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  float sum = 0.0f;
+  for (int j = 0; j < na; ++j) {
+    const float s = c * tanhf((O[(long long)b * op + j] + b2[j]) / c);
+    const float t = O[(long long)b * op + na + j] + b2[na + j];
+    const long long ia = (long long)b * P + aidx[j];
+    const float xo = (x[ia] - t) * expf(-s);
+    x[ia] = xo;
+    s_out[(long long)b * na + j] = s;
+    xo_out[(long long)b * na + j] = xo;
+    sum += s;
+  }
+  logdet[b] -= sum;
+}
+
+// backward of the forward coupling (objective includes + sum s); writes dO and
+// updates g on the active coordinates
+__global__ void k_coupling_fwd_bwd(float* __restrict__ g, int P, const float* __restrict__ s_in,
+                                   const float* __restrict__ xs_in, const int* __restrict__ aidx, int na, int op,
+                                   float c, bf16_t* __restrict__ gO, float* __restrict__ gOf, int B) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  for (int j = 0; j < na; ++j) {
+    const long long ia = (long long)b * P + aidx[j];
+    const float ga = g[ia];
+    const float s = s_in[(long long)b * na + j];
+    const float gs = ga * xs_in[(long long)b * na + j] + 1.0f;
+    const float r = s / c;
+    const float gos = gs * (1.0f - r * r);
+    gO[(long long)b * op + j] = f2bf(gos);
+    gO[(long long)b * op + na + j] = f2bf(ga);
+    if (gOf) {
+      gOf[(long long)b * op + j] = gos;
+      gOf[(long long)b * op + na + j] = ga;
+    }
+    g[ia] = ga * expf(s);
+  }
+  for (int j = 2 * na; j < op; ++j) {
+    gO[(long long)b * op + j] = 0;
+    if (gOf) gOf[(long long)b * op + j] = 0.0f;
+  }
+}
+
+// backward of the inverse coupling for the loss 0.5|z|^2 + sum s
+__global__ void k_coupling_inv_bwd(float* __restrict__ g, int P, const float* __restrict__ s_in,
+                                   const float* __restrict__ xo_in, const int* __restrict__ aidx, int na, int op,
+                                   float c, bf16_t* __restrict__ gO, float* __restrict__ gOf, int B) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  for (int j = 0; j < na; ++j) {
+    const long long ia = (long long)b * P + aidx[j];
+    const float go = g[ia];
+    const float s = s_in[(long long)b * na + j];
+    const float e = expf(-s);
+    const float gt = -go * e;
+    const float gs = -go * xo_in[(long long)b * na + j] + 1.0f;
+    const float r = s / c;
+    const float gos = gs * (1.0f - r * r);
+    gO[(long long)b * op + j] = f2bf(gos);
+    gO[(long long)b * op + na + j] = f2bf(gt);
+    gOf[(long long)b * op + j] = gos;
+    gOf[(long long)b * op + na + j] = gt;
+    g[ia] = go * e;
+  }
+  for (int j = 2 * na; j < op; ++j) {
+    gO[(long long)b * op + j] = 0;
+    gOf[(long long)b * op + j] = 0.0f;
+  }
+}
+
+__global__ void k_tanh_bwd(const float* __restrict__ gH, const float* __restrict__ h, bf16_t* __restrict__ gU,
+                           float* __restrict__ gUf, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float v = gH[i] * (1.0f - h[i] * h[i]);
+  gU[i] = f2bf(v);
+  if (gUf) gUf[i] = v;
+}
+
+__global__ void k_scatter_add(float* __restrict__ g, int P, const float* __restrict__ gXp, int kp,
+                              const int* __restrict__ pidx, int np, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const long long b = i / np;
+  const int k = (int)(i % np);
+  g[b * P + pidx[k]] += gXp[b * kp + k];
+}
+
+// out[col] = sum over rows (fixed order)
+__global__ void k_colsum(const float* __restrict__ M, int B, int n, float* __restrict__ out) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  float s = 0.0f;
+  for (int b = 0; b < B; ++b) s += M[(long long)b * n + c];
+  out[c] = s;
+}
+
+// out (Cc x R) = in (R x Cc)^T, bf16, 32 x 32 tiles through LDS
+__global__ void k_transpose_bf16(const bf16_t* __restrict__ in, int R, int Cc, bf16_t* __restrict__ out) {
+  __shared__ bf16_t tile[32][33];
+  const int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 256 threads: 8 rows per pass
+  for (int k = ty; k < 32; k += 8) tile[k][tx] = in[(long long)(r0 + k) * Cc + c0 + tx];
+  __syncthreads();
+  for (int k = ty; k < 32; k += 8) out[(long long)(c0 + k) * R + r0 + tx] = tile[tx][k];
+}
+
+__global__ void k_weights_bf16(const float* __restrict__ W, int R, int Cc, bf16_t* __restrict__ Wb,
+                               bf16_t* __restrict__ WTb) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long long)R * Cc) return;
+  const int r = (int)(i / Cc), c = (int)(i % Cc);
+  const bf16_t v = f2bf(W[i]);
+  Wb[i] = v;
+  WTb[(long long)c * R + r] = v;
+}
+
+__global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
+                       float* __restrict__ v, long long n, float lr, float b1, float b2, float eps, float bc1,
+                       float bc2, float scale) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i] * scale;
+  const float mi = b1 * m[i] + (1.0f - b1) * gi;
+  const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  p[i] -= lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
+}
+
+__global__ void k_gather_rows(const float* __restrict__ src, long long n, int P, float* __restrict__ dst, int B,
+                              uint32_t seed, uint32_t step) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long long)B * P) return;
+  const int b = (int)(i / P), c = (int)(i % P);
+  const double u = rand_uniform_d(seed, step, 9u, (uint32_t)b);
+  long long r = (long long)(u * (double)n);
+  if (r >= n) r = n - 1;
+  dst[i] = src[r * P + c];
+}
+
+__global__ void k_add_logdet(float* __restrict__ lj, const float* __restrict__ logdet, int B) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < B) lj[b] = lj[b] + logdet[b];
+}
+__global__ void k_sub_logdet(double* __restrict__ lj, const float* __restrict__ logdet, int B) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < B) lj[b] = lj[b] - (double)logdet[b];
+}
+
+__global__ void k_row_loss(const float* __restrict__ z, const float* __restrict__ logdet_inv, int B, int P,
+                           double* __restrict__ out) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  double s = 0.0;
+  for (int c = 0; c < P; ++c) {
+    const double v = z[(long long)b * P + c];
+    s += v * v;
+  }
+  out[b] = 0.5 * s - (double)logdet_inv[b] + 0.5 * P * 1.8378770664093453;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ host side
+template <class T>
+static hipError_t own(Flow& f, T** p, size_t count) {
+  *p = nullptr;
+  if (count == 0) return hipSuccess;
+  hipError_t e = hipMalloc((void**)p, count * sizeof(T));
+  if (e == hipSuccess) f.owned.push_back(*p);
+  return e;
+}
+
+hipError_t flow_alloc(Flow& f, int P, int L, int H) {
+  f.P = P;
+  f.L = L;
+  f.H = H;
+  long long off = 2LL * P;
+  f.layers.assign(L, FlowLayer());
+  std::vector<std::vector<int>> pid(L), aid(L);
+  for (int l = 0; l < L; ++l) {
+    for (int i = 0; i < P; ++i) ((i + l) % 2 == 0 ? pid[l] : aid[l]).push_back(i);
+    FlowLayer& ly = f.layers[l];
+    ly.np = (int)pid[l].size();
+    ly.na = (int)aid[l].size();
+    const int kp = round_up(ly.np, FLOW_PAD), op = round_up(2 * ly.na, FLOW_PAD);
+    ly.oW1 = off;
+    off += (long long)H * kp;
+    ly.ob1 = off;
+    off += H;
+    ly.oW2 = off;
+    off += (long long)op * H;
+    ly.ob2 = off;
+    off += op;
+  }
+  f.nparams = off;
+  hipError_t e = own(f, &f.params, (size_t)off);
+  if (e == hipSuccess) e = own(f, &f.grads, (size_t)off);
+  if (e == hipSuccess) e = own(f, &f.adam_m, (size_t)off);
+  if (e == hipSuccess) e = own(f, &f.adam_v, (size_t)off);
+  if (e == hipSuccess) e = hipMemset(f.adam_m, 0, (size_t)off * 4);
+  if (e == hipSuccess) e = hipMemset(f.adam_v, 0, (size_t)off * 4);
+  for (int l = 0; l < L && e == hipSuccess; ++l) {
+    FlowLayer& ly = f.layers[l];
+    const int kp = round_up(ly.np, FLOW_PAD), op = round_up(2 * ly.na, FLOW_PAD);
+    e = own(f, &ly.pidx, (size_t)ly.np);
+    if (e == hipSuccess) e = own(f, &ly.aidx, (size_t)std::max(ly.na, 1));
+    if (e == hipSuccess) e = hipMemcpy(ly.pidx, pid[l].data(), ly.np * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess && ly.na) e = hipMemcpy(ly.aidx, aid[l].data(), ly.na * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = own(f, &ly.W1b, (size_t)H * kp);
+    if (e == hipSuccess) e = own(f, &ly.W1Tb, (size_t)H * kp);
+    if (e == hipSuccess) e = own(f, &ly.W2b, (size_t)H * op);
+    if (e == hipSuccess) e = own(f, &ly.W2Tb, (size_t)H * op);
+  }
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&f.stream, hipStreamNonBlocking);
+  return e;
+}
+
+void flow_free(Flow& f) {
+  for (void* p : f.owned) (void)hipFree(p);
+  f.owned.clear();
+  if (f.stream) (void)hipStreamDestroy(f.stream);
+  f.stream = nullptr;
+}
+
+hipError_t flow_scratch(Flow& f, int B) {
+  if (B <= f.sc.B) return hipSuccess;
+  FlowScratch& s = f.sc;
+  s = FlowScratch();
+  s.B = B;
+  int kpm = FLOW_PAD, opm = FLOW_PAD;
+  for (const FlowLayer& ly : f.layers) {
+    kpm = std::max(kpm, round_up(ly.np, FLOW_PAD));
+    opm = std::max(opm, round_up(2 * ly.na, FLOW_PAD));
+  }
+  const size_t H = f.H;
+  hipError_t e = hipSuccess;
+  s.xp.assign(f.L, nullptr);
+  s.h.assign(f.L, nullptr);
+  s.hb.assign(f.L, nullptr);
+  s.s.assign(f.L, nullptr);
+  s.xa.assign(f.L, nullptr);
+  for (int l = 0; l < f.L && e == hipSuccess; ++l) {
+    const FlowLayer& ly = f.layers[l];
+    e = own(f, &s.xp[l], (size_t)B * round_up(ly.np, FLOW_PAD));
+    if (e == hipSuccess) e = own(f, &s.h[l], (size_t)B * H);
+    if (e == hipSuccess) e = own(f, &s.hb[l], (size_t)B * H);
+    if (e == hipSuccess) e = own(f, &s.s[l], (size_t)B * std::max(ly.na, 1));
+    if (e == hipSuccess) e = own(f, &s.xa[l], (size_t)B * std::max(ly.na, 1));
+  }
+  const size_t wide = std::max<size_t>(H, (size_t)std::max(kpm, opm));
+  if (e == hipSuccess) e = own(f, &s.U, (size_t)B * H);
+  if (e == hipSuccess) e = own(f, &s.O, (size_t)B * opm);
+  if (e == hipSuccess) e = own(f, &s.gO, (size_t)B * opm);
+  if (e == hipSuccess) e = own(f, &s.gOf, (size_t)B * opm);
+  if (e == hipSuccess) e = own(f, &s.gH, (size_t)B * H);
+  if (e == hipSuccess) e = own(f, &s.gU, (size_t)B * H);
+  if (e == hipSuccess) e = own(f, &s.gUf, (size_t)B * H);
+  if (e == hipSuccess) e = own(f, &s.gXp, (size_t)B * kpm);
+  if (e == hipSuccess) e = own(f, &s.tA, (size_t)B * wide);
+  if (e == hipSuccess) e = own(f, &s.tB, (size_t)B * wide);
+  if (e == hipSuccess) e = own(f, &s.x, (size_t)B * f.P);
+  if (e == hipSuccess) e = own(f, &s.g, (size_t)B * f.P);
+  if (e == hipSuccess) e = own(f, &s.logdet, (size_t)B);
+  return e;
+}
+
+hipError_t flow_sync_weights(Flow& f, hipStream_t st) {
+  for (FlowLayer& ly : f.layers) {
+    const int kp = round_up(ly.np, FLOW_PAD), op = round_up(2 * ly.na, FLOW_PAD);
+    hipLaunchKernelGGL(k_weights_bf16, dim3(nblk((long long)f.H * kp)), dim3(T256), 0, st, f.params + ly.oW1, f.H,
+                       kp, ly.W1b, ly.W1Tb);
+    hipLaunchKernelGGL(k_weights_bf16, dim3(nblk((long long)op * f.H)), dim3(T256), 0, st, f.params + ly.oW2, op,
+                       f.H, ly.W2b, ly.W2Tb);
+  }
+  return hipGetLastError();
+}
+
+// conditioner of layer l on the current rows of sc.x: sc.O = h W2^T (bias added by the coupling)
+static hipError_t conditioner(Flow& f, int l, int B, hipStream_t st) {
+  FlowLayer& ly = f.layers[l];
+  FlowScratch& s = f.sc;
+  const int kp = round_up(ly.np, FLOW_PAD), op = round_up(2 * ly.na, FLOW_PAD), H = f.H;
+  hipLaunchKernelGGL(k_gather, dim3(nblk((long long)B * kp)), dim3(T256), 0, st, s.x, f.P, ly.pidx, ly.np, kp,
+                     s.xp[l], (long long)B * kp);
+  hipError_t e = launch_gemm_nt(s.xp[l], kp, ly.W1b, kp, s.U, H, B, H, kp, 1, 0, st);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_bias_tanh, dim3(nblk((long long)B * H)), dim3(T256), 0, st, s.U, f.params + ly.ob1, H, s.h[l],
+                     s.hb[l], (long long)B * H);
+  return launch_gemm_nt(s.hb[l], H, ly.W2b, H, s.O, op, B, op, H, 1, 0, st);
+}
+
+// backward through the conditioner of layer l from sc.gO into sc.g (passive coordinates)
+static hipError_t conditioner_bwd(Flow& f, int l, int B, bool weights, hipStream_t st) {
+  FlowLayer& ly = f.layers[l];
+  FlowScratch& s = f.sc;
+  const int kp = round_up(ly.np, FLOW_PAD), op = round_up(2 * ly.na, FLOW_PAD), H = f.H;
+  hipError_t e = launch_gemm_nt(s.gO, op, ly.W2Tb, op, s.gH, H, B, H, op, 1, 0, st);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_tanh_bwd, dim3(nblk((long long)B * H)), dim3(T256), 0, st, s.gH, s.h[l], s.gU,
+                     weights ? s.gUf : nullptr, (long long)B * H);
+  e = launch_gemm_nt(s.gU, H, ly.W1Tb, H, s.gXp, kp, B, kp, H, 1, 0, st);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_scatter_add, dim3(nblk((long long)B * ly.np)), dim3(T256), 0, st, s.g, f.P, s.gXp, kp,
+                     ly.pidx, ly.np, (long long)B * ly.np);
+  if (!weights) return hipGetLastError();
+  // dW2 (op x H) = gO^T h ; dW1 (H x kp) = gU^T X_p ; bias column sums
+  hipLaunchKernelGGL(k_transpose_bf16, dim3(op / 32, B / 32), dim3(T256), 0, st, s.gO, B, op, s.tA);
+  hipLaunchKernelGGL(k_transpose_bf16, dim3(H / 32, B / 32), dim3(T256), 0, st, s.hb[l], B, H, s.tB);
+  e = launch_gemm_nt(s.tA, B, s.tB, B, f.grads + ly.oW2, H, op, H, B, 1, 0, st);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_transpose_bf16, dim3(H / 32, B / 32), dim3(T256), 0, st, s.gU, B, H, s.tA);
+  hipLaunchKernelGGL(k_transpose_bf16, dim3(kp / 32, B / 32), dim3(T256), 0, st, s.xp[l], B, kp, s.tB);
+  e = launch_gemm_nt(s.tA, B, s.tB, B, f.grads + ly.oW1, kp, H, kp, B, 1, 0, st);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_colsum, dim3(nblk(H)), dim3(T256), 0, st, s.gUf, B, H, f.grads + ly.ob1);
+  hipLaunchKernelGGL(k_colsum, dim3(nblk(op)), dim3(T256), 0, st, s.gOf, B, op, f.grads + ly.ob2);
+  return hipGetLastError();
+}
+
+hipError_t flow_forward(Flow& f, const float* z, float* q, float* logdet, int B, hipStream_t st) {
+  if (B % FLOW_PAD || B > f.sc.B) return hipErrorInvalidValue;
+  FlowScratch& s = f.sc;
+  const long long n = (long long)B * f.P;
+  hipLaunchKernelGGL(k_logdet_init, dim3(nblk(B)), dim3(T256), 0, st, f.params + f.P, f.P, 1.0f, s.logdet, B);
+  hipLaunchKernelGGL(k_base_fwd, dim3(nblk(n)), dim3(T256), 0, st, z, f.params, f.P, s.x, n);
+  for (int l = 0; l < f.L; ++l) {
+    FlowLayer& ly = f.layers[l];
+    hipError_t e = conditioner(f, l, B, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_coupling_fwd, dim3(nblk(B)), dim3(T256), 0, st, s.x, f.P, s.O,
+                       round_up(2 * ly.na, FLOW_PAD), f.params + ly.ob2, ly.aidx, ly.na, f.clamp, s.s[l], s.xa[l],
+                       s.logdet, B);
+  }
+  if (q && q != s.x) {
+    hipError_t e = hipMemcpyAsync(q, s.x, n * 4, hipMemcpyDeviceToDevice, st);
+    if (e != hipSuccess) return e;
+  }
+  if (logdet && logdet != s.logdet) {
+    hipError_t e = hipMemcpyAsync(logdet, s.logdet, (size_t)B * 4, hipMemcpyDeviceToDevice, st);
+    if (e != hipSuccess) return e;
+  }
+  return hipGetLastError();
+}
+
+hipError_t flow_backward(Flow& f, const float* g_q, float* g_z, int B, hipStream_t st) {
+  if (B % FLOW_PAD || B > f.sc.B) return hipErrorInvalidValue;
+  FlowScratch& s = f.sc;
+  const long long n = (long long)B * f.P;
+  if (g_q != s.g) {
+    hipError_t e = hipMemcpyAsync(s.g, g_q, n * 4, hipMemcpyDeviceToDevice, st);
+    if (e != hipSuccess) return e;
+  }
+  for (int l = f.L - 1; l >= 0; --l) {
+    FlowLayer& ly = f.layers[l];
+    const int op = round_up(2 * ly.na, FLOW_PAD);
+    hipLaunchKernelGGL(k_coupling_fwd_bwd, dim3(nblk(B)), dim3(T256), 0, st, s.g, f.P, s.s[l], s.xa[l], ly.aidx,
+                       ly.na, op, f.clamp, s.gO, (float*)nullptr, B);
+    hipError_t e = conditioner_bwd(f, l, B, false, st);
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(k_base_fwd_bwd, dim3(nblk(n)), dim3(T256), 0, st, s.g, f.params, f.P, n);
+  if (g_z != s.g) {
+    hipError_t e = hipMemcpyAsync(g_z, s.g, n * 4, hipMemcpyDeviceToDevice, st);
+    if (e != hipSuccess) return e;
+  }
+  return hipGetLastError();
+}
+
+hipError_t flow_inverse(Flow& f, const float* q, float* z, float* logdet_inv, int B, hipStream_t st) {
+  if (B % FLOW_PAD || B > f.sc.B) return hipErrorInvalidValue;
+  FlowScratch& s = f.sc;
+  const long long n = (long long)B * f.P;
+  if (q != s.x) {
+    hipError_t e = hipMemcpyAsync(s.x, q, n * 4, hipMemcpyDeviceToDevice, st);
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(k_logdet_init, dim3(nblk(B)), dim3(T256), 0, st, f.params + f.P, f.P, -1.0f, s.logdet, B);
+  for (int l = f.L - 1; l >= 0; --l) {
+    FlowLayer& ly = f.layers[l];
+    hipError_t e = conditioner(f, l, B, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_coupling_inv, dim3(nblk(B)), dim3(T256), 0, st, s.x, f.P, s.O,
+                       round_up(2 * ly.na, FLOW_PAD), f.params + ly.ob2, ly.aidx, ly.na, f.clamp, s.s[l], s.xa[l],
+                       s.logdet, B);
+  }
+  hipLaunchKernelGGL(k_base_inv, dim3(nblk(n)), dim3(T256), 0, st, s.x, f.params, f.P, z, n);
+  if (logdet_inv && logdet_inv != s.logdet) {
+    hipError_t e = hipMemcpyAsync(logdet_inv, s.logdet, (size_t)B * 4, hipMemcpyDeviceToDevice, st);
+    if (e != hipSuccess) return e;
+  }
+  return hipGetLastError();
+}
+
+hipError_t flow_loss_backward(Flow& f, const float* z, int B, hipStream_t st) {
+  if (B % FLOW_PAD || B > f.sc.B) return hipErrorInvalidValue;
+  FlowScratch& s = f.sc;
+  const long long n = (long long)B * f.P;
+  hipLaunchKernelGGL(k_base_inv_bwd, dim3(nblk(n)), dim3(T256), 0, st, z, f.params, f.P, s.g, n);
+  hipLaunchKernelGGL(k_base_param_grads, dim3(nblk(f.P)), dim3(T256), 0, st, z, f.params, f.P, B, f.grads);
+  for (int l = 0; l < f.L; ++l) {
+    FlowLayer& ly = f.layers[l];
+    const int op = round_up(2 * ly.na, FLOW_PAD);
+    hipLaunchKernelGGL(k_coupling_inv_bwd, dim3(nblk(B)), dim3(T256), 0, st, s.g, f.P, s.s[l], s.xa[l], ly.aidx,
+                       ly.na, op, f.clamp, s.gO, s.gOf, B);
+    hipError_t e = conditioner_bwd(f, l, B, true, st);
+    if (e != hipSuccess) return e;
+  }
+  return hipGetLastError();
+}
+
+hipError_t flow_adam(Flow& f, int B, double lr, double b1, double b2, double eps, hipStream_t st) {
+  f.adam_t += 1;
+  const double bc1 = 1.0 - pow(b1, (double)f.adam_t), bc2 = 1.0 - pow(b2, (double)f.adam_t);
+  hipLaunchKernelGGL(k_adam, dim3(nblk(f.nparams)), dim3(T256), 0, st, f.params, f.grads, f.adam_m, f.adam_v,
+                     f.nparams, (float)lr, (float)b1, (float)b2, (float)eps, (float)bc1, (float)bc2,
+                     1.0f / (float)f.sc.B);
+  return hipGetLastError();
+}
+
+hipError_t flow_gather_rows(const float* src, long long n, int P, float* dst, int B, uint32_t seed, uint32_t step,
+                            hipStream_t st) {
+  hipLaunchKernelGGL(k_gather_rows, dim3(nblk((long long)B * P)), dim3(T256), 0, st, src, n, P, dst, B, seed, step);
+  return hipGetLastError();
+}
+
+hipError_t flow_add_logdet(float* lj, const float* logdet, int B, hipStream_t st) {
+  hipLaunchKernelGGL(k_add_logdet, dim3(nblk(B)), dim3(T256), 0, st, lj, logdet, B);
+  return hipGetLastError();
+}
+hipError_t flow_sub_logdet(double* lj, const float* logdet, int B, hipStream_t st) {
+  hipLaunchKernelGGL(k_sub_logdet, dim3(nblk(B)), dim3(T256), 0, st, lj, logdet, B);
+  return hipGetLastError();
+}
+
+hipError_t flow_loss_value(const float* z, const float* logdet_inv, int B, int P, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_row_loss, dim3(nblk(B)), dim3(T256), 0, st, z, logdet_inv, B, P, out);
+  return hipGetLastError();
+}
+
+}  // namespace ut

@@ -1,0 +1,136 @@
+"""Affine-coupling flow preconditioner (the C ABI's ut_flow).
+
+Structure follows the reference's FlowGuide (src/inference/vi/guides.jl:1-21,
+282-369): an elementwise affine base x0 = mu + exp(log_scale) z and coupling
+layers with alternating parity masks, active x <- x exp(s) + t.  The reference's
+conditioners are linear and trained host-side for ADVI; here they are MLPs of width
+`hidden` trained on the GPU by maximum likelihood on posterior draws, and the
+trained flow preconditions the lock-step HMC / ChEES samplers (NeuTra):
+`batched_hmc(model, ..., flow=flow)`.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+
+import numpy as np
+
+from . import _lib as L
+
+
+def _pad_rows(a: np.ndarray, mult: int = 128) -> tuple[np.ndarray, int]:
+    n = a.shape[0]
+    m = -(-n // mult) * mult
+    if m == n:
+        return np.ascontiguousarray(a, dtype=np.float32), n
+    out = np.zeros((m, a.shape[1]), dtype=np.float32)
+    out[:n] = a
+    return out, n
+
+
+class AffineCouplingFlow:
+    """q = f(z) on R^P with `num_layers` coupling layers and MLP width `hidden`."""
+
+    def __init__(self, num_params: int, num_layers: int = 6, hidden: int = 256, scale_clamp: float = 3.0,
+                 seed: int = 0):
+        self.num_params = int(num_params)
+        self.num_layers = int(num_layers)
+        self.hidden = int(hidden)
+        self.scale_clamp = float(scale_clamp)
+        h = ct.c_void_p()
+        L.check(L.lib().ut_flow_create(self.num_params, self.num_layers, self.hidden, self.scale_clamp,
+                                       int(seed) & 0xFFFFFFFFFFFFFFFF, ct.byref(h)))
+        self._h = h
+        self._destroy = L.lib().ut_flow_destroy  # kept: module globals may be gone at exit
+        n = ct.c_int64()
+        L.check(L.lib().ut_flow_num_weights(h, ct.byref(n)))
+        self.num_weights = int(n.value)
+        self.loss_trace: np.ndarray | None = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    def __del__(self):
+        if getattr(self, "_h", None) is not None:
+            self._destroy(self._h)
+            self._h = None
+
+    def layout(self):
+        """Per layer: (num_passive, num_active, {W1, b1, W2, b2} offsets)."""
+        out = []
+        for layer in range(self.num_layers):
+            npv, nav = ct.c_int32(), ct.c_int32()
+            off = np.zeros(4, dtype=np.int64)
+            L.check(L.lib().ut_flow_layer_layout(self._h, layer, ct.byref(npv), ct.byref(nav), L.ptr(off)))
+            out.append((int(npv.value), int(nav.value), dict(zip(("W1", "b1", "W2", "b2"), off.tolist()))))
+        return out
+
+    @property
+    def weights(self) -> np.ndarray:
+        w = np.empty(self.num_weights, dtype=np.float32)
+        L.check(L.lib().ut_flow_get_weights(self._h, L.ptr(w)))
+        return w
+
+    @weights.setter
+    def weights(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        if w.shape != (self.num_weights,):
+            raise ValueError(f"DimensionMismatch: expected {self.num_weights} weights")
+        L.check(L.lib().ut_flow_set_weights(self._h, L.ptr(w)))
+
+    def set_base(self, mu, log_scale):
+        mu = np.ascontiguousarray(mu, dtype=np.float64)
+        ls = np.ascontiguousarray(log_scale, dtype=np.float64)
+        if mu.shape != (self.num_params,) or ls.shape != (self.num_params,):
+            raise ValueError("DimensionMismatch: mu and log_scale must have num_params entries")
+        L.check(L.lib().ut_flow_set_base(self._h, L.ptr(mu), L.ptr(ls)))
+
+    def fit(self, draws, steps: int = 2000, batch: int = 4096, learning_rate: float = 3e-4, seed: int = 0,
+            init_base: bool = True) -> np.ndarray:
+        """Maximum-likelihood training on `draws` (n x P, unconstrained space) with
+        Adam on the GPU; returns (and keeps) the per-step minibatch loss."""
+        d = np.ascontiguousarray(np.asarray(draws, dtype=np.float32).reshape(-1, self.num_params))
+        if not np.all(np.isfinite(d)):
+            d = d[np.all(np.isfinite(d), axis=1)]
+        if init_base:
+            mu = d.mean(axis=0, dtype=np.float64)
+            sd = d.std(axis=0, dtype=np.float64)
+            self.set_base(mu, np.log(np.maximum(sd, 1e-6)))
+        trace = np.zeros(int(steps), dtype=np.float64)
+        L.check(L.lib().ut_flow_train(self._h, L.ptr(d), d.shape[0], int(steps), int(batch), float(learning_rate),
+                                      int(seed) & 0xFFFFFFFFFFFFFFFF, L.ptr(trace)))
+        self.loss_trace = trace
+        return trace
+
+    def forward(self, z):
+        """(q, log|det dq/dz|) for rows of z."""
+        zp, n = _pad_rows(np.asarray(z).reshape(-1, self.num_params))
+        q = np.empty_like(zp)
+        ld = np.empty(zp.shape[0], dtype=np.float32)
+        L.check(L.lib().ut_flow_forward(self._h, L.ptr(zp), zp.shape[0], L.ptr(q), L.ptr(ld)))
+        return q[:n], ld[:n]
+
+    def inverse(self, q):
+        """(z, log|det dz/dq|) for rows of q."""
+        qp, n = _pad_rows(np.asarray(q).reshape(-1, self.num_params))
+        z = np.empty_like(qp)
+        ld = np.empty(qp.shape[0], dtype=np.float32)
+        L.check(L.lib().ut_flow_inverse(self._h, L.ptr(qp), qp.shape[0], L.ptr(z), L.ptr(ld)))
+        return z[:n], ld[:n]
+
+    def vjp(self, z, g_q):
+        """(dq/dz)^T g_q + d log|det dq/dz| / dz for rows of z."""
+        zp, n = _pad_rows(np.asarray(z).reshape(-1, self.num_params))
+        gp, _ = _pad_rows(np.asarray(g_q).reshape(-1, self.num_params))
+        gz = np.empty_like(zp)
+        L.check(L.lib().ut_flow_vjp(self._h, L.ptr(zp), zp.shape[0], L.ptr(gp), L.ptr(gz)))
+        return gz[:n]
+
+    def loss_grad(self, q):
+        """(mean negative log-likelihood, its gradient over the flat weights); rows
+        must be a multiple of 128."""
+        qq = np.ascontiguousarray(np.asarray(q, dtype=np.float32).reshape(-1, self.num_params))
+        g = np.empty(self.num_weights, dtype=np.float32)
+        loss = ct.c_double()
+        L.check(L.lib().ut_flow_loss_grad(self._h, L.ptr(qq), qq.shape[0], ct.byref(loss), L.ptr(g)))
+        return float(loss.value), g

@@ -112,10 +112,14 @@ def _opts(num_samples, num_warmup, step_size, target_accept, divergence_threshol
 
 
 def _run(kind: str, model: TeaModel, num_chains: int, opts, initial_params,
-         max_tree_depth: int, num_leapfrog_steps: int, target_accept: float) -> HMCChains:
+         max_tree_depth: int, num_leapfrog_steps: int, target_accept: float, flow=None) -> HMCChains:
     if num_chains <= 0:
         raise ValueError("num_chains must be positive")
     ws = DeviceBatchedWorkspace(model, num_chains, np.float32)
+    if flow is not None:
+        if flow.num_params != model.num_params:
+            raise ValueError("DimensionMismatch: the flow and the model have different parameter counts")
+        L.check(L.lib().ut_workspace_set_flow(ws.handle, flow.handle))
     base = opts.base if kind == "chees" else opts
     P, C, S = model.num_params, int(num_chains), int(base.num_samples)
     init = None
@@ -150,8 +154,9 @@ def _run(kind: str, model: TeaModel, num_chains: int, opts, initial_params,
         trace = np.zeros(int(base.num_warmup), dtype=np.float64)
         L.check(L.lib().ut_chees_run(ws.handle, ct.byref(opts), ct.byref(out), L.ptr(trace)))
     else:
-        fn = L.lib().ut_nuts_run if kind == "nuts" else L.lib().ut_hmc_run
+        fn = {"nuts": L.lib().ut_nuts_run, "hmc": L.lib().ut_hmc_run, "hmc_shared": L.lib().ut_hmc_shared_run}[kind]
         L.check(fn(ws.handle, ct.byref(opts), ct.byref(out)))
+    kind = "hmc" if kind == "hmc_shared" else kind
     return HMCChains(kind, model, samples, lj, acc, en, err, accepted.view(bool), div.view(bool), depth,
                      steps, step, im, max_tree_depth, num_leapfrog_steps, target_accept, out.warmup_seconds,
                      out.sampling_seconds, int(out.sampling_gradient_evals), out.sampling_kernel_seconds,
@@ -187,18 +192,26 @@ def batched_hmc(model: TeaModel, args=(), constraints=None, *, num_chains: int, 
                 adapt_step_size: bool = True, adapt_mass_matrix: bool = True, per_chain_adaptation=None,
                 find_reasonable_step_size: bool = False, divergence_threshold: float = 1000.0,
                 mass_matrix_regularization: float = 1e-3, mass_matrix_min_samples: int = 10,
-                backend: str = "hip", precision=np.float32, seed: int = 0) -> HMCChains:
-    """Many fixed-length HMC chains, device-resident (momentum, accept and adaptation on the GPU)."""
+                backend: str = "hip", precision=np.float32, seed: int = 0, flow=None) -> HMCChains:
+    """Many fixed-length HMC chains, device-resident (momentum, accept and adaptation on the GPU).
+
+    per_chain_adaptation (default True, as the reference): each chain adapts its own
+    step size inside the persistent kernel.  False: the reference's shared mode -- all
+    chains in lock step with one dual-averaged step size.  `flow` (an
+    AffineCouplingFlow) preconditions the shared mode: chains run in the flow's base
+    space and draws are returned mapped through the flow."""
     if backend != "hip":
         raise ValueError(f"backend must be 'hip', got {backend!r}")
-    if per_chain_adaptation is False:
-        raise ValueError("the device path adapts a per-chain step with a pooled mass (per_chain_adaptation=True)")
     if np.dtype(precision) != np.float32:
         raise ValueError("the device samplers run in float32 with float64 energies")
+    if flow is not None and per_chain_adaptation:
+        raise ValueError("a flow preconditions the shared-adaptation (lock-step) mode; pass per_chain_adaptation=False")
+    shared = per_chain_adaptation is False or flow is not None
     o = _opts(num_samples, num_warmup, step_size, target_accept, divergence_threshold, mass_matrix_regularization,
               mass_matrix_min_samples, adapt_step_size, adapt_mass_matrix, find_reasonable_step_size, init_strategy,
               init_max_retries, seed, num_leapfrog_steps=num_leapfrog_steps)
-    return _run("hmc", model, num_chains, o, initial_params, 0, num_leapfrog_steps, target_accept)
+    return _run("hmc_shared" if shared else "hmc", model, num_chains, o, initial_params, 0, num_leapfrog_steps,
+                target_accept, flow=flow)
 
 
 def batched_chees(model: TeaModel, args=(), constraints=None, *, num_chains: int, num_samples: int,
@@ -211,7 +224,7 @@ def batched_chees(model: TeaModel, args=(), constraints=None, *, num_chains: int
                   adapt_mass_matrix: bool = True, find_reasonable_step_size: bool = False,
                   divergence_threshold: float = 1000.0, mass_matrix_regularization: float = 1e-3,
                   mass_matrix_min_samples: int = 10, backend: str = "hip", precision=np.float32,
-                  seed: int = 0, forward_velocity: bool = False) -> HMCChains:
+                  seed: int = 0, forward_velocity: bool = False, flow=None) -> HMCChains:
     """ChEES-HMC: every chain integrates the same Halton-jittered number of leapfrog
     steps with a shared step size; warmup adapts the step (dual averaging to the
     harmonic-mean acceptance), a pooled diagonal mass and the trajectory time T
@@ -241,4 +254,4 @@ def batched_chees(model: TeaModel, args=(), constraints=None, *, num_chains: int
     co.trajectory_adam_beta2 = float(trajectory_adam_beta2)
     co.trajectory_adam_epsilon = float(trajectory_adam_epsilon)
     co.forward_velocity = int(bool(forward_velocity))
-    return _run("chees", model, num_chains, co, initial_params, 0, num_leapfrog_steps, target_accept)
+    return _run("chees", model, num_chains, co, initial_params, 0, num_leapfrog_steps, target_accept, flow=flow)
