@@ -20,6 +20,13 @@ X ~ N(0, 1/D) stored in bf16, 4096 chains) under batched_chees with the fused
 matrix-core gradient; roofline against the bf16 MFMA peak with the algorithmic
 4 N D flops per chain-gradient, CPU baseline = the binary64 oracle gradient on the
 host cores (projected: CPU s per batch gradient x mean leapfrog steps).
+
+`--workload funnel_flow`: Neal's funnel D = 100 (P = 101), 4096 chains, batched_hmc
+L = 10: plain (per-chain adaptation, persistent kernel) and flow-preconditioned
+(a 6-layer, width-256 affine-coupling flow trained on the GPU on the plain run's
+2000 draws per chain, then shared-adaptation lock-step HMC in the flow's base
+space).  `value` is the flow run's min bulk-ESS/s; the plain run is reported beside
+it.  Roofline: the flow's conditioner GEMMs against the bf16 MFMA peak.
 """
 from __future__ import annotations
 
@@ -59,7 +66,7 @@ def cpu_cores() -> int:
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=("hier_nuts", "glm_chees"), default="hier_nuts")
+    ap.add_argument("--workload", choices=("hier_nuts", "glm_chees", "funnel_flow"), default="hier_nuts")
     ap.add_argument("--steps", type=int, default=None, help="sampling iterations")
     ap.add_argument("--warmup", type=int, default=None, help="warmup iterations")
     ap.add_argument("--chains", type=int, default=None)
@@ -88,6 +95,8 @@ def main() -> int:
 
     if args.workload == "glm_chees":
         return bench_glm_chees(args)
+    if args.workload == "funnel_flow":
+        return bench_funnel_flow(args)
     args.steps = 3000 if args.steps is None else args.steps
     args.warmup = 1000 if args.warmup is None else args.warmup
     args.chains = 1024 if args.chains is None else args.chains
@@ -163,6 +172,87 @@ def main() -> int:
         "mean_tree_depth": float(res.tree_depths.mean()), "divergence_rate": res.divergence_rate(),
         "roofline": roofline, "cpu_baseline": cpu,
         "vs_cpu": (value / cpu["value"]) if cpu else None,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def bench_funnel_flow(args) -> int:
+    import uncertaintea_amd as ut
+    from uncertaintea_amd import diagnostics as D
+
+    steps = 2000 if args.steps is None else args.steps
+    warmup = 1000 if args.warmup is None else args.warmup
+    chains = 4096 if args.chains is None else args.chains
+    dim, L, H, layers = 100, 10, 256, 6
+    model = ut.NealsFunnel(dim, 3.0)
+
+    def summarise(res, label):
+        rh, eb = D.rhat_and_ess_bulk(res.constrained_samples)
+        out = {"min_bulk_ess": float(np.nanmin(eb)), "rhat_max": float(np.nanmax(rh)),
+               "ess_bulk_v": float(eb[0]), "sampling_seconds": res.sampling_seconds,
+               "warmup_seconds": res.warmup_seconds, "value": float(np.nanmin(eb)) / res.sampling_seconds,
+               "gradient_evals_per_s": res.sampling_gradient_evals / res.sampling_seconds,
+               "divergence_rate": res.divergence_rate(), "step_size": float(np.median(res.step_size)),
+               "v_mean": float(res.unconstrained_samples[:, :, 0].mean()),
+               "v_sd": float(res.unconstrained_samples[:, :, 0].std())}
+        out["rhat_ok"] = out["rhat_max"] < 1.01
+        log(f"bench: {label}: min ESS {out['min_bulk_ess']:.1f} rhat {out['rhat_max']:.4f} -> {out['value']:.1f} "
+            f"ESS/s; v mean {out['v_mean']:.3f} sd {out['v_sd']:.3f}; div {out['divergence_rate']:.4f}")
+        return out
+
+    log(f"bench: funnel D={dim} plain batched_hmc chains={chains} warmup={warmup} steps={steps}")
+    plain = ut.batched_hmc(model, num_chains=chains, num_samples=steps, num_warmup=warmup, num_leapfrog_steps=L,
+                           seed=args.seed)
+    plain_s = summarise(plain, "plain")
+    draws = plain.unconstrained_samples.reshape(-1, model.num_params)
+    flow = ut.AffineCouplingFlow(model.num_params, layers, H, seed=args.seed)
+    t0 = time.perf_counter()
+    train_steps = 2000
+    trace = flow.fit(draws, steps=train_steps, batch=4096, seed=args.seed)
+    t_train = time.perf_counter() - t0
+    log(f"bench: flow trained on {draws.shape[0]} draws, {train_steps} steps in {t_train:.2f}s, "
+        f"loss {trace[:20].mean():.2f} -> {trace[-100:].mean():.2f}")
+    del draws
+    fres = ut.batched_hmc(model, num_chains=chains, num_samples=steps, num_warmup=warmup // 2, num_leapfrog_steps=L,
+                          per_chain_adaptation=False, flow=flow, seed=args.seed + 1)
+    flow_s = summarise(fres, "flow")
+    # conditioner GEMM flops per chain-gradient: forward 2 GEMMs + backward 2 GEMMs per layer
+    flops_grad = 0.0
+    for npass, nact, _ in flow.layout():
+        flops_grad += 2 * 2.0 * (npass * H + H * 2 * nact)
+    achieved = fres.sampling_gradient_evals * flops_grad / fres.sampling_kernel_seconds / 1e12
+    roofline = {"bound": "mfma", "compute": "bf16 MFMA conditioner GEMMs (launch-bound at 4096 rows)",
+                "achieved": achieved, "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
+                "frac": achieved / PEAK_BF16_TFLOPS, "traffic": None, "kernel": "k_gemm_nt (flow conditioners)",
+                "timing": "HIP events around all sampling launches of the flow run",
+                "flops_per_chain_gradient": flops_grad}
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import binding as B
+        om = B.OracleModel(model)
+        cores = cpu_cores()
+        n_it = max(1, min(200, int(args.cpu_seconds * 20)))
+        t = time.perf_counter()
+        om.run("hmc", "f64", chains, num_samples=n_it, num_warmup=0, num_leapfrog_steps=L,
+               step_size=float(np.median(plain.step_size)), adapt_step_size=False, seed=args.seed, threads=cores)
+        s_iter = (time.perf_counter() - t) / n_it
+        cpu = {"value": (plain_s["min_bulk_ess"] / steps) / s_iter, "unit": "ESS/s (projected, plain HMC)",
+               "cores": cores, "kind": "port",
+               "sample": f"{n_it} binary64 oracle HMC iterations x {chains} chains (L={L}); projected = plain GPU "
+                         f"bulk ESS per iteration / CPU s per iteration", "cpu_s_per_iteration": s_iter}
+        log(f"bench: cpu {s_iter:.3f} s/iter -> projected {cpu['value']:.2f} ESS/s")
+    line = {
+        "metric": METRIC, "value": flow_s["value"], "unit": "ESS/s", "n_gpus": 1, "steps": steps,
+        "warmup": warmup // 2, "ms_per_step": 1e3 * fres.sampling_seconds / steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32+bf16", "data": "synthetic (no data; seed %d)" % args.seed,
+        "config": {"workload": f"Neal's funnel D={dim}, batched_hmc L={L}, {chains} chains, plain vs affine-coupling "
+                               f"flow ({layers} layers, width {H}) trained on the plain run's {steps} draws/chain",
+                   "chains": chains},
+        "rhat_max": flow_s["rhat_max"], "rhat_ok": flow_s["rhat_ok"], "min_bulk_ess": flow_s["min_bulk_ess"],
+        "flow": flow_s, "plain": plain_s, "flow_training_seconds": t_train,
+        "flow_vs_plain": flow_s["value"] / plain_s["value"] if plain_s["value"] > 0 else None,
+        "roofline": roofline, "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
     return 0

@@ -48,11 +48,11 @@ struct FlowScratch {
   float* gXp = nullptr;       // B x 128
   uint16_t* tA = nullptr;     // transposes for the weight gradients (128 or H) x B
   uint16_t* tB = nullptr;
-  float* dW = nullptr;        // H x 128 weight-gradient tile
   float* x = nullptr;         // B x P work positions
   float* g = nullptr;         // B x P work gradient
+  float* aux = nullptr;       // B x P per-element terms of the base-parameter gradients
   float* logdet = nullptr;    // B
-  float* colsum = nullptr;    // 2 x P base-parameter gradient sums
+  float* colsum = nullptr;    // column-sum partials (64 chunks x widest row)
 };
 
 struct Flow {

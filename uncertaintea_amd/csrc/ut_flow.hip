@@ -67,19 +67,14 @@ __global__ void k_base_inv_bwd(const float* __restrict__ z, const float* __restr
   if (i >= n) return;
   g[i] = z[i] * expf(-prm[P + (int)(i % P)]);
 }
-__global__ void k_base_param_grads(const float* __restrict__ z, const float* __restrict__ prm, int P, int B,
-                                   float* __restrict__ grads) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= P) return;
-  const float e = expf(-prm[P + c]);
-  float gm = 0.0f, gl = 0.0f;
-  for (int b = 0; b < B; ++b) {
-    const float v = z[(long long)b * P + c];
-    gm -= v * e;
-    gl += 1.0f - v * v;
-  }
-  grads[c] = gm;
-  grads[P + c] = gl;
+// base-parameter gradient terms per element: -z e^-ls (mu) and 1 - z^2 (log scale)
+__global__ void k_base_param_terms(const float* __restrict__ z, const float* __restrict__ prm, int P,
+                                   float* __restrict__ tmu, float* __restrict__ tls, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float v = z[i];
+  tmu[i] = -v * expf(-prm[P + (int)(i % P)]);
+  tls[i] = 1.0f - v * v;
 }
 
 __global__ void k_gather(const float* __restrict__ x, int P, const int* __restrict__ pidx, int np, int kp,
@@ -100,100 +95,119 @@ __global__ void k_bias_tanh(const float* __restrict__ U, const float* __restrict
   hb[i] = f2bf(v);
 }
 
-// forward coupling, one thread per row
-__global__ void k_coupling_fwd(float* __restrict__ x, int P, const float* __restrict__ O, int op,
-                               const float* __restrict__ b2, const int* __restrict__ aidx, int na, float c,
-                               float* __restrict__ s_out, float* __restrict__ xs_out, float* __restrict__ logdet,
-                               int B) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+// Coupling kernels: one wavefront per row, lane j handles active coordinates
+// j, j + 64, ...; the row's sum of s is a wavefront reduction.
+__device__ inline float wave_sum_f(float v) {
+  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m);
+  return v;
+}
+
+// forward coupling: x_a <- x_a e^s + t, logdet += sum s
+__global__ __launch_bounds__(256) void k_coupling_fwd(float* __restrict__ x, int P, const float* __restrict__ O,
+                                                      int op, const float* __restrict__ b2,
+                                                      const int* __restrict__ aidx, int na, float c,
+                                                      float* __restrict__ s_out, float* __restrict__ xs_out,
+                                                      float* __restrict__ logdet, int B) {
+  const long long b = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
   if (b >= B) return;
   float sum = 0.0f;
-  for (int j = 0; j < na; ++j) {
-    const float s = c * tanhf((O[(long long)b * op + j] + b2[j]) / c);
-    const float t = O[(long long)b * op + na + j] + b2[na + j];
-    const long long ia = (long long)b * P + aidx[j];
+  for (int j = lane; j < na; j += 64) {
+    const float s = c * tanhf((O[b * op + j] + b2[j]) / c);
+    const float t = O[b * op + na + j] + b2[na + j];
+    const long long ia = b * P + aidx[j];
     const float xs = x[ia] * expf(s);
     x[ia] = xs + t;
-    s_out[(long long)b * na + j] = s;
-    xs_out[(long long)b * na + j] = xs;
+    s_out[b * na + j] = s;
+    xs_out[b * na + j] = xs;
     sum += s;
   }
-  logdet[b] += sum;
+  sum = wave_sum_f(sum);
+  if (lane == 0) logdet[b] += sum;
 }
 
 // inverse coupling: x_a <- (x_a - t) e^-s, logdet_inv -= sum s
-__global__ void k_coupling_inv(float* __restrict__ x, int P, const float* __restrict__ O, int op,
-                               const float* __restrict__ b2, const int* __restrict__ aidx, int na, float c,
-                               float* __restrict__ s_out, float* __restrict__ xo_out, float* __restrict__ logdet,
-                               int B) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void k_coupling_inv(float* __restrict__ x, int P, const float* __restrict__ O,
+                                                      int op, const float* __restrict__ b2,
+                                                      const int* __restrict__ aidx, int na, float c,
+                                                      float* __restrict__ s_out, float* __restrict__ xo_out,
+                                                      float* __restrict__ logdet, int B) {
+  const long long b = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
   if (b >= B) return;
   float sum = 0.0f;
-  for (int j = 0; j < na; ++j) {
-    const float s = c * tanhf((O[(long long)b * op + j] + b2[j]) / c);
-    const float t = O[(long long)b * op + na + j] + b2[na + j];
-    const long long ia = (long long)b * P + aidx[j];
+  for (int j = lane; j < na; j += 64) {
+    const float s = c * tanhf((O[b * op + j] + b2[j]) / c);
+    const float t = O[b * op + na + j] + b2[na + j];
+    const long long ia = b * P + aidx[j];
     const float xo = (x[ia] - t) * expf(-s);
     x[ia] = xo;
-    s_out[(long long)b * na + j] = s;
-    xo_out[(long long)b * na + j] = xo;
+    s_out[b * na + j] = s;
+    xo_out[b * na + j] = xo;
     sum += s;
   }
-  logdet[b] -= sum;
+  sum = wave_sum_f(sum);
+  if (lane == 0) logdet[b] -= sum;
 }
 
-// backward of the forward coupling (objective includes + sum s); writes dO and
-// updates g on the active coordinates
-__global__ void k_coupling_fwd_bwd(float* __restrict__ g, int P, const float* __restrict__ s_in,
-                                   const float* __restrict__ xs_in, const int* __restrict__ aidx, int na, int op,
-                                   float c, bf16_t* __restrict__ gO, float* __restrict__ gOf, int B) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+// backward of the forward coupling (objective includes + sum s); writes dO (and
+// pads) and updates g on the active coordinates
+__global__ __launch_bounds__(256) void k_coupling_fwd_bwd(float* __restrict__ g, int P,
+                                                          const float* __restrict__ s_in,
+                                                          const float* __restrict__ xs_in,
+                                                          const int* __restrict__ aidx, int na, int op, float c,
+                                                          bf16_t* __restrict__ gO, float* __restrict__ gOf, int B) {
+  const long long b = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
   if (b >= B) return;
-  for (int j = 0; j < na; ++j) {
-    const long long ia = (long long)b * P + aidx[j];
+  for (int j = lane; j < na; j += 64) {
+    const long long ia = b * P + aidx[j];
     const float ga = g[ia];
-    const float s = s_in[(long long)b * na + j];
-    const float gs = ga * xs_in[(long long)b * na + j] + 1.0f;
+    const float s = s_in[b * na + j];
+    const float gs = ga * xs_in[b * na + j] + 1.0f;
     const float r = s / c;
     const float gos = gs * (1.0f - r * r);
-    gO[(long long)b * op + j] = f2bf(gos);
-    gO[(long long)b * op + na + j] = f2bf(ga);
+    gO[b * op + j] = f2bf(gos);
+    gO[b * op + na + j] = f2bf(ga);
     if (gOf) {
-      gOf[(long long)b * op + j] = gos;
-      gOf[(long long)b * op + na + j] = ga;
+      gOf[b * op + j] = gos;
+      gOf[b * op + na + j] = ga;
     }
     g[ia] = ga * expf(s);
   }
-  for (int j = 2 * na; j < op; ++j) {
-    gO[(long long)b * op + j] = 0;
-    if (gOf) gOf[(long long)b * op + j] = 0.0f;
+  for (int j = 2 * na + lane; j < op; j += 64) {
+    gO[b * op + j] = 0;
+    if (gOf) gOf[b * op + j] = 0.0f;
   }
 }
 
 // backward of the inverse coupling for the loss 0.5|z|^2 + sum s
-__global__ void k_coupling_inv_bwd(float* __restrict__ g, int P, const float* __restrict__ s_in,
-                                   const float* __restrict__ xo_in, const int* __restrict__ aidx, int na, int op,
-                                   float c, bf16_t* __restrict__ gO, float* __restrict__ gOf, int B) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void k_coupling_inv_bwd(float* __restrict__ g, int P,
+                                                          const float* __restrict__ s_in,
+                                                          const float* __restrict__ xo_in,
+                                                          const int* __restrict__ aidx, int na, int op, float c,
+                                                          bf16_t* __restrict__ gO, float* __restrict__ gOf, int B) {
+  const long long b = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
   if (b >= B) return;
-  for (int j = 0; j < na; ++j) {
-    const long long ia = (long long)b * P + aidx[j];
+  for (int j = lane; j < na; j += 64) {
+    const long long ia = b * P + aidx[j];
     const float go = g[ia];
-    const float s = s_in[(long long)b * na + j];
+    const float s = s_in[b * na + j];
     const float e = expf(-s);
     const float gt = -go * e;
-    const float gs = -go * xo_in[(long long)b * na + j] + 1.0f;
+    const float gs = -go * xo_in[b * na + j] + 1.0f;
     const float r = s / c;
     const float gos = gs * (1.0f - r * r);
-    gO[(long long)b * op + j] = f2bf(gos);
-    gO[(long long)b * op + na + j] = f2bf(gt);
-    gOf[(long long)b * op + j] = gos;
-    gOf[(long long)b * op + na + j] = gt;
+    gO[b * op + j] = f2bf(gos);
+    gO[b * op + na + j] = f2bf(gt);
+    gOf[b * op + j] = gos;
+    gOf[b * op + na + j] = gt;
     g[ia] = go * e;
   }
-  for (int j = 2 * na; j < op; ++j) {
-    gO[(long long)b * op + j] = 0;
-    gOf[(long long)b * op + j] = 0.0f;
+  for (int j = 2 * na + lane; j < op; j += 64) {
+    gO[b * op + j] = 0;
+    gOf[b * op + j] = 0.0f;
   }
 }
 
@@ -215,12 +229,22 @@ __global__ void k_scatter_add(float* __restrict__ g, int P, const float* __restr
   g[b * P + pidx[k]] += gXp[b * kp + k];
 }
 
-// out[col] = sum over rows (fixed order)
-__global__ void k_colsum(const float* __restrict__ M, int B, int n, float* __restrict__ out) {
+// column sums in two passes (fixed order): COLSUM_CHUNKS row chunks, then chunks
+constexpr int COLSUM_CHUNKS = 64;
+__global__ void k_colsum_part(const float* __restrict__ M, int B, int n, float* __restrict__ part) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  const int per = (B + COLSUM_CHUNKS - 1) / COLSUM_CHUNKS;
+  const int r0 = blockIdx.y * per, r1 = min(B, r0 + per);
+  float s = 0.0f;
+  for (int b = r0; b < r1; ++b) s += M[(long long)b * n + c];
+  part[(long long)blockIdx.y * n + c] = s;
+}
+__global__ void k_colsum_final(const float* __restrict__ part, int n, float* __restrict__ out) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n) return;
   float s = 0.0f;
-  for (int b = 0; b < B; ++b) s += M[(long long)b * n + c];
+  for (int k = 0; k < COLSUM_CHUNKS; ++k) s += part[(long long)k * n + c];
   out[c] = s;
 }
 
@@ -277,16 +301,18 @@ __global__ void k_sub_logdet(double* __restrict__ lj, const float* __restrict__ 
   if (b < B) lj[b] = lj[b] - (double)logdet[b];
 }
 
-__global__ void k_row_loss(const float* __restrict__ z, const float* __restrict__ logdet_inv, int B, int P,
-                           double* __restrict__ out) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void k_row_loss(const float* __restrict__ z, const float* __restrict__ logdet_inv,
+                                                  int B, int P, double* __restrict__ out) {
+  const long long b = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
   if (b >= B) return;
   double s = 0.0;
-  for (int c = 0; c < P; ++c) {
-    const double v = z[(long long)b * P + c];
+  for (int c = lane; c < P; c += 64) {
+    const double v = z[b * P + c];
     s += v * v;
   }
-  out[b] = 0.5 * s - (double)logdet_inv[b] + 0.5 * P * 1.8378770664093453;
+  for (int m = 32; m >= 1; m >>= 1) s += __shfl_xor(s, m);
+  if (lane == 0) out[b] = 0.5 * s - (double)logdet_inv[b] + 0.5 * P * 1.8378770664093453;
 }
 
 }  // namespace
@@ -391,7 +417,9 @@ hipError_t flow_scratch(Flow& f, int B) {
   if (e == hipSuccess) e = own(f, &s.tB, (size_t)B * wide);
   if (e == hipSuccess) e = own(f, &s.x, (size_t)B * f.P);
   if (e == hipSuccess) e = own(f, &s.g, (size_t)B * f.P);
+  if (e == hipSuccess) e = own(f, &s.aux, (size_t)B * f.P);
   if (e == hipSuccess) e = own(f, &s.logdet, (size_t)B);
+  if (e == hipSuccess) e = own(f, &s.colsum, (size_t)COLSUM_CHUNKS * std::max<size_t>(wide, (size_t)f.P));
   return e;
 }
 
@@ -404,6 +432,11 @@ hipError_t flow_sync_weights(Flow& f, hipStream_t st) {
                        f.H, ly.W2b, ly.W2Tb);
   }
   return hipGetLastError();
+}
+
+static void colsum(const float* M, int B, int n, float* out, float* part, hipStream_t st) {
+  hipLaunchKernelGGL(k_colsum_part, dim3(nblk(n), COLSUM_CHUNKS), dim3(T256), 0, st, M, B, n, part);
+  hipLaunchKernelGGL(k_colsum_final, dim3(nblk(n)), dim3(T256), 0, st, part, n, out);
 }
 
 // conditioner of layer l on the current rows of sc.x: sc.O = h W2^T (bias added by the coupling)
@@ -443,8 +476,8 @@ static hipError_t conditioner_bwd(Flow& f, int l, int B, bool weights, hipStream
   hipLaunchKernelGGL(k_transpose_bf16, dim3(kp / 32, B / 32), dim3(T256), 0, st, s.xp[l], B, kp, s.tB);
   e = launch_gemm_nt(s.tA, B, s.tB, B, f.grads + ly.oW1, kp, H, kp, B, 1, 0, st);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_colsum, dim3(nblk(H)), dim3(T256), 0, st, s.gUf, B, H, f.grads + ly.ob1);
-  hipLaunchKernelGGL(k_colsum, dim3(nblk(op)), dim3(T256), 0, st, s.gOf, B, op, f.grads + ly.ob2);
+  colsum(s.gUf, B, H, f.grads + ly.ob1, s.colsum, st);
+  colsum(s.gOf, B, op, f.grads + ly.ob2, s.colsum, st);
   return hipGetLastError();
 }
 
@@ -458,7 +491,7 @@ hipError_t flow_forward(Flow& f, const float* z, float* q, float* logdet, int B,
     FlowLayer& ly = f.layers[l];
     hipError_t e = conditioner(f, l, B, st);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_coupling_fwd, dim3(nblk(B)), dim3(T256), 0, st, s.x, f.P, s.O,
+    hipLaunchKernelGGL(k_coupling_fwd, dim3((B + 3) / 4), dim3(T256), 0, st, s.x, f.P, s.O,
                        round_up(2 * ly.na, FLOW_PAD), f.params + ly.ob2, ly.aidx, ly.na, f.clamp, s.s[l], s.xa[l],
                        s.logdet, B);
   }
@@ -484,7 +517,7 @@ hipError_t flow_backward(Flow& f, const float* g_q, float* g_z, int B, hipStream
   for (int l = f.L - 1; l >= 0; --l) {
     FlowLayer& ly = f.layers[l];
     const int op = round_up(2 * ly.na, FLOW_PAD);
-    hipLaunchKernelGGL(k_coupling_fwd_bwd, dim3(nblk(B)), dim3(T256), 0, st, s.g, f.P, s.s[l], s.xa[l], ly.aidx,
+    hipLaunchKernelGGL(k_coupling_fwd_bwd, dim3((B + 3) / 4), dim3(T256), 0, st, s.g, f.P, s.s[l], s.xa[l], ly.aidx,
                        ly.na, op, f.clamp, s.gO, (float*)nullptr, B);
     hipError_t e = conditioner_bwd(f, l, B, false, st);
     if (e != hipSuccess) return e;
@@ -510,7 +543,7 @@ hipError_t flow_inverse(Flow& f, const float* q, float* z, float* logdet_inv, in
     FlowLayer& ly = f.layers[l];
     hipError_t e = conditioner(f, l, B, st);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_coupling_inv, dim3(nblk(B)), dim3(T256), 0, st, s.x, f.P, s.O,
+    hipLaunchKernelGGL(k_coupling_inv, dim3((B + 3) / 4), dim3(T256), 0, st, s.x, f.P, s.O,
                        round_up(2 * ly.na, FLOW_PAD), f.params + ly.ob2, ly.aidx, ly.na, f.clamp, s.s[l], s.xa[l],
                        s.logdet, B);
   }
@@ -527,11 +560,17 @@ hipError_t flow_loss_backward(Flow& f, const float* z, int B, hipStream_t st) {
   FlowScratch& s = f.sc;
   const long long n = (long long)B * f.P;
   hipLaunchKernelGGL(k_base_inv_bwd, dim3(nblk(n)), dim3(T256), 0, st, z, f.params, f.P, s.g, n);
-  hipLaunchKernelGGL(k_base_param_grads, dim3(nblk(f.P)), dim3(T256), 0, st, z, f.params, f.P, B, f.grads);
+  // mu and log-scale gradients: per-element terms summed over rows (s.x is free
+  // once the inverse is done: the backward reads the per-layer activations)
+  float* tmu = s.x;
+  float* tls = s.aux;
+  hipLaunchKernelGGL(k_base_param_terms, dim3(nblk(n)), dim3(T256), 0, st, z, f.params, f.P, tmu, tls, n);
+  colsum(tmu, B, f.P, f.grads, s.colsum, st);
+  colsum(tls, B, f.P, f.grads + f.P, s.colsum, st);
   for (int l = 0; l < f.L; ++l) {
     FlowLayer& ly = f.layers[l];
     const int op = round_up(2 * ly.na, FLOW_PAD);
-    hipLaunchKernelGGL(k_coupling_inv_bwd, dim3(nblk(B)), dim3(T256), 0, st, s.g, f.P, s.s[l], s.xa[l], ly.aidx,
+    hipLaunchKernelGGL(k_coupling_inv_bwd, dim3((B + 3) / 4), dim3(T256), 0, st, s.g, f.P, s.s[l], s.xa[l], ly.aidx,
                        ly.na, op, f.clamp, s.gO, s.gOf, B);
     hipError_t e = conditioner_bwd(f, l, B, true, st);
     if (e != hipSuccess) return e;
@@ -564,7 +603,7 @@ hipError_t flow_sub_logdet(double* lj, const float* logdet, int B, hipStream_t s
 }
 
 hipError_t flow_loss_value(const float* z, const float* logdet_inv, int B, int P, double* out, hipStream_t st) {
-  hipLaunchKernelGGL(k_row_loss, dim3(nblk(B)), dim3(T256), 0, st, z, logdet_inv, B, P, out);
+  hipLaunchKernelGGL(k_row_loss, dim3((B + 3) / 4), dim3(T256), 0, st, z, logdet_inv, B, P, out);
   return hipGetLastError();
 }
 
