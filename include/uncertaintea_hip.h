@@ -229,6 +229,31 @@ int ut_chees_run(ut_workspace* ws, const ut_chees_opts* opts, ut_chain_out* out,
  * the matrix-core GLM plan and flow-preconditioned workspaces.                  */
 int ut_hmc_shared_run(ut_workspace* ws, const ut_sampler_opts* opts, ut_chain_out* out);
 
+/* MEADS: generalized HMC with one leapfrog step per iteration, persistent
+ * partially refreshed momentum and a persistent-slice Metropolis test; step size,
+ * damping, slice drift and a diagonal preconditioner come every iteration from the
+ * neighbouring fold's ensemble statistics (K complementary folds, a rotating
+ * skipped fold) -- reference batched_meads, src/inference/api_batched_meads.jl:
+ * 170-547.  No warmup adaptation phase: num_warmup only discards draws.  The fold
+ * reductions (Gram trace-ratio eigenvalue estimates) run on the device in
+ * binary64.  adapt_trace, if not NULL, receives per iteration the mean step, mean
+ * damping and largest standard deviation over folds (3 x (num_warmup +
+ * num_samples), the reference `_adapt_trace`).  out->step_size gets the final
+ * mean step for every chain; out->inverse_mass the last preconditioner of fold 0. */
+typedef struct ut_meads_opts {
+  int32_t num_samples;
+  int32_t num_warmup;
+  int32_t num_folds;              /* 4 */
+  int32_t init_strategy;          /* UT_INIT_PRIOR */
+  int32_t init_max_retries;       /* 100 */
+  double divergence_threshold;    /* 1000 */
+  const double* initial_params;   /* P x C or NULL */
+  uint64_t seed;
+} ut_meads_opts;
+
+void ut_meads_opts_default(ut_meads_opts* opts);
+int ut_meads_run(ut_workspace* ws, const ut_meads_opts* opts, ut_chain_out* out, double* adapt_trace);
+
 /* ---- flow preconditioner ----------------------------------------------------------
  * An affine-coupling normalising flow q = f(z) (the reference's FlowGuide
  * structure, src/inference/vi/guides.jl:1-21,282-369: elementwise affine base

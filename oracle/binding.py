@@ -42,6 +42,8 @@ def _load():
         "orc_dual_averaging": (None, [dbl, dbl, vp, i32, vp, vp]),
         "orc_running_variance": (dbl, [vp, vp, vp, i32, i32, i32, i32, i32, dbl, vp]),
         "orc_chees_run": (ct.c_int, [vp, i32, vp, i32, i32, vp, vp, vp, i32]),
+        "orc_meads_run": (ct.c_int, [vp, i32, vp, i32, i32, vp, vp, vp]),
+        "orc_meads_max_eig": (dbl, [vp, i32, i32]),
         "orc_flow_forward": (None, [i32, i32, i32, dbl, i32, vp, vp, i64, vp, vp, vp]),
         "orc_flow_inverse": (None, [i32, i32, i32, dbl, i32, vp, vp, i64, vp, vp]),
         "orc_flow_loss_grad": (dbl, [i32, i32, i32, dbl, i32, vp, vp, i64, vp]),
@@ -145,8 +147,8 @@ class OracleModel:
             step_size: float = 0.1, max_tree_depth: int = 10, num_leapfrog_steps: int = 10,
             initial_params=None, init_strategy: str = "prior", target_accept=None, seed: int = 0,
             adapt_step_size=True, adapt_mass_matrix=True, find_reasonable_step_size=False, threads: int = 0,
-            stop_after: int = -1, divergence_threshold=1000.0, chees=None):
-        """Run the oracle sampler.  kind 'nuts'|'hmc'|'hmc_shared'|'chees'; mode 'twin' (float32,
+            stop_after: int = -1, divergence_threshold=1000.0, chees=None, num_folds: int = 4):
+        """Run the oracle sampler.  kind 'nuts'|'hmc'|'hmc_shared'|'chees'|'meads'; mode 'twin' (float32,
         kernel-exact) or 'f64' (binary64 reference formulas).  `chees` holds the
         ChEES keyword arguments (reference names).  Returns a dict."""
         from uncertaintea_amd import _lib as UL  # struct layouts only
@@ -188,7 +190,19 @@ class OracleModel:
         out.step_size = step.ctypes.data_as(ct.POINTER(ct.c_double))
         out.inverse_mass = im.ctypes.data_as(ct.POINTER(ct.c_double))
         trace = None
-        if lockstep:
+        if kind == "meads":
+            mo = UL.MeadsOpts()
+            mo.num_samples, mo.num_warmup = num_samples, num_warmup
+            mo.num_folds = int(num_folds)
+            mo.init_strategy = o.init_strategy
+            mo.init_max_retries = 100
+            mo.divergence_threshold = divergence_threshold
+            mo.initial_params = o.initial_params
+            mo.seed = seed
+            trace = np.zeros((num_warmup + num_samples, 3))
+            rc = _load().orc_meads_run(self.h, 0 if mode == "twin" else 1, ct.byref(mo), C, threads, _p(samples),
+                                       ct.byref(out), _p(trace))
+        elif lockstep:
             co = UL.CheesOpts()
             co.base = o  # ChEES defaults restated from the reference's keyword defaults
             kw = dict(jitter_amount=1.0, adapt_trajectory_length=True, max_leapfrog_steps=1000,
@@ -312,3 +326,10 @@ class OracleFlow:
         loss = _load().orc_flow_loss_grad(self.P, self.L, self.H, self.clamp, int(self.bf16), _p(self.w), _p(q),
                                           q.shape[0], _p(g))
         return float(loss), g
+
+
+def meads_max_eig(columns):
+    """Trace-ratio largest-eigenvalue estimate of a P x N ensemble (columns = members)."""
+    x = np.ascontiguousarray(np.asarray(columns, dtype=np.float64).T)
+    n, P = x.shape
+    return float(_load().orc_meads_max_eig(_p(x), n, P))

@@ -1397,6 +1397,264 @@ int run_chees(const Pol& pol, const ut_chees_opts* co, int C, int threads, void*
   return UT_OK;
 }
 
+// ------------------------------------------------------------------ MEADS
+// Restates src/inference/api_batched_meads.jl (constants :32-34, _meads_max_eig
+// :48-77, _meads_fold_parameters! :89-136, slice drift :141, iteration :272-470)
+// with the device's reduction orders (member norms and off-diagonal squared Gram
+// entries per member, then members in order) and Philox draws.
+enum { S_MEADS_REFRESH = 7, S_MEADS_INIT = 8 };
+
+double meads_max_eig(const std::vector<double>& x, int n, int P) {
+  if (n < 2) return 0.0;
+  std::vector<double> norms(n), rows(n);
+  for (int a = 0; a < n; ++a) {
+    const double* xa = &x[(size_t)a * P];
+    double nn = 0.0;
+    for (int p = 0; p < P; ++p) nn += xa[p] * xa[p];
+    norms[a] = nn;
+    double r = 0.0;
+    for (int b = 0; b < n; ++b) {
+      if (b == a) continue;
+      const double* xb = &x[(size_t)b * P];
+      double g = 0.0;
+      for (int p = 0; p < P; ++p) g += xa[p] * xb[p];
+      r += g * g;
+    }
+    rows[a] = r;
+  }
+  double trace = 0.0, off = 0.0;
+  for (int a = 0; a < n; ++a) trace += norms[a];
+  for (int a = 0; a < n; ++a) off += rows[a];
+  const double lb = trace / (double)n;
+  if (!(lb > 0.0)) return 0.0;
+  return (off / ((double)n * (double)(n - 1))) / lb;
+}
+
+template <class Pol>
+int run_meads(const Pol& pol, const ut_meads_opts* mo, int C, int threads, void* samples, ut_chain_out* out,
+              double* trace) {
+  using T = typename Pol::T;
+  constexpr bool F32 = std::is_same<T, float>::value;
+  const Model& m = *pol.m;
+  const int P = m.P, S = mo->num_samples, W = mo->num_warmup, K = mo->num_folds;
+  const uint32_t rs = run_seed_of(mo->seed);
+  if (threads > 0) omp_set_num_threads(threads);
+  const size_t PC = (size_t)P * C;
+  std::vector<double> pos(PC);
+  if (mo->initial_params) std::memcpy(pos.data(), mo->initial_params, PC * 8);
+  else
+    for (int c = 0; c < C; ++c) draw_initial_column(m, mo->init_strategy, (uint32_t)c, rs, &pos[(size_t)c * P]);
+  std::vector<T> q(PC), g(PC), lj(C);
+  for (int attempt = 0;; ++attempt) {
+    std::vector<int> bad;
+    for (int c = 0; c < C; ++c) {
+      for (int a = 0; a < P; ++a) q[(size_t)c * P + a] = F32 ? (T)(float)pos[(size_t)c * P + a] : (T)pos[(size_t)c * P + a];
+      lj[c] = pol.ls_eval(&q[(size_t)c * P], &g[(size_t)c * P]);
+      bool ok = std::isfinite(lj[c]);
+      for (int a = 0; a < P; ++a) ok = ok && std::isfinite(g[(size_t)c * P + a]);
+      if (!ok) bad.push_back(c);
+    }
+    if (bad.empty()) break;
+    if (mo->initial_params || attempt >= mo->init_max_retries) return UT_E_NONFINITE_INIT;
+    for (int c : bad)
+      draw_initial_column(m, mo->init_strategy, (uint32_t)c, rs + (uint32_t)(attempt + 1), &pos[(size_t)c * P]);
+  }
+  std::vector<int> start(K + 1, 0), fold_of(C);
+  const int base = C / K, rem = C % K;
+  for (int k = 0; k < K; ++k) start[k + 1] = start[k] + base + (k < rem ? 1 : 0);
+  for (int k = 0; k < K; ++k)
+    for (int c = start[k]; c < start[k + 1]; ++c) fold_of[c] = k;
+  std::vector<T> mom(PC), refreshed(PC), wq(PC), wp(PC), wg(PC), wlj(C);
+  std::vector<double> slice(C), h0(C), ap(C), err(C);
+  std::vector<char> valid(C), div(C), acc(C);
+  for (int c = 0; c < C; ++c) {
+    for (int a = 0; a < P; ++a) mom[(size_t)c * P + a] = (T)rand_normal_f((uint32_t)c, rs, S_MEADS_INIT, (uint32_t)a);
+    slice[c] = 2.0 * rand_uniform_d((uint32_t)c, rs, S_MEADS_INIT, (uint32_t)P) - 1.0;
+  }
+  std::vector<double> sigma((size_t)K * P), fold((size_t)4 * K);
+  std::vector<T> sigma_t((size_t)K * P), im_t((size_t)K * P);
+  double final_step = 0.0;
+  for (int it = 1; it <= W + S; ++it) {
+    const uint32_t itc = rs + (uint32_t)it;
+    const int skip = (it - 1) % K;
+    for (int k = 0; k < K; ++k) {
+      const int nb = (k - 1 + K) % K;
+      const int c0 = start[nb], n = start[nb + 1] - c0;
+      std::vector<double> wpos((size_t)n * P), wgrd((size_t)n * P);
+      double mx = 0.0;
+      for (int p = 0; p < P; ++p) {
+        double mean = 0.0;
+        for (int j = 0; j < n; ++j) mean += (double)q[(size_t)(c0 + j) * P + p];
+        mean /= (double)n;
+        double var = 0.0;
+        for (int j = 0; j < n; ++j) {
+          const double d = (double)q[(size_t)(c0 + j) * P + p] - mean;
+          var += d * d;
+        }
+        var /= (double)n;
+        double sd = sqrt_d(var);
+        sd = sd > 1e-10 ? sd : 1e-10;
+        sigma[(size_t)k * P + p] = sd;
+        sigma_t[(size_t)k * P + p] = F32 ? (T)(float)sd : (T)sd;
+        im_t[(size_t)k * P + p] = F32 ? (T)(float)(sd * sd) : (T)(sd * sd);
+        const double inv = 1.0 / sd;
+        for (int j = 0; j < n; ++j) {
+          wpos[(size_t)j * P + p] = ((double)q[(size_t)(c0 + j) * P + p] - mean) * inv;
+          wgrd[(size_t)j * P + p] = (double)g[(size_t)(c0 + j) * P + p] * sd;
+        }
+        mx = std::fmax(mx, sd);
+      }
+      const double eg = std::max(meads_max_eig(wgrd, n, P), 1e-12);
+      const double ep = std::max(meads_max_eig(wpos, n, P), 1e-12);
+      double step = 0.5 / sqrt_d(eg);
+      step = step < 1.0 ? step : 1.0;
+      const double floor_g = 1.0 / ((double)it * step);
+      const double inv_p = 1.0 / sqrt_d(ep);
+      const double gamma = floor_g > inv_p ? floor_g : inv_p;
+      double alpha = 1.0 - exp_d(-2.0 * step * gamma);
+      alpha = alpha < 0.0 ? 0.0 : (alpha > 1.0 ? 1.0 : alpha);
+      fold[4 * k + 0] = step;
+      fold[4 * k + 1] = alpha;
+      fold[4 * k + 2] = alpha / 2.0;
+      fold[4 * k + 3] = mx;
+    }
+    double ssum = 0.0, asum = 0.0, smax = 0.0;
+    for (int k = 0; k < K; ++k) {
+      ssum += fold[4 * k];
+      asum += fold[4 * k + 1];
+      smax = std::max(smax, fold[4 * k + 3]);
+    }
+    final_step = ssum / K;
+    if (trace) {
+      trace[3 * (size_t)(it - 1) + 0] = ssum / K;
+      trace[3 * (size_t)(it - 1) + 1] = asum / K;
+      trace[3 * (size_t)(it - 1) + 2] = smax;
+    }
+    // refresh + half kick + drift
+#pragma omp parallel for schedule(static)
+    for (int c = 0; c < C; ++c) {
+      const int f = fold_of[c];
+      const double alpha = fold[4 * f + 1];
+      const T keep = F32 ? (T)(float)sqrt_d(1.0 - alpha > 0.0 ? 1.0 - alpha : 0.0) : (T)sqrt_d(std::max(1.0 - alpha, 0.0));
+      const T refresh = F32 ? (T)(float)sqrt_d(alpha > 0.0 ? alpha : 0.0) : (T)sqrt_d(std::max(alpha, 0.0));
+      const T eps = F32 ? (T)(float)fold[4 * f] : (T)fold[4 * f];
+      T* pc = &wp[(size_t)c * P];
+      T* qc = &wq[(size_t)c * P];
+      const T* im = &im_t[(size_t)f * P];
+      bool fin = true;
+      for (int a = 0; a < P; ++a) {
+        const T xi = draw_normal<T>((uint32_t)c, itc, S_MEADS_REFRESH, (uint32_t)a);
+        const T mt = keep * mom[(size_t)c * P + a] + refresh * xi;
+        refreshed[(size_t)c * P + a] = mt;
+        pc[a] = mt / sigma_t[(size_t)f * P + a];
+        qc[a] = q[(size_t)c * P + a];
+        fin = fin && std::isfinite(g[(size_t)c * P + a]);
+      }
+      h0[c] = pol.ls_kinetic(pc, im) - (double)lj[c];
+      valid[c] = fin;
+      if (fin) {
+        const T half = T(0.5) * eps;
+        for (int a = 0; a < P; ++a) {
+          pc[a] = pc[a] + half * g[(size_t)c * P + a];
+          qc[a] = qc[a] + (eps * im[a]) * pc[a];
+        }
+      }
+    }
+#pragma omp parallel for schedule(static)
+    for (int c = 0; c < C; ++c) wlj[c] = pol.ls_eval(&wq[(size_t)c * P], &wg[(size_t)c * P]);
+    // final half kick, flip, persistent-slice accept
+#pragma omp parallel for schedule(static)
+    for (int c = 0; c < C; ++c) {
+      const int f = fold_of[c];
+      const T eps = F32 ? (T)(float)fold[4 * f] : (T)fold[4 * f];
+      const double delta = fold[4 * f + 2];
+      T* pc = &wp[(size_t)c * P];
+      const T* gc = &wg[(size_t)c * P];
+      const T* im = &im_t[(size_t)f * P];
+      bool fin = true;
+      for (int a = 0; a < P; ++a) fin = fin && std::isfinite(gc[a]);
+      const bool ok = valid[c] && fin && std::isfinite(wlj[c]);
+      if (ok) {
+        const T half = T(0.5) * eps;
+        for (int a = 0; a < P; ++a) pc[a] = -(pc[a] + half * gc[a]);
+      }
+      double a_p = 0.0, e = inf_d(), lar = -inf_d();
+      bool d = true;
+      if (ok) {
+        const double H1 = pol.ls_kinetic(pc, im) - (double)wlj[c];
+        lar = h0[c] - H1;
+        e = H1 - h0[c];
+        d = !std::isfinite(e) || e > mo->divergence_threshold;
+        a_p = !std::isfinite(lar) ? 0.0 : (lar >= 0.0 ? 1.0 : exp_d(lar));
+      }
+      ap[c] = a_p;
+      err[c] = e;
+      div[c] = d;
+      acc[c] = false;
+      if (f == skip) continue;
+      const double x = slice[c] + 1.0 + delta;
+      const double drifted = (x - 2.0 * std::floor(x / 2.0)) - 1.0;
+      const bool moved = ok && log_d(std::fabs(drifted)) <= lar;
+      acc[c] = moved;
+      for (int a = 0; a < P; ++a) {
+        if (moved) {
+          q[(size_t)c * P + a] = wq[(size_t)c * P + a];
+          g[(size_t)c * P + a] = gc[a];
+          mom[(size_t)c * P + a] = -pc[a] * sigma_t[(size_t)f * P + a];
+        } else {
+          mom[(size_t)c * P + a] = -refreshed[(size_t)c * P + a];
+        }
+      }
+      if (moved) {
+        lj[c] = wlj[c];
+        slice[c] = drifted * exp_d(e);
+      } else {
+        slice[c] = drifted;
+      }
+    }
+    if (it <= W) continue;
+    const int s = it - W - 1;
+    for (int c = 0; c < C; ++c) {
+      const size_t row = (size_t)c * S + s;
+      const bool skipped = fold_of[c] == skip;
+      if (samples)
+        for (int a = 0; a < P; ++a) {
+          if (F32) ((float*)samples)[row * P + a] = (float)q[(size_t)c * P + a];
+          else ((double*)samples)[row * P + a] = (double)q[(size_t)c * P + a];
+        }
+      if (!out) continue;
+      double kin;
+      if (F32) {
+        double lane[WAVE];
+        for (int l = 0; l < WAVE; ++l) {
+          double acc2 = 0.0;
+          for (int a = l; a < P; a += WAVE) acc2 = acc2 + (double)mom[(size_t)c * P + a] * (double)mom[(size_t)c * P + a];
+          lane[l] = acc2;
+        }
+        kin = bfly(lane);
+      } else {
+        kin = 0.0;
+        for (int a = 0; a < P; ++a) kin += (double)mom[(size_t)c * P + a] * (double)mom[(size_t)c * P + a];
+      }
+      if (out->logjoint) out->logjoint[row] = (double)lj[c];
+      if (out->energy) out->energy[row] = -(double)lj[c] + 0.5 * kin;
+      if (out->accept_stat) out->accept_stat[row] = skipped ? 1.0 : ap[c];
+      if (out->accepted) out->accepted[row] = skipped ? 1 : (acc[c] ? 1 : 0);
+      if (out->divergent) out->divergent[row] = skipped ? 0 : (div[c] ? 1 : 0);
+      if (out->energy_error) out->energy_error[row] = skipped ? 0.0 : err[c];
+      if (out->tree_depth) out->tree_depth[row] = 0;
+      if (out->integration_steps) out->integration_steps[row] = skipped ? 0 : 1;
+    }
+  }
+  if (out) {
+    if (out->step_size)
+      for (int c = 0; c < C; ++c) out->step_size[c] = final_step;
+    if (out->inverse_mass)
+      for (int a = 0; a < P; ++a) out->inverse_mass[a] = sigma[a] * sigma[a];
+  }
+  return UT_OK;
+}
+
 // ------------------------------------------------------------------ flow
 // Affine-coupling flow, binary64 restatement of the device flow (structure of the
 // reference FlowGuide, src/inference/vi/guides.jl:282-369, with MLP conditioners):
@@ -1775,6 +2033,19 @@ int orc_chees_run(void* h, int32_t mode, const ut_chees_opts* co, int32_t C, int
   const Model* m = (const Model*)h;
   if (mode == 0) return run_chees(TwinF32{m}, co, C, threads, samples, out, trace, hmc != 0);
   return run_chees(RefF64{m}, co, C, threads, samples, out, trace, hmc != 0);
+}
+
+// MEADS run.  mode 0: TwinF32 (kernel-exact); mode 1: RefF64.
+int orc_meads_run(void* h, int32_t mode, const ut_meads_opts* mo, int32_t C, int32_t threads, void* samples,
+                  ut_chain_out* out, double* trace) {
+  const Model* m = (const Model*)h;
+  if (mode == 0) return run_meads(TwinF32{m}, mo, C, threads, samples, out, trace);
+  return run_meads(RefF64{m}, mo, C, threads, samples, out, trace);
+}
+// the trace-ratio eigenvalue estimate on an n x P member-major ensemble
+double orc_meads_max_eig(const double* x, int32_t n, int32_t P) {
+  std::vector<double> v(x, x + (size_t)n * P);
+  return meads_max_eig(v, n, P);
 }
 
 // ChEES scalar pieces for known-answer tests: Halton values, step counts and a

@@ -32,7 +32,7 @@ EXPORTED = (
     "ut_workspace_create", "ut_workspace_destroy",
     "ut_logjoint", "ut_logjoint_gradient",
     "ut_sampler_opts_default", "ut_nuts_run", "ut_hmc_run", "ut_chees_opts_default", "ut_chees_run",
-    "ut_hmc_shared_run", "ut_flow_create", "ut_flow_destroy", "ut_flow_num_weights", "ut_flow_layer_layout",
+    "ut_hmc_shared_run", "ut_meads_opts_default", "ut_meads_run", "ut_flow_create", "ut_flow_destroy", "ut_flow_num_weights", "ut_flow_layer_layout",
     "ut_flow_get_weights", "ut_flow_set_weights", "ut_flow_set_base", "ut_flow_train", "ut_flow_forward",
     "ut_flow_inverse", "ut_flow_vjp", "ut_flow_loss_grad", "ut_workspace_set_flow",
     "ut_philox4x32", "ut_philox_fill", "ut_host_alloc", "ut_host_free", "ut_gemm_bf16",
@@ -72,6 +72,19 @@ class CheesOpts(ct.Structure):
         ("trajectory_adam_beta2", ct.c_double),
         ("trajectory_adam_epsilon", ct.c_double),
         ("forward_velocity", ct.c_int32),
+    ]
+
+
+class MeadsOpts(ct.Structure):
+    _fields_ = [
+        ("num_samples", ct.c_int32),
+        ("num_warmup", ct.c_int32),
+        ("num_folds", ct.c_int32),
+        ("init_strategy", ct.c_int32),
+        ("init_max_retries", ct.c_int32),
+        ("divergence_threshold", ct.c_double),
+        ("initial_params", ct.POINTER(ct.c_double)),
+        ("seed", ct.c_uint64),
     ]
 
 
@@ -139,6 +152,8 @@ def lib() -> ct.CDLL:
         "ut_chees_opts_default": (None, [ct.POINTER(CheesOpts)]),
         "ut_chees_run": (ct.c_int, [vp, ct.POINTER(CheesOpts), ct.POINTER(ChainOut), vp]),
         "ut_hmc_shared_run": (ct.c_int, [vp, ct.POINTER(SamplerOpts), ct.POINTER(ChainOut)]),
+        "ut_meads_opts_default": (None, [ct.POINTER(MeadsOpts)]),
+        "ut_meads_run": (ct.c_int, [vp, ct.POINTER(MeadsOpts), ct.POINTER(ChainOut), vp]),
         "ut_flow_create": (ct.c_int, [i32, i32, i32, dbl, ct.c_uint64, pp]),
         "ut_flow_destroy": (ct.c_int, [vp]),
         "ut_flow_num_weights": (ct.c_int, [vp, ct.POINTER(i64)]),

@@ -24,6 +24,7 @@
 #include "ut_kernels.h"
 #include "ut_lockstep.h"
 #include "ut_math.h"
+#include "ut_meads.h"
 
 using namespace ut;
 
@@ -1077,9 +1078,189 @@ int run_lockstep(ut_workspace* ws, const ut_chees_opts* co, ut_chain_out* out, d
   return UT_OK;
 }
 
+// ---------------------------------------------------------------- MEADS driver
+// Reference batched_meads (api_batched_meads.jl:170-547): contiguous folds, the
+// neighbour map k <- (k - 1) mod K, skipped fold (t - 1) mod K, initial persistent
+// momentum and slice from Philox (iteration coordinate run_seed), then per
+// iteration fold parameters -> refresh + half step -> gradient -> accept -> record.
+int run_meads(ut_workspace* ws, const ut_meads_opts* mo, ut_chain_out* out, double* trace) {
+  if (!ws || !mo || !out) return fail(UT_E_ARG, "batched_meads: null pointer");
+  if (!out->samples) return fail(UT_E_ARG, "batched_meads: out->samples is required");
+  if (ws->precision != UT_FLOAT32) return fail(UT_E_PRECISION, "batched_meads runs in UT_FLOAT32 (binary64 energies)");
+  const int P = ws->P, C = ws->C, S = mo->num_samples, W = mo->num_warmup, K = mo->num_folds;
+  if (K < 2) return fail(UT_E_ARG, "batched_meads num_folds must be >= 2");
+  if (C < 2 * K) return fail(UT_E_ARG, "batched_meads needs num_chains >= 2 * num_folds");
+  if (S <= 0) return fail(UT_E_ARG, "batched_meads num_samples must be > 0");
+  if (W < 0) return fail(UT_E_ARG, "batched_meads num_warmup must be >= 0");
+  if (mo->init_strategy != UT_INIT_PRIOR && mo->init_strategy != UT_INIT_UNIFORM)
+    return fail(UT_E_ARG, "batched_meads init_strategy must be UT_INIT_PRIOR or UT_INIT_UNIFORM");
+  if (mo->init_max_retries < 0) return fail(UT_E_ARG, "batched_meads init_max_retries must be >= 0");
+  if (P > 1024) return fail(UT_E_UNSUPPORTED, "batched_meads: more than 1024 parameters");
+  if (ws->flow && C % FLOW_PAD != 0)
+    return fail(UT_E_DIM, "a flow-preconditioned workspace needs num_chains a multiple of 128");
+  hipStream_t stream = ws->stream;
+  const double t_start = now_seconds();
+  const uint32_t rs = run_seed_of(mo->seed);
+  ut_sampler_opts io;
+  ut_sampler_opts_default(&io);
+  io.init_strategy = mo->init_strategy;
+  io.init_max_retries = mo->init_max_retries;
+  io.initial_params = mo->initial_params;
+  io.seed = mo->seed;
+  int rc = init_chains(ws, &io, rs);
+  if (rc) return rc;
+
+  // folds: contiguous partition, the first C mod K folds one chain larger
+  std::vector<int> start(K + 1, 0), fold_of(C);
+  const int base = C / K, rem = C % K;
+  for (int k = 0; k < K; ++k) start[k + 1] = start[k] + base + (k < rem ? 1 : 0);
+  for (int k = 0; k < K; ++k)
+    for (int c = start[k]; c < start[k + 1]; ++c) fold_of[c] = k;
+  const int nmax = base + (rem ? 1 : 0);
+
+  DeviceBuffers buf;
+  MeadsState m;
+  m.pos = ws->st.pos;
+  m.grad = ws->st.grad;
+  m.lj = ws->st.lj;
+  const size_t PC = (size_t)P * C;
+  UT_HIP(buf.alloc(&m.mom, PC));
+  UT_HIP(buf.alloc(&m.slice, (size_t)C));
+  UT_HIP(buf.alloc(&m.wq, PC));
+  UT_HIP(buf.alloc(&m.wp, PC));
+  UT_HIP(buf.alloc(&m.wg, PC));
+  UT_HIP(buf.alloc(&m.wlj, (size_t)C));
+  UT_HIP(buf.alloc(&m.refreshed, PC));
+  UT_HIP(buf.alloc(&m.h0, (size_t)C));
+  UT_HIP(buf.alloc(&m.valid, (size_t)C));
+  UT_HIP(buf.alloc(&m.sigma, (size_t)K * P));
+  UT_HIP(buf.alloc(&m.sigma_f, (size_t)K * P));
+  UT_HIP(buf.alloc(&m.white, (size_t)2 * K * nmax * P));
+  UT_HIP(buf.alloc(&m.gram_row, (size_t)2 * K * nmax));
+  UT_HIP(buf.alloc(&m.fold, (size_t)4 * K));
+  UT_HIP(buf.alloc(&m.fold_start, (size_t)K + 1));
+  int* d_fold_of = nullptr;
+  UT_HIP(buf.alloc(&d_fold_of, (size_t)C));
+  UT_HIP(hipMemcpyAsync(m.fold_start, start.data(), (K + 1) * 4, hipMemcpyHostToDevice, stream));
+  UT_HIP(hipMemcpyAsync(d_fold_of, fold_of.data(), (size_t)C * 4, hipMemcpyHostToDevice, stream));
+  MeadsRecord rec;
+  UT_HIP(buf.alloc(&rec.accept_prob, (size_t)C));
+  UT_HIP(buf.alloc(&rec.flags, (size_t)C));
+  UT_HIP(buf.alloc(&rec.energy_error, (size_t)C));
+  const int T = W + S;
+  double* d_trace = nullptr;
+  UT_HIP(buf.alloc(&d_trace, (size_t)4 * K * T));
+
+  SamplerParams sp;
+  sp.P = P;
+  sp.C = C;
+  sp.num_warmup = W;
+  sp.num_samples = S;
+  UT_HIP(buf.alloc(&sp.draws, (size_t)S * C * P));
+  UT_HIP(buf.alloc(&sp.o_lj, (size_t)S * C));
+  UT_HIP(buf.alloc(&sp.o_acc, (size_t)S * C));
+  UT_HIP(buf.alloc(&sp.o_energy, (size_t)S * C));
+  UT_HIP(buf.alloc(&sp.o_err, (size_t)S * C));
+  UT_HIP(buf.alloc(&sp.o_accepted, (size_t)S * C));
+  UT_HIP(buf.alloc(&sp.o_divergent, (size_t)S * C));
+  UT_HIP(buf.alloc(&sp.o_depth, (size_t)S * C));
+  UT_HIP(buf.alloc(&sp.o_steps, (size_t)S * C));
+
+  UT_HIP(launch_meads_init(m, P, C, rs, stream));
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  UT_HIP(hipEventCreate(&ev0));
+  UT_HIP(hipEventCreate(&ev1));
+  double t_warm = t_start;
+  for (int it = 1; it <= T; ++it) {
+    if (it == W + 1) {
+      UT_HIP(hipStreamSynchronize(stream));
+      t_warm = now_seconds();
+      UT_HIP(hipEventRecord(ev0, stream));
+    }
+    const int skip = (it - 1) % K;
+    UT_HIP(launch_meads_fold_params(m, P, K, nmax, it, stream));
+    UT_HIP(hipMemcpyAsync(d_trace + (size_t)4 * K * (it - 1), m.fold, (size_t)4 * K * 8, hipMemcpyDeviceToDevice,
+                          stream));
+    UT_HIP(launch_meads_begin(m, d_fold_of, P, C, rs + (uint32_t)it, stream));
+    UT_HIP(batched_grad(ws, m.wq, m.wlj, m.wg, true));
+    UT_HIP(launch_meads_end(m, rec, d_fold_of, P, C, skip, mo->divergence_threshold, stream));
+    UT_HIP(launch_meads_record(m, rec, d_fold_of, sp, it, skip, stream));
+  }
+  UT_HIP(hipEventRecord(ev1, stream));
+  const size_t SC = (size_t)S * C;
+  if (ws->flow) {
+    Flow& f = ws->flow->f;
+    for (size_t r = 0; r < SC; r += (size_t)C) {
+      float* rows = sp.draws + r * P;
+      UT_HIP(flow_forward(f, rows, rows, nullptr, C, stream));
+      UT_HIP(flow_sub_logdet(sp.o_lj + r, f.sc.logdet, C, stream));
+    }
+  }
+  auto copy = [&](void* dst, const void* src, size_t bytes) -> hipError_t {
+    if (!dst || bytes == 0) return hipSuccess;
+    return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream);
+  };
+  UT_HIP(copy(out->samples, sp.draws, SC * P * 4));
+  UT_HIP(copy(out->logjoint, sp.o_lj, SC * 8));
+  UT_HIP(copy(out->accept_stat, sp.o_acc, SC * 8));
+  UT_HIP(copy(out->energy, sp.o_energy, SC * 8));
+  UT_HIP(copy(out->energy_error, sp.o_err, SC * 8));
+  UT_HIP(copy(out->accepted, sp.o_accepted, SC));
+  UT_HIP(copy(out->divergent, sp.o_divergent, SC));
+  UT_HIP(copy(out->tree_depth, sp.o_depth, SC * 4));
+  UT_HIP(copy(out->integration_steps, sp.o_steps, SC * 4));
+  std::vector<double> tr((size_t)4 * K * T), sig((size_t)P);
+  UT_HIP(copy(tr.data(), d_trace, tr.size() * 8));
+  UT_HIP(copy(sig.data(), m.sigma, (size_t)P * 8));
+  UT_HIP(hipStreamSynchronize(stream));
+  const double t_end = now_seconds();
+  float kernel_ms = 0.0f;
+  UT_HIP(hipEventElapsedTime(&kernel_ms, ev0, ev1));
+  (void)hipEventDestroy(ev0);
+  (void)hipEventDestroy(ev1);
+  double final_step = 0.0;
+  for (int it = 0; it < T; ++it) {
+    double ssum = 0.0, asum = 0.0, mx = 0.0;
+    for (int k = 0; k < K; ++k) {
+      ssum += tr[(size_t)4 * K * it + 4 * k + 0];
+      asum += tr[(size_t)4 * K * it + 4 * k + 1];
+      mx = std::max(mx, tr[(size_t)4 * K * it + 4 * k + 3]);
+    }
+    if (trace) {
+      trace[3 * (size_t)it + 0] = ssum / K;
+      trace[3 * (size_t)it + 1] = asum / K;
+      trace[3 * (size_t)it + 2] = mx;
+    }
+    final_step = ssum / K;
+  }
+  if (out->step_size)
+    for (int c = 0; c < C; ++c) out->step_size[c] = final_step;
+  if (out->inverse_mass)
+    for (int a = 0; a < P; ++a) out->inverse_mass[a] = sig[a] * sig[a];
+  out->warmup_seconds = t_warm - t_start;
+  out->sampling_seconds = t_end - t_warm;
+  out->sampling_gradient_evals = (int64_t)S * C;
+  out->sampling_kernel_seconds = 1e-3 * (double)kernel_ms;
+  out->sampling_launches = S * 4;
+  return UT_OK;
+}
+
 }  // namespace
 
 extern "C" {
+
+void ut_meads_opts_default(ut_meads_opts* o) {
+  if (!o) return;
+  std::memset(o, 0, sizeof(*o));
+  o->num_folds = 4;
+  o->init_strategy = UT_INIT_PRIOR;
+  o->init_max_retries = 100;
+  o->divergence_threshold = 1000.0;
+}
+
+int ut_meads_run(ut_workspace* ws, const ut_meads_opts* opts, ut_chain_out* out, double* adapt_trace) {
+  return run_meads(ws, opts, out, adapt_trace);
+}
 
 void ut_chees_opts_default(ut_chees_opts* co) {
   if (!co) return;

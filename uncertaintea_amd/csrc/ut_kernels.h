@@ -19,7 +19,9 @@ enum Stream : uint32_t {
   STREAM_MERGE = 3,
   STREAM_STEP_SEARCH = 4,
   STREAM_INIT = 5,
-  STREAM_HMC_ACCEPT = 6
+  STREAM_HMC_ACCEPT = 6,
+  STREAM_MEADS_REFRESH = 7,
+  STREAM_MEADS_INIT = 8
 };
 
 // Per-iteration flag bits stored next to warmup history and draws.

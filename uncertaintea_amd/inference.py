@@ -150,7 +150,10 @@ def _run(kind: str, model: TeaModel, num_chains: int, opts, initial_params,
     out.step_size = L.dptr(step, ct.c_double)
     out.inverse_mass = L.dptr(im, ct.c_double)
     trace = None
-    if kind == "chees":
+    if kind == "meads":
+        trace = np.zeros((int(base.num_warmup) + int(base.num_samples), 3), dtype=np.float64)
+        L.check(L.lib().ut_meads_run(ws.handle, ct.byref(opts), ct.byref(out), L.ptr(trace)))
+    elif kind == "chees":
         trace = np.zeros(int(base.num_warmup), dtype=np.float64)
         L.check(L.lib().ut_chees_run(ws.handle, ct.byref(opts), ct.byref(out), L.ptr(trace)))
     else:
@@ -255,3 +258,32 @@ def batched_chees(model: TeaModel, args=(), constraints=None, *, num_chains: int
     co.trajectory_adam_epsilon = float(trajectory_adam_epsilon)
     co.forward_velocity = int(bool(forward_velocity))
     return _run("chees", model, num_chains, co, initial_params, 0, num_leapfrog_steps, target_accept, flow=flow)
+
+
+def batched_meads(model: TeaModel, args=(), constraints=None, *, num_chains: int, num_samples: int,
+                  num_warmup: int = 0, num_folds: int = 4, initial_params=None, init_strategy: str = "prior",
+                  init_max_retries: int = 100, divergence_threshold: float = 1000.0, backend: str = "hip",
+                  precision=np.float32, seed: int = 0, flow=None) -> HMCChains:
+    """MEADS: generalized HMC, one leapfrog step per iteration with persistent
+    momentum and a persistent-slice accept; step, damping, drift and a diagonal
+    preconditioner from the neighbouring fold every iteration (no adaptation phase,
+    `num_warmup` only discards draws).  `trajectory_trace` holds per iteration
+    (mean step, mean damping, largest std) -- the reference's `_adapt_trace`.
+    The reference runs MEADS on the host only; here the fold reductions and the
+    transitions run on the device."""
+    if backend != "hip":
+        raise ValueError(f"backend must be 'hip', got {backend!r}")
+    if np.dtype(precision) != np.float32:
+        raise ValueError("the device samplers run in float32 with float64 energies")
+    if init_strategy not in ("prior", "uniform"):
+        raise ValueError(f"batched_meads init_strategy must be 'prior' or 'uniform', got {init_strategy!r}")
+    mo = L.MeadsOpts()
+    L.lib().ut_meads_opts_default(ct.byref(mo))
+    mo.num_samples = int(num_samples)
+    mo.num_warmup = int(num_warmup)
+    mo.num_folds = int(num_folds)
+    mo.init_strategy = L.UT_INIT_PRIOR if init_strategy == "prior" else L.UT_INIT_UNIFORM
+    mo.init_max_retries = int(init_max_retries)
+    mo.divergence_threshold = float(divergence_threshold)
+    mo.seed = int(seed) & 0xFFFFFFFFFFFFFFFF
+    return _run("meads", model, num_chains, mo, initial_params, 0, 1, float("nan"), flow=flow)
