@@ -109,6 +109,28 @@ int ut_plan_hier_normal_nc(const float* y, const int32_t* group_offsets,
                            const double* sigma, int32_t num_groups, double mu_loc,
                            double mu_scale, double tau_scale, ut_plan** out);
 
+/* Latent Gaussian process with Poisson counts (direct latent-function form of
+ * gp_cholesky, src/distributions/gaussian_process.jl:180-188,283-294):
+ *   log l_d ~ normal(ls_loc, ls_scale) (l_d ~ lognormal), d = 1..D (D <= 2)
+ *   a ~ halfnormal(amp_scale) (log transform)
+ *   z ~ normal(0, 1)^N; f = L z with L L^T = K,
+ *   K_ij = a^2 exp(-0.5 sum_d (x_di - x_dj)^2 / l_d^2) + (noise^2 + 1e-8) delta_ij
+ *   y_i ~ poisson(exp(f_i))          (poisson src/device/math.jl:230-235)
+ * inputs is N x D row-major, counts N non-negative.  Parameter order
+ * (log l_1..log l_D, log a, z_1..z_N); P = D + 1 + N; N a multiple of 32.  The
+ * density and gradient of all chains at once: binary64 kernel matrices and a
+ * batched Cholesky that carries the D + 1 hyperparameter tangents.  Workspaces of
+ * this plan are UT_FLOAT32 and are sampled by the lock-step samplers.           */
+int ut_plan_gp_poisson(const double* inputs, const int32_t* counts, int32_t n, int32_t d, double noise,
+                       double ls_loc, double ls_scale, double amp_scale, ut_plan** out);
+/* Verification entry point: the device factorisation of K at the log
+ * hyperparameters hyper (C x (D+1)): L (C x N x N, lower), its tangents dL
+ * (C x (D+1) x N x N) along each log hyperparameter, and log det K = 2 sum log L_ii.
+ * Any output may be NULL.  No reference counterpart (reference: LinearAlgebra
+ * cholesky with ForwardDiff duals, gaussian_process.jl:283-294).                */
+int ut_gp_factor(const ut_plan* plan, const double* hyper, int32_t num_chains, double* L, double* dL,
+                 double* logdet);
+
 /* Neal's funnel: v ~ normal(0, v_scale); x_i ~ normal(0, exp(v/2)), i = 1..dim.
  * Parameter order: (v, x_1..x_dim); P = dim + 1.                                */
 int ut_plan_funnel(int32_t dim, double v_scale, ut_plan** out);

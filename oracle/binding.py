@@ -32,6 +32,8 @@ def _load():
         "orc_model_logistic": (vp, [vp, vp, i64, i32, dbl, dbl]),
         "orc_model_hier": (vp, [vp, vp, vp, i32, dbl, dbl, dbl]),
         "orc_model_funnel": (vp, [i32, dbl]),
+        "orc_model_gp": (vp, [vp, vp, i32, i32, dbl, dbl, dbl, dbl]),
+        "orc_gp_eval": (None, [vp, vp, i32, vp, vp, vp]),
         "orc_model_free": (None, [vp]),
         "orc_model_num_params": (i32, [vp]),
         "orc_logjoint_grad_f64": (None, [vp, vp, i32, vp, vp]),
@@ -115,6 +117,10 @@ class OracleModel:
                                       model.mu_loc, model.mu_scale, model.tau_scale)
         elif kind == "NealsFunnel":
             self.h = L.orc_model_funnel(model.dim, model.v_scale)
+        elif kind == "GaussianProcessPoisson":
+            self._keep = (model.inputs, model.counts)
+            self.h = L.orc_model_gp(_p(model.inputs), _p(model.counts), model.num_points, model.input_dim,
+                                    model.noise, model.ls_loc, model.ls_scale, model.amp_scale)
         else:
             raise TypeError(kind)
         self.P = L.orc_model_num_params(self.h)
@@ -132,6 +138,15 @@ class OracleModel:
         g = np.empty((C, self.P))
         _load().orc_logjoint_grad_f64(self.h, _p(p), C, _p(lj), _p(g))
         return lj, g.T
+
+    def gp_eval(self, params):
+        """GP model: binary64 (logjoint, gradient P x C, log det K) for P x C params."""
+        p = np.ascontiguousarray(np.asarray(params, dtype=np.float64).T)
+        C = p.shape[0]
+        lj, ld = np.empty(C), np.empty(C)
+        g = np.empty((C, self.P))
+        _load().orc_gp_eval(self.h, _p(p), C, _p(lj), _p(g), _p(ld))
+        return lj, g.T, ld
 
     def logjoint_grad_twin(self, params, precision=np.float32):
         """Kernel-exact values/gradients (float32 or float64) for P x C params."""

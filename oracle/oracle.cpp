@@ -46,7 +46,7 @@ namespace {
 constexpr double HALF_LOG_2PI = 0.9189385332046728;
 constexpr double LOG_PI = 1.1447298858494002;
 constexpr double LOG_2 = 0.6931471805599453;
-enum { KIND_LOGISTIC = 1, KIND_HIER = 2, KIND_FUNNEL = 3 };
+enum { KIND_LOGISTIC = 1, KIND_HIER = 2, KIND_FUNNEL = 3, KIND_GP = 5 };
 enum { S_MOM = 0, S_DIR = 1, S_LEAF = 2, S_MERGE = 3, S_SEARCH = 4, S_INIT = 5, S_ACCEPT = 6 };
 
 // ------------------------------------------------------------------ model data
@@ -68,6 +68,10 @@ struct Model {
   // funnel
   int dim = 0;
   double v_scale = 1;
+  // latent GP Poisson: inputs (N x D), counts, kernel nugget, priors
+  std::vector<double> gx, gy;
+  int gn = 0, gd = 0;
+  double noise2 = 0, ls_loc = 0, ls_scale = 0.5, amp_scale = 1;
   // twin layout
   int K = 1;
   double total_const = 0;
@@ -111,6 +115,78 @@ inline double log1p_exp_ref(double eta) {
 }
 
 // logjoint + gradient in api order (binary64, reference formulas)
+// latent GP Poisson in binary64: K and its tangents along each log hyperparameter,
+// an unblocked Cholesky carrying the tangents (dL_jj = dA_jj / (2 L_jj),
+// dL_ij = (dA_ij - L_ij dL_jj) / L_jj, d(A - L L^T) = dA - dL L^T - L dL^T), then
+// f = L z, the Poisson terms and the priors (reference gp_cholesky,
+// distributions/gaussian_process.jl:180-188,283-294; poisson math.jl:230-235)
+double gp_eval(const Model& m, const double* q, double* g, double* logdet_out = nullptr) {
+  const int N = m.gn, D = m.gd, T = D + 1;
+  std::vector<double> A((size_t)(T + 1) * N * N, 0.0);
+  auto at = [&](int comp, int i, int j) -> double& { return A[((size_t)comp * N + i) * N + j]; };
+  for (int i = 0; i < N; ++i)
+    for (int j = 0; j <= i; ++j) {
+      double s = 0.0, terms[2] = {0.0, 0.0};
+      for (int d = 0; d < D; ++d) {
+        const double diff = (m.gx[(size_t)i * D + d] - m.gx[(size_t)j * D + d]) * std::exp(-q[d]);
+        terms[d] = diff * diff;
+        s += terms[d];
+      }
+      const double k = std::exp(2.0 * q[D]) * std::exp(-0.5 * s);
+      at(0, i, j) = k + (i == j ? m.noise2 : 0.0);
+      for (int d = 0; d < D; ++d) at(1 + d, i, j) = k * terms[d];
+      at(T, i, j) = 2.0 * k;
+    }
+  for (int j = 0; j < N; ++j) {
+    const double l = std::sqrt(at(0, j, j));
+    at(0, j, j) = l;
+    for (int k = 1; k <= T; ++k) at(k, j, j) /= 2.0 * l;
+    for (int i = j + 1; i < N; ++i) {
+      const double v = at(0, i, j) / l;
+      for (int k = 1; k <= T; ++k) at(k, i, j) = (at(k, i, j) - v * at(k, j, j)) / l;
+      at(0, i, j) = v;
+    }
+    for (int i = j + 1; i < N; ++i)
+      for (int kk = j + 1; kk <= i; ++kk) {
+        const double lij = at(0, i, j), lkj = at(0, kk, j);
+        at(0, i, kk) -= lij * lkj;
+        for (int k = 1; k <= T; ++k) at(k, i, kk) -= at(k, i, j) * lkj + lij * at(k, kk, j);
+      }
+  }
+  const double* z = q + D + 1;
+  std::vector<double> r(N);
+  double ll = 0.0, zz = 0.0, ld = 0.0, gh[3] = {0.0, 0.0, 0.0};
+  for (int i = 0; i < N; ++i) {
+    double f = 0.0, df[3] = {0.0, 0.0, 0.0};
+    for (int j = 0; j <= i; ++j) {
+      f += at(0, i, j) * z[j];
+      for (int k = 0; k < T; ++k) df[k] += at(k + 1, i, j) * z[j];
+    }
+    const double ef = std::exp(f);
+    r[i] = m.gy[i] - ef;
+    ll += m.gy[i] * f - ef - std::lgamma(m.gy[i] + 1.0);
+    for (int k = 0; k < T; ++k) gh[k] += r[i] * df[k];
+    zz += z[i] * z[i];
+    ld += std::log(at(0, i, i));
+  }
+  for (int j = 0; j < N; ++j) {
+    double s = 0.0;
+    for (int i = j; i < N; ++i) s += at(0, i, j) * r[i];
+    g[D + 1 + j] = s - z[j];
+  }
+  double lp = 0.0;
+  for (int d = 0; d < D; ++d) {
+    const double w = (q[d] - m.ls_loc) / m.ls_scale;
+    lp += -0.5 * w * w - std::log(m.ls_scale) - HALF_LOG_2PI;
+    g[d] = gh[d] - w / m.ls_scale;
+  }
+  const double a = std::exp(q[D]), wa = a / m.amp_scale;
+  lp += LOG_2 - HALF_LOG_2PI - std::log(m.amp_scale) - 0.5 * wa * wa + q[D];
+  g[D] = gh[D] - wa * wa + 1.0;
+  if (logdet_out) *logdet_out = 2.0 * ld;
+  return ll + lp - 0.5 * zz - N * HALF_LOG_2PI;
+}
+
 double ref_eval(const Model& m, const double* q, double* g, bool& ok) {
   double lp = 0.0;
   const int P = m.P;
@@ -149,6 +225,8 @@ double ref_eval(const Model& m, const double* q, double* g, bool& ok) {
       g[1] += gth * tau * r;
       g[2 + j] += gth * tau;
     }
+  } else if (m.kind == KIND_GP) {
+    lp = gp_eval(m, q, g);
   } else if (m.kind == KIND_FUNNEL) {
     const double v = q[0];
     lp += normal_lpdf(0.0, m.v_scale, v);
@@ -204,6 +282,7 @@ template <class T>
 T twin_eval(const Model& m, const T* q, T* g, bool& ok) {
   const int K = m.K;
   bool fin = true;
+
   if (m.kind == KIND_HIER) {
     const T mu = q[1 * WAVE + 0], ltau = q[1 * WAVE + 1];
     const T tau = texp<T>(ltau);
@@ -370,6 +449,14 @@ struct TwinF32 {
   }
   // the density kernel on reference-order vectors (P entries)
   T ls_eval(const T* q_ref, T* g_ref) const {
+    if (m->kind == KIND_GP) {
+      // no kernel-order twin for the GP (blocked device Cholesky): binary64, rounded
+      std::vector<double> qd(m->P), gd(m->P);
+      for (int a = 0; a < m->P; ++a) qd[a] = (double)q_ref[a];
+      const double v = gp_eval(*m, qd.data(), gd.data());
+      for (int a = 0; a < m->P; ++a) g_ref[a] = (T)gd[a];
+      return (T)v;
+    }
     std::vector<T> q(n(), T(0)), g(n(), T(0));
     for (int i = 0; i < n(); ++i) {
       const int a = api(i);
@@ -1963,6 +2050,31 @@ void* orc_model_funnel(int32_t dim, double v_scale) {
   m->P = dim + 1;
   finish_twin_layout(*m);
   return m;
+}
+
+void* orc_model_gp(const double* inputs, const int32_t* counts, int32_t n, int32_t d, double noise, double ls_loc,
+                   double ls_scale, double amp_scale) {
+  auto* m = new Model();
+  m->kind = KIND_GP;
+  m->gn = n;
+  m->gd = d;
+  m->gx.assign(inputs, inputs + (size_t)n * d);
+  m->gy.resize(n);
+  for (int i = 0; i < n; ++i) m->gy[i] = (double)counts[i];
+  m->noise2 = noise * noise + 1e-8;
+  m->ls_loc = ls_loc;
+  m->ls_scale = ls_scale;
+  m->amp_scale = amp_scale;
+  m->P = d + 1 + n;
+  m->K = 1;  // the twin layout is the reference order for this model
+  return m;
+}
+
+// binary64 GP log density and gradient with the log-determinant of K (C chains)
+void orc_gp_eval(void* h, const double* params, int32_t C, double* lj, double* grad, double* logdet) {
+  const Model& m = *(Model*)h;
+#pragma omp parallel for
+  for (int c = 0; c < C; ++c) lj[c] = gp_eval(m, params + (size_t)c * m.P, grad + (size_t)c * m.P, logdet + c);
 }
 
 void orc_model_free(void* h) { delete (Model*)h; }

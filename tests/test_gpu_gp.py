@@ -1,0 +1,84 @@
+"""GPU: the latent-GP Poisson model.  The batched binary64 Cholesky against
+numpy.linalg.cholesky of the same kernel matrix (|dL| <= 1e-11 max|L|, L L^T = K,
+log det against numpy's slogdet), its hyperparameter tangents against central
+finite differences of numpy's factor (relative 1e-6); logjoint and gradient of all
+chains against the oracle's binary64 evaluation (which matches torch autograd) at
+|d lj| <= 1e-6 |lj| and |d grad| <= 1e-5 max(1, |grad|_inf); MEADS and shared HMC on
+the GP agree on the posterior within MCSE."""
+import sys
+
+import numpy as np
+import pytest
+
+import uncertaintea_amd as ut
+from oracle import binding as B
+from uncertaintea_amd import diagnostics as D
+from uncertaintea_amd import models as M
+from uncertaintea_amd import synthetic as S
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.mark.parametrize("n", [64, 96])
+def test_gp_cholesky_against_numpy(n):
+    say("gp cholesky", n)
+    X, y, _ = S.gp_poisson_data(n, 2, seed=3)
+    model = M.GaussianProcessPoisson(X, y, noise=1e-2)
+    hyper = np.array([[np.log(0.3), np.log(0.6), 0.0], [np.log(0.8), np.log(0.2), np.log(1.7)],
+                      [np.log(0.5), np.log(0.5), np.log(0.4)]])
+    Ld, dLd, ld = model.factor(hyper)
+    for c, h in enumerate(hyper):
+        K = S.gp_kernel(X, np.exp(h[:2]), np.exp(h[2]), 1e-2)
+        Ln = np.linalg.cholesky(K)
+        err = np.max(np.abs(Ld[c] - Ln)) / np.max(np.abs(Ln))
+        assert err <= 1e-11, err
+        assert np.max(np.abs(Ld[c] @ Ld[c].T - K)) <= 1e-11 * np.max(np.abs(K))
+        assert ld[c] == pytest.approx(np.linalg.slogdet(K)[1], rel=1e-11, abs=1e-9)
+        for k in range(3):
+            e = np.zeros(3)
+            e[k] = 1e-6
+            Kp = S.gp_kernel(X, np.exp((h + e)[:2]), np.exp((h + e)[2]), 1e-2)
+            Km = S.gp_kernel(X, np.exp((h - e)[:2]), np.exp((h - e)[2]), 1e-2)
+            fd = (np.linalg.cholesky(Kp) - np.linalg.cholesky(Km)) / 2e-6
+            rel = np.max(np.abs(dLd[c, k] - fd)) / np.max(np.abs(fd))
+            assert rel <= 1e-6, (c, k, rel)
+    say(f"  max rel L error {err:.2e}")
+
+
+def test_gp_density_against_oracle():
+    say("gp density")
+    n = 128
+    X, y, _ = S.gp_poisson_data(n, 2, seed=4)
+    model = M.GaussianProcessPoisson(X, y, noise=1e-2)
+    om = B.OracleModel(model)
+    rng = np.random.default_rng(6)
+    C = 64
+    params = np.concatenate([rng.normal(size=(3, C)) * 0.4, rng.normal(size=(n, C))], axis=0).astype(np.float32)
+    v, g = ut.device_batched_logjoint_gradient(model, params)
+    rv, rg, _ = om.gp_eval(params.astype(np.float64))
+    dlj = np.max(np.abs(v - rv) / np.abs(rv))
+    dg = np.max(np.abs(g - rg)) / max(1.0, np.max(np.abs(rg)))
+    say(f"  logjoint rel err {dlj:.2e}, gradient err {dg:.2e}")
+    assert dlj <= 1e-6 and dg <= 1e-5
+
+
+def test_gp_meads_matches_hmc():
+    say("gp meads vs shared hmc")
+    n = 64
+    X, y, _ = S.gp_poisson_data(n, 2, seed=5)
+    model = M.GaussianProcessPoisson(X, y, noise=1e-2)
+    me = ut.batched_meads(model, num_chains=256, num_samples=1500, num_warmup=1500, seed=9)
+    hm = ut.batched_hmc(model, num_chains=256, num_samples=600, num_warmup=400, num_leapfrog_steps=10,
+                        per_chain_adaptation=False, seed=10)
+    a = me.constrained_samples[:, :, :3]
+    b = hm.constrained_samples[:, :, :3]
+    se = np.sqrt(a.var(axis=(0, 1)) / np.asarray(D.ess_bulk(a)) + b.var(axis=(0, 1)) / np.asarray(D.ess_bulk(b)))
+    z = np.abs(a.mean(axis=(0, 1)) - b.mean(axis=(0, 1))) / se
+    say(f"  meads hyper means {a.mean(axis=(0, 1))}, hmc {b.mean(axis=(0, 1))}, z {z}, "
+        f"meads rhat {float(np.max(np.asarray(D.rhat(me.unconstrained_samples)))):.3f}")
+    assert z.max() < 5.0

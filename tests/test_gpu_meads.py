@@ -53,7 +53,8 @@ def test_meads_posterior_matches_nuts():
     say("meads posterior")
     x, y, _ = S.logistic_data(1000, 25)
     model = M.LogisticRegression(x, y)
-    me = ut.batched_meads(model, num_chains=256, num_samples=1500, num_warmup=300, seed=7)
+    # MEADS has no adaptation phase: discard the transient from the prior draws
+    me = ut.batched_meads(model, num_chains=256, num_samples=1500, num_warmup=1500, seed=7)
     nu = ut.batched_nuts(model, num_chains=256, num_samples=300, num_warmup=300, seed=8)
     a, b = me.unconstrained_samples.astype(np.float64), nu.unconstrained_samples.astype(np.float64)
     rh = float(np.max(np.asarray(D.rhat(a))))

@@ -52,7 +52,7 @@ def test_slice_drift_known_answers():
 def test_meads_posterior_matches_nuts():
     x, y, _ = S.logistic_data(1000, 25)
     om = B.OracleModel(M.LogisticRegression(x, y))
-    me = om.run("meads", "f64", 64, num_samples=1500, num_warmup=300, seed=3, threads=8)
+    me = om.run("meads", "f64", 64, num_samples=1500, num_warmup=1500, seed=3, threads=8)
     nu = om.run("nuts", "f64", 64, num_samples=300, num_warmup=300, seed=6, threads=8)
     a, b = me["samples"], nu["samples"]
     assert float(np.max(np.asarray(D.rhat(a)))) < 1.05

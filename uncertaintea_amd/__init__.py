@@ -6,10 +6,11 @@ from .device import (DeviceBatchedWorkspace, device_batched_logjoint, device_bat
                      device_batched_logjoint_gradient_inplace, device_batched_logjoint_inplace)
 from .flow import AffineCouplingFlow
 from .inference import HMCChains, batched_chees, batched_hmc, batched_meads, batched_nuts
-from .models import HierarchicalNormalNC, LogisticRegression, LogisticRegressionBF16, NealsFunnel, TeaModel
+from .models import (GaussianProcessPoisson, HierarchicalNormalNC, LogisticRegression, LogisticRegressionBF16,
+                     NealsFunnel, TeaModel)
 
 __all__ = [
-    "AffineCouplingFlow", "DeviceBatchedWorkspace", "DeviceError", "HMCChains", "HierarchicalNormalNC", "LogisticRegression",
+    "AffineCouplingFlow", "DeviceBatchedWorkspace", "GaussianProcessPoisson", "DeviceError", "HMCChains", "HierarchicalNormalNC", "LogisticRegression",
     "LogisticRegressionBF16", "NealsFunnel", "NonFiniteInitError", "TeaModel", "batched_chees", "batched_hmc", "batched_meads", "batched_nuts", "device_available",
     "device_batched_logjoint", "device_batched_logjoint_gradient", "device_batched_logjoint_gradient_inplace",
     "device_batched_logjoint_inplace", "diagnostics", "philox4x32", "philox_fill", "synthetic",

@@ -28,6 +28,7 @@ UT_INIT_UNIFORM = 1
 EXPORTED = (
     "ut_last_error", "ut_abi_version", "ut_device_available",
     "ut_plan_logistic_glm", "ut_plan_logistic_glm_bf16", "ut_plan_hier_normal_nc", "ut_plan_funnel",
+    "ut_plan_gp_poisson", "ut_gp_factor",
     "ut_plan_destroy", "ut_plan_num_params",
     "ut_workspace_create", "ut_workspace_destroy",
     "ut_logjoint", "ut_logjoint_gradient",
@@ -140,6 +141,8 @@ def lib() -> ct.CDLL:
         "ut_plan_logistic_glm_bf16": (ct.c_int, [vp, vp, i64, i32, dbl, dbl, pp]),
         "ut_plan_hier_normal_nc": (ct.c_int, [vp, vp, vp, i32, dbl, dbl, dbl, pp]),
         "ut_plan_funnel": (ct.c_int, [i32, dbl, pp]),
+        "ut_plan_gp_poisson": (ct.c_int, [vp, vp, i32, i32, dbl, dbl, dbl, dbl, pp]),
+        "ut_gp_factor": (ct.c_int, [vp, vp, i32, vp, vp, vp]),
         "ut_plan_destroy": (ct.c_int, [vp]),
         "ut_plan_num_params": (ct.c_int, [vp, ct.POINTER(i32)]),
         "ut_workspace_create": (ct.c_int, [vp, i32, i32, pp]),

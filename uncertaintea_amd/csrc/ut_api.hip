@@ -21,6 +21,7 @@
 #include "ut_flow.h"
 #include "ut_gemm.h"
 #include "ut_glm.h"
+#include "ut_gp.h"
 #include "ut_kernels.h"
 #include "ut_lockstep.h"
 #include "ut_math.h"
@@ -82,6 +83,8 @@ struct ut_plan {
   DevModelDesc dev;
   // matrix-core GLM (MODEL_GLM_MFMA): evaluated for all chains at once
   GlmPlan glm;
+  // latent GP Poisson (MODEL_GP): evaluated for all chains at once
+  GpPlan gp;
   // host copies for initial draws and validation
   int P = 0;
   std::vector<void*> owned;  // device allocations
@@ -123,10 +126,11 @@ struct ut_workspace {
   double* d_im = nullptr;
   double* d_summary = nullptr;
   GlmScratch glm;  // MODEL_GLM_MFMA only
+  GpScratch gp;    // MODEL_GP only
   ~ut_workspace() {
     void* ptrs[] = {d_params, d_values, d_grads, st.pos, st.grad, st.lj,
                     st.step, st.dual, st.im_f, d_im, d_summary,
-                    glm.bhi, glm.blo, glm.gpart, glm.llpart};
+                    glm.bhi, glm.blo, glm.gpart, glm.llpart, gp.A, gp.logdet};
     for (void* p : ptrs) (void)hipFree(p);
     if (stream) (void)hipStreamDestroy(stream);
     if (copy_stream) (void)hipStreamDestroy(copy_stream);
@@ -236,6 +240,49 @@ int ut_plan_logistic_glm_bf16(const uint16_t* x_bf16, const float* y, int64_t n,
   return UT_OK;
 }
 
+int ut_plan_gp_poisson(const double* inputs, const int32_t* counts, int32_t n, int32_t d, double noise,
+                       double ls_loc, double ls_scale, double amp_scale, ut_plan** out) {
+  if (!out || !inputs || !counts) return fail(UT_E_ARG, "ut_plan_gp_poisson: null pointer");
+  *out = nullptr;
+  if (n <= 0 || n % GP_TILE != 0) return fail(UT_E_UNSUPPORTED, "ut_plan_gp_poisson: n must be a positive multiple of 32");
+  if (d < 1 || d + 1 > GP_MAX_T) return fail(UT_E_UNSUPPORTED, "ut_plan_gp_poisson: 1 <= d <= 2 input dimensions");
+  if (!(noise >= 0) || !(ls_scale > 0) || !(amp_scale > 0)) return fail(UT_E_ARG, "ut_plan_gp_poisson: invalid scale");
+  for (int i = 0; i < n; ++i)
+    if (counts[i] < 0) return fail(UT_E_ARG, "ut_plan_gp_poisson: counts must be >= 0");
+  auto* plan = new ut_plan();
+  std::vector<double> X(inputs, inputs + (size_t)n * d), y(n);
+  double lg = 0.0;
+  for (int i = 0; i < n; ++i) {
+    y[i] = (double)counts[i];
+    lg += std::lgamma(y[i] + 1.0);
+  }
+  DevModelDesc& m = plan->dev;
+  m.kind = MODEL_GP;
+  m.P = d + 1 + n;
+  m.D = d;
+  m.n = n;
+  m.loc = ls_loc;
+  m.scale = ls_scale;
+  m.v_scale = amp_scale;
+  GpPlan& g = plan->gp;
+  g.N = n;
+  g.D = d;
+  g.noise2 = noise * noise + 1e-8;
+  g.ls_loc = ls_loc;
+  g.ls_scale = ls_scale;
+  g.amp_scale = amp_scale;
+  g.total_const = -lg + (double)n * (-HALF_LOG_2PI) + (double)d * (-log_d(ls_scale) - HALF_LOG_2PI) +
+                  (LOG_2 - HALF_LOG_2PI - log_d(amp_scale));
+  int rc;
+  if ((rc = plan->stage(&g.X, X)) || (rc = plan->stage(&g.y, y))) {
+    delete plan;
+    return rc;
+  }
+  plan->P = m.P;
+  *out = plan;
+  return UT_OK;
+}
+
 int ut_plan_hier_normal_nc(const float* y, const int32_t* group_offsets, const double* sigma,
                            int32_t num_groups, double mu_loc, double mu_scale, double tau_scale,
                            ut_plan** out) {
@@ -338,8 +385,9 @@ int ut_workspace_create(const ut_plan* plan, int32_t num_chains, int32_t precisi
   if (precision != UT_FLOAT32 && precision != UT_FLOAT64)
     return fail(UT_E_PRECISION, "ut_workspace_create: precision must be UT_FLOAT32 or UT_FLOAT64");
   const bool mfma = plan->dev.kind == MODEL_GLM_MFMA;
-  if (mfma && precision != UT_FLOAT32)
-    return fail(UT_E_PRECISION, "ut_workspace_create: the matrix-core GLM plan evaluates in UT_FLOAT32");
+  const bool gp = plan->dev.kind == MODEL_GP;
+  if ((mfma || gp) && precision != UT_FLOAT32)
+    return fail(UT_E_PRECISION, "ut_workspace_create: the matrix-core GLM and GP plans evaluate in UT_FLOAT32");
   if (mfma && num_chains % 128 != 0)
     return fail(UT_E_DIM, "ut_workspace_create: the matrix-core GLM plan needs num_chains a multiple of 128");
   if (!ut_device_available()) return fail(UT_E_HIP, "ut_workspace_create: no HIP device visible");
@@ -371,6 +419,11 @@ int ut_workspace_create(const ut_plan* plan, int32_t num_chains, int32_t precisi
     if (e == hipSuccess) e = dalloc(&g.gpart, PC * (size_t)g.splits);
     if (e == hipSuccess) e = dalloc(&g.llpart, (size_t)num_chains * g.splits);
   }
+  if (gp) {
+    const size_t nn = (size_t)plan->gp.N * plan->gp.N;
+    if (e == hipSuccess) e = dalloc(&ws->gp.A, (size_t)num_chains * (plan->gp.D + 2) * nn);
+    if (e == hipSuccess) e = dalloc(&ws->gp.logdet, (size_t)num_chains);
+  }
   if (e != hipSuccess) {
     delete ws;
     return fail(UT_E_HIP, std::string("ut_workspace_create: ") + hipGetErrorString(e));
@@ -389,6 +442,7 @@ int ut_workspace_destroy(ut_workspace* ws) {
 static hipError_t model_grad(ut_workspace* ws, const float* q, float* lj, float* grad, bool need_lj) {
   if (ws->plan->dev.kind == MODEL_GLM_MFMA)
     return launch_glm_grad(ws->plan->glm, q, grad, lj, ws->C, need_lj, ws->glm, ws->stream);
+  if (ws->plan->dev.kind == MODEL_GP) return launch_gp_grad(ws->plan->gp, q, grad, lj, ws->C, ws->gp, ws->stream);
   return launch_density_f32(ws->plan->dev, q, lj, grad, ws->C, ws->stream);
 }
 
@@ -412,7 +466,7 @@ static int density_call(ut_workspace* ws, const void* params, void* values, void
   const size_t es = ws->precision == UT_FLOAT32 ? 4 : 8;
   UT_HIP(hipMemcpyAsync(ws->d_params, params, PC * es, hipMemcpyHostToDevice, ws->stream));
   hipError_t e;
-  if (ws->plan->dev.kind == MODEL_GLM_MFMA)
+  if (ws->plan->dev.kind == MODEL_GLM_MFMA || ws->plan->dev.kind == MODEL_GP)
     e = model_grad(ws, (const float*)ws->d_params, (float*)ws->d_values, (float*)ws->d_grads, true);
   else if (ws->precision == UT_FLOAT32)
     e = launch_density_f32(ws->plan->dev, (const float*)ws->d_params, (float*)ws->d_values,
@@ -560,6 +614,10 @@ void draw_initial_column(const DevModelDesc& m, int strategy, uint32_t chain, ui
     const double tau = m.tau_scale * std::fabs(z(1) / z(P + 1));
     col[1] = log_d(tau);
     for (int j = 0; j < m.J; ++j) col[2 + j] = z(2 + j);
+  } else if (m.kind == MODEL_GP) {
+    for (int k = 0; k < m.D; ++k) col[k] = m.loc + m.scale * z(k);
+    col[m.D] = log_d(m.v_scale * std::fabs(z(m.D)));
+    for (int i = 0; i < (int)m.n; ++i) col[m.D + 1 + i] = z(m.D + 1 + i);
   } else if (m.kind == MODEL_FUNNEL) {
     const double v = m.v_scale * z(0);
     col[0] = v;
@@ -656,9 +714,10 @@ int init_chains(ut_workspace* ws, const ut_sampler_opts* o, uint32_t rs) {
 int run_sampler(ut_workspace* ws, const ut_sampler_opts* o, ut_chain_out* out, bool nuts) {
   int rc = validate_opts(ws, o, out, nuts);
   if (rc) return rc;
-  if (ws->plan->dev.kind == MODEL_GLM_MFMA)
+  if (ws->plan->dev.kind == MODEL_GLM_MFMA || ws->plan->dev.kind == MODEL_GP)
     return fail(UT_E_UNSUPPORTED,
-                "the matrix-core GLM plan evaluates all chains at once; sample it with the lock-step samplers (ut_chees_run)");
+                "this plan evaluates all chains at once; sample it with the lock-step samplers "
+                "(ut_chees_run, ut_hmc_shared_run, ut_meads_run)");
   if (ws->flow)
     return fail(UT_E_UNSUPPORTED,
                 "a flow preconditions the lock-step samplers (ut_chees_run, ut_hmc_shared_run); detach it first");
@@ -1569,6 +1628,44 @@ int ut_workspace_set_flow(ut_workspace* ws, ut_flow* flow) {
     UT_HIP(flow_scratch(flow->f, ws->C));
   }
   ws->flow = flow;
+  return UT_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int ut_gp_factor(const ut_plan* plan, const double* hyper, int32_t num_chains, double* L, double* dL,
+                 double* logdet) {
+  if (!plan || !hyper) return fail(UT_E_ARG, "ut_gp_factor: null pointer");
+  if (plan->dev.kind != MODEL_GP) return fail(UT_E_ARG, "ut_gp_factor: not a GP plan");
+  if (num_chains <= 0) return fail(UT_E_DIM, "ut_gp_factor: num_chains must be positive");
+  if (!ut_device_available()) return fail(UT_E_HIP, "ut_gp_factor: no HIP device visible");
+  const GpPlan& g = plan->gp;
+  const int T = g.D + 1;
+  const size_t nn = (size_t)g.N * g.N;
+  DeviceBuffers buf;
+  GpScratch s;
+  double* dh = nullptr;
+  UT_HIP(buf.alloc(&s.A, (size_t)num_chains * (T + 1) * nn));
+  UT_HIP(buf.alloc(&s.logdet, (size_t)num_chains));
+  UT_HIP(buf.alloc(&dh, (size_t)num_chains * T));
+  UT_HIP(hipMemcpy(dh, hyper, (size_t)num_chains * T * 8, hipMemcpyHostToDevice));
+  UT_HIP(launch_gp_factor(g, nullptr, T, dh, num_chains, s, nullptr));
+  std::vector<double> A((size_t)num_chains * (T + 1) * nn);
+  UT_HIP(hipMemcpy(A.data(), s.A, A.size() * 8, hipMemcpyDeviceToHost));
+  for (int c = 0; c < num_chains; ++c) {
+    const double* Ac = &A[(size_t)c * (T + 1) * nn];
+    double ld = 0.0;
+    for (int i = 0; i < g.N; ++i) ld += std::log(Ac[(size_t)i * g.N + i]);
+    if (logdet) logdet[c] = 2.0 * ld;
+    for (int comp = 0; comp <= T; ++comp) {
+      double* dst = comp == 0 ? (L ? L + (size_t)c * nn : nullptr) : (dL ? dL + ((size_t)c * T + comp - 1) * nn : nullptr);
+      if (!dst) continue;
+      for (int i = 0; i < g.N; ++i)
+        for (int j = 0; j < g.N; ++j) dst[(size_t)i * g.N + j] = j <= i ? Ac[comp * nn + (size_t)i * g.N + j] : 0.0;
+    }
+  }
   return UT_OK;
 }
 

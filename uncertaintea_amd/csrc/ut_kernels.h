@@ -8,7 +8,8 @@ namespace ut {
 
 // MODEL_GLM_MFMA has no wavefront-per-chain evaluator: its density and gradient run
 // for all chains at once on the matrix cores (ut_glm.h).
-enum ModelKind { MODEL_LOGISTIC = 1, MODEL_HIER = 2, MODEL_FUNNEL = 3, MODEL_GLM_MFMA = 4 };
+// MODEL_GP (latent GP Poisson, ut_gp.h) likewise evaluates all chains at once.
+enum ModelKind { MODEL_LOGISTIC = 1, MODEL_HIER = 2, MODEL_FUNNEL = 3, MODEL_GLM_MFMA = 4, MODEL_GP = 5 };
 
 // Philox stream ids.  NUTS uses 0..3 exactly as the reference's persistent kernel
 // (src/device/persistent_nuts.jl:70-73); the remaining purposes get their own ids.

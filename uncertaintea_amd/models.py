@@ -168,3 +168,61 @@ class NealsFunnel(TeaModel):
 
     def _create_plan(self, out):
         return L.lib().ut_plan_funnel(self.dim, self.v_scale, out)
+
+
+class GaussianProcessPoisson(TeaModel):
+    """Latent GP with Poisson counts (direct latent-function form of gp_cholesky):
+    log l_d ~ normal(ls_loc, ls_scale); a ~ halfnormal(amp_scale); z ~ normal(0, 1)^N;
+    f = chol(K) z; y_i ~ poisson(exp(f_i)), K the ARD squared-exponential kernel plus
+    (noise^2 + 1e-8) I.  Unconstrained layout (log l_1..log l_D, log a, z_1..z_N)."""
+
+    name = "gp_poisson"
+
+    def __init__(self, inputs: np.ndarray, counts: np.ndarray, noise: float = 1e-2, ls_loc: float = 0.0,
+                 ls_scale: float = 0.5, amp_scale: float = 1.0):
+        super().__init__()
+        self.inputs = np.ascontiguousarray(inputs, dtype=np.float64)
+        if self.inputs.ndim == 1:
+            self.inputs = self.inputs[:, None]
+        self.counts = np.ascontiguousarray(counts, dtype=np.int32).reshape(-1)
+        if self.inputs.shape[0] != self.counts.shape[0]:
+            raise ValueError("DimensionMismatch: inputs must be N x D with N = len(counts)")
+        self.noise, self.ls_loc, self.ls_scale, self.amp_scale = float(noise), float(ls_loc), float(ls_scale), \
+            float(amp_scale)
+
+    @property
+    def num_points(self):
+        return self.inputs.shape[0]
+
+    @property
+    def input_dim(self):
+        return self.inputs.shape[1]
+
+    @property
+    def num_params(self):
+        return self.input_dim + 1 + self.num_points
+
+    @property
+    def param_names(self):
+        return [f"lengthscale[{d + 1}]" for d in range(self.input_dim)] + ["amplitude"] + \
+            [f"z[{i + 1}]" for i in range(self.num_points)]
+
+    def constrain(self, u):
+        c = np.array(u, dtype=np.float64, copy=True)
+        c[..., :self.input_dim + 1] = np.exp(c[..., :self.input_dim + 1])
+        return c
+
+    def _create_plan(self, out):
+        n, d = self.inputs.shape
+        return L.lib().ut_plan_gp_poisson(L.ptr(self.inputs), L.ptr(self.counts), n, d, self.noise, self.ls_loc,
+                                          self.ls_scale, self.amp_scale, out)
+
+    def factor(self, log_hyper):
+        """Device factorisation at log hyperparameters (C x (D+1)): (L, dL, logdet)."""
+        h = np.ascontiguousarray(np.atleast_2d(log_hyper), dtype=np.float64)
+        C, N, T = h.shape[0], self.num_points, self.input_dim + 1
+        Lm = np.empty((C, N, N))
+        dL = np.empty((C, T, N, N))
+        ld = np.empty(C)
+        L.check(L.lib().ut_gp_factor(self.plan, L.ptr(h), C, L.ptr(Lm), L.ptr(dL), L.ptr(ld)))
+        return Lm, dL, ld

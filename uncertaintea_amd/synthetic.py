@@ -83,3 +83,31 @@ def hierarchical_data(num_groups: int = 64, per_group: int = 32, seed: int = DEF
     y = (theta[:, None] + sigma[:, None] * eps).astype(np.float32).reshape(-1)
     offsets = (np.arange(num_groups + 1) * per_group).astype(np.int32)
     return y, offsets, sigma, theta
+
+
+def gp_kernel(inputs: np.ndarray, lengthscales, amplitude: float, noise: float) -> np.ndarray:
+    """ARD squared-exponential kernel matrix plus (noise^2 + 1e-8) on the diagonal."""
+    x = np.asarray(inputs, dtype=np.float64) / np.asarray(lengthscales, dtype=np.float64)
+    sq = ((x[:, None, :] - x[None, :, :]) ** 2).sum(axis=2)
+    return amplitude ** 2 * np.exp(-0.5 * sq) + (noise ** 2 + 1e-8) * np.eye(x.shape[0])
+
+
+def gp_poisson_data(n: int = 512, d: int = 2, seed: int = DEFAULT_SEED, cfg: int = 5, lengthscales=(0.3, 0.6),
+                    amplitude: float = 1.0, noise: float = 1e-2):
+    """inputs ~ U([0,1]^d), z* ~ N(0, I), f* = chol(K*) z*, y ~ Poisson(exp(f*)) by
+    inversion of Philox uniforms.  Returns (inputs, counts, f*)."""
+    inputs = uniforms(seed, cfg, 0, n * d).reshape(n, d)
+    z = normals(seed, cfg, 1, n)
+    f = np.linalg.cholesky(gp_kernel(inputs, lengthscales[:d], amplitude, noise)) @ z
+    u = uniforms(seed, cfg, 2, n)
+    counts = np.zeros(n, dtype=np.int32)
+    for i in range(n):
+        lam = np.exp(f[i])
+        k, p = 0, np.exp(-lam)
+        cdf = p
+        while u[i] > cdf and k < 10000:
+            k += 1
+            p *= lam / k
+            cdf += p
+        counts[i] = k
+    return inputs, counts, f
