@@ -130,8 +130,9 @@ struct ut_workspace {
   ~ut_workspace() {
     void* ptrs[] = {d_params, d_values, d_grads, st.pos, st.grad, st.lj,
                     st.step, st.dual, st.im_f, d_im, d_summary,
-                    glm.bhi, glm.blo, glm.gpart, glm.llpart, gp.A, gp.logdet};
+                    glm.bhi, glm.blo, glm.gpart, glm.llpart};
     for (void* p : ptrs) (void)hipFree(p);
+    gp_scratch_free(gp);
     if (stream) (void)hipStreamDestroy(stream);
     if (copy_stream) (void)hipStreamDestroy(copy_stream);
   }
@@ -420,9 +421,7 @@ int ut_workspace_create(const ut_plan* plan, int32_t num_chains, int32_t precisi
     if (e == hipSuccess) e = dalloc(&g.llpart, (size_t)num_chains * g.splits);
   }
   if (gp) {
-    const size_t nn = (size_t)plan->gp.N * plan->gp.N;
-    if (e == hipSuccess) e = dalloc(&ws->gp.A, (size_t)num_chains * (plan->gp.D + 2) * nn);
-    if (e == hipSuccess) e = dalloc(&ws->gp.logdet, (size_t)num_chains);
+    if (e == hipSuccess) e = gp_scratch_alloc(ws->gp, plan->gp, num_chains);
   }
   if (e != hipSuccess) {
     delete ws;
@@ -1196,6 +1195,7 @@ int run_meads(ut_workspace* ws, const ut_meads_opts* mo, ut_chain_out* out, doub
   UT_HIP(buf.alloc(&m.sigma_f, (size_t)K * P));
   UT_HIP(buf.alloc(&m.white, (size_t)2 * K * nmax * P));
   UT_HIP(buf.alloc(&m.gram_row, (size_t)2 * K * nmax));
+  UT_HIP(buf.alloc(&m.gram, (size_t)2 * K * nmax * nmax));
   UT_HIP(buf.alloc(&m.fold, (size_t)4 * K));
   UT_HIP(buf.alloc(&m.fold_start, (size_t)K + 1));
   int* d_fold_of = nullptr;
@@ -1644,26 +1644,39 @@ int ut_gp_factor(const ut_plan* plan, const double* hyper, int32_t num_chains, d
   const GpPlan& g = plan->gp;
   const int T = g.D + 1;
   const size_t nn = (size_t)g.N * g.N;
+  const int nb = g.N / GP_TILE;
   DeviceBuffers buf;
   GpScratch s;
   double* dh = nullptr;
-  UT_HIP(buf.alloc(&s.A, (size_t)num_chains * (T + 1) * nn));
-  UT_HIP(buf.alloc(&s.logdet, (size_t)num_chains));
+  UT_HIP(gp_scratch_alloc(s, g, num_chains));
+  struct Release {
+    GpScratch& s;
+    ~Release() { gp_scratch_free(s); }
+  } release{s};
   UT_HIP(buf.alloc(&dh, (size_t)num_chains * T));
   UT_HIP(hipMemcpy(dh, hyper, (size_t)num_chains * T * 8, hipMemcpyHostToDevice));
   UT_HIP(launch_gp_factor(g, nullptr, T, dh, num_chains, s, nullptr));
-  std::vector<double> A((size_t)num_chains * (T + 1) * nn);
+  std::vector<double> A((size_t)num_chains * (T + 1) * gp_tiles(g.N) * GP_TILE * GP_TILE);
   UT_HIP(hipMemcpy(A.data(), s.A, A.size() * 8, hipMemcpyDeviceToHost));
+  // tile-major, column-major tiles -> row-major lower triangles
   for (int c = 0; c < num_chains; ++c) {
-    const double* Ac = &A[(size_t)c * (T + 1) * nn];
     double ld = 0.0;
-    for (int i = 0; i < g.N; ++i) ld += std::log(Ac[(size_t)i * g.N + i]);
+    for (int ib = 0; ib < nb; ++ib) {
+      const double* dt = &A[gp_tile_offset(g.N, T, c, 0, ib, ib)];
+      for (int r = 0; r < GP_TILE; ++r) ld += std::log(dt[r * GP_TILE + r]);
+    }
     if (logdet) logdet[c] = 2.0 * ld;
     for (int comp = 0; comp <= T; ++comp) {
       double* dst = comp == 0 ? (L ? L + (size_t)c * nn : nullptr) : (dL ? dL + ((size_t)c * T + comp - 1) * nn : nullptr);
       if (!dst) continue;
-      for (int i = 0; i < g.N; ++i)
-        for (int j = 0; j < g.N; ++j) dst[(size_t)i * g.N + j] = j <= i ? Ac[comp * nn + (size_t)i * g.N + j] : 0.0;
+      std::fill(dst, dst + nn, 0.0);
+      for (int ib = 0; ib < nb; ++ib)
+        for (int jb = 0; jb <= ib; ++jb) {
+          const double* tp = &A[gp_tile_offset(g.N, T, c, comp, ib, jb)];
+          for (int col = 0; col < GP_TILE; ++col)
+            for (int row = 0; row < GP_TILE; ++row)
+              dst[(size_t)(ib * GP_TILE + row) * g.N + jb * GP_TILE + col] = tp[col * GP_TILE + row];
+        }
     }
   }
   return UT_OK;

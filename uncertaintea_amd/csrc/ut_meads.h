@@ -25,6 +25,7 @@ struct MeadsState {
   double* sigma = nullptr;  // K x P neighbour-fold standard deviations
   float* sigma_f = nullptr; // K x P
   double* white = nullptr;  // 2 x K x Nmax x P whitened positions / gradients (member-major)
+  double* gram = nullptr;      // 2 x K x Nmax x Nmax Gram matrices of the whitened ensembles
   double* gram_row = nullptr;  // 2 x K x Nmax row partials of the off-diagonal squared Gram sum
   double* fold = nullptr;   // K x 4: step, alpha, delta, (unused)
   int* fold_start = nullptr;   // K + 1 chain offsets

@@ -5,7 +5,7 @@
 // Every iteration: per fold k, the neighbour fold (k - 1) mod K supplies the
 // diagonal preconditioner (population std per parameter), the whitened position
 // and gradient ensembles, and through their Gram trace ratios the step size,
-// damping and slice drift (k_meads_fold_params, one workgroup per fold, binary64,
+// damping and slice drift (k_meads_white, k_meads_gram, k_meads_fold_params; binary64,
 // fixed reduction orders).  Then every chain refreshes its persistent whitened
 // momentum, takes ONE leapfrog step with its fold's step and inverse mass, and the
 // persistent-slice Metropolis test decides (k_meads_begin / gradient /
@@ -24,23 +24,89 @@ constexpr double MEADS_EIG_FLOOR = 1e-12;
 constexpr double MEADS_STD_FLOOR = 1e-10;
 constexpr int FOLD_THREADS = 256;
 
-// Trace ratio lambda2_bar / lambda_bar of the ensemble x (n members x P, member-major),
-// or 0 for a degenerate ensemble.  Row partials in rows[], fixed orders: member
-// norms over parameters, off-diagonal squared Gram entries over partner members,
-// then rows in member order.
-__device__ double max_eig(const double* x, int n, int P, double* rows, double* norms) {
+// Whitening of the neighbour fold (k - 1) mod K for fold k, one thread per parameter:
+// population mean and std over the members in member order, sigma, the whitened
+// positions (x - mean) / sd and gradients g * sd (member-major).
+__global__ __launch_bounds__(64) void k_meads_white(MeadsState m, int P, int K, int nmax) {
+  const int k = blockIdx.y;
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  const int nb = (k - 1 + K) % K;
+  const int c0 = m.fold_start[nb], n = m.fold_start[nb + 1] - c0;
+  double* wpos = m.white + ((long long)(0 * K + k) * nmax) * P;
+  double* wgrd = m.white + ((long long)(1 * K + k) * nmax) * P;
+  double mean = 0.0;
+  for (int j = 0; j < n; ++j) mean += (double)m.pos[(long long)(c0 + j) * P + p];
+  mean /= (double)n;
+  double var = 0.0;
+  for (int j = 0; j < n; ++j) {
+    const double d = (double)m.pos[(long long)(c0 + j) * P + p] - mean;
+    var += d * d;
+  }
+  var /= (double)n;
+  double sd = sqrt_d(var);
+  sd = sd > MEADS_STD_FLOOR ? sd : MEADS_STD_FLOOR;
+  m.sigma[(long long)k * P + p] = sd;
+  m.sigma_f[(long long)k * P + p] = (float)sd;
+  const double inv = 1.0 / sd;
+  for (int j = 0; j < n; ++j) {
+    wpos[(long long)j * P + p] = ((double)m.pos[(long long)(c0 + j) * P + p] - mean) * inv;
+    wgrd[(long long)j * P + p] = (double)m.grad[(long long)(c0 + j) * P + p] * sd;
+  }
+}
+
+// Gram matrices G = W W^T of the whitened ensembles, 64 x 64 entries per workgroup,
+// 4 x 4 per thread; every entry is the sum over parameters in ascending order
+// (separate multiply and add), the order of the binary64 oracle.
+constexpr int GT = 64, GK = 16;
+__global__ __launch_bounds__(256) void k_meads_gram(MeadsState m, int P, int K, int nmax) {
+  const int z = blockIdx.z, k = z % K;
+  const int nb = (k - 1 + K) % K;
+  const int n = m.fold_start[nb + 1] - m.fold_start[nb];
+  const int a0 = blockIdx.x * GT, b0 = blockIdx.y * GT;
+  if (a0 >= n || b0 >= n) return;
+  const double* W = m.white + ((long long)z * nmax) * P;  // z = mat * K + k
+  double* G = m.gram + (long long)z * nmax * nmax;
+  __shared__ double As[GK][GT], Bs[GK][GT];
+  const int t = threadIdx.x, ta = t & 15, tb = t >> 4;
+  double acc[4][4];
+  for (int u = 0; u < 4; ++u)
+    for (int v = 0; v < 4; ++v) acc[u][v] = 0.0;
+  for (int p0 = 0; p0 < P; p0 += GK) {
+    __syncthreads();
+    for (int s = 0; s < 4; ++s) {
+      const int e = t + 256 * s, row = e >> 4, pp = e & 15;
+      const bool pin = p0 + pp < P;
+      As[pp][row] = (a0 + row < n && pin) ? W[(long long)(a0 + row) * P + p0 + pp] : 0.0;
+      Bs[pp][row] = (b0 + row < n && pin) ? W[(long long)(b0 + row) * P + p0 + pp] : 0.0;
+    }
+    __syncthreads();
+    for (int pp = 0; pp < GK; ++pp) {
+      double a[4], b[4];
+      for (int u = 0; u < 4; ++u) a[u] = As[pp][ta + 16 * u];
+      for (int v = 0; v < 4; ++v) b[v] = Bs[pp][tb + 16 * v];
+      for (int u = 0; u < 4; ++u)
+        for (int v = 0; v < 4; ++v) acc[u][v] = acc[u][v] + a[u] * b[v];
+    }
+  }
+  for (int u = 0; u < 4; ++u)
+    for (int v = 0; v < 4; ++v) {
+      const int a = a0 + ta + 16 * u, b = b0 + tb + 16 * v;
+      if (a < n && b < n) G[(long long)a * nmax + b] = acc[u][v];
+    }
+}
+
+// Trace ratio lambda2_bar / lambda_bar from the Gram matrix G (n x n, row stride
+// nmax), or 0 for a degenerate ensemble.  Fixed orders: squared off-diagonal entries
+// over partner members, then norms and rows in member order.
+__device__ double max_eig(const double* G, int n, int nmax, double* rows, double* norms) {
   for (int a = threadIdx.x; a < n; a += blockDim.x) {
-    const double* xa = x + (long long)a * P;
-    double nn = 0.0;
-    for (int p = 0; p < P; ++p) nn += xa[p] * xa[p];
-    norms[a] = nn;
+    const double* ga = G + (long long)a * nmax;
+    norms[a] = ga[a];
     double r = 0.0;
     for (int b = 0; b < n; ++b) {
       if (b == a) continue;
-      const double* xb = x + (long long)b * P;
-      double g = 0.0;
-      for (int p = 0; p < P; ++p) g += xa[p] * xb[p];
-      r += g * g;
+      r += ga[b] * ga[b];
     }
     rows[a] = r;
   }
@@ -64,41 +130,14 @@ __device__ double max_eig(const double* x, int n, int P, double* rows, double* n
 __global__ __launch_bounds__(FOLD_THREADS) void k_meads_fold_params(MeadsState m, int P, int K, int nmax, int t) {
   const int k = blockIdx.x;                 // fold receiving the parameters
   const int nb = (k - 1 + K) % K;           // its neighbour
-  const int c0 = m.fold_start[nb], n = m.fold_start[nb + 1] - c0;
-  double* wpos = m.white + ((long long)(0 * K + k) * nmax) * P;
-  double* wgrd = m.white + ((long long)(1 * K + k) * nmax) * P;
+  const int n = m.fold_start[nb + 1] - m.fold_start[nb];
   double* rows = m.gram_row + (long long)(0 * K + k) * nmax;
   double* norms = m.gram_row + (long long)(1 * K + k) * nmax;
-  __shared__ double smax[FOLD_THREADS];
-  double local_max = 0.0;
-  for (int p = threadIdx.x; p < P; p += blockDim.x) {
-    double mean = 0.0;
-    for (int j = 0; j < n; ++j) mean += (double)m.pos[(long long)(c0 + j) * P + p];
-    mean /= (double)n;
-    double var = 0.0;
-    for (int j = 0; j < n; ++j) {
-      const double d = (double)m.pos[(long long)(c0 + j) * P + p] - mean;
-      var += d * d;
-    }
-    var /= (double)n;
-    double sd = sqrt_d(var);
-    sd = sd > MEADS_STD_FLOOR ? sd : MEADS_STD_FLOOR;
-    m.sigma[(long long)k * P + p] = sd;
-    m.sigma_f[(long long)k * P + p] = (float)sd;
-    const double inv = 1.0 / sd;
-    for (int j = 0; j < n; ++j) {
-      wpos[(long long)j * P + p] = ((double)m.pos[(long long)(c0 + j) * P + p] - mean) * inv;
-      wgrd[(long long)j * P + p] = (double)m.grad[(long long)(c0 + j) * P + p] * sd;
-    }
-    local_max = fmax(local_max, sd);
-  }
-  smax[threadIdx.x] = local_max;
-  __syncthreads();
-  const double eig_g = max_eig(wgrd, n, P, rows, norms);
-  const double eig_p = max_eig(wpos, n, P, rows, norms);
+  const double eig_g = max_eig(m.gram + (long long)(1 * K + k) * nmax * nmax, n, nmax, rows, norms);
+  const double eig_p = max_eig(m.gram + (long long)(0 * K + k) * nmax * nmax, n, nmax, rows, norms);
   if (threadIdx.x == 0) {
     double mx = 0.0;
-    for (int i = 0; i < blockDim.x; ++i) mx = fmax(mx, smax[i]);
+    for (int p = 0; p < P; ++p) mx = fmax(mx, m.sigma[(long long)k * P + p]);
     const double eg = eig_g > MEADS_EIG_FLOOR ? eig_g : MEADS_EIG_FLOOR;
     const double ep = eig_p > MEADS_EIG_FLOOR ? eig_p : MEADS_EIG_FLOOR;
     double step = MEADS_STEP_MARGIN / sqrt_d(eg);
@@ -305,6 +344,9 @@ hipError_t by_slots(int P, F&& f) {
 }  // namespace
 
 hipError_t launch_meads_fold_params(const MeadsState& m, int P, int K, int nmax, int t, hipStream_t stream) {
+  hipLaunchKernelGGL(k_meads_white, dim3((P + 63) / 64, K), dim3(64), 0, stream, m, P, K, nmax);
+  const int nt = (nmax + GT - 1) / GT;
+  hipLaunchKernelGGL(k_meads_gram, dim3(nt, nt, 2 * K), dim3(256), 0, stream, m, P, K, nmax);
   hipLaunchKernelGGL(k_meads_fold_params, dim3(K), dim3(FOLD_THREADS), 0, stream, m, P, K, nmax, t);
   return hipGetLastError();
 }
