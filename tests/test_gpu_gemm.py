@@ -1,5 +1,7 @@
 """Matrix-core GEMM (bf16 x bf16 -> fp32) against numpy on the same bf16 values.
-Tolerance: fp32 accumulation of K products, |err| <= 1e-5 * K^0.5 * max|C| scale."""
+Tolerance: fp32 accumulation of K products, |err| <= 1e-5 * K^0.5 * max|C| scale.
+Small grids run the 64 x 64 workgroup tiles, grids of >= 256 128-wide tiles the
+128 x 128 tiles."""
 import sys
 
 import numpy as np
@@ -11,7 +13,7 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.mark.parametrize("m,n,k,splits", [(128, 128, 32, 1), (256, 384, 96, 1), (384, 128, 4096, 4),
-                                          (128, 256, 512, 3)])
+                                          (128, 256, 512, 3), (2048, 2048, 64, 1), (1024, 1024, 96, 5)])
 def test_gemm_matches_numpy(m, n, k, splits):
     print("gemm", m, n, k, splits, flush=True)
     rng = np.random.default_rng(m + n + k)

@@ -17,6 +17,7 @@
 namespace ut {
 
 constexpr int FLOW_PAD = 128;  // passive inputs and conditioner outputs are padded to 128
+constexpr int FLOW_WSPLITS = 32;  // split-K of the weight-gradient GEMMs (K = rows)
 
 struct FlowLayer {
   int np = 0, na = 0;      // passive / active counts (2 na <= 128, np <= 128)
@@ -53,6 +54,7 @@ struct FlowScratch {
   float* aux = nullptr;       // B x P per-element terms of the base-parameter gradients
   float* logdet = nullptr;    // B
   float* colsum = nullptr;    // column-sum partials (64 chunks x widest row)
+  float* wpart = nullptr;     // split-K partials of the weight-gradient GEMMs (FLOW_WSPLITS x H x widest)
 };
 
 struct Flow {

@@ -420,6 +420,7 @@ hipError_t flow_scratch(Flow& f, int B) {
   if (e == hipSuccess) e = own(f, &s.aux, (size_t)B * f.P);
   if (e == hipSuccess) e = own(f, &s.logdet, (size_t)B);
   if (e == hipSuccess) e = own(f, &s.colsum, (size_t)COLSUM_CHUNKS * std::max<size_t>(wide, (size_t)f.P));
+  if (e == hipSuccess) e = own(f, &s.wpart, (size_t)FLOW_WSPLITS * H * wide);
   return e;
 }
 
@@ -470,11 +471,12 @@ static hipError_t conditioner_bwd(Flow& f, int l, int B, bool weights, hipStream
   // dW2 (op x H) = gO^T h ; dW1 (H x kp) = gU^T X_p ; bias column sums
   hipLaunchKernelGGL(k_transpose_bf16, dim3(op / 32, B / 32), dim3(T256), 0, st, s.gO, B, op, s.tA);
   hipLaunchKernelGGL(k_transpose_bf16, dim3(H / 32, B / 32), dim3(T256), 0, st, s.hb[l], B, H, s.tB);
-  e = launch_gemm_nt(s.tA, B, s.tB, B, f.grads + ly.oW2, H, op, H, B, 1, 0, st);
+  const int splits = std::min(FLOW_WSPLITS, B / FLOW_PAD);  // K = B rows: split so the grid fills the CUs
+  e = launch_gemm_nt_splitk(s.tA, B, s.tB, B, f.grads + ly.oW2, H, op, H, B, splits, s.wpart, st);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_transpose_bf16, dim3(H / 32, B / 32), dim3(T256), 0, st, s.gU, B, H, s.tA);
   hipLaunchKernelGGL(k_transpose_bf16, dim3(kp / 32, B / 32), dim3(T256), 0, st, s.xp[l], B, kp, s.tB);
-  e = launch_gemm_nt(s.tA, B, s.tB, B, f.grads + ly.oW1, kp, H, kp, B, 1, 0, st);
+  e = launch_gemm_nt_splitk(s.tA, B, s.tB, B, f.grads + ly.oW1, kp, H, kp, B, splits, s.wpart, st);
   if (e != hipSuccess) return e;
   colsum(s.gUf, B, H, f.grads + ly.ob1, s.colsum, st);
   colsum(s.gOf, B, op, f.grads + ly.ob2, s.colsum, st);

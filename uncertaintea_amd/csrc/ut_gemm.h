@@ -5,8 +5,12 @@
 
 namespace ut {
 // C (+ z * split_stride for split z) = A[M x K] * Bt[N x K]^T over the split's K range.
-// A, Bt: bfloat16 row-major; C fp32 row-major.  M, N multiples of 128, K of 32,
+// A, Bt: bfloat16 row-major; C fp32 row-major.  M, N multiples of 64, K of 32,
 // lda and ldb multiples of 8.
 hipError_t launch_gemm_nt(const uint16_t* A, int lda, const uint16_t* Bt, int ldb, float* C, int ldc,
                           int M, int N, int K, int splits, long long split_stride, hipStream_t stream);
+// The same product with split-K into part (splits x M x N fp32), then summed over the
+// splits in order into C (row stride ldc).
+hipError_t launch_gemm_nt_splitk(const uint16_t* A, int lda, const uint16_t* Bt, int ldb, float* C, int ldc, int M,
+                                 int N, int K, int splits, float* part, hipStream_t stream);
 }  // namespace ut

@@ -3,23 +3,26 @@
 //   C[M x N] = A[M x K] * B[K x N],  A given row-major (lda), B given transposed as
 //   Bt[N x K] row-major (ldb), C row-major fp32 (ldc).
 //
-// Workgroup tile 128 x 128, four wavefronts of 64 x 64 (2 x 2 MFMA 32x32 tiles,
-// 64 fp32 accumulators per lane), K stepped by 32 through LDS with the next
-// step's global loads in flight while the current step multiplies.  Optional
-// split-K over grid.z writes one partial product per split (reduced by the caller's
-// epilogue kernel).  M and N must be multiples of 128 and K of 32 (callers pad).
+// Workgroup tile 64S x 64S (S = 2 for large problems, 1 when the 128-wide grid would
+// leave CUs idle), four wavefronts of 32S x 32S (S x S MFMA 32x32 tiles), K stepped
+// by 32 through LDS with the next step's global loads in flight while the current
+// step multiplies.  Optional split-K over grid.z writes one partial product per
+// split; launch_gemm_nt_splitk sums the partials in split order.  M and N must be
+// multiples of 64 (128 for S = 2) and K of 32 (callers pad).
 #include "ut_gemm.h"
 #include "ut_mfma.h"
 
 namespace ut {
 
 namespace {
-constexpr int BM = 128, BN = 128, BK = 32, LDS_STRIDE = BK + 8;  // +8 bf16 breaks bank aliasing
+constexpr int BK = 32, LDS_STRIDE = BK + 8;  // +8 bf16 breaks bank aliasing
 
+template <int S>
 __global__ __launch_bounds__(256) void k_gemm_nt(const bf16_t* __restrict__ A, int lda,
                                                  const bf16_t* __restrict__ Bt, int ldb,
                                                  float* __restrict__ C, int ldc, long long split_stride,
                                                  int K, int k_per_split) {
+  constexpr int BM = 64 * S, BN = 64 * S;
   __shared__ __attribute__((aligned(16))) bf16_t As[BM * LDS_STRIDE];
   __shared__ __attribute__((aligned(16))) bf16_t Bs[BN * LDS_STRIDE];
   const int tid = threadIdx.x;
@@ -31,27 +34,27 @@ __global__ __launch_bounds__(256) void k_gemm_nt(const bf16_t* __restrict__ A, i
   const int kb = blockIdx.z * k_per_split;
   const int ke = min(K, kb + k_per_split);
 
-  // each thread moves two 16-byte chunks of A and two of B per K step
-  int row[2], col[2];
+  // each thread moves S 16-byte chunks of A and S of B per K step
+  int row[S], col[S];
 #pragma unroll
-  for (int j = 0; j < 2; ++j) {
+  for (int j = 0; j < S; ++j) {
     const int c = tid + 256 * j;
     row[j] = c >> 2;
     col[j] = (c & 3) * 8;
   }
-  uint4 ra[2], rb[2];
+  uint4 ra[S], rb[S];
   auto gload = [&](int k0) {
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
+    for (int j = 0; j < S; ++j) {
       ra[j] = *(const uint4*)(A + (m0 + row[j]) * lda + k0 + col[j]);
       rb[j] = *(const uint4*)(Bt + (n0 + row[j]) * ldb + k0 + col[j]);
     }
   };
-  f32x16 acc[2][2];
+  f32x16 acc[S][S];
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < S; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+    for (int j = 0; j < S; ++j)
 #pragma unroll
       for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
 
@@ -59,7 +62,7 @@ __global__ __launch_bounds__(256) void k_gemm_nt(const bf16_t* __restrict__ A, i
   for (int k0 = kb; k0 < ke; k0 += BK) {
     __syncthreads();
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
+    for (int j = 0; j < S; ++j) {
       *(uint4*)(As + row[j] * LDS_STRIDE + col[j]) = ra[j];
       *(uint4*)(Bs + row[j] * LDS_STRIDE + col[j]) = rb[j];
     }
@@ -67,41 +70,68 @@ __global__ __launch_bounds__(256) void k_gemm_nt(const bf16_t* __restrict__ A, i
     if (k0 + BK < ke) gload(k0 + BK);
 #pragma unroll
     for (int ks = 0; ks < BK; ks += 16) {
-      bf16x8 a[2], b[2];
+      bf16x8 a[S], b[S];
       const int kk = ks + 8 * (lane >> 5);
 #pragma unroll
-      for (int i = 0; i < 2; ++i)
-        a[i] = *(const bf16x8*)(As + (64 * wm + 32 * i + (lane & 31)) * LDS_STRIDE + kk);
+      for (int i = 0; i < S; ++i)
+        a[i] = *(const bf16x8*)(As + (32 * S * wm + 32 * i + (lane & 31)) * LDS_STRIDE + kk);
 #pragma unroll
-      for (int j = 0; j < 2; ++j)
-        b[j] = *(const bf16x8*)(Bs + (64 * wn + 32 * j + (lane & 31)) * LDS_STRIDE + kk);
+      for (int j = 0; j < S; ++j)
+        b[j] = *(const bf16x8*)(Bs + (32 * S * wn + 32 * j + (lane & 31)) * LDS_STRIDE + kk);
 #pragma unroll
-      for (int i = 0; i < 2; ++i)
+      for (int i = 0; i < S; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) acc[i][j] = mfma32(a[i], b[j], acc[i][j]);
+        for (int j = 0; j < S; ++j) acc[i][j] = mfma32(a[i], b[j], acc[i][j]);
     }
   }
   float* Cz = C + (long long)blockIdx.z * split_stride;
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < S; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+    for (int j = 0; j < S; ++j)
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
-        const long long m = m0 + 64 * wm + 32 * i + mfma32_row(lane, r);
-        const long long n = n0 + 64 * wn + 32 * j + (lane & 31);
+        const long long m = m0 + 32 * S * wm + 32 * i + mfma32_row(lane, r);
+        const long long n = n0 + 32 * S * wn + 32 * j + (lane & 31);
         Cz[m * ldc + n] = acc[i][j][r];
       }
+}
+
+// C[m * ldc + n] = sum over splits z (in order) of part[z][m][n]
+__global__ void k_splitk_sum(const float* __restrict__ part, int splits, int M, int N, float* __restrict__ C,
+                             int ldc) {
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long mn = (long long)M * N;
+  if (idx >= mn) return;
+  float s = part[idx];
+  for (int z = 1; z < splits; ++z) s += part[z * mn + idx];
+  C[(idx / N) * ldc + idx % N] = s;
 }
 }  // namespace
 
 hipError_t launch_gemm_nt(const bf16_t* A, int lda, const bf16_t* Bt, int ldb, float* C, int ldc,
                           int M, int N, int K, int splits, long long split_stride, hipStream_t stream) {
-  if (M % BM || N % BN || K % BK || splits < 1) return hipErrorInvalidValue;
+  if (M % 64 || N % 64 || K % BK || splits < 1) return hipErrorInvalidValue;
   const int ksteps = K / BK;
   const int per = (ksteps + splits - 1) / splits * BK;
-  dim3 grid(N / BN, M / BM, splits);
-  hipLaunchKernelGGL(k_gemm_nt, grid, dim3(256), 0, stream, A, lda, Bt, ldb, C, ldc, split_stride, K, per);
+  const bool wide = M % 128 == 0 && N % 128 == 0 && (long long)(M / 128) * (N / 128) * splits >= 256;
+  if (wide) {
+    hipLaunchKernelGGL(k_gemm_nt<2>, dim3(N / 128, M / 128, splits), dim3(256), 0, stream, A, lda, Bt, ldb, C, ldc,
+                       split_stride, K, per);
+  } else {
+    hipLaunchKernelGGL(k_gemm_nt<1>, dim3(N / 64, M / 64, splits), dim3(256), 0, stream, A, lda, Bt, ldb, C, ldc,
+                       split_stride, K, per);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_gemm_nt_splitk(const bf16_t* A, int lda, const bf16_t* Bt, int ldb, float* C, int ldc, int M,
+                                 int N, int K, int splits, float* part, hipStream_t stream) {
+  const long long mn = (long long)M * N;
+  hipError_t e = launch_gemm_nt(A, lda, Bt, ldb, part, N, M, N, K, splits, mn, stream);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_splitk_sum, dim3((unsigned)((mn + 255) / 256)), dim3(256), 0, stream, part, splits, M, N, C,
+                     ldc);
   return hipGetLastError();
 }
 
