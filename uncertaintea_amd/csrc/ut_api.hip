@@ -662,9 +662,13 @@ int validate_opts(const ut_workspace* ws, const ut_sampler_opts* o, const ut_cha
   return UT_OK;
 }
 
-// Iterations per sampler launch: long enough to amortise the launch, short enough
-// that one launch stays in the millisecond range.
+// Sampling iterations per launch: at least kChunk, and the sampling phase in at most
+// kSampleLaunches launches.  A launch ends when its slowest chain does, and chains
+// occasionally build much deeper NUTS trees than the rest, so short launches sum
+// many per-launch maxima; longer ones average them out.  Each launch's draws are
+// copied out while the next one runs.
 constexpr int kChunk = 50;
+constexpr int kSampleLaunches = 6;
 
 // Initial positions, logjoint and gradient of every chain (reference
 // _initial_batched_hmc_positions + the non-finite retry loop,
@@ -764,14 +768,9 @@ int run_sampler(ut_workspace* ws, const ut_sampler_opts* o, ut_chain_out* out, b
   UT_HIP(buf.alloc(&sp.o_depth, (size_t)S * C));
   UT_HIP(buf.alloc(&sp.o_steps, (size_t)S * C));
 
+  // a warmup window runs as one launch (nothing is copied out during warmup)
   auto run_range = [&](int a, int b) -> hipError_t {
-    const int chunk = kChunk;
-    for (int s = a; s <= b; s += chunk) {
-      const int e = std::min(b, s + chunk - 1);
-      hipError_t err = nuts ? launch_nuts(m, ws->st, sp, s, e, stream) : launch_hmc(m, ws->st, sp, s, e, stream);
-      if (err != hipSuccess) return err;
-    }
-    return hipSuccess;
+    return nuts ? launch_nuts(m, ws->st, sp, a, b, stream) : launch_hmc(m, ws->st, sp, a, b, stream);
   };
 
   // ---- warmup: one launch range per mass window, device replay at each window end
@@ -808,8 +807,9 @@ int run_sampler(ut_workspace* ws, const ut_sampler_opts* o, ut_chain_out* out, b
   UT_HIP(hipEventCreate(&ev1));
   std::vector<hipEvent_t> done;
   UT_HIP(hipEventRecord(ev0, stream));
-  for (int s0 = 0; s0 < S; s0 += kChunk) {
-    const int s1 = std::min(S, s0 + kChunk);
+  const int chunk = std::max(kChunk, (S + kSampleLaunches - 1) / kSampleLaunches);
+  for (int s0 = 0; s0 < S; s0 += chunk) {
+    const int s1 = std::min(S, s0 + chunk);
     hipError_t e = nuts ? launch_nuts(m, ws->st, sp, W + 1 + s0, W + s1, stream)
                         : launch_hmc(m, ws->st, sp, W + 1 + s0, W + s1, stream);
     UT_HIP(e);

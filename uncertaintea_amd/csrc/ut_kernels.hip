@@ -7,6 +7,8 @@
 // not involved at all.  Working precision is binary32 for positions, momenta,
 // gradients and the log density; Hamiltonians, energy errors, acceptance
 // probabilities, multinomial log-weights and adaptation state are binary64.
+#include <algorithm>
+
 #include "ut_kernels.h"
 #include "ut_models.h"
 
@@ -26,14 +28,13 @@ static FunnelModel<K> make_funnel(const DevModelDesc& d) {
   m.dim = d.dim; m.v_scale = d.v_scale; m.total_const = d.total_const;
   return m;
 }
-template <int NMAX>
-static HierModel<NMAX> make_hier(const DevModelDesc& d) {
-  HierModel<NMAX> m;
+template <int NMAX, bool DENSE = false>
+static HierModel<NMAX, DENSE> make_hier(const DevModelDesc& d) {
+  HierModel<NMAX, DENSE> m;
   m.y_f = d.y_f; m.y_d = d.y_d; m.nobs = d.nobs; m.lane_f = d.lane_f; m.lane_d = d.lane_d;
   m.J = d.J; m.max_n = d.max_n; m.mu_loc = d.mu_loc; m.mu_scale = d.mu_scale;
   m.tau_scale = d.tau_scale; m.total_const = d.total_const;
   m.inv_mu_scale = d.inv_mu_scale; m.inv_tau_scale = d.inv_tau_scale;
-  m.dense = (d.dense && d.max_n == NMAX) ? 1 : 0;
   return m;
 }
 
@@ -41,7 +42,11 @@ template <class F>
 static hipError_t dispatch(const DevModelDesc& d, F&& f) {
   switch (d.kind) {
     case MODEL_HIER:
-      // observations per group held in registers when they fit
+      // observations per group held in registers when they fit; equal group sizes of
+      // 16, 32 or 64 take the mask-free instances
+      if (d.dense && d.max_n == 16) return f(make_hier<16, true>(d));
+      if (d.dense && d.max_n == 32) return f(make_hier<32, true>(d));
+      if (d.dense && d.max_n == 64) return f(make_hier<64, true>(d));
       if (d.max_n <= 8) return f(make_hier<8>(d));
       if (d.max_n <= 16) return f(make_hier<16>(d));
       if (d.max_n <= 32) return f(make_hier<32>(d));
@@ -733,10 +738,16 @@ __device__ inline double window_progress(long long count, int wl) {
   return r < 1.0 ? r : 1.0;
 }
 
+// Each iteration's history is staged through LDS in blocks of `cb` chains (positions
+// as one contiguous copy, weights computed lane-parallel), so the sequential fold reads
+// LDS instead of waiting on a global load per step.
 __global__ __launch_bounds__(REPLAY_THREADS) void k_mass_replay(
     const float* __restrict__ hist_pos, const double* __restrict__ hist_acc,
     const uint8_t* __restrict__ hist_flags, int C, int P, int it_lo, int it_hi, int wl,
-    int use_accepted, double reg, int min_samples, double* im_d, float* im_f, double* summary) {
+    int use_accepted, double reg, int min_samples, double* im_d, float* im_f, double* summary, int cb) {
+  extern __shared__ double replay_sh[];
+  double* sw = replay_sh;                  // cb weights
+  float* sx = (float*)(replay_sh + cb);    // cb x P positions
   const int t = threadIdx.x;
   double mean[REPLAY_PER_THREAD], m2[REPLAY_PER_THREAD];
 #pragma unroll
@@ -745,46 +756,63 @@ __global__ __launch_bounds__(REPLAY_THREADS) void k_mass_replay(
   double wsum = 0.0, wsq = 0.0;
   for (int it = it_lo; it <= it_hi; ++it) {
     const double progress = window_progress(count, wl);
-    for (int c = 0; c < C; ++c) {
-      const long long row = (long long)(it - 1) * C + c;
-      const uint8_t fl = hist_flags[row];
-      const double ap = hist_acc[row];
-      double w;
-      if (fl & FLAG_DIVERGENT) w = 0.0;
-      else if (use_accepted && (fl & FLAG_ACCEPTED)) w = 1.0;
-      else if (!isfinite_d(ap)) w = 0.0;
-      else {
-        const double rej = ap < 0.0 ? 0.0 : (ap > 1.0 ? 1.0 : ap);
-        w = 1.0 + (rej - 1.0) * progress;
-      }
-      if (w == 0.0) continue;
-      const bool clip = count >= 4;
-      double clip_scale = 8.0;
-      if (count > 4) clip_scale = 8.0 + (5.0 - 8.0) * window_progress(count, wl);
-      const double denom_c = (double)(count - 1 > 1 ? count - 1 : 1);
-      const double nw = wsum + w;
-      const double frac = w / nw;
-#pragma unroll
-      for (int j = 0; j < REPLAY_PER_THREAD; ++j) {
-        const int p = t + j * REPLAY_THREADS;
-        if (p < P) {
-          double x = (double)hist_pos[row * P + p];
-          if (clip) {
-            const double var = m2[j] / denom_c;
-            const double bound = clip_scale * sqrt_d(var > 1e-3 ? var : 1e-3);
-            double delta = x - mean[j];
-            delta = delta > bound ? bound : (delta < -bound ? -bound : delta);
-            x = mean[j] + delta;
-          }
-          const double delta = x - mean[j];
-          mean[j] += frac * delta;
-          const double delta2 = x - mean[j];
-          m2[j] += (w * delta) * delta2;
+    for (int c0 = 0; c0 < C; c0 += cb) {
+      const int nb = C - c0 < cb ? C - c0 : cb;
+      const long long row0 = (long long)(it - 1) * C + c0;
+      __syncthreads();
+      for (int e = t; e < nb * P; e += blockDim.x) sx[e] = hist_pos[row0 * P + e];
+      for (int c = t; c < nb; c += blockDim.x) {
+        const uint8_t fl = hist_flags[row0 + c];
+        const double ap = hist_acc[row0 + c];
+        double w;
+        if (fl & FLAG_DIVERGENT) w = 0.0;
+        else if (use_accepted && (fl & FLAG_ACCEPTED)) w = 1.0;
+        else if (!isfinite_d(ap)) w = 0.0;
+        else {
+          const double rej = ap < 0.0 ? 0.0 : (ap > 1.0 ? 1.0 : ap);
+          w = 1.0 + (rej - 1.0) * progress;
         }
+        sw[c] = w;
       }
-      count += 1;
-      wsum = nw;
-      wsq += w * w;
+      __syncthreads();
+      for (int c = 0; c < nb; ++c) {
+        const double w = sw[c];
+        if (w == 0.0) continue;
+        const bool clip = count >= 4;
+        double clip_scale = 8.0;
+        if (count > 4) clip_scale = 8.0 + (5.0 - 8.0) * window_progress(count, wl);
+        const double denom_c = (double)(count - 1 > 1 ? count - 1 : 1);
+        const double nw = wsum + w;
+        const double frac = w / nw;
+        // |delta| <= bound  <=>  delta^2 denom_c <= clip_scale^2 max(m2, 1e-3 denom_c); with a
+        // 1e-12 relative margin the test below implies the exact comparison, so the divide
+        // and square root are only evaluated for samples near or beyond the bound
+        const double inside = clip_scale * clip_scale * (1.0 - 1e-12);
+#pragma unroll
+        for (int j = 0; j < REPLAY_PER_THREAD; ++j) {
+          const int p = t + j * REPLAY_THREADS;
+          if (p < P) {
+            double x = (double)sx[c * P + p];
+            if (clip) {
+              double delta = x - mean[j];
+              const double floor_m2 = 1e-3 * denom_c;
+              if (!(delta * delta * denom_c <= inside * (m2[j] > floor_m2 ? m2[j] : floor_m2))) {
+                const double var = m2[j] / denom_c;
+                const double bound = clip_scale * sqrt_d(var > 1e-3 ? var : 1e-3);
+                delta = delta > bound ? bound : (delta < -bound ? -bound : delta);
+              }
+              x = mean[j] + delta;
+            }
+            const double delta = x - mean[j];
+            mean[j] += frac * delta;
+            const double delta2 = x - mean[j];
+            m2[j] += (w * delta) * delta2;
+          }
+        }
+        count += 1;
+        wsum = nw;
+        wsq += w * w;
+      }
     }
   }
   const double eff = wsq <= 0.0 ? 0.0 : (wsum * wsum) / wsq;
@@ -886,9 +914,11 @@ hipError_t launch_mass_replay(const float* hist_pos, const double* hist_acc,
                               int wl, int use_accepted, double reg, int min_samples,
                               double* im_d, float* im_f, double* summary, hipStream_t stream) {
   if (P > REPLAY_THREADS * REPLAY_PER_THREAD) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_mass_replay, dim3(1), dim3(REPLAY_THREADS), 0, stream, hist_pos, hist_acc,
+  const int cb = std::max(1, std::min(256, 65536 / (4 * P + 8)));  // staged block within 64 KB of LDS
+  const size_t shmem = (size_t)cb * sizeof(double) + (size_t)cb * P * sizeof(float);
+  hipLaunchKernelGGL(k_mass_replay, dim3(1), dim3(REPLAY_THREADS), shmem, stream, hist_pos, hist_acc,
                      hist_flags, C, P, it_lo, it_hi, wl, use_accepted, reg, min_samples, im_d,
-                     im_f, summary);
+                     im_f, summary, cb);
   return hipGetLastError();
 }
 

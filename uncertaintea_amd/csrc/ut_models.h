@@ -103,8 +103,10 @@ template <> struct Pick<double> {
 // Observations are stored y[i * 64 + j] (zero padded to max_n rows) so that the lanes
 // of one row read consecutive words.  With NMAX > 0 every lane keeps its group's
 // observations in registers for the whole launch (prepare()); NMAX == 0 streams
-// them from memory on every evaluation.  The arithmetic order is the same.
-template <int NMAX>
+// them from memory on every evaluation.  DENSE instances serve plans whose groups all
+// hold exactly NMAX observations and carry no per-observation masks.  The arithmetic
+// order is the same in every instance.
+template <int NMAX, bool DENSE = false>
 struct HierModel {
   static constexpr int K = 2;
   const float* y_f;
@@ -114,7 +116,6 @@ struct HierModel {
   const double* lane_d;
   int J;
   int max_n;
-  int dense;  // 1 when every group has exactly max_n == NMAX observations
   double mu_loc, mu_scale, tau_scale, total_const;
   double inv_mu_scale, inv_tau_scale;
 
@@ -154,7 +155,7 @@ struct HierModel {
     const T theta = mu + tau * qr;
     const int n = ld.n;
     T sd = T(0), ss = T(0);
-    if (NMAX > 0 && dense) {
+    if constexpr (NMAX > 0 && DENSE) {
       // every group has exactly NMAX observations: no per-observation mask; lanes
       // past the last group are zeroed after the loop
 #pragma unroll
