@@ -78,6 +78,9 @@ def main() -> int:
     ap.add_argument("--chains", type=int, default=None)
     ap.add_argument("--forward-velocity", action="store_true",
                     help="glm_chees: ChEES criterion with the forward end velocity (not the reference's sign)")
+    ap.add_argument("--flow-rounds", type=int, default=3,
+                    help="funnel_flow / gp_meads: train the flow on the plain run's draws, then re-train it "
+                         "on each flow run's draws this many times in all (iterative NeuTra)")
     ap.add_argument("--seed", type=int, default=20261203)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -225,6 +228,20 @@ def bench_funnel_flow(args) -> int:
     fres = ut.batched_hmc(model, num_chains=chains, num_samples=steps, num_warmup=warmup // 2, num_leapfrog_steps=L,
                           per_chain_adaptation=False, flow=flow, seed=args.seed + 1)
     flow_s = summarise(fres, "flow")
+    rounds = [dict(flow_s, training_seconds=t_train)]
+    for r in range(2, max(1, args.flow_rounds) + 1):
+        # re-train (warm start) on the flow run's draws, which reach the neck the plain run missed
+        draws = fres.unconstrained_samples.reshape(-1, model.num_params)
+        t0 = time.perf_counter()
+        trace = flow.fit(draws, steps=train_steps, batch=4096, seed=args.seed + r, init_base=False)
+        t_r = time.perf_counter() - t0
+        t_train += t_r
+        del draws
+        log(f"bench: flow round {r} re-trained in {t_r:.2f}s, loss {trace[:20].mean():.2f} -> {trace[-100:].mean():.2f}")
+        fres = ut.batched_hmc(model, num_chains=chains, num_samples=steps, num_warmup=warmup // 2,
+                              num_leapfrog_steps=L, per_chain_adaptation=False, flow=flow, seed=args.seed + r)
+        flow_s = summarise(fres, f"flow round {r}")
+        rounds.append(dict(flow_s, training_seconds=t_r))
     # conditioner GEMM flops per chain-gradient: forward 2 GEMMs + backward 2 GEMMs per layer
     flops_grad = 0.0
     for npass, nact, _ in flow.layout():
@@ -255,10 +272,10 @@ def bench_funnel_flow(args) -> int:
         "warmup": warmup // 2, "ms_per_step": 1e3 * fres.sampling_seconds / steps, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32+bf16", "data": "synthetic (no data; seed %d)" % args.seed,
         "config": {"workload": f"Neal's funnel D={dim}, batched_hmc L={L}, {chains} chains, plain vs affine-coupling "
-                               f"flow ({layers} layers, width {H}) trained on the plain run's {steps} draws/chain",
-                   "chains": chains},
+                               f"flow ({layers} layers, width {H}) trained on the plain run's {steps} draws/chain, "
+                               f"{max(1, args.flow_rounds)} training round(s)", "chains": chains},
         "rhat_max": flow_s["rhat_max"], "rhat_ok": flow_s["rhat_ok"], "min_bulk_ess": flow_s["min_bulk_ess"],
-        "flow": flow_s, "plain": plain_s, "flow_training_seconds": t_train,
+        "flow": flow_s, "plain": plain_s, "flow_training_seconds": t_train, "flow_rounds": rounds,
         "flow_vs_plain": flow_s["value"] / plain_s["value"] if plain_s["value"] > 0 else None,
         "roofline": roofline, "cpu_baseline": cpu,
     }
@@ -307,6 +324,17 @@ def bench_gp_meads(args) -> int:
     fres = ut.batched_meads(model, num_chains=chains, num_samples=steps, num_warmup=warmup, seed=args.seed + 1,
                             initial_params=init, flow=flow)
     flow_s = summarise(fres, "flow")
+    rounds = [dict(flow_s, training_seconds=t_train)]
+    for r in range(2, max(1, args.flow_rounds) + 1):
+        t0 = time.perf_counter()
+        flow.fit(fres.unconstrained_samples.reshape(-1, model.num_params), steps=1000, batch=1024, seed=args.seed + r,
+                 init_base=False)
+        t_r = time.perf_counter() - t0
+        t_train += t_r
+        fres = ut.batched_meads(model, num_chains=chains, num_samples=steps, num_warmup=warmup, seed=args.seed + r,
+                                initial_params=init, flow=flow)
+        flow_s = summarise(fres, f"flow round {r}")
+        rounds.append(dict(flow_s, training_seconds=t_r))
     flops = 7.0 * N ** 3 / 3.0 + 8.0 * N * N  # factorisation with three tangents + the four triangular products
     achieved = plain.sampling_gradient_evals * flops / plain.sampling_kernel_seconds / 1e12
     roofline = {"bound": "mfma", "compute": "binary64 batched Cholesky with tangents (vector FP64)",
@@ -340,7 +368,7 @@ def bench_gp_meads(args) -> int:
                                "without / with an affine-coupling flow", "chains": chains,
                    "init": "prior medians + N(0, 0.1^2) jitter"},
         "rhat_max": flow_s["rhat_max"], "rhat_ok": flow_s["rhat_ok"], "min_bulk_ess": flow_s["min_bulk_ess"],
-        "flow": flow_s, "plain": plain_s, "flow_training_seconds": t_train,
+        "flow": flow_s, "plain": plain_s, "flow_training_seconds": t_train, "flow_rounds": rounds,
         "roofline": roofline, "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
