@@ -61,6 +61,24 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def moments_vs(D, a, b) -> dict:
+    """Posterior means and variances of draws `a` against `b` (chains x draws x params)
+    in units of their combined Monte Carlo standard errors: z = (m_a - m_b) / sqrt(se_a^2
+    + se_b^2) for the mean and for the variance (the latter from the MCSE of the mean of
+    the squared deviations).  max|z| below 4 over a few hundred parameters is what
+    agreement within MCSE looks like."""
+    def mom(x):
+        m = x.mean(axis=(0, 1))
+        dev = (x - m) ** 2
+        return m, np.asarray(D.mcse_mean(x)), dev.mean(axis=(0, 1)), np.asarray(D.mcse_mean(dev))
+    ma, sa, va, sva = mom(np.asarray(a, dtype=np.float64))
+    mb, sb, vb, svb = mom(np.asarray(b, dtype=np.float64))
+    zm = np.abs(ma - mb) / np.sqrt(sa ** 2 + sb ** 2)
+    zv = np.abs(va - vb) / np.sqrt(sva ** 2 + svb ** 2)
+    return {"max_abs_z_mean": float(np.nanmax(zm)), "max_abs_z_var": float(np.nanmax(zv)),
+            "within_mcse": bool(np.nanmax(zm) < 4.0 and np.nanmax(zv) < 4.0)}
+
+
 def cpu_cores() -> int:
     try:
         n = len(os.sched_getaffinity(0))
@@ -76,6 +94,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=None, help="sampling iterations")
     ap.add_argument("--warmup", type=int, default=None, help="warmup iterations")
     ap.add_argument("--chains", type=int, default=None)
+    ap.add_argument("--flow", action="store_true",
+                    help="glm_chees: also train a flow on the plain run's draws and run ChEES with it")
     ap.add_argument("--forward-velocity", action="store_true",
                     help="glm_chees: ChEES criterion with the forward end velocity (not the reference's sign)")
     ap.add_argument("--flow-rounds", type=int, default=3,
@@ -277,6 +297,7 @@ def bench_funnel_flow(args) -> int:
         "rhat_max": flow_s["rhat_max"], "rhat_ok": flow_s["rhat_ok"], "min_bulk_ess": flow_s["min_bulk_ess"],
         "flow": flow_s, "plain": plain_s, "flow_training_seconds": t_train, "flow_rounds": rounds,
         "flow_vs_plain": flow_s["value"] / plain_s["value"] if plain_s["value"] > 0 else None,
+        "flow_moments_vs_plain": moments_vs(D, fres.constrained_samples, plain.constrained_samples),
         "roofline": roofline, "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
@@ -441,6 +462,28 @@ def bench_glm_chees(args) -> int:
         "step_size": float(res.step_size[0]), "divergence_rate": res.divergence_rate(),
         "roofline": roofline, "cpu_baseline": cpu, "vs_cpu": (value / cpu["value"]) if cpu else None,
     }
+    if args.flow:
+        flow = ut.AffineCouplingFlow(model.num_params, 6, 256, seed=args.seed)
+        t0 = time.perf_counter()
+        trace = flow.fit(res.unconstrained_samples.reshape(-1, model.num_params), steps=2000, batch=4096,
+                         seed=args.seed)
+        t_train = time.perf_counter() - t0
+        log(f"bench: flow trained in {t_train:.2f}s, loss {trace[:20].mean():.2f} -> {trace[-100:].mean():.2f}")
+        fres = ut.batched_chees(model, num_chains=chains, num_samples=steps, num_warmup=warmup, seed=args.seed + 1,
+                                forward_velocity=args.forward_velocity, flow=flow)
+        frh, feb = D.rhat_and_ess_bulk(fres.constrained_samples)
+        fmin = float(np.nanmin(feb))
+        fval = fmin / fres.sampling_seconds
+        mom = moments_vs(D, fres.constrained_samples, res.constrained_samples)
+        log(f"bench: flow: rhat {float(np.nanmax(frh)):.4f} min ESS {fmin:.1f} -> {fval:.1f} ESS/s, "
+            f"{float(fres.integration_steps.mean()):.2f} steps/iter; vs plain {mom}")
+        line["flow"] = {"value": fval, "min_bulk_ess": fmin, "rhat_max": float(np.nanmax(frh)),
+                        "sampling_seconds": fres.sampling_seconds, "warmup_seconds": fres.warmup_seconds,
+                        "mean_leapfrog_steps": float(fres.integration_steps.mean()),
+                        "gradient_evals_per_s": fres.sampling_gradient_evals / fres.sampling_seconds,
+                        "step_size": float(fres.step_size[0]), "training_seconds": t_train,
+                        "moments_vs_plain": mom}
+        line["flow_vs_plain"] = fval / value if value > 0 else None
     print(json.dumps(line), flush=True)
     return 0
 
