@@ -50,6 +50,38 @@ def test_gp_cholesky_against_numpy(n):
     say(f"  max rel L error {err:.2e}")
 
 
+def test_gp_one_input_dimension():
+    """D = 1 (two hyperparameters, two tangents: the second kernel instance) against
+    numpy's factor, finite differences of it, and the oracle's density."""
+    say("gp d=1")
+    n = 64
+    X, y, _ = S.gp_poisson_data(n, 1, seed=7, lengthscales=(0.25,))
+    model = M.GaussianProcessPoisson(X, y, noise=1e-2)
+    hyper = np.array([[np.log(0.25), 0.0], [np.log(0.7), np.log(1.5)]])
+    Ld, dLd, ld = model.factor(hyper)
+    assert dLd.shape[1] == 2
+    for c, h in enumerate(hyper):
+        K = S.gp_kernel(X, np.exp(h[:1]), np.exp(h[1]), 1e-2)
+        Ln = np.linalg.cholesky(K)
+        assert np.max(np.abs(Ld[c] - Ln)) <= 1e-11 * np.max(np.abs(Ln))
+        assert ld[c] == pytest.approx(np.linalg.slogdet(K)[1], rel=1e-11, abs=1e-9)
+        for k in range(2):
+            e = np.zeros(2)
+            e[k] = 1e-6
+            Kp = S.gp_kernel(X, np.exp((h + e)[:1]), np.exp((h + e)[1]), 1e-2)
+            Km = S.gp_kernel(X, np.exp((h - e)[:1]), np.exp((h - e)[1]), 1e-2)
+            fd = (np.linalg.cholesky(Kp) - np.linalg.cholesky(Km)) / 2e-6
+            assert np.max(np.abs(dLd[c, k] - fd)) <= 1e-6 * np.max(np.abs(fd)), (c, k)
+    om = B.OracleModel(model)
+    rng = np.random.default_rng(8)
+    C = 32
+    params = np.concatenate([rng.normal(size=(2, C)) * 0.4, rng.normal(size=(n, C))], axis=0).astype(np.float32)
+    v, g = ut.device_batched_logjoint_gradient(model, params)
+    rv, rg, _ = om.gp_eval(params.astype(np.float64))
+    assert np.max(np.abs(v - rv) / np.abs(rv)) <= 1e-6
+    assert np.max(np.abs(g - rg)) <= 1e-5 * max(1.0, np.max(np.abs(rg)))
+
+
 def test_gp_density_against_oracle():
     say("gp density")
     n = 128

@@ -57,3 +57,27 @@ def test_sampler_defaults_match_reference():
     assert (o.max_tree_depth, o.num_leapfrog_steps, o.step_size, o.target_accept) == (10, 10, 0.1, 0.8)
     assert (o.divergence_threshold, o.mass_matrix_regularization, o.mass_matrix_min_samples) == (1000.0, 1e-3, 10)
     assert (o.adapt_step_size, o.adapt_mass_matrix, o.init_max_retries) == (1, 1, 100)
+
+
+def test_product_modules_do_not_use_the_oracle():
+    """The oracle is test infrastructure: no module of the package imports or loads it
+    (the build step compiles it beside the library, which is not using it), and no
+    device source includes it."""
+    pkg = ROOT / "uncertaintea_amd"
+    for f in sorted(pkg.glob("*.py")):
+        text = f.read_text()
+        assert not re.search(r"^\s*(from|import)\s+oracle\b", text, re.M), f.name
+        if f.name != "_build.py":
+            assert "liboracle" not in text and "ORACLE" not in text, f.name
+    for f in sorted((pkg / "csrc").glob("*")):
+        if f.suffix in (".hip", ".h", ".cpp"):
+            assert not re.search(r"#include\s*[<\"][^>\"]*oracle", f.read_text()), f.name
+
+
+def test_bench_refuses_more_than_one_gpu():
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, str(ROOT / "bench.py"), "--gpus", "2"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr
