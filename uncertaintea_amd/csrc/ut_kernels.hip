@@ -747,7 +747,12 @@ __global__ __launch_bounds__(REPLAY_THREADS) void k_mass_replay(
     int use_accepted, double reg, int min_samples, double* im_d, float* im_f, double* summary, int cb) {
   extern __shared__ double replay_sh[];
   double* sw = replay_sh;                  // cb weights
-  float* sx = (float*)(replay_sh + cb);    // cb x P positions
+  double* sfrac = sw + cb;                 // cb w / (running weight sum)
+  double* sclip = sfrac + cb;              // cb clip scale, or -1 before the clip starts
+  double* sden = sclip + cb;               // cb max(count - 1, 1)
+  long long* scnt = (long long*)(sden + cb);  // cb running count before the step
+  double* sstate = (double*)(scnt + cb);   // running weight sum / square sum / count after the block
+  float* sx = (float*)(sstate + 4);        // cb x P positions
   const int t = threadIdx.x;
   double mean[REPLAY_PER_THREAD], m2[REPLAY_PER_THREAD];
 #pragma unroll
@@ -775,15 +780,45 @@ __global__ __launch_bounds__(REPLAY_THREADS) void k_mass_replay(
         sw[c] = w;
       }
       __syncthreads();
+      // the state's scalars in chain order: running count and weight sums (one thread;
+      // the weight sum's additions keep their order) ...
+      if (t == 0) {
+        long long cnt = count;
+        double ws = wsum, wq = wsq;
+        for (int c = 0; c < nb; ++c) {
+          const double w = sw[c];
+          scnt[c] = cnt;
+          if (w != 0.0) {
+            ws = ws + w;
+            sfrac[c] = ws;
+            wq += w * w;
+            cnt += 1;
+          }
+        }
+        sstate[0] = ws;
+        sstate[1] = wq;
+        sstate[2] = (double)cnt;
+      }
+      __syncthreads();
+      // ... then every step's clip scale, divisor and weight fraction in parallel
+      for (int c = t; c < nb; c += blockDim.x) {
+        const double w = sw[c];
+        if (w == 0.0) continue;
+        const long long cnt = scnt[c];
+        double clip_scale = -1.0;
+        if (cnt >= 4) clip_scale = cnt > 4 ? 8.0 + (5.0 - 8.0) * window_progress(cnt, wl) : 8.0;
+        sclip[c] = clip_scale;
+        sden[c] = (double)(cnt - 1 > 1 ? cnt - 1 : 1);
+        sfrac[c] = w / sfrac[c];
+      }
+      __syncthreads();
       for (int c = 0; c < nb; ++c) {
         const double w = sw[c];
         if (w == 0.0) continue;
-        const bool clip = count >= 4;
-        double clip_scale = 8.0;
-        if (count > 4) clip_scale = 8.0 + (5.0 - 8.0) * window_progress(count, wl);
-        const double denom_c = (double)(count - 1 > 1 ? count - 1 : 1);
-        const double nw = wsum + w;
-        const double frac = w / nw;
+        const double clip_scale = sclip[c];
+        const bool clip = clip_scale >= 0.0;
+        const double denom_c = sden[c];
+        const double frac = sfrac[c];
         // |delta| <= bound  <=>  delta^2 denom_c <= clip_scale^2 max(m2, 1e-3 denom_c); with a
         // 1e-12 relative margin the test below implies the exact comparison, so the divide
         // and square root are only evaluated for samples near or beyond the bound
@@ -809,10 +844,10 @@ __global__ __launch_bounds__(REPLAY_THREADS) void k_mass_replay(
             m2[j] += (w * delta) * delta2;
           }
         }
-        count += 1;
-        wsum = nw;
-        wsq += w * w;
       }
+      wsum = sstate[0];
+      wsq = sstate[1];
+      count = (long long)sstate[2];
     }
   }
   const double eff = wsq <= 0.0 ? 0.0 : (wsum * wsum) / wsq;
@@ -914,8 +949,9 @@ hipError_t launch_mass_replay(const float* hist_pos, const double* hist_acc,
                               int wl, int use_accepted, double reg, int min_samples,
                               double* im_d, float* im_f, double* summary, hipStream_t stream) {
   if (P > REPLAY_THREADS * REPLAY_PER_THREAD) return hipErrorInvalidValue;
-  const int cb = std::max(1, std::min(256, 65536 / (4 * P + 8)));  // staged block within 64 KB of LDS
-  const size_t shmem = (size_t)cb * sizeof(double) + (size_t)cb * P * sizeof(float);
+  // staged block within 64 KB of LDS: per chain 5 scalars of 8 bytes and P positions
+  const int cb = std::max(1, std::min(256, (65536 - 64) / (4 * P + 40)));
+  const size_t shmem = (size_t)cb * 40 + 32 + (size_t)cb * P * sizeof(float);
   hipLaunchKernelGGL(k_mass_replay, dim3(1), dim3(REPLAY_THREADS), shmem, stream, hist_pos, hist_acc,
                      hist_flags, C, P, it_lo, it_hi, wl, use_accepted, reg, min_samples, im_d,
                      im_f, summary, cb);

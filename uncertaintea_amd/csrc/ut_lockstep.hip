@@ -211,18 +211,39 @@ __global__ void k_ls_record(LockstepState s, LockstepRecord r, SamplerParams sp,
 // (api_batched_chees.jl:171-203)
 __global__ void k_chees_means(LockstepState s, LockstepRecord r, const double* im_d, int P, int C,
                               double* stats) {
+  // the sums run in chain order; each group of 8 chains is loaded before it is added
+  // so that the loads overlap instead of each waiting for the previous addition
+  constexpr int G = 8;
   for (int a = threadIdx.x; a < P; a += blockDim.x) {
     double isum = 0.0, psum = 0.0, wsum = 0.0;
     long long icount = 0;
-    for (int c = 0; c < C; ++c) {
-      if (r.ifinite[c]) {
-        isum += (double)s.initials[(long long)c * P + a];
-        icount += 1;
+    for (int c0 = 0; c0 < C; c0 += G) {
+      float iv[G], qv[G];
+      double wv[G];
+      bool iok[G], qok[G];
+#pragma unroll
+      for (int k = 0; k < G; ++k) {
+        const int c = c0 + k;
+        if (c < C) {
+          iok[k] = r.ifinite[c] != 0;
+          qok[k] = !(r.flags[c] & FLAG_DIVERGENT) && r.qfinite[c];
+          iv[k] = s.initials[(long long)c * P + a];
+          qv[k] = s.wq[(long long)c * P + a];
+          wv[k] = r.accept_prob[c];
+        } else {
+          iok[k] = qok[k] = false;
+        }
       }
-      if (!(r.flags[c] & FLAG_DIVERGENT) && r.qfinite[c]) {
-        const double w = r.accept_prob[c];
-        wsum += w;
-        psum += w * (double)s.wq[(long long)c * P + a];
+#pragma unroll
+      for (int k = 0; k < G; ++k) {
+        if (iok[k]) {
+          isum += (double)iv[k];
+          icount += 1;
+        }
+        if (qok[k]) {
+          wsum += wv[k];
+          psum += wv[k] * (double)qv[k];
+        }
       }
     }
     r.imean[a] = icount > 0 ? isum / (double)icount : 0.0;
@@ -261,21 +282,40 @@ __global__ void k_chees_terms(LockstepState s, LockstepRecord r, const double* i
 // [2] harmonic-mean acceptance of non-divergent chains, [3] mean acceptance with
 // divergent chains counted as 0
 __global__ void k_chees_final(LockstepRecord r, int C, double* stats) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  // chunks of the per-chain values are staged in LDS by the whole workgroup; one
+  // thread then adds them in chain order
+  constexpr int CH = 1024;
+  __shared__ double s_term[CH], s_acc[CH];
+  __shared__ uint8_t s_flag[CH], s_inc[CH];
   double num = 0.0, den = 0.0, hsum = 0.0, msum = 0.0;
   long long hcount = 0;
-  for (int c = 0; c < C; ++c) {
-    const bool div = r.flags[c] & FLAG_DIVERGENT;
-    if (r.include && r.include[c]) {
-      num += r.term[c];
-      den += r.accept_prob[c] + 1e-20;
+  for (int c0 = 0; c0 < C; c0 += CH) {
+    const int nb = C - c0 < CH ? C - c0 : CH;
+    __syncthreads();
+    for (int k = threadIdx.x; k < nb; k += blockDim.x) {
+      const int c = c0 + k;
+      s_acc[k] = r.accept_prob[c];
+      s_flag[k] = r.flags[c];
+      s_inc[k] = (r.include && r.include[c]) ? 1 : 0;
+      s_term[k] = s_inc[k] ? r.term[c] : 0.0;
     }
-    if (!div) {
-      hsum += 1.0 / r.accept_prob[c];
-      hcount += 1;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int k = 0; k < nb; ++k) {
+        const bool div = s_flag[k] & FLAG_DIVERGENT;
+        if (s_inc[k]) {
+          num += s_term[k];
+          den += s_acc[k] + 1e-20;
+        }
+        if (!div) {
+          hsum += 1.0 / s_acc[k];
+          hcount += 1;
+        }
+        msum += div ? 0.0 : s_acc[k];
+      }
     }
-    msum += div ? 0.0 : r.accept_prob[c];
   }
+  if (threadIdx.x != 0) return;
   double hm = 0.0;
   if (hcount > 0) {
     hm = (double)hcount / hsum;
@@ -351,7 +391,7 @@ hipError_t launch_chees_stats(const LockstepState& s, const LockstepRecord& r, c
   }
   LockstepRecord rr = r;
   if (!trajectory_terms) rr.include = nullptr;
-  hipLaunchKernelGGL(k_chees_final, dim3(1), dim3(64), 0, stream, rr, C, stats);
+  hipLaunchKernelGGL(k_chees_final, dim3(1), dim3(256), 0, stream, rr, C, stats);
   return hipGetLastError();
 }
 
