@@ -101,6 +101,9 @@ def main() -> int:
     ap.add_argument("--flow-rounds", type=int, default=3,
                     help="funnel_flow / gp_meads: train the flow on the plain run's draws, then re-train it "
                          "on each flow run's draws this many times in all (iterative NeuTra)")
+    ap.add_argument("--flow-target-accept", type=float, default=0.9,
+                    help="funnel_flow: target acceptance of the flow runs' step-size adaptation (0.8 gives "
+                         "~1 %% divergent transitions in the neck)")
     ap.add_argument("--seed", type=int, default=20261203)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -246,20 +249,23 @@ def bench_funnel_flow(args) -> int:
         f"loss {trace[:20].mean():.2f} -> {trace[-100:].mean():.2f}")
     del draws
     fres = ut.batched_hmc(model, num_chains=chains, num_samples=steps, num_warmup=warmup // 2, num_leapfrog_steps=L,
-                          per_chain_adaptation=False, flow=flow, seed=args.seed + 1)
+                          per_chain_adaptation=False, flow=flow, seed=args.seed + 1,
+                          target_accept=args.flow_target_accept)
     flow_s = summarise(fres, "flow")
     rounds = [dict(flow_s, training_seconds=t_train)]
     for r in range(2, max(1, args.flow_rounds) + 1):
         # re-train (warm start) on the flow run's draws, which reach the neck the plain run missed
         draws = fres.unconstrained_samples.reshape(-1, model.num_params)
         t0 = time.perf_counter()
-        trace = flow.fit(draws, steps=train_steps, batch=4096, seed=args.seed + r, init_base=False)
+        # warm start, twice the first round's steps
+        trace = flow.fit(draws, steps=2 * train_steps, batch=4096, seed=args.seed + r, init_base=False)
         t_r = time.perf_counter() - t0
         t_train += t_r
         del draws
         log(f"bench: flow round {r} re-trained in {t_r:.2f}s, loss {trace[:20].mean():.2f} -> {trace[-100:].mean():.2f}")
         fres = ut.batched_hmc(model, num_chains=chains, num_samples=steps, num_warmup=warmup // 2,
-                              num_leapfrog_steps=L, per_chain_adaptation=False, flow=flow, seed=args.seed + r)
+                              num_leapfrog_steps=L, per_chain_adaptation=False, flow=flow, seed=args.seed + r,
+                              target_accept=args.flow_target_accept)
         flow_s = summarise(fres, f"flow round {r}")
         rounds.append(dict(flow_s, training_seconds=t_r))
     # conditioner GEMM flops per chain-gradient: forward 2 GEMMs + backward 2 GEMMs per layer
