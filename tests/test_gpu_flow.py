@@ -139,6 +139,28 @@ def test_hmc_shared_bitexact_against_twin():
         assert np.array_equal(getattr(dev, k_dev), orc[k_orc]), k_dev
 
 
+def test_flow_graph_replay_matches_eager(monkeypatch):
+    """The flow-wrapped batched gradient is captured once per buffer set as a HIP graph
+    and replayed at every leapfrog step, with the samplers' own kernels in between:
+    whole shared-HMC and ChEES runs must equal the eager launches bit for bit."""
+    say("flow graph replay vs eager")
+    model = M.NealsFunnel(10, 3.0)
+    flow = ut.AffineCouplingFlow(model.num_params, 4, 256, seed=3)
+    flow.weights = _random_weights(flow, 4, scale=0.3)
+    runs = {}
+    for mode in ("0", "1"):
+        monkeypatch.setenv("UT_FLOW_GRAPHS", mode)
+        h = ut.batched_hmc(model, num_chains=128, num_samples=20, num_warmup=30, num_leapfrog_steps=6,
+                           per_chain_adaptation=False, flow=flow, seed=21)
+        c = ut.batched_chees(model, num_chains=128, num_samples=20, num_warmup=30, flow=flow, seed=22)
+        runs[mode] = (h, c)
+    for a, b in zip(runs["0"], runs["1"]):
+        assert np.array_equal(a.unconstrained_samples.view(np.uint32), b.unconstrained_samples.view(np.uint32))
+        assert np.array_equal(a.logjoint_values, b.logjoint_values)
+        assert np.array_equal(a.step_size, b.step_size)
+        assert np.array_equal(a.integration_steps, b.integration_steps)
+
+
 def test_flow_hmc_funnel_gains_over_plain():
     """NeuTra on Neal's funnel (dim 10): the same shared-adaptation HMC budget with
     and without a flow trained on exact funnel draws."""

@@ -12,6 +12,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -127,7 +128,24 @@ struct ut_workspace {
   double* d_summary = nullptr;
   GlmScratch glm;  // MODEL_GLM_MFMA only
   GpScratch gp;    // MODEL_GP only
+  // flow-wrapped batched gradients captured as HIP graphs, one per buffer set
+  struct GradGraph {
+    const float* q;
+    float* lj;
+    float* grad;
+    bool need_lj;
+    const void* flow;
+    unsigned gen;
+    hipGraphExec_t exec;
+  };
+  std::vector<GradGraph> graphs;
+  bool use_graphs = true;
+  void clear_graphs() {
+    for (GradGraph& g : graphs) (void)hipGraphExecDestroy(g.exec);
+    graphs.clear();
+  }
   ~ut_workspace() {
+    clear_graphs();
     void* ptrs[] = {d_params, d_values, d_grads, st.pos, st.grad, st.lj,
                     st.step, st.dual, st.im_f, d_im, d_summary,
                     glm.bhi, glm.blo, glm.gpart, glm.llpart};
@@ -397,6 +415,11 @@ int ut_workspace_create(const ut_plan* plan, int32_t num_chains, int32_t precisi
   ws->C = num_chains;
   ws->P = plan->P;
   ws->precision = precision;
+  {
+    // UT_FLOW_GRAPHS=0 keeps the flow-wrapped gradient eager (for comparison runs)
+    const char* g = std::getenv("UT_FLOW_GRAPHS");
+    ws->use_graphs = !(g && std::strcmp(g, "0") == 0);
+  }
   const size_t PC = (size_t)ws->P * num_chains;
   const size_t es = precision == UT_FLOAT32 ? 4 : 8;
   hipError_t e = hipStreamCreateWithFlags(&ws->stream, hipStreamNonBlocking);
@@ -449,14 +472,45 @@ static hipError_t model_grad(ut_workspace* ws, const float* q, float* lj, float*
 // z: q = f(z), the model's logjoint and gradient at q, then
 //   log p_z(z) = log p(q) + log |det df/dz|,  grad_z = (dq/dz)^T grad_q + grad_z logdet
 // (the NeuTra reparameterisation).
-static hipError_t batched_grad(ut_workspace* ws, const float* q, float* lj, float* grad, bool need_lj) {
-  if (!ws->flow) return model_grad(ws, q, lj, grad, need_lj);
+static hipError_t flow_grad_eager(ut_workspace* ws, const float* q, float* lj, float* grad, bool need_lj) {
   Flow& f = ws->flow->f;
   hipError_t e = flow_forward(f, q, nullptr, nullptr, ws->C, ws->stream);
   if (e == hipSuccess) e = model_grad(ws, f.sc.x, lj, f.sc.g, need_lj);
   if (e == hipSuccess) e = flow_backward(f, f.sc.g, grad, ws->C, ws->stream);
   if (e == hipSuccess && need_lj) e = flow_add_logdet(lj, f.sc.logdet, ws->C, ws->stream);
   return e;
+}
+
+// With a flow attached, the ~40 launches of one batched gradient are host-bound, so
+// the sequence is captured once per buffer set as a HIP graph (kernels and
+// device-to-device copies only) and replayed.
+static hipError_t batched_grad(ut_workspace* ws, const float* q, float* lj, float* grad, bool need_lj) {
+  if (!ws->flow) return model_grad(ws, q, lj, grad, need_lj);
+  if (!ws->use_graphs) return flow_grad_eager(ws, q, lj, grad, need_lj);
+  const Flow& f = ws->flow->f;
+  for (const ut_workspace::GradGraph& g : ws->graphs) {
+    if (g.flow != &f || g.gen != f.gen) {
+      ws->clear_graphs();
+      break;
+    }
+  }
+  for (const ut_workspace::GradGraph& g : ws->graphs)
+    if (g.q == q && g.lj == lj && g.grad == grad && g.need_lj == need_lj) return hipGraphLaunch(g.exec, ws->stream);
+  hipError_t e = hipStreamBeginCapture(ws->stream, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) return e;
+  const hipError_t ec = flow_grad_eager(ws, q, lj, grad, need_lj);
+  hipGraph_t graph = nullptr;
+  e = hipStreamEndCapture(ws->stream, &graph);
+  if (ec != hipSuccess || e != hipSuccess) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return ec != hipSuccess ? ec : e;
+  }
+  hipGraphExec_t exec = nullptr;
+  e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) return e;
+  ws->graphs.push_back({q, lj, grad, need_lj, &f, f.gen, exec});
+  return hipGraphLaunch(exec, ws->stream);
 }
 
 static int density_call(ut_workspace* ws, const void* params, void* values, void* gradients) {
@@ -1627,6 +1681,7 @@ int ut_workspace_set_flow(ut_workspace* ws, ut_flow* flow) {
       return fail(UT_E_DIM, "ut_workspace_set_flow: the workspace needs UT_FLOAT32 and num_chains a multiple of 128");
     UT_HIP(flow_scratch(flow->f, ws->C));
   }
+  ws->clear_graphs();
   ws->flow = flow;
   return UT_OK;
 }

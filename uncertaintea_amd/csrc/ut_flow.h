@@ -39,14 +39,11 @@ struct FlowScratch {
   std::vector<uint16_t*> hb;  // per layer B x H bf16
   std::vector<float*> s;      // per layer B x na
   std::vector<float*> xa;     // per layer B x na: x_a e^s (forward) / the inverted x_a (inverse)
-  float* U = nullptr;         // B x H
   float* O = nullptr;         // B x 128
   uint16_t* gO = nullptr;     // B x 128
   float* gOf = nullptr;       // B x 128
-  float* gH = nullptr;        // B x H
   uint16_t* gU = nullptr;     // B x H
   float* gUf = nullptr;       // B x H
-  float* gXp = nullptr;       // B x 128
   uint16_t* tA = nullptr;     // transposes for the weight gradients (128 or H) x B
   uint16_t* tB = nullptr;
   float* x = nullptr;         // B x P work positions
@@ -70,6 +67,7 @@ struct Flow {
   FlowScratch sc;           // sized for sc.B rows
   std::vector<void*> owned;
   hipStream_t stream = nullptr;
+  unsigned gen = 0;         // bumped whenever the scratch buffers move (captured graphs go stale)
 };
 
 hipError_t flow_alloc(Flow& f, int P, int L, int H);

@@ -4,12 +4,12 @@
 // Rows are chains (sampling) or draws (training); every per-row quantity is
 // chain-major like the samplers' state.  Per coupling layer the forward pass is
 //   gather passive x -> bf16 [B x kp]          k_gather
-//   U = X_p W1^T                               MFMA GEMM (K = kp)
-//   h = tanh(U + b1) (fp32 and bf16)           k_bias_tanh
+//   h = tanh(X_p W1^T + b1) (fp32 and bf16)    MFMA GEMM (K = kp), bias and tanh in its epilogue
 //   O = h W2^T                                 MFMA GEMM (K = H)
 //   s, t from O + b2; x_a <- x_a e^s + t       k_coupling_fwd   (k_coupling_inv: inverse)
 // and the backward pass mirrors it with the transposed bf16 weight copies
-//   dO (coupling), dH = dO W2, dU = dH (1 - h^2), dX_p = dU W1, scatter-add;
+//   dO (coupling), dU = (dO W2) (1 - h^2) (GEMM + epilogue), dX_p = dU W1 added
+//   into the passive coordinates' gradient by the GEMM's epilogue;
 // training adds dW2 = dO^T h and dW1 = dU^T X_p (transposed operands through LDS
 // tiles) and the bias column sums.
 #include <math.h>
@@ -84,15 +84,6 @@ __global__ void k_gather(const float* __restrict__ x, int P, const int* __restri
   const long long b = i / kp;
   const int k = (int)(i % kp);
   xp[i] = k < np ? f2bf(x[b * P + pidx[k]]) : (bf16_t)0;
-}
-
-__global__ void k_bias_tanh(const float* __restrict__ U, const float* __restrict__ b1, int H, float* __restrict__ h,
-                            bf16_t* __restrict__ hb, long long n) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const float v = tanhf(U[i] + b1[i % H]);
-  h[i] = v;
-  hb[i] = f2bf(v);
 }
 
 // Coupling kernels: one wavefront per row, lane j handles active coordinates
@@ -209,24 +200,6 @@ __global__ __launch_bounds__(256) void k_coupling_inv_bwd(float* __restrict__ g,
     gO[b * op + j] = 0;
     gOf[b * op + j] = 0.0f;
   }
-}
-
-__global__ void k_tanh_bwd(const float* __restrict__ gH, const float* __restrict__ h, bf16_t* __restrict__ gU,
-                           float* __restrict__ gUf, long long n) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const float v = gH[i] * (1.0f - h[i] * h[i]);
-  gU[i] = f2bf(v);
-  if (gUf) gUf[i] = v;
-}
-
-__global__ void k_scatter_add(float* __restrict__ g, int P, const float* __restrict__ gXp, int kp,
-                              const int* __restrict__ pidx, int np, long long n) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const long long b = i / np;
-  const int k = (int)(i % np);
-  g[b * P + pidx[k]] += gXp[b * kp + k];
 }
 
 // column sums in two passes (fixed order): COLSUM_CHUNKS row chunks, then chunks
@@ -384,6 +357,7 @@ hipError_t flow_scratch(Flow& f, int B) {
   FlowScratch& s = f.sc;
   s = FlowScratch();
   s.B = B;
+  f.gen += 1;
   int kpm = FLOW_PAD, opm = FLOW_PAD;
   for (const FlowLayer& ly : f.layers) {
     kpm = std::max(kpm, round_up(ly.np, FLOW_PAD));
@@ -405,14 +379,11 @@ hipError_t flow_scratch(Flow& f, int B) {
     if (e == hipSuccess) e = own(f, &s.xa[l], (size_t)B * std::max(ly.na, 1));
   }
   const size_t wide = std::max<size_t>(H, (size_t)std::max(kpm, opm));
-  if (e == hipSuccess) e = own(f, &s.U, (size_t)B * H);
   if (e == hipSuccess) e = own(f, &s.O, (size_t)B * opm);
   if (e == hipSuccess) e = own(f, &s.gO, (size_t)B * opm);
   if (e == hipSuccess) e = own(f, &s.gOf, (size_t)B * opm);
-  if (e == hipSuccess) e = own(f, &s.gH, (size_t)B * H);
   if (e == hipSuccess) e = own(f, &s.gU, (size_t)B * H);
   if (e == hipSuccess) e = own(f, &s.gUf, (size_t)B * H);
-  if (e == hipSuccess) e = own(f, &s.gXp, (size_t)B * kpm);
   if (e == hipSuccess) e = own(f, &s.tA, (size_t)B * wide);
   if (e == hipSuccess) e = own(f, &s.tB, (size_t)B * wide);
   if (e == hipSuccess) e = own(f, &s.x, (size_t)B * f.P);
@@ -447,10 +418,10 @@ static hipError_t conditioner(Flow& f, int l, int B, hipStream_t st) {
   const int kp = round_up(ly.np, FLOW_PAD), op = round_up(2 * ly.na, FLOW_PAD), H = f.H;
   hipLaunchKernelGGL(k_gather, dim3(nblk((long long)B * kp)), dim3(T256), 0, st, s.x, f.P, ly.pidx, ly.np, kp,
                      s.xp[l], (long long)B * kp);
-  hipError_t e = launch_gemm_nt(s.xp[l], kp, ly.W1b, kp, s.U, H, B, H, kp, 1, 0, st);
+  // h = tanh(x_p W1^T + b1) straight from the GEMM's epilogue
+  hipError_t e = launch_gemm_nt_bias_tanh(s.xp[l], kp, ly.W1b, kp, B, H, kp,
+                                          GemmEpiBiasTanh{f.params + ly.ob1, s.h[l], s.hb[l], H}, st);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_bias_tanh, dim3(nblk((long long)B * H)), dim3(T256), 0, st, s.U, f.params + ly.ob1, H, s.h[l],
-                     s.hb[l], (long long)B * H);
   return launch_gemm_nt(s.hb[l], H, ly.W2b, H, s.O, op, B, op, H, 1, 0, st);
 }
 
@@ -459,14 +430,12 @@ static hipError_t conditioner_bwd(Flow& f, int l, int B, bool weights, hipStream
   FlowLayer& ly = f.layers[l];
   FlowScratch& s = f.sc;
   const int kp = round_up(ly.np, FLOW_PAD), op = round_up(2 * ly.na, FLOW_PAD), H = f.H;
-  hipError_t e = launch_gemm_nt(s.gO, op, ly.W2Tb, op, s.gH, H, B, H, op, 1, 0, st);
+  // gU = (gO W2) (1 - h^2) and g[passive] += gU W1, both in the GEMMs' epilogues
+  hipError_t e = launch_gemm_nt_tanh_bwd(s.gO, op, ly.W2Tb, op, B, H, op,
+                                         GemmEpiTanhBwd{s.h[l], s.gU, weights ? s.gUf : nullptr, H}, st);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_tanh_bwd, dim3(nblk((long long)B * H)), dim3(T256), 0, st, s.gH, s.h[l], s.gU,
-                     weights ? s.gUf : nullptr, (long long)B * H);
-  e = launch_gemm_nt(s.gU, H, ly.W1Tb, H, s.gXp, kp, B, kp, H, 1, 0, st);
+  e = launch_gemm_nt_scatter_add(s.gU, H, ly.W1Tb, H, B, kp, H, GemmEpiScatterAdd{s.g, f.P, ly.pidx, ly.np}, st);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_scatter_add, dim3(nblk((long long)B * ly.np)), dim3(T256), 0, st, s.g, f.P, s.gXp, kp,
-                     ly.pidx, ly.np, (long long)B * ly.np);
   if (!weights) return hipGetLastError();
   // dW2 (op x H) = gO^T h ; dW1 (H x kp) = gU^T X_p ; bias column sums
   hipLaunchKernelGGL(k_transpose_bf16, dim3(op / 32, B / 32), dim3(T256), 0, st, s.gO, B, op, s.tA);

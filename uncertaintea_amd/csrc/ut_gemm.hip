@@ -17,10 +17,42 @@ namespace ut {
 namespace {
 constexpr int BK = 32, LDS_STRIDE = BK + 8;  // +8 bf16 breaks bank aliasing
 
-template <int S>
+// epilogues: what happens to output element (m, n) of split z
+struct EpiStore {
+  float* C;
+  int ldc;
+  long long split_stride;
+  __device__ void operator()(long long m, long long n, int z, float v) const {
+    C[(long long)z * split_stride + m * ldc + n] = v;
+  }
+};
+struct EpiBiasTanh {
+  GemmEpiBiasTanh e;
+  __device__ void operator()(long long m, long long n, int, float v) const {
+    const float t = tanhf(v + e.bias[n]);
+    e.h[m * e.ld + n] = t;
+    e.hb[m * e.ld + n] = f2bf(t);
+  }
+};
+struct EpiTanhBwd {
+  GemmEpiTanhBwd e;
+  __device__ void operator()(long long m, long long n, int, float v) const {
+    const float h = e.h[m * e.ld + n];
+    const float g = v * (1.0f - h * h);
+    e.gU[m * e.ld + n] = f2bf(g);
+    if (e.gUf) e.gUf[m * e.ld + n] = g;
+  }
+};
+struct EpiScatterAdd {
+  GemmEpiScatterAdd e;
+  __device__ void operator()(long long m, long long n, int, float v) const {
+    if (n < e.np) e.g[m * e.P + e.pidx[n]] += v;
+  }
+};
+
+template <int S, class Epi>
 __global__ __launch_bounds__(256) void k_gemm_nt(const bf16_t* __restrict__ A, int lda,
-                                                 const bf16_t* __restrict__ Bt, int ldb,
-                                                 float* __restrict__ C, int ldc, long long split_stride,
+                                                 const bf16_t* __restrict__ Bt, int ldb, Epi epi,
                                                  int K, int k_per_split) {
   constexpr int BM = 64 * S, BN = 64 * S;
   __shared__ __attribute__((aligned(16))) bf16_t As[BM * LDS_STRIDE];
@@ -84,7 +116,6 @@ __global__ __launch_bounds__(256) void k_gemm_nt(const bf16_t* __restrict__ A, i
         for (int j = 0; j < S; ++j) acc[i][j] = mfma32(a[i], b[j], acc[i][j]);
     }
   }
-  float* Cz = C + (long long)blockIdx.z * split_stride;
 #pragma unroll
   for (int i = 0; i < S; ++i)
 #pragma unroll
@@ -93,7 +124,7 @@ __global__ __launch_bounds__(256) void k_gemm_nt(const bf16_t* __restrict__ A, i
       for (int r = 0; r < 16; ++r) {
         const long long m = m0 + 32 * S * wm + 32 * i + mfma32_row(lane, r);
         const long long n = n0 + 32 * S * wn + 32 * j + (lane & 31);
-        Cz[m * ldc + n] = acc[i][j][r];
+        epi(m, n, (int)blockIdx.z, acc[i][j][r]);
       }
 }
 
@@ -109,20 +140,41 @@ __global__ void k_splitk_sum(const float* __restrict__ part, int splits, int M, 
 }
 }  // namespace
 
-hipError_t launch_gemm_nt(const bf16_t* A, int lda, const bf16_t* Bt, int ldb, float* C, int ldc,
-                          int M, int N, int K, int splits, long long split_stride, hipStream_t stream) {
+template <class Epi>
+static hipError_t gemm_nt(const bf16_t* A, int lda, const bf16_t* Bt, int ldb, const Epi& epi, int M, int N, int K,
+                          int splits, hipStream_t stream) {
   if (M % 64 || N % 64 || K % BK || splits < 1) return hipErrorInvalidValue;
   const int ksteps = K / BK;
   const int per = (ksteps + splits - 1) / splits * BK;
   const bool wide = M % 128 == 0 && N % 128 == 0 && (long long)(M / 128) * (N / 128) * splits >= 256;
   if (wide) {
-    hipLaunchKernelGGL(k_gemm_nt<2>, dim3(N / 128, M / 128, splits), dim3(256), 0, stream, A, lda, Bt, ldb, C, ldc,
-                       split_stride, K, per);
+    hipLaunchKernelGGL((k_gemm_nt<2, Epi>), dim3(N / 128, M / 128, splits), dim3(256), 0, stream, A, lda, Bt, ldb,
+                       epi, K, per);
   } else {
-    hipLaunchKernelGGL(k_gemm_nt<1>, dim3(N / 64, M / 64, splits), dim3(256), 0, stream, A, lda, Bt, ldb, C, ldc,
-                       split_stride, K, per);
+    hipLaunchKernelGGL((k_gemm_nt<1, Epi>), dim3(N / 64, M / 64, splits), dim3(256), 0, stream, A, lda, Bt, ldb,
+                       epi, K, per);
   }
   return hipGetLastError();
+}
+
+hipError_t launch_gemm_nt(const bf16_t* A, int lda, const bf16_t* Bt, int ldb, float* C, int ldc,
+                          int M, int N, int K, int splits, long long split_stride, hipStream_t stream) {
+  return gemm_nt(A, lda, Bt, ldb, EpiStore{C, ldc, split_stride}, M, N, K, splits, stream);
+}
+
+hipError_t launch_gemm_nt_bias_tanh(const bf16_t* A, int lda, const bf16_t* Bt, int ldb, int M, int N, int K,
+                                    const GemmEpiBiasTanh& e, hipStream_t stream) {
+  return gemm_nt(A, lda, Bt, ldb, EpiBiasTanh{e}, M, N, K, 1, stream);
+}
+
+hipError_t launch_gemm_nt_tanh_bwd(const bf16_t* A, int lda, const bf16_t* Bt, int ldb, int M, int N, int K,
+                                   const GemmEpiTanhBwd& e, hipStream_t stream) {
+  return gemm_nt(A, lda, Bt, ldb, EpiTanhBwd{e}, M, N, K, 1, stream);
+}
+
+hipError_t launch_gemm_nt_scatter_add(const bf16_t* A, int lda, const bf16_t* Bt, int ldb, int M, int N, int K,
+                                      const GemmEpiScatterAdd& e, hipStream_t stream) {
+  return gemm_nt(A, lda, Bt, ldb, EpiScatterAdd{e}, M, N, K, 1, stream);
 }
 
 hipError_t launch_gemm_nt_splitk(const bf16_t* A, int lda, const bf16_t* Bt, int ldb, float* C, int ldc, int M,
