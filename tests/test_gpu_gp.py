@@ -24,7 +24,7 @@ def say(*a):
     sys.stdout.flush()
 
 
-@pytest.mark.parametrize("n", [64, 96])
+@pytest.mark.parametrize("n", [32, 64, 96])
 def test_gp_cholesky_against_numpy(n):
     say("gp cholesky", n)
     X, y, _ = S.gp_poisson_data(n, 2, seed=3)
