@@ -91,3 +91,17 @@ def test_textbook_values():
     assert np.allclose(D.ess_bulk(many), [D.ess_bulk(iid), D.ess_bulk(ar[:, :2000])])
     r, e = D.rhat_and_ess_bulk(many)
     assert np.allclose(r, D.rhat(many)) and np.allclose(e, D.ess_bulk(many))
+
+
+def test_parameter_chunks_agree_with_whole_array(monkeypatch):
+    """Large draw arrays are processed in parameter chunks; every statistic is per
+    parameter, so chunking changes nothing beyond reduction-order rounding."""
+    import uncertaintea_amd.diagnostics as DD
+    rng = np.random.default_rng(3)
+    x = rng.normal(size=(8, 400, 50)).astype(np.float32)
+    whole = (*D.rhat_and_ess_bulk(x), D.mcse_mean(x), D.ess_tail(x))
+    monkeypatch.setattr(DD, "_CHUNK_DRAWS", 1 << 13)
+    chunked = (*D.rhat_and_ess_bulk(x), D.mcse_mean(x), D.ess_tail(x))
+    for a, b in zip(whole, chunked):
+        assert a.shape == b.shape == (50,)
+        np.testing.assert_allclose(a, b, rtol=1e-12)

@@ -12,12 +12,32 @@ FFT, which equals the reference's direct lag sum up to rounding.
 The arithmetic runs in torch (binary64) on the GPU when one is visible, else on the
 CPU, so diagnostics of a million-draw run take seconds.  Functions accept numpy
 arrays shaped (chains, samples) for one parameter or (chains, samples, params) for
-many, and return numpy arrays.
+many, and return numpy arrays.  Every statistic is per parameter, so large arrays are
+processed in parameter chunks of at most 2^27 draws (the rank transform needs several
+binary64 copies of its input).
 """
 from __future__ import annotations
 
+import functools
+
 import numpy as np
 import torch
+
+_CHUNK_DRAWS = 1 << 27
+
+
+def _per_param_chunks(fn):
+    @functools.wraps(fn)
+    def wrapper(draws, *args, **kwargs):
+        a = np.asarray(draws)
+        if a.ndim < 3 or a.size <= _CHUNK_DRAWS:
+            return fn(a, *args, **kwargs)
+        step = max(1, _CHUNK_DRAWS // (a.shape[0] * a.shape[1]))
+        outs = [fn(a[:, :, p0:p0 + step], *args, **kwargs) for p0 in range(0, a.shape[2], step)]
+        if isinstance(outs[0], tuple):
+            return tuple(np.concatenate([o[i] for o in outs]) for i in range(len(outs[0])))
+        return np.concatenate(outs)
+    return wrapper
 
 
 def _device():
@@ -98,6 +118,7 @@ def split_chains(draws) -> np.ndarray:
     return _np(_split(_t(draws)))
 
 
+@_per_param_chunks
 def rhat(draws, method: str = "rank") -> np.ndarray:
     """Per-parameter R-hat of (C, S[, P]) draws; method 'split' or 'rank'."""
     if method not in ("split", "rank"):
@@ -115,16 +136,19 @@ def rhat(draws, method: str = "rank") -> np.ndarray:
     return _np(torch.maximum(bulk, folded))
 
 
+@_per_param_chunks
 def ess(draws) -> np.ndarray:
     """Split ESS of the raw draws (the reference's `ess`)."""
     return _np(_ess_of(_split(_t(draws))))
 
 
+@_per_param_chunks
 def ess_bulk(draws) -> np.ndarray:
     """Bulk ESS: split ESS of the rank-normalised split draws."""
     return _np(_ess_of(_rank_normal(_split(_t(draws)))))
 
 
+@_per_param_chunks
 def rhat_and_ess_bulk(draws):
     """(rank-normalised R-hat, bulk ESS) sharing one transfer and one rank transform."""
     sp = _split(_t(draws))
@@ -141,18 +165,26 @@ def rhat_and_ess_bulk(draws):
     return _np(torch.maximum(bulk_r, folded)), _np(ess_b)
 
 
+@_per_param_chunks
 def ess_tail(draws) -> np.ndarray:
     """Tail ESS: min over the 5% and 95% quantile indicator ESS."""
     sp = _split(_t(draws))
     flat = sp.reshape(sp.shape[0] * sp.shape[1], -1)
     out = None
+    srt = torch.sort(flat, dim=0).values if flat.shape[0] > 16_000_000 else None
     for q in (0.05, 0.95):
-        qv = torch.quantile(flat, q, dim=0) if flat.shape[0] <= 16_000_000 else None
+        if srt is None:
+            qv = torch.quantile(flat, q, dim=0)
+        else:  # torch.quantile's size limit: the same linear interpolation on sorted draws
+            pos = q * (flat.shape[0] - 1)
+            lo = int(np.floor(pos))
+            qv = srt[lo] + (pos - lo) * (srt[min(lo + 1, flat.shape[0] - 1)] - srt[lo])
         e = _ess_of((sp <= qv.reshape(sp.shape[2:])).to(torch.float64))
         out = e if out is None else torch.minimum(out, e)
     return _np(out)
 
 
+@_per_param_chunks
 def mcse_mean(draws) -> np.ndarray:
     d = _t(draws)
     sd = d.reshape((-1,) + tuple(d.shape[2:])).std(dim=0, unbiased=True)
