@@ -139,6 +139,37 @@ def test_hmc_shared_bitexact_against_twin():
         assert np.array_equal(getattr(dev, k_dev), orc[k_orc]), k_dev
 
 
+@pytest.mark.parametrize("P", [101, 2, 300])
+def test_fused_layers_match_unfused(monkeypatch, P):
+    """The LDS-staged coupling-layer kernels of the sampling path (passive and output
+    widths of 128) against the per-step kernels: forward values, log-determinants and
+    the sampler's VJP bit for bit; P = 300 exercises the per-step fallback."""
+    say("fused vs unfused flow layers", P)
+    flows = {}
+    for mode in ("0", "1"):
+        monkeypatch.setenv("UT_FLOW_FUSED", mode)
+        flows[mode] = ut.AffineCouplingFlow(P, 6, 256, seed=5)
+    w = _random_weights(flows["0"], 6, scale=0.3)
+    for f in flows.values():
+        f.weights = w
+    rng = np.random.default_rng(7)
+    z = rng.normal(size=(256, P)).astype(np.float32)
+    gq = rng.normal(size=(256, P)).astype(np.float32)
+    q0, l0 = flows["0"].forward(z)
+    q1, l1 = flows["1"].forward(z)
+    assert np.array_equal(q0.view(np.uint32), q1.view(np.uint32))
+    assert np.array_equal(l0.view(np.uint32), l1.view(np.uint32))
+    g0 = flows["0"].vjp(z, gq)
+    g1 = flows["1"].vjp(z, gq)
+    assert np.array_equal(g0.view(np.uint32), g1.view(np.uint32))
+    if P == 101:
+        model = M.NealsFunnel(100, 3.0)
+        runs = [ut.batched_hmc(model, num_chains=128, num_samples=10, num_warmup=20, num_leapfrog_steps=5,
+                               per_chain_adaptation=False, flow=f, seed=23) for f in (flows["0"], flows["1"])]
+        assert np.array_equal(runs[0].unconstrained_samples.view(np.uint32),
+                              runs[1].unconstrained_samples.view(np.uint32))
+
+
 def test_flow_graph_replay_matches_eager(monkeypatch):
     """The flow-wrapped batched gradient is captured once per buffer set as a HIP graph
     and replayed at every leapfrog step, with the samplers' own kernels in between:

@@ -68,6 +68,7 @@ struct Flow {
   std::vector<void*> owned;
   hipStream_t stream = nullptr;
   unsigned gen = 0;         // bumped whenever the scratch buffers move (captured graphs go stale)
+  bool fused = true;        // LDS-staged layer kernels on the sampling path where the widths allow
 };
 
 hipError_t flow_alloc(Flow& f, int P, int L, int H);

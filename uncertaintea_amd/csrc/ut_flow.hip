@@ -14,6 +14,9 @@
 // tiles) and the bias column sums.
 #include <math.h>
 
+#include <cstdlib>
+#include <cstring>
+
 #include "ut_flow.h"
 #include "ut_gemm.h"
 #include "ut_math.h"
@@ -202,6 +205,219 @@ __global__ __launch_bounds__(256) void k_coupling_inv_bwd(float* __restrict__ g,
   }
 }
 
+// ---- fused coupling layers (sampling path, small widths) ---------------------------
+// One workgroup per 32 rows runs a whole layer with both weight matrices staged in LDS
+// once (all of their loads in flight together): the forward gathers the passive inputs,
+// multiplies by W1 with bias and tanh applied to the accumulators, multiplies by W2 and
+// applies the coupling; the backward runs the coupling's backward, multiplies by W2 with
+// the tanh backward applied to the accumulators, multiplies by W1 and adds into the
+// passive coordinates.  Each accumulator sums k in the order of k_gemm_nt (32-wide
+// steps, two MFMAs each), so results equal the per-step kernels bit for bit.  Serves
+// H = 256 with passive inputs and conditioner outputs of width 128 (both matrices then
+// fit in LDS: 2 x 256 x 136 bf16 + the row tiles).
+constexpr int LR = 32;              // rows per workgroup
+constexpr int LH = 256;             // hidden width served
+constexpr int LW = 128;             // passive / output width served
+constexpr int LHP = LH + 8, LWP = LW + 8;  // LDS row strides (bf16)
+constexpr size_t LAYER_LDS = (size_t)(2 * LH * LWP + LR * LHP) * sizeof(bf16_t);
+
+// acc[t] = A[32 x K] (LDS, row stride lda) * Bs[col_t .. col_t + 31][K]^T (LDS, row
+// stride ldb) for the wavefront's tiles col_t = 32 wave + 128 t, t < ntw
+template <int NT>
+__device__ inline void lds_gemm(const bf16_t* As, int lda, const bf16_t* Bs, int ldb, int K, int ntw,
+                                f32x16 (&acc)[NT]) {
+  const int lane = threadIdx.x & 63, l32 = lane & 31, half = lane >> 5, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int t = 0; t < NT; ++t)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[t][r] = 0.0f;
+  for (int k0 = 0; k0 < K; k0 += 32) {
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks) {
+      const bf16x8 a = *(const bf16x8*)(As + l32 * lda + k0 + 16 * ks + 8 * half);
+#pragma unroll
+      for (int t = 0; t < NT; ++t)
+        if (t < ntw) {
+          const bf16x8 b = *(const bf16x8*)(Bs + (32 * wave + 128 * t + l32) * ldb + k0 + 16 * ks + 8 * half);
+          acc[t] = mfma32(a, b, acc[t]);
+        }
+    }
+  }
+}
+
+// the layer's two weight matrices (LH x LW and LW x LH bf16, row-major) -> LDS with row
+// strides LWP and LHP: every thread issues all of its 16-byte loads before any store
+__device__ inline void stage_weights(bf16_t* d1, const bf16_t* __restrict__ s1, bf16_t* d2,
+                                     const bf16_t* __restrict__ s2) {
+  constexpr int N = LH * LW / 8 / 256;  // chunks per thread and matrix
+  uint4 v1[N], v2[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    const int e = threadIdx.x + 256 * i;
+    v1[i] = *(const uint4*)(s1 + (long long)(e / (LW / 8)) * LW + (e % (LW / 8)) * 8);
+    v2[i] = *(const uint4*)(s2 + (long long)(e / (LH / 8)) * LH + (e % (LH / 8)) * 8);
+  }
+  __builtin_amdgcn_sched_barrier(0);  // keep every load ahead of the first store
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    const int e = threadIdx.x + 256 * i;
+    *(uint4*)(d1 + (e / (LW / 8)) * LWP + (e % (LW / 8)) * 8) = v1[i];
+    *(uint4*)(d2 + (e / (LH / 8)) * LHP + (e % (LH / 8)) * 8) = v2[i];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_layer_fwd(float* __restrict__ x, int P, const int* __restrict__ pidx, int np,
+                                                   const bf16_t* __restrict__ W1b, const float* __restrict__ b1,
+                                                   const bf16_t* __restrict__ W2b, float* __restrict__ O,
+                                                   const float* __restrict__ b2, const int* __restrict__ aidx, int na,
+                                                   float c, float* __restrict__ h_out, float* __restrict__ s_out,
+                                                   float* __restrict__ xs_out, float* __restrict__ logdet) {
+  extern __shared__ __attribute__((aligned(16))) bf16_t lsh[];
+  bf16_t* W1s = lsh;                  // LH x LWP: W1 [H x kp]
+  bf16_t* W2s = lsh + LH * LWP;       // LW x LHP: W2 [op x H]
+  bf16_t* R = lsh + 2 * LH * LWP;     // LR x LHP: passive inputs (stride LWP), then h
+  const long long r0 = (long long)blockIdx.x * LR;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, l32 = lane & 31;
+  stage_weights(W1s, W1b, W2s, W2b);
+  {
+    constexpr int G = LR * LW / 256;
+    float xv[G];
+#pragma unroll
+    for (int i = 0; i < G; ++i) {
+      const int e = tid + 256 * i, r = e / LW, k = e % LW;
+      xv[i] = k < np ? x[(r0 + r) * P + pidx[k]] : 0.0f;
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int i = 0; i < G; ++i) {
+      const int e = tid + 256 * i, r = e / LW, k = e % LW;
+      R[r * LWP + k] = k < np ? f2bf(xv[i]) : (bf16_t)0;
+    }
+  }
+  __syncthreads();
+  f32x16 acc[2];
+  lds_gemm<2>(R, LWP, W1s, LWP, LW, 2, acc);
+  __syncthreads();  // every wavefront is done with the passive inputs; R now takes h
+#pragma unroll
+  for (int t = 0; t < 2; ++t)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int row = mfma32_row(lane, r), col = 32 * wave + 128 * t + l32;
+      const float v = tanhf(acc[t][r] + b1[col]);
+      h_out[(r0 + row) * LH + col] = v;
+      R[row * LHP + col] = f2bf(v);
+    }
+  __syncthreads();
+  f32x16 acc2[1];
+  lds_gemm<1>(R, LHP, W2s, LHP, LH, 1, acc2);
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int row = mfma32_row(lane, r), col = 32 * wave + l32;
+    O[(r0 + row) * LW + col] = acc2[0][r];
+  }
+  __syncthreads();
+  // coupling, as k_coupling_fwd (na <= 64: lane j owns active coordinate j), eight rows
+  // per wavefront with all of their loads issued first
+  constexpr int RW = LR / 4;
+  const bool act = lane < na;
+  float os[RW], ot[RW], xv[RW];
+  long long ia[RW];
+#pragma unroll
+  for (int rr = 0; rr < RW; ++rr) {
+    const long long b = r0 + wave * RW + rr;
+    if (act) {
+      os[rr] = O[b * LW + lane];
+      ot[rr] = O[b * LW + na + lane];
+      ia[rr] = b * P + aidx[lane];
+      xv[rr] = x[ia[rr]];
+    }
+  }
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int rr = 0; rr < RW; ++rr) {
+    const long long b = r0 + wave * RW + rr;
+    float sum = 0.0f;
+    if (act) {
+      const float sv = c * tanhf((os[rr] + b2[lane]) / c);
+      const float tv = ot[rr] + b2[na + lane];
+      const float xs = xv[rr] * expf(sv);
+      x[ia[rr]] = xs + tv;
+      s_out[b * na + lane] = sv;
+      xs_out[b * na + lane] = xs;
+      sum += sv;
+    }
+    sum = wave_sum_f(sum);
+    if (lane == 0) logdet[b] += sum;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_layer_bwd(float* __restrict__ g, int P, const float* __restrict__ s_in,
+                                                   const float* __restrict__ xs_in, const int* __restrict__ aidx,
+                                                   int na, float c, const bf16_t* __restrict__ W2Tb,
+                                                   const float* __restrict__ h_in, const bf16_t* __restrict__ W1Tb,
+                                                   const int* __restrict__ pidx, int np) {
+  extern __shared__ __attribute__((aligned(16))) bf16_t lsh[];
+  bf16_t* W2Ts = lsh;                 // LH x LWP: W2^T [H x op]
+  bf16_t* W1Ts = lsh + LH * LWP;      // LW x LHP: W1^T [kp x H]
+  bf16_t* R = lsh + 2 * LH * LWP;     // LR x LHP: dO (stride LWP), then dU
+  const long long r0 = (long long)blockIdx.x * LR;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, l32 = lane & 31;
+  stage_weights(W2Ts, W2Tb, W1Ts, W1Tb);
+  // coupling backward, as k_coupling_fwd_bwd (na <= 64: lane j owns active coordinate j),
+  // into LDS, eight rows per wavefront with all of their loads issued first
+  {
+    constexpr int RW = LR / 4;
+    const bool act = lane < na;
+    float gv[RW], sv[RW], xv[RW];
+    long long ia[RW];
+#pragma unroll
+    for (int rr = 0; rr < RW; ++rr) {
+      const long long b = r0 + wave * RW + rr;
+      if (act) {
+        ia[rr] = b * P + aidx[lane];
+        gv[rr] = g[ia[rr]];
+        sv[rr] = s_in[b * na + lane];
+        xv[rr] = xs_in[b * na + lane];
+      }
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int rr = 0; rr < RW; ++rr) {
+      const int row = wave * RW + rr;
+      if (act) {
+        const float ga = gv[rr];
+        const float gs = ga * xv[rr] + 1.0f;
+        const float r = sv[rr] / c;
+        const float gos = gs * (1.0f - r * r);
+        R[row * LWP + lane] = f2bf(gos);
+        R[row * LWP + na + lane] = f2bf(ga);
+        g[ia[rr]] = ga * expf(sv[rr]);
+      }
+      for (int j = 2 * na + lane; j < LW; j += 64) R[row * LWP + j] = (bf16_t)0;
+    }
+  }
+  __syncthreads();
+  f32x16 acc[2];
+  lds_gemm<2>(R, LWP, W2Ts, LWP, LW, 2, acc);
+  __syncthreads();  // every wavefront is done with dO; R now takes dU
+#pragma unroll
+  for (int t = 0; t < 2; ++t)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int row = mfma32_row(lane, r), col = 32 * wave + 128 * t + l32;
+      const float hv = h_in[(r0 + row) * LH + col];
+      R[row * LHP + col] = f2bf(acc[t][r] * (1.0f - hv * hv));
+    }
+  __syncthreads();
+  f32x16 acc2[1];
+  lds_gemm<1>(R, LHP, W1Ts, LHP, LH, 1, acc2);
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int row = mfma32_row(lane, r), col = 32 * wave + l32;
+    if (col < np) g[(r0 + row) * P + pidx[col]] += acc2[0][r];
+  }
+}
+
 // column sums in two passes (fixed order): COLSUM_CHUNKS row chunks, then chunks
 constexpr int COLSUM_CHUNKS = 64;
 __global__ void k_colsum_part(const float* __restrict__ M, int B, int n, float* __restrict__ part) {
@@ -304,6 +520,17 @@ hipError_t flow_alloc(Flow& f, int P, int L, int H) {
   f.P = P;
   f.L = L;
   f.H = H;
+  {
+    // UT_FLOW_FUSED=0 keeps the per-step kernels on the sampling path (for comparison runs)
+    const char* e = std::getenv("UT_FLOW_FUSED");
+    f.fused = !(e && std::strcmp(e, "0") == 0);
+    static bool lds_set = false;  // the layer kernels use more than 64 KB of dynamic LDS
+    if (!lds_set) {
+      (void)hipFuncSetAttribute((const void*)k_layer_fwd, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LAYER_LDS);
+      (void)hipFuncSetAttribute((const void*)k_layer_bwd, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LAYER_LDS);
+      lds_set = true;
+    }
+  }
   long long off = 2LL * P;
   f.layers.assign(L, FlowLayer());
   std::vector<std::vector<int>> pid(L), aid(L);
@@ -411,6 +638,12 @@ static void colsum(const float* M, int B, int n, float* out, float* part, hipStr
   hipLaunchKernelGGL(k_colsum_final, dim3(nblk(n)), dim3(T256), 0, st, part, n, out);
 }
 
+// the LDS-staged layer kernels serve H = 256 with passive and output widths of 128
+static bool fused_layer(const Flow& f, int l) {
+  const FlowLayer& ly = f.layers[l];
+  return f.fused && f.H == LH && round_up(ly.np, FLOW_PAD) == LW && round_up(2 * ly.na, FLOW_PAD) == LW;
+}
+
 // conditioner of layer l on the current rows of sc.x: sc.O = h W2^T (bias added by the coupling)
 static hipError_t conditioner(Flow& f, int l, int B, hipStream_t st) {
   FlowLayer& ly = f.layers[l];
@@ -460,6 +693,12 @@ hipError_t flow_forward(Flow& f, const float* z, float* q, float* logdet, int B,
   hipLaunchKernelGGL(k_base_fwd, dim3(nblk(n)), dim3(T256), 0, st, z, f.params, f.P, s.x, n);
   for (int l = 0; l < f.L; ++l) {
     FlowLayer& ly = f.layers[l];
+    if (fused_layer(f, l)) {
+      hipLaunchKernelGGL(k_layer_fwd, dim3(B / LR), dim3(T256), LAYER_LDS, st, s.x, f.P, ly.pidx, ly.np, ly.W1b,
+                         f.params + ly.ob1, ly.W2b, s.O, f.params + ly.ob2, ly.aidx, ly.na, f.clamp, s.h[l], s.s[l],
+                         s.xa[l], s.logdet);
+      continue;
+    }
     hipError_t e = conditioner(f, l, B, st);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_coupling_fwd, dim3((B + 3) / 4), dim3(T256), 0, st, s.x, f.P, s.O,
@@ -488,6 +727,11 @@ hipError_t flow_backward(Flow& f, const float* g_q, float* g_z, int B, hipStream
   for (int l = f.L - 1; l >= 0; --l) {
     FlowLayer& ly = f.layers[l];
     const int op = round_up(2 * ly.na, FLOW_PAD);
+    if (fused_layer(f, l)) {
+      hipLaunchKernelGGL(k_layer_bwd, dim3(B / LR), dim3(T256), LAYER_LDS, st, s.g, f.P, s.s[l], s.xa[l], ly.aidx,
+                         ly.na, f.clamp, ly.W2Tb, s.h[l], ly.W1Tb, ly.pidx, ly.np);
+      continue;
+    }
     hipLaunchKernelGGL(k_coupling_fwd_bwd, dim3((B + 3) / 4), dim3(T256), 0, st, s.g, f.P, s.s[l], s.xa[l], ly.aidx,
                        ly.na, op, f.clamp, s.gO, (float*)nullptr, B);
     hipError_t e = conditioner_bwd(f, l, B, false, st);
