@@ -319,6 +319,33 @@ int ut_flow_loss_grad(ut_flow* flow, const float* q, int64_t n, double* loss, fl
 /* attach (or detach with NULL) a flow: the lock-step samplers then precondition
  * with it.  The workspace must be UT_FLOAT32 with num_chains a multiple of 128.   */
 int ut_workspace_set_flow(ut_workspace* ws, ut_flow* flow);
+/* Reparameterised-ELBO fit of the flow against the workspace's model: the
+ * reference's flow ADVI (batched_advi(...; guide = :flow, elbo = :standard),
+ * src/inference/vi.jl:700-870, Adam ascent src/inference/vi.jl:205-227, flat-norm
+ * clip src/inference/vi/guides.jl:492-502) on the device.  Each iteration draws
+ * num_chains particles z ~ N(0, I) (Philox, key = (seed, iteration)), maps them
+ * through the flow, scores them with the workspace's gradient engine (the plain
+ * model, whether or not a flow is attached) and takes one Adam ascent step
+ * (beta 0.9 / 0.999, eps 1e-8, fresh moments per call) on the mean over the valid
+ * particles (finite logjoint, log-determinant and gradient) of the gradient of
+ * g . f(z) + log|det df/dz| with g = grad log p(f(z)) held fixed.
+ * elbo_history[i] = the sampled standard ELBO (mean log p - log q over the valid
+ * particles), grad_norm_history[i] = the gradient norm, clipped at gradient_clip
+ * (INFINITY: no clipping); either may be NULL.  The flow keeps the last weights;
+ * best_weights (ut_flow_num_weights floats, or NULL) receives the weights of the
+ * best ELBO seen and best_elbo that value.  UT_E_ARG when an iteration has no
+ * valid particle (reference: ArgumentError "encountered only non-finite ...").
+ * The workspace must be UT_FLOAT32 with num_chains a multiple of 128.            */
+int ut_flow_fit_elbo(ut_workspace* ws, ut_flow* flow, int32_t num_iterations, double learning_rate,
+                     double gradient_clip, uint64_t seed, double* elbo_history, double* grad_norm_history,
+                     float* best_weights, double* best_elbo);
+/* One evaluation of that objective at given particles z (num_chains x P host
+ * rows): the sampled ELBO, the unclipped mean weight gradient over the valid
+ * particles (ut_flow_num_weights floats), and optionally the model gradient at
+ * f(z) that was held fixed (num_chains x P, zero on invalid particles) and the
+ * validity flags (num_chains).  The analogue of ut_flow_loss_grad for the ELBO.   */
+int ut_flow_elbo_grad(ut_workspace* ws, ut_flow* flow, const float* z, double* elbo, float* grad, float* g_q,
+                      int32_t* valid);
 
 /* ---- pinned host memory ----------------------------------------------------------
  * Page-locked host buffers for sampler outputs: the samplers copy each launch

@@ -50,6 +50,7 @@ def _load():
         "orc_flow_inverse": (None, [i32, i32, i32, dbl, i32, vp, vp, i64, vp, vp]),
         "orc_flow_loss_grad": (dbl, [i32, i32, i32, dbl, i32, vp, vp, i64, vp]),
         "orc_flow_num_weights": (i64, [i32, i32, i32]),
+        "orc_flow_elbo_grad": (None, [i32, i32, i32, dbl, i32, vp, vp, vp, vp, i64, vp]),
         "orc_chees_halton": (None, [i32, vp]),
         "orc_chees_steps": (i32, [dbl, dbl, dbl]),
         "orc_chees_trajectory": (None, [dbl, dbl, dbl, dbl, dbl, dbl, i32, vp, vp, vp, vp, i32, vp]),
@@ -341,6 +342,17 @@ class OracleFlow:
         loss = _load().orc_flow_loss_grad(self.P, self.L, self.H, self.clamp, int(self.bf16), _p(self.w), _p(q),
                                           q.shape[0], _p(g))
         return float(loss), g
+
+    def elbo_grad(self, z, g_q, valid=None):
+        """Mean over valid rows of the weight gradient of g_q . f(z) + log|det df/dz|
+        with g_q held fixed (the flow ADVI gradient, src/inference/vi.jl:807-826)."""
+        z = np.ascontiguousarray(np.asarray(z, dtype=np.float64).reshape(-1, self.P))
+        g_q = np.ascontiguousarray(np.asarray(g_q, dtype=np.float64).reshape(-1, self.P))
+        v = None if valid is None else np.ascontiguousarray(np.asarray(valid, dtype=np.int32))
+        g = np.empty(self.w.shape[0])
+        _load().orc_flow_elbo_grad(self.P, self.L, self.H, self.clamp, int(self.bf16), _p(self.w), _p(z), _p(g_q),
+                                   _p(v), z.shape[0], _p(g))
+        return g
 
 
 def meads_max_eig(columns):

@@ -1846,8 +1846,10 @@ struct FlowSpec {
     }
   }
   // q = f(z); returns log|det df/dz|.  With g_q, also overwrites g_q by
-  // d/dz [g_q . f(z) + logdet].
-  double forward(const double* z, double* q, double* g_q) const {
+  // d/dz [g_q . f(z) + logdet]; with gw as well, accumulates the gradient of that
+  // quantity (g_q held fixed) over the weights (the reparameterised ELBO gradient
+  // of the reference's flow ADVI, src/inference/vi.jl:807-826).
+  double forward(const double* z, double* q, double* g_q, double* gw = nullptr) const {
     std::vector<double> x(P);
     double ld = 0.0;
     for (int i = 0; i < P; ++i) {
@@ -1881,9 +1883,16 @@ struct FlowSpec {
         dT[i] = ga;
         g[ly.a[i]] = ga * std::exp(S[l][i]);
       }
-      cond_bwd(ly, X[l].data(), Hs[l], S[l], dS, dT, g.data(), nullptr);
+      cond_bwd(ly, X[l].data(), Hs[l], S[l], dS, dT, g.data(), gw);
     }
-    for (int i = 0; i < P; ++i) g_q[i] = g[i] * std::exp((double)w[P + i]);
+    for (int i = 0; i < P; ++i) {
+      const double e = std::exp((double)w[P + i]);
+      if (gw) {
+        gw[i] += g[i];
+        gw[P + i] += g[i] * e * z[i] + 1.0;
+      }
+      g_q[i] = g[i] * e;
+    }
     return ld;
   }
   // z = f^-1(q); returns log|det df^-1/dq|.  With gw, accumulates the gradient of
@@ -2218,6 +2227,33 @@ double orc_flow_loss_grad(int32_t P, int32_t L, int32_t H, double clamp, int32_t
   }
   for (long long i = 0; i < np; ++i) grad[i] = total[i] / (double)n;
   return loss_total / (double)n;
+}
+// Mean over the valid rows (valid[r] != 0, or all rows when valid is NULL) of the
+// weight gradient of g_q . f(z) + log|det df/dz| with g_q held fixed.
+void orc_flow_elbo_grad(int32_t P, int32_t L, int32_t H, double clamp, int32_t bf16, const float* weights,
+                        const double* z, const double* g_q, const int32_t* valid, int64_t n, double* grad) {
+  FlowSpec fs(P, L, H, clamp, bf16 != 0, weights);
+  const long long np = fs.nparams();
+  std::vector<double> total((size_t)np, 0.0);
+  int64_t nv = 0;
+#pragma omp parallel
+  {
+    std::vector<double> gw((size_t)np, 0.0), q(P), g(P);
+    int64_t cnt = 0;
+#pragma omp for schedule(static)
+    for (int64_t r = 0; r < n; ++r) {
+      if (valid && !valid[r]) continue;
+      std::copy(g_q + r * P, g_q + (r + 1) * P, g.begin());
+      fs.forward(z + r * P, q.data(), g.data(), gw.data());
+      ++cnt;
+    }
+#pragma omp critical
+    {
+      for (long long i = 0; i < np; ++i) total[i] += gw[i];
+      nv += cnt;
+    }
+  }
+  for (long long i = 0; i < np; ++i) grad[i] = nv ? total[i] / (double)nv : 0.0;
 }
 int64_t orc_flow_num_weights(int32_t P, int32_t L, int32_t H) {
   FlowSpec fs(P, L, H, 1.0, false, nullptr);

@@ -9,11 +9,13 @@
 // device result buffers, copied out once at the end.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <limits>
 #include <string>
 #include <vector>
 
@@ -1683,6 +1685,134 @@ int ut_workspace_set_flow(ut_workspace* ws, ut_flow* flow) {
   }
   ws->clear_graphs();
   ws->flow = flow;
+  return UT_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+int elbo_check(ut_workspace* ws, ut_flow* flow, const char* who) {
+  if (!ws || !flow) return fail(UT_E_ARG, std::string(who) + ": null workspace or flow");
+  if (flow->f.P != ws->P) return fail(UT_E_DIM, std::string(who) + ": flow and workspace parameter counts differ");
+  if (ws->C % FLOW_PAD != 0 || ws->precision != UT_FLOAT32)
+    return fail(UT_E_DIM, std::string(who) + ": the workspace needs UT_FLOAT32 and num_chains a multiple of 128");
+  return UT_OK;
+}
+
+// Device buffers of one ELBO evaluation over the workspace's C particles.
+struct ElboBuffers {
+  DeviceBuffers buf;
+  float *z = nullptr, *lj = nullptr, *rw = nullptr;
+  double *logw = nullptr, *part = nullptr, *stats = nullptr;
+  hipError_t alloc(int B, int P) {
+    hipError_t e = buf.alloc(&z, (size_t)B * P);
+    if (e == hipSuccess) e = buf.alloc(&lj, (size_t)B);
+    if (e == hipSuccess) e = buf.alloc(&rw, (size_t)B);
+    if (e == hipSuccess) e = buf.alloc(&logw, (size_t)B);
+    if (e == hipSuccess) e = buf.alloc(&part, 256);
+    if (e == hipSuccess) e = buf.alloc(&stats, 4);
+    return e;
+  }
+};
+
+// particles z (in b.z) -> flow -> model logjoint and gradient -> per-particle
+// validity and log-weights -> summed weight gradient (f.grads) -> b.stats
+hipError_t elbo_eval(ut_workspace* ws, Flow& f, ElboBuffers& b, double clip) {
+  const int B = ws->C;
+  hipStream_t st = ws->stream;
+  hipError_t e = flow_forward_train(f, b.z, B, st);
+  if (e == hipSuccess) e = model_grad(ws, f.sc.x, b.lj, f.sc.g, true);
+  if (e == hipSuccess) e = flow_elbo_rows(f, b.z, b.lj, B, b.rw, b.logw, st);
+  if (e == hipSuccess) e = flow_elbo_backward(f, b.z, b.rw, B, st);
+  if (e == hipSuccess) e = flow_elbo_stats(f, b.rw, b.logw, B, clip, b.part, b.stats, st);
+  return e;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ut_flow_elbo_grad(ut_workspace* ws, ut_flow* flow, const float* z, double* elbo, float* grad, float* g_q,
+                      int32_t* valid) {
+  int rc = elbo_check(ws, flow, "ut_flow_elbo_grad");
+  if (rc) return rc;
+  if (!z || !grad) return fail(UT_E_ARG, "ut_flow_elbo_grad: null pointer");
+  Flow& f = flow->f;
+  const int B = ws->C, P = ws->P;
+  hipStream_t st = ws->stream;
+  UT_HIP(hipStreamSynchronize(f.stream));
+  UT_HIP(flow_scratch(f, B));
+  ElboBuffers b;
+  UT_HIP(b.alloc(B, P));
+  UT_HIP(hipMemcpyAsync(b.z, z, (size_t)B * P * 4, hipMemcpyHostToDevice, st));
+  UT_HIP(elbo_eval(ws, f, b, std::numeric_limits<double>::infinity()));
+  double h[4];
+  std::vector<float> rw((size_t)B);
+  if (g_q) {  // the model gradient at f(z) (zero on invalid rows) before the backward pass consumed it
+    UT_HIP(hipMemcpyAsync(b.z, z, (size_t)B * P * 4, hipMemcpyHostToDevice, st));
+    UT_HIP(flow_forward_train(f, b.z, B, st));
+    UT_HIP(model_grad(ws, f.sc.x, b.lj, f.sc.g, true));
+    UT_HIP(flow_elbo_rows(f, b.z, b.lj, B, b.rw, b.logw, st));
+    UT_HIP(hipMemcpyAsync(g_q, f.sc.g, (size_t)B * P * 4, hipMemcpyDeviceToHost, st));
+  }
+  UT_HIP(hipMemcpyAsync(h, b.stats, sizeof(h), hipMemcpyDeviceToHost, st));
+  UT_HIP(hipMemcpyAsync(rw.data(), b.rw, (size_t)B * 4, hipMemcpyDeviceToHost, st));
+  UT_HIP(hipMemcpyAsync(grad, f.grads, (size_t)f.nparams * 4, hipMemcpyDeviceToHost, st));
+  UT_HIP(hipStreamSynchronize(st));
+  if (valid)
+    for (int i = 0; i < B; ++i) valid[i] = rw[i] != 0.0f;
+  if (elbo) *elbo = h[1];
+  const float sc = h[0] > 0 ? (float)(1.0 / h[0]) : 0.0f;
+  for (long long i = 0; i < f.nparams; ++i) grad[i] *= sc;
+  return UT_OK;
+}
+
+int ut_flow_fit_elbo(ut_workspace* ws, ut_flow* flow, int32_t num_iterations, double learning_rate,
+                     double gradient_clip, uint64_t seed, double* elbo_history, double* grad_norm_history,
+                     float* best_weights, double* best_elbo) {
+  int rc = elbo_check(ws, flow, "ut_flow_fit_elbo");
+  if (rc) return rc;
+  if (num_iterations < 0) return fail(UT_E_ARG, "ut_flow_fit_elbo: num_iterations must be >= 0");
+  if (!(learning_rate > 0)) return fail(UT_E_ARG, "ut_flow_fit_elbo: learning_rate must be > 0");
+  if (!(gradient_clip > 0)) return fail(UT_E_ARG, "ut_flow_fit_elbo: gradient_clip must be > 0 (inf: no clipping)");
+  Flow& f = flow->f;
+  const int B = ws->C, P = ws->P;
+  hipStream_t st = ws->stream;
+  UT_HIP(hipStreamSynchronize(f.stream));  // the flow's own calls run on its stream
+  UT_HIP(flow_scratch(f, B));
+  ElboBuffers b;
+  UT_HIP(b.alloc(B, P));
+  DeviceBuffers buf;
+  float* best = nullptr;
+  UT_HIP(buf.alloc(&best, (size_t)f.nparams));
+  UT_HIP(hipMemcpyAsync(best, f.params, (size_t)f.nparams * 4, hipMemcpyDeviceToDevice, st));
+  // fresh Adam moments per fit, as each reference ADVI run starts its own
+  UT_HIP(hipMemsetAsync(f.adam_m, 0, (size_t)f.nparams * 4, st));
+  UT_HIP(hipMemsetAsync(f.adam_v, 0, (size_t)f.nparams * 4, st));
+  f.adam_t = 0;
+  double best_obj = -std::numeric_limits<double>::infinity();
+  double h[4];
+  for (int it = 0; it < num_iterations; ++it) {
+    UT_HIP(flow_normal_fill(b.z, B, P, (uint32_t)seed, (uint32_t)it, st));
+    UT_HIP(elbo_eval(ws, f, b, gradient_clip));
+    UT_HIP(hipMemcpyAsync(h, b.stats, sizeof(h), hipMemcpyDeviceToHost, st));
+    UT_HIP(hipStreamSynchronize(st));
+    if (!(h[0] > 0))
+      return fail(UT_E_ARG, "ut_flow_fit_elbo: only non-finite logjoint values or gradients at iteration " +
+                                std::to_string(it));
+    if (elbo_history) elbo_history[it] = h[1];
+    if (grad_norm_history) grad_norm_history[it] = std::isfinite(gradient_clip) ? std::min(h[2], gradient_clip) : h[2];
+    if (h[1] > best_obj) {  // the weights this objective was evaluated at
+      best_obj = h[1];
+      UT_HIP(hipMemcpyAsync(best, f.params, (size_t)f.nparams * 4, hipMemcpyDeviceToDevice, st));
+    }
+    UT_HIP(flow_adam_ascent(f, b.stats, learning_rate, 0.9, 0.999, 1e-8, st));
+    UT_HIP(flow_sync_weights(f, st));
+  }
+  if (best_weights) UT_HIP(hipMemcpyAsync(best_weights, best, (size_t)f.nparams * 4, hipMemcpyDeviceToHost, st));
+  UT_HIP(hipStreamSynchronize(st));
+  if (best_elbo) *best_elbo = best_obj;
   return UT_OK;
 }
 

@@ -96,4 +96,26 @@ hipError_t flow_gather_rows(const float* src, long long n, int P, float* dst, in
 // per-row loss 0.5 |z|^2 - logdet_inv + P/2 log(2 pi), summed into *out (device double)
 hipError_t flow_loss_value(const float* z, const float* logdet_inv, int B, int P, double* out, hipStream_t st);
 
+// ---- reparameterised ELBO training (the reference's flow ADVI, src/inference/vi.jl:750-840)
+// z ~ N(0, I): B x P Philox normals, key = (seed, step)
+hipError_t flow_normal_fill(float* z, int B, int P, uint32_t seed, uint32_t step, hipStream_t st);
+// forward pass that keeps every activation the weight gradient needs (never the
+// LDS-staged sampling kernels, which keep only what the input gradient needs)
+hipError_t flow_forward_train(Flow& f, const float* z, int B, hipStream_t st);
+// Per row of the last flow_forward_train with the model's logjoint lj and gradient
+// in sc.g: valid = lj, logdet and the gradient finite; an invalid row's gradient is
+// zeroed and rw[b] = 0 (else 1); logw[b] = lj + 0.5|z|^2 + P/2 log 2 pi + logdet.
+hipError_t flow_elbo_rows(Flow& f, const float* z, const float* lj, int B, float* rw, double* logw,
+                          hipStream_t st);
+// Sum over rows of the weight gradient of rw[b] (g_q . f(z) + logdet) with g_q = sc.g
+// held fixed (invalid rows contribute nothing), into f.grads
+hipError_t flow_elbo_backward(Flow& f, const float* z, const float* rw, int B, hipStream_t st);
+// stats (device, 4 doubles) = {valid rows, mean logw over them, gradient norm of the
+// mean, update scale = min(1, clip / norm) / valid rows}; part: >= 256 doubles
+hipError_t flow_elbo_stats(Flow& f, const float* rw, const double* logw, int B, double clip, double* part,
+                           double* stats, hipStream_t st);
+// Adam ascent on f.grads scaled by stats[3]; no update when stats[0] == 0
+hipError_t flow_adam_ascent(Flow& f, const double* stats, double lr, double b1, double b2, double eps,
+                            hipStream_t st);
+
 }  // namespace ut

@@ -80,6 +80,23 @@ __global__ void k_base_param_terms(const float* __restrict__ z, const float* __r
   tls[i] = 1.0f - v * v;
 }
 
+// forward-direction log-scale gradient terms per element: g_x0 e^ls z + rw (the
+// base's log-determinant sum ls per valid row)
+__global__ void k_base_fwd_param_terms(const float* __restrict__ z, const float* __restrict__ g,
+                                       const float* __restrict__ prm, int P, const float* __restrict__ rw,
+                                       float* __restrict__ tls, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int c = (int)(i % P);
+  tls[i] = g[i] * expf(prm[P + c]) * z[i] + rw[i / P];
+}
+
+__global__ void k_normal_fill(float* __restrict__ z, long long n, uint32_t seed, uint32_t step) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  z[i] = rand_normal_f(seed, step, 11u, (uint32_t)i);
+}
+
 __global__ void k_gather(const float* __restrict__ x, int P, const int* __restrict__ pidx, int np, int kp,
                          bf16_t* __restrict__ xp, long long n) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -150,15 +167,17 @@ __global__ __launch_bounds__(256) void k_coupling_fwd_bwd(float* __restrict__ g,
                                                           const float* __restrict__ s_in,
                                                           const float* __restrict__ xs_in,
                                                           const int* __restrict__ aidx, int na, int op, float c,
-                                                          bf16_t* __restrict__ gO, float* __restrict__ gOf, int B) {
+                                                          bf16_t* __restrict__ gO, float* __restrict__ gOf, int B,
+                                                          const float* __restrict__ rw) {
   const long long b = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (b >= B) return;
+  const float wl = rw ? rw[b] : 1.0f;  // weight of the row's log-determinant term
   for (int j = lane; j < na; j += 64) {
     const long long ia = b * P + aidx[j];
     const float ga = g[ia];
     const float s = s_in[b * na + j];
-    const float gs = ga * xs_in[b * na + j] + 1.0f;
+    const float gs = ga * xs_in[b * na + j] + wl;
     const float r = s / c;
     const float gos = gs * (1.0f - r * r);
     gO[b * op + j] = f2bf(gos);
@@ -504,6 +523,97 @@ __global__ __launch_bounds__(256) void k_row_loss(const float* __restrict__ z, c
   if (lane == 0) out[b] = 0.5 * s - (double)logdet_inv[b] + 0.5 * P * 1.8378770664093453;
 }
 
+// one wavefront per row: validity, gradient masking and the row's log-weight
+__global__ __launch_bounds__(256) void k_elbo_rows(const float* __restrict__ z, const float* __restrict__ lj,
+                                                   const float* __restrict__ logdet, float* __restrict__ g, int B,
+                                                   int P, float* __restrict__ rw, double* __restrict__ logw) {
+  const long long b = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (b >= B) return;
+  bool ok = true;
+  double zz = 0.0;
+  for (int c = lane; c < P; c += 64) {
+    ok = ok && isfinite(g[b * P + c]);
+    const double v = z[b * P + c];
+    zz += v * v;
+  }
+  for (int m = 32; m >= 1; m >>= 1) zz += __shfl_xor(zz, m);
+  const bool valid = __all(ok) && isfinite(lj[b]) && isfinite(logdet[b]);
+  if (!valid)
+    for (int c = lane; c < P; c += 64) g[b * P + c] = 0.0f;
+  if (lane == 0) {
+    rw[b] = valid ? 1.0f : 0.0f;
+    logw[b] = valid ? (double)lj[b] + 0.5 * zz + 0.9189385332046727 * (double)P + (double)logdet[b] : 0.0;
+  }
+}
+
+// block partials: part[blk] = sum of squares of g over a grid-stride slice
+__global__ __launch_bounds__(256) void k_sumsq_part(const float* __restrict__ g, long long n,
+                                                    double* __restrict__ part) {
+  __shared__ double sh[256];
+  double a = 0.0;
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+    const double v = g[i];
+    a += v * v;
+  }
+  sh[threadIdx.x] = a;
+  __syncthreads();
+  for (int w = 128; w >= 1; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+// one block: valid rows, mean log-weight, clipped update scale (fixed summation order)
+__global__ __launch_bounds__(256) void k_elbo_stats(const float* __restrict__ rw, const double* __restrict__ logw,
+                                                    int B, const double* __restrict__ part, int nparts, double clip,
+                                                    double* __restrict__ stats) {
+  __shared__ double sv[256], sl[256], sq[256];
+  double v = 0.0, l = 0.0, q = 0.0;
+  for (int b = threadIdx.x; b < B; b += 256) {
+    v += (double)rw[b];
+    l += logw[b];
+  }
+  for (int i = threadIdx.x; i < nparts; i += 256) q += part[i];
+  sv[threadIdx.x] = v;
+  sl[threadIdx.x] = l;
+  sq[threadIdx.x] = q;
+  __syncthreads();
+  for (int w = 128; w >= 1; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      sv[threadIdx.x] += sv[threadIdx.x + w];
+      sl[threadIdx.x] += sl[threadIdx.x + w];
+      sq[threadIdx.x] += sq[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double nv = sv[0];
+    const double norm = nv > 0.0 ? sqrt(sq[0]) / nv : 0.0;
+    const double cf = (isfinite(clip) && norm > clip) ? clip / norm : 1.0;
+    stats[0] = nv;
+    stats[1] = nv > 0.0 ? sl[0] / nv : 0.0;
+    stats[2] = norm;
+    stats[3] = nv > 0.0 ? cf / nv : 0.0;
+  }
+}
+
+// Adam ascent (the reference's _adam_ascent_step!, src/inference/vi.jl:205-227) on
+// the gradient scaled by stats[3]
+__global__ void k_adam_ascent(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
+                              float* __restrict__ v, long long n, float lr, float b1, float b2, float eps, float bc1,
+                              float bc2, const double* __restrict__ stats) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || !(stats[0] > 0.0)) return;
+  const float gi = g[i] * (float)stats[3];
+  const float mi = b1 * m[i] + (1.0f - b1) * gi;
+  const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  p[i] += lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
+}
+
 }  // namespace
 
 // ------------------------------------------------------------------ host side
@@ -685,7 +795,8 @@ static hipError_t conditioner_bwd(Flow& f, int l, int B, bool weights, hipStream
   return hipGetLastError();
 }
 
-hipError_t flow_forward(Flow& f, const float* z, float* q, float* logdet, int B, hipStream_t st) {
+static hipError_t forward_impl(Flow& f, const float* z, float* q, float* logdet, int B, bool allow_fused,
+                               hipStream_t st) {
   if (B % FLOW_PAD || B > f.sc.B) return hipErrorInvalidValue;
   FlowScratch& s = f.sc;
   const long long n = (long long)B * f.P;
@@ -693,7 +804,7 @@ hipError_t flow_forward(Flow& f, const float* z, float* q, float* logdet, int B,
   hipLaunchKernelGGL(k_base_fwd, dim3(nblk(n)), dim3(T256), 0, st, z, f.params, f.P, s.x, n);
   for (int l = 0; l < f.L; ++l) {
     FlowLayer& ly = f.layers[l];
-    if (fused_layer(f, l)) {
+    if (allow_fused && fused_layer(f, l)) {
       hipLaunchKernelGGL(k_layer_fwd, dim3(B / LR), dim3(T256), LAYER_LDS, st, s.x, f.P, ly.pidx, ly.np, ly.W1b,
                          f.params + ly.ob1, ly.W2b, s.O, f.params + ly.ob2, ly.aidx, ly.na, f.clamp, s.h[l], s.s[l],
                          s.xa[l], s.logdet);
@@ -716,6 +827,14 @@ hipError_t flow_forward(Flow& f, const float* z, float* q, float* logdet, int B,
   return hipGetLastError();
 }
 
+hipError_t flow_forward(Flow& f, const float* z, float* q, float* logdet, int B, hipStream_t st) {
+  return forward_impl(f, z, q, logdet, B, true, st);
+}
+
+hipError_t flow_forward_train(Flow& f, const float* z, int B, hipStream_t st) {
+  return forward_impl(f, z, nullptr, nullptr, B, false, st);
+}
+
 hipError_t flow_backward(Flow& f, const float* g_q, float* g_z, int B, hipStream_t st) {
   if (B % FLOW_PAD || B > f.sc.B) return hipErrorInvalidValue;
   FlowScratch& s = f.sc;
@@ -733,7 +852,7 @@ hipError_t flow_backward(Flow& f, const float* g_q, float* g_z, int B, hipStream
       continue;
     }
     hipLaunchKernelGGL(k_coupling_fwd_bwd, dim3((B + 3) / 4), dim3(T256), 0, st, s.g, f.P, s.s[l], s.xa[l], ly.aidx,
-                       ly.na, op, f.clamp, s.gO, (float*)nullptr, B);
+                       ly.na, op, f.clamp, s.gO, (float*)nullptr, B, (const float*)nullptr);
     hipError_t e = conditioner_bwd(f, l, B, false, st);
     if (e != hipSuccess) return e;
   }
@@ -798,7 +917,58 @@ hipError_t flow_adam(Flow& f, int B, double lr, double b1, double b2, double eps
   const double bc1 = 1.0 - pow(b1, (double)f.adam_t), bc2 = 1.0 - pow(b2, (double)f.adam_t);
   hipLaunchKernelGGL(k_adam, dim3(nblk(f.nparams)), dim3(T256), 0, st, f.params, f.grads, f.adam_m, f.adam_v,
                      f.nparams, (float)lr, (float)b1, (float)b2, (float)eps, (float)bc1, (float)bc2,
-                     1.0f / (float)f.sc.B);
+                     1.0f / (float)B);
+  return hipGetLastError();
+}
+
+hipError_t flow_normal_fill(float* z, int B, int P, uint32_t seed, uint32_t step, hipStream_t st) {
+  const long long n = (long long)B * P;
+  hipLaunchKernelGGL(k_normal_fill, dim3(nblk(n)), dim3(T256), 0, st, z, n, seed, step);
+  return hipGetLastError();
+}
+
+hipError_t flow_elbo_rows(Flow& f, const float* z, const float* lj, int B, float* rw, double* logw,
+                          hipStream_t st) {
+  if (B % FLOW_PAD || B > f.sc.B) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_elbo_rows, dim3((B + 3) / 4), dim3(T256), 0, st, z, lj, f.sc.logdet, f.sc.g, B, f.P, rw,
+                     logw);
+  return hipGetLastError();
+}
+
+hipError_t flow_elbo_backward(Flow& f, const float* z, const float* rw, int B, hipStream_t st) {
+  if (B % FLOW_PAD || B > f.sc.B) return hipErrorInvalidValue;
+  FlowScratch& s = f.sc;
+  const long long n = (long long)B * f.P;
+  for (int l = f.L - 1; l >= 0; --l) {
+    FlowLayer& ly = f.layers[l];
+    const int op = round_up(2 * ly.na, FLOW_PAD);
+    hipLaunchKernelGGL(k_coupling_fwd_bwd, dim3((B + 3) / 4), dim3(T256), 0, st, s.g, f.P, s.s[l], s.xa[l], ly.aidx,
+                       ly.na, op, f.clamp, s.gO, s.gOf, B, rw);
+    hipError_t e = conditioner_bwd(f, l, B, true, st);
+    if (e != hipSuccess) return e;
+  }
+  // s.g is now d/dx0: mu's gradient is its column sum; log-scale terms g e^ls z + rw
+  colsum(s.g, B, f.P, f.grads, s.colsum, st);
+  hipLaunchKernelGGL(k_base_fwd_param_terms, dim3(nblk(n)), dim3(T256), 0, st, z, s.g, f.params, f.P, rw, s.aux, n);
+  colsum(s.aux, B, f.P, f.grads + f.P, s.colsum, st);
+  return hipGetLastError();
+}
+
+hipError_t flow_elbo_stats(Flow& f, const float* rw, const double* logw, int B, double clip, double* part,
+                           double* stats, hipStream_t st) {
+  constexpr int NPART = 256;
+  hipLaunchKernelGGL(k_sumsq_part, dim3(NPART), dim3(T256), 0, st, f.grads, f.nparams, part);
+  hipLaunchKernelGGL(k_elbo_stats, dim3(1), dim3(T256), 0, st, rw, logw, B, part, NPART, clip, stats);
+  return hipGetLastError();
+}
+
+hipError_t flow_adam_ascent(Flow& f, const double* stats, double lr, double b1, double b2, double eps,
+                            hipStream_t st) {
+  f.adam_t += 1;
+  const double bc1 = 1.0 - pow(b1, (double)f.adam_t), bc2 = 1.0 - pow(b2, (double)f.adam_t);
+  hipLaunchKernelGGL(k_adam_ascent, dim3(nblk(f.nparams)), dim3(T256), 0, st, f.params, f.grads, f.adam_m,
+                     f.adam_v, f.nparams, (float)lr, (float)b1, (float)b2, (float)eps, (float)bc1, (float)bc2,
+                     stats);
   return hipGetLastError();
 }
 

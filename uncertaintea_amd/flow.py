@@ -4,8 +4,10 @@ Structure follows the reference's FlowGuide (src/inference/vi/guides.jl:1-21,
 282-369): an elementwise affine base x0 = mu + exp(log_scale) z and coupling
 layers with alternating parity masks, active x <- x exp(s) + t.  The reference's
 conditioners are linear and trained host-side for ADVI; here they are MLPs of width
-`hidden` trained on the GPU by maximum likelihood on posterior draws, and the
-trained flow preconditions the lock-step HMC / ChEES samplers (NeuTra):
+`hidden` trained on the GPU, either by the reference's objective (`fit_elbo`: the
+reparameterised ELBO of batched_advi(...; guide=:flow), src/inference/vi.jl:700-870)
+or by maximum likelihood on posterior draws (`fit`), and the trained flow
+preconditions the lock-step HMC / ChEES / MEADS samplers (NeuTra):
 `batched_hmc(model, ..., flow=flow)`.
 """
 from __future__ import annotations
@@ -45,6 +47,7 @@ class AffineCouplingFlow:
         L.check(L.lib().ut_flow_num_weights(h, ct.byref(n)))
         self.num_weights = int(n.value)
         self.loss_trace: np.ndarray | None = None
+        self.elbo_trace: np.ndarray | None = None
 
     @property
     def handle(self):
@@ -101,6 +104,46 @@ class AffineCouplingFlow:
                                       int(seed) & 0xFFFFFFFFFFFFFFFF, L.ptr(trace)))
         self.loss_trace = trace
         return trace
+
+    def fit_elbo(self, model, num_particles: int = 4096, num_iterations: int = 1000, learning_rate: float = 1e-3,
+                 gradient_clip: float = float("inf"), seed: int = 0, keep_best: bool = False) -> dict:
+        """Reparameterised-ELBO fit against `model` on the GPU, the reference's flow
+        ADVI (src/inference/vi.jl:700-870): `num_iterations` Adam ascent steps, each
+        on `num_particles` fresh particles (a multiple of 128); invalid particles
+        (non-finite logjoint or gradient) carry no weight.  Returns elbo_history,
+        grad_norm_history, best_elbo and best_weights; the flow keeps its last
+        weights unless keep_best."""
+        from .device import DeviceBatchedWorkspace
+        if model.num_params != self.num_params:
+            raise ValueError("DimensionMismatch: the flow and the model have different parameter counts")
+        ws = DeviceBatchedWorkspace(model, int(num_particles), np.float32)
+        n = int(num_iterations)
+        elbo = np.zeros(n)
+        gnorm = np.zeros(n)
+        best = np.empty(self.num_weights, dtype=np.float32)
+        best_elbo = ct.c_double()
+        L.check(L.lib().ut_flow_fit_elbo(ws.handle, self._h, n, float(learning_rate), float(gradient_clip),
+                                         int(seed) & 0xFFFFFFFFFFFFFFFF, L.ptr(elbo), L.ptr(gnorm), L.ptr(best),
+                                         ct.byref(best_elbo)))
+        if keep_best and n > 0:
+            self.weights = best
+        self.elbo_trace = elbo
+        return {"elbo_history": elbo, "grad_norm_history": gnorm, "best_elbo": float(best_elbo.value),
+                "best_weights": best}
+
+    def elbo_grad(self, model, z):
+        """One ELBO evaluation at particles z (rows, a multiple of 128): (elbo, mean
+        weight gradient, model gradient at f(z), validity flags)."""
+        from .device import DeviceBatchedWorkspace
+        zz = np.ascontiguousarray(np.asarray(z, dtype=np.float32).reshape(-1, self.num_params))
+        ws = DeviceBatchedWorkspace(model, zz.shape[0], np.float32)
+        elbo = ct.c_double()
+        g = np.empty(self.num_weights, dtype=np.float32)
+        gq = np.empty_like(zz)
+        valid = np.empty(zz.shape[0], dtype=np.int32)
+        L.check(L.lib().ut_flow_elbo_grad(ws.handle, self._h, L.ptr(zz), ct.byref(elbo), L.ptr(g), L.ptr(gq),
+                                          L.ptr(valid)))
+        return float(elbo.value), g, gq, valid.astype(bool)
 
     def forward(self, z):
         """(q, log|det dq/dz|) for rows of z."""
