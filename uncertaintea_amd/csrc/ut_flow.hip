@@ -438,13 +438,14 @@ __global__ __launch_bounds__(256) void k_layer_bwd(float* __restrict__ g, int P,
 }
 
 // column sums in two passes (fixed order): COLSUM_CHUNKS row chunks, then chunks
-constexpr int COLSUM_CHUNKS = 64;
+constexpr int COLSUM_CHUNKS = 256;  // 256 workgroups in the first pass (16 rows each at 4096 rows)
 __global__ void k_colsum_part(const float* __restrict__ M, int B, int n, float* __restrict__ part) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n) return;
   const int per = (B + COLSUM_CHUNKS - 1) / COLSUM_CHUNKS;
   const int r0 = blockIdx.y * per, r1 = min(B, r0 + per);
   float s = 0.0f;
+#pragma unroll 16
   for (int b = r0; b < r1; ++b) s += M[(long long)b * n + c];
   part[(long long)blockIdx.y * n + c] = s;
 }
@@ -452,6 +453,7 @@ __global__ void k_colsum_final(const float* __restrict__ part, int n, float* __r
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n) return;
   float s = 0.0f;
+#pragma unroll 16
   for (int k = 0; k < COLSUM_CHUNKS; ++k) s += part[(long long)k * n + c];
   out[c] = s;
 }
