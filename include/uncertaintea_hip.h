@@ -282,10 +282,11 @@ int ut_meads_run(ut_workspace* ws, const ut_meads_opts* opts, ut_chain_out* out,
  * x0 = mu + exp(log_scale) z, then num_layers coupling layers with the alternating
  * parity masks, active x <- x exp(s) + t), with MLP conditioners
  * [s_raw, t] = W2 tanh(W1 x_passive + b1) + b2 of width `hidden` and
- * s = c tanh(s_raw / c) (c = scale_clamp).  Not a reference sampler feature: the
- * reference trains linear conditioners host-side for ADVI; here the network is
- * trained on the GPU by maximum likelihood on draws (Adam) and then preconditions
- * HMC (NeuTra): attached to a workspace, the lock-step samplers run in z space on
+ * s = c tanh(s_raw / c) (c = scale_clamp).  num_layers = 0 leaves the base alone:
+ * the reference's mean-field Gaussian guide.  The reference trains linear
+ * conditioners host-side for ADVI; here the network is trained on the GPU, by the
+ * ELBO (ut_flow_fit_elbo) or by maximum likelihood on draws (ut_flow_train), and
+ * can then precondition HMC (NeuTra): attached to a workspace, the lock-step samplers run in z space on
  * log p(f(z)) + log|det df/dz| and return draws mapped back through f.
  * Conditioner GEMMs run on the matrix cores with bf16 operands and fp32
  * accumulation; weights are fp32.  Row counts of the host-array calls below must
@@ -326,7 +327,8 @@ int ut_workspace_set_flow(ut_workspace* ws, ut_flow* flow);
  * num_chains particles z ~ N(0, I) (Philox, key = (seed, iteration)), maps them
  * through the flow, scores them with the workspace's gradient engine (the plain
  * model, whether or not a flow is attached) and takes one Adam ascent step
- * (beta 0.9 / 0.999, eps 1e-8, fresh moments per call) on the mean over the valid
+ * (beta1, beta2, adam_epsilon; the reference's defaults 0.9, 0.999, 1e-8; fresh
+ * moments per call) on the mean over the valid
  * particles (finite logjoint, log-determinant and gradient) of the gradient of
  * g . f(z) + log|det df/dz| with g = grad log p(f(z)) held fixed.
  * elbo_history[i] = the sampled standard ELBO (mean log p - log q over the valid
@@ -336,9 +338,16 @@ int ut_workspace_set_flow(ut_workspace* ws, ut_flow* flow);
  * best ELBO seen and best_elbo that value.  UT_E_ARG when an iteration has no
  * valid particle (reference: ArgumentError "encountered only non-finite ...").
  * The workspace must be UT_FLOAT32 with num_chains a multiple of 128.            */
-int ut_flow_fit_elbo(ut_workspace* ws, ut_flow* flow, int32_t num_iterations, double learning_rate,
-                     double gradient_clip, uint64_t seed, double* elbo_history, double* grad_norm_history,
-                     float* best_weights, double* best_elbo);
+int ut_flow_fit_elbo(ut_workspace* ws, ut_flow* flow, int32_t num_iterations, double learning_rate, double beta1,
+                     double beta2, double adam_epsilon, double gradient_clip, uint64_t seed, double* elbo_history,
+                     double* grad_norm_history, float* best_weights, double* best_elbo);
+/* num_chains initial positions (num_chains x P, chain-major, unconstrained) as the
+ * samplers draw them for `seed` before their non-finite re-draw loop: prior draws
+ * (UT_INIT_PRIOR) or Uniform(-2, 2) (UT_INIT_UNIFORM) from Philox; host-only.
+ * Reference: _initial_hmc_position / _initial_batched_hmc_positions
+ * (src/inference/hmc_support.jl:114-200, 266-290), used by batched_advi for its starting
+ * location (src/inference/vi.jl:83-93).                                          */
+int ut_initial_params(const ut_plan* plan, int32_t strategy, uint64_t seed, int32_t num_chains, double* out);
 /* One evaluation of that objective at given particles z (num_chains x P host
  * rows): the sampled ELBO, the unclipped mean weight gradient over the valid
  * particles (ut_flow_num_weights floats), and optionally the model gradient at

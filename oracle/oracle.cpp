@@ -1944,7 +1944,7 @@ struct FlowSpec {
     }
     return ld;
   }
-  long long nparams() const { return layers.back().ob2 + layers.back().op; }
+  long long nparams() const { return layers.empty() ? 2LL * P : layers.back().ob2 + layers.back().op; }
 };
 
 void finish_twin_layout(Model& m) {

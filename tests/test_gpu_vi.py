@@ -1,0 +1,97 @@
+"""GPU: batched_advi on the device (mean-field and flow guides).
+
+Tolerance: the mean-field guide's ELBO weight gradient (the flow without coupling
+layers) against the oracle's binary64 flow fed the device's model gradient, within
+1e-5 normwise (no bf16 operands on this path: fp32 arithmetic only).  Statistical:
+on the 25-feature logistic regression (N = 1000) the mean-field fit's location
+sits within half a posterior sd of NUTS's posterior means and its scales within
+[0.6, 1.1] of the posterior sds (the mean-field optimum under-states the sd by the
+posterior's correlations, which are weak here); the flow guide's draws match the
+funnel better than the mean-field guide (a higher ELBO; v's spread is printed)."""
+import sys
+
+import numpy as np
+import pytest
+
+import uncertaintea_amd as ut
+from oracle import binding as B
+from uncertaintea_amd import _lib as L
+from uncertaintea_amd import models as M
+from uncertaintea_amd import synthetic as S
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def test_meanfield_elbo_gradient_against_oracle():
+    say("mean-field ELBO gradient vs oracle")
+    x, y, _ = S.logistic_data(1000, 25)
+    model = M.LogisticRegression(x, y)
+    P = model.num_params
+    flow = ut.AffineCouplingFlow(P, 0, 256, seed=1)
+    rng = np.random.default_rng(2)
+    flow.set_base(rng.normal(size=P) * 0.3, rng.normal(size=P) * 0.2 - 1.0)
+    z = rng.normal(size=(256, P)).astype(np.float32)
+    elbo, g, gq, valid = flow.elbo_grad(model, z)
+    assert valid.all() and g.shape == (2 * P,)
+    rg = B.OracleFlow.of(flow).elbo_grad(z, gq)
+    e = np.linalg.norm(g - rg) / np.linalg.norm(rg)
+    q, ld = B.OracleFlow.of(flow).forward(z)
+    lj, _ = B.OracleModel(model).logjoint_grad_f64(np.ascontiguousarray(q.T))
+    ref = np.mean(lj + 0.5 * np.sum(z.astype(np.float64) ** 2, axis=1) + 0.5 * P * np.log(2 * np.pi) + ld)
+    say(f"  weight gradient vs oracle {e:.2e}; elbo {elbo:.6f} vs {ref:.6f}")
+    assert e <= 1e-5
+    assert abs(elbo - ref) <= 1e-5 * (1 + abs(ref))
+
+
+def test_initial_params_hierarchical_prior():
+    """The hierarchical plan (staged on the device): mu ~ N(0, 5), log tau with
+    tau ~ halfcauchy(5) (median 5), theta_raw ~ N(0, 1)."""
+    y = np.zeros(8, dtype=np.float32)
+    h = ut.HierarchicalNormalNC(y, np.array([0, 4, 8], dtype=np.int32), np.ones(2))
+    b = np.empty((40000, h.num_params))
+    L.check(L.lib().ut_initial_params(h.plan, L.UT_INIT_PRIOR, 3, 40000, L.ptr(b)))
+    assert abs(np.median(np.exp(b[:, 1])) / 5.0 - 1.0) < 0.05
+    assert abs(b[:, 0].std() / 5.0 - 1.0) < 0.03 and abs(b[:, 2:].std() - 1.0) < 0.02
+
+
+def test_meanfield_advi_logistic_against_nuts():
+    say("mean-field ADVI on the logistic regression")
+    x, y, _ = S.logistic_data(1000, 25)
+    model = M.LogisticRegression(x, y)
+    res = ut.batched_advi(model, num_iterations=1500, num_particles=256, learning_rate=0.02, seed=3)
+    h = res.elbo_history
+    nuts = ut.batched_nuts(model, num_chains=128, num_samples=500, num_warmup=500, seed=4)
+    d = nuts.unconstrained_samples.reshape(-1, model.num_params)
+    m, sd = d.mean(axis=0), d.std(axis=0)
+    loc, sc = res.best_location, np.exp(res.best_log_scale)
+    say(f"  elbo {h[:20].mean():.2f} -> {h[-100:].mean():.2f} (best {res.best_elbo:.2f}); max |loc - mean| / sd "
+        f"{np.max(np.abs(loc - m) / sd):.3f}; scale / sd in [{np.min(sc / sd):.3f}, {np.max(sc / sd):.3f}]")
+    assert h[-100:].mean() > h[:20].mean()
+    assert np.max(np.abs(loc - m) / sd) < 0.5
+    assert np.min(sc / sd) > 0.6 and np.max(sc / sd) < 1.1
+    mean = ut.variational_mean(res, space="unconstrained")
+    assert np.array_equal(mean, res.best_location)
+    cov = ut.variational_covariance(res)
+    assert np.allclose(np.diag(cov), sc ** 2)
+    s = ut.variational_samples(res, 4096, space="unconstrained")
+    assert s.shape == (model.num_params, 4096)
+    assert np.max(np.abs(s.mean(axis=1) - loc) / sc) < 5 / np.sqrt(4096) * 1.5
+
+
+def test_flow_advi_funnel_spread():
+    say("flow vs mean-field ADVI on the funnel")
+    model = M.NealsFunnel(10, 3.0)
+    out = {}
+    for guide in ("meanfield", "flow"):
+        r = ut.batched_advi(model, num_iterations=1500, num_particles=512, learning_rate=2e-3, guide=guide,
+                            num_flow_layers=6, initial_params=np.zeros(11), initial_log_scale=0.0, seed=5)
+        s = ut.variational_samples(r, 16384, space="unconstrained")
+        out[guide] = (r.best_elbo, s[0].std())
+        say(f"  {guide}: best elbo {r.best_elbo:.3f}, v sd {s[0].std():.3f} (exact 3)")
+    # the flow family contains the mean-field one (zero conditioners): a better bound
+    assert out["flow"][0] > out["meanfield"][0] + 0.5

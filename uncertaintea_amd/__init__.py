@@ -8,9 +8,11 @@ from .flow import AffineCouplingFlow
 from .inference import HMCChains, batched_chees, batched_hmc, batched_meads, batched_nuts
 from .models import (GaussianProcessPoisson, HierarchicalNormalNC, LogisticRegression, LogisticRegressionBF16,
                      NealsFunnel, TeaModel)
+from .vi import ADVIResult, batched_advi, variational_covariance, variational_mean, variational_samples
 
 __all__ = [
-    "AffineCouplingFlow", "DeviceBatchedWorkspace", "GaussianProcessPoisson", "DeviceError", "HMCChains", "HierarchicalNormalNC", "LogisticRegression",
+    "ADVIResult", "AffineCouplingFlow", "batched_advi", "variational_covariance", "variational_mean",
+    "variational_samples", "DeviceBatchedWorkspace", "GaussianProcessPoisson", "DeviceError", "HMCChains", "HierarchicalNormalNC", "LogisticRegression",
     "LogisticRegressionBF16", "NealsFunnel", "NonFiniteInitError", "TeaModel", "batched_chees", "batched_hmc", "batched_meads", "batched_nuts", "device_available",
     "device_batched_logjoint", "device_batched_logjoint_gradient", "device_batched_logjoint_gradient_inplace",
     "device_batched_logjoint_inplace", "diagnostics", "philox4x32", "philox_fill", "synthetic",

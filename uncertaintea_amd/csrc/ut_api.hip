@@ -1467,7 +1467,7 @@ int ut_flow_create(int32_t num_params, int32_t num_layers, int32_t hidden, doubl
   if (!out) return fail(UT_E_ARG, "ut_flow_create: null pointer");
   *out = nullptr;
   if (num_params < 2) return fail(UT_E_DIM, "ut_flow_create: at least 2 parameters");
-  if (num_layers < 1) return fail(UT_E_ARG, "ut_flow_create: num_layers must be >= 1");
+  if (num_layers < 0) return fail(UT_E_ARG, "ut_flow_create: num_layers must be >= 0");
   if (hidden < 128 || hidden % 128 != 0) return fail(UT_E_ARG, "ut_flow_create: hidden must be a positive multiple of 128");
   if (!(scale_clamp > 0)) return fail(UT_E_ARG, "ut_flow_create: scale_clamp must be > 0");
   if (!ut_device_available()) return fail(UT_E_HIP, "ut_flow_create: no HIP device visible");
@@ -1768,13 +1768,27 @@ int ut_flow_elbo_grad(ut_workspace* ws, ut_flow* flow, const float* z, double* e
   return UT_OK;
 }
 
-int ut_flow_fit_elbo(ut_workspace* ws, ut_flow* flow, int32_t num_iterations, double learning_rate,
-                     double gradient_clip, uint64_t seed, double* elbo_history, double* grad_norm_history,
-                     float* best_weights, double* best_elbo) {
+int ut_initial_params(const ut_plan* plan, int32_t strategy, uint64_t seed, int32_t num_chains, double* out) {
+  if (!plan || !out) return fail(UT_E_ARG, "ut_initial_params: null pointer");
+  if (num_chains <= 0) return fail(UT_E_DIM, "ut_initial_params: num_chains must be positive");
+  if (strategy != UT_INIT_PRIOR && strategy != UT_INIT_UNIFORM)
+    return fail(UT_E_ARG, "ut_initial_params: strategy must be UT_INIT_PRIOR or UT_INIT_UNIFORM");
+  const int P = plan->dev.P;
+  for (int c = 0; c < num_chains; ++c)
+    draw_initial_column(plan->dev, strategy, (uint32_t)c, run_seed_of(seed), out + (size_t)c * P);
+  return UT_OK;
+}
+
+int ut_flow_fit_elbo(ut_workspace* ws, ut_flow* flow, int32_t num_iterations, double learning_rate, double beta1,
+                     double beta2, double adam_epsilon, double gradient_clip, uint64_t seed, double* elbo_history,
+                     double* grad_norm_history, float* best_weights, double* best_elbo) {
   int rc = elbo_check(ws, flow, "ut_flow_fit_elbo");
   if (rc) return rc;
   if (num_iterations < 0) return fail(UT_E_ARG, "ut_flow_fit_elbo: num_iterations must be >= 0");
   if (!(learning_rate > 0)) return fail(UT_E_ARG, "ut_flow_fit_elbo: learning_rate must be > 0");
+  if (!(beta1 >= 0 && beta1 < 1) || !(beta2 >= 0 && beta2 < 1))
+    return fail(UT_E_ARG, "ut_flow_fit_elbo: requires 0 <= beta1 < 1 and 0 <= beta2 < 1");
+  if (!(adam_epsilon > 0)) return fail(UT_E_ARG, "ut_flow_fit_elbo: adam_epsilon must be > 0");
   if (!(gradient_clip > 0)) return fail(UT_E_ARG, "ut_flow_fit_elbo: gradient_clip must be > 0 (inf: no clipping)");
   Flow& f = flow->f;
   const int B = ws->C, P = ws->P;
@@ -1807,7 +1821,7 @@ int ut_flow_fit_elbo(ut_workspace* ws, ut_flow* flow, int32_t num_iterations, do
       best_obj = h[1];
       UT_HIP(hipMemcpyAsync(best, f.params, (size_t)f.nparams * 4, hipMemcpyDeviceToDevice, st));
     }
-    UT_HIP(flow_adam_ascent(f, b.stats, learning_rate, 0.9, 0.999, 1e-8, st));
+    UT_HIP(flow_adam_ascent(f, b.stats, learning_rate, beta1, beta2, adam_epsilon, st));
     UT_HIP(flow_sync_weights(f, st));
   }
   if (best_weights) UT_HIP(hipMemcpyAsync(best_weights, best, (size_t)f.nparams * 4, hipMemcpyDeviceToHost, st));

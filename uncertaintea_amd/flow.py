@@ -106,7 +106,8 @@ class AffineCouplingFlow:
         return trace
 
     def fit_elbo(self, model, num_particles: int = 4096, num_iterations: int = 1000, learning_rate: float = 1e-3,
-                 gradient_clip: float = float("inf"), seed: int = 0, keep_best: bool = False) -> dict:
+                 gradient_clip: float = float("inf"), seed: int = 0, keep_best: bool = False, beta1: float = 0.9,
+                 beta2: float = 0.999, adam_epsilon: float = 1e-8) -> dict:
         """Reparameterised-ELBO fit against `model` on the GPU, the reference's flow
         ADVI (src/inference/vi.jl:700-870): `num_iterations` Adam ascent steps, each
         on `num_particles` fresh particles (a multiple of 128); invalid particles
@@ -122,9 +123,9 @@ class AffineCouplingFlow:
         gnorm = np.zeros(n)
         best = np.empty(self.num_weights, dtype=np.float32)
         best_elbo = ct.c_double()
-        L.check(L.lib().ut_flow_fit_elbo(ws.handle, self._h, n, float(learning_rate), float(gradient_clip),
-                                         int(seed) & 0xFFFFFFFFFFFFFFFF, L.ptr(elbo), L.ptr(gnorm), L.ptr(best),
-                                         ct.byref(best_elbo)))
+        L.check(L.lib().ut_flow_fit_elbo(ws.handle, self._h, n, float(learning_rate), float(beta1), float(beta2),
+                                         float(adam_epsilon), float(gradient_clip), int(seed) & 0xFFFFFFFFFFFFFFFF,
+                                         L.ptr(elbo), L.ptr(gnorm), L.ptr(best), ct.byref(best_elbo)))
         if keep_best and n > 0:
             self.weights = best
         self.elbo_trace = elbo
