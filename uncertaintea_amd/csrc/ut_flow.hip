@@ -264,65 +264,93 @@ __device__ inline void lds_gemm(const bf16_t* As, int lda, const bf16_t* Bs, int
   }
 }
 
-// the layer's two weight matrices (LH x LW and LW x LH bf16, row-major) -> LDS with row
-// strides LWP and LHP: every thread issues all of its 16-byte loads before any store
-__device__ inline void stage_weights(bf16_t* d1, const bf16_t* __restrict__ s1, bf16_t* d2,
-                                     const bf16_t* __restrict__ s2) {
-  constexpr int N = LH * LW / 8 / 256;  // chunks per thread and matrix
-  uint4 v1[N], v2[N];
+// The layer kernels issue every global load they need before their first wait (the
+// weights, the row tile's inputs, biases, the coupling's operands and the values the
+// epilogues combine with), so a workgroup waits on memory once; with one workgroup per
+// CU and no other work in flight, each further dependent round trip would add its full
+// latency to the layer.  The conditioner output stays in LDS (over the first matrix,
+// dead once the first GEMM is done).
+constexpr int NSTAGE = LH * LW / 8 / 256;  // 16-byte weight chunks per thread and matrix
+constexpr int OSP = LW + 4;                // fp32 row stride of the conditioner output in LDS
+
+__device__ inline void load_weights(const bf16_t* __restrict__ s1, const bf16_t* __restrict__ s2, uint4 (&v1)[NSTAGE],
+                                    uint4 (&v2)[NSTAGE]) {
 #pragma unroll
-  for (int i = 0; i < N; ++i) {
+  for (int i = 0; i < NSTAGE; ++i) {
     const int e = threadIdx.x + 256 * i;
     v1[i] = *(const uint4*)(s1 + (long long)(e / (LW / 8)) * LW + (e % (LW / 8)) * 8);
     v2[i] = *(const uint4*)(s2 + (long long)(e / (LH / 8)) * LH + (e % (LH / 8)) * 8);
   }
-  __builtin_amdgcn_sched_barrier(0);  // keep every load ahead of the first store
+}
+
+// first matrix (LH x LW) -> LDS row stride LWP, second (LW x LH) -> LHP
+__device__ inline void store_weights(bf16_t* d1, bf16_t* d2, const uint4 (&v1)[NSTAGE], const uint4 (&v2)[NSTAGE]) {
 #pragma unroll
-  for (int i = 0; i < N; ++i) {
+  for (int i = 0; i < NSTAGE; ++i) {
     const int e = threadIdx.x + 256 * i;
     *(uint4*)(d1 + (e / (LW / 8)) * LWP + (e % (LW / 8)) * 8) = v1[i];
     *(uint4*)(d2 + (e / (LH / 8)) * LHP + (e % (LH / 8)) * 8) = v2[i];
   }
 }
 
-__global__ __launch_bounds__(256) void k_layer_fwd(float* __restrict__ x, int P, const int* __restrict__ pidx, int np,
+// `par` is the layer's parity: its passive coordinates are 2k + par and its active ones
+// 2j + 1 - par (the alternating masks), so no index list is loaded before the gathers
+__global__ __launch_bounds__(256) void k_layer_fwd(float* __restrict__ x, int P, int par, int np,
                                                    const bf16_t* __restrict__ W1b, const float* __restrict__ b1,
-                                                   const bf16_t* __restrict__ W2b, float* __restrict__ O,
-                                                   const float* __restrict__ b2, const int* __restrict__ aidx, int na,
-                                                   float c, float* __restrict__ h_out, float* __restrict__ s_out,
+                                                   const bf16_t* __restrict__ W2b, const float* __restrict__ b2,
+                                                   int na, float c,
+                                                   float* __restrict__ h_out, float* __restrict__ s_out,
                                                    float* __restrict__ xs_out, float* __restrict__ logdet) {
   extern __shared__ __attribute__((aligned(16))) bf16_t lsh[];
   bf16_t* W1s = lsh;                  // LH x LWP: W1 [H x kp]
   bf16_t* W2s = lsh + LH * LWP;       // LW x LHP: W2 [op x H]
   bf16_t* R = lsh + 2 * LH * LWP;     // LR x LHP: passive inputs (stride LWP), then h
+  float* Os = (float*)lsh;            // LR x OSP fp32 conditioner output, over W1 after the first GEMM
   const long long r0 = (long long)blockIdx.x * LR;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, l32 = lane & 31;
-  stage_weights(W1s, W1b, W2s, W2b);
-  {
-    constexpr int G = LR * LW / 256;
-    float xv[G];
+  constexpr int G = LR * LW / 256;    // passive inputs gathered per thread
+  constexpr int RW = LR / 4;          // coupling rows per wavefront
+  const bool act = lane < na;
+  // ---- every global load up front (indices first: the gathers depend on them)
+  const int kq = tid & (LW - 1);      // the passive column this thread gathers (fixed: 256 = 2 LW)
+  const int pk = 2 * kq + par;
+  const int ak = 2 * lane + 1 - par;
+  uint4 v1[NSTAGE], v2[NSTAGE];
+  load_weights(W1b, W2b, v1, v2);
+  float xv[G];
 #pragma unroll
-    for (int i = 0; i < G; ++i) {
-      const int e = tid + 256 * i, r = e / LW, k = e % LW;
-      xv[i] = k < np ? x[(r0 + r) * P + pidx[k]] : 0.0f;
-    }
-    __builtin_amdgcn_sched_barrier(0);
+  for (int i = 0; i < G; ++i) {
+    const int r = (tid + 256 * i) / LW;
+    xv[i] = kq < np ? x[(r0 + r) * P + pk] : 0.0f;
+  }
+  float b1v[2];
 #pragma unroll
-    for (int i = 0; i < G; ++i) {
-      const int e = tid + 256 * i, r = e / LW, k = e % LW;
-      R[r * LWP + k] = k < np ? f2bf(xv[i]) : (bf16_t)0;
-    }
+  for (int t = 0; t < 2; ++t) b1v[t] = b1[32 * wave + 128 * t + l32];
+  const float b2s = act ? b2[lane] : 0.0f, b2t = act ? b2[na + lane] : 0.0f;
+  float xa[RW], ldv[RW];
+#pragma unroll
+  for (int rr = 0; rr < RW; ++rr) {
+    const long long b = r0 + wave * RW + rr;
+    xa[rr] = act ? x[b * P + ak] : 0.0f;
+    ldv[rr] = lane == 0 ? logdet[b] : 0.0f;
+  }
+  __builtin_amdgcn_sched_barrier(0);
+  store_weights(W1s, W2s, v1, v2);
+#pragma unroll
+  for (int i = 0; i < G; ++i) {
+    const int r = (tid + 256 * i) / LW;
+    R[r * LWP + kq] = kq < np ? f2bf(xv[i]) : (bf16_t)0;
   }
   __syncthreads();
   f32x16 acc[2];
   lds_gemm<2>(R, LWP, W1s, LWP, LW, 2, acc);
-  __syncthreads();  // every wavefront is done with the passive inputs; R now takes h
+  __syncthreads();  // every wavefront is done with the passive inputs and W1; R now takes h
 #pragma unroll
   for (int t = 0; t < 2; ++t)
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
       const int row = mfma32_row(lane, r), col = 32 * wave + 128 * t + l32;
-      const float v = tanhf(acc[t][r] + b1[col]);
+      const float v = tanhf(acc[t][r] + b1v[t]);
       h_out[(r0 + row) * LH + col] = v;
       R[row * LHP + col] = f2bf(v);
     }
@@ -330,90 +358,82 @@ __global__ __launch_bounds__(256) void k_layer_fwd(float* __restrict__ x, int P,
   f32x16 acc2[1];
   lds_gemm<1>(R, LHP, W2s, LHP, LH, 1, acc2);
 #pragma unroll
-  for (int r = 0; r < 16; ++r) {
-    const int row = mfma32_row(lane, r), col = 32 * wave + l32;
-    O[(r0 + row) * LW + col] = acc2[0][r];
-  }
+  for (int r = 0; r < 16; ++r) Os[mfma32_row(lane, r) * OSP + 32 * wave + l32] = acc2[0][r];
   __syncthreads();
   // coupling, as k_coupling_fwd (na <= 64: lane j owns active coordinate j), eight rows
-  // per wavefront with all of their loads issued first
-  constexpr int RW = LR / 4;
-  const bool act = lane < na;
-  float os[RW], ot[RW], xv[RW];
-  long long ia[RW];
+  // per wavefront
 #pragma unroll
   for (int rr = 0; rr < RW; ++rr) {
-    const long long b = r0 + wave * RW + rr;
-    if (act) {
-      os[rr] = O[b * LW + lane];
-      ot[rr] = O[b * LW + na + lane];
-      ia[rr] = b * P + aidx[lane];
-      xv[rr] = x[ia[rr]];
-    }
-  }
-  __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-  for (int rr = 0; rr < RW; ++rr) {
-    const long long b = r0 + wave * RW + rr;
+    const int row = wave * RW + rr;
+    const long long b = r0 + row;
     float sum = 0.0f;
     if (act) {
-      const float sv = c * tanhf((os[rr] + b2[lane]) / c);
-      const float tv = ot[rr] + b2[na + lane];
-      const float xs = xv[rr] * expf(sv);
-      x[ia[rr]] = xs + tv;
+      const float sv = c * tanhf((Os[row * OSP + lane] + b2s) / c);
+      const float tv = Os[row * OSP + na + lane] + b2t;
+      const float xs = xa[rr] * expf(sv);
+      x[b * P + ak] = xs + tv;
       s_out[b * na + lane] = sv;
       xs_out[b * na + lane] = xs;
       sum += sv;
     }
     sum = wave_sum_f(sum);
-    if (lane == 0) logdet[b] += sum;
+    if (lane == 0) logdet[b] = ldv[rr] + sum;
   }
 }
 
 __global__ __launch_bounds__(256) void k_layer_bwd(float* __restrict__ g, int P, const float* __restrict__ s_in,
-                                                   const float* __restrict__ xs_in, const int* __restrict__ aidx,
-                                                   int na, float c, const bf16_t* __restrict__ W2Tb,
-                                                   const float* __restrict__ h_in, const bf16_t* __restrict__ W1Tb,
-                                                   const int* __restrict__ pidx, int np) {
+                                                   const float* __restrict__ xs_in, int par, int na, float c,
+                                                   const bf16_t* __restrict__ W2Tb, const float* __restrict__ h_in,
+                                                   const bf16_t* __restrict__ W1Tb, int np) {
   extern __shared__ __attribute__((aligned(16))) bf16_t lsh[];
   bf16_t* W2Ts = lsh;                 // LH x LWP: W2^T [H x op]
   bf16_t* W1Ts = lsh + LH * LWP;      // LW x LHP: W1^T [kp x H]
   bf16_t* R = lsh + 2 * LH * LWP;     // LR x LHP: dO (stride LWP), then dU
   const long long r0 = (long long)blockIdx.x * LR;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, l32 = lane & 31;
-  stage_weights(W2Ts, W2Tb, W1Ts, W1Tb);
+  constexpr int RW = LR / 4;
+  const bool act = lane < na;
+  const int pcol = 32 * wave + l32;   // the passive output column of the second GEMM
+  // ---- every global load up front: indices, weights, the coupling backward's operands,
+  // the activations of the tanh backward and the passive gradients the result adds to
+  // (the coupling writes only active coordinates)
+  const int ak = 2 * lane + 1 - par;
+  const int pk = 2 * pcol + par;
+  uint4 v1[NSTAGE], v2[NSTAGE];
+  load_weights(W2Tb, W1Tb, v1, v2);
+  float gv[RW], sv[RW], xv[RW];
+#pragma unroll
+  for (int rr = 0; rr < RW; ++rr) {
+    const long long b = r0 + wave * RW + rr;
+    gv[rr] = act ? g[b * P + ak] : 0.0f;
+    sv[rr] = act ? s_in[b * na + lane] : 0.0f;
+    xv[rr] = act ? xs_in[b * na + lane] : 0.0f;
+  }
+  float hv[2][16], gp[16];
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int row = mfma32_row(lane, r);
+#pragma unroll
+    for (int t = 0; t < 2; ++t) hv[t][r] = h_in[(r0 + row) * LH + 32 * wave + 128 * t + l32];
+    gp[r] = pcol < np ? g[(r0 + row) * P + pk] : 0.0f;
+  }
+  __builtin_amdgcn_sched_barrier(0);
+  store_weights(W2Ts, W1Ts, v1, v2);
   // coupling backward, as k_coupling_fwd_bwd (na <= 64: lane j owns active coordinate j),
-  // into LDS, eight rows per wavefront with all of their loads issued first
-  {
-    constexpr int RW = LR / 4;
-    const bool act = lane < na;
-    float gv[RW], sv[RW], xv[RW];
-    long long ia[RW];
+  // into LDS, eight rows per wavefront
 #pragma unroll
-    for (int rr = 0; rr < RW; ++rr) {
-      const long long b = r0 + wave * RW + rr;
-      if (act) {
-        ia[rr] = b * P + aidx[lane];
-        gv[rr] = g[ia[rr]];
-        sv[rr] = s_in[b * na + lane];
-        xv[rr] = xs_in[b * na + lane];
-      }
+  for (int rr = 0; rr < RW; ++rr) {
+    const int row = wave * RW + rr;
+    if (act) {
+      const float ga = gv[rr];
+      const float gs = ga * xv[rr] + 1.0f;
+      const float r = sv[rr] / c;
+      const float gos = gs * (1.0f - r * r);
+      R[row * LWP + lane] = f2bf(gos);
+      R[row * LWP + na + lane] = f2bf(ga);
+      g[(r0 + row) * P + ak] = ga * expf(sv[rr]);
     }
-    __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-    for (int rr = 0; rr < RW; ++rr) {
-      const int row = wave * RW + rr;
-      if (act) {
-        const float ga = gv[rr];
-        const float gs = ga * xv[rr] + 1.0f;
-        const float r = sv[rr] / c;
-        const float gos = gs * (1.0f - r * r);
-        R[row * LWP + lane] = f2bf(gos);
-        R[row * LWP + na + lane] = f2bf(ga);
-        g[ia[rr]] = ga * expf(sv[rr]);
-      }
-      for (int j = 2 * na + lane; j < LW; j += 64) R[row * LWP + j] = (bf16_t)0;
-    }
+    for (int j = 2 * na + lane; j < LW; j += 64) R[row * LWP + j] = (bf16_t)0;
   }
   __syncthreads();
   f32x16 acc[2];
@@ -424,17 +444,14 @@ __global__ __launch_bounds__(256) void k_layer_bwd(float* __restrict__ g, int P,
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
       const int row = mfma32_row(lane, r), col = 32 * wave + 128 * t + l32;
-      const float hv = h_in[(r0 + row) * LH + col];
-      R[row * LHP + col] = f2bf(acc[t][r] * (1.0f - hv * hv));
+      R[row * LHP + col] = f2bf(acc[t][r] * (1.0f - hv[t][r] * hv[t][r]));
     }
   __syncthreads();
   f32x16 acc2[1];
   lds_gemm<1>(R, LHP, W1Ts, LHP, LH, 1, acc2);
 #pragma unroll
-  for (int r = 0; r < 16; ++r) {
-    const int row = mfma32_row(lane, r), col = 32 * wave + l32;
-    if (col < np) g[(r0 + row) * P + pidx[col]] += acc2[0][r];
-  }
+  for (int r = 0; r < 16; ++r)
+    if (pcol < np) g[(r0 + mfma32_row(lane, r)) * P + pk] = gp[r] + acc2[0][r];
 }
 
 // column sums in two passes (fixed order): COLSUM_CHUNKS row chunks, then chunks
@@ -807,9 +824,9 @@ static hipError_t forward_impl(Flow& f, const float* z, float* q, float* logdet,
   for (int l = 0; l < f.L; ++l) {
     FlowLayer& ly = f.layers[l];
     if (allow_fused && fused_layer(f, l)) {
-      hipLaunchKernelGGL(k_layer_fwd, dim3(B / LR), dim3(T256), LAYER_LDS, st, s.x, f.P, ly.pidx, ly.np, ly.W1b,
-                         f.params + ly.ob1, ly.W2b, s.O, f.params + ly.ob2, ly.aidx, ly.na, f.clamp, s.h[l], s.s[l],
-                         s.xa[l], s.logdet);
+      hipLaunchKernelGGL(k_layer_fwd, dim3(B / LR), dim3(T256), LAYER_LDS, st, s.x, f.P, l & 1, ly.np, ly.W1b,
+                         f.params + ly.ob1, ly.W2b, f.params + ly.ob2, ly.na, f.clamp, s.h[l], s.s[l], s.xa[l],
+                         s.logdet);
       continue;
     }
     hipError_t e = conditioner(f, l, B, st);
@@ -849,8 +866,8 @@ hipError_t flow_backward(Flow& f, const float* g_q, float* g_z, int B, hipStream
     FlowLayer& ly = f.layers[l];
     const int op = round_up(2 * ly.na, FLOW_PAD);
     if (fused_layer(f, l)) {
-      hipLaunchKernelGGL(k_layer_bwd, dim3(B / LR), dim3(T256), LAYER_LDS, st, s.g, f.P, s.s[l], s.xa[l], ly.aidx,
-                         ly.na, f.clamp, ly.W2Tb, s.h[l], ly.W1Tb, ly.pidx, ly.np);
+      hipLaunchKernelGGL(k_layer_bwd, dim3(B / LR), dim3(T256), LAYER_LDS, st, s.g, f.P, s.s[l], s.xa[l], l & 1,
+                         ly.na, f.clamp, ly.W2Tb, s.h[l], ly.W1Tb, ly.np);
       continue;
     }
     hipLaunchKernelGGL(k_coupling_fwd_bwd, dim3((B + 3) / 4), dim3(T256), 0, st, s.g, f.P, s.s[l], s.xa[l], ly.aidx,
