@@ -330,17 +330,23 @@ int ut_workspace_set_flow(ut_workspace* ws, ut_flow* flow);
  * (beta1, beta2, adam_epsilon; the reference's defaults 0.9, 0.999, 1e-8; fresh
  * moments per call) on the mean over the valid
  * particles (finite logjoint, log-determinant and gradient) of the gradient of
- * g . f(z) + log|det df/dz| with g = grad log p(f(z)) held fixed.
- * elbo_history[i] = the sampled standard ELBO (mean log p - log q over the valid
- * particles), grad_norm_history[i] = the gradient norm, clipped at gradient_clip
+ * g . f(z) + log|det df/dz| with g = grad log p(f(z)) held fixed.  iwae_group = K > 1
+ * (dividing num_chains) optimises the importance-weighted bound instead
+ * (elbo = :iwae, src/inference/vi/guides.jl:373-425): per group of K particles the
+ * log-mean-exp of log w = log p - log q, and the gradient weights each particle by
+ * its within-group softmax weight, averaged over the groups with a valid particle.
+ * elbo_history[i] = the optimised objective, standard_elbo_history[i] = the sampled
+ * standard ELBO (mean log w over the valid particles), grad_norm_history[i] = the
+ * gradient norm, clipped at gradient_clip
  * (INFINITY: no clipping); either may be NULL.  The flow keeps the last weights;
  * best_weights (ut_flow_num_weights floats, or NULL) receives the weights of the
  * best ELBO seen and best_elbo that value.  UT_E_ARG when an iteration has no
  * valid particle (reference: ArgumentError "encountered only non-finite ...").
  * The workspace must be UT_FLOAT32 with num_chains a multiple of 128.            */
 int ut_flow_fit_elbo(ut_workspace* ws, ut_flow* flow, int32_t num_iterations, double learning_rate, double beta1,
-                     double beta2, double adam_epsilon, double gradient_clip, uint64_t seed, double* elbo_history,
-                     double* grad_norm_history, float* best_weights, double* best_elbo);
+                     double beta2, double adam_epsilon, double gradient_clip, int32_t iwae_group, uint64_t seed,
+                     double* elbo_history, double* standard_elbo_history, double* grad_norm_history,
+                     float* best_weights, double* best_elbo);
 /* num_chains initial positions (num_chains x P, chain-major, unconstrained) as the
  * samplers draw them for `seed` before their non-finite re-draw loop: prior draws
  * (UT_INIT_PRIOR) or Uniform(-2, 2) (UT_INIT_UNIFORM) from Philox; host-only.
@@ -352,9 +358,11 @@ int ut_initial_params(const ut_plan* plan, int32_t strategy, uint64_t seed, int3
  * rows): the sampled ELBO, the unclipped mean weight gradient over the valid
  * particles (ut_flow_num_weights floats), and optionally the model gradient at
  * f(z) that was held fixed (num_chains x P, zero on invalid particles) and the
- * validity flags (num_chains).  The analogue of ut_flow_loss_grad for the ELBO.   */
-int ut_flow_elbo_grad(ut_workspace* ws, ut_flow* flow, const float* z, double* elbo, float* grad, float* g_q,
-                      int32_t* valid);
+ * validity flags (num_chains), and the particles' weights in the gradient
+ * (num_chains: 1 / valid count, or the importance weights over the groups used).
+ * The analogue of ut_flow_loss_grad for the ELBO.                                 */
+int ut_flow_elbo_grad(ut_workspace* ws, ut_flow* flow, const float* z, int32_t iwae_group, double* elbo, float* grad,
+                      float* g_q, int32_t* valid, float* row_weights);
 
 /* ---- pinned host memory ----------------------------------------------------------
  * Page-locked host buffers for sampler outputs: the samplers copy each launch

@@ -50,7 +50,7 @@ def _load():
         "orc_flow_inverse": (None, [i32, i32, i32, dbl, i32, vp, vp, i64, vp, vp]),
         "orc_flow_loss_grad": (dbl, [i32, i32, i32, dbl, i32, vp, vp, i64, vp]),
         "orc_flow_num_weights": (i64, [i32, i32, i32]),
-        "orc_flow_elbo_grad": (None, [i32, i32, i32, dbl, i32, vp, vp, vp, vp, i64, vp]),
+        "orc_flow_elbo_grad": (None, [i32, i32, i32, dbl, i32, vp, vp, vp, vp, vp, i64, vp]),
         "orc_chees_halton": (None, [i32, vp]),
         "orc_chees_steps": (i32, [dbl, dbl, dbl]),
         "orc_chees_trajectory": (None, [dbl, dbl, dbl, dbl, dbl, dbl, i32, vp, vp, vp, vp, i32, vp]),
@@ -343,16 +343,42 @@ class OracleFlow:
                                           q.shape[0], _p(g))
         return float(loss), g
 
-    def elbo_grad(self, z, g_q, valid=None):
+    def elbo_grad(self, z, g_q, valid=None, row_weights=None):
         """Mean over valid rows of the weight gradient of g_q . f(z) + log|det df/dz|
-        with g_q held fixed (the flow ADVI gradient, src/inference/vi.jl:807-826)."""
+        with g_q held fixed (the flow ADVI gradient, src/inference/vi.jl:807-826);
+        with row_weights, the weighted sum over rows instead (the IWAE gradient)."""
         z = np.ascontiguousarray(np.asarray(z, dtype=np.float64).reshape(-1, self.P))
         g_q = np.ascontiguousarray(np.asarray(g_q, dtype=np.float64).reshape(-1, self.P))
         v = None if valid is None else np.ascontiguousarray(np.asarray(valid, dtype=np.int32))
+        w = None if row_weights is None else np.ascontiguousarray(np.asarray(row_weights, dtype=np.float64))
         g = np.empty(self.w.shape[0])
         _load().orc_flow_elbo_grad(self.P, self.L, self.H, self.clamp, int(self.bf16), _p(self.w), _p(z), _p(g_q),
-                                   _p(v), z.shape[0], _p(g))
+                                   _p(v), _p(w), z.shape[0], _p(g))
         return g
+
+
+def iwae_weights_and_bound(logw, valid, group_size):
+    """The reference's _iwae_weights_and_bound! (src/inference/vi/guides.jl:383-425):
+    per group of `group_size` particles the log-mean-exp of the valid log-weights,
+    self-normalised weights within the group, all divided by the groups used.
+    Returns (weights, bound)."""
+    logw = np.asarray(logw, dtype=np.float64)
+    valid = np.asarray(valid, dtype=bool)
+    n = logw.size
+    w = np.zeros(n)
+    bound, used = 0.0, 0
+    for g0 in range(0, n - n % group_size, group_size):
+        idx = np.arange(g0, g0 + group_size)[valid[g0:g0 + group_size]]
+        if idx.size == 0:
+            continue
+        m = logw[idx].max()
+        lse = m + np.log(np.sum(np.exp(logw[idx] - m)))
+        bound += lse - np.log(idx.size)
+        w[idx] = np.exp(logw[idx] - lse)
+        used += 1
+    if used == 0:
+        raise ValueError("batched_advi encountered only non-finite importance weights")
+    return w / used, bound / used
 
 
 def meads_max_eig(columns):

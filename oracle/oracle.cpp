@@ -2229,22 +2229,31 @@ double orc_flow_loss_grad(int32_t P, int32_t L, int32_t H, double clamp, int32_t
   return loss_total / (double)n;
 }
 // Mean over the valid rows (valid[r] != 0, or all rows when valid is NULL) of the
-// weight gradient of g_q . f(z) + log|det df/dz| with g_q held fixed.
+// weight gradient of g_q . f(z) + log|det df/dz| with g_q held fixed; with
+// row_weights, instead the sum over rows of row_weights[r] times that gradient (the
+// importance-weighted bound's gradient, src/inference/vi/guides.jl:373-381).
 void orc_flow_elbo_grad(int32_t P, int32_t L, int32_t H, double clamp, int32_t bf16, const float* weights,
-                        const double* z, const double* g_q, const int32_t* valid, int64_t n, double* grad) {
+                        const double* z, const double* g_q, const int32_t* valid, const double* row_weights,
+                        int64_t n, double* grad) {
   FlowSpec fs(P, L, H, clamp, bf16 != 0, weights);
   const long long np = fs.nparams();
   std::vector<double> total((size_t)np, 0.0);
   int64_t nv = 0;
 #pragma omp parallel
   {
-    std::vector<double> gw((size_t)np, 0.0), q(P), g(P);
+    std::vector<double> gw((size_t)np, 0.0), gr((size_t)np), q(P), g(P);
     int64_t cnt = 0;
 #pragma omp for schedule(static)
     for (int64_t r = 0; r < n; ++r) {
       if (valid && !valid[r]) continue;
       std::copy(g_q + r * P, g_q + (r + 1) * P, g.begin());
-      fs.forward(z + r * P, q.data(), g.data(), gw.data());
+      if (row_weights) {
+        std::fill(gr.begin(), gr.end(), 0.0);
+        fs.forward(z + r * P, q.data(), g.data(), gr.data());
+        for (long long i = 0; i < np; ++i) gw[i] += row_weights[r] * gr[i];
+      } else {
+        fs.forward(z + r * P, q.data(), g.data(), gw.data());
+      }
       ++cnt;
     }
 #pragma omp critical
@@ -2253,7 +2262,7 @@ void orc_flow_elbo_grad(int32_t P, int32_t L, int32_t H, double clamp, int32_t b
       nv += cnt;
     }
   }
-  for (long long i = 0; i < np; ++i) grad[i] = nv ? total[i] / (double)nv : 0.0;
+  for (long long i = 0; i < np; ++i) grad[i] = row_weights ? total[i] : (nv ? total[i] / (double)nv : 0.0);
 }
 int64_t orc_flow_num_weights(int32_t P, int32_t L, int32_t H) {
   FlowSpec fs(P, L, H, 1.0, false, nullptr);

@@ -49,7 +49,9 @@ def test_initial_params_argument_errors():
 @pytest.mark.parametrize("kw,msg", [
     ({"guide": "fullrank"}, "CPU-only"),
     ({"guide": "bogus"}, "guide must be"),
-    ({"elbo": "iwae"}, "standard ELBO"),
+    ({"elbo": "bogus"}, "elbo must be"),
+    ({"elbo": "iwae", "iwae_samples": 100}, "divide num_particles evenly"),
+    ({"elbo": "iwae", "iwae_samples": 256}, "1 <= iwae_samples"),
     ({"num_iterations": 0}, "num_iterations > 0"),
     ({"num_particles": 100}, "multiple of 128"),
     ({"learning_rate": 0.0}, "learning_rate > 0"),

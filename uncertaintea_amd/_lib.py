@@ -171,10 +171,10 @@ def lib() -> ct.CDLL:
         "ut_flow_vjp": (ct.c_int, [vp, vp, i64, vp, vp]),
         "ut_flow_loss_grad": (ct.c_int, [vp, vp, i64, ct.POINTER(dbl), vp]),
         "ut_workspace_set_flow": (ct.c_int, [vp, vp]),
-        "ut_flow_fit_elbo": (ct.c_int, [vp, vp, i32, dbl, dbl, dbl, dbl, dbl, ct.c_uint64, vp, vp, vp,
+        "ut_flow_fit_elbo": (ct.c_int, [vp, vp, i32, dbl, dbl, dbl, dbl, dbl, i32, ct.c_uint64, vp, vp, vp, vp,
                                          ct.POINTER(dbl)]),
         "ut_initial_params": (ct.c_int, [vp, i32, ct.c_uint64, i32, vp]),
-        "ut_flow_elbo_grad": (ct.c_int, [vp, vp, vp, ct.POINTER(dbl), vp, vp, vp]),
+        "ut_flow_elbo_grad": (ct.c_int, [vp, vp, vp, i32, ct.POINTER(dbl), vp, vp, vp, vp]),
         "ut_philox4x32": (None, [vp, vp, vp]),
         "ut_philox_fill": (ct.c_int, [u32, u32, u32, u32, i64, vp, vp, vp, vp]),
         "ut_host_alloc": (ct.c_int, [i64, pp]),

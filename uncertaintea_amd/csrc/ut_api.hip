@@ -1703,41 +1703,58 @@ int elbo_check(ut_workspace* ws, ut_flow* flow, const char* who) {
 // Device buffers of one ELBO evaluation over the workspace's C particles.
 struct ElboBuffers {
   DeviceBuffers buf;
-  float *z = nullptr, *lj = nullptr, *rw = nullptr;
-  double *logw = nullptr, *part = nullptr, *stats = nullptr;
+  float *z = nullptr, *lj = nullptr, *rw = nullptr, *wts = nullptr;
+  double *logw = nullptr, *part = nullptr, *stats = nullptr, *gstat = nullptr;
   hipError_t alloc(int B, int P) {
     hipError_t e = buf.alloc(&z, (size_t)B * P);
     if (e == hipSuccess) e = buf.alloc(&lj, (size_t)B);
     if (e == hipSuccess) e = buf.alloc(&rw, (size_t)B);
+    if (e == hipSuccess) e = buf.alloc(&wts, (size_t)B);
     if (e == hipSuccess) e = buf.alloc(&logw, (size_t)B);
     if (e == hipSuccess) e = buf.alloc(&part, 256);
-    if (e == hipSuccess) e = buf.alloc(&stats, 4);
+    if (e == hipSuccess) e = buf.alloc(&stats, 5);
+    if (e == hipSuccess) e = buf.alloc(&gstat, 2 * (size_t)B);
     return e;
   }
 };
 
 // particles z (in b.z) -> flow -> model logjoint and gradient -> per-particle
-// validity and log-weights -> summed weight gradient (f.grads) -> b.stats
-hipError_t elbo_eval(ut_workspace* ws, Flow& f, ElboBuffers& b, double clip) {
+// validity and log-weights -> (importance weights of groups of K > 1) -> summed
+// weight gradient (f.grads) -> b.stats
+hipError_t elbo_eval(ut_workspace* ws, Flow& f, ElboBuffers& b, double clip, int K) {
   const int B = ws->C;
   hipStream_t st = ws->stream;
   hipError_t e = flow_forward_train(f, b.z, B, st);
   if (e == hipSuccess) e = model_grad(ws, f.sc.x, b.lj, f.sc.g, true);
   if (e == hipSuccess) e = flow_elbo_rows(f, b.z, b.lj, B, b.rw, b.logw, st);
-  if (e == hipSuccess) e = flow_elbo_backward(f, b.z, b.rw, B, st);
-  if (e == hipSuccess) e = flow_elbo_stats(f, b.rw, b.logw, B, clip, b.part, b.stats, st);
+  if (K > 1) {
+    if (e == hipSuccess) e = flow_iwae_weights(f, b.rw, b.logw, B, K, b.wts, b.gstat, st);
+    if (e == hipSuccess) e = flow_elbo_backward(f, b.z, b.wts, B, st);
+    if (e == hipSuccess) e = flow_elbo_stats(f, b.rw, b.logw, B, b.gstat, B / K, clip, b.part, b.stats, st);
+  } else {
+    if (e == hipSuccess) e = flow_elbo_backward(f, b.z, b.rw, B, st);
+    if (e == hipSuccess) e = flow_elbo_stats(f, b.rw, b.logw, B, nullptr, 0, clip, b.part, b.stats, st);
+  }
   return e;
+}
+
+int iwae_check(int B, int32_t K, const char* who) {
+  if (K < 1 || B % K != 0)
+    return fail(UT_E_ARG, std::string(who) + ": iwae_group must be >= 1 and divide num_chains (1: the standard ELBO)");
+  return UT_OK;
 }
 
 }  // namespace
 
 extern "C" {
 
-int ut_flow_elbo_grad(ut_workspace* ws, ut_flow* flow, const float* z, double* elbo, float* grad, float* g_q,
-                      int32_t* valid) {
+int ut_flow_elbo_grad(ut_workspace* ws, ut_flow* flow, const float* z, int32_t iwae_group, double* elbo, float* grad,
+                      float* g_q, int32_t* valid, float* row_weights) {
   int rc = elbo_check(ws, flow, "ut_flow_elbo_grad");
   if (rc) return rc;
   if (!z || !grad) return fail(UT_E_ARG, "ut_flow_elbo_grad: null pointer");
+  rc = iwae_check(ws->C, iwae_group, "ut_flow_elbo_grad");
+  if (rc) return rc;
   Flow& f = flow->f;
   const int B = ws->C, P = ws->P;
   hipStream_t st = ws->stream;
@@ -1746,9 +1763,10 @@ int ut_flow_elbo_grad(ut_workspace* ws, ut_flow* flow, const float* z, double* e
   ElboBuffers b;
   UT_HIP(b.alloc(B, P));
   UT_HIP(hipMemcpyAsync(b.z, z, (size_t)B * P * 4, hipMemcpyHostToDevice, st));
-  UT_HIP(elbo_eval(ws, f, b, std::numeric_limits<double>::infinity()));
-  double h[4];
-  std::vector<float> rw((size_t)B);
+  UT_HIP(elbo_eval(ws, f, b, std::numeric_limits<double>::infinity(), iwae_group));
+  double h[5];
+  std::vector<float> rw((size_t)B), wt((size_t)B);
+  if (iwae_group > 1) UT_HIP(hipMemcpyAsync(wt.data(), b.wts, (size_t)B * 4, hipMemcpyDeviceToHost, st));
   if (g_q) {  // the model gradient at f(z) (zero on invalid rows) before the backward pass consumed it
     UT_HIP(hipMemcpyAsync(b.z, z, (size_t)B * P * 4, hipMemcpyHostToDevice, st));
     UT_HIP(flow_forward_train(f, b.z, B, st));
@@ -1765,6 +1783,8 @@ int ut_flow_elbo_grad(ut_workspace* ws, ut_flow* flow, const float* z, double* e
   if (elbo) *elbo = h[1];
   const float sc = h[0] > 0 ? (float)(1.0 / h[0]) : 0.0f;
   for (long long i = 0; i < f.nparams; ++i) grad[i] *= sc;
+  if (row_weights)
+    for (int i = 0; i < B; ++i) row_weights[i] = iwae_group > 1 ? wt[i] * sc : rw[i] * sc;
   return UT_OK;
 }
 
@@ -1780,9 +1800,12 @@ int ut_initial_params(const ut_plan* plan, int32_t strategy, uint64_t seed, int3
 }
 
 int ut_flow_fit_elbo(ut_workspace* ws, ut_flow* flow, int32_t num_iterations, double learning_rate, double beta1,
-                     double beta2, double adam_epsilon, double gradient_clip, uint64_t seed, double* elbo_history,
-                     double* grad_norm_history, float* best_weights, double* best_elbo) {
+                     double beta2, double adam_epsilon, double gradient_clip, int32_t iwae_group, uint64_t seed,
+                     double* elbo_history, double* standard_elbo_history, double* grad_norm_history,
+                     float* best_weights, double* best_elbo) {
   int rc = elbo_check(ws, flow, "ut_flow_fit_elbo");
+  if (rc) return rc;
+  rc = iwae_check(ws->C, iwae_group, "ut_flow_fit_elbo");
   if (rc) return rc;
   if (num_iterations < 0) return fail(UT_E_ARG, "ut_flow_fit_elbo: num_iterations must be >= 0");
   if (!(learning_rate > 0)) return fail(UT_E_ARG, "ut_flow_fit_elbo: learning_rate must be > 0");
@@ -1806,16 +1829,18 @@ int ut_flow_fit_elbo(ut_workspace* ws, ut_flow* flow, int32_t num_iterations, do
   UT_HIP(hipMemsetAsync(f.adam_v, 0, (size_t)f.nparams * 4, st));
   f.adam_t = 0;
   double best_obj = -std::numeric_limits<double>::infinity();
-  double h[4];
+  double h[5];
   for (int it = 0; it < num_iterations; ++it) {
     UT_HIP(flow_normal_fill(b.z, B, P, (uint32_t)seed, (uint32_t)it, st));
-    UT_HIP(elbo_eval(ws, f, b, gradient_clip));
+    UT_HIP(elbo_eval(ws, f, b, gradient_clip, iwae_group));
     UT_HIP(hipMemcpyAsync(h, b.stats, sizeof(h), hipMemcpyDeviceToHost, st));
     UT_HIP(hipStreamSynchronize(st));
     if (!(h[0] > 0))
-      return fail(UT_E_ARG, "ut_flow_fit_elbo: only non-finite logjoint values or gradients at iteration " +
-                                std::to_string(it));
+      return fail(UT_E_ARG, std::string("ut_flow_fit_elbo: only non-finite ") +
+                                (iwae_group > 1 ? "importance weights" : "logjoint values or gradients") +
+                                " at iteration " + std::to_string(it));
     if (elbo_history) elbo_history[it] = h[1];
+    if (standard_elbo_history) standard_elbo_history[it] = h[4];
     if (grad_norm_history) grad_norm_history[it] = std::isfinite(gradient_clip) ? std::min(h[2], gradient_clip) : h[2];
     if (h[1] > best_obj) {  // the weights this objective was evaluated at
       best_obj = h[1];

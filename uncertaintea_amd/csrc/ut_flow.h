@@ -110,10 +110,18 @@ hipError_t flow_elbo_rows(Flow& f, const float* z, const float* lj, int B, float
 // Sum over rows of the weight gradient of rw[b] (g_q . f(z) + logdet) with g_q = sc.g
 // held fixed (invalid rows contribute nothing), into f.grads
 hipError_t flow_elbo_backward(Flow& f, const float* z, const float* rw, int B, hipStream_t st);
-// stats (device, 4 doubles) = {valid rows, mean logw over them, gradient norm of the
-// mean, update scale = min(1, clip / norm) / valid rows}; part: >= 256 doubles
-hipError_t flow_elbo_stats(Flow& f, const float* rw, const double* logw, int B, double clip, double* part,
-                           double* stats, hipStream_t st);
+// stats (device, 5 doubles) = {divisor, objective, gradient norm of the mean, update
+// scale = min(1, clip / norm) / divisor, standard ELBO}: divisor = valid rows and
+// objective = the standard ELBO (mean logw over them), or with group statistics
+// (gstat, ngroups) divisor = groups used and objective = the importance-weighted
+// bound; part: >= 256 doubles
+hipError_t flow_elbo_stats(Flow& f, const float* rw, const double* logw, int B, const double* gstat, int ngroups,
+                           double clip, double* part, double* stats, hipStream_t st);
+// importance weights of groups of K particles (K divides B): wts[b] within-group
+// self-normalised weights (0 when invalid), gstat[2g] the group's log-mean-exp bound,
+// gstat[2g + 1] whether it had a valid particle; scales sc.g's rows by wts
+hipError_t flow_iwae_weights(Flow& f, const float* rw, const double* logw, int B, int K, float* wts, double* gstat,
+                             hipStream_t st);
 // Adam ascent on f.grads scaled by stats[3]; no update when stats[0] == 0
 hipError_t flow_adam_ascent(Flow& f, const double* stats, double lr, double b1, double b2, double eps,
                             hipStream_t st);

@@ -584,38 +584,94 @@ __global__ __launch_bounds__(256) void k_sumsq_part(const float* __restrict__ g,
   if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
 }
 
-// one block: valid rows, mean log-weight, clipped update scale (fixed summation order)
+// one block: valid rows, mean log-weight, the importance-weighted bound when group
+// statistics are given, clipped update scale (fixed summation order).  stats =
+// {divisor (valid rows, or groups used), objective, gradient norm of the mean,
+// update scale, standard ELBO}
 __global__ __launch_bounds__(256) void k_elbo_stats(const float* __restrict__ rw, const double* __restrict__ logw,
-                                                    int B, const double* __restrict__ part, int nparts, double clip,
+                                                    int B, const double* __restrict__ part, int nparts,
+                                                    const double* __restrict__ gstat, int ngroups, double clip,
                                                     double* __restrict__ stats) {
-  __shared__ double sv[256], sl[256], sq[256];
-  double v = 0.0, l = 0.0, q = 0.0;
+  __shared__ double sv[256], sl[256], sq[256], su[256], sb[256];
+  double v = 0.0, l = 0.0, q = 0.0, u = 0.0, bd = 0.0;
   for (int b = threadIdx.x; b < B; b += 256) {
     v += (double)rw[b];
     l += logw[b];
   }
   for (int i = threadIdx.x; i < nparts; i += 256) q += part[i];
+  if (gstat)
+    for (int g = threadIdx.x; g < ngroups; g += 256) {
+      bd += gstat[2 * g];
+      u += gstat[2 * g + 1];
+    }
   sv[threadIdx.x] = v;
   sl[threadIdx.x] = l;
   sq[threadIdx.x] = q;
+  su[threadIdx.x] = u;
+  sb[threadIdx.x] = bd;
   __syncthreads();
   for (int w = 128; w >= 1; w >>= 1) {
     if ((int)threadIdx.x < w) {
       sv[threadIdx.x] += sv[threadIdx.x + w];
       sl[threadIdx.x] += sl[threadIdx.x + w];
       sq[threadIdx.x] += sq[threadIdx.x + w];
+      su[threadIdx.x] += su[threadIdx.x + w];
+      sb[threadIdx.x] += sb[threadIdx.x + w];
     }
     __syncthreads();
   }
   if (threadIdx.x == 0) {
     const double nv = sv[0];
-    const double norm = nv > 0.0 ? sqrt(sq[0]) / nv : 0.0;
+    const double dv = gstat ? su[0] : nv;  // IWAE: group weights already sum to one per group
+    const double norm = dv > 0.0 ? sqrt(sq[0]) / dv : 0.0;
     const double cf = (isfinite(clip) && norm > clip) ? clip / norm : 1.0;
-    stats[0] = nv;
-    stats[1] = nv > 0.0 ? sl[0] / nv : 0.0;
+    const double elbo = nv > 0.0 ? sl[0] / nv : 0.0;
+    stats[0] = dv;
+    stats[1] = gstat ? (dv > 0.0 ? sb[0] / dv : 0.0) : elbo;
     stats[2] = norm;
-    stats[3] = nv > 0.0 ? cf / nv : 0.0;
+    stats[3] = dv > 0.0 ? cf / dv : 0.0;
+    stats[4] = elbo;
   }
+}
+
+// one wavefront per group of K particles (the reference's _iwae_weights_and_bound!,
+// src/inference/vi/guides.jl:383-425): log-sum-exp of the valid log-weights,
+// self-normalised weights wts (zero for invalid particles; divided by the groups
+// used later, through the update scale) and gstat[2g] = lse - log(valid count),
+// gstat[2g + 1] = 1 when the group has a valid particle
+__global__ __launch_bounds__(256) void k_iwae_groups(const float* __restrict__ rw, const double* __restrict__ logw,
+                                                     int K, int ngroups, float* __restrict__ wts,
+                                                     double* __restrict__ gstat) {
+  const int g = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (g >= ngroups) return;
+  const long long b0 = (long long)g * K;
+  double m = -INFINITY, cnt = 0.0;
+  for (int k = lane; k < K; k += 64)
+    if (rw[b0 + k] != 0.0f) {
+      m = fmax(m, logw[b0 + k]);
+      cnt += 1.0;
+    }
+  for (int o = 32; o >= 1; o >>= 1) {
+    m = fmax(m, __shfl_xor(m, o));
+    cnt += __shfl_xor(cnt, o);
+  }
+  double se = 0.0;
+  for (int k = lane; k < K; k += 64)
+    if (rw[b0 + k] != 0.0f) se += exp(logw[b0 + k] - m);
+  for (int o = 32; o >= 1; o >>= 1) se += __shfl_xor(se, o);
+  const double lse = m + log(se);
+  for (int k = lane; k < K; k += 64) wts[b0 + k] = rw[b0 + k] != 0.0f ? (float)exp(logw[b0 + k] - lse) : 0.0f;
+  if (lane == 0) {
+    gstat[2 * g] = cnt > 0.0 ? lse - log(cnt) : 0.0;
+    gstat[2 * g + 1] = cnt > 0.0 ? 1.0 : 0.0;
+  }
+}
+
+// g[b, :] *= wts[b] (the model-gradient half of the weighted accumulation)
+__global__ void k_scale_rows(float* __restrict__ g, const float* __restrict__ wts, int P, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) g[i] *= wts[i / P];
 }
 
 // Adam ascent (the reference's _adam_ascent_step!, src/inference/vi.jl:205-227) on
@@ -973,11 +1029,22 @@ hipError_t flow_elbo_backward(Flow& f, const float* z, const float* rw, int B, h
   return hipGetLastError();
 }
 
-hipError_t flow_elbo_stats(Flow& f, const float* rw, const double* logw, int B, double clip, double* part,
-                           double* stats, hipStream_t st) {
+hipError_t flow_elbo_stats(Flow& f, const float* rw, const double* logw, int B, const double* gstat, int ngroups,
+                           double clip, double* part, double* stats, hipStream_t st) {
   constexpr int NPART = 256;
   hipLaunchKernelGGL(k_sumsq_part, dim3(NPART), dim3(T256), 0, st, f.grads, f.nparams, part);
-  hipLaunchKernelGGL(k_elbo_stats, dim3(1), dim3(T256), 0, st, rw, logw, B, part, NPART, clip, stats);
+  hipLaunchKernelGGL(k_elbo_stats, dim3(1), dim3(T256), 0, st, rw, logw, B, part, NPART, gstat, ngroups, clip,
+                     stats);
+  return hipGetLastError();
+}
+
+hipError_t flow_iwae_weights(Flow& f, const float* rw, const double* logw, int B, int K, float* wts, double* gstat,
+                             hipStream_t st) {
+  if (K < 1 || B % K) return hipErrorInvalidValue;
+  const int ngroups = B / K;
+  hipLaunchKernelGGL(k_iwae_groups, dim3((ngroups + 3) / 4), dim3(T256), 0, st, rw, logw, K, ngroups, wts, gstat);
+  const long long n = (long long)B * f.P;
+  hipLaunchKernelGGL(k_scale_rows, dim3(nblk(n)), dim3(T256), 0, st, f.sc.g, wts, f.P, n);
   return hipGetLastError();
 }
 

@@ -107,11 +107,13 @@ class AffineCouplingFlow:
 
     def fit_elbo(self, model, num_particles: int = 4096, num_iterations: int = 1000, learning_rate: float = 1e-3,
                  gradient_clip: float = float("inf"), seed: int = 0, keep_best: bool = False, beta1: float = 0.9,
-                 beta2: float = 0.999, adam_epsilon: float = 1e-8) -> dict:
+                 beta2: float = 0.999, adam_epsilon: float = 1e-8, iwae_group: int = 1) -> dict:
         """Reparameterised-ELBO fit against `model` on the GPU, the reference's flow
         ADVI (src/inference/vi.jl:700-870): `num_iterations` Adam ascent steps, each
         on `num_particles` fresh particles (a multiple of 128); invalid particles
-        (non-finite logjoint or gradient) carry no weight.  Returns elbo_history,
+        (non-finite logjoint or gradient) carry no weight.  iwae_group = K > 1
+        optimises the importance-weighted bound over groups of K particles.  Returns
+        elbo_history (the optimised objective), standard_elbo_history,
         grad_norm_history, best_elbo and best_weights; the flow keeps its last
         weights unless keep_best."""
         from .device import DeviceBatchedWorkspace
@@ -120,21 +122,24 @@ class AffineCouplingFlow:
         ws = DeviceBatchedWorkspace(model, int(num_particles), np.float32)
         n = int(num_iterations)
         elbo = np.zeros(n)
+        std_elbo = np.zeros(n)
         gnorm = np.zeros(n)
         best = np.empty(self.num_weights, dtype=np.float32)
         best_elbo = ct.c_double()
         L.check(L.lib().ut_flow_fit_elbo(ws.handle, self._h, n, float(learning_rate), float(beta1), float(beta2),
-                                         float(adam_epsilon), float(gradient_clip), int(seed) & 0xFFFFFFFFFFFFFFFF,
-                                         L.ptr(elbo), L.ptr(gnorm), L.ptr(best), ct.byref(best_elbo)))
+                                         float(adam_epsilon), float(gradient_clip), int(iwae_group),
+                                         int(seed) & 0xFFFFFFFFFFFFFFFF, L.ptr(elbo), L.ptr(std_elbo), L.ptr(gnorm),
+                                         L.ptr(best), ct.byref(best_elbo)))
         if keep_best and n > 0:
             self.weights = best
         self.elbo_trace = elbo
-        return {"elbo_history": elbo, "grad_norm_history": gnorm, "best_elbo": float(best_elbo.value),
-                "best_weights": best}
+        return {"elbo_history": elbo, "standard_elbo_history": std_elbo, "grad_norm_history": gnorm,
+                "best_elbo": float(best_elbo.value), "best_weights": best}
 
-    def elbo_grad(self, model, z):
-        """One ELBO evaluation at particles z (rows, a multiple of 128): (elbo, mean
-        weight gradient, model gradient at f(z), validity flags)."""
+    def elbo_grad(self, model, z, iwae_group: int = 1, return_weights: bool = False):
+        """One ELBO (or, with iwae_group > 1, importance-weighted bound) evaluation at
+        particles z (rows, a multiple of 128): (objective, mean weight gradient, model
+        gradient at f(z), validity flags[, the particles' gradient weights])."""
         from .device import DeviceBatchedWorkspace
         zz = np.ascontiguousarray(np.asarray(z, dtype=np.float32).reshape(-1, self.num_params))
         ws = DeviceBatchedWorkspace(model, zz.shape[0], np.float32)
@@ -142,9 +147,11 @@ class AffineCouplingFlow:
         g = np.empty(self.num_weights, dtype=np.float32)
         gq = np.empty_like(zz)
         valid = np.empty(zz.shape[0], dtype=np.int32)
-        L.check(L.lib().ut_flow_elbo_grad(ws.handle, self._h, L.ptr(zz), ct.byref(elbo), L.ptr(g), L.ptr(gq),
-                                          L.ptr(valid)))
-        return float(elbo.value), g, gq, valid.astype(bool)
+        rw = np.empty(zz.shape[0], dtype=np.float32)
+        L.check(L.lib().ut_flow_elbo_grad(ws.handle, self._h, L.ptr(zz), int(iwae_group), ct.byref(elbo), L.ptr(g),
+                                          L.ptr(gq), L.ptr(valid), L.ptr(rw)))
+        out = (float(elbo.value), g, gq, valid.astype(bool))
+        return out + (rw,) if return_weights else out
 
     def forward(self, z):
         """(q, log|det dq/dz|) for rows of z."""

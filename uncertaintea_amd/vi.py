@@ -12,7 +12,9 @@ gradient held fixed, clips its flat norm and takes one Adam ascent step
 (src/inference/vi.jl:205-227, 807-832).  The particles come from Philox on the
 device, so results agree with the reference statistically, not draw for draw (as
 the reference's own device ADVI path states, src/inference/vi.jl:367-368).
-Not built: the full-rank and low-rank guides and the IWAE objective.
+elbo="iwae" optimises the importance-weighted bound over groups of
+`iwae_samples` particles (src/inference/vi/guides.jl:373-425).  Not built: the
+full-rank and low-rank guides.
 """
 from __future__ import annotations
 
@@ -83,10 +85,11 @@ def _scale_vector(initial_log_scale, P: int) -> np.ndarray:
 
 def batched_advi(model: TeaModel, num_iterations: int, num_particles: int = 128, learning_rate: float = 0.05,
                  initial_params=None, initial_log_scale=-1.0, guide: str = "meanfield", num_flow_layers: int = 4,
-                 flow_hidden: int = 256, elbo: str = "standard", beta1: float = 0.9, beta2: float = 0.999,
-                 adam_epsilon: float = 1e-8, gradient_clip: float = float("inf"), seed: int = 0) -> ADVIResult:
-    """batched_advi(model; num_iterations, num_particles, learning_rate, guide, ...)
-    (src/inference/vi.jl:300-440) on the device.  `num_particles` must be a multiple
+                 flow_hidden: int = 256, elbo: str = "standard", iwae_samples: int = 0, beta1: float = 0.9,
+                 beta2: float = 0.999, adam_epsilon: float = 1e-8, gradient_clip: float = float("inf"),
+                 seed: int = 0) -> ADVIResult:
+    """batched_advi(model; num_iterations, num_particles, learning_rate, guide, elbo,
+    iwae_samples, ...) (src/inference/vi.jl:300-440) on the device.  `num_particles` must be a multiple
     of 128 (the particles are the rows of one batched gradient; the reference's
     default is 32).  Argument checks and messages follow the reference's."""
     if guide not in _GUIDES:
@@ -94,14 +97,20 @@ def batched_advi(model: TeaModel, num_iterations: int, num_particles: int = 128,
             raise ValueError(f"batched_advi guide={guide} is CPU-only in the reference and not built here; the "
                              "device path supports meanfield and flow")
         raise ValueError(f"batched_advi guide must be :meanfield, :fullrank, :lowrank, or :flow, got {guide}")
-    if elbo != "standard":
-        raise ValueError("batched_advi elbo=:iwae is CPU-only; the device path supports the standard ELBO")
+    if elbo not in ("standard", "iwae"):
+        raise ValueError(f"batched_advi elbo must be :standard or :iwae, got :{elbo}")
     if num_iterations <= 0:
         raise ValueError("batched_advi requires num_iterations > 0")
     if num_particles <= 0:
         raise ValueError("batched_advi requires num_particles > 0")
     if num_particles % 128 != 0:
         raise ValueError("DimensionMismatch: the device batched_advi needs num_particles a multiple of 128")
+    group = (int(iwae_samples) if iwae_samples > 0 else int(num_particles)) if elbo == "iwae" else 1
+    if elbo == "iwae":
+        if not 1 <= group <= num_particles:
+            raise ValueError("batched_advi requires 1 <= iwae_samples <= num_particles")
+        if num_particles % group != 0:
+            raise ValueError("batched_advi requires iwae_samples to divide num_particles evenly")
     if not learning_rate > 0:
         raise ValueError("batched_advi requires learning_rate > 0")
     if not 0 <= beta1 < 1:
@@ -120,7 +129,7 @@ def batched_advi(model: TeaModel, num_iterations: int, num_particles: int = 128,
     flow.set_base(location, log_scale)
     out = flow.fit_elbo(model, num_particles=num_particles, num_iterations=num_iterations,
                         learning_rate=learning_rate, gradient_clip=gradient_clip, seed=seed, beta1=beta1,
-                        beta2=beta2, adam_epsilon=adam_epsilon)
+                        beta2=beta2, adam_epsilon=adam_epsilon, iwae_group=group)
     w = flow.weights
     best = out["best_weights"]
     best_flow = AffineCouplingFlow(P, layers, flow_hidden, seed=seed)
@@ -129,7 +138,8 @@ def batched_advi(model: TeaModel, num_iterations: int, num_particles: int = 128,
                       best_location=best[:P].astype(np.float64), best_log_scale=best[P:2 * P].astype(np.float64),
                       elbo_history=out["elbo_history"], gradient_norm_history=out["grad_norm_history"],
                       best_elbo=out["best_elbo"], num_particles=int(num_particles),
-                      learning_rate=float(learning_rate), guide=guide, flow=flow, best_flow=best_flow)
+                      learning_rate=float(learning_rate), guide=guide, flow=flow, best_flow=best_flow, elbo=elbo,
+                      iwae_samples=group, standard_elbo_history=out["standard_elbo_history"])
 
 
 def variational_samples(result: ADVIResult, num_samples: int, space: str = "constrained", use_best: bool = True,
