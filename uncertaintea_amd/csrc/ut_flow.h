@@ -52,6 +52,9 @@ struct FlowScratch {
   float* logdet = nullptr;    // B
   float* colsum = nullptr;    // column-sum partials (64 chunks x widest row)
   float* wpart = nullptr;     // split-K partials of the weight-gradient GEMMs (FLOW_WSPLITS x H x widest)
+  // per layer: the last forward ran the LDS-staged kernel (its h is in that kernel's
+  // private fragment order, readable only by the matching backward kernel)
+  std::vector<char> fwd_fused;
 };
 
 struct Flow {

@@ -124,7 +124,7 @@ __global__ __launch_bounds__(256) void k_coupling_fwd(float* __restrict__ x, int
   if (b >= B) return;
   float sum = 0.0f;
   for (int j = lane; j < na; j += 64) {
-    const float s = c * tanhf((O[b * op + j] + b2[j]) / c);
+    const float s = c * flow_tanh((O[b * op + j] + b2[j]) * (1.0f / c));
     const float t = O[b * op + na + j] + b2[na + j];
     const long long ia = b * P + aidx[j];
     const float xs = x[ia] * expf(s);
@@ -148,7 +148,7 @@ __global__ __launch_bounds__(256) void k_coupling_inv(float* __restrict__ x, int
   if (b >= B) return;
   float sum = 0.0f;
   for (int j = lane; j < na; j += 64) {
-    const float s = c * tanhf((O[b * op + j] + b2[j]) / c);
+    const float s = c * flow_tanh((O[b * op + j] + b2[j]) * (1.0f / c));
     const float t = O[b * op + na + j] + b2[na + j];
     const long long ia = b * P + aidx[j];
     const float xo = (x[ia] - t) * expf(-s);
@@ -178,7 +178,7 @@ __global__ __launch_bounds__(256) void k_coupling_fwd_bwd(float* __restrict__ g,
     const float ga = g[ia];
     const float s = s_in[b * na + j];
     const float gs = ga * xs_in[b * na + j] + wl;
-    const float r = s / c;
+    const float r = s * (1.0f / c);
     const float gos = gs * (1.0f - r * r);
     gO[b * op + j] = f2bf(gos);
     gO[b * op + na + j] = f2bf(ga);
@@ -210,7 +210,7 @@ __global__ __launch_bounds__(256) void k_coupling_inv_bwd(float* __restrict__ g,
     const float e = expf(-s);
     const float gt = -go * e;
     const float gs = -go * xo_in[b * na + j] + 1.0f;
-    const float r = s / c;
+    const float r = s * (1.0f / c);
     const float gos = gs * (1.0f - r * r);
     gO[b * op + j] = f2bf(gos);
     gO[b * op + na + j] = f2bf(gt);
@@ -271,6 +271,15 @@ __device__ inline void lds_gemm(const bf16_t* As, int lda, const bf16_t* Bs, int
 // latency to the layer.  The conditioner output stays in LDS (over the first matrix,
 // dead once the first GEMM is done).
 constexpr int NSTAGE = LH * LW / 8 / 256;  // 16-byte weight chunks per thread and matrix
+
+// The fused pair keeps the tanh activations h (LR x LH per workgroup) in the first
+// GEMM's accumulator order, private to k_layer_fwd / k_layer_bwd: element r of
+// tile t of lane `lane` in wavefront `wave` of block `blk` sits at
+// ((((blk * 4 + wave) * 2 + t) * 4 + r / 4) * 64 + lane) * 4 + r % 4, so each thread
+// moves its 32 values as eight coalesced 16-byte vectors.
+__device__ inline long long hfrag(int blk, int wave, int t, int rq, int lane) {
+  return ((((long long)blk * 4 + wave) * 2 + t) * 4 + rq) * 256 + lane * 4;
+}
 constexpr int OSP = LW + 4;                // fp32 row stride of the conditioner output in LDS
 
 __device__ inline void load_weights(const bf16_t* __restrict__ s1, const bf16_t* __restrict__ s2, uint4 (&v1)[NSTAGE],
@@ -348,11 +357,18 @@ __global__ __launch_bounds__(256) void k_layer_fwd(float* __restrict__ x, int P,
 #pragma unroll
   for (int t = 0; t < 2; ++t)
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
-      const int row = mfma32_row(lane, r), col = 32 * wave + 128 * t + l32;
-      const float v = tanhf(acc[t][r] + b1v[t]);
-      h_out[(r0 + row) * LH + col] = v;
-      R[row * LHP + col] = f2bf(v);
+    for (int rq = 0; rq < 4; ++rq) {
+      float4 hv;
+      float* hp = (float*)&hv;
+#pragma unroll
+      for (int r3 = 0; r3 < 4; ++r3) {
+        const int r = 4 * rq + r3;
+        const int row = mfma32_row(lane, r), col = 32 * wave + 128 * t + l32;
+        const float v = flow_tanh(acc[t][r] + b1v[t]);
+        hp[r3] = v;
+        R[row * LHP + col] = f2bf(v);
+      }
+      *(float4*)(h_out + hfrag(blockIdx.x, wave, t, rq, lane)) = hv;
     }
   __syncthreads();
   f32x16 acc2[1];
@@ -361,24 +377,30 @@ __global__ __launch_bounds__(256) void k_layer_fwd(float* __restrict__ x, int P,
   for (int r = 0; r < 16; ++r) Os[mfma32_row(lane, r) * OSP + 32 * wave + l32] = acc2[0][r];
   __syncthreads();
   // coupling, as k_coupling_fwd (na <= 64: lane j owns active coordinate j), eight rows
-  // per wavefront
+  // per wavefront; the rows' log-determinant sums are reduced together (each in the
+  // butterfly order of wave_sum_f) so their shuffle latencies overlap
+  float sum[RW];
 #pragma unroll
   for (int rr = 0; rr < RW; ++rr) {
     const int row = wave * RW + rr;
     const long long b = r0 + row;
-    float sum = 0.0f;
+    sum[rr] = 0.0f;
     if (act) {
-      const float sv = c * tanhf((Os[row * OSP + lane] + b2s) / c);
+      const float sv = c * flow_tanh((Os[row * OSP + lane] + b2s) * (1.0f / c));
       const float tv = Os[row * OSP + na + lane] + b2t;
       const float xs = xa[rr] * expf(sv);
       x[b * P + ak] = xs + tv;
       s_out[b * na + lane] = sv;
       xs_out[b * na + lane] = xs;
-      sum += sv;
+      sum[rr] += sv;
     }
-    sum = wave_sum_f(sum);
-    if (lane == 0) logdet[b] = ldv[rr] + sum;
   }
+  for (int m = 32; m >= 1; m >>= 1)
+#pragma unroll
+    for (int rr = 0; rr < RW; ++rr) sum[rr] += __shfl_xor(sum[rr], m);
+  if (lane == 0)
+#pragma unroll
+    for (int rr = 0; rr < RW; ++rr) logdet[r0 + wave * RW + rr] = ldv[rr] + sum[rr];
 }
 
 __global__ __launch_bounds__(256) void k_layer_bwd(float* __restrict__ g, int P, const float* __restrict__ s_in,
@@ -409,14 +431,14 @@ __global__ __launch_bounds__(256) void k_layer_bwd(float* __restrict__ g, int P,
     sv[rr] = act ? s_in[b * na + lane] : 0.0f;
     xv[rr] = act ? xs_in[b * na + lane] : 0.0f;
   }
-  float hv[2][16], gp[16];
+  float4 hq[2][4];
+  float gp[16];
 #pragma unroll
-  for (int r = 0; r < 16; ++r) {
-    const int row = mfma32_row(lane, r);
+  for (int t = 0; t < 2; ++t)
 #pragma unroll
-    for (int t = 0; t < 2; ++t) hv[t][r] = h_in[(r0 + row) * LH + 32 * wave + 128 * t + l32];
-    gp[r] = pcol < np ? g[(r0 + row) * P + pk] : 0.0f;
-  }
+    for (int rq = 0; rq < 4; ++rq) hq[t][rq] = *(const float4*)(h_in + hfrag(blockIdx.x, wave, t, rq, lane));
+#pragma unroll
+  for (int r = 0; r < 16; ++r) gp[r] = pcol < np ? g[(r0 + mfma32_row(lane, r)) * P + pk] : 0.0f;
   __builtin_amdgcn_sched_barrier(0);
   store_weights(W2Ts, W1Ts, v1, v2);
   // coupling backward, as k_coupling_fwd_bwd (na <= 64: lane j owns active coordinate j),
@@ -427,7 +449,7 @@ __global__ __launch_bounds__(256) void k_layer_bwd(float* __restrict__ g, int P,
     if (act) {
       const float ga = gv[rr];
       const float gs = ga * xv[rr] + 1.0f;
-      const float r = sv[rr] / c;
+      const float r = sv[rr] * (1.0f / c);
       const float gos = gs * (1.0f - r * r);
       R[row * LWP + lane] = f2bf(gos);
       R[row * LWP + na + lane] = f2bf(ga);
@@ -444,7 +466,8 @@ __global__ __launch_bounds__(256) void k_layer_bwd(float* __restrict__ g, int P,
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
       const int row = mfma32_row(lane, r), col = 32 * wave + 128 * t + l32;
-      R[row * LHP + col] = f2bf(acc[t][r] * (1.0f - hv[t][r] * hv[t][r]));
+      const float hv = ((const float*)&hq[t][r / 4])[r % 4];
+      R[row * LHP + col] = f2bf(acc[t][r] * (1.0f - hv * hv));
     }
   __syncthreads();
   f32x16 acc2[1];
@@ -782,6 +805,7 @@ hipError_t flow_scratch(Flow& f, int B) {
   s.hb.assign(f.L, nullptr);
   s.s.assign(f.L, nullptr);
   s.xa.assign(f.L, nullptr);
+  s.fwd_fused.assign(f.L, 0);
   for (int l = 0; l < f.L && e == hipSuccess; ++l) {
     const FlowLayer& ly = f.layers[l];
     e = own(f, &s.xp[l], (size_t)B * round_up(ly.np, FLOW_PAD));
@@ -879,7 +903,8 @@ static hipError_t forward_impl(Flow& f, const float* z, float* q, float* logdet,
   hipLaunchKernelGGL(k_base_fwd, dim3(nblk(n)), dim3(T256), 0, st, z, f.params, f.P, s.x, n);
   for (int l = 0; l < f.L; ++l) {
     FlowLayer& ly = f.layers[l];
-    if (allow_fused && fused_layer(f, l)) {
+    s.fwd_fused[l] = allow_fused && fused_layer(f, l);
+    if (s.fwd_fused[l]) {
       hipLaunchKernelGGL(k_layer_fwd, dim3(B / LR), dim3(T256), LAYER_LDS, st, s.x, f.P, l & 1, ly.np, ly.W1b,
                          f.params + ly.ob1, ly.W2b, f.params + ly.ob2, ly.na, f.clamp, s.h[l], s.s[l], s.xa[l],
                          s.logdet);
@@ -921,7 +946,7 @@ hipError_t flow_backward(Flow& f, const float* g_q, float* g_z, int B, hipStream
   for (int l = f.L - 1; l >= 0; --l) {
     FlowLayer& ly = f.layers[l];
     const int op = round_up(2 * ly.na, FLOW_PAD);
-    if (fused_layer(f, l)) {
+    if (s.fwd_fused[l]) {
       hipLaunchKernelGGL(k_layer_bwd, dim3(B / LR), dim3(T256), LAYER_LDS, st, s.g, f.P, s.s[l], s.xa[l], l & 1,
                          ly.na, f.clamp, ly.W2Tb, s.h[l], ly.W1Tb, ly.np);
       continue;

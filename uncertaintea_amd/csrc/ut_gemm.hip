@@ -29,7 +29,7 @@ struct EpiStore {
 struct EpiBiasTanh {
   GemmEpiBiasTanh e;
   __device__ void operator()(long long m, long long n, int, float v) const {
-    const float t = tanhf(v + e.bias[n]);
+    const float t = flow_tanh(v + e.bias[n]);
     e.h[m * e.ld + n] = t;
     e.hb[m * e.ld + n] = f2bf(t);
   }

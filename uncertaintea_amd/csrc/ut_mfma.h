@@ -32,6 +32,27 @@ __host__ __device__ inline float bf2f(bf16_t h) {
   return __builtin_bit_cast(float, (uint32_t)h << 16);
 }
 
+// tanh for the flow's conditioners and scale clamps (~17 instructions against ~60 for
+// tanhf; within 2 ulp of tanh in float32 emulation plus the hardware exp2 / rcp):
+// |x| < 0.55: odd Taylor series of tanh through x^15 (the x^17 term is < 5e-8
+// relative there); otherwise 1 - 2 / (1 + e^(2|x|)), which is >= 0.5 so the
+// subtraction loses nothing, with the sign of x restored.
+__device__ inline float flow_tanh(float x) {
+  const float a = fabsf(x);
+  const float u = x * x;
+  float p = -0.0014558344f;     // -929569 / 638512875
+  p = fmaf(p, u, 0.003592128f);  // 21844 / 6081075
+  p = fmaf(p, u, -0.008863236f); // -1382 / 155925
+  p = fmaf(p, u, 0.021869488f);  // 62 / 2835
+  p = fmaf(p, u, -0.053968254f); // -17 / 315
+  p = fmaf(p, u, 0.13333334f);   // 2 / 15
+  p = fmaf(p, u, -0.33333334f);  // -1 / 3
+  const float small = fmaf(x * u, p, x);
+  const float e = __builtin_amdgcn_exp2f(a * 2.8853900817779268f);  // e^(2a) = 2^(2a log2(e))
+  const float big = fmaf(-2.0f, __builtin_amdgcn_rcpf(1.0f + e), 1.0f);
+  return a < 0.55f ? small : copysignf(big, x);
+}
+
 // Row index in a 32x32 MFMA result held by lane `lane`, register `r`.
 __device__ inline int mfma32_row(int lane, int r) { return 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3); }
 
