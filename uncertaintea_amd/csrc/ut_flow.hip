@@ -282,8 +282,8 @@ __device__ inline long long hfrag(int blk, int wave, int t, int rq, int lane) {
 }
 constexpr int OSP = LW + 4;                // fp32 row stride of the conditioner output in LDS
 
-__device__ inline void load_weights(const bf16_t* __restrict__ s1, const bf16_t* __restrict__ s2, uint4 (&v1)[NSTAGE],
-                                    uint4 (&v2)[NSTAGE]) {
+__device__ inline void load_weights(const bf16_t* __restrict__ s1, const bf16_t* __restrict__ s2,
+                                    uint4 (&v1)[NSTAGE], uint4 (&v2)[NSTAGE]) {
 #pragma unroll
   for (int i = 0; i < NSTAGE; ++i) {
     const int e = threadIdx.x + 256 * i;
@@ -304,10 +304,13 @@ __device__ inline void store_weights(bf16_t* d1, bf16_t* d2, const uint4 (&v1)[N
 
 // `par` is the layer's parity: its passive coordinates are 2k + par and its active ones
 // 2j + 1 - par (the alternating masks), so no index list is loaded before the gathers
-__global__ __launch_bounds__(256) void k_layer_fwd(float* __restrict__ x, int P, int par, int np,
+// one workgroup per CU (the weights fill the LDS), so one wavefront per SIMD: the
+// whole register file is available to keep every staged load in flight.  With the
+// default occupancy target the compiler sinks each weight load to its LDS store to
+// save registers, and the 32 loads then run one round trip at a time.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_layer_fwd(float* __restrict__ x, int P, int par, int np,
                                                    const bf16_t* __restrict__ W1b, const float* __restrict__ b1,
-                                                   const bf16_t* __restrict__ W2b, const float* __restrict__ b2,
-                                                   int na, float c,
+                                                   const bf16_t* __restrict__ W2b, const float* __restrict__ b2, int na, float c,
                                                    float* __restrict__ h_out, float* __restrict__ s_out,
                                                    float* __restrict__ xs_out, float* __restrict__ logdet) {
   extern __shared__ __attribute__((aligned(16))) bf16_t lsh[];
@@ -403,7 +406,7 @@ __global__ __launch_bounds__(256) void k_layer_fwd(float* __restrict__ x, int P,
     for (int rr = 0; rr < RW; ++rr) logdet[r0 + wave * RW + rr] = ldv[rr] + sum[rr];
 }
 
-__global__ __launch_bounds__(256) void k_layer_bwd(float* __restrict__ g, int P, const float* __restrict__ s_in,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_layer_bwd(float* __restrict__ g, int P, const float* __restrict__ s_in,
                                                    const float* __restrict__ xs_in, int par, int na, float c,
                                                    const bf16_t* __restrict__ W2Tb, const float* __restrict__ h_in,
                                                    const bf16_t* __restrict__ W1Tb, int np) {
