@@ -195,11 +195,13 @@ def test_hmc_shared_bitexact_against_twin():
         assert np.array_equal(getattr(dev, k_dev), orc[k_orc]), k_dev
 
 
-@pytest.mark.parametrize("P", [101, 2, 300])
+@pytest.mark.parametrize("P", [101, 2, 300, 129])
 def test_fused_layers_match_unfused(monkeypatch, P):
     """The LDS-staged coupling-layer kernels of the sampling path (passive and output
     widths of 128) against the per-step kernels: forward values, log-determinants and
-    the sampler's VJP bit for bit; P = 300 exercises the per-step fallback."""
+    the sampler's VJP bit for bit.  P = 101 and 2 run every layer in one launch per
+    direction; P = 129 mixes per-layer LDS-staged kernels (even layers) with per-step
+    ones (odd layers, 2 na = 130); P = 300 exercises the per-step fallback."""
     say("fused vs unfused flow layers", P)
     flows = {}
     for mode in ("0", "1"):

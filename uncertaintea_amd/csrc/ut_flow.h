@@ -31,6 +31,27 @@ struct FlowLayer {
   uint16_t* W2Tb = nullptr;  // H x 128
 };
 
+// Device descriptors of the layers served by the LDS-staged kernels (rebuilt whenever
+// the scratch buffers move): weights, biases and the activations each direction uses.
+struct FusedFwdLayer {
+  const uint16_t* W1b;
+  const float* b1;
+  const uint16_t* W2b;
+  const float* b2;
+  float* h;   // the layer's tanh activations, in the fused kernels' fragment order
+  float* s;   // B x na
+  float* xs;  // B x na
+  int np, na, par;
+};
+struct FusedBwdLayer {
+  const uint16_t* W2Tb;
+  const uint16_t* W1Tb;
+  const float* h;
+  const float* s;
+  const float* xs;
+  int np, na, par;
+};
+
 // Per-batch activations kept for the backward pass.
 struct FlowScratch {
   int B = 0;
@@ -71,6 +92,8 @@ struct Flow {
   std::vector<void*> owned;
   hipStream_t stream = nullptr;
   unsigned gen = 0;         // bumped whenever the scratch buffers move (captured graphs go stale)
+  FusedFwdLayer* dfwd = nullptr;  // device, L descriptors
+  FusedBwdLayer* dbwd = nullptr;
   bool fused = true;        // LDS-staged layer kernels on the sampling path where the widths allow
 };
 

@@ -282,38 +282,50 @@ __device__ inline long long hfrag(int blk, int wave, int t, int rq, int lane) {
 }
 constexpr int OSP = LW + 4;                // fp32 row stride of the conditioner output in LDS
 
+// staged weights travel as native 4 x 32-bit vectors (HIP's uint4 is a struct, whose
+// copies through a plain pointer keep the staging arrays in scratch memory)
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// Pointers read from the layer descriptors are generic; accesses through them would
+// compile to flat instructions, which also count against the LDS wait counter (every
+// LDS wait would then wait for the next layer's weights in flight).  They are global
+// memory: say so.
+#define UT_GLOBAL __attribute__((address_space(1)))
+template <class T>
+__device__ inline UT_GLOBAL T* glb(T* p) {
+  return (UT_GLOBAL T*)p;
+}
+
 __device__ inline void load_weights(const bf16_t* __restrict__ s1, const bf16_t* __restrict__ s2,
-                                    uint4 (&v1)[NSTAGE], uint4 (&v2)[NSTAGE]) {
+                                    u32x4 (&v1)[NSTAGE], u32x4 (&v2)[NSTAGE]) {
 #pragma unroll
   for (int i = 0; i < NSTAGE; ++i) {
     const int e = threadIdx.x + 256 * i;
-    v1[i] = *(const uint4*)(s1 + (long long)(e / (LW / 8)) * LW + (e % (LW / 8)) * 8);
-    v2[i] = *(const uint4*)(s2 + (long long)(e / (LH / 8)) * LH + (e % (LH / 8)) * 8);
+    v1[i] = *glb((const u32x4*)(s1 + (long long)(e / (LW / 8)) * LW + (e % (LW / 8)) * 8));
+    v2[i] = *glb((const u32x4*)(s2 + (long long)(e / (LH / 8)) * LH + (e % (LH / 8)) * 8));
   }
 }
 
 // first matrix (LH x LW) -> LDS row stride LWP, second (LW x LH) -> LHP
-__device__ inline void store_weights(bf16_t* d1, bf16_t* d2, const uint4 (&v1)[NSTAGE], const uint4 (&v2)[NSTAGE]) {
+__device__ inline void store_weights(bf16_t* d1, bf16_t* d2, const u32x4 (&v1)[NSTAGE], const u32x4 (&v2)[NSTAGE]) {
 #pragma unroll
   for (int i = 0; i < NSTAGE; ++i) {
     const int e = threadIdx.x + 256 * i;
-    *(uint4*)(d1 + (e / (LW / 8)) * LWP + (e % (LW / 8)) * 8) = v1[i];
-    *(uint4*)(d2 + (e / (LH / 8)) * LHP + (e % (LH / 8)) * 8) = v2[i];
+    *(u32x4*)(d1 + (e / (LW / 8)) * LWP + (e % (LW / 8)) * 8) = v1[i];
+    *(u32x4*)(d2 + (e / (LH / 8)) * LHP + (e % (LH / 8)) * 8) = v2[i];
   }
 }
 
 // `par` is the layer's parity: its passive coordinates are 2k + par and its active ones
 // 2j + 1 - par (the alternating masks), so no index list is loaded before the gathers
-// one workgroup per CU (the weights fill the LDS), so one wavefront per SIMD: the
-// whole register file is available to keep every staged load in flight.  With the
-// default occupancy target the compiler sinks each weight load to its LDS store to
-// save registers, and the 32 loads then run one round trip at a time.
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_layer_fwd(float* __restrict__ x, int P, int par, int np,
-                                                   const bf16_t* __restrict__ W1b, const float* __restrict__ b1,
-                                                   const bf16_t* __restrict__ W2b, const float* __restrict__ b2, int na, float c,
-                                                   float* __restrict__ h_out, float* __restrict__ s_out,
-                                                   float* __restrict__ xs_out, float* __restrict__ logdet) {
-  extern __shared__ __attribute__((aligned(16))) bf16_t lsh[];
+// One layer of the fused forward for this workgroup's LR rows.  v1 / v2 hold the
+// layer's weights (loaded, or still in flight); with `next` the next layer's weight
+// loads are issued into them as soon as this layer's weights are in LDS, so they fly
+// while this layer computes.
+__device__ __forceinline__ void layer_fwd(float* __restrict__ x, int P, const FusedFwdLayer& ly,
+                                          const FusedFwdLayer* next, float c, float* __restrict__ logdet,
+                                          u32x4 (&v1)[NSTAGE], u32x4 (&v2)[NSTAGE], bf16_t* lsh) {
   bf16_t* W1s = lsh;                  // LH x LWP: W1 [H x kp]
   bf16_t* W2s = lsh + LH * LWP;       // LW x LHP: W2 [op x H]
   bf16_t* R = lsh + 2 * LH * LWP;     // LR x LHP: passive inputs (stride LWP), then h
@@ -322,13 +334,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, l32 = lane & 31;
   constexpr int G = LR * LW / 256;    // passive inputs gathered per thread
   constexpr int RW = LR / 4;          // coupling rows per wavefront
+  const int np = ly.np, na = ly.na, par = ly.par;
+  const UT_GLOBAL float* b1 = glb(ly.b1);
+  const UT_GLOBAL float* b2 = glb(ly.b2);
+  UT_GLOBAL float* hout = glb(ly.h);
+  UT_GLOBAL float* sout = glb(ly.s);
+  UT_GLOBAL float* xsout = glb(ly.xs);
   const bool act = lane < na;
-  // ---- every global load up front (indices first: the gathers depend on them)
+  // ---- every row load of the layer up front; the passive and active coordinates
+  // are 2k + par and 2j + 1 - par (the alternating masks)
   const int kq = tid & (LW - 1);      // the passive column this thread gathers (fixed: 256 = 2 LW)
   const int pk = 2 * kq + par;
   const int ak = 2 * lane + 1 - par;
-  uint4 v1[NSTAGE], v2[NSTAGE];
-  load_weights(W1b, W2b, v1, v2);
   float xv[G];
 #pragma unroll
   for (int i = 0; i < G; ++i) {
@@ -348,6 +365,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
   }
   __builtin_amdgcn_sched_barrier(0);
   store_weights(W1s, W2s, v1, v2);
+  if (next) load_weights(next->W1b, next->W2b, v1, v2);
 #pragma unroll
   for (int i = 0; i < G; ++i) {
     const int r = (tid + 256 * i) / LW;
@@ -361,17 +379,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
   for (int t = 0; t < 2; ++t)
 #pragma unroll
     for (int rq = 0; rq < 4; ++rq) {
-      float4 hv;
-      float* hp = (float*)&hv;
+      f32x4 hv;
 #pragma unroll
       for (int r3 = 0; r3 < 4; ++r3) {
         const int r = 4 * rq + r3;
         const int row = mfma32_row(lane, r), col = 32 * wave + 128 * t + l32;
         const float v = flow_tanh(acc[t][r] + b1v[t]);
-        hp[r3] = v;
+        hv[r3] = v;
         R[row * LHP + col] = f2bf(v);
       }
-      *(float4*)(h_out + hfrag(blockIdx.x, wave, t, rq, lane)) = hv;
+      *(UT_GLOBAL f32x4*)(hout + hfrag(blockIdx.x, wave, t, rq, lane)) = hv;
     }
   __syncthreads();
   f32x16 acc2[1];
@@ -393,8 +410,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
       const float tv = Os[row * OSP + na + lane] + b2t;
       const float xs = xa[rr] * expf(sv);
       x[b * P + ak] = xs + tv;
-      s_out[b * na + lane] = sv;
-      xs_out[b * na + lane] = xs;
+      sout[b * na + lane] = sv;
+      xsout[b * na + lane] = xs;
       sum[rr] += sv;
     }
   }
@@ -406,44 +423,47 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
     for (int rr = 0; rr < RW; ++rr) logdet[r0 + wave * RW + rr] = ldv[rr] + sum[rr];
 }
 
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_layer_bwd(float* __restrict__ g, int P, const float* __restrict__ s_in,
-                                                   const float* __restrict__ xs_in, int par, int na, float c,
-                                                   const bf16_t* __restrict__ W2Tb, const float* __restrict__ h_in,
-                                                   const bf16_t* __restrict__ W1Tb, int np) {
-  extern __shared__ __attribute__((aligned(16))) bf16_t lsh[];
+// One layer of the fused backward (the coupling's backward, dO W2 with the tanh
+// backward, dU W1 added into the passive gradients); weights as in layer_fwd.
+__device__ __forceinline__ void layer_bwd(float* __restrict__ g, int P, const FusedBwdLayer& ly,
+                                          const FusedBwdLayer* next, float c, u32x4 (&v1)[NSTAGE],
+                                          u32x4 (&v2)[NSTAGE], bf16_t* lsh) {
   bf16_t* W2Ts = lsh;                 // LH x LWP: W2^T [H x op]
   bf16_t* W1Ts = lsh + LH * LWP;      // LW x LHP: W1^T [kp x H]
   bf16_t* R = lsh + 2 * LH * LWP;     // LR x LHP: dO (stride LWP), then dU
   const long long r0 = (long long)blockIdx.x * LR;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, l32 = lane & 31;
   constexpr int RW = LR / 4;
+  const int np = ly.np, na = ly.na, par = ly.par;
   const bool act = lane < na;
   const int pcol = 32 * wave + l32;   // the passive output column of the second GEMM
-  // ---- every global load up front: indices, weights, the coupling backward's operands,
-  // the activations of the tanh backward and the passive gradients the result adds to
-  // (the coupling writes only active coordinates)
+  // ---- every row load up front: the coupling backward's operands, the activations
+  // of the tanh backward and the passive gradients the result adds to (the coupling
+  // writes only active coordinates)
   const int ak = 2 * lane + 1 - par;
   const int pk = 2 * pcol + par;
-  uint4 v1[NSTAGE], v2[NSTAGE];
-  load_weights(W2Tb, W1Tb, v1, v2);
+  const UT_GLOBAL float* sin_ = glb(ly.s);
+  const UT_GLOBAL float* xsin = glb(ly.xs);
+  const UT_GLOBAL float* hin = glb(ly.h);
   float gv[RW], sv[RW], xv[RW];
 #pragma unroll
   for (int rr = 0; rr < RW; ++rr) {
     const long long b = r0 + wave * RW + rr;
     gv[rr] = act ? g[b * P + ak] : 0.0f;
-    sv[rr] = act ? s_in[b * na + lane] : 0.0f;
-    xv[rr] = act ? xs_in[b * na + lane] : 0.0f;
+    sv[rr] = act ? sin_[b * na + lane] : 0.0f;
+    xv[rr] = act ? xsin[b * na + lane] : 0.0f;
   }
-  float4 hq[2][4];
+  f32x4 hq[2][4];
   float gp[16];
 #pragma unroll
   for (int t = 0; t < 2; ++t)
 #pragma unroll
-    for (int rq = 0; rq < 4; ++rq) hq[t][rq] = *(const float4*)(h_in + hfrag(blockIdx.x, wave, t, rq, lane));
+    for (int rq = 0; rq < 4; ++rq) hq[t][rq] = *(const UT_GLOBAL f32x4*)(hin + hfrag(blockIdx.x, wave, t, rq, lane));
 #pragma unroll
   for (int r = 0; r < 16; ++r) gp[r] = pcol < np ? g[(r0 + mfma32_row(lane, r)) * P + pk] : 0.0f;
   __builtin_amdgcn_sched_barrier(0);
   store_weights(W2Ts, W1Ts, v1, v2);
+  if (next) load_weights(next->W2Tb, next->W1Tb, v1, v2);
   // coupling backward, as k_coupling_fwd_bwd (na <= 64: lane j owns active coordinate j),
   // into LDS, eight rows per wavefront
 #pragma unroll
@@ -469,7 +489,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
       const int row = mfma32_row(lane, r), col = 32 * wave + 128 * t + l32;
-      const float hv = ((const float*)&hq[t][r / 4])[r % 4];
+      const float hv = hq[t][r / 4][r % 4];
       R[row * LHP + col] = f2bf(acc[t][r] * (1.0f - hv * hv));
     }
   __syncthreads();
@@ -478,6 +498,56 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 #pragma unroll
   for (int r = 0; r < 16; ++r)
     if (pcol < np) g[(r0 + mfma32_row(lane, r)) * P + pk] = gp[r] + acc2[0][r];
+}
+
+// The kernels run one workgroup per CU (the weights fill the LDS), so one wavefront
+// per SIMD: the whole register file is available to keep every staged load in flight.
+// With the default occupancy target the compiler sinks each weight load to its LDS
+// store to save registers, and the 32 loads then run one round trip at a time.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_layer_fwd(
+    float* __restrict__ x, int P, const FusedFwdLayer* __restrict__ layers, int l, float c,
+    float* __restrict__ logdet) {
+  extern __shared__ __attribute__((aligned(16))) bf16_t lsh[];
+  const FusedFwdLayer ly = layers[l];
+  u32x4 v1[NSTAGE], v2[NSTAGE];
+  load_weights(ly.W1b, ly.W2b, v1, v2);
+  layer_fwd(x, P, ly, nullptr, c, logdet, v1, v2, lsh);
+}
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_layer_bwd(
+    float* __restrict__ g, int P, const FusedBwdLayer* __restrict__ layers, int l, float c) {
+  extern __shared__ __attribute__((aligned(16))) bf16_t lsh[];
+  const FusedBwdLayer ly = layers[l];
+  u32x4 v1[NSTAGE], v2[NSTAGE];
+  load_weights(ly.W2Tb, ly.W1Tb, v1, v2);
+  layer_bwd(g, P, ly, nullptr, c, v1, v2, lsh);
+}
+
+// All L layers in one launch: a workgroup carries its rows through every layer (rows
+// are independent), with the next layer's weights in flight during each layer.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_flow_fwd_all(
+    float* __restrict__ x, int P, const FusedFwdLayer* __restrict__ layers, int L, float c,
+    float* __restrict__ logdet) {
+  extern __shared__ __attribute__((aligned(16))) bf16_t lsh[];
+  u32x4 v1[NSTAGE], v2[NSTAGE];
+  load_weights(layers[0].W1b, layers[0].W2b, v1, v2);
+  for (int l = 0; l < L; ++l) {
+    const FusedFwdLayer ly = layers[l];
+    layer_fwd(x, P, ly, l + 1 < L ? &layers[l + 1] : nullptr, c, logdet, v1, v2, lsh);
+    __syncthreads();  // the layer's rows and log-determinants are written; the LDS is free
+  }
+}
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_flow_bwd_all(
+    float* __restrict__ g, int P, const FusedBwdLayer* __restrict__ layers, int L, float c) {
+  extern __shared__ __attribute__((aligned(16))) bf16_t lsh[];
+  u32x4 v1[NSTAGE], v2[NSTAGE];
+  load_weights(layers[L - 1].W2Tb, layers[L - 1].W1Tb, v1, v2);
+  for (int l = L - 1; l >= 0; --l) {
+    const FusedBwdLayer ly = layers[l];
+    layer_bwd(g, P, ly, l > 0 ? &layers[l - 1] : nullptr, c, v1, v2, lsh);
+    __syncthreads();  // the layer's gradients are written; the LDS is free
+  }
 }
 
 // column sums in two passes (fixed order): COLSUM_CHUNKS row chunks, then chunks
@@ -739,6 +809,10 @@ hipError_t flow_alloc(Flow& f, int P, int L, int H) {
     if (!lds_set) {
       (void)hipFuncSetAttribute((const void*)k_layer_fwd, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LAYER_LDS);
       (void)hipFuncSetAttribute((const void*)k_layer_bwd, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LAYER_LDS);
+      (void)hipFuncSetAttribute((const void*)k_flow_fwd_all, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)LAYER_LDS);
+      (void)hipFuncSetAttribute((const void*)k_flow_bwd_all, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)LAYER_LDS);
       lds_set = true;
     }
   }
@@ -762,6 +836,8 @@ hipError_t flow_alloc(Flow& f, int P, int L, int H) {
   }
   f.nparams = off;
   hipError_t e = own(f, &f.params, (size_t)off);
+  if (e == hipSuccess && L > 0) e = own(f, &f.dfwd, (size_t)L);
+  if (e == hipSuccess && L > 0) e = own(f, &f.dbwd, (size_t)L);
   if (e == hipSuccess) e = own(f, &f.grads, (size_t)off);
   if (e == hipSuccess) e = own(f, &f.adam_m, (size_t)off);
   if (e == hipSuccess) e = own(f, &f.adam_v, (size_t)off);
@@ -831,6 +907,17 @@ hipError_t flow_scratch(Flow& f, int B) {
   if (e == hipSuccess) e = own(f, &s.logdet, (size_t)B);
   if (e == hipSuccess) e = own(f, &s.colsum, (size_t)COLSUM_CHUNKS * std::max<size_t>(wide, (size_t)f.P));
   if (e == hipSuccess) e = own(f, &s.wpart, (size_t)FLOW_WSPLITS * H * wide);
+  if (e == hipSuccess && f.L > 0) {
+    std::vector<FusedFwdLayer> fw(f.L);
+    std::vector<FusedBwdLayer> bw(f.L);
+    for (int l = 0; l < f.L; ++l) {
+      const FlowLayer& ly = f.layers[l];
+      fw[l] = {ly.W1b, f.params + ly.ob1, ly.W2b, f.params + ly.ob2, s.h[l], s.s[l], s.xa[l], ly.np, ly.na, l & 1};
+      bw[l] = {ly.W2Tb, ly.W1Tb, s.h[l], s.s[l], s.xa[l], ly.np, ly.na, l & 1};
+    }
+    e = hipMemcpy(f.dfwd, fw.data(), fw.size() * sizeof(FusedFwdLayer), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(f.dbwd, bw.data(), bw.size() * sizeof(FusedBwdLayer), hipMemcpyHostToDevice);
+  }
   return e;
 }
 
@@ -904,20 +991,28 @@ static hipError_t forward_impl(Flow& f, const float* z, float* q, float* logdet,
   const long long n = (long long)B * f.P;
   hipLaunchKernelGGL(k_logdet_init, dim3(nblk(B)), dim3(T256), 0, st, f.params + f.P, f.P, 1.0f, s.logdet, B);
   hipLaunchKernelGGL(k_base_fwd, dim3(nblk(n)), dim3(T256), 0, st, z, f.params, f.P, s.x, n);
+  bool all = allow_fused && f.L > 0;
   for (int l = 0; l < f.L; ++l) {
-    FlowLayer& ly = f.layers[l];
     s.fwd_fused[l] = allow_fused && fused_layer(f, l);
-    if (s.fwd_fused[l]) {
-      hipLaunchKernelGGL(k_layer_fwd, dim3(B / LR), dim3(T256), LAYER_LDS, st, s.x, f.P, l & 1, ly.np, ly.W1b,
-                         f.params + ly.ob1, ly.W2b, f.params + ly.ob2, ly.na, f.clamp, s.h[l], s.s[l], s.xa[l],
-                         s.logdet);
-      continue;
+    all = all && s.fwd_fused[l];
+  }
+  if (all) {
+    hipLaunchKernelGGL(k_flow_fwd_all, dim3(B / LR), dim3(T256), LAYER_LDS, st, s.x, f.P, f.dfwd, f.L, f.clamp,
+                       s.logdet);
+  } else {
+    for (int l = 0; l < f.L; ++l) {
+      FlowLayer& ly = f.layers[l];
+      if (s.fwd_fused[l]) {
+        hipLaunchKernelGGL(k_layer_fwd, dim3(B / LR), dim3(T256), LAYER_LDS, st, s.x, f.P, f.dfwd, l, f.clamp,
+                           s.logdet);
+        continue;
+      }
+      hipError_t e = conditioner(f, l, B, st);
+      if (e != hipSuccess) return e;
+      hipLaunchKernelGGL(k_coupling_fwd, dim3((B + 3) / 4), dim3(T256), 0, st, s.x, f.P, s.O,
+                         round_up(2 * ly.na, FLOW_PAD), f.params + ly.ob2, ly.aidx, ly.na, f.clamp, s.s[l], s.xa[l],
+                         s.logdet, B);
     }
-    hipError_t e = conditioner(f, l, B, st);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_coupling_fwd, dim3((B + 3) / 4), dim3(T256), 0, st, s.x, f.P, s.O,
-                       round_up(2 * ly.na, FLOW_PAD), f.params + ly.ob2, ly.aidx, ly.na, f.clamp, s.s[l], s.xa[l],
-                       s.logdet, B);
   }
   if (q && q != s.x) {
     hipError_t e = hipMemcpyAsync(q, s.x, n * 4, hipMemcpyDeviceToDevice, st);
@@ -946,18 +1041,23 @@ hipError_t flow_backward(Flow& f, const float* g_q, float* g_z, int B, hipStream
     hipError_t e = hipMemcpyAsync(s.g, g_q, n * 4, hipMemcpyDeviceToDevice, st);
     if (e != hipSuccess) return e;
   }
-  for (int l = f.L - 1; l >= 0; --l) {
-    FlowLayer& ly = f.layers[l];
-    const int op = round_up(2 * ly.na, FLOW_PAD);
-    if (s.fwd_fused[l]) {
-      hipLaunchKernelGGL(k_layer_bwd, dim3(B / LR), dim3(T256), LAYER_LDS, st, s.g, f.P, s.s[l], s.xa[l], l & 1,
-                         ly.na, f.clamp, ly.W2Tb, s.h[l], ly.W1Tb, ly.np);
-      continue;
+  bool all = f.L > 0;
+  for (int l = 0; l < f.L; ++l) all = all && s.fwd_fused[l];
+  if (all) {
+    hipLaunchKernelGGL(k_flow_bwd_all, dim3(B / LR), dim3(T256), LAYER_LDS, st, s.g, f.P, f.dbwd, f.L, f.clamp);
+  } else {
+    for (int l = f.L - 1; l >= 0; --l) {
+      FlowLayer& ly = f.layers[l];
+      const int op = round_up(2 * ly.na, FLOW_PAD);
+      if (s.fwd_fused[l]) {
+        hipLaunchKernelGGL(k_layer_bwd, dim3(B / LR), dim3(T256), LAYER_LDS, st, s.g, f.P, f.dbwd, l, f.clamp);
+        continue;
+      }
+      hipLaunchKernelGGL(k_coupling_fwd_bwd, dim3((B + 3) / 4), dim3(T256), 0, st, s.g, f.P, s.s[l], s.xa[l],
+                         ly.aidx, ly.na, op, f.clamp, s.gO, (float*)nullptr, B, (const float*)nullptr);
+      hipError_t e = conditioner_bwd(f, l, B, false, st);
+      if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL(k_coupling_fwd_bwd, dim3((B + 3) / 4), dim3(T256), 0, st, s.g, f.P, s.s[l], s.xa[l], ly.aidx,
-                       ly.na, op, f.clamp, s.gO, (float*)nullptr, B, (const float*)nullptr);
-    hipError_t e = conditioner_bwd(f, l, B, false, st);
-    if (e != hipSuccess) return e;
   }
   hipLaunchKernelGGL(k_base_fwd_bwd, dim3(nblk(n)), dim3(T256), 0, st, s.g, f.params, f.P, n);
   if (g_z != s.g) {
