@@ -523,14 +523,29 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
   layer_bwd(g, P, ly, nullptr, c, v1, v2, lsh);
 }
 
-// All L layers in one launch: a workgroup carries its rows through every layer (rows
-// are independent), with the next layer's weights in flight during each layer.
+// The whole forward in one launch: the elementwise base (x = mu + e^ls z and the
+// log-determinant start sum ls, in the order of k_base_fwd / k_logdet_init), then all
+// L coupling layers; a workgroup carries its rows through every layer (rows are
+// independent), with the next layer's weights in flight during each layer.
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_flow_fwd_all(
-    float* __restrict__ x, int P, const FusedFwdLayer* __restrict__ layers, int L, float c,
-    float* __restrict__ logdet) {
+    const float* __restrict__ z, const float* __restrict__ prm, float* __restrict__ x, int P,
+    const FusedFwdLayer* __restrict__ layers, int L, float c, float* __restrict__ logdet) {
   extern __shared__ __attribute__((aligned(16))) bf16_t lsh[];
   u32x4 v1[NSTAGE], v2[NSTAGE];
   load_weights(layers[0].W1b, layers[0].W2b, v1, v2);
+  const long long r0 = (long long)blockIdx.x * LR;
+  const int tid = threadIdx.x;
+  for (int e = tid; e < LR * P; e += 256) {
+    const int cidx = e % P;
+    const long long i = r0 * P + e;
+    x[i] = prm[cidx] + expf(prm[P + cidx]) * z[i];
+  }
+  if ((tid & 63) == 0) {  // lane 0 of each wavefront: the log-determinant start of its rows
+    float sl = 0.0f;
+    for (int i = 0; i < P; ++i) sl += prm[P + i];
+    for (int rr = 0; rr < LR / 4; ++rr) logdet[r0 + (tid >> 6) * (LR / 4) + rr] = sl;
+  }
+  __syncthreads();  // the base rows are written
   for (int l = 0; l < L; ++l) {
     const FusedFwdLayer ly = layers[l];
     layer_fwd(x, P, ly, l + 1 < L ? &layers[l + 1] : nullptr, c, logdet, v1, v2, lsh);
@@ -538,8 +553,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
   }
 }
 
+// The whole backward in one launch: all L coupling layers in reverse, then the base's
+// g_z = g e^ls (as k_base_fwd_bwd) written straight to the output.
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_flow_bwd_all(
-    float* __restrict__ g, int P, const FusedBwdLayer* __restrict__ layers, int L, float c) {
+    float* __restrict__ g, const float* __restrict__ prm, float* gz, int P,
+    const FusedBwdLayer* __restrict__ layers, int L, float c) {
   extern __shared__ __attribute__((aligned(16))) bf16_t lsh[];
   u32x4 v1[NSTAGE], v2[NSTAGE];
   load_weights(layers[L - 1].W2Tb, layers[L - 1].W1Tb, v1, v2);
@@ -547,6 +565,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
     const FusedBwdLayer ly = layers[l];
     layer_bwd(g, P, ly, l > 0 ? &layers[l - 1] : nullptr, c, v1, v2, lsh);
     __syncthreads();  // the layer's gradients are written; the LDS is free
+  }
+  const long long r0 = (long long)blockIdx.x * LR;
+  for (int e = threadIdx.x; e < LR * P; e += 256) {
+    const long long i = r0 * P + e;
+    gz[i] = g[i] * expf(prm[P + e % P]);
   }
 }
 
@@ -989,17 +1012,17 @@ static hipError_t forward_impl(Flow& f, const float* z, float* q, float* logdet,
   if (B % FLOW_PAD || B > f.sc.B) return hipErrorInvalidValue;
   FlowScratch& s = f.sc;
   const long long n = (long long)B * f.P;
-  hipLaunchKernelGGL(k_logdet_init, dim3(nblk(B)), dim3(T256), 0, st, f.params + f.P, f.P, 1.0f, s.logdet, B);
-  hipLaunchKernelGGL(k_base_fwd, dim3(nblk(n)), dim3(T256), 0, st, z, f.params, f.P, s.x, n);
   bool all = allow_fused && f.L > 0;
   for (int l = 0; l < f.L; ++l) {
     s.fwd_fused[l] = allow_fused && fused_layer(f, l);
     all = all && s.fwd_fused[l];
   }
   if (all) {
-    hipLaunchKernelGGL(k_flow_fwd_all, dim3(B / LR), dim3(T256), LAYER_LDS, st, s.x, f.P, f.dfwd, f.L, f.clamp,
-                       s.logdet);
+    hipLaunchKernelGGL(k_flow_fwd_all, dim3(B / LR), dim3(T256), LAYER_LDS, st, z, f.params, s.x, f.P, f.dfwd, f.L,
+                       f.clamp, s.logdet);
   } else {
+    hipLaunchKernelGGL(k_logdet_init, dim3(nblk(B)), dim3(T256), 0, st, f.params + f.P, f.P, 1.0f, s.logdet, B);
+    hipLaunchKernelGGL(k_base_fwd, dim3(nblk(n)), dim3(T256), 0, st, z, f.params, f.P, s.x, n);
     for (int l = 0; l < f.L; ++l) {
       FlowLayer& ly = f.layers[l];
       if (s.fwd_fused[l]) {
@@ -1044,7 +1067,10 @@ hipError_t flow_backward(Flow& f, const float* g_q, float* g_z, int B, hipStream
   bool all = f.L > 0;
   for (int l = 0; l < f.L; ++l) all = all && s.fwd_fused[l];
   if (all) {
-    hipLaunchKernelGGL(k_flow_bwd_all, dim3(B / LR), dim3(T256), LAYER_LDS, st, s.g, f.P, f.dbwd, f.L, f.clamp);
+    // base backward fused in, written straight to g_z
+    hipLaunchKernelGGL(k_flow_bwd_all, dim3(B / LR), dim3(T256), LAYER_LDS, st, s.g, f.params, g_z, f.P, f.dbwd,
+                       f.L, f.clamp);
+    return hipGetLastError();
   } else {
     for (int l = f.L - 1; l >= 0; --l) {
       FlowLayer& ly = f.layers[l];
