@@ -535,14 +535,36 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
   load_weights(layers[0].W1b, layers[0].W2b, v1, v2);
   const long long r0 = (long long)blockIdx.x * LR;
   const int tid = threadIdx.x;
-  for (int e = tid; e < LR * P; e += 256) {
-    const int cidx = e % P;
-    const long long i = r0 * P + e;
-    x[i] = prm[cidx] + expf(prm[P + cidx]) * z[i];
+  // loads in batches of four elements per thread (a plain loop would wait on each)
+  for (int e0 = tid; e0 < LR * P; e0 += 4 * 256) {
+    float zv[4], mv[4], lv[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int e = e0 + 256 * k;
+      const bool in = e < LR * P;
+      const int cidx = in ? e % P : 0;
+      zv[k] = in ? z[r0 * P + e] : 0.0f;
+      mv[k] = prm[cidx];
+      lv[k] = prm[P + cidx];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int e = e0 + 256 * k;
+      if (e < LR * P) x[r0 * P + e] = mv[k] + expf(lv[k]) * zv[k];
+    }
   }
   if ((tid & 63) == 0) {  // lane 0 of each wavefront: the log-determinant start of its rows
+    // sequential sum (the order of k_logdet_init), loads eight at a time
     float sl = 0.0f;
-    for (int i = 0; i < P; ++i) sl += prm[P + i];
+    int i = 0;
+    for (; i + 8 <= P; i += 8) {
+      float t[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) t[k] = prm[P + i + k];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) sl += t[k];
+    }
+    for (; i < P; ++i) sl += prm[P + i];
     for (int rr = 0; rr < LR / 4; ++rr) logdet[r0 + (tid >> 6) * (LR / 4) + rr] = sl;
   }
   __syncthreads();  // the base rows are written
@@ -567,9 +589,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
     __syncthreads();  // the layer's gradients are written; the LDS is free
   }
   const long long r0 = (long long)blockIdx.x * LR;
-  for (int e = threadIdx.x; e < LR * P; e += 256) {
-    const long long i = r0 * P + e;
-    gz[i] = g[i] * expf(prm[P + e % P]);
+  for (int e0 = threadIdx.x; e0 < LR * P; e0 += 4 * 256) {
+    float gv[4], lv[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int e = e0 + 256 * k;
+      const bool in = e < LR * P;
+      gv[k] = in ? g[r0 * P + e] : 0.0f;
+      lv[k] = prm[P + (in ? e % P : 0)];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int e = e0 + 256 * k;
+      if (e < LR * P) gz[r0 * P + e] = gv[k] * expf(lv[k]);
+    }
   }
 }
 
