@@ -730,6 +730,16 @@ constexpr int kSampleLaunches = 6;
 // _initial_batched_hmc_positions + the non-finite retry loop,
 // api_batched_nuts.jl:184-200): columns whose logjoint or gradient is not finite
 // are re-drawn with the next attempt key, up to init_max_retries times.
+// UT_PROGRESS=n: the lock-step drivers wait for every n-th iteration to finish and
+// report it on stderr (long runs stay visibly alive); unset or 0: silent.
+int progress_every() {
+  static const int n = [] {
+    const char* e = std::getenv("UT_PROGRESS");
+    return e ? std::max(0, std::atoi(e)) : 0;
+  }();
+  return n;
+}
+
 int init_chains(ut_workspace* ws, const ut_sampler_opts* o, uint32_t rs) {
   const int P = ws->P, C = ws->C;
   const DevModelDesc& m = ws->plan->dev;
@@ -1102,6 +1112,11 @@ int run_lockstep(ut_workspace* ws, const ut_chees_opts* co, ut_chain_out* out, d
   long long evals = 0;
   int launches = 0;
   for (int it = 1; it <= W + S; ++it) {
+    if (progress_every() > 0 && it % progress_every() == 0) {
+      UT_HIP(hipStreamSynchronize(stream));
+      std::fprintf(stderr, "lock-step sampler: iteration %d of %d done\n", it - 1, W + S);
+      std::fflush(stderr);
+    }
     if (it == W + 1) {
       UT_HIP(hipStreamSynchronize(stream));
       t_warm = now_seconds();
@@ -1291,6 +1306,11 @@ int run_meads(ut_workspace* ws, const ut_meads_opts* mo, ut_chain_out* out, doub
       UT_HIP(hipStreamSynchronize(stream));
       t_warm = now_seconds();
       UT_HIP(hipEventRecord(ev0, stream));
+    }
+    if (progress_every() > 0 && it % progress_every() == 0) {
+      UT_HIP(hipStreamSynchronize(stream));
+      std::fprintf(stderr, "ut_meads_run: iteration %d of %d done\n", it - 1, T);
+      std::fflush(stderr);
     }
     const int skip = (it - 1) % K;
     UT_HIP(launch_meads_fold_params(m, P, K, nmax, it, stream));
