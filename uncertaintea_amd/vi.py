@@ -18,7 +18,6 @@ full-rank and low-rank guides.
 """
 from __future__ import annotations
 
-import ctypes as ct
 from dataclasses import dataclass, field
 
 import numpy as np
