@@ -355,9 +355,15 @@ def _funnel_report(args, D, model, flow, plain, plain_s, fres, flow_s, rounds, t
     for npass, nact, _ in flow.layout():
         flops_grad += 2 * 2.0 * (npass * H + H * 2 * nact)
     achieved = fres.sampling_gradient_evals * flops_grad / fres.sampling_kernel_seconds / 1e12
-    roofline = {"bound": "mfma", "compute": "bf16 MFMA conditioner GEMMs (launch-bound at 4096 rows)",
+    traffic = None
+    pmc = ROOT / "profiles" / "k_flow_cfg4_pmc.json"
+    if pmc.exists():  # HBM bytes of one forward + one backward launch, from the committed FETCH_SIZE/WRITE_SIZE runs
+        rec = json.loads(pmc.read_text())
+        traffic = sum(rec[k]["traffic_bytes_per_launch"] for k in ("k_flow_fwd_all", "k_flow_bwd_all"))
+    roofline = {"bound": "mfma", "compute": "bf16 MFMA conditioner GEMMs (latency-bound at 4096 rows)",
                 "achieved": achieved, "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
-                "frac": achieved / PEAK_BF16_TFLOPS, "traffic": None, "kernel": "k_gemm_nt (flow conditioners)",
+                "frac": achieved / PEAK_BF16_TFLOPS, "traffic": traffic,
+                "kernel": "k_flow_fwd_all + k_flow_bwd_all (one launch of each per batched gradient)",
                 "timing": "HIP events around all sampling launches of the flow run",
                 "flops_per_chain_gradient": flops_grad}
     cpu = None
