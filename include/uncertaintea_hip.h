@@ -293,6 +293,16 @@ int ut_meads_run(ut_workspace* ws, const ut_meads_opts* opts, ut_chain_out* out,
  * be multiples of 128.                                                           */
 int ut_flow_create(int32_t num_params, int32_t num_layers, int32_t hidden, double scale_clamp,
                    uint64_t seed, ut_flow** out);
+/* The full-rank Gaussian guide as a flow without coupling layers:
+ * q = mu + exp(log_scale) z + F z with F strictly lower triangular (replaces the
+ * reference's _draw_fullrank_particles! / _accumulate_fullrank_gradients!,
+ * src/inference/vi/guides.jl:96-117,164-199, host-only there).  Weights: mu[P],
+ * log_scale[P], F[P x P] row-major (entries on and above the diagonal are ignored
+ * and get zero gradient); created at zero, the reference's starting point
+ * (src/inference/vi.jl:459-460).  Supports ut_flow_forward / inverse / vjp,
+ * ut_flow_elbo_grad, ut_flow_fit_elbo and attaching to a workspace (dense-metric
+ * HMC in z space); ut_flow_train and ut_flow_loss_grad return UT_E_ARG.          */
+int ut_flow_create_fullrank(int32_t num_params, ut_flow** out);
 int ut_flow_destroy(ut_flow* flow);
 /* flat fp32 weight vector: mu[P], log_scale[P], then per layer W1[hidden x kp],
  * b1[hidden], W2[op x hidden], b2[op] with kp = np rounded up to 128 and

@@ -51,6 +51,10 @@ def _load():
         "orc_flow_loss_grad": (dbl, [i32, i32, i32, dbl, i32, vp, vp, i64, vp]),
         "orc_flow_num_weights": (i64, [i32, i32, i32]),
         "orc_flow_elbo_grad": (None, [i32, i32, i32, dbl, i32, vp, vp, vp, vp, vp, i64, vp]),
+        "orc_fullrank_forward": (None, [i32, vp, vp, i64, vp, vp]),
+        "orc_fullrank_inverse": (None, [i32, vp, vp, i64, vp, vp]),
+        "orc_fullrank_vjp": (None, [i32, vp, vp, i64, vp]),
+        "orc_fullrank_elbo_grad": (None, [i32, vp, vp, vp, vp, vp, i64, vp]),
         "orc_chees_halton": (None, [i32, vp]),
         "orc_chees_steps": (i32, [dbl, dbl, dbl]),
         "orc_chees_trajectory": (None, [dbl, dbl, dbl, dbl, dbl, dbl, i32, vp, vp, vp, vp, i32, vp]),
@@ -354,6 +358,55 @@ class OracleFlow:
         g = np.empty(self.w.shape[0])
         _load().orc_flow_elbo_grad(self.P, self.L, self.H, self.clamp, int(self.bf16), _p(self.w), _p(z), _p(g_q),
                                    _p(v), _p(w), z.shape[0], _p(g))
+        return g
+
+
+class OracleFullRank:
+    """Binary64 restatement of the full-rank Gaussian guide with the device's flat
+    weights: mu[P], log_scale[P], F[P x P] row-major (strict lower triangle used)."""
+
+    def __init__(self, P: int, weights):
+        self.P = int(P)
+        self.w = np.ascontiguousarray(weights, dtype=np.float32)
+        if self.w.shape != (2 * self.P + self.P * self.P,):
+            raise ValueError(f"expected {2 * self.P + self.P * self.P} weights, got {self.w.shape}")
+
+    @classmethod
+    def of(cls, guide):
+        return cls(guide.num_params, guide.weights)
+
+    def _rows(self, a):
+        return np.ascontiguousarray(np.asarray(a, dtype=np.float64).reshape(-1, self.P))
+
+    def forward(self, z):
+        z = self._rows(z)
+        q = np.empty_like(z)
+        ld = np.empty(z.shape[0])
+        _load().orc_fullrank_forward(self.P, _p(self.w), _p(z), z.shape[0], _p(q), _p(ld))
+        return q, ld
+
+    def inverse(self, q):
+        q = self._rows(q)
+        z = np.empty_like(q)
+        ld = np.empty(q.shape[0])
+        _load().orc_fullrank_inverse(self.P, _p(self.w), _p(q), q.shape[0], _p(z), _p(ld))
+        return z, ld
+
+    def vjp(self, g_q):
+        g = self._rows(g_q)
+        out = np.empty_like(g)
+        _load().orc_fullrank_vjp(self.P, _p(self.w), _p(g), g.shape[0], _p(out))
+        return out
+
+    def elbo_grad(self, z, g_q, valid=None, row_weights=None):
+        """As OracleFlow.elbo_grad for the full-rank guide (src/inference/vi/guides.jl:
+        164-199 and, with row_weights, 459-489)."""
+        z = self._rows(z)
+        g_q = self._rows(g_q)
+        v = None if valid is None else np.ascontiguousarray(np.asarray(valid, dtype=np.int32))
+        w = None if row_weights is None else np.ascontiguousarray(np.asarray(row_weights, dtype=np.float64))
+        g = np.empty(self.w.shape[0])
+        _load().orc_fullrank_elbo_grad(self.P, _p(self.w), _p(z), _p(g_q), _p(v), _p(w), z.shape[0], _p(g))
         return g
 
 

@@ -29,6 +29,9 @@
 //                          src/inference/data_types.jl:189-193
 //   sampler drivers       src/inference/api_batched_nuts.jl:761-980,
 //                          src/device/hmc_kernels.jl:306-723
+//   full-rank guide       src/inference/vi/guides.jl:96-117,164-199,459-489
+//                          (orc_fullrank_*: pinned by the Gaussian-target optimum,
+//                          finite differences and the matrix forms of L, L^-1, L^T)
 #include <omp.h>
 
 #include <algorithm>
@@ -2264,6 +2267,74 @@ void orc_flow_elbo_grad(int32_t P, int32_t L, int32_t H, double clamp, int32_t b
   }
   for (long long i = 0; i < np; ++i) grad[i] = row_weights ? total[i] : (nv ? total[i] / (double)nv : 0.0);
 }
+// ---- full-rank Gaussian guide (binary64): q = mu + e^ls z + F z with F the strict
+// lower triangle of w[2P:] (P x P row-major), src/inference/vi/guides.jl:96-117;
+// log|det dq/dz| = sum ls.
+void orc_fullrank_forward(int32_t P, const float* w, const double* z, int64_t n, double* q, double* logdet) {
+  const float* F = w + 2LL * P;
+  double ld = 0.0;
+  for (int i = 0; i < P; ++i) ld += (double)w[P + i];
+  for (int64_t r = 0; r < n; ++r) {
+    const double* zr = z + r * P;
+    for (int i = 0; i < P; ++i) {
+      double v = (double)w[i] + std::exp((double)w[P + i]) * zr[i];
+      for (int j = 0; j < i; ++j) v += (double)F[(long long)i * P + j] * zr[j];
+      q[r * P + i] = v;
+    }
+    logdet[r] = ld;
+  }
+}
+// z = L^-1 (q - mu) by forward substitution; logdet_inv = -sum ls
+void orc_fullrank_inverse(int32_t P, const float* w, const double* q, int64_t n, double* z, double* logdet_inv) {
+  const float* F = w + 2LL * P;
+  double ld = 0.0;
+  for (int i = 0; i < P; ++i) ld -= (double)w[P + i];
+  for (int64_t r = 0; r < n; ++r) {
+    double* zr = z + r * P;
+    for (int i = 0; i < P; ++i) {
+      double v = q[r * P + i] - (double)w[i];
+      for (int j = 0; j < i; ++j) v -= (double)F[(long long)i * P + j] * zr[j];
+      zr[i] = v / std::exp((double)w[P + i]);
+    }
+    logdet_inv[r] = ld;
+  }
+}
+// g_z = L^T g_q (the log-determinant does not depend on z)
+void orc_fullrank_vjp(int32_t P, const float* w, const double* g_q, int64_t n, double* g_z) {
+  const float* F = w + 2LL * P;
+  for (int64_t r = 0; r < n; ++r) {
+    const double* g = g_q + r * P;
+    for (int j = 0; j < P; ++j) {
+      double v = std::exp((double)w[P + j]) * g[j];
+      for (int i = j + 1; i < P; ++i) v += (double)F[(long long)i * P + j] * g[i];
+      g_z[r * P + j] = v;
+    }
+  }
+}
+// The weight gradient of g_q . q(z) + sum ls with g_q held fixed, averaged over the
+// valid rows, or summed with row_weights (as orc_flow_elbo_grad):
+// d/dmu = g, d/dls_i = g_i e^ls_i z_i + 1, d/dF[i, j] = g_i z_j for j < i, 0 on and
+// above the diagonal (src/inference/vi/guides.jl:164-199, 459-489).
+void orc_fullrank_elbo_grad(int32_t P, const float* w, const double* z, const double* g_q, const int32_t* valid,
+                            const double* row_weights, int64_t n, double* grad) {
+  const long long np = 2LL * P + (long long)P * P;
+  std::vector<double> total((size_t)np, 0.0);
+  double nv = 0.0;
+  for (int64_t r = 0; r < n; ++r) {
+    if (valid && !valid[r]) continue;
+    const double wr = row_weights ? row_weights[r] : 1.0;
+    const double* g = g_q + r * P;
+    const double* zr = z + r * P;
+    for (int i = 0; i < P; ++i) {
+      total[i] += wr * g[i];
+      total[P + i] += wr * (g[i] * std::exp((double)w[P + i]) * zr[i] + 1.0);
+      for (int j = 0; j < i; ++j) total[2LL * P + (long long)i * P + j] += wr * g[i] * zr[j];
+    }
+    nv += 1.0;
+  }
+  for (long long i = 0; i < np; ++i) grad[i] = row_weights ? total[i] : (nv > 0 ? total[i] / nv : 0.0);
+}
+
 int64_t orc_flow_num_weights(int32_t P, int32_t L, int32_t H) {
   FlowSpec fs(P, L, H, 1.0, false, nullptr);
   return fs.nparams();

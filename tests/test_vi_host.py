@@ -47,7 +47,7 @@ def test_initial_params_argument_errors():
 
 
 @pytest.mark.parametrize("kw,msg", [
-    ({"guide": "fullrank"}, "CPU-only"),
+    ({"guide": "lowrank"}, "CPU-only"),
     ({"guide": "bogus"}, "guide must be"),
     ({"elbo": "bogus"}, "elbo must be"),
     ({"elbo": "iwae", "iwae_samples": 100}, "divide num_particles evenly"),

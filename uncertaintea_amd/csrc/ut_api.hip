@@ -1521,6 +1521,25 @@ int ut_flow_create(int32_t num_params, int32_t num_layers, int32_t hidden, doubl
   return UT_OK;
 }
 
+int ut_flow_create_fullrank(int32_t num_params, ut_flow** out) {
+  if (!out) return fail(UT_E_ARG, "ut_flow_create_fullrank: null pointer");
+  *out = nullptr;
+  if (num_params < 2) return fail(UT_E_DIM, "ut_flow_create_fullrank: at least 2 parameters");
+  if (!ut_device_available()) return fail(UT_E_HIP, "ut_flow_create_fullrank: no HIP device visible");
+  auto* fl = new ut_flow();
+  Flow& f = fl->f;
+  // H only sizes unused scratch here (no coupling layers)
+  hipError_t e = flow_alloc(f, num_params, 0, 128, true);
+  // mu = 0, ls = 0 and a zero factor: the mean-field start (src/inference/vi.jl:459-460)
+  if (e == hipSuccess) e = hipMemset(f.params, 0, (size_t)f.nparams * 4);
+  if (e != hipSuccess) {
+    delete fl;
+    return fail(UT_E_HIP, std::string("ut_flow_create_fullrank: ") + hipGetErrorString(e));
+  }
+  *out = fl;
+  return UT_OK;
+}
+
 int ut_flow_destroy(ut_flow* flow) {
   delete flow;
   return UT_OK;
@@ -1575,6 +1594,7 @@ int ut_flow_set_base(ut_flow* flow, const double* mu, const double* log_scale) {
 int ut_flow_train(ut_flow* flow, const float* draws, int64_t n, int32_t steps, int32_t batch, double learning_rate,
                   uint64_t seed, double* loss_trace) {
   if (!flow || !draws) return fail(UT_E_ARG, "ut_flow_train: null pointer");
+  if (flow->f.factor) return fail(UT_E_ARG, "ut_flow_train: the full-rank guide is fitted by ut_flow_fit_elbo");
   if (n <= 0 || steps < 0) return fail(UT_E_ARG, "ut_flow_train: n must be > 0 and steps >= 0");
   int rc = flow_rows_check(flow, batch, "ut_flow_train (batch)");
   if (rc) return rc;
@@ -1671,6 +1691,7 @@ int ut_flow_loss_grad(ut_flow* flow, const float* q, int64_t n, double* loss, fl
   int rc = flow_rows_check(flow, n, "ut_flow_loss_grad");
   if (rc) return rc;
   if (!q || !grad) return fail(UT_E_ARG, "ut_flow_loss_grad: null pointer");
+  if (flow->f.factor) return fail(UT_E_ARG, "ut_flow_loss_grad: not defined for the full-rank guide");
   Flow& f = flow->f;
   UT_HIP(flow_scratch(f, (int)n));
   DeviceBuffers buf;

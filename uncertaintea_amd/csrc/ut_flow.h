@@ -18,6 +18,7 @@ namespace ut {
 
 constexpr int FLOW_PAD = 128;  // passive inputs and conditioner outputs are padded to 128
 constexpr int FLOW_WSPLITS = 32;  // split-K of the weight-gradient GEMMs (K = rows)
+constexpr int FACTOR_SPLITS = 64;  // row splits of the full-rank factor gradient
 
 struct FlowLayer {
   int np = 0, na = 0;      // passive / active counts (2 na <= 128, np <= 128)
@@ -95,9 +96,16 @@ struct Flow {
   FusedFwdLayer* dfwd = nullptr;  // device, L descriptors
   FusedBwdLayer* dbwd = nullptr;
   bool fused = true;        // LDS-staged layer kernels on the sampling path where the widths allow
+  // full-rank Gaussian guide (no coupling layers): q = mu + e^ls z + F z with F the
+  // strictly lower P x P factor stored row-major at params + ofac
+  bool factor = false;
+  long long ofac = 0;
+  double* fpart = nullptr;  // FACTOR_SPLITS x P x P split partials of F's gradient
 };
 
-hipError_t flow_alloc(Flow& f, int P, int L, int H);
+// factor = true: the full-rank guide (L must be 0), mu, ls and the strictly lower
+// factor as its weights
+hipError_t flow_alloc(Flow& f, int P, int L, int H, bool factor = false);
 void flow_free(Flow& f);
 hipError_t flow_scratch(Flow& f, int B);
 // refresh the bf16 weight copies after the fp32 masters changed

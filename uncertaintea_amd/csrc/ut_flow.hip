@@ -841,6 +841,82 @@ __global__ void k_adam_ascent(float* __restrict__ p, const float* __restrict__ g
   p[i] += lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
 }
 
+// ---- full-rank Gaussian guide: q = mu + e^ls z + F z, F strictly lower
+// (src/inference/vi/guides.jl:96-117).  One thread per element; sums in binary64.
+
+// x[b, i] += sum_{j<i} F[i, j] z[b, j]  (x already holds mu + e^ls z)
+__global__ __launch_bounds__(256) void k_factor_fwd(const float* __restrict__ z, const float* __restrict__ F, int P,
+                                                    float* __restrict__ x, long long n) {
+  const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= n) return;
+  const long long b = t / P;
+  const int i = (int)(t - b * P);
+  const float* zr = z + b * P;
+  const float* Fr = F + (long long)i * P;
+  double a = 0.0;
+  for (int j = 0; j < i; ++j) a += (double)Fr[j] * (double)zr[j];
+  x[t] = (float)((double)x[t] + a);
+}
+
+// out[b, j] = sum_{i>j} F[i, j] g[b, i]: the factor's part of (dq/dz)^T g
+__global__ __launch_bounds__(256) void k_factor_vjp(const float* __restrict__ g, const float* __restrict__ F, int P,
+                                                    float* __restrict__ out, long long n) {
+  const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= n) return;
+  const long long b = t / P;
+  const int j = (int)(t - b * P);
+  const float* gr = g + b * P;
+  double a = 0.0;
+  for (int i = j + 1; i < P; ++i) a += (double)F[(long long)i * P + j] * (double)gr[i];
+  out[t] = (float)a;
+}
+
+__global__ __launch_bounds__(256) void k_add_to(float* __restrict__ x, const float* __restrict__ a, long long n) {
+  const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (t < n) x[t] += a[t];
+}
+
+// z = L^-1 (x - mu), L = diag(e^ls) + F: forward substitution, one thread per row
+__global__ __launch_bounds__(256) void k_factor_inv(const float* __restrict__ x, const float* __restrict__ prm, int P,
+                                                    float* __restrict__ z, int B) {
+  const int b = blockIdx.x * 256 + threadIdx.x;
+  if (b >= B) return;
+  const float* xr = x + (long long)b * P;
+  float* zr = z + (long long)b * P;
+  const float* F = prm + 2LL * P;
+  for (int i = 0; i < P; ++i) {
+    double a = (double)xr[i] - (double)prm[i];
+    const float* Fr = F + (long long)i * P;
+    for (int j = 0; j < i; ++j) a -= (double)Fr[j] * (double)zr[j];
+    zr[i] = (float)(a * exp(-(double)prm[P + i]));
+  }
+}
+
+// part[y][i, j] = sum over row chunk y of g[b, i] z[b, j] for j < i (0 elsewhere):
+// the factor's ELBO gradient (src/inference/vi/guides.jl:164-199) before the row sum
+__global__ __launch_bounds__(256) void k_factor_grad_part(const float* __restrict__ g, const float* __restrict__ z,
+                                                          int B, int P, double* __restrict__ part) {
+  const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+  const long long PP = (long long)P * P;
+  if (t >= PP) return;
+  const int i = (int)(t / P), j = (int)(t - (long long)i * P);
+  const int per = (B + (int)gridDim.y - 1) / (int)gridDim.y;
+  const int b0 = (int)blockIdx.y * per, b1 = min(B, b0 + per);
+  double a = 0.0;
+  if (j < i)
+    for (int b = b0; b < b1; ++b) a += (double)g[(long long)b * P + i] * (double)z[(long long)b * P + j];
+  part[(long long)blockIdx.y * PP + t] = a;
+}
+
+__global__ __launch_bounds__(256) void k_factor_grad_final(const double* __restrict__ part, int splits, long long PP,
+                                                           float* __restrict__ out) {
+  const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= PP) return;
+  double a = 0.0;
+  for (int k = 0; k < splits; ++k) a += part[(long long)k * PP + t];
+  out[t] = (float)a;
+}
+
 }  // namespace
 
 // ------------------------------------------------------------------ host side
@@ -853,10 +929,12 @@ static hipError_t own(Flow& f, T** p, size_t count) {
   return e;
 }
 
-hipError_t flow_alloc(Flow& f, int P, int L, int H) {
+hipError_t flow_alloc(Flow& f, int P, int L, int H, bool factor) {
+  if (factor && L != 0) return hipErrorInvalidValue;
   f.P = P;
   f.L = L;
   f.H = H;
+  f.factor = factor;
   {
     // UT_FLOW_FUSED=0 keeps the per-step kernels on the sampling path (for comparison runs)
     const char* e = std::getenv("UT_FLOW_FUSED");
@@ -890,8 +968,13 @@ hipError_t flow_alloc(Flow& f, int P, int L, int H) {
     ly.ob2 = off;
     off += op;
   }
+  if (factor) {
+    f.ofac = off;
+    off += (long long)P * P;
+  }
   f.nparams = off;
   hipError_t e = own(f, &f.params, (size_t)off);
+  if (e == hipSuccess && factor) e = own(f, &f.fpart, (size_t)FACTOR_SPLITS * P * P);
   if (e == hipSuccess && L > 0) e = own(f, &f.dfwd, (size_t)L);
   if (e == hipSuccess && L > 0) e = own(f, &f.dbwd, (size_t)L);
   if (e == hipSuccess) e = own(f, &f.grads, (size_t)off);
@@ -1056,6 +1139,8 @@ static hipError_t forward_impl(Flow& f, const float* z, float* q, float* logdet,
   } else {
     hipLaunchKernelGGL(k_logdet_init, dim3(nblk(B)), dim3(T256), 0, st, f.params + f.P, f.P, 1.0f, s.logdet, B);
     hipLaunchKernelGGL(k_base_fwd, dim3(nblk(n)), dim3(T256), 0, st, z, f.params, f.P, s.x, n);
+    if (f.factor)
+      hipLaunchKernelGGL(k_factor_fwd, dim3(nblk(n)), dim3(T256), 0, st, z, f.params + f.ofac, f.P, s.x, n);
     for (int l = 0; l < f.L; ++l) {
       FlowLayer& ly = f.layers[l];
       if (s.fwd_fused[l]) {
@@ -1118,7 +1203,10 @@ hipError_t flow_backward(Flow& f, const float* g_q, float* g_z, int B, hipStream
       if (e != hipSuccess) return e;
     }
   }
+  if (f.factor)  // F^T g from the gradient before the base scales it
+    hipLaunchKernelGGL(k_factor_vjp, dim3(nblk(n)), dim3(T256), 0, st, s.g, f.params + f.ofac, f.P, s.aux, n);
   hipLaunchKernelGGL(k_base_fwd_bwd, dim3(nblk(n)), dim3(T256), 0, st, s.g, f.params, f.P, n);
+  if (f.factor) hipLaunchKernelGGL(k_add_to, dim3(nblk(n)), dim3(T256), 0, st, s.g, s.aux, n);
   if (g_z != s.g) {
     hipError_t e = hipMemcpyAsync(g_z, s.g, n * 4, hipMemcpyDeviceToDevice, st);
     if (e != hipSuccess) return e;
@@ -1143,7 +1231,10 @@ hipError_t flow_inverse(Flow& f, const float* q, float* z, float* logdet_inv, in
                        round_up(2 * ly.na, FLOW_PAD), f.params + ly.ob2, ly.aidx, ly.na, f.clamp, s.s[l], s.xa[l],
                        s.logdet, B);
   }
-  hipLaunchKernelGGL(k_base_inv, dim3(nblk(n)), dim3(T256), 0, st, s.x, f.params, f.P, z, n);
+  if (f.factor)
+    hipLaunchKernelGGL(k_factor_inv, dim3(nblk(B)), dim3(T256), 0, st, s.x, f.params, f.P, z, B);
+  else
+    hipLaunchKernelGGL(k_base_inv, dim3(nblk(n)), dim3(T256), 0, st, s.x, f.params, f.P, z, n);
   if (logdet_inv && logdet_inv != s.logdet) {
     hipError_t e = hipMemcpyAsync(logdet_inv, s.logdet, (size_t)B * 4, hipMemcpyDeviceToDevice, st);
     if (e != hipSuccess) return e;
@@ -1213,6 +1304,13 @@ hipError_t flow_elbo_backward(Flow& f, const float* z, const float* rw, int B, h
   colsum(s.g, B, f.P, f.grads, s.colsum, st);
   hipLaunchKernelGGL(k_base_fwd_param_terms, dim3(nblk(n)), dim3(T256), 0, st, z, s.g, f.params, f.P, rw, s.aux, n);
   colsum(s.aux, B, f.P, f.grads + f.P, s.colsum, st);
+  if (f.factor) {  // d/dF[i, j] = sum over rows of g_i z_j, j < i
+    const long long PP = (long long)f.P * f.P;
+    hipLaunchKernelGGL(k_factor_grad_part, dim3(nblk(PP), FACTOR_SPLITS), dim3(T256), 0, st, s.g, z, B, f.P,
+                       f.fpart);
+    hipLaunchKernelGGL(k_factor_grad_final, dim3(nblk(PP)), dim3(T256), 0, st, f.fpart, FACTOR_SPLITS, PP,
+                       f.grads + f.ofac);
+  }
   return hipGetLastError();
 }
 

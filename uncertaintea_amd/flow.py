@@ -41,6 +41,9 @@ class AffineCouplingFlow:
         h = ct.c_void_p()
         L.check(L.lib().ut_flow_create(self.num_params, self.num_layers, self.hidden, self.scale_clamp,
                                        int(seed) & 0xFFFFFFFFFFFFFFFF, ct.byref(h)))
+        self._adopt(h)
+
+    def _adopt(self, h):
         self._h = h
         self._destroy = L.lib().ut_flow_destroy  # kept: module globals may be gone at exit
         n = ct.c_int64()
@@ -185,3 +188,37 @@ class AffineCouplingFlow:
         loss = ct.c_double()
         L.check(L.lib().ut_flow_loss_grad(self._h, L.ptr(qq), qq.shape[0], ct.byref(loss), L.ptr(g)))
         return float(loss.value), g
+
+
+class FullRankGaussian(AffineCouplingFlow):
+    """The full-rank Gaussian guide q = mu + exp(log_scale) z + F z, F strictly
+    lower triangular (the reference's :fullrank guide, src/inference/vi/guides.jl:
+    96-117,164-199), on the flow interface: the same weights, ELBO fit (`fit_elbo`),
+    sampling (`forward`), `inverse` and `vjp`, and it preconditions the lock-step
+    samplers like a flow (HMC in z space with the dense metric L L^T).  Weights:
+    mu[P], log_scale[P], F[P x P] row-major, created at zero."""
+
+    def __init__(self, num_params: int):
+        self.num_params = int(num_params)
+        self.num_layers = 0
+        self.hidden = 128
+        self.scale_clamp = 3.0
+        h = ct.c_void_p()
+        L.check(L.lib().ut_flow_create_fullrank(self.num_params, ct.byref(h)))
+        self._adopt(h)
+
+    def factor_of(self, weights=None) -> np.ndarray:
+        """The strict lower triangle of the Cholesky factor (the reference's
+        ADVIResult.scale_factor, src/inference/vi.jl:18-21), float64."""
+        w = self.weights if weights is None else np.asarray(weights)
+        P = self.num_params
+        return np.tril(np.asarray(w[2 * P:], dtype=np.float64).reshape(P, P), -1)
+
+    def cholesky(self, weights=None) -> np.ndarray:
+        """L = diag(exp(log_scale)) + F, so the guide's covariance is L L^T."""
+        w = self.weights if weights is None else np.asarray(weights)
+        P = self.num_params
+        return self.factor_of(w) + np.diag(np.exp(np.asarray(w[P:2 * P], dtype=np.float64)))
+
+    def fit(self, *args, **kwargs):
+        raise ValueError("FullRankGaussian is fitted by fit_elbo (the full-rank guide has no maximum-likelihood fit)")

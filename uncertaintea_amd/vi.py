@@ -1,10 +1,13 @@
-"""Variational inference on the device: batched_advi with the mean-field and the
-flow guides (reference: src/inference/vi.jl, src/inference/vi/guides.jl).
+"""Variational inference on the device: batched_advi with the mean-field, the
+full-rank and the flow guides (reference: src/inference/vi.jl,
+src/inference/vi/guides.jl).
 
-Both guides run through the C ABI's ut_flow_fit_elbo: the mean-field Gaussian
+All three guides run through the C ABI's ut_flow_fit_elbo: the mean-field Gaussian
 guide is the flow with no coupling layers (its elementwise affine base
-x = mu + exp(log_scale) z), the flow guide adds `num_flow_layers` affine-coupling
-layers with MLP conditioners (linear in the reference).  Each iteration draws the
+x = mu + exp(log_scale) z), the full-rank guide adds a strictly lower factor F
+(x = mu + exp(log_scale) z + F z, ut_flow_create_fullrank), the flow guide adds
+`num_flow_layers` affine-coupling layers with MLP conditioners (linear in the
+reference).  Each iteration draws the
 particles on the device, scores them with the model's batched gradient engine,
 drops invalid particles (non-finite logjoint or gradient; src/inference/vi.jl:
 760-771), forms the reparameterised standard-ELBO gradient with the model
@@ -14,7 +17,7 @@ device, so results agree with the reference statistically, not draw for draw (as
 the reference's own device ADVI path states, src/inference/vi.jl:367-368).
 elbo="iwae" optimises the importance-weighted bound over groups of
 `iwae_samples` particles (src/inference/vi/guides.jl:373-425).  Not built: the
-full-rank and low-rank guides.
+low-rank guide.
 """
 from __future__ import annotations
 
@@ -23,17 +26,20 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import _lib as L
-from .flow import AffineCouplingFlow
+from .flow import AffineCouplingFlow, FullRankGaussian
 from .models import TeaModel
 
-_GUIDES = ("meanfield", "flow")
+_GUIDES = ("meanfield", "fullrank", "flow")
 
 
 @dataclass
 class ADVIResult:
     """Fields of the reference's ADVIResult (src/inference/vi.jl:3-30) that the
     device guides produce; `flow` / `best_flow` hold the guide (for the mean-field
-    guide a flow without coupling layers) with the last / best weights."""
+    guide a flow without coupling layers, for the full-rank guide a
+    FullRankGaussian) with the last / best weights; `scale_factor` /
+    `best_scale_factor` are the full-rank guide's strict lower Cholesky triangle
+    (None for the other guides)."""
     model: TeaModel
     location: np.ndarray
     log_scale: np.ndarray
@@ -51,6 +57,8 @@ class ADVIResult:
     elbo: str = "standard"
     iwae_samples: int = 1
     standard_elbo_history: np.ndarray = field(default=None)
+    scale_factor: np.ndarray | None = None
+    best_scale_factor: np.ndarray | None = None
 
     def __post_init__(self):
         if self.standard_elbo_history is None:
@@ -92,9 +100,9 @@ def batched_advi(model: TeaModel, num_iterations: int, num_particles: int = 128,
     of 128 (the particles are the rows of one batched gradient; the reference's
     default is 32).  Argument checks and messages follow the reference's."""
     if guide not in _GUIDES:
-        if guide in ("fullrank", "lowrank"):
-            raise ValueError(f"batched_advi guide={guide} is CPU-only in the reference and not built here; the "
-                             "device path supports meanfield and flow")
+        if guide == "lowrank":
+            raise ValueError("batched_advi guide=lowrank is CPU-only in the reference and not built here; the "
+                             "device path supports meanfield, fullrank and flow")
         raise ValueError(f"batched_advi guide must be :meanfield, :fullrank, :lowrank, or :flow, got {guide}")
     if elbo not in ("standard", "iwae"):
         raise ValueError(f"batched_advi elbo must be :standard or :iwae, got :{elbo}")
@@ -124,21 +132,27 @@ def batched_advi(model: TeaModel, num_iterations: int, num_particles: int = 128,
     location = _initial_location(model, initial_params, seed)
     log_scale = _scale_vector(initial_log_scale, P)
     layers = int(num_flow_layers) if guide == "flow" else 0
-    flow = AffineCouplingFlow(P, layers, flow_hidden, seed=seed)
+
+    def make():
+        return FullRankGaussian(P) if guide == "fullrank" else AffineCouplingFlow(P, layers, flow_hidden, seed=seed)
+
+    flow = make()
     flow.set_base(location, log_scale)
     out = flow.fit_elbo(model, num_particles=num_particles, num_iterations=num_iterations,
                         learning_rate=learning_rate, gradient_clip=gradient_clip, seed=seed, beta1=beta1,
                         beta2=beta2, adam_epsilon=adam_epsilon, iwae_group=group)
     w = flow.weights
     best = out["best_weights"]
-    best_flow = AffineCouplingFlow(P, layers, flow_hidden, seed=seed)
+    best_flow = make()
     best_flow.weights = best
+    factors = (flow.factor_of(w), flow.factor_of(best)) if guide == "fullrank" else (None, None)
     return ADVIResult(model=model, location=w[:P].astype(np.float64), log_scale=w[P:2 * P].astype(np.float64),
                       best_location=best[:P].astype(np.float64), best_log_scale=best[P:2 * P].astype(np.float64),
                       elbo_history=out["elbo_history"], gradient_norm_history=out["grad_norm_history"],
                       best_elbo=out["best_elbo"], num_particles=int(num_particles),
                       learning_rate=float(learning_rate), guide=guide, flow=flow, best_flow=best_flow, elbo=elbo,
-                      iwae_samples=group, standard_elbo_history=out["standard_elbo_history"])
+                      iwae_samples=group, standard_elbo_history=out["standard_elbo_history"],
+                      scale_factor=factors[0], best_scale_factor=factors[1])
 
 
 def variational_samples(result: ADVIResult, num_samples: int, space: str = "constrained", use_best: bool = True,
@@ -171,9 +185,13 @@ def variational_mean(result: ADVIResult, space: str = "constrained", use_best: b
 
 def variational_covariance(result: ADVIResult, use_best: bool = True) -> np.ndarray:
     """src/inference/vi.jl:129-150: diagonal exp(2 log_scale) for the mean-field
-    guide, a fixed-seed 20000-draw estimate for the flow (unconstrained space)."""
+    guide, L L^T for the full-rank guide, a fixed-seed 20000-draw estimate for the
+    flow (unconstrained space)."""
     if result.guide == "flow":
         s = variational_samples(result, 20000, space="unconstrained", use_best=use_best)
         return np.cov(s)
     ls = result.best_log_scale if use_best else result.log_scale
+    if result.guide == "fullrank":
+        chol = (result.best_scale_factor if use_best else result.scale_factor) + np.diag(np.exp(ls))
+        return chol @ chol.T
     return np.diag(np.exp(2.0 * ls))
