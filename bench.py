@@ -28,6 +28,13 @@ L = 10: plain (per-chain adaptation, persistent kernel) and flow-preconditioned
 space).  `value` is the flow run's min bulk-ESS/s; the plain run is reported beside
 it.  Roofline: the flow's conditioner GEMMs against the bf16 MFMA peak.
 
+`--workload advi`: batched_advi on the same hierarchical model (P = 66), 1024
+particles per iteration, `--guide` meanfield / fullrank (default) / flow (4 layers,
+width 256), standard ELBO, lr 0.01; `value` is Adam iterations/s of the whole call
+(one iteration = draw, guide forward, batched model gradient over the particles,
+ELBO reduction, guide backward, clip, Adam).  CPU baseline: the same iteration in
+numpy around the oracle's binary64 gradient (full-rank guide), bounded in time.
+
 `--workload gp_meads`: the latent GP with Poisson counts (N = 512 inputs in
 [0,1]^2, ARD squared-exponential kernel, P = 515), batched_meads with 1024 chains,
 binary64 batched Cholesky with hyperparameter tangents; without and with a flow
@@ -110,7 +117,10 @@ def cpu_cores() -> int:
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=("hier_nuts", "glm_chees", "funnel_flow", "gp_meads"), default="hier_nuts")
+    ap.add_argument("--workload", choices=("hier_nuts", "glm_chees", "funnel_flow", "gp_meads", "advi"),
+                    default="hier_nuts")
+    ap.add_argument("--guide", choices=("meanfield", "fullrank", "flow"), default="fullrank",
+                    help="advi: the variational guide")
     ap.add_argument("--steps", type=int, default=None, help="sampling iterations")
     ap.add_argument("--warmup", type=int, default=None, help="warmup iterations")
     ap.add_argument("--chains", type=int, default=None)
@@ -162,6 +172,8 @@ def main() -> int:
         return bench_funnel_flow(args)
     if args.workload == "gp_meads":
         return bench_gp_meads(args)
+    if args.workload == "advi":
+        return bench_advi(args)
     args.steps = 3000 if args.steps is None else args.steps
     args.warmup = 1000 if args.warmup is None else args.warmup
     args.chains = 1024 if args.chains is None else args.chains
@@ -397,6 +409,81 @@ def _funnel_report(args, D, model, flow, plain, plain_s, fres, flow_s, rounds, t
             D, _funnel_standardised(fres.constrained_samples[:, ::max(1, final_steps // steps)]),
             np.zeros(model.num_params), np.concatenate([[9.0], np.ones(model.num_params - 1)])),
         "roofline": roofline, "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def bench_advi(args) -> int:
+    import uncertaintea_amd as ut
+    from uncertaintea_amd import synthetic as S
+
+    iters = 2000 if args.steps is None else args.steps
+    warm = 100 if args.warmup is None else args.warmup
+    n = 1024 if args.chains is None else args.chains
+    y, off, sig, _ = S.hierarchical_data(64, 32, seed=args.seed)
+    model = ut.HierarchicalNormalNC(y, off, sig, mu_loc=0.0, mu_scale=5.0, tau_scale=5.0)
+    P = model.num_params
+    kw = dict(num_particles=n, learning_rate=0.01, guide=args.guide, num_flow_layers=4,
+              initial_params=np.zeros(P), initial_log_scale=-1.0, seed=args.seed)
+    log(f"bench: batched_advi guide={args.guide} hierarchical J=64x32 particles={n} iterations={iters}")
+    ut.batched_advi(model, num_iterations=max(warm, 1), **kw)
+    t0 = time.perf_counter()
+    r = ut.batched_advi(model, num_iterations=iters, **kw)
+    wall = time.perf_counter() - t0
+    h = r.elbo_history
+    value = iters / wall
+    log(f"bench: {wall:.3f}s -> {value:.1f} iterations/s; elbo {h[:50].mean():.2f} -> {h[-50:].mean():.2f}")
+    # end-to-end figure: model-gradient flops of all particles over the wall time
+    achieved = iters * n * FLOPS_PER_GRADIENT / wall / 1e12
+    roofline = {"bound": "mfma", "compute": "fp32 vector ALU (model gradient); end to end, not one kernel",
+                "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                "kernel": "whole ADVI iteration (per-kernel times in profiles/advi_hier_kernel_trace.md)",
+                "timing": "wall time of batched_advi"}
+    cpu = None
+    if not args.no_cpu_baseline and args.guide == "fullrank":
+        from oracle import binding as B
+        om = B.OracleModel(model)
+        rng = np.random.default_rng(args.seed)
+        mu, ls, F = np.zeros(P), np.full(P, -1.0), np.zeros((P, P))
+        theta = [mu, ls, F]
+        m1 = [np.zeros_like(a) for a in theta]
+        m2 = [np.zeros_like(a) for a in theta]
+        k, t_cpu = 0, 0.0
+        while t_cpu < args.cpu_seconds and k < iters:
+            ts = time.perf_counter()
+            z = rng.standard_normal((n, P))
+            Lc = np.tril(F, -1) + np.diag(np.exp(ls))
+            x = mu + z @ Lc.T
+            lj, g = om.logjoint_grad_f64(x.T)
+            g = g.T
+            ok = np.isfinite(lj) & np.all(np.isfinite(g), axis=1)
+            gv, zv = g[ok], z[ok]
+            grads = [gv.mean(axis=0), 1.0 + np.exp(ls) * (gv * zv).mean(axis=0), np.tril(gv.T @ zv, -1) / ok.sum()]
+            for a, gr, ma, va in zip(theta, grads, m1, m2):
+                ma *= 0.9
+                ma += 0.1 * gr
+                va *= 0.999
+                va += 0.001 * gr * gr
+                a += 0.01 * (ma / (1 - 0.9 ** (k + 1))) / (np.sqrt(va / (1 - 0.999 ** (k + 1))) + 1e-8)
+            t_cpu += time.perf_counter() - ts
+            k += 1
+        cpu = {"value": k / t_cpu, "unit": "iterations/s", "cores": cpu_cores(), "kind": "port",
+               "sample": f"{k} full-rank ADVI iterations x {n} particles in numpy around the oracle's binary64 "
+                         f"gradient ({t_cpu:.1f} s)"}
+        log(f"bench: cpu {cpu['value']:.2f} iterations/s")
+    line = {
+        "metric": "batched_advi iterations/s", "value": value, "unit": "iterations/s", "n_gpus": 1,
+        "steps": iters, "warmup": warm, "ms_per_step": 1e3 * wall / iters, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (Philox4x32-10, seed %d)" % args.seed,
+        "config": {"workload": f"batched_advi guide={args.guide}, hierarchical normal non-centred J=64x32 "
+                               f"(P=66), {n} particles, standard ELBO, lr 0.01",
+                   "particles": n, "num_params": P, "guide": args.guide},
+        "elbo_first50": float(h[:50].mean()), "elbo_last50": float(h[-50:].mean()), "best_elbo": r.best_elbo,
+        "particle_gradients_per_s": iters * n / wall,
+        "roofline": roofline, "cpu_baseline": cpu, "vs_cpu": (value / cpu["value"]) if cpu else None,
     }
     print(json.dumps(line), flush=True)
     return 0

@@ -853,9 +853,16 @@ __global__ __launch_bounds__(256) void k_factor_fwd(const float* __restrict__ z,
   const int i = (int)(t - b * P);
   const float* zr = z + b * P;
   const float* Fr = F + (long long)i * P;
-  double a = 0.0;
-  for (int j = 0; j < i; ++j) a += (double)Fr[j] * (double)zr[j];
-  x[t] = (float)((double)x[t] + a);
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;  // four chains: the sum is latency-bound
+  int j = 0;
+  for (; j + 4 <= i; j += 4) {
+    a0 += (double)Fr[j] * (double)zr[j];
+    a1 += (double)Fr[j + 1] * (double)zr[j + 1];
+    a2 += (double)Fr[j + 2] * (double)zr[j + 2];
+    a3 += (double)Fr[j + 3] * (double)zr[j + 3];
+  }
+  for (; j < i; ++j) a0 += (double)Fr[j] * (double)zr[j];
+  x[t] = (float)((double)x[t] + ((a0 + a1) + (a2 + a3)));
 }
 
 // out[b, j] = sum_{i>j} F[i, j] g[b, i]: the factor's part of (dq/dz)^T g
@@ -866,9 +873,14 @@ __global__ __launch_bounds__(256) void k_factor_vjp(const float* __restrict__ g,
   const long long b = t / P;
   const int j = (int)(t - b * P);
   const float* gr = g + b * P;
-  double a = 0.0;
-  for (int i = j + 1; i < P; ++i) a += (double)F[(long long)i * P + j] * (double)gr[i];
-  out[t] = (float)a;
+  double a0 = 0.0, a1 = 0.0;
+  int i = j + 1;
+  for (; i + 2 <= P; i += 2) {
+    a0 += (double)F[(long long)i * P + j] * (double)gr[i];
+    a1 += (double)F[(long long)(i + 1) * P + j] * (double)gr[i + 1];
+  }
+  for (; i < P; ++i) a0 += (double)F[(long long)i * P + j] * (double)gr[i];
+  out[t] = (float)(a0 + a1);
 }
 
 __global__ __launch_bounds__(256) void k_add_to(float* __restrict__ x, const float* __restrict__ a, long long n) {
@@ -902,19 +914,31 @@ __global__ __launch_bounds__(256) void k_factor_grad_part(const float* __restric
   const int i = (int)(t / P), j = (int)(t - (long long)i * P);
   const int per = (B + (int)gridDim.y - 1) / (int)gridDim.y;
   const int b0 = (int)blockIdx.y * per, b1 = min(B, b0 + per);
-  double a = 0.0;
-  if (j < i)
-    for (int b = b0; b < b1; ++b) a += (double)g[(long long)b * P + i] * (double)z[(long long)b * P + j];
-  part[(long long)blockIdx.y * PP + t] = a;
+  double a0 = 0.0, a1 = 0.0;
+  if (j < i) {
+    int b = b0;
+    for (; b + 2 <= b1; b += 2) {
+      a0 += (double)g[(long long)b * P + i] * (double)z[(long long)b * P + j];
+      a1 += (double)g[(long long)(b + 1) * P + i] * (double)z[(long long)(b + 1) * P + j];
+    }
+    for (; b < b1; ++b) a0 += (double)g[(long long)b * P + i] * (double)z[(long long)b * P + j];
+  }
+  part[(long long)blockIdx.y * PP + t] = a0 + a1;
 }
 
+// out[t] = sum over the splits of part[k][t]: 8 lanes per entry (32 entries per
+// workgroup), each summing every 8th split, then a fixed xor tree over the 8 lanes
 __global__ __launch_bounds__(256) void k_factor_grad_final(const double* __restrict__ part, int splits, long long PP,
                                                            float* __restrict__ out) {
-  const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
-  if (t >= PP) return;
+  const long long t = (long long)blockIdx.x * 32 + (threadIdx.x >> 3);
+  const int sub = threadIdx.x & 7;
   double a = 0.0;
-  for (int k = 0; k < splits; ++k) a += part[(long long)k * PP + t];
-  out[t] = (float)a;
+  if (t < PP)
+    for (int k = sub; k < splits; k += 8) a += part[(long long)k * PP + t];
+  a += __shfl_xor(a, 1, 8);
+  a += __shfl_xor(a, 2, 8);
+  a += __shfl_xor(a, 4, 8);
+  if (t < PP && sub == 0) out[t] = (float)a;
 }
 
 }  // namespace
@@ -1308,8 +1332,8 @@ hipError_t flow_elbo_backward(Flow& f, const float* z, const float* rw, int B, h
     const long long PP = (long long)f.P * f.P;
     hipLaunchKernelGGL(k_factor_grad_part, dim3(nblk(PP), FACTOR_SPLITS), dim3(T256), 0, st, s.g, z, B, f.P,
                        f.fpart);
-    hipLaunchKernelGGL(k_factor_grad_final, dim3(nblk(PP)), dim3(T256), 0, st, f.fpart, FACTOR_SPLITS, PP,
-                       f.grads + f.ofac);
+    hipLaunchKernelGGL(k_factor_grad_final, dim3((unsigned)((PP + 31) / 32)), dim3(T256), 0, st, f.fpart,
+                       FACTOR_SPLITS, PP, f.grads + f.ofac);
   }
   return hipGetLastError();
 }
