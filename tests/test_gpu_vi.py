@@ -139,3 +139,20 @@ def test_iwae_fit_bound_above_elbo():
     assert np.all(h >= s - 1e-9)
     assert h[-50:].mean() > h[:20].mean()
     assert r.elbo == "iwae" and r.iwae_samples == 16
+
+
+def test_fit_without_valid_particles_stops_and_keeps_weights():
+    """Every particle of a guide centred at v = -1e4 overflows the funnel's logjoint:
+    ut_flow_fit_elbo reports the first such iteration (as the reference throws there,
+    src/inference/vi.jl:762-764) and leaves the weights as they were, although the
+    iterations queue on the device without a host check in between."""
+    model = M.NealsFunnel(10, 3.0)
+    P = model.num_params
+    flow = ut.AffineCouplingFlow(P, 0, 256, seed=12)
+    mu = np.zeros(P)
+    mu[0] = -1e4
+    flow.set_base(mu, np.full(P, -1.0))
+    w0 = flow.weights.copy()
+    with pytest.raises(ValueError, match="at iteration 0"):
+        flow.fit_elbo(model, num_particles=256, num_iterations=20, learning_rate=1e-2, seed=13)
+    assert np.array_equal(flow.weights, w0)

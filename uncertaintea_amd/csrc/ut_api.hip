@@ -1869,30 +1869,39 @@ int ut_flow_fit_elbo(ut_workspace* ws, ut_flow* flow, int32_t num_iterations, do
   UT_HIP(hipMemsetAsync(f.adam_m, 0, (size_t)f.nparams * 4, st));
   UT_HIP(hipMemsetAsync(f.adam_v, 0, (size_t)f.nparams * 4, st));
   f.adam_t = 0;
-  double best_obj = -std::numeric_limits<double>::infinity();
-  double h[5];
+  // The bookkeeping (histories, best weights, the stop at an iteration without a valid
+  // particle) runs on the device, so the iterations queue without a host round trip;
+  // after a failed iteration the weights and the best copy stay as they were.
+  double* hist = nullptr;
+  ElboTrack* track = nullptr;
+  UT_HIP(buf.alloc(&hist, 3 * (size_t)std::max(num_iterations, 1)));
+  UT_HIP(buf.alloc(&track, 1));
+  ElboTrack t0{-std::numeric_limits<double>::infinity(), 0, -1, 0, 0};
+  UT_HIP(hipMemcpyAsync(track, &t0, sizeof(t0), hipMemcpyHostToDevice, st));
   for (int it = 0; it < num_iterations; ++it) {
     UT_HIP(flow_normal_fill(b.z, B, P, (uint32_t)seed, (uint32_t)it, st));
     UT_HIP(elbo_eval(ws, f, b, gradient_clip, iwae_group));
-    UT_HIP(hipMemcpyAsync(h, b.stats, sizeof(h), hipMemcpyDeviceToHost, st));
-    UT_HIP(hipStreamSynchronize(st));
-    if (!(h[0] > 0))
-      return fail(UT_E_ARG, std::string("ut_flow_fit_elbo: only non-finite ") +
-                                (iwae_group > 1 ? "importance weights" : "logjoint values or gradients") +
-                                " at iteration " + std::to_string(it));
-    if (elbo_history) elbo_history[it] = h[1];
-    if (standard_elbo_history) standard_elbo_history[it] = h[4];
-    if (grad_norm_history) grad_norm_history[it] = std::isfinite(gradient_clip) ? std::min(h[2], gradient_clip) : h[2];
-    if (h[1] > best_obj) {  // the weights this objective was evaluated at
-      best_obj = h[1];
-      UT_HIP(hipMemcpyAsync(best, f.params, (size_t)f.nparams * 4, hipMemcpyDeviceToDevice, st));
-    }
-    UT_HIP(flow_adam_ascent(f, b.stats, learning_rate, beta1, beta2, adam_epsilon, st));
+    UT_HIP(flow_elbo_track(f, b.stats, it, hist, track, best, st));
+    UT_HIP(flow_adam_ascent(f, b.stats, learning_rate, beta1, beta2, adam_epsilon, st, track));
     UT_HIP(flow_sync_weights(f, st));
   }
+  ElboTrack tr;
+  std::vector<double> h(3 * (size_t)std::max(num_iterations, 1));
+  UT_HIP(hipMemcpyAsync(&tr, track, sizeof(tr), hipMemcpyDeviceToHost, st));
+  if (num_iterations > 0) UT_HIP(hipMemcpyAsync(h.data(), hist, h.size() * 8, hipMemcpyDeviceToHost, st));
   if (best_weights) UT_HIP(hipMemcpyAsync(best_weights, best, (size_t)f.nparams * 4, hipMemcpyDeviceToHost, st));
   UT_HIP(hipStreamSynchronize(st));
-  if (best_elbo) *best_elbo = best_obj;
+  if (tr.failed)
+    return fail(UT_E_ARG, std::string("ut_flow_fit_elbo: only non-finite ") +
+                              (iwae_group > 1 ? "importance weights" : "logjoint values or gradients") +
+                              " at iteration " + std::to_string(tr.fail_iter));
+  for (int it = 0; it < num_iterations; ++it) {
+    if (elbo_history) elbo_history[it] = h[3 * (size_t)it];
+    if (standard_elbo_history) standard_elbo_history[it] = h[3 * (size_t)it + 1];
+    const double gn = h[3 * (size_t)it + 2];
+    if (grad_norm_history) grad_norm_history[it] = std::isfinite(gradient_clip) ? std::min(gn, gradient_clip) : gn;
+  }
+  if (best_elbo) *best_elbo = tr.best;
   return UT_OK;
 }
 

@@ -156,8 +156,21 @@ hipError_t flow_elbo_stats(Flow& f, const float* rw, const double* logw, int B, 
 // gstat[2g + 1] whether it had a valid particle; scales sc.g's rows by wts
 hipError_t flow_iwae_weights(Flow& f, const float* rw, const double* logw, int B, int K, float* wts, double* gstat,
                              hipStream_t st);
-// Adam ascent on f.grads scaled by stats[3]; no update when stats[0] == 0
+// Adam ascent on f.grads scaled by stats[3]; no update when stats[0] == 0 or, with
+// track, once track->failed is set
+struct ElboTrack {
+  double best;    // best objective so far (-inf at the start)
+  int failed;     // set at the first iteration without a valid particle (weight)
+  int fail_iter;  // that iteration
+  int take;       // this iteration's objective is a new best
+  int pad;
+};
 hipError_t flow_adam_ascent(Flow& f, const double* stats, double lr, double b1, double b2, double eps,
-                            hipStream_t st);
+                            hipStream_t st, const ElboTrack* track = nullptr);
+// per-iteration bookkeeping of an ELBO fit, on the device: hist[3 it + {0, 1, 2}] =
+// objective, standard ELBO, gradient norm; the best objective and the failure flag in
+// *track; then best_w = params when the objective is a new best (params as evaluated)
+hipError_t flow_elbo_track(Flow& f, const double* stats, int it, double* hist, ElboTrack* track, float* best_w,
+                           hipStream_t st);
 
 }  // namespace ut

@@ -830,15 +830,43 @@ __global__ void k_scale_rows(float* __restrict__ g, const float* __restrict__ wt
 // the gradient scaled by stats[3]
 __global__ void k_adam_ascent(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
                               float* __restrict__ v, long long n, float lr, float b1, float b2, float eps, float bc1,
-                              float bc2, const double* __restrict__ stats) {
+                              float bc2, const double* __restrict__ stats, const ElboTrack* __restrict__ track) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n || !(stats[0] > 0.0)) return;
+  if (i >= n || !(stats[0] > 0.0) || (track && track->failed)) return;
   const float gi = g[i] * (float)stats[3];
   const float mi = b1 * m[i] + (1.0f - b1) * gi;
   const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
   m[i] = mi;
   v[i] = vi;
   p[i] += lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
+}
+
+// one thread: record iteration `it`'s statistics, the failure flag and whether the
+// objective is a new best (the host loop's bookkeeping, kept on the device so the
+// fit runs without a per-iteration synchronisation)
+__global__ void k_elbo_track(const double* __restrict__ stats, int it, double* __restrict__ hist,
+                             ElboTrack* __restrict__ tr) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  tr->take = 0;
+  if (tr->failed) return;
+  if (!(stats[0] > 0.0)) {
+    tr->failed = 1;
+    tr->fail_iter = it;
+    return;
+  }
+  hist[3LL * it] = stats[1];
+  hist[3LL * it + 1] = stats[4];
+  hist[3LL * it + 2] = stats[2];
+  if (stats[1] > tr->best) {
+    tr->best = stats[1];
+    tr->take = 1;
+  }
+}
+
+__global__ void k_copy_if_best(float* __restrict__ dst, const float* __restrict__ src, long long n,
+                               const ElboTrack* __restrict__ tr) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && tr->take) dst[i] = src[i];
 }
 
 // ---- full-rank Gaussian guide: q = mu + e^ls z + F z, F strictly lower
@@ -1358,12 +1386,20 @@ hipError_t flow_iwae_weights(Flow& f, const float* rw, const double* logw, int B
 }
 
 hipError_t flow_adam_ascent(Flow& f, const double* stats, double lr, double b1, double b2, double eps,
-                            hipStream_t st) {
+                            hipStream_t st, const ElboTrack* track) {
   f.adam_t += 1;
   const double bc1 = 1.0 - pow(b1, (double)f.adam_t), bc2 = 1.0 - pow(b2, (double)f.adam_t);
   hipLaunchKernelGGL(k_adam_ascent, dim3(nblk(f.nparams)), dim3(T256), 0, st, f.params, f.grads, f.adam_m,
                      f.adam_v, f.nparams, (float)lr, (float)b1, (float)b2, (float)eps, (float)bc1, (float)bc2,
-                     stats);
+                     stats, track);
+  return hipGetLastError();
+}
+
+hipError_t flow_elbo_track(Flow& f, const double* stats, int it, double* hist, ElboTrack* track, float* best_w,
+                           hipStream_t st) {
+  hipLaunchKernelGGL(k_elbo_track, dim3(1), dim3(64), 0, st, stats, it, hist, track);
+  hipLaunchKernelGGL(k_copy_if_best, dim3(nblk(f.nparams)), dim3(T256), 0, st, best_w, f.params, f.nparams,
+                     (const ElboTrack*)track);
   return hipGetLastError();
 }
 
