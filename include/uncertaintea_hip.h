@@ -351,7 +351,11 @@ int ut_workspace_set_flow(ut_workspace* ws, ut_flow* flow);
  * (INFINITY: no clipping); either may be NULL.  The flow keeps the last weights;
  * best_weights (ut_flow_num_weights floats, or NULL) receives the weights of the
  * best ELBO seen and best_elbo that value.  UT_E_ARG when an iteration has no
- * valid particle (reference: ArgumentError "encountered only non-finite ...").
+ * valid particle (reference: ArgumentError "encountered only non-finite ..."): the
+ * iterations run without a host round trip, so the error is returned after the
+ * loop, names the first such iteration, and the weights stay as they were before it
+ * (as if the loop had stopped there).  Works for every guide: coupling flows, the
+ * mean-field base (num_layers = 0) and ut_flow_create_fullrank's full-rank guide.
  * The workspace must be UT_FLOAT32 with num_chains a multiple of 128.            */
 int ut_flow_fit_elbo(ut_workspace* ws, ut_flow* flow, int32_t num_iterations, double learning_rate, double beta1,
                      double beta2, double adam_epsilon, double gradient_clip, int32_t iwae_group, uint64_t seed,
