@@ -108,6 +108,30 @@ def test_fullrank_elbo_and_iwae_gradients_against_oracle():
     assert bound >= elbo - 1e-4
 
 
+def test_fullrank_large_p_gradient():
+    """P = 643 (the latent GP with N = 640 inputs): the factor gradient's row splits
+    drop from 64 to 40 so the binary64 partials stay within 128 MiB; forward and ELBO
+    gradient still match the oracle."""
+    say("full-rank guide at P=643")
+    from uncertaintea_amd import synthetic as S
+    rng = np.random.default_rng(51)
+    X, y, _ = S.gp_poisson_data(640, 2, seed=53)
+    model = ut.GaussianProcessPoisson(X, y)
+    P = model.num_params
+    assert P == 643
+    g = _guide(P, 52, fscale=0.01)
+    o = B.OracleFullRank.of(g)
+    z = rng.normal(size=(128, P)).astype(np.float32)
+    q, _ = g.forward(z)
+    rq, _ = o.forward(z)
+    elbo, grad, gq, valid = g.elbo_grad(model, z)
+    rg = o.elbo_grad(z, gq)
+    e_q, e_g = _close(q, rq), np.linalg.norm(grad - rg) / np.linalg.norm(rg)
+    say(f"  forward {e_q:.1e}; gradient vs oracle {e_g:.2e}")
+    assert valid.all() and e_q <= 1e-5 and e_g <= 1e-5
+    assert np.all(np.triu(grad[2 * P:].reshape(P, P)) == 0.0)
+
+
 def test_fullrank_invalid_particles_and_iwae_fit():
     """A particle pushed to v = -1e4 on the funnel overflows the logjoint: it is flagged
     invalid, carries no weight, and the gradient over the valid rows matches the oracle;

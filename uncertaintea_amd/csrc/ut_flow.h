@@ -100,7 +100,8 @@ struct Flow {
   // strictly lower P x P factor stored row-major at params + ofac
   bool factor = false;
   long long ofac = 0;
-  double* fpart = nullptr;  // FACTOR_SPLITS x P x P split partials of F's gradient
+  double* fpart = nullptr;  // fsplits x P x P split partials of F's gradient
+  int fsplits = 0;          // FACTOR_SPLITS, fewer for large P (the partials stay <= 128 MiB)
 };
 
 // factor = true: the full-rank guide (L must be 0), mu, ls and the strictly lower

@@ -1026,7 +1026,8 @@ hipError_t flow_alloc(Flow& f, int P, int L, int H, bool factor) {
   }
   f.nparams = off;
   hipError_t e = own(f, &f.params, (size_t)off);
-  if (e == hipSuccess && factor) e = own(f, &f.fpart, (size_t)FACTOR_SPLITS * P * P);
+  if (factor) f.fsplits = (int)std::max(1LL, std::min((long long)FACTOR_SPLITS, (16LL << 20) / ((long long)P * P)));
+  if (e == hipSuccess && factor) e = own(f, &f.fpart, (size_t)f.fsplits * P * P);
   if (e == hipSuccess && L > 0) e = own(f, &f.dfwd, (size_t)L);
   if (e == hipSuccess && L > 0) e = own(f, &f.dbwd, (size_t)L);
   if (e == hipSuccess) e = own(f, &f.grads, (size_t)off);
@@ -1358,10 +1359,10 @@ hipError_t flow_elbo_backward(Flow& f, const float* z, const float* rw, int B, h
   colsum(s.aux, B, f.P, f.grads + f.P, s.colsum, st);
   if (f.factor) {  // d/dF[i, j] = sum over rows of g_i z_j, j < i
     const long long PP = (long long)f.P * f.P;
-    hipLaunchKernelGGL(k_factor_grad_part, dim3(nblk(PP), FACTOR_SPLITS), dim3(T256), 0, st, s.g, z, B, f.P,
+    hipLaunchKernelGGL(k_factor_grad_part, dim3(nblk(PP), f.fsplits), dim3(T256), 0, st, s.g, z, B, f.P,
                        f.fpart);
     hipLaunchKernelGGL(k_factor_grad_final, dim3((unsigned)((PP + 31) / 32)), dim3(T256), 0, st, f.fpart,
-                       FACTOR_SPLITS, PP, f.grads + f.ofac);
+                       f.fsplits, PP, f.grads + f.ofac);
   }
   return hipGetLastError();
 }
