@@ -198,3 +198,23 @@ def test_fullrank_advi_correlated_logistic():
     e_m = np.max(np.abs(dh.reshape(-1, P).mean(axis=0) - d.mean(axis=0)) / sd_n)
     say(f"  preconditioned hmc: eps {float(h.step_size[0]):.3f}, rhat {rh:.4f}, max |mean - nuts| / sd {e_m:.3f}")
     assert rh < 1.02 and e_m < 0.1
+
+
+def test_fullrank_graph_replay_matches_eager(monkeypatch):
+    """Attached to a workspace, the full-rank guide's gradient (forward with F z, model
+    gradient, L^T g) is captured as a HIP graph and replayed at every leapfrog step:
+    whole shared-HMC and ChEES runs must equal the eager launches bit for bit."""
+    say("full-rank guide: graph replay vs eager")
+    model = M.NealsFunnel(10, 3.0)
+    g = _guide(model.num_params, 23, fscale=0.1)
+    runs = {}
+    for mode in ("0", "1"):
+        monkeypatch.setenv("UT_FLOW_GRAPHS", mode)
+        h = ut.batched_hmc(model, num_chains=128, num_samples=20, num_warmup=30, num_leapfrog_steps=6,
+                           per_chain_adaptation=False, flow=g, seed=24)
+        c = ut.batched_chees(model, num_chains=128, num_samples=20, num_warmup=30, flow=g, seed=25)
+        runs[mode] = (h, c)
+    for a, b in zip(runs["0"], runs["1"]):
+        assert np.array_equal(a.unconstrained_samples.view(np.uint32), b.unconstrained_samples.view(np.uint32))
+        assert np.array_equal(a.logjoint_values, b.logjoint_values)
+        assert np.array_equal(a.step_size, b.step_size)
