@@ -361,6 +361,17 @@ int ut_flow_fit_elbo(ut_workspace* ws, ut_flow* flow, int32_t num_iterations, do
                      double beta2, double adam_epsilon, double gradient_clip, int32_t iwae_group, uint64_t seed,
                      double* elbo_history, double* standard_elbo_history, double* grad_norm_history,
                      float* best_weights, double* best_elbo);
+/* Pointwise log-likelihood of the observed sites for draws in unconstrained space:
+ * the reference's pointwise_loglikelihood (src/inference/result_interface.jl:384-410,
+ * src/evaluator_pointwise.jl:342), the log_likelihood group of its four-argument
+ * to_arviz_dict (src/inference/export.jl:276-338).  out is num_draws x
+ * num_observations row-major (binary64); draws num_draws x P row-major.  Logistic
+ * plan: y_i ~ bernoullilogit(x_i . beta), observations in data order.
+ * Hierarchical plan: y ~ normal(mu + tau theta_raw_j, sigma_j), observations group by
+ * group in the order of group_offsets.  Funnel plan: no observations (size 0).
+ * Other plans: UT_E_UNSUPPORTED.                                                */
+int ut_pointwise_loglik_size(const ut_plan* plan, int64_t* num_observations);
+int ut_pointwise_loglik(const ut_plan* plan, const double* draws, int64_t num_draws, double* out);
 /* num_chains initial positions (num_chains x P, chain-major, unconstrained) as the
  * samplers draw them for `seed` before their non-finite re-draw loop: prior draws
  * (UT_INIT_PRIOR) or Uniform(-2, 2) (UT_INIT_UNIFORM) from Philox; host-only.

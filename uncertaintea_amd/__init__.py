@@ -4,7 +4,8 @@ from . import diagnostics, synthetic
 from ._lib import DeviceError, NonFiniteInitError, device_available, philox4x32, philox_fill
 from .device import (DeviceBatchedWorkspace, device_batched_logjoint, device_batched_logjoint_gradient,
                      device_batched_logjoint_gradient_inplace, device_batched_logjoint_inplace)
-from .export import parameter_names, posterior_array, to_arviz_dict, write_crossppl_run
+from .export import (parameter_names, pointwise_loglikelihood, posterior_array, to_arviz_dict,
+                     write_crossppl_run)
 from .flow import AffineCouplingFlow, FullRankGaussian
 from .inference import HMCChains, batched_chees, batched_hmc, batched_meads, batched_nuts
 from .models import (GaussianProcessPoisson, HierarchicalNormalNC, LogisticRegression, LogisticRegressionBF16,
@@ -13,7 +14,7 @@ from .vi import ADVIResult, batched_advi, variational_covariance, variational_me
 
 __all__ = [
     "ADVIResult", "AffineCouplingFlow", "FullRankGaussian", "batched_advi", "variational_covariance", "variational_mean",
-    "variational_samples", "parameter_names", "posterior_array", "to_arviz_dict", "write_crossppl_run",
+    "variational_samples", "parameter_names", "pointwise_loglikelihood", "posterior_array", "to_arviz_dict", "write_crossppl_run",
     "DeviceBatchedWorkspace", "GaussianProcessPoisson", "DeviceError", "HMCChains", "HierarchicalNormalNC", "LogisticRegression",
     "LogisticRegressionBF16", "NealsFunnel", "NonFiniteInitError", "TeaModel", "batched_chees", "batched_hmc", "batched_meads", "batched_nuts", "device_available",
     "device_batched_logjoint", "device_batched_logjoint_gradient", "device_batched_logjoint_gradient_inplace",

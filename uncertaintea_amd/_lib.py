@@ -33,7 +33,7 @@ EXPORTED = (
     "ut_workspace_create", "ut_workspace_destroy",
     "ut_logjoint", "ut_logjoint_gradient",
     "ut_sampler_opts_default", "ut_nuts_run", "ut_hmc_run", "ut_chees_opts_default", "ut_chees_run",
-    "ut_hmc_shared_run", "ut_meads_opts_default", "ut_meads_run", "ut_flow_create", "ut_flow_create_fullrank", "ut_flow_destroy", "ut_flow_num_weights", "ut_flow_layer_layout",
+    "ut_hmc_shared_run", "ut_meads_opts_default", "ut_meads_run", "ut_flow_create", "ut_flow_create_fullrank", "ut_pointwise_loglik_size", "ut_pointwise_loglik", "ut_flow_destroy", "ut_flow_num_weights", "ut_flow_layer_layout",
     "ut_flow_get_weights", "ut_flow_set_weights", "ut_flow_set_base", "ut_flow_train", "ut_flow_forward",
     "ut_flow_inverse", "ut_flow_vjp", "ut_flow_loss_grad", "ut_workspace_set_flow", "ut_flow_fit_elbo",
     "ut_flow_elbo_grad", "ut_initial_params",
@@ -160,6 +160,8 @@ def lib() -> ct.CDLL:
         "ut_meads_run": (ct.c_int, [vp, ct.POINTER(MeadsOpts), ct.POINTER(ChainOut), vp]),
         "ut_flow_create": (ct.c_int, [i32, i32, i32, dbl, ct.c_uint64, pp]),
         "ut_flow_create_fullrank": (ct.c_int, [i32, pp]),
+        "ut_pointwise_loglik_size": (ct.c_int, [vp, ct.POINTER(i64)]),
+        "ut_pointwise_loglik": (ct.c_int, [vp, vp, i64, vp]),
         "ut_flow_destroy": (ct.c_int, [vp]),
         "ut_flow_num_weights": (ct.c_int, [vp, ct.POINTER(i64)]),
         "ut_flow_layer_layout": (ct.c_int, [vp, i32, ct.POINTER(i32), ct.POINTER(i32), vp]),

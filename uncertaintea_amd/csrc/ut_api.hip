@@ -1829,6 +1829,122 @@ int ut_flow_elbo_grad(ut_workspace* ws, ut_flow* flow, const float* z, int32_t i
   return UT_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Pointwise log-likelihood of the observed sites (the reference's
+// pointwise_loglikelihood, src/evaluator_pointwise.jl:342, result_interface.jl:384-410):
+// one thread per (draw, observation), binary64, draws in unconstrained space.
+// y_i ~ bernoullilogit(x_i . beta): y eta - log(1 + e^eta)
+__global__ __launch_bounds__(256) void k_ll_logistic(const double* __restrict__ draws, long long S, int D,
+                                                     const double* __restrict__ xT, const double* __restrict__ yv,
+                                                     long long n, double* __restrict__ out) {
+  const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= S * n) return;
+  const long long s = t / n, i = t - s * n;
+  const double* b = draws + s * D;
+  double eta = 0.0;
+  for (int d = 0; d < D; ++d) eta += xT[(long long)d * n + i] * b[d];
+  const double softplus = eta > 0.0 ? eta + log1p(exp(-eta)) : log1p(exp(eta));
+  out[t] = yv[i] * eta - softplus;
+}
+
+// y ~ normal(mu + e^{log tau} theta_raw_j, sigma_j) for observation k = (group grp[k],
+// row idx[k] of the plan's lane-padded y); inv_var[j] = 1 / sigma_j^2
+__global__ __launch_bounds__(256) void k_ll_hier(const double* __restrict__ draws, long long S, int J,
+                                                 const double* __restrict__ ypad, const double* __restrict__ inv_var,
+                                                 const int* __restrict__ grp, const int* __restrict__ idx,
+                                                 long long n, double* __restrict__ out) {
+  const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= S * n) return;
+  const long long s = t / n, k = t - s * n;
+  const int j = grp[k];
+  const double* th = draws + s * (J + 2);
+  const double theta = th[0] + exp(th[1]) * th[2 + j];
+  const double r = ypad[(long long)idx[k] * WAVE + j] - theta;
+  const double iv = inv_var[j];
+  out[t] = -HALF_LOG_2PI + 0.5 * log(iv) - 0.5 * r * r * iv;
+}
+
+// observation count of a plan: logistic n, hierarchical sum n_j, funnel 0
+int ll_count(const ut_plan* plan, std::vector<int>* nobs_out, int64_t* n) {
+  const DevModelDesc& m = plan->dev;
+  switch (m.kind) {
+    case MODEL_LOGISTIC: *n = m.n; return UT_OK;
+    case MODEL_FUNNEL: *n = 0; return UT_OK;
+    case MODEL_HIER: {
+      std::vector<int> nobs(WAVE, 0);
+      UT_HIP(hipMemcpy(nobs.data(), m.nobs, WAVE * sizeof(int), hipMemcpyDeviceToHost));
+      int64_t total = 0;
+      for (int j = 0; j < m.J; ++j) total += nobs[j];
+      if (nobs_out) *nobs_out = nobs;
+      *n = total;
+      return UT_OK;
+    }
+    default:
+      return fail(UT_E_UNSUPPORTED, "ut_pointwise_loglik: the pointwise log-likelihood is built for the "
+                                    "logistic, hierarchical and funnel plans");
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int ut_pointwise_loglik_size(const ut_plan* plan, int64_t* num_observations) {
+  if (!plan || !num_observations) return fail(UT_E_ARG, "ut_pointwise_loglik_size: null pointer");
+  if (!ut_device_available()) return fail(UT_E_HIP, "ut_pointwise_loglik_size: no HIP device visible");
+  return ll_count(plan, nullptr, num_observations);
+}
+
+int ut_pointwise_loglik(const ut_plan* plan, const double* draws, int64_t num_draws, double* out) {
+  if (!plan || (num_draws > 0 && (!draws || !out))) return fail(UT_E_ARG, "ut_pointwise_loglik: null pointer");
+  if (num_draws < 0) return fail(UT_E_DIM, "ut_pointwise_loglik: num_draws must be >= 0");
+  if (!ut_device_available()) return fail(UT_E_HIP, "ut_pointwise_loglik: no HIP device visible");
+  const DevModelDesc& m = plan->dev;
+  std::vector<int> nobs;
+  int64_t n = 0;
+  int rc = ll_count(plan, &nobs, &n);
+  if (rc) return rc;
+  if (n == 0 || num_draws == 0) return UT_OK;
+  const int P = m.P;
+  DeviceBuffers buf;
+  int *grp = nullptr, *idx = nullptr;
+  if (m.kind == MODEL_HIER) {  // observation order: group by group, rows in data order
+    std::vector<int> hg((size_t)n), hi((size_t)n);
+    size_t k = 0;
+    for (int j = 0; j < m.J; ++j)
+      for (int i = 0; i < nobs[j]; ++i, ++k) {
+        hg[k] = j;
+        hi[k] = i;
+      }
+    UT_HIP(buf.alloc(&grp, (size_t)n));
+    UT_HIP(buf.alloc(&idx, (size_t)n));
+    UT_HIP(hipMemcpy(grp, hg.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice));
+    UT_HIP(hipMemcpy(idx, hi.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice));
+  }
+  // draws in chunks so the device output stays within 512 MiB
+  const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(num_draws, (int64_t(1) << 26) / n));
+  double *dd = nullptr, *dout = nullptr;
+  UT_HIP(buf.alloc(&dd, (size_t)chunk * P));
+  UT_HIP(buf.alloc(&dout, (size_t)chunk * n));
+  for (int64_t s0 = 0; s0 < num_draws; s0 += chunk) {
+    const int64_t S = std::min(chunk, num_draws - s0);
+    UT_HIP(hipMemcpy(dd, draws + s0 * P, (size_t)S * P * 8, hipMemcpyHostToDevice));
+    const unsigned blocks = (unsigned)((S * n + 255) / 256);
+    if (m.kind == MODEL_LOGISTIC)
+      hipLaunchKernelGGL(k_ll_logistic, dim3(blocks), dim3(256), 0, 0, dd, (long long)S, m.D, m.xT_d, m.yv_d,
+                         (long long)n, dout);
+    else
+      hipLaunchKernelGGL(k_ll_hier, dim3(blocks), dim3(256), 0, 0, dd, (long long)S, m.J, m.y_d, m.lane_d + 2 * WAVE,
+                         grp, idx, (long long)n, dout);
+    UT_HIP(hipGetLastError());
+    UT_HIP(hipMemcpy(out + s0 * n, dout, (size_t)S * n * 8, hipMemcpyDeviceToHost));
+  }
+  return UT_OK;
+}
+
 int ut_initial_params(const ut_plan* plan, int32_t strategy, uint64_t seed, int32_t num_chains, double* out) {
   if (!plan || !out) return fail(UT_E_ARG, "ut_initial_params: null pointer");
   if (num_chains <= 0) return fail(UT_E_DIM, "ut_initial_params: num_chains must be positive");

@@ -8,12 +8,14 @@ file and the run's settings from a JSON sidecar) consume device runs directly.
 """
 from __future__ import annotations
 
+import ctypes as ct
 import json
 import os
 import re
 
 import numpy as np
 
+from . import _lib as L
 from .inference import HMCChains
 
 _LAYOUTS = ("draw_chain", "chain_draw")
@@ -49,13 +51,34 @@ def _per_draw(chains: HMCChains, a) -> np.ndarray:
     return np.ascontiguousarray(a.reshape(C, S).T)
 
 
-def to_arviz_dict(chains: HMCChains, layout: str = "draw_chain", flatten_vectors: bool = True) -> dict:
+def pointwise_loglikelihood(chains: HMCChains) -> np.ndarray:
+    """(num_chains * num_samples, num_observations) log-likelihood of every observed
+    site at every draw, rows chain-major (chain 1's draws, then chain 2's, ...): the
+    reference's pointwise_loglikelihood(model, args, constraints, chains)
+    (src/inference/result_interface.jl:384-410), computed on the device
+    (ut_pointwise_loglik).  Observations: the logistic model's y in data order, the
+    hierarchical model's y group by group; the funnel has none."""
+    model = chains.model
+    n = ct.c_int64()
+    L.check(L.lib().ut_pointwise_loglik_size(model.plan, ct.byref(n)))
+    d = np.ascontiguousarray(np.asarray(chains.unconstrained_samples, dtype=np.float64).reshape(-1, model.num_params))
+    out = np.empty((d.shape[0], int(n.value)), dtype=np.float64)
+    if out.size:
+        L.check(L.lib().ut_pointwise_loglik(model.plan, L.ptr(d), d.shape[0], L.ptr(out)))
+    return out
+
+
+def to_arviz_dict(chains: HMCChains, layout: str = "draw_chain", flatten_vectors: bool = True,
+                  log_likelihood: bool = False) -> dict:
     """ArviZ-style nested dictionary (export.jl:187-272): "posterior" (one array per
     constrained parameter, or per vector variable with a trailing component axis
     when flatten_vectors is False, with "coords" / "dims" in the from_dict
     convention), "sample_stats" (diverging, energy, tree_depth, acceptance_rate,
     lp, step_size, n_steps) and "attrs".  layout "chain_draw" gives the
-    (chain, draw) order Python ArviZ expects."""
+    (chain, draw) order Python ArviZ expects.  log_likelihood=True adds the
+    "log_likelihood" group of the reference's four-argument form (export.jl:276-338):
+    one "y[k]" matrix per observation, or with flatten_vectors=False one "y" array
+    with a trailing observation axis (dims "y_dim_0", coords 1..N)."""
     if layout not in _LAYOUTS:
         raise ValueError(f"to_arviz_dict layout must be :draw_chain or :chain_draw, got :{layout}")
     if chains.num_chains == 0:
@@ -94,6 +117,19 @@ def to_arviz_dict(chains: HMCChains, layout: str = "draw_chain", flatten_vectors
            "sample_stats": {k: _layout(v, layout) for k, v in stats.items()},
            "attrs": {"layout": layout, "inference_library": "UncertainTea",
                      "backend": "uncertaintea_amd (MI355X HIP)", "sampler": chains.sampler}}
+    if log_likelihood:
+        ll = pointwise_loglikelihood(chains)
+        C, S = chains.num_chains, chains.num_samples
+        cube = np.ascontiguousarray(np.transpose(ll.reshape(C, S, -1), (1, 0, 2)))  # (S, C, N)
+        group: dict = {}
+        if flatten_vectors:
+            for k in range(cube.shape[2]):
+                group[f"y[{k + 1}]"] = _layout(cube[:, :, k], layout)
+        elif cube.shape[2] > 0:
+            group["y"] = _layout(cube, layout)
+            dims["y"] = ["y_dim_0"]
+            coords["y_dim_0"] = list(range(1, cube.shape[2] + 1))
+        out["log_likelihood"] = group
     if not flatten_vectors:
         out["coords"] = coords
         out["dims"] = dims
