@@ -366,7 +366,8 @@ int ut_flow_fit_elbo(ut_workspace* ws, ut_flow* flow, int32_t num_iterations, do
  * src/evaluator_pointwise.jl:342), the log_likelihood group of its four-argument
  * to_arviz_dict (src/inference/export.jl:276-338).  out is num_draws x
  * num_observations row-major (binary64); draws num_draws x P row-major.  Logistic
- * plan: y_i ~ bernoullilogit(x_i . beta), observations in data order.
+ * plans (float and bf16, the latter with its bf16 X): y_i ~ bernoullilogit(x_i . beta),
+ * observations in data order.
  * Hierarchical plan: y ~ normal(mu + tau theta_raw_j, sigma_j), observations group by
  * group in the order of group_offsets.  Funnel plan: no observations (size 0).
  * Other plans: UT_E_UNSUPPORTED.                                                */

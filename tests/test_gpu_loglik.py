@@ -48,6 +48,24 @@ def test_logistic_pointwise_against_scipy_and_logjoint():
     assert ll.shape == (300, 300) and e <= 1e-12 and e_sum <= 1e-10
 
 
+def test_bf16_glm_pointwise_against_scipy_and_logjoint():
+    """The matrix-core GLM plan (D = 256): the log-likelihood uses its bf16 X, which the
+    model holds widened as x_model."""
+    rng = np.random.default_rng(4)
+    x = (rng.normal(size=(256, 256)) / 16).astype(np.float32)
+    y = (rng.random(256) < 0.5).astype(np.float32)
+    model = ut.LogisticRegressionBF16(x, y)
+    d = rng.normal(size=(2 * 64, 256))
+    ll = ut.pointwise_loglikelihood(_Draws(model, d, 2))
+    eta = d @ model.x_model.astype(np.float64).T
+    ref = y[None, :] * log_expit(eta) + (1 - y[None, :]) * log_expit(-eta)
+    lj, _ = B.OracleModel(model).logjoint_grad_f64(d.T)
+    e = _close(ll, ref)
+    e_sum = _close(ll.sum(axis=1) + stats.norm.logpdf(d).sum(axis=1), lj)
+    print(f"bf16 glm: vs scipy {e:.1e}; sum + prior vs oracle logjoint {e_sum:.1e}", flush=True)
+    assert ll.shape == (128, 256) and e <= 1e-12 and e_sum <= 1e-10
+
+
 def test_hierarchical_pointwise_against_scipy_and_logjoint():
     rng = np.random.default_rng(2)
     sizes = [3, 5, 1, 7, 4, 2, 6, 3]

@@ -1850,6 +1850,21 @@ __global__ __launch_bounds__(256) void k_ll_logistic(const double* __restrict__ 
   out[t] = yv[i] * eta - softplus;
 }
 
+// the same for the bf16 plan: X is the plan's bf16 matrix (transposed, Xt[d n + i]),
+// widened exactly, as the matrix-core gradient evaluates it
+__global__ __launch_bounds__(256) void k_ll_glm_bf16(const double* __restrict__ draws, long long S, int D,
+                                                     const uint16_t* __restrict__ Xt, const float* __restrict__ yv,
+                                                     long long n, double* __restrict__ out) {
+  const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= S * n) return;
+  const long long s = t / n, i = t - s * n;
+  const double* b = draws + s * D;
+  double eta = 0.0;
+  for (int d = 0; d < D; ++d) eta += (double)__uint_as_float((uint32_t)Xt[(long long)d * n + i] << 16) * b[d];
+  const double softplus = eta > 0.0 ? eta + log1p(exp(-eta)) : log1p(exp(eta));
+  out[t] = (double)yv[i] * eta - softplus;
+}
+
 // y ~ normal(mu + e^{log tau} theta_raw_j, sigma_j) for observation k = (group grp[k],
 // row idx[k] of the plan's lane-padded y); inv_var[j] = 1 / sigma_j^2
 __global__ __launch_bounds__(256) void k_ll_hier(const double* __restrict__ draws, long long S, int J,
@@ -1872,6 +1887,7 @@ int ll_count(const ut_plan* plan, std::vector<int>* nobs_out, int64_t* n) {
   const DevModelDesc& m = plan->dev;
   switch (m.kind) {
     case MODEL_LOGISTIC: *n = m.n; return UT_OK;
+    case MODEL_GLM_MFMA: *n = plan->glm.N; return UT_OK;
     case MODEL_FUNNEL: *n = 0; return UT_OK;
     case MODEL_HIER: {
       std::vector<int> nobs(WAVE, 0);
@@ -1884,7 +1900,7 @@ int ll_count(const ut_plan* plan, std::vector<int>* nobs_out, int64_t* n) {
     }
     default:
       return fail(UT_E_UNSUPPORTED, "ut_pointwise_loglik: the pointwise log-likelihood is built for the "
-                                    "logistic, hierarchical and funnel plans");
+                                    "logistic (float and bf16), hierarchical and funnel plans");
   }
 }
 
@@ -1933,7 +1949,10 @@ int ut_pointwise_loglik(const ut_plan* plan, const double* draws, int64_t num_dr
     const int64_t S = std::min(chunk, num_draws - s0);
     UT_HIP(hipMemcpy(dd, draws + s0 * P, (size_t)S * P * 8, hipMemcpyHostToDevice));
     const unsigned blocks = (unsigned)((S * n + 255) / 256);
-    if (m.kind == MODEL_LOGISTIC)
+    if (m.kind == MODEL_GLM_MFMA)
+      hipLaunchKernelGGL(k_ll_glm_bf16, dim3(blocks), dim3(256), 0, 0, dd, (long long)S, plan->glm.D, plan->glm.Xt,
+                         plan->glm.y, (long long)n, dout);
+    else if (m.kind == MODEL_LOGISTIC)
       hipLaunchKernelGGL(k_ll_logistic, dim3(blocks), dim3(256), 0, 0, dd, (long long)S, m.D, m.xT_d, m.yv_d,
                          (long long)n, dout);
     else
