@@ -369,8 +369,9 @@ int ut_flow_fit_elbo(ut_workspace* ws, ut_flow* flow, int32_t num_iterations, do
  * plans (float and bf16, the latter with its bf16 X): y_i ~ bernoullilogit(x_i . beta),
  * observations in data order.
  * Hierarchical plan: y ~ normal(mu + tau theta_raw_j, sigma_j), observations group by
- * group in the order of group_offsets.  Funnel plan: no observations (size 0).
- * Other plans: UT_E_UNSUPPORTED.                                                */
+ * group in the order of group_offsets.  GP plan: y_i ~ poisson(exp(f_i)), f = L z
+ * from the binary64 factorisation at binary32 positions (as the model's evaluator
+ * takes them).  Funnel plan: no observations (size 0).                           */
 int ut_pointwise_loglik_size(const ut_plan* plan, int64_t* num_observations);
 int ut_pointwise_loglik(const ut_plan* plan, const double* draws, int64_t num_draws, double* out);
 /* num_chains initial positions (num_chains x P, chain-major, unconstrained) as the

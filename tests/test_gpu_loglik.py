@@ -88,14 +88,30 @@ def test_hierarchical_pointwise_against_scipy_and_logjoint():
     assert ll.shape == (200, off[-1]) and e <= 1e-12 and e_sum <= 1e-10
 
 
-def test_funnel_has_no_observations_and_gp_is_unsupported():
+def test_funnel_has_no_observations():
     model = ut.NealsFunnel(4, 3.0)
     ll = ut.pointwise_loglikelihood(_Draws(model, np.zeros((10, 5)), 2))
     assert ll.shape == (10, 0)
-    X, yc, _ = S.gp_poisson_data(64, 2, seed=3)
+
+
+def test_gp_pointwise_against_scipy_and_logjoint():
+    """The latent GP (N = 96, three tiles): f = chol(K) z by numpy, poisson logpmf by
+    scipy, at the binary32-rounded draws the device evaluates."""
+    X, yc, _ = S.gp_poisson_data(96, 2, seed=3)
     gp = ut.GaussianProcessPoisson(X, yc)
-    with pytest.raises(ValueError, match="pointwise"):
-        ut.pointwise_loglikelihood(_Draws(gp, np.zeros((2, gp.num_params)), 1))
+    P = gp.num_params
+    rng = np.random.default_rng(5)
+    d = np.concatenate([rng.normal(size=(40, 3)) * 0.3, rng.normal(size=(40, P - 3))], axis=1)
+    d = d.astype(np.float32).astype(np.float64)
+    ll = ut.pointwise_loglikelihood(_Draws(gp, d, 2))
+    ref = np.array([stats.poisson.logpmf(yc, np.exp(np.linalg.cholesky(
+        S.gp_kernel(X, np.exp(r[:2]), np.exp(r[2]), 1e-2)) @ r[3:])) for r in d])
+    prior = (stats.norm.logpdf(d[:, :2], 0.0, 0.5).sum(axis=1) + stats.halfnorm.logpdf(np.exp(d[:, 2]), scale=1.0)
+             + d[:, 2] + stats.norm.logpdf(d[:, 3:]).sum(axis=1))
+    lj = B.OracleModel(gp).gp_eval(d.T)[0]
+    e, e_sum = _close(ll, ref), _close(ll.sum(axis=1) + prior, lj)
+    print(f"gp: vs scipy/numpy {e:.1e}; sum + prior vs oracle logjoint {e_sum:.1e}", flush=True)
+    assert ll.shape == (40, 96) and e <= 1e-10 and e_sum <= 1e-10
 
 
 def test_arviz_log_likelihood_group():

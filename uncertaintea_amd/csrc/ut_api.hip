@@ -1865,6 +1865,20 @@ __global__ __launch_bounds__(256) void k_ll_glm_bf16(const double* __restrict__ 
   out[t] = (double)yv[i] * eta - softplus;
 }
 
+// latent GP: y_i ~ poisson(e^{f_i}) with f = L z summed from the factorisation's
+// per-tile-column partial products (value component, tile columns kb <= i's tile, in
+// the order the gradient kernel sums them)
+__global__ __launch_bounds__(256) void k_ll_gp(const double* __restrict__ pf, int nb, int T, int N,
+                                               const double* __restrict__ y, long long S, double* __restrict__ out) {
+  const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= S * N) return;
+  const long long s = t / N;
+  const int i = (int)(t - s * N), ib = i / GP_TILE;
+  double f = 0.0;
+  for (int kb = 0; kb <= ib; ++kb) f += pf[((size_t)s * nb + kb) * (T + 1) * N + i];
+  out[t] = y[i] * f - exp(f) - lgamma(y[i] + 1.0);
+}
+
 // y ~ normal(mu + e^{log tau} theta_raw_j, sigma_j) for observation k = (group grp[k],
 // row idx[k] of the plan's lane-padded y); inv_var[j] = 1 / sigma_j^2
 __global__ __launch_bounds__(256) void k_ll_hier(const double* __restrict__ draws, long long S, int J,
@@ -1888,6 +1902,7 @@ int ll_count(const ut_plan* plan, std::vector<int>* nobs_out, int64_t* n) {
   switch (m.kind) {
     case MODEL_LOGISTIC: *n = m.n; return UT_OK;
     case MODEL_GLM_MFMA: *n = plan->glm.N; return UT_OK;
+    case MODEL_GP: *n = plan->gp.N; return UT_OK;
     case MODEL_FUNNEL: *n = 0; return UT_OK;
     case MODEL_HIER: {
       std::vector<int> nobs(WAVE, 0);
@@ -1900,7 +1915,7 @@ int ll_count(const ut_plan* plan, std::vector<int>* nobs_out, int64_t* n) {
     }
     default:
       return fail(UT_E_UNSUPPORTED, "ut_pointwise_loglik: the pointwise log-likelihood is built for the "
-                                    "logistic (float and bf16), hierarchical and funnel plans");
+                                    "logistic (float and bf16), hierarchical, funnel and GP plans");
   }
 }
 
@@ -1926,6 +1941,34 @@ int ut_pointwise_loglik(const ut_plan* plan, const double* draws, int64_t num_dr
   if (n == 0 || num_draws == 0) return UT_OK;
   const int P = m.P;
   DeviceBuffers buf;
+  if (m.kind == MODEL_GP) {  // factorise K per draw (binary32 positions, as the model's evaluator takes them)
+    const GpPlan& g = plan->gp;
+    const int T = g.D + 1, nb = g.N / GP_TILE;
+    const size_t per_draw = ((size_t)(T + 1) * gp_tiles(g.N) * GP_TILE * GP_TILE + (size_t)nb * (T + 1) * g.N) * 8;
+    const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>({num_draws, 128, (int64_t)((size_t(1) << 30) / per_draw)}));
+    GpScratch gs;
+    UT_HIP(gp_scratch_alloc(gs, g, (int)chunk));
+    struct Release {
+      GpScratch& s;
+      ~Release() { gp_scratch_free(s); }
+    } release{gs};
+    float* dq = nullptr;
+    double* dout = nullptr;
+    UT_HIP(buf.alloc(&dq, (size_t)chunk * P));
+    UT_HIP(buf.alloc(&dout, (size_t)chunk * n));
+    std::vector<float> hq((size_t)chunk * P);
+    for (int64_t s0 = 0; s0 < num_draws; s0 += chunk) {
+      const int64_t S = std::min(chunk, num_draws - s0);
+      for (size_t k = 0; k < (size_t)S * P; ++k) hq[k] = (float)draws[(size_t)s0 * P + k];
+      UT_HIP(hipMemcpy(dq, hq.data(), (size_t)S * P * 4, hipMemcpyHostToDevice));
+      UT_HIP(launch_gp_factor(g, dq, P, nullptr, (int)S, gs, 0));
+      hipLaunchKernelGGL(k_ll_gp, dim3((unsigned)((S * n + 255) / 256)), dim3(256), 0, 0, gs.pf, nb, T, g.N, g.y,
+                         (long long)S, dout);
+      UT_HIP(hipGetLastError());
+      UT_HIP(hipMemcpy(out + s0 * n, dout, (size_t)S * n * 8, hipMemcpyDeviceToHost));
+    }
+    return UT_OK;
+  }
   int *grp = nullptr, *idx = nullptr;
   if (m.kind == MODEL_HIER) {  // observation order: group by group, rows in data order
     std::vector<int> hg((size_t)n), hi((size_t)n);
