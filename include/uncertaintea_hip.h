@@ -155,6 +155,31 @@ int ut_logjoint(ut_workspace* ws, const void* params, void* values);
 int ut_logjoint_gradient(ut_workspace* ws, const void* params, void* values,
                          void* gradients);
 
+/* Allocation-free density and gradient.  No counterpart in the reference: this is
+ * for callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  It computes what
+ * ut_logjoint_gradient computes for a UT_FLOAT32 workspace of `plan` with
+ * num_chains chains, with the same kernels, but takes DEVICE pointers and calls no
+ * hipMalloc/hipFree and no allocating library routine: params (P x C,
+ * column-major, fp32), values (C) and gradients (P x C) are the caller's device
+ * buffers, and the matrix-core GLM plan's scratch (bf16 position planes and
+ * per-split partials) is a file-scope device array of UT_NOALLOC_SCRATCH_BYTES
+ * that each GPU holds until the process ends.  The work is enqueued on `stream`
+ * (a hipStream_t, NULL for the null stream) and the call returns without waiting:
+ * results are ready once that stream has reached this point.  Every call on one
+ * GPU shares that GPU's scratch, so calls that overlap on one GPU (on different
+ * streams, or from several threads) race; the scratch keeps the last call's
+ * contents, and nothing in it needs clearing (the kernels write every byte they
+ * read).  ut_noalloc_scratch_bytes returns the scratch a call needs (0 for the
+ * wavefront-per-chain plans, which use none; -1 for the GP plan, which is
+ * rejected with UT_E_UNSUPPORTED because its factorisation scratch is about
+ * 4.5 MiB per chain); a call needing more than UT_NOALLOC_SCRATCH_BYTES fails with
+ * UT_E_DIM (at D = 256: 4096 chains need 36.3 MiB).                            */
+#define UT_NOALLOC_SCRATCH_BYTES (48u << 20)
+int64_t ut_noalloc_scratch_bytes(const ut_plan* plan, int32_t num_chains);
+int ut_logjoint_gradient_noalloc(const ut_plan* plan, int32_t num_chains, const float* params, float* values,
+                                 float* gradients, void* stream);
+
 /* ---- samplers ------------------------------------------------------------------
  * Options mirror the keyword arguments of batched_nuts / batched_hmc
  * (src/inference/api_batched_nuts.jl:22-51, api_batched_hmc.jl:16-39).          */

@@ -31,7 +31,7 @@ EXPORTED = (
     "ut_plan_gp_poisson", "ut_gp_factor",
     "ut_plan_destroy", "ut_plan_num_params",
     "ut_workspace_create", "ut_workspace_destroy",
-    "ut_logjoint", "ut_logjoint_gradient",
+    "ut_logjoint", "ut_logjoint_gradient", "ut_noalloc_scratch_bytes", "ut_logjoint_gradient_noalloc",
     "ut_sampler_opts_default", "ut_nuts_run", "ut_hmc_run", "ut_chees_opts_default", "ut_chees_run",
     "ut_hmc_shared_run", "ut_meads_opts_default", "ut_meads_run", "ut_flow_create", "ut_flow_create_fullrank", "ut_pointwise_loglik_size", "ut_pointwise_loglik", "ut_flow_destroy", "ut_flow_num_weights", "ut_flow_layer_layout",
     "ut_flow_get_weights", "ut_flow_set_weights", "ut_flow_set_base", "ut_flow_train", "ut_flow_forward",
@@ -150,6 +150,8 @@ def lib() -> ct.CDLL:
         "ut_workspace_destroy": (ct.c_int, [vp]),
         "ut_logjoint": (ct.c_int, [vp, vp, vp]),
         "ut_logjoint_gradient": (ct.c_int, [vp, vp, vp, vp]),
+        "ut_noalloc_scratch_bytes": (i64, [vp, i32]),
+        "ut_logjoint_gradient_noalloc": (ct.c_int, [vp, i32, vp, vp, vp, vp]),
         "ut_sampler_opts_default": (None, [ct.POINTER(SamplerOpts)]),
         "ut_nuts_run": (ct.c_int, [vp, ct.POINTER(SamplerOpts), ct.POINTER(ChainOut)]),
         "ut_hmc_run": (ct.c_int, [vp, ct.POINTER(SamplerOpts), ct.POINTER(ChainOut)]),

@@ -16,6 +16,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <limits>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -157,6 +158,40 @@ struct ut_workspace {
     if (copy_stream) (void)hipStreamDestroy(copy_stream);
   }
 };
+
+// ------------------------------------------------------- allocation-free density
+// Scratch of ut_logjoint_gradient_noalloc: one file-scope device array per GPU,
+// held until the process ends.  Host code never passes `scratch` itself to a
+// kernel (in host code the name is a stand-in, not a device address): the device
+// address comes from hipGetSymbolAddress, looked up once per device with that
+// device current, and only that address is used.
+static __device__ __align__(256) unsigned char scratch[UT_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// device address of the current device's copy of `scratch`
+int noalloc_scratch(void** out) {
+  int dev = -1;
+  UT_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(UT_E_HIP, "ut_logjoint_gradient_noalloc: device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(UT_E_HIP, std::string("ut_logjoint_gradient_noalloc: hipGetSymbolAddress(scratch): ") +
+                                hipGetErrorString(e));
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return UT_OK;
+}
+
+size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+}  // namespace
 
 extern "C" {
 
@@ -544,6 +579,59 @@ int ut_logjoint(ut_workspace* ws, const void* params, void* values) {
 int ut_logjoint_gradient(ut_workspace* ws, const void* params, void* values, void* gradients) {
   if (!gradients) return fail(UT_E_ARG, "ut_logjoint_gradient: null gradients");
   return density_call(ws, params, values, gradients);
+}
+
+int64_t ut_noalloc_scratch_bytes(const ut_plan* plan, int32_t num_chains) {
+  if (!plan || num_chains <= 0) return -1;
+  if (plan->dev.kind == MODEL_GP) return -1;
+  if (plan->dev.kind != MODEL_GLM_MFMA) return 0;
+  const size_t PC = (size_t)plan->P * (size_t)num_chains;
+  const size_t S = (size_t)glm_splits(plan->glm.N, num_chains);
+  return (int64_t)(2 * align256(PC * sizeof(uint16_t)) + align256(S * PC * sizeof(float)) +
+                   align256(S * (size_t)num_chains * sizeof(double)));
+}
+
+int ut_logjoint_gradient_noalloc(const ut_plan* plan, int32_t num_chains, const float* params, float* values,
+                                 float* gradients, void* stream) {
+  if (!plan || !params || !values || !gradients)
+    return fail(UT_E_ARG, "ut_logjoint_gradient_noalloc: null pointer");
+  if (num_chains <= 0) return fail(UT_E_DIM, "ut_logjoint_gradient_noalloc: num_chains must be positive");
+  if (plan->dev.kind == MODEL_GP)
+    return fail(UT_E_UNSUPPORTED,
+                "ut_logjoint_gradient_noalloc: the GP plan's factorisation scratch (about 4.5 MiB per chain) "
+                "does not fit the fixed scratch; use ut_logjoint_gradient");
+  hipStream_t s = (hipStream_t)stream;
+  if (plan->dev.kind != MODEL_GLM_MFMA) {
+    // wavefront-per-chain density kernels: no scratch beyond inputs and outputs
+    UT_HIP(launch_density_f32(plan->dev, params, values, gradients, num_chains, s));
+    return UT_OK;
+  }
+  if (num_chains % 128 != 0)
+    return fail(UT_E_DIM, "ut_logjoint_gradient_noalloc: the matrix-core GLM plan needs num_chains a multiple of 128");
+  const int64_t need = ut_noalloc_scratch_bytes(plan, num_chains);
+  if (need > (int64_t)UT_NOALLOC_SCRATCH_BYTES)
+    return fail(UT_E_DIM, "ut_logjoint_gradient_noalloc: " + std::to_string(num_chains) + " chains need " +
+                              std::to_string(need) + " bytes of scratch, more than the fixed " +
+                              std::to_string((long long)UT_NOALLOC_SCRATCH_BYTES) + " (UT_NOALLOC_SCRATCH_BYTES)");
+  void* base = nullptr;
+  const int rc = noalloc_scratch(&base);
+  if (rc != UT_OK) return rc;
+  // carve the scratch: bf16 hi and lo positions, per-split gradient and
+  // log-likelihood partials.  Every byte used is written by the kernels before it
+  // is read (each split stores its whole partial block), so nothing is cleared.
+  const size_t PC = (size_t)plan->P * (size_t)num_chains;
+  GlmScratch g;
+  g.splits = glm_splits(plan->glm.N, num_chains);
+  unsigned char* p = (unsigned char*)base;
+  g.bhi = (uint16_t*)p;
+  p += align256(PC * sizeof(uint16_t));
+  g.blo = (uint16_t*)p;
+  p += align256(PC * sizeof(uint16_t));
+  g.gpart = (float*)p;
+  p += align256((size_t)g.splits * PC * sizeof(float));
+  g.llpart = (double*)p;
+  UT_HIP(launch_glm_grad(plan->glm, params, gradients, values, num_chains, true, g, s));
+  return UT_OK;
 }
 
 void ut_sampler_opts_default(ut_sampler_opts* o) {
