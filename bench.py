@@ -1,45 +1,56 @@
-"""Benchmark: min bulk-ESS/s at R-hat < 1.01 of batched NUTS on one MI355X.
+"""Benchmark: min bulk-ESS/s at R-hat < 1.01, plain vs flow-preconditioned HMC, one MI355X.
 
-Default workload (`--workload hier_nuts`, the line the driver records):
+Default workload (`--workload funnel_flow`, the line the driver records): Neal's funnel
+D = 100 (P = 101, v ~ normal(0, 3), x_i ~ normal(0, exp(v/2))), batched_hmc with 4096
+chains and L = 10 leapfrog steps, plain and flow-preconditioned.
 
-Workload: the non-centred hierarchical normal (J = 64 groups x 32 observations,
-sigma_j ~ U(0.5, 2)), batched_nuts with 1024 chains, max tree depth 10, binary32
-state with binary64 energies; W warmup + K sampling iterations.  One "step" is one
-NUTS iteration of all 1024 chains.  `value` is min over parameters of bulk ESS
-(rank-normalised, constrained space) divided by the sampling wall time (inputs and
-chain state resident in HBM; the time includes copying the draws back to the host).
+  plain  per-chain adaptation (the persistent kernel): 2 x W x 100 warmup iterations,
+         then 2000 draws per chain (8.19 M draws in all).
+  flow   a 6-layer, width-256 affine-coupling flow (fp32 weights, bf16 GEMM operands on
+         the matrix cores), trained on the GPU by the project's own kernels, then
+         shared-adaptation lock-step HMC in the flow's base space: W x 100 warmup
+         iterations and K x 200 sampling iterations.  `--flow-fit elbo` (default):
+         the reparameterised ELBO against the model (the reference's flow-ADVI
+         objective), 50k Adam iterations of 4096 particles from the identity flow.
+         `--flow-fit ml`: maximum likelihood on the plain run's 2000 draws per chain,
+         re-trained on each flow run's draws (`--flow-rounds`); `ml+elbo`: that fit
+         refined by the ELBO.  The plain draws miss the funnel's neck (v sd 2.81
+         against 3), so a flow fitted to them inherits the gap: the ML fit stalls
+         near R-hat 1.03 and, as a start for the ELBO, leaves it at 1.13 after 30k
+         iterations, while the ELBO from the identity reaches 1.0075.
 
-Also reported: gradient evaluations/s; the roofline of the dominant kernel (the
-persistent NUTS kernel, FP32 VALU bound) from HIP-event device time; and the CPU
-baseline -- the oracle's binary64 NUTS on the host cores, timed on a bounded number
-of sampling iterations from the GPU run's adapted state, giving CPU s/iteration and
-a projected ESS/s (GPU ESS per iteration / CPU s per iteration).
+One step is one block of 200 sampling iterations of all 4096 chains and one warmup
+unit 100 warmup iterations, so the driver's `--steps 20 --warmup 5` samples 4000 draws
+per chain after 500 warmup iterations.  `value` is the flow run's min over parameters
+of rank-normalised bulk ESS (constrained space) divided by its sampling wall time
+(chain state resident in HBM; the time includes copying the draws back), and is null
+unless every parameter has split R-hat < 1.01 (the reference harness's gate,
+bench/crossppl/python/analyze.py:76-97); `ungated_value` keeps the raw quotient.
+The plain run is reported beside it under the same gate.
 
-`--workload glm_chees`: the large logistic regression (N = 131072 rows, D = 256,
-X ~ N(0, 1/D) stored in bf16, 4096 chains) under batched_chees with the fused
-matrix-core gradient; roofline against the bf16 MFMA peak with the algorithmic
-4 N D flops per chain-gradient, CPU baseline = the binary64 oracle gradient on the
-host cores (projected: CPU s per batch gradient x mean leapfrog steps).
+Also reported: gradient evaluations/s; the roofline of the dominant kernel pair (the
+flow's all-layer forward and backward launches, bf16 MFMA) from HIP-event device time
+over the timed region, with the HBM bytes per launch from the committed counter runs
+of the same 4096-row launch shape; and the CPU baseline -- the oracle's binary64 plain
+HMC on every host core in the affinity mask (and on one thread), timed on a bounded
+sample of iterations, giving CPU s/iteration and a projected plain ESS/s.
 
-`--workload funnel_flow`: Neal's funnel D = 100 (P = 101), 4096 chains, batched_hmc
-L = 10: plain (per-chain adaptation, persistent kernel) and flow-preconditioned
-(a 6-layer, width-256 affine-coupling flow trained on the GPU on the plain run's
-2000 draws per chain, then shared-adaptation lock-step HMC in the flow's base
-space).  `value` is the flow run's min bulk-ESS/s; the plain run is reported beside
-it.  Roofline: the flow's conditioner GEMMs against the bf16 MFMA peak.
+`--workload hier_nuts`: the non-centred hierarchical normal (J = 64 groups x 32
+observations), batched_nuts with 1024 chains, max tree depth 10, 1000 + 3000
+iterations (here a step is one NUTS iteration); the CPU baseline runs the same
+sampler in full (binary64 oracle NUTS, 1024 chains, 1000 + 3000, all cores) and is
+measured, not projected; a 1-thread figure is projected from a bounded sample.
 
-`--workload advi`: batched_advi on the same hierarchical model (P = 66), 1024
-particles per iteration, `--guide` meanfield / fullrank (default) / flow (4 layers,
-width 256), standard ELBO, lr 0.01; `value` is Adam iterations/s of the whole call
-(one iteration = draw, guide forward, batched model gradient over the particles,
-ELBO reduction, guide backward, clip, Adam).  CPU baseline: the same iteration in
-numpy around the oracle's binary64 gradient (full-rank guide), bounded in time.
+`--workload glm_chees`: the large logistic regression (N = 131072, D = 256, X in bf16,
+4096 chains) under batched_chees with the fused matrix-core gradient (steps =
+iterations); CPU baseline projected from binary64 oracle gradients (a full CPU run
+is infeasible: 5.5e11 flops per batch gradient).
 
-`--workload gp_meads`: the latent GP with Poisson counts (N = 512 inputs in
-[0,1]^2, ARD squared-exponential kernel, P = 515), batched_meads with 1024 chains,
-binary64 batched Cholesky with hyperparameter tangents; without and with a flow
-trained on the plain run's draws.  Roofline: the factorisation's binary64 flops
-(7 N^3 / 3 per chain-gradient with three tangents) against the FP64 peak.
+`--workload gp_meads`: the latent GP with Poisson counts (N = 512, P = 515),
+batched_meads with 1024 chains, without and with a flow; CPU baseline projected.
+
+`--workload advi`: batched_advi on the hierarchical model (P = 66, 1024 particles);
+`value` is Adam iterations/s.
 """
 from __future__ import annotations
 
@@ -56,8 +67,14 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 METRIC = "min bulk-ESS/sec at R̂<1.01, plain vs flow-preconditioned HMC, 1 MI355X"
+RHAT_MAX = 1.01
 PEAK_FP32_TFLOPS = 157.3
+PEAK_FP64_TFLOPS = 78.6
 PEAK_BF16_TFLOPS = 2500.0
+# funnel workload: one step = STEP_ITERS sampling iterations, one warmup unit = WARMUP_ITERS
+STEP_ITERS = 200
+WARMUP_ITERS = 100
+TRAIN_DRAWS = 2000
 # Algorithmic work per NUTS leaf of the J=64 x 32 model: one chain-gradient
 # (6 N + 10 J + 20 = 12,948 flops) plus kick/drift/U-turn/energy (12 P = 792 flops).
 FLOPS_PER_GRADIENT = 6 * 2048 + 10 * 64 + 20
@@ -66,6 +83,34 @@ FLOPS_PER_LEAF_EXTRA = 12 * 66
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def gated(value, rhat_max):
+    """The metric counts a throughput only when every parameter has R-hat < 1.01."""
+    return value if (value is not None and np.isfinite(rhat_max) and rhat_max < RHAT_MAX) else None
+
+
+def cpu_info() -> dict:
+    """Host cores in this process's affinity mask and the CPU model string."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"affinity_cores": n, "cpu_model": model, "nproc": os.cpu_count(),
+            "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS")}
+
+
+def cpu_cores() -> int:
+    return max(1, cpu_info()["affinity_cores"])
 
 
 def moments_vs(D, a, b) -> dict:
@@ -106,47 +151,60 @@ def _funnel_standardised(draws):
     return out
 
 
-def cpu_cores() -> int:
-    try:
-        n = len(os.sched_getaffinity(0))
-    except AttributeError:
-        n = os.cpu_count() or 1
-    return max(1, min(n, 16))
+def _cpu_hmc_seconds_per_iteration(om, chains, L, step, seconds, threads, seed, chunk=20):
+    """Wall seconds per binary64 oracle HMC iteration of `chains` chains (fixed step,
+    no adaptation), from chunks of iterations until `seconds` have been spent."""
+    it, t = 0, 0.0
+    while t < seconds or it == 0:
+        ts = time.perf_counter()
+        om.run("hmc", "f64", chains, num_samples=chunk, num_warmup=0, num_leapfrog_steps=L, step_size=step,
+               adapt_step_size=False, seed=seed + it, threads=threads)
+        t += time.perf_counter() - ts
+        it += chunk
+    return t / it, it, t
 
 
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=("hier_nuts", "glm_chees", "funnel_flow", "gp_meads", "advi"),
-                    default="hier_nuts")
+    ap.add_argument("--workload", choices=("funnel_flow", "hier_nuts", "glm_chees", "gp_meads", "advi"),
+                    default="funnel_flow")
     ap.add_argument("--guide", choices=("meanfield", "fullrank", "flow"), default="fullrank",
                     help="advi: the variational guide")
-    ap.add_argument("--steps", type=int, default=None, help="sampling iterations")
-    ap.add_argument("--warmup", type=int, default=None, help="warmup iterations")
+    ap.add_argument("--steps", type=int, default=None,
+                    help="funnel_flow: blocks of 200 sampling iterations (default 20); other workloads: "
+                         "sampling iterations")
+    ap.add_argument("--warmup", type=int, default=None,
+                    help="funnel_flow: blocks of 100 warmup iterations (default 5); other workloads: warmup "
+                         "iterations")
     ap.add_argument("--chains", type=int, default=None)
     ap.add_argument("--flow", action="store_true",
                     help="glm_chees: also train a flow on the plain run's draws and run ChEES with it")
     ap.add_argument("--forward-velocity", action="store_true",
                     help="glm_chees: ChEES criterion with the forward end velocity (not the reference's sign)")
     ap.add_argument("--flow-rounds", type=int, default=3,
-                    help="funnel_flow / gp_meads: train the flow on the plain run's draws, then re-train it "
-                         "on each flow run's draws this many times in all (iterative NeuTra)")
+                    help="funnel_flow --flow-fit ml / gp_meads: train the flow on the plain run's draws, then "
+                         "re-train it on each flow run's draws this many times in all (iterative NeuTra)")
     ap.add_argument("--flow-target-accept", type=float, default=None,
-                    help="funnel_flow: target acceptance of the flow runs' step-size adaptation (default 0.97 "
-                         "after an ELBO fit: with L fixed at 10 the smaller step keeps the trajectory near a "
-                         "quarter period of the near-Gaussian base; 0.9 after maximum likelihood, where 0.8 "
-                         "gives ~1 %% divergent transitions in the neck)")
-    ap.add_argument("--flow-fit", choices=("elbo", "ml"), default="elbo",
-                    help="funnel_flow: fit the flow by the reparameterised ELBO against the model (the "
-                         "reference's flow ADVI objective; no draws needed) or by maximum likelihood on the "
-                         "plain run's draws with --flow-rounds re-training rounds")
-    ap.add_argument("--elbo-schedule", default="30000:1e-3,10000:3e-4,10000:1e-4",
-                    help="funnel_flow --flow-fit elbo: stages iterations:learning_rate, 4096 particles each")
+                    help="funnel_flow: target acceptance of the flow run's step-size adaptation (default 0.97 "
+                         "when the fit ends with the ELBO: with L fixed at 10 the smaller step keeps the "
+                         "trajectory near a quarter period of the near-Gaussian base; 0.9 after maximum "
+                         "likelihood alone)")
+    ap.add_argument("--flow-fit", choices=("elbo", "ml", "ml+elbo"), default="elbo",
+                    help="funnel_flow: the reparameterised ELBO against the model from the identity flow "
+                         "(default); maximum likelihood on the plain run's draws with --flow-rounds re-training "
+                         "rounds on the flow runs' draws; or that maximum-likelihood fit refined by the ELBO")
+    ap.add_argument("--ml-steps", type=int, default=2000,
+                    help="funnel_flow: Adam steps of the maximum-likelihood fit (4096-draw minibatches)")
+    ap.add_argument("--elbo-schedule", default=None,
+                    help="funnel_flow: ELBO stages iterations:learning_rate, 4096 particles each (default "
+                         "30000:1e-3,10000:3e-4,10000:1e-4 from the identity flow, "
+                         "20000:3e-4,10000:1e-4 after maximum likelihood)")
     ap.add_argument("--final-steps", type=int, default=None,
-                    help="funnel_flow / gp_meads: sampling iterations of the reported flow run (funnel default: "
-                         "2 x --steps after an ELBO fit, --steps after maximum likelihood; GP default: --steps)")
+                    help="gp_meads: sampling iterations of the reported flow run (default --steps)")
     ap.add_argument("--seed", type=int, default=20261203)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0,
+                    help="wall seconds of the bounded all-core CPU sample (the 1-thread sample gets a third)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (HBM counter calibration)")
@@ -165,23 +223,18 @@ def main() -> int:
         print(f"bench.py runs on exactly one GPU; --gpus {args.gpus} is not supported "
               "(chains shard across GPUs without communication; see the design notes)", file=sys.stderr)
         return 2
+    return {"funnel_flow": bench_funnel_flow, "hier_nuts": bench_hier_nuts, "glm_chees": bench_glm_chees,
+            "gp_meads": bench_gp_meads, "advi": bench_advi}[args.workload](args)
 
-    if args.workload == "glm_chees":
-        return bench_glm_chees(args)
-    if args.workload == "funnel_flow":
-        return bench_funnel_flow(args)
-    if args.workload == "gp_meads":
-        return bench_gp_meads(args)
-    if args.workload == "advi":
-        return bench_advi(args)
-    args.steps = 3000 if args.steps is None else args.steps
-    args.warmup = 1000 if args.warmup is None else args.warmup
-    args.chains = 1024 if args.chains is None else args.chains
 
+def bench_hier_nuts(args) -> int:
     import uncertaintea_amd as ut
     from uncertaintea_amd import diagnostics as D
     from uncertaintea_amd import synthetic as S
 
+    args.steps = 3000 if args.steps is None else args.steps
+    args.warmup = 1000 if args.warmup is None else args.warmup
+    args.chains = 1024 if args.chains is None else args.chains
     y, off, sig, _ = S.hierarchical_data(64, 32, seed=args.seed)
     model = ut.HierarchicalNormalNC(y, off, sig, mu_loc=0.0, mu_scale=5.0, tau_scale=5.0)
     log(f"bench: batched_nuts hierarchical J=64x32 chains={args.chains} warmup={args.warmup} steps={args.steps}")
@@ -190,50 +243,72 @@ def main() -> int:
                           max_tree_depth=10, seed=args.seed)
     wall = time.perf_counter() - t0
     log(f"bench: sampler done warmup {res.warmup_seconds:.3f}s sampling {res.sampling_seconds:.3f}s wall {wall:.3f}s")
-    draws = res.constrained_samples
-    rh, eb = D.rhat_and_ess_bulk(draws)
+    rh, eb = D.rhat_and_ess_bulk(res.constrained_samples)
     min_ess = float(np.nanmin(eb))
     rhat_max = float(np.nanmax(rh))
-    value = min_ess / res.sampling_seconds
-    log(f"bench: rhat max {rhat_max:.4f} min bulk ESS {min_ess:.1f} -> {value:.1f} ESS/s")
+    raw = min_ess / res.sampling_seconds
+    value = gated(raw, rhat_max)
+    log(f"bench: rhat max {rhat_max:.4f} min bulk ESS {min_ess:.1f} -> {raw:.1f} ESS/s "
+        f"({'gated in' if value is not None else 'R-hat gate failed: value null'})")
 
     evals = res.sampling_gradient_evals
     flops = evals * (FLOPS_PER_GRADIENT + FLOPS_PER_LEAF_EXTRA)
     ksec = res.sampling_kernel_seconds
     launches = max(res.sampling_launches, 1)
     achieved = flops / ksec / 1e12 if ksec > 0 else 0.0
+    # HBM bytes of a launch: the tree runs on chip, so a launch's traffic is its draws
+    # and per-draw statistics; the committed FETCH_SIZE/WRITE_SIZE runs give bytes per
+    # sampling iteration, scaled here to the iterations of the launches timed
     traffic = None
     pmc = ROOT / "profiles" / "k_nuts_hier_cfg2_pmc.json"
-    if pmc.exists():  # HBM bytes per launch from the committed FETCH_SIZE/WRITE_SIZE runs
-        traffic = json.loads(pmc.read_text()).get("traffic_bytes_per_launch")
-    roofline = {"bound": "mfma", "compute": "fp32 vector ALU (no HBM traffic in the tree)", "achieved": achieved,
-                "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
+    if pmc.exists():
+        rec = json.loads(pmc.read_text())
+        per_it = rec.get("traffic_bytes_per_launch", 0.0) / max(1, rec.get("iterations_per_launch", 500))
+        traffic = per_it * args.steps / launches
+    roofline = {"bound": "fp32-valu", "compute": "fp32 vector ALU (the tree and the model gradient stay on chip)",
+                "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
                 "kernel": "k_nuts<HierModel>", "launches": res.sampling_launches,
-                "avg_launch_ms": 1e3 * ksec / launches, "flops_per_launch": flops / launches}
+                "avg_launch_ms": 1e3 * ksec / launches, "flops_per_launch": flops / launches,
+                "timing": "HIP events around every sampling launch on the workspace stream"}
 
     cpu = None
     if not args.no_cpu_baseline:
         from oracle import binding as B
         om = B.OracleModel(model)
-        cores = cpu_cores()
+        info = cpu_info()
+        cores = max(1, info["affinity_cores"])
+        log(f"bench: cpu full run: binary64 oracle NUTS, {args.chains} chains, {args.warmup} + {args.steps}, "
+            f"{cores} threads")
+        t0 = time.perf_counter()
+        orc = om.run("nuts", "f64", args.chains, num_samples=args.steps, num_warmup=args.warmup, max_tree_depth=10,
+                     seed=args.seed, threads=cores)
+        t_full = time.perf_counter() - t0
+        crh, ceb = D.rhat_and_ess_bulk(model.constrain(orc["samples"]))
+        c_ess, c_rhat = float(np.nanmin(ceb)), float(np.nanmax(crh))
+        c_raw = c_ess / orc["sampling_seconds"]
+        # one thread: a bounded sample of sampling iterations from the GPU run's adapted state
         pos = res.unconstrained_samples[:, -1, :].astype(np.float64).T.copy()
         it = args.warmup + args.steps + 1
-        n_iter, n_evals, t_cpu = 0, 0, 0.0
-        while t_cpu < args.cpu_seconds and n_iter < 5000:
+        n1, t1 = 0, 0.0
+        while t1 < args.cpu_seconds / 3 and n1 < 2000:
             ts = time.perf_counter()
-            ev, pos = B.nuts_sample_from(om, "f64", pos, res.step_size, res.mass_matrix, it + n_iter, 1,
-                                         seed=args.seed, threads=cores)
-            t_cpu += time.perf_counter() - ts
-            n_iter += 1
-            n_evals += ev
-        s_per_iter = t_cpu / n_iter
-        ess_per_iter = min_ess / args.steps
-        cpu = {"value": ess_per_iter / s_per_iter, "unit": "ESS/s (projected)", "cores": cores, "kind": "port",
-               "sample": f"{n_iter} sampling iterations x {args.chains} chains of the binary64 oracle NUTS from the "
-                         f"GPU run's adapted state ({t_cpu:.1f} s); projected = GPU bulk ESS per iteration / "
-                         f"CPU s per iteration",
-               "cpu_s_per_iteration": s_per_iter, "cpu_gradient_evals_per_s": n_evals / t_cpu}
-        log(f"bench: cpu {s_per_iter:.3f} s/iter on {cores} threads -> projected {cpu['value']:.1f} ESS/s")
+            B.nuts_sample_from(om, "f64", pos, res.step_size, res.mass_matrix, it + n1, 1, seed=args.seed, threads=1)
+            t1 += time.perf_counter() - ts
+            n1 += 1
+        s1 = t1 / n1
+        cpu = {"value": gated(c_raw, c_rhat), "ungated_value": c_raw, "unit": "ESS/s (measured, full run)",
+               "cores": cores, "kind": "port", "cpu_model": info["cpu_model"], "host": info,
+               "sample": f"the whole sampler run on the host: binary64 oracle NUTS, {args.chains} chains, "
+                         f"{args.warmup} warmup + {args.steps} sampling iterations ({t_full:.1f} s wall)",
+               "warmup_seconds": orc["warmup_seconds"], "sampling_seconds": orc["sampling_seconds"],
+               "rhat_max": c_rhat, "min_bulk_ess": c_ess,
+               "one_thread": {"s_per_iteration": s1, "projected_ess_per_s": (c_ess / args.steps) / s1,
+                              "sample": f"{n1} sampling iterations x {args.chains} chains on 1 thread "
+                                        f"({t1:.1f} s); projected = CPU run's bulk ESS per iteration / s per "
+                                        f"iteration"}}
+        log(f"bench: cpu {cores} threads: warmup {orc['warmup_seconds']:.1f}s sampling {orc['sampling_seconds']:.1f}s "
+            f"rhat {c_rhat:.4f} -> {c_raw:.1f} ESS/s; 1 thread {s1:.3f} s/iter")
 
     line = {
         "metric": METRIC, "value": value, "unit": "ESS/s", "n_gpus": 1, "steps": args.steps,
@@ -241,14 +316,14 @@ def main() -> int:
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (Philox4x32-10, seed %d)" % args.seed,
         "config": {"workload": "hierarchical normal non-centred J=64x32, batched_nuts persistent, 1024 chains, "
-                               "max depth 10, fp32 state + fp64 energies",
+                               "max depth 10, fp32 state + fp64 energies (plain NUTS, no flow)",
                    "chains": args.chains, "num_params": model.num_params, "max_tree_depth": 10},
-        "rhat_max": rhat_max, "rhat_ok": rhat_max < 1.01, "min_bulk_ess": min_ess,
+        "ungated_value": raw, "rhat_max": rhat_max, "rhat_ok": rhat_max < RHAT_MAX, "min_bulk_ess": min_ess,
         "sampling_seconds": res.sampling_seconds, "warmup_seconds": res.warmup_seconds,
         "gradient_evals_per_s": evals / res.sampling_seconds,
         "mean_tree_depth": float(res.tree_depths.mean()), "divergence_rate": res.divergence_rate(),
         "roofline": roofline, "cpu_baseline": cpu,
-        "vs_cpu": (value / cpu["value"]) if cpu else None,
+        "vs_cpu": (value / cpu["value"]) if (cpu and value is not None and cpu["value"]) else None,
     }
     print(json.dumps(line), flush=True)
     return 0
@@ -258,157 +333,156 @@ def bench_funnel_flow(args) -> int:
     import uncertaintea_amd as ut
     from uncertaintea_amd import diagnostics as D
 
-    steps = 2000 if args.steps is None else args.steps
-    warmup = 1000 if args.warmup is None else args.warmup
+    steps_blocks = 20 if args.steps is None else args.steps
+    warm_blocks = 5 if args.warmup is None else args.warmup
+    S_flow = STEP_ITERS * steps_blocks
+    W_flow = WARMUP_ITERS * warm_blocks
+    W_plain = 2 * W_flow
     chains = 4096 if args.chains is None else args.chains
     dim, L, H, layers = 100, 10, 256, 6
     model = ut.NealsFunnel(dim, 3.0)
 
     def summarise(res, label):
         rh, eb = D.rhat_and_ess_bulk(res.constrained_samples)
-        out = {"min_bulk_ess": float(np.nanmin(eb)), "rhat_max": float(np.nanmax(rh)),
+        raw = float(np.nanmin(eb)) / res.sampling_seconds
+        rmax = float(np.nanmax(rh))
+        out = {"min_bulk_ess": float(np.nanmin(eb)), "rhat_max": rmax, "rhat_ok": rmax < RHAT_MAX,
                "ess_bulk_v": float(eb[0]), "sampling_seconds": res.sampling_seconds,
-               "warmup_seconds": res.warmup_seconds, "value": float(np.nanmin(eb)) / res.sampling_seconds,
+               "warmup_seconds": res.warmup_seconds, "value": gated(raw, rmax), "ungated_value": raw,
+               "num_samples": res.num_samples,
                "gradient_evals_per_s": res.sampling_gradient_evals / res.sampling_seconds,
                "divergence_rate": res.divergence_rate(), "step_size": float(np.median(res.step_size)),
                "v_mean": float(res.unconstrained_samples[:, :, 0].mean()),
                "v_sd": float(res.unconstrained_samples[:, :, 0].std())}
-        out["rhat_ok"] = out["rhat_max"] < 1.01
-        log(f"bench: {label}: min ESS {out['min_bulk_ess']:.1f} rhat {out['rhat_max']:.4f} -> {out['value']:.1f} "
-            f"ESS/s; v mean {out['v_mean']:.3f} sd {out['v_sd']:.3f}; div {out['divergence_rate']:.4f}")
+        log(f"bench: {label}: min ESS {out['min_bulk_ess']:.1f} rhat {rmax:.4f} -> {raw:.1f} ESS/s "
+            f"({'gated in' if out['value'] is not None else 'fails the R-hat gate'}); v mean {out['v_mean']:.3f} "
+            f"sd {out['v_sd']:.3f}; div {out['divergence_rate']:.4f}")
         return out
 
-    log(f"bench: funnel D={dim} plain batched_hmc chains={chains} warmup={warmup} steps={steps}")
-    plain = ut.batched_hmc(model, num_chains=chains, num_samples=steps, num_warmup=warmup, num_leapfrog_steps=L,
-                           seed=args.seed)
+    log(f"bench: funnel D={dim} plain batched_hmc chains={chains} warmup={W_plain} draws={TRAIN_DRAWS}")
+    plain = ut.batched_hmc(model, num_chains=chains, num_samples=TRAIN_DRAWS, num_warmup=W_plain,
+                           num_leapfrog_steps=L, seed=args.seed)
     plain_s = summarise(plain, "plain")
     flow = ut.AffineCouplingFlow(model.num_params, layers, H, seed=args.seed)
-    elbo = args.flow_fit == "elbo"
+    fit = args.flow_fit
     if args.flow_target_accept is None:
-        args.flow_target_accept = 0.97 if elbo else 0.9
-    # 4096 chains need ~50 effective draws per chain for split R-hat < 1.01; the ELBO
-    # run reaches 36-80 per 2000 draws, so its reported run samples twice as many
-    final_steps = args.final_steps if args.final_steps else (2 * steps if elbo else steps)
-    if elbo:
-        return _funnel_elbo_flow(args, ut, D, model, flow, plain, plain_s, summarise, steps, warmup, chains, dim, L,
-                                 H, layers, final_steps)
-    draws = plain.unconstrained_samples.reshape(-1, model.num_params)
-    t0 = time.perf_counter()
-    train_steps = 2000
-    trace = flow.fit(draws, steps=train_steps, batch=4096, seed=args.seed)
-    t_train = time.perf_counter() - t0
-    log(f"bench: flow trained on {draws.shape[0]} draws, {train_steps} steps in {t_train:.2f}s, "
-        f"loss {trace[:20].mean():.2f} -> {trace[-100:].mean():.2f}")
-    del draws
-    nrounds = max(1, args.flow_rounds)
-    # the reported (last) flow run samples final_steps per chain (default: steps).  With
-    # 4096 chains split R-hat < 1.01 needs ~50 effective draws per chain; 4 x steps gives
-    # R-hat 1.019 here, ESS grows 1.7x for 4x the draws
-    fres = ut.batched_hmc(model, num_chains=chains, num_samples=final_steps if nrounds == 1 else steps,
-                          num_warmup=warmup // 2, num_leapfrog_steps=L, per_chain_adaptation=False, flow=flow,
-                          seed=args.seed + 1, target_accept=args.flow_target_accept)
-    flow_s = summarise(fres, "flow")
-    rounds = [dict(flow_s, training_seconds=t_train)]
-    for r in range(2, nrounds + 1):
-        # re-train (warm start) on the flow run's draws, which reach the neck the plain run missed
-        draws = fres.unconstrained_samples.reshape(-1, model.num_params)
+        args.flow_target_accept = 0.9 if fit == "ml" else 0.97
+    if args.elbo_schedule is None:
+        args.elbo_schedule = "20000:3e-4,10000:1e-4" if fit == "ml+elbo" else "30000:1e-3,10000:3e-4,10000:1e-4"
+    stages = [(int(a.split(":")[0]), float(a.split(":")[1])) for a in args.elbo_schedule.split(",") if a]
+    fit_log, t_train = [], 0.0
+    if fit in ("ml", "ml+elbo"):
+        draws = plain.unconstrained_samples.reshape(-1, model.num_params)
         t0 = time.perf_counter()
-        # warm start, twice the first round's steps
-        trace = flow.fit(draws, steps=2 * train_steps, batch=4096, seed=args.seed + r, init_base=False)
-        t_r = time.perf_counter() - t0
-        t_train += t_r
-        del draws
-        log(f"bench: flow round {r} re-trained in {t_r:.2f}s, loss {trace[:20].mean():.2f} -> {trace[-100:].mean():.2f}")
-        fres = ut.batched_hmc(model, num_chains=chains, num_samples=final_steps if r == nrounds else steps,
-                              num_warmup=warmup // 2, num_leapfrog_steps=L, per_chain_adaptation=False, flow=flow,
-                              seed=args.seed + r,
-                              target_accept=args.flow_target_accept)
-        flow_s = summarise(fres, f"flow round {r}")
-        rounds.append(dict(flow_s, training_seconds=t_r))
-    desc = (f"Neal's funnel D={dim}, batched_hmc L={L}, {chains} chains, plain vs affine-coupling flow ({layers} "
-            f"layers, width {H}) trained on the plain run's {steps} draws/chain, {max(1, args.flow_rounds)} training "
-            f"round(s)")
-    return _funnel_report(args, D, model, flow, plain, plain_s, fres, flow_s, rounds, t_train, steps, warmup, chains,
-                          L, H, final_steps, desc)
-
-
-def _funnel_elbo_flow(args, ut, D, model, flow, plain, plain_s, summarise, steps, warmup, chains, dim, L, H, layers,
-                      final_steps) -> int:
-    """Flow fitted by the reparameterised ELBO against the funnel itself (the
-    reference's flow ADVI objective, run on the device in stages of decreasing
-    learning rate), then flow-preconditioned shared-adaptation HMC."""
-    stages = [(int(a.split(":")[0]), float(a.split(":")[1])) for a in args.elbo_schedule.split(",")]
-    t_train, rounds = 0.0, []
-    for k, (n, lr) in enumerate(stages):
-        t0 = time.perf_counter()
-        out = flow.fit_elbo(model, num_particles=4096, num_iterations=n, learning_rate=lr, seed=args.seed + 100 + k)
+        trace = flow.fit(draws, steps=args.ml_steps, batch=4096, seed=args.seed)
         dt = time.perf_counter() - t0
         t_train += dt
-        h = out["elbo_history"]
-        log(f"bench: ELBO stage {k}: {n} iterations at lr {lr} in {dt:.2f}s, ELBO {h[:50].mean():.3f} -> "
-            f"{h[-200:].mean():.3f} (log Z = 0)")
-        rounds.append({"iterations": n, "learning_rate": lr, "seconds": dt, "elbo_first": float(h[:50].mean()),
-                       "elbo_last": float(h[-200:].mean()), "best_elbo": out["best_elbo"]})
-    fres = ut.batched_hmc(model, num_chains=chains, num_samples=final_steps, num_warmup=warmup // 2,
-                          num_leapfrog_steps=L, per_chain_adaptation=False, flow=flow, seed=args.seed + 1,
-                          target_accept=args.flow_target_accept)
-    flow_s = summarise(fres, "flow (ELBO fit)")
-    total = sum(n for n, _ in stages)
-    desc = (f"Neal's funnel D={dim}, batched_hmc L={L}, {chains} chains, plain vs affine-coupling flow ({layers} "
-            f"layers, width {H}) fitted by the reparameterised ELBO ({total} Adam iterations x 4096 particles)")
-    return _funnel_report(args, D, model, flow, plain, plain_s, fres, flow_s, rounds, t_train, steps, warmup, chains,
-                          L, H, final_steps, desc)
+        del draws
+        log(f"bench: flow fitted by maximum likelihood on the plain run's {chains} x {TRAIN_DRAWS} draws, "
+            f"{args.ml_steps} steps in {dt:.2f}s, loss {trace[:20].mean():.3f} -> {trace[-100:].mean():.3f}")
+        fit_log.append({"objective": "maximum likelihood on the plain draws", "steps": args.ml_steps,
+                        "seconds": dt, "loss_first": float(trace[:20].mean()), "loss_last": float(trace[-100:].mean())})
+    if fit in ("elbo", "ml+elbo"):
+        for k, (n, lr) in enumerate(stages):
+            t0 = time.perf_counter()
+            out = flow.fit_elbo(model, num_particles=4096, num_iterations=n, learning_rate=lr, seed=args.seed + 100 + k)
+            dt = time.perf_counter() - t0
+            t_train += dt
+            h = out["elbo_history"]
+            log(f"bench: ELBO stage {k}: {n} iterations at lr {lr} in {dt:.2f}s, ELBO {h[:50].mean():.4f} -> "
+                f"{h[-200:].mean():.4f} (log Z = 0)")
+            fit_log.append({"objective": "reparameterised ELBO", "iterations": n, "learning_rate": lr, "seconds": dt,
+                            "elbo_first": float(h[:50].mean()), "elbo_last": float(h[-200:].mean()),
+                            "best_elbo": out["best_elbo"]})
+    rounds = max(1, args.flow_rounds) if fit == "ml" else 1
+    fres = None
+    for r in range(1, rounds + 1):
+        if r > 1:
+            # iterative NeuTra: re-train (warm start) on the previous flow run's draws
+            t0 = time.perf_counter()
+            trace = flow.fit(fres.unconstrained_samples.reshape(-1, model.num_params), steps=2 * args.ml_steps,
+                             batch=4096, seed=args.seed + r, init_base=False)
+            dt = time.perf_counter() - t0
+            t_train += dt
+            fit_log.append({"objective": f"maximum likelihood on flow round {r - 1}'s draws",
+                            "steps": 2 * args.ml_steps, "seconds": dt, "loss_last": float(trace[-100:].mean())})
+        fres = ut.batched_hmc(model, num_chains=chains, num_samples=S_flow, num_warmup=W_flow, num_leapfrog_steps=L,
+                              per_chain_adaptation=False, flow=flow, seed=args.seed + r,
+                              target_accept=args.flow_target_accept)
+    flow_s = summarise(fres, "flow")
 
-
-def _funnel_report(args, D, model, flow, plain, plain_s, fres, flow_s, rounds, t_train, steps, warmup, chains, L, H,
-                   final_steps, desc) -> int:
     # conditioner GEMM flops per chain-gradient: forward 2 GEMMs + backward 2 GEMMs per layer
     flops_grad = 0.0
     for npass, nact, _ in flow.layout():
         flops_grad += 2 * 2.0 * (npass * H + H * 2 * nact)
-    achieved = fres.sampling_gradient_evals * flops_grad / fres.sampling_kernel_seconds / 1e12
+    ksec = fres.sampling_kernel_seconds
+    evals = fres.sampling_gradient_evals
+    achieved = evals * flops_grad / ksec / 1e12
     traffic = None
     pmc = ROOT / "profiles" / "k_flow_cfg4_pmc.json"
-    if pmc.exists():  # HBM bytes of one forward + one backward launch, from the committed FETCH_SIZE/WRITE_SIZE runs
+    if pmc.exists():  # HBM bytes of one forward + one backward launch at 4096 rows (the launch shape timed)
         rec = json.loads(pmc.read_text())
         traffic = sum(rec[k]["traffic_bytes_per_launch"] for k in ("k_flow_fwd_all", "k_flow_bwd_all"))
-    roofline = {"bound": "mfma", "compute": "bf16 MFMA conditioner GEMMs (latency-bound at 4096 rows)",
+    roofline = {"bound": "mfma", "compute": "bf16 MFMA conditioner GEMMs, fp32 accumulation (latency-bound at "
+                                           "4096 rows)",
                 "achieved": achieved, "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
                 "frac": achieved / PEAK_BF16_TFLOPS, "traffic": traffic,
-                "kernel": "k_flow_fwd_all + k_flow_bwd_all (one launch of each per batched gradient)",
-                "timing": "HIP events around all sampling launches of the flow run",
+                "kernel": "k_flow_fwd_all + k_flow_bwd_all (one launch of each per batched gradient of 4096 rows)",
+                "timing": "HIP events around every sampling launch of the flow run on the workspace stream "
+                          "(the per-gradient time includes the model gradient and the lock-step leapfrog kernels)",
+                "us_per_batched_gradient": 1e6 * ksec / max(1, evals // chains),
                 "flops_per_chain_gradient": flops_grad}
     cpu = None
     if not args.no_cpu_baseline:
         from oracle import binding as B
         om = B.OracleModel(model)
-        cores = cpu_cores()
-        n_it = max(1, min(200, int(args.cpu_seconds * 20)))
-        t = time.perf_counter()
-        om.run("hmc", "f64", chains, num_samples=n_it, num_warmup=0, num_leapfrog_steps=L,
-               step_size=float(np.median(plain.step_size)), adapt_step_size=False, seed=args.seed, threads=cores)
-        s_iter = (time.perf_counter() - t) / n_it
-        cpu = {"value": (plain_s["min_bulk_ess"] / steps) / s_iter, "unit": "ESS/s (projected, plain HMC)",
-               "cores": cores, "kind": "port",
-               "sample": f"{n_it} binary64 oracle HMC iterations x {chains} chains (L={L}); projected = plain GPU "
-                         f"bulk ESS per iteration / CPU s per iteration", "cpu_s_per_iteration": s_iter}
-        log(f"bench: cpu {s_iter:.3f} s/iter -> projected {cpu['value']:.2f} ESS/s")
+        info = cpu_info()
+        cores = max(1, info["affinity_cores"])
+        step = float(np.median(plain.step_size))
+        s_iter, n_it, t_all = _cpu_hmc_seconds_per_iteration(om, chains, L, step, args.cpu_seconds, cores, args.seed)
+        s_iter1, n_it1, t_1 = _cpu_hmc_seconds_per_iteration(om, chains // 16, L, step, args.cpu_seconds / 3, 1,
+                                                             args.seed, chunk=10)
+        s_iter1 *= 16  # scaled from chains/16 chains to all chains
+        ess_it = plain_s["min_bulk_ess"] / TRAIN_DRAWS
+        cpu = {"value": ess_it / s_iter, "unit": "ESS/s (projected, plain HMC, not R-hat gated)", "cores": cores,
+               "kind": "port", "cpu_model": info["cpu_model"], "host": info,
+               "sample": f"{n_it} binary64 oracle plain-HMC iterations x {chains} chains (L={L}, the plain run's "
+                         f"median step) on {cores} threads ({t_all:.1f} s); projected = the plain GPU run's bulk ESS "
+                         f"per iteration / CPU s per iteration (the reference CPU path is plain HMC; like the plain "
+                         f"GPU run it misses the funnel's neck, so this figure carries no R-hat gate)",
+               "cpu_s_per_iteration": s_iter, "cpu_gradient_evals_per_s": chains * L / s_iter,
+               "one_thread": {"s_per_iteration": s_iter1, "projected_ess_per_s": ess_it / s_iter1,
+                              "sample": f"{n_it1} iterations x {chains // 16} chains on 1 thread ({t_1:.1f} s), "
+                                        f"scaled to {chains} chains"}}
+        log(f"bench: cpu {s_iter:.4f} s/iter on {cores} threads ({info['cpu_model']}) -> projected "
+            f"{cpu['value']:.1f} plain ESS/s; 1 thread {s_iter1:.3f} s/iter")
+    how = {"elbo": "fitted by the reparameterised ELBO", "ml": f"fitted by maximum likelihood on the plain run's "
+           f"{TRAIN_DRAWS} draws/chain", "ml+elbo": f"fitted by maximum likelihood on the plain run's {TRAIN_DRAWS} "
+           f"draws/chain, then the ELBO"}[fit]
+    desc = (f"Neal's funnel D={dim}, batched_hmc L={L}, {chains} chains: plain (per-chain adaptation, {W_plain} "
+            f"warmup + {TRAIN_DRAWS} draws) vs an affine-coupling flow ({layers} layers, width {H}) {how}, "
+            f"shared-adaptation HMC in its base space ({W_flow} warmup + {S_flow} draws)")
+    thin = max(1, S_flow // TRAIN_DRAWS)
     line = {
-        "metric": METRIC, "value": flow_s["value"], "unit": "ESS/s", "n_gpus": 1, "steps": final_steps,
-        "warmup": warmup // 2, "ms_per_step": 1e3 * fres.sampling_seconds / final_steps, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32+bf16", "data": "synthetic (no data; seed %d)" % args.seed,
-        "config": {"workload": desc, "chains": chains, "flow_fit": args.flow_fit},
-        "rhat_max": flow_s["rhat_max"], "rhat_ok": flow_s["rhat_ok"], "min_bulk_ess": flow_s["min_bulk_ess"],
-        "flow": flow_s, "plain": plain_s, "flow_training_seconds": t_train, "flow_rounds": rounds,
-        "flow_vs_plain": flow_s["value"] / plain_s["value"] if plain_s["value"] > 0 else None,
-        "flow_moments_vs_plain": moments_vs(D, fres.constrained_samples[:, ::max(1, final_steps // steps)],
-                                            plain.constrained_samples),
+        "metric": METRIC, "value": flow_s["value"], "unit": "ESS/s", "n_gpus": 1, "steps": steps_blocks,
+        "warmup": warm_blocks, "ms_per_step": 1e3 * fres.sampling_seconds / steps_blocks, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32 (bf16 flow GEMMs)",
+        "data": "synthetic (no data; seed %d)" % args.seed,
+        "config": {"workload": desc, "chains": chains, "flow_fit": fit, "step": f"{STEP_ITERS} sampling iterations",
+                   "warmup_unit": f"{WARMUP_ITERS} warmup iterations", "num_samples": S_flow, "num_warmup": W_flow},
+        "ungated_value": flow_s["ungated_value"], "rhat_max": flow_s["rhat_max"], "rhat_ok": flow_s["rhat_ok"],
+        "min_bulk_ess": flow_s["min_bulk_ess"], "flow": flow_s, "plain": plain_s,
+        "flow_training_seconds": t_train, "flow_fit": fit_log,
+        "flow_vs_plain": (flow_s["value"] / plain_s["value"]) if (flow_s["value"] and plain_s["value"]) else None,
+        "flow_vs_plain_ungated": flow_s["ungated_value"] / plain_s["ungated_value"],
+        "flow_moments_vs_plain": moments_vs(D, fres.constrained_samples[:, ::thin], plain.constrained_samples),
         # the funnel's exact law: v ~ N(0, 3) and u_i = x_i e^(-v/2) ~ N(0, 1) independent of v
         # (the raw x_i have E x_i^2 = e^(9/2) carried by rare draws: an unreliable MCSE)
         "flow_moments_vs_exact": moments_vs_exact(
-            D, _funnel_standardised(fres.constrained_samples[:, ::max(1, final_steps // steps)]),
-            np.zeros(model.num_params), np.concatenate([[9.0], np.ones(model.num_params - 1)])),
+            D, _funnel_standardised(fres.constrained_samples[:, ::thin]), np.zeros(model.num_params),
+            np.concatenate([[9.0], np.ones(model.num_params - 1)])),
         "roofline": roofline, "cpu_baseline": cpu,
+        "vs_cpu": (flow_s["value"] / cpu["value"]) if (cpu and flow_s["value"]) else None,
     }
     print(json.dumps(line), flush=True)
     return 0
@@ -436,7 +510,7 @@ def bench_advi(args) -> int:
     log(f"bench: {wall:.3f}s -> {value:.1f} iterations/s; elbo {h[:50].mean():.2f} -> {h[-50:].mean():.2f}")
     # end-to-end figure: model-gradient flops of all particles over the wall time
     achieved = iters * n * FLOPS_PER_GRADIENT / wall / 1e12
-    roofline = {"bound": "mfma", "compute": "fp32 vector ALU (model gradient); end to end, not one kernel",
+    roofline = {"bound": "fp32-valu", "compute": "fp32 vector ALU (model gradient); end to end, not one kernel",
                 "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                 "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
                 "kernel": "whole ADVI iteration (per-kernel times in profiles/advi_hier_kernel_trace.md)",
@@ -470,6 +544,7 @@ def bench_advi(args) -> int:
             t_cpu += time.perf_counter() - ts
             k += 1
         cpu = {"value": k / t_cpu, "unit": "iterations/s", "cores": cpu_cores(), "kind": "port",
+               "cpu_model": cpu_info()["cpu_model"],
                "sample": f"{k} full-rank ADVI iterations x {n} particles in numpy around the oracle's binary64 "
                          f"gradient ({t_cpu:.1f} s)"}
         log(f"bench: cpu {cpu['value']:.2f} iterations/s")
@@ -514,9 +589,11 @@ def bench_gp_meads(args) -> int:
                "gradient_evals_per_s": res.sampling_gradient_evals / res.sampling_seconds,
                "divergence_rate": res.divergence_rate(), "step_size": float(res.step_size[0]),
                "hyper_mean": [float(v) for v in res.constrained_samples[:, :, :3].mean(axis=(0, 1))]}
-        out["rhat_ok"] = out["rhat_max"] < 1.01
-        log(f"bench: {label}: min ESS {out['min_bulk_ess']:.1f} rhat {out['rhat_max']:.4f} -> {out['value']:.2f} "
-            f"ESS/s; hyper means {out['hyper_mean']}")
+        out["rhat_ok"] = out["rhat_max"] < RHAT_MAX
+        out["ungated_value"] = out["value"]
+        out["value"] = gated(out["value"], out["rhat_max"])
+        log(f"bench: {label}: min ESS {out['min_bulk_ess']:.1f} rhat {out['rhat_max']:.4f} -> "
+            f"{out['ungated_value']:.2f} ESS/s; hyper means {out['hyper_mean']}")
         return out
 
     log(f"bench: GP Poisson N={N} batched_meads chains={chains} warmup={warmup} steps={steps}")
@@ -550,15 +627,19 @@ def bench_gp_meads(args) -> int:
         rounds.append(dict(flow_s, training_seconds=t_r))
     flops = 7.0 * N ** 3 / 3.0 + 8.0 * N * N  # factorisation with three tangents + the four triangular products
     achieved = plain.sampling_gradient_evals * flops / plain.sampling_kernel_seconds / 1e12
-    roofline = {"bound": "mfma", "compute": "binary64 batched Cholesky with tangents (vector FP64)",
-                "achieved": achieved, "peak": 78.6, "unit": "TFLOP/s", "frac": achieved / 78.6, "traffic": None,
+    roofline = {"bound": "fp64-valu", "compute": "binary64 batched Cholesky with tangents (vector FP64)",
+                "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS,
+                "traffic": None,
                 "kernel": "k_gp_column / k_gp_panel", "flops_per_chain_gradient": flops,
                 "timing": "HIP events around all sampling launches of the plain run (MEADS kernels included)"}
     cpu = None
     if not args.no_cpu_baseline:
         from oracle import binding as B
         om = B.OracleModel(model)
-        sub = 16
+        info = cpu_info()
+        cores = max(1, info["affinity_cores"])
+        B.set_threads(cores)
+        sub = min(chains, max(16, cores))
         params = plain.unconstrained_samples[:sub, -1, :].astype(np.float64).T.copy()
         n, t_cpu = 0, 0.0
         while t_cpu < args.cpu_seconds and n < 20:
@@ -567,14 +648,14 @@ def bench_gp_meads(args) -> int:
             t_cpu += time.perf_counter() - ts
             n += 1
         s_iter = t_cpu / n * chains / sub  # MEADS: one gradient per chain per iteration
-        cores = min(int(os.environ.get("OMP_NUM_THREADS") or cpu_cores()), os.cpu_count() or 1)
-        cpu = {"value": (plain_s["min_bulk_ess"] / steps) / s_iter, "unit": "ESS/s (projected)", "cores": cores,
-               "kind": "port", "sample": f"{n} binary64 oracle GP gradients of {sub} chains ({t_cpu:.1f} s), scaled "
+        cpu = {"value": (plain_s["min_bulk_ess"] / steps) / s_iter, "unit": "ESS/s (projected, not R-hat gated)",
+               "cores": cores, "kind": "port", "cpu_model": info["cpu_model"], "host": info, "sample": f"{n} binary64 oracle GP gradients of {sub} chains ({t_cpu:.1f} s), scaled "
                                           f"to {chains} chains (one gradient per MEADS iteration)",
                "cpu_s_per_iteration": s_iter}
         log(f"bench: cpu {s_iter:.1f} s/iter -> projected {cpu['value']:.4f} ESS/s")
     line = {
-        "metric": METRIC, "value": flow_s["value"], "unit": "ESS/s", "n_gpus": 1, "steps": fres.num_samples,
+        "metric": METRIC, "value": flow_s["value"], "ungated_value": flow_s["ungated_value"], "unit": "ESS/s",
+        "n_gpus": 1, "steps": fres.num_samples,
         "warmup": warmup, "ms_per_step": 1e3 * fres.sampling_seconds / fres.num_samples, "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None, "dtype": "f64 (factorisation) + f32", "data": "synthetic (Philox4x32-10, seed %d)" % args.seed,
@@ -607,8 +688,9 @@ def bench_glm_chees(args) -> int:
         f"T {res.trajectory_trace[-1] if warmup else 0:.4f} eps {res.step_size[0]:.5f}")
     rh, eb = D.rhat_and_ess_bulk(res.constrained_samples)
     min_ess, rhat_max = float(np.nanmin(eb)), float(np.nanmax(rh))
-    value = min_ess / res.sampling_seconds
-    log(f"bench: rhat max {rhat_max:.4f} min bulk ESS {min_ess:.1f} -> {value:.1f} ESS/s")
+    raw = min_ess / res.sampling_seconds
+    value = gated(raw, rhat_max)
+    log(f"bench: rhat max {rhat_max:.4f} min bulk ESS {min_ess:.1f} -> {raw:.1f} ESS/s")
     evals = res.sampling_gradient_evals
     flops = evals * 4.0 * N * Dm
     ksec = res.sampling_kernel_seconds
@@ -624,8 +706,10 @@ def bench_glm_chees(args) -> int:
         from oracle import binding as B
         om = B.OracleModel(model)
         # the oracle gradient runs an OpenMP loop over chains with the default team
-        cores = min(int(os.environ.get("OMP_NUM_THREADS") or cpu_cores()), os.cpu_count() or 1)
-        sub = 64
+        info = cpu_info()
+        cores = max(1, info["affinity_cores"])
+        B.set_threads(cores)
+        sub = min(chains, max(64, cores))
         params = res.unconstrained_samples[:sub, -1, :].astype(np.float64).T.copy()
         n, t_cpu = 0, 0.0
         while t_cpu < args.cpu_seconds and n < 50:
@@ -636,14 +720,16 @@ def bench_glm_chees(args) -> int:
         s_batch = t_cpu / n * chains / sub
         s_iter = s_batch * mean_steps
         cpu = {"value": (min_ess / steps) / s_iter, "unit": "ESS/s (projected)", "cores": cores, "kind": "port",
+               "cpu_model": info["cpu_model"], "host": info,
                "sample": f"{n} binary64 oracle gradients of {sub} chains ({t_cpu:.1f} s), scaled to {chains} "
                          f"chains x {mean_steps:.2f} leapfrog steps per iteration; projected = GPU bulk ESS per "
                          f"iteration / CPU s per iteration",
                "cpu_s_per_batch_gradient": s_batch}
         log(f"bench: cpu {s_batch:.2f} s per batch gradient -> projected {cpu['value']:.3f} ESS/s")
     line = {
-        "metric": METRIC, "value": value, "unit": "ESS/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
-        "ms_per_step": 1e3 * res.sampling_seconds / steps, "higher_is_better": True, "scaling": "weak",
+        "metric": METRIC, "value": value, "ungated_value": raw, "unit": "ESS/s", "n_gpus": 1, "steps": steps,
+        "warmup": warmup, "ms_per_step": 1e3 * res.sampling_seconds / steps, "higher_is_better": True,
+        "scaling": "weak",
         "vs_baseline": None, "dtype": "bf16", "data": "synthetic (Philox4x32-10, seed %d)" % args.seed,
         "config": {"workload": "logistic regression N=131072 D=256 bf16 X, batched_chees lock-step, "
                                f"{chains} chains, fused MFMA gradient",
@@ -653,7 +739,8 @@ def bench_glm_chees(args) -> int:
         "gradient_evals_per_s": evals / res.sampling_seconds, "mean_leapfrog_steps": mean_steps,
         "trajectory_time": float(res.trajectory_trace[-1]) if warmup else None,
         "step_size": float(res.step_size[0]), "divergence_rate": res.divergence_rate(),
-        "roofline": roofline, "cpu_baseline": cpu, "vs_cpu": (value / cpu["value"]) if cpu else None,
+        "roofline": roofline, "cpu_baseline": cpu,
+        "vs_cpu": (value / cpu["value"]) if (cpu and value is not None) else None,
     }
     if args.flow:
         flow = ut.AffineCouplingFlow(model.num_params, 6, 256, seed=args.seed)
@@ -666,17 +753,18 @@ def bench_glm_chees(args) -> int:
                                 forward_velocity=args.forward_velocity, flow=flow)
         frh, feb = D.rhat_and_ess_bulk(fres.constrained_samples)
         fmin = float(np.nanmin(feb))
-        fval = fmin / fres.sampling_seconds
+        fval_raw = fmin / fres.sampling_seconds
+        fval = gated(fval_raw, float(np.nanmax(frh)))
         mom = moments_vs(D, fres.constrained_samples, res.constrained_samples)
         log(f"bench: flow: rhat {float(np.nanmax(frh)):.4f} min ESS {fmin:.1f} -> {fval:.1f} ESS/s, "
             f"{float(fres.integration_steps.mean()):.2f} steps/iter; vs plain {mom}")
-        line["flow"] = {"value": fval, "min_bulk_ess": fmin, "rhat_max": float(np.nanmax(frh)),
+        line["flow"] = {"value": fval, "ungated_value": fval_raw, "min_bulk_ess": fmin, "rhat_max": float(np.nanmax(frh)),
                         "sampling_seconds": fres.sampling_seconds, "warmup_seconds": fres.warmup_seconds,
                         "mean_leapfrog_steps": float(fres.integration_steps.mean()),
                         "gradient_evals_per_s": fres.sampling_gradient_evals / fres.sampling_seconds,
                         "step_size": float(fres.step_size[0]), "training_seconds": t_train,
                         "moments_vs_plain": mom}
-        line["flow_vs_plain"] = fval / value if value > 0 else None
+        line["flow_vs_plain"] = fval / value if (fval is not None and value) else None
     print(json.dumps(line), flush=True)
     return 0
 

@@ -66,7 +66,8 @@ typedef struct ut_flow ut_flow;
 
 /* Thread-local message for the last failing call on this thread. */
 const char* ut_last_error(void);
-/* ABI version: bumps on any incompatible change. */
+/* ABI version: bumps on any incompatible change (2: progress callbacks in the
+ * sampler option structs). */
 int32_t ut_abi_version(void);
 /* 1 when a HIP device is visible, 0 otherwise (no error). */
 int32_t ut_device_available(void);
@@ -183,6 +184,21 @@ int ut_logjoint_gradient_noalloc(const ut_plan* plan, int32_t num_chains, const 
 /* ---- samplers ------------------------------------------------------------------
  * Options mirror the keyword arguments of batched_nuts / batched_hmc
  * (src/inference/api_batched_nuts.jl:22-51, api_batched_hmc.jl:16-39).          */
+
+/* Progress callback (reference `callback` / `callback_every` keywords and
+ * _invoke_progress_callback, src/inference/api_single_chain.jl:1-30): called on the
+ * calling thread with phase UT_PHASE_WARMUP or UT_PHASE_SAMPLE, the 1-based
+ * iteration within the phase, the phase's total, the current step size (the mean
+ * over chains where each chain adapts its own; NaN where meaningless) and the
+ * cumulative divergent transitions of the phase, every callback_every iterations
+ * (<= 0: only at the end of each phase) and on each phase's final iteration.
+ * NULL (the default) adds no work.  Draws do not depend on whether a callback is
+ * set; each call costs one synchronisation (the persistent samplers then end a
+ * launch at every reported iteration).                                          */
+enum { UT_PHASE_WARMUP = 0, UT_PHASE_SAMPLE = 1 };
+typedef void (*ut_progress_fn)(void* user, int32_t phase, int32_t iteration, int32_t total, double step_size,
+                               int64_t divergences);
+
 typedef struct ut_sampler_opts {
   int32_t num_samples;
   int32_t num_warmup;
@@ -200,6 +216,9 @@ typedef struct ut_sampler_opts {
   int32_t init_max_retries;       /* 100 */
   const double* initial_params;   /* P x C or NULL */
   uint64_t seed;                  /* run seed; draws are a pure function of it */
+  ut_progress_fn callback;        /* NULL: no progress reports */
+  void* callback_user;            /* passed back to callback */
+  int32_t callback_every;         /* 10 */
 } ut_sampler_opts;
 
 /* Fills opts with the reference defaults (num_samples/num_warmup zero). */
@@ -296,6 +315,9 @@ typedef struct ut_meads_opts {
   double divergence_threshold;    /* 1000 */
   const double* initial_params;   /* P x C or NULL */
   uint64_t seed;
+  ut_progress_fn callback;        /* as in ut_sampler_opts */
+  void* callback_user;
+  int32_t callback_every;         /* 10 */
 } ut_meads_opts;
 
 void ut_meads_opts_default(ut_meads_opts* opts);

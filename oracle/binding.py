@@ -35,6 +35,7 @@ def _load():
         "orc_model_gp": (vp, [vp, vp, i32, i32, dbl, dbl, dbl, dbl]),
         "orc_gp_eval": (None, [vp, vp, i32, vp, vp, vp]),
         "orc_model_free": (None, [vp]),
+        "orc_set_threads": (None, [i32]),
         "orc_model_num_params": (i32, [vp]),
         "orc_logjoint_grad_f64": (None, [vp, vp, i32, vp, vp]),
         "orc_logjoint_grad_twin": (None, [vp, i32, vp, i32, vp, vp]),
@@ -65,6 +66,11 @@ def _load():
         f.argtypes = args
     _lib = L
     return L
+
+
+def set_threads(n: int) -> None:
+    """OpenMP threads of the oracle entry points that take no threads argument."""
+    _load().orc_set_threads(int(n))
 
 
 def _p(a):
@@ -240,7 +246,8 @@ class OracleModel:
         if rc != 0:
             raise RuntimeError(f"oracle sampler failed with status {rc}")
         res.update(samples=samples, accepted=acc.astype(bool), divergent=div.astype(bool), tree_depth=depth,
-                   integration_steps=steps, step_size=step, inverse_mass=im, trajectory_trace=trace)
+                   integration_steps=steps, step_size=step, inverse_mass=im, trajectory_trace=trace,
+                   warmup_seconds=out.warmup_seconds, sampling_seconds=out.sampling_seconds)
         return res
 
 

@@ -35,6 +35,7 @@
 #include <omp.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -1005,7 +1006,12 @@ int run_sampler(const Pol& pol, const ut_sampler_opts* o, int C, bool nuts, int 
   std::vector<TransOut> outs(C);
   std::vector<double> colbuf(P);
   const int last = stop_after ? std::min(*stop_after, W + S) : W + S;
+  // phase wall times (the reference harness times warmup and sampling apart)
+  using clk = std::chrono::steady_clock;
+  const clk::time_point t_start = clk::now();
+  clk::time_point t_warm = t_start;
   for (int it = 1; it <= last; ++it) {
+    if (it == W + 1) t_warm = clk::now();
     const uint32_t itc = rs + (uint32_t)it;
 #pragma omp parallel for schedule(dynamic, 1)
     for (int c = 0; c < C; ++c) {
@@ -1077,6 +1083,10 @@ int run_sampler(const Pol& pol, const ut_sampler_opts* o, int C, bool nuts, int 
   if (out) {
     if (out->step_size) for (int c = 0; c < C; ++c) out->step_size[c] = step[c];
     if (out->inverse_mass) for (int a = 0; a < P; ++a) out->inverse_mass[a] = im_d[a];
+    const clk::time_point t_end = clk::now();
+    if (last <= W) t_warm = t_end;
+    out->warmup_seconds = std::chrono::duration<double>(t_warm - t_start).count();
+    out->sampling_seconds = std::chrono::duration<double>(t_end - t_warm).count();
   }
   return UT_OK;
 }
@@ -2087,6 +2097,11 @@ void orc_gp_eval(void* h, const double* params, int32_t C, double* lj, double* g
   const Model& m = *(Model*)h;
 #pragma omp parallel for
   for (int c = 0; c < C; ++c) lj[c] = gp_eval(m, params + (size_t)c * m.P, grad + (size_t)c * m.P, logdet + c);
+}
+
+// OpenMP team size of the entry points without a threads argument (n <= 0 keeps it)
+void orc_set_threads(int32_t n) {
+  if (n > 0) omp_set_num_threads(n);
 }
 
 void orc_model_free(void* h) { delete (Model*)h; }

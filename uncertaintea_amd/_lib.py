@@ -59,7 +59,16 @@ class SamplerOpts(ct.Structure):
         ("init_max_retries", ct.c_int32),
         ("initial_params", ct.POINTER(ct.c_double)),
         ("seed", ct.c_uint64),
+        ("callback", ct.c_void_p),
+        ("callback_user", ct.c_void_p),
+        ("callback_every", ct.c_int32),
     ]
+
+
+# void (*)(void* user, int32_t phase, int32_t iteration, int32_t total, double step_size, int64_t divergences)
+PROGRESS_FN = ct.CFUNCTYPE(None, ct.c_void_p, ct.c_int32, ct.c_int32, ct.c_int32, ct.c_double, ct.c_int64)
+UT_PHASE_WARMUP = 0
+UT_PHASE_SAMPLE = 1
 
 
 class CheesOpts(ct.Structure):
@@ -87,6 +96,9 @@ class MeadsOpts(ct.Structure):
         ("divergence_threshold", ct.c_double),
         ("initial_params", ct.POINTER(ct.c_double)),
         ("seed", ct.c_uint64),
+        ("callback", ct.c_void_p),
+        ("callback_user", ct.c_void_p),
+        ("callback_every", ct.c_int32),
     ]
 
 

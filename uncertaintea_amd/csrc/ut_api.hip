@@ -39,6 +39,9 @@ thread_local std::string g_last_error;
 
 int fail(int code, const std::string& msg) {
   g_last_error = msg;
+  // unsupported models point at the lowering report, as the reference's
+  // _device_unsupported_message does (src/device/workspace.jl:24-35)
+  if (code == UT_E_UNSUPPORTED) g_last_error += "\nSee device_lowering_report(model) for details.";
   return code;
 }
 
@@ -196,7 +199,7 @@ size_t align256(size_t b) { return (b + 255) / 256 * 256; }
 extern "C" {
 
 const char* ut_last_error(void) { return g_last_error.c_str(); }
-int32_t ut_abi_version(void) { return 1; }
+int32_t ut_abi_version(void) { return 2; }
 
 int32_t ut_device_available(void) {
   int n = 0;
@@ -651,6 +654,9 @@ void ut_sampler_opts_default(ut_sampler_opts* o) {
   o->init_max_retries = 100;
   o->initial_params = nullptr;
   o->seed = 0;
+  o->callback = nullptr;
+  o->callback_user = nullptr;
+  o->callback_every = 10;
 }
 
 int ut_host_alloc(int64_t bytes, void** out) {
@@ -828,6 +834,68 @@ int progress_every() {
   return n;
 }
 
+// Progress callback of the sampler options (reference _invoke_progress_callback,
+// src/inference/api_single_chain.jl:12-30): due every `every` iterations of a phase
+// and on its final iteration.  Divergences are counted on the host from the
+// per-iteration flags the samplers already record (warmup: hist_flags, W x C;
+// sampling: o_divergent, C x S), over the iterations since the last report.
+struct Progress {
+  ut_progress_fn fn = nullptr;
+  void* user = nullptr;
+  int every = 10;
+  long long warm_divs = 0, samp_divs = 0;
+  int warm_done = 0, samp_done = 0;
+  std::vector<uint8_t> buf;
+  bool due(int iteration, int total) const {
+    return fn && iteration > 0 && ((every > 0 && iteration % every == 0) || iteration == total);
+  }
+  // next iteration in (from, to] at which a report is due, or `to`
+  int next_stop(int from, int to, int total) const {
+    if (!fn) return to;
+    for (int i = from + 1; i < to; ++i)
+      if (due(i, total)) return i;
+    return to;
+  }
+  hipError_t warmup(const SamplerParams& sp, int C, int it, int W, double step, hipStream_t st) {
+    if (!due(it, W)) return hipSuccess;
+    const size_t n = (size_t)(it - warm_done) * C;
+    buf.resize(n);
+    hipError_t e = hipStreamSynchronize(st);
+    if (e == hipSuccess && n && sp.hist_flags) {
+      e = hipMemcpy(buf.data(), sp.hist_flags + (size_t)warm_done * C, n, hipMemcpyDeviceToHost);
+      for (uint8_t f : buf) warm_divs += (f & FLAG_DIVERGENT) ? 1 : 0;
+    }
+    if (e != hipSuccess) return e;
+    warm_done = it;
+    fn(user, UT_PHASE_WARMUP, it, W, step, warm_divs);
+    return hipSuccess;
+  }
+  hipError_t sample(const SamplerParams& sp, int C, int s, int S, double step, hipStream_t st) {
+    if (!due(s, S)) return hipSuccess;
+    const size_t w = (size_t)(s - samp_done);
+    buf.resize(w * C);
+    hipError_t e = hipStreamSynchronize(st);
+    if (e == hipSuccess && w)
+      e = hipMemcpy2D(buf.data(), w, sp.o_divergent + samp_done, (size_t)S, w, (size_t)C, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return e;
+    for (uint8_t f : buf) samp_divs += f ? 1 : 0;
+    samp_done = s;
+    fn(user, UT_PHASE_SAMPLE, s, S, step, samp_divs);
+    return hipSuccess;
+  }
+};
+
+// mean over chains of the per-chain step sizes (the reported step of the
+// per-chain samplers)
+hipError_t mean_step(const double* d_step, int C, std::vector<double>& tmp, double* out) {
+  tmp.resize(C);
+  hipError_t e = hipMemcpy(tmp.data(), d_step, (size_t)C * 8, hipMemcpyDeviceToHost);
+  double s = 0.0;
+  for (double v : tmp) s += v;
+  *out = C > 0 ? s / C : std::numeric_limits<double>::quiet_NaN();
+  return e;
+}
+
 int init_chains(ut_workspace* ws, const ut_sampler_opts* o, uint32_t rs) {
   const int P = ws->P, C = ws->C;
   const DevModelDesc& m = ws->plan->dev;
@@ -935,8 +1003,32 @@ int run_sampler(ut_workspace* ws, const ut_sampler_opts* o, ut_chain_out* out, b
   if (W > 0 && (bounds.empty() || bounds.back() != W)) bounds.push_back(W);
   int it = 1;
   int window = 0;
+  Progress prog;
+  prog.fn = o->callback;
+  prog.user = o->callback_user;
+  prog.every = o->callback_every;
+  std::vector<double> step_tmp;
+  auto report_step = [&](double* st) -> hipError_t {
+    hipError_t e = hipStreamSynchronize(stream);
+    return e == hipSuccess ? mean_step(ws->st.step, C, step_tmp, st) : e;
+  };
   for (int b : bounds) {
+    // with a progress callback the window's launch ends at every reported iteration
+    for (int a = it; a < b;) {
+      const int e = prog.next_stop(a - 1, b, W);
+      if (e == b) break;
+      UT_HIP(run_range(a, e));
+      double st = 0.0;
+      UT_HIP(report_step(&st));
+      UT_HIP(prog.warmup(sp, C, e, W, st, stream));
+      it = a = e + 1;
+    }
     UT_HIP(run_range(it, b));
+    if (prog.due(b, W)) {
+      double st = 0.0;
+      UT_HIP(report_step(&st));
+      UT_HIP(prog.warmup(sp, C, b, W, st, stream));
+    }
     const bool is_window_end = o->adapt_mass_matrix && window < (int)sch.ends.size() && sch.ends[window] == b;
     if (is_window_end) {
       const int start = window == 0 ? sch.initial_buffer + 1 : sch.ends[window - 1] + 1;
@@ -962,8 +1054,8 @@ int run_sampler(ut_workspace* ws, const ut_sampler_opts* o, ut_chain_out* out, b
   std::vector<hipEvent_t> done;
   UT_HIP(hipEventRecord(ev0, stream));
   const int chunk = std::max(kChunk, (S + kSampleLaunches - 1) / kSampleLaunches);
-  for (int s0 = 0; s0 < S; s0 += chunk) {
-    const int s1 = std::min(S, s0 + chunk);
+  for (int s0 = 0, s1 = 0; s0 < S; s0 = s1) {
+    s1 = prog.next_stop(s0, std::min(S, s0 + chunk), S);
     hipError_t e = nuts ? launch_nuts(m, ws->st, sp, W + 1 + s0, W + s1, stream)
                         : launch_hmc(m, ws->st, sp, W + 1 + s0, W + s1, stream);
     UT_HIP(e);
@@ -988,6 +1080,11 @@ int run_sampler(ut_workspace* ws, const ut_sampler_opts* o, ut_chain_out* out, b
     UT_HIP(copy2d(out->divergent, sp.o_divergent, 1));
     UT_HIP(copy2d(out->tree_depth, sp.o_depth, 4));
     UT_HIP(copy2d(out->integration_steps, sp.o_steps, 4));
+    if (prog.due(s1, S)) {
+      double st = 0.0;
+      UT_HIP(report_step(&st));
+      UT_HIP(prog.sample(sp, C, s1, S, st, stream));
+    }
   }
   UT_HIP(hipEventRecord(ev1, stream));
   unsigned long long evals = 0;
@@ -1199,6 +1296,10 @@ int run_lockstep(ut_workspace* ws, const ut_chees_opts* co, ut_chain_out* out, d
   double t_warm = t_start;
   long long evals = 0;
   int launches = 0;
+  Progress prog;
+  prog.fn = o->callback;
+  prog.user = o->callback_user;
+  prog.every = o->callback_every;
   for (int it = 1; it <= W + S; ++it) {
     if (progress_every() > 0 && it % progress_every() == 0) {
       UT_HIP(hipStreamSynchronize(stream));
@@ -1226,6 +1327,7 @@ int run_lockstep(ut_workspace* ws, const ut_chees_opts* co, ut_chain_out* out, d
     if (!warm) {
       evals += (long long)L * C;
       launches += 2 * L + 2;
+      UT_HIP(prog.sample(sp, C, it - W, S, eps, stream));
       continue;
     }
     UT_HIP(launch_chees_stats(ls, rec, ws->d_im, P, C, adapt_traj, co->forward_velocity ? -1.0 : 1.0, d_stats,
@@ -1247,6 +1349,7 @@ int run_lockstep(ut_workspace* ws, const ut_chees_opts* co, ut_chain_out* out, d
     if (adapt_traj) traj.apply(num, den, jitter, eps);
     if (trace && !hmc) trace[it - 1] = traj.T;
     if (it == W && o->adapt_step_size) eps = dual.final_step();
+    UT_HIP(prog.warmup(sp, C, it, W, eps, stream));
   }
   if (S == 0) {
     UT_HIP(hipStreamSynchronize(stream));
@@ -1389,6 +1492,10 @@ int run_meads(ut_workspace* ws, const ut_meads_opts* mo, ut_chain_out* out, doub
   UT_HIP(hipEventCreate(&ev0));
   UT_HIP(hipEventCreate(&ev1));
   double t_warm = t_start;
+  Progress prog;
+  prog.fn = mo->callback;
+  prog.user = mo->callback_user;
+  prog.every = mo->callback_every;
   for (int it = 1; it <= T; ++it) {
     if (it == W + 1) {
       UT_HIP(hipStreamSynchronize(stream));
@@ -1408,6 +1515,10 @@ int run_meads(ut_workspace* ws, const ut_meads_opts* mo, ut_chain_out* out, doub
     UT_HIP(batched_grad(ws, m.wq, m.wlj, m.wg, true));
     UT_HIP(launch_meads_end(m, rec, d_fold_of, P, C, skip, mo->divergence_threshold, stream));
     UT_HIP(launch_meads_record(m, rec, d_fold_of, sp, it, skip, stream));
+    // MEADS has no single step size (one per fold) and records no warmup flags
+    const double nan = std::numeric_limits<double>::quiet_NaN();
+    if (it <= W) UT_HIP(prog.warmup(sp, C, it, W, nan, stream));
+    else UT_HIP(prog.sample(sp, C, it - W, S, nan, stream));
   }
   UT_HIP(hipEventRecord(ev1, stream));
   const size_t SC = (size_t)S * C;
@@ -1479,6 +1590,7 @@ void ut_meads_opts_default(ut_meads_opts* o) {
   o->init_strategy = UT_INIT_PRIOR;
   o->init_max_retries = 100;
   o->divergence_threshold = 1000.0;
+  o->callback_every = 10;
 }
 
 int ut_meads_run(ut_workspace* ws, const ut_meads_opts* opts, ut_chain_out* out, double* adapt_trace) {
@@ -1699,8 +1811,10 @@ int ut_flow_train(ut_flow* flow, const float* draws, int64_t n, int32_t steps, i
   UT_HIP(buf.alloc(&d_rows, (size_t)batch));
   UT_HIP(hipMemcpyAsync(d_draws, draws, (size_t)n * f.P * 4, hipMemcpyHostToDevice, st));
   std::vector<double> rows(batch);
+  // minibatch rows are Philox draws keyed by both halves of the 64-bit seed
+  const uint32_t key = run_seed_of(seed);
   for (int s = 0; s < steps; ++s) {
-    UT_HIP(flow_gather_rows(d_draws, n, f.P, d_batch, batch, (uint32_t)seed, (uint32_t)s, st));
+    UT_HIP(flow_gather_rows(d_draws, n, f.P, d_batch, batch, key, (uint32_t)s, st));
     UT_HIP(flow_inverse(f, d_batch, d_z, nullptr, batch, st));
     if (loss_trace) UT_HIP(flow_loss_value(d_z, f.sc.logdet, batch, f.P, d_rows, st));
     UT_HIP(flow_loss_backward(f, d_z, batch, st));
@@ -2144,8 +2258,10 @@ int ut_flow_fit_elbo(ut_workspace* ws, ut_flow* flow, int32_t num_iterations, do
   UT_HIP(buf.alloc(&track, 1));
   ElboTrack t0{-std::numeric_limits<double>::infinity(), 0, -1, 0, 0};
   UT_HIP(hipMemcpyAsync(track, &t0, sizeof(t0), hipMemcpyHostToDevice, st));
+  // particles are Philox normals keyed by both halves of the 64-bit seed
+  const uint32_t key = run_seed_of(seed);
   for (int it = 0; it < num_iterations; ++it) {
-    UT_HIP(flow_normal_fill(b.z, B, P, (uint32_t)seed, (uint32_t)it, st));
+    UT_HIP(flow_normal_fill(b.z, B, P, key, (uint32_t)it, st));
     UT_HIP(elbo_eval(ws, f, b, gradient_clip, iwae_group));
     UT_HIP(flow_elbo_track(f, b.stats, it, hist, track, best, st));
     UT_HIP(flow_adam_ascent(f, b.stats, learning_rate, beta1, beta2, adam_epsilon, st, track));

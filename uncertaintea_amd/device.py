@@ -76,9 +76,11 @@ def device_batched_logjoint_gradient_inplace(ws: DeviceBatchedWorkspace, params)
 
 def device_batched_logjoint(model: TeaModel, params, args=(), constraints=None, *, backend="hip",
                             precision=np.float32) -> np.ndarray:
-    """Per-chain unconstrained logjoint of `params` (P x C)."""
+    """Per-chain unconstrained logjoint of `params` (P x C), the model conditioned on
+    `args` / `constraints` (TeaModel.condition)."""
     if backend != "hip":
         raise ValueError(f"backend must be 'hip', got {backend!r}")
+    model = model.condition(args, constraints)
     params = np.asarray(params)
     ws = DeviceBatchedWorkspace(model, params.shape[1], precision)
     return device_batched_logjoint_inplace(ws, params)
@@ -86,9 +88,25 @@ def device_batched_logjoint(model: TeaModel, params, args=(), constraints=None, 
 
 def device_batched_logjoint_gradient(model: TeaModel, params, args=(), constraints=None, *, backend="hip",
                                      precision=np.float32):
-    """(values[C], gradients[P x C]) of the unconstrained logjoint."""
+    """(values[C], gradients[P x C]) of the unconstrained logjoint, the model
+    conditioned on `args` / `constraints` (TeaModel.condition)."""
     if backend != "hip":
         raise ValueError(f"backend must be 'hip', got {backend!r}")
+    model = model.condition(args, constraints)
     params = np.asarray(params)
     ws = DeviceBatchedWorkspace(model, params.shape[1], precision)
     return device_batched_logjoint_gradient_inplace(ws, params)
+
+
+def device_lowering_report(model) -> dict:
+    """Whether and how `model` reaches the device (reference device_lowering_report,
+    the report every unsupported-model error points at): the plan it lowers to, its
+    parameter count, the arguments and observed choices it can be conditioned on,
+    the samplers that run it, and the issues that keep it off the device (empty
+    when lowerable).  Host-only: no device is touched."""
+    rep = model.lowering_report()
+    kind = rep.get("plan")
+    batched = kind in ("LogisticRegressionBF16", "GaussianProcessPoisson") or rep.get("batched_only", False)
+    rep["samplers"] = (["batched_chees", "batched_hmc(per_chain_adaptation=False)", "batched_meads"] if batched else
+                       ["batched_nuts", "batched_hmc", "batched_chees", "batched_meads"]) if rep["lowerable"] else []
+    return rep

@@ -12,6 +12,7 @@ from __future__ import annotations
 
 import ctypes as ct
 from dataclasses import dataclass, field
+from typing import NamedTuple
 
 import numpy as np
 
@@ -19,6 +20,45 @@ from . import _lib as L
 from . import diagnostics as D
 from .device import DeviceBatchedWorkspace
 from .models import TeaModel
+
+
+class ProgressInfo(NamedTuple):
+    """What a progress callback receives (the reference's NamedTuple,
+    src/inference/api_single_chain.jl:1-11)."""
+
+    phase: str          # "warmup" or "sample"
+    iteration: int      # 1-based within the phase
+    total: int          # iterations in the phase
+    step_size: float    # mean over chains of the per-chain steps; NaN for MEADS
+    divergences: int    # cumulative divergent transitions of the phase (warmup MEADS: 0, untracked)
+
+
+class _Callback:
+    """Holds the C function pointer for one sampler call and re-raises the first
+    exception the Python callback raised once the call has returned."""
+
+    def __init__(self, fn):
+        self.fn = fn
+        self.error = None
+
+        def trampoline(user, phase, iteration, total, step_size, divergences):
+            if self.error is not None:
+                return
+            try:
+                self.fn(ProgressInfo("warmup" if phase == L.UT_PHASE_WARMUP else "sample", int(iteration),
+                                     int(total), float(step_size), int(divergences)))
+            except BaseException as e:  # noqa: BLE001 -- re-raised after the C call returns
+                self.error = e
+
+        self.c_fn = L.PROGRESS_FN(trampoline)
+
+    def install(self, opts, every: int) -> None:
+        opts.callback = ct.cast(self.c_fn, ct.c_void_p)
+        opts.callback_every = int(every)
+
+    def reraise(self) -> None:
+        if self.error is not None:
+            raise self.error
 
 
 @dataclass
@@ -112,9 +152,16 @@ def _opts(num_samples, num_warmup, step_size, target_accept, divergence_threshol
 
 
 def _run(kind: str, model: TeaModel, num_chains: int, opts, initial_params,
-         max_tree_depth: int, num_leapfrog_steps: int, target_accept: float, flow=None) -> HMCChains:
+         max_tree_depth: int, num_leapfrog_steps: int, target_accept: float, flow=None, callback=None,
+         callback_every: int = 10) -> HMCChains:
     if num_chains <= 0:
         raise ValueError("num_chains must be positive")
+    cb = None
+    if callback is not None:
+        if not callable(callback):
+            raise ValueError("callback must be callable (it receives a ProgressInfo)")
+        cb = _Callback(callback)
+        cb.install(opts.base if kind == "chees" else opts, callback_every)
     ws = DeviceBatchedWorkspace(model, num_chains, np.float32)
     if flow is not None:
         if flow.num_params != model.num_params:
@@ -159,6 +206,8 @@ def _run(kind: str, model: TeaModel, num_chains: int, opts, initial_params,
     else:
         fn = {"nuts": L.lib().ut_nuts_run, "hmc": L.lib().ut_hmc_run, "hmc_shared": L.lib().ut_hmc_shared_run}[kind]
         L.check(fn(ws.handle, ct.byref(opts), ct.byref(out)))
+    if cb is not None:
+        cb.reraise()
     kind = "hmc" if kind == "hmc_shared" else kind
     return HMCChains(kind, model, samples, lj, acc, en, err, accepted.view(bool), div.view(bool), depth,
                      steps, step, im, max_tree_depth, num_leapfrog_steps, target_accept, out.warmup_seconds,
@@ -173,8 +222,9 @@ def batched_nuts(model: TeaModel, args=(), constraints=None, *, num_chains: int,
                  find_reasonable_step_size: bool = False, divergence_threshold: float = 1000.0,
                  mass_matrix_regularization: float = 1e-3, mass_matrix_min_samples: int = 10,
                  tree_strategy: str = "persistent", backend: str = "hip", precision=np.float32,
-                 seed: int = 0) -> HMCChains:
+                 seed: int = 0, callback=None, callback_every: int = 10) -> HMCChains:
     """Many NUTS chains, each building its whole tree in one persistent kernel."""
+    model = model.condition(args, constraints)
     if tree_strategy != "persistent":
         raise ValueError(f"the gfx950 backend implements tree_strategy='persistent', got {tree_strategy!r}")
     if backend != "hip":
@@ -186,7 +236,8 @@ def batched_nuts(model: TeaModel, args=(), constraints=None, *, num_chains: int,
     o = _opts(num_samples, num_warmup, step_size, target_accept, divergence_threshold, mass_matrix_regularization,
               mass_matrix_min_samples, adapt_step_size, adapt_mass_matrix, find_reasonable_step_size, init_strategy,
               init_max_retries, seed, max_tree_depth=max_tree_depth)
-    return _run("nuts", model, num_chains, o, initial_params, max_tree_depth, 0, target_accept)
+    return _run("nuts", model, num_chains, o, initial_params, max_tree_depth, 0, target_accept, callback=callback,
+                callback_every=callback_every)
 
 
 def batched_hmc(model: TeaModel, args=(), constraints=None, *, num_chains: int, num_samples: int,
@@ -195,7 +246,8 @@ def batched_hmc(model: TeaModel, args=(), constraints=None, *, num_chains: int, 
                 adapt_step_size: bool = True, adapt_mass_matrix: bool = True, per_chain_adaptation=None,
                 find_reasonable_step_size: bool = False, divergence_threshold: float = 1000.0,
                 mass_matrix_regularization: float = 1e-3, mass_matrix_min_samples: int = 10,
-                backend: str = "hip", precision=np.float32, seed: int = 0, flow=None) -> HMCChains:
+                backend: str = "hip", precision=np.float32, seed: int = 0, flow=None, callback=None,
+                callback_every: int = 10) -> HMCChains:
     """Many fixed-length HMC chains, device-resident (momentum, accept and adaptation on the GPU).
 
     per_chain_adaptation (default True, as the reference): each chain adapts its own
@@ -203,6 +255,7 @@ def batched_hmc(model: TeaModel, args=(), constraints=None, *, num_chains: int, 
     chains in lock step with one dual-averaged step size.  `flow` (an
     AffineCouplingFlow) preconditions the shared mode: chains run in the flow's base
     space and draws are returned mapped through the flow."""
+    model = model.condition(args, constraints)
     if backend != "hip":
         raise ValueError(f"backend must be 'hip', got {backend!r}")
     if np.dtype(precision) != np.float32:
@@ -214,7 +267,7 @@ def batched_hmc(model: TeaModel, args=(), constraints=None, *, num_chains: int, 
               mass_matrix_min_samples, adapt_step_size, adapt_mass_matrix, find_reasonable_step_size, init_strategy,
               init_max_retries, seed, num_leapfrog_steps=num_leapfrog_steps)
     return _run("hmc_shared" if shared else "hmc", model, num_chains, o, initial_params, 0, num_leapfrog_steps,
-                target_accept, flow=flow)
+                target_accept, flow=flow, callback=callback, callback_every=callback_every)
 
 
 def batched_chees(model: TeaModel, args=(), constraints=None, *, num_chains: int, num_samples: int,
@@ -227,7 +280,8 @@ def batched_chees(model: TeaModel, args=(), constraints=None, *, num_chains: int
                   adapt_mass_matrix: bool = True, find_reasonable_step_size: bool = False,
                   divergence_threshold: float = 1000.0, mass_matrix_regularization: float = 1e-3,
                   mass_matrix_min_samples: int = 10, backend: str = "hip", precision=np.float32,
-                  seed: int = 0, forward_velocity: bool = False, flow=None) -> HMCChains:
+                  seed: int = 0, forward_velocity: bool = False, flow=None, callback=None,
+                  callback_every: int = 10) -> HMCChains:
     """ChEES-HMC: every chain integrates the same Halton-jittered number of leapfrog
     steps with a shared step size; warmup adapts the step (dual averaging to the
     harmonic-mean acceptance), a pooled diagonal mass and the trajectory time T
@@ -238,6 +292,7 @@ def batched_chees(model: TeaModel, args=(), constraints=None, *, num_chains: int
     reference feeds the criterion the proposal momentum after its final flip, so
     its Adam ascent shortens T; True uses the velocity in the direction of
     integration, under which T grows where longer trajectories pay off."""
+    model = model.condition(args, constraints)
     if backend != "hip":
         raise ValueError(f"backend must be 'hip', got {backend!r}")
     if np.dtype(precision) != np.float32:
@@ -257,13 +312,15 @@ def batched_chees(model: TeaModel, args=(), constraints=None, *, num_chains: int
     co.trajectory_adam_beta2 = float(trajectory_adam_beta2)
     co.trajectory_adam_epsilon = float(trajectory_adam_epsilon)
     co.forward_velocity = int(bool(forward_velocity))
-    return _run("chees", model, num_chains, co, initial_params, 0, num_leapfrog_steps, target_accept, flow=flow)
+    return _run("chees", model, num_chains, co, initial_params, 0, num_leapfrog_steps, target_accept, flow=flow,
+                callback=callback, callback_every=callback_every)
 
 
 def batched_meads(model: TeaModel, args=(), constraints=None, *, num_chains: int, num_samples: int,
                   num_warmup: int = 0, num_folds: int = 4, initial_params=None, init_strategy: str = "prior",
                   init_max_retries: int = 100, divergence_threshold: float = 1000.0, backend: str = "hip",
-                  precision=np.float32, seed: int = 0, flow=None) -> HMCChains:
+                  precision=np.float32, seed: int = 0, flow=None, callback=None,
+                  callback_every: int = 10) -> HMCChains:
     """MEADS: generalized HMC, one leapfrog step per iteration with persistent
     momentum and a persistent-slice accept; step, damping, drift and a diagonal
     preconditioner from the neighbouring fold every iteration (no adaptation phase,
@@ -271,6 +328,7 @@ def batched_meads(model: TeaModel, args=(), constraints=None, *, num_chains: int
     (mean step, mean damping, largest std) -- the reference's `_adapt_trace`.
     The reference runs MEADS on the host only; here the fold reductions and the
     transitions run on the device."""
+    model = model.condition(args, constraints)
     if backend != "hip":
         raise ValueError(f"backend must be 'hip', got {backend!r}")
     if np.dtype(precision) != np.float32:
@@ -286,4 +344,5 @@ def batched_meads(model: TeaModel, args=(), constraints=None, *, num_chains: int
     mo.init_max_retries = int(init_max_retries)
     mo.divergence_threshold = float(divergence_threshold)
     mo.seed = int(seed) & 0xFFFFFFFFFFFFFFFF
-    return _run("meads", model, num_chains, mo, initial_params, 0, 1, float("nan"), flow=flow)
+    return _run("meads", model, num_chains, mo, initial_params, 0, 1, float("nan"), flow=flow, callback=callback,
+                callback_every=callback_every)

@@ -36,6 +36,42 @@ class TeaModel:
         """Map unconstrained draws (..., P) to constrained values (..., P)."""
         return np.asarray(unconstrained, dtype=np.float64)
 
+    # conditioning -----------------------------------------------------------
+    # The reference conditions a model at workspace creation on its arguments and a
+    # choicemap of observed values (src/device/api.jl:34-88, staged once by
+    # src/device/staging.jl:499-550).  Here a model object carries its data, so
+    # conditioning returns a model of the same family with the arguments and
+    # observations replaced; `args` and `constraints` name what each family takes.
+    arg_names: tuple = ()
+    observed_names: tuple = ()
+
+    def condition(self, args=(), constraints=None) -> "TeaModel":
+        """The model conditioned on `args` (a tuple in the order of `arg_names`;
+        () keeps the model's own) and `constraints` (a mapping from the names in
+        `observed_names` to observed values; None or {} keeps the model's own)."""
+        args = tuple(args) if args is not None else ()
+        cons = dict(constraints or {})
+        if not args and not cons:
+            return self
+        if args and len(args) != len(self.arg_names):
+            raise ValueError(f"DimensionMismatch: {self.name} takes {len(self.arg_names)} model arguments "
+                             f"{self.arg_names}, got {len(args)}")
+        unknown = sorted(set(cons) - set(self.observed_names))
+        if unknown:
+            raise ValueError(f"model {self.name} cannot be lowered to the device logjoint path with constraints on "
+                             f"{unknown}: only the observed choices {list(self.observed_names)} can be constrained "
+                             f"(latent choices stay parameters).\nSee device_lowering_report(model) for details.")
+        return self._conditioned(args, cons)
+
+    def _conditioned(self, args: tuple, constraints: dict) -> "TeaModel":
+        raise ValueError(f"model {self.name} takes no arguments or observations.\n"
+                         "See device_lowering_report(model) for details.")
+
+    def lowering_report(self) -> dict:
+        """How this model reaches the device (see device_lowering_report)."""
+        return {"model": self.name, "lowerable": True, "plan": type(self).__name__, "num_params": self.num_params,
+                "arguments": list(self.arg_names), "observed": list(self.observed_names), "issues": []}
+
     # device plan ------------------------------------------------------------
     def _create_plan(self, out: ct.c_void_p) -> int:
         raise NotImplementedError
@@ -69,6 +105,9 @@ class LogisticRegression(TeaModel):
         self.prior_loc = float(prior_loc)
         self.prior_scale = float(prior_scale)
 
+    arg_names = ("x",)
+    observed_names = ("y",)
+
     @property
     def num_params(self):
         return self.x.shape[1]
@@ -76,6 +115,10 @@ class LogisticRegression(TeaModel):
     @property
     def param_names(self):
         return [f"beta[{d + 1}]" for d in range(self.num_params)]
+
+    def _conditioned(self, args, constraints):
+        x = args[0] if args else (self.x_bf16 if hasattr(self, "x_bf16") else self.x)
+        return type(self)(x, constraints.get("y", self.y), self.prior_loc, self.prior_scale)
 
     def _create_plan(self, out):
         n, d = self.x.shape
@@ -125,6 +168,14 @@ class HierarchicalNormalNC(TeaModel):
         if self.offsets.shape[0] != self.sigma.shape[0] + 1 or self.offsets[-1] != self.y.shape[0]:
             raise ValueError("DimensionMismatch: group_offsets must have J+1 entries ending at len(y)")
         self.mu_loc, self.mu_scale, self.tau_scale = float(mu_loc), float(mu_scale), float(tau_scale)
+
+    arg_names = ("group_offsets", "sigma")
+    observed_names = ("y",)
+
+    def _conditioned(self, args, constraints):
+        off, sig = args if args else (self.offsets, self.sigma)
+        return HierarchicalNormalNC(constraints.get("y", self.y), off, sig, self.mu_loc, self.mu_scale,
+                                    self.tau_scale)
 
     @property
     def num_groups(self):
@@ -189,6 +240,14 @@ class GaussianProcessPoisson(TeaModel):
             raise ValueError("DimensionMismatch: inputs must be N x D with N = len(counts)")
         self.noise, self.ls_loc, self.ls_scale, self.amp_scale = float(noise), float(ls_loc), float(ls_scale), \
             float(amp_scale)
+
+    arg_names = ("inputs",)
+    observed_names = ("y",)
+
+    def _conditioned(self, args, constraints):
+        inputs = args[0] if args else self.inputs
+        return GaussianProcessPoisson(inputs, constraints.get("y", self.counts), self.noise, self.ls_loc,
+                                      self.ls_scale, self.amp_scale)
 
     @property
     def num_points(self):
