@@ -136,6 +136,125 @@ int ut_gp_factor(const ut_plan* plan, const double* hyper, int32_t num_chains, d
  * Parameter order: (v, x_1..x_dim); P = dim + 1.                                */
 int ut_plan_funnel(int32_t dim, double v_scale, ut_plan** out);
 
+/* ---- general static execution plans -------------------------------------------
+ * The flat, serialised form of the reference's isbits DeviceExecutionPlan
+ * (src/device/plan/steps.jl:1-539, lowered by src/device/plan/lowering.jl:1360-1478)
+ * with its staged observation stream (src/device/staging.jl:499-550): a step list
+ * in the reference's pre-order, expression nodes, and the observed rows, trip
+ * counts and loop starts.  Indices into params and slots are 1-based as in the
+ * reference.
+ *
+ * Steps.  UT_STEP_CHOICE scores a family at argument expressions args[0..2]; a
+ * latent choice (value_source > 0) reads unconstrained row value_source through
+ * `transform` and adds the transform's log-abs-det; an observed choice
+ * (value_source == 0) reads the next staged row at top level, or row value_offset
+ * of the iteration's block inside a loop; binding_slot > 0 stores the
+ * (constrained) value.  UT_FAMILY_NONCENTERED_NORMAL scores its latent against
+ * N(0, 1) and binds mu + sigma z.  UT_FAMILY_DIAG_NORMAL (top level) scores
+ * `dimension` rows against one normal(args[0], args[1]) (the reference's
+ * DeviceDiagNormalChoiceStepDyn; observed: `dimension` staged rows).
+ * UT_FAMILY_BERNOULLI_LOGIT_GLM (loop bodies) scores the staged outcome at
+ * value_offset + dimension against bernoullilogit(args[0] + sum_d
+ * coef[d] x_d), coef[d] = row coef_source + d, x_d = staged row value_offset + d
+ * (DeviceBernoulliLogitGLMChoiceStepDyn).  UT_STEP_DETERMINISTIC binds args[0].
+ * UT_STEP_LOOP runs its body (the next body_length steps) trip_counts[loop_id]
+ * times, iterator_slot = loop_starts[loop_id] + t, each iteration reading a block
+ * of `stride` staged rows.  As in the reference: loops are not nested and hold no
+ * latent choices; values bound inside a body are read only inside it.
+ *
+ * Expression nodes (post-order: operands precede their node; an expression is
+ * named by its root node).  UT_OP_LITERAL `value`; UT_OP_SLOT slot `index`;
+ * UT_OP_PARAM unconstrained row index + (a >= 0 ? round(node a) : 0), where node a
+ * depends on staged data only (a vector latent indexed by data, e.g. theta[g[i]]);
+ * UT_OP_OBS row `index` of the iteration's block (loop bodies only: covariates,
+ * group indices, known scales); arithmetic ops on nodes a (and b).
+ *
+ * Families and transforms follow src/device/math.jl:50-235, 1137-1159: out of
+ * support scores -Inf, never an error.  Gradients are reverse mode through the
+ * plan (per step and per loop iteration), fp32 or fp64 as the workspace.
+ *
+ * flags: UT_PLAN_GENERIC always runs the plan interpreter; otherwise the plans of
+ * the fused families (iid-normal logistic GLM, non-centred hierarchical normal,
+ * Neal's funnel) run their dedicated kernels, bit-identical to the ut_plan_*
+ * constructors; UT_PLAN_MATRIX_CORES lets the GLM with D = 256 take the bf16
+ * matrix-core plan (X rounded to bf16).  Limits: P <= 256 (P <= 128 when a loop
+ * body reads parameters), 32 slots, 24 nodes per expression, 256 steps.        */
+enum { UT_STEP_CHOICE = 1, UT_STEP_DETERMINISTIC = 2, UT_STEP_LOOP = 3 };
+enum {
+  UT_FAMILY_NORMAL = 1,
+  UT_FAMILY_NONCENTERED_NORMAL = 2,
+  UT_FAMILY_LOGNORMAL = 3,
+  UT_FAMILY_HALFNORMAL = 4,
+  UT_FAMILY_HALFCAUCHY = 5,
+  UT_FAMILY_EXPONENTIAL = 6,
+  UT_FAMILY_CAUCHY = 7,
+  UT_FAMILY_LAPLACE = 8,
+  UT_FAMILY_LOGISTIC = 9,
+  UT_FAMILY_STUDENTT = 10, /* (nu, mu, sigma) */
+  UT_FAMILY_GAMMA = 11,    /* (shape, rate) */
+  UT_FAMILY_BERNOULLI = 12,
+  UT_FAMILY_BERNOULLI_LOGIT = 13,
+  UT_FAMILY_POISSON = 14,
+  UT_FAMILY_DIAG_NORMAL = 15,
+  UT_FAMILY_BERNOULLI_LOGIT_GLM = 16
+};
+enum { UT_TRANSFORM_IDENTITY = 0, UT_TRANSFORM_LOG = 1, UT_TRANSFORM_LOGIT = 2 };
+enum {
+  UT_OP_LITERAL = 1, UT_OP_SLOT = 2, UT_OP_PARAM = 3, UT_OP_OBS = 4,
+  UT_OP_ADD = 5, UT_OP_SUB = 6, UT_OP_MUL = 7, UT_OP_DIV = 8, UT_OP_NEG = 9,
+  UT_OP_EXP = 10, UT_OP_LOG = 11, UT_OP_LOG1P = 12, UT_OP_SQRT = 13, UT_OP_POW = 14,
+  UT_OP_ABS = 15, UT_OP_TANH = 16, UT_OP_LOGISTIC = 17
+};
+enum { UT_PLAN_GENERIC = 1, UT_PLAN_MATRIX_CORES = 2 };
+/* which engine serves a plan (ut_plan_engine) */
+enum { UT_ENGINE_INTERPRETER = 0, UT_ENGINE_LOGISTIC = 1, UT_ENGINE_HIER = 2, UT_ENGINE_FUNNEL = 3,
+       UT_ENGINE_GLM_MATRIX_CORES = 4, UT_ENGINE_GP = 5 };
+
+typedef struct ut_plan_expr {
+  int32_t op;
+  int32_t a, b;   /* operand node indices (-1 unused) */
+  int32_t index;  /* SLOT: slot; PARAM: base row; OBS: row within the iteration's block */
+  double value;   /* LITERAL */
+} ut_plan_expr;
+
+typedef struct ut_plan_step {
+  int32_t kind;
+  int32_t family;
+  int32_t value_source;  /* latent: first unconstrained row; observed: 0 */
+  int32_t transform;
+  int32_t binding_slot;  /* 0: none */
+  int32_t args[3];       /* root nodes of the family arguments / the deterministic expression; -1 unused */
+  int32_t dimension;     /* DIAG_NORMAL rows; BERNOULLI_LOGIT_GLM coefficients */
+  int32_t coef_source;   /* BERNOULLI_LOGIT_GLM first coefficient row */
+  int32_t value_offset;  /* observed choice in a loop body: row within the iteration's block */
+  int32_t loop_id;       /* LOOP: 1-based */
+  int32_t iterator_slot; /* LOOP: 0 none */
+  int32_t body_length;   /* LOOP */
+  int32_t stride;        /* LOOP: staged rows per iteration */
+} ut_plan_step;
+
+typedef struct ut_plan_desc {
+  int32_t num_params;
+  int32_t num_slots;
+  int32_t num_loops;
+  int32_t num_steps;
+  const ut_plan_step* steps;
+  int32_t num_exprs;
+  const ut_plan_expr* exprs;
+  int64_t num_observed;     /* staged rows, one column shared by all chains */
+  const double* observed;
+  const int32_t* trip_counts;  /* num_loops */
+  const int32_t* loop_starts;  /* num_loops */
+  int32_t flags;
+} ut_plan_desc;
+
+/* Host-only audit of a plan description (no device touched): UT_OK, or
+ * UT_E_UNSUPPORTED / UT_E_ARG / UT_E_DIM with the reason in ut_last_error(), the
+ * checks of the reference's lowering and read/write audit.                     */
+int ut_plan_validate(const ut_plan_desc* desc);
+int ut_plan_create(const ut_plan_desc* desc, ut_plan** out);
+int ut_plan_engine(const ut_plan* plan, int32_t* engine);
+
 int ut_plan_destroy(ut_plan* plan);
 int ut_plan_num_params(const ut_plan* plan, int32_t* num_params);
 

@@ -35,6 +35,9 @@ def _load():
         "orc_model_gp": (vp, [vp, vp, i32, i32, dbl, dbl, dbl, dbl]),
         "orc_gp_eval": (None, [vp, vp, i32, vp, vp, vp]),
         "orc_model_free": (None, [vp]),
+        "orc_model_plan": (vp, [vp]),
+        "orc_plan_error": (ct.c_char_p, []),
+        "orc_plan_prior": (None, [vp, i32, u32, u32, vp]),
         "orc_set_threads": (None, [i32]),
         "orc_model_num_params": (i32, [vp]),
         "orc_logjoint_grad_f64": (None, [vp, vp, i32, vp, vp]),
@@ -128,6 +131,12 @@ class OracleModel:
                                       model.mu_loc, model.mu_scale, model.tau_scale)
         elif kind == "NealsFunnel":
             self.h = L.orc_model_funnel(model.dim, model.v_scale)
+        elif kind == "PlanModel":
+            import ctypes as _ct
+            self._keep = (model,)
+            self.h = L.orc_model_plan(_ct.byref(model.desc))
+            if not self.h:
+                raise ValueError(L.orc_plan_error().decode())
         elif kind == "GaussianProcessPoisson":
             self._keep = (model.inputs, model.counts)
             self.h = L.orc_model_gp(_p(model.inputs), _p(model.counts), model.num_points, model.input_dim,
@@ -149,6 +158,13 @@ class OracleModel:
         g = np.empty((C, self.P))
         _load().orc_logjoint_grad_f64(self.h, _p(p), C, _p(lj), _p(g))
         return lj, g.T
+
+    def prior_column(self, strategy: int, chain: int, key: int) -> np.ndarray:
+        """A plan model's initial column as the samplers draw it (UT_INIT_PRIOR or
+        UT_INIT_UNIFORM) for the Philox coordinates (chain, key)."""
+        col = np.empty(self.P)
+        _load().orc_plan_prior(self.h, int(strategy), int(chain), int(key), _p(col))
+        return col
 
     def gp_eval(self, params):
         """GP model: binary64 (logjoint, gradient P x C, log det K) for P x C params."""

@@ -42,6 +42,7 @@
 
 #include "../include/uncertaintea_hip.h"
 #include "../uncertaintea_amd/csrc/ut_math.h"
+#include "oracle_plan.h"
 
 using namespace ut;
 
@@ -50,7 +51,7 @@ namespace {
 constexpr double HALF_LOG_2PI = 0.9189385332046728;
 constexpr double LOG_PI = 1.1447298858494002;
 constexpr double LOG_2 = 0.6931471805599453;
-enum { KIND_LOGISTIC = 1, KIND_HIER = 2, KIND_FUNNEL = 3, KIND_GP = 5 };
+enum { KIND_LOGISTIC = 1, KIND_HIER = 2, KIND_FUNNEL = 3, KIND_GP = 5, KIND_PLAN = 6 };
 enum { S_MOM = 0, S_DIR = 1, S_LEAF = 2, S_MERGE = 3, S_SEARCH = 4, S_INIT = 5, S_ACCEPT = 6 };
 
 // ------------------------------------------------------------------ model data
@@ -76,6 +77,8 @@ struct Model {
   std::vector<double> gx, gy;
   int gn = 0, gd = 0;
   double noise2 = 0, ls_loc = 0, ls_scale = 0.5, amp_scale = 1;
+  // general plan (interpreter restatements in oracle_plan.h)
+  std::shared_ptr<plan_oracle::PlanData> plan;
   // twin layout
   int K = 1;
   double total_const = 0;
@@ -96,6 +99,8 @@ struct Model {
       case KIND_FUNNEL:
         if (i < dim) return 1 + i;
         return i == dim ? 0 : -1;
+      case KIND_PLAN:
+        return i < P ? i : -1;
     }
     return -1;
   }
@@ -231,6 +236,8 @@ double ref_eval(const Model& m, const double* q, double* g, bool& ok) {
     }
   } else if (m.kind == KIND_GP) {
     lp = gp_eval(m, q, g);
+  } else if (m.kind == KIND_PLAN) {
+    lp = plan_oracle::plan_ref_eval(*m.plan, q, g);
   } else if (m.kind == KIND_FUNNEL) {
     const double v = q[0];
     lp += normal_lpdf(0.0, m.v_scale, v);
@@ -286,6 +293,7 @@ template <class T>
 T twin_eval(const Model& m, const T* q, T* g, bool& ok) {
   const int K = m.K;
   bool fin = true;
+  if (m.kind == KIND_PLAN) return plan_oracle::plan_twin_eval<T>(*m.plan, K, q, g, ok);
 
   if (m.kind == KIND_HIER) {
     const T mu = q[1 * WAVE + 0], ltau = q[1 * WAVE + 1];
@@ -906,6 +914,8 @@ void draw_initial_column(const Model& m, int strategy, uint32_t chain, uint32_t 
     const double tau = m.tau_scale * std::fabs(z(1) / z(P + 1));
     col[1] = log_d(tau);
     for (int j = 0; j < m.J; ++j) col[2 + j] = z(2 + j);
+  } else if (m.kind == KIND_PLAN) {
+    plan_prior_draw(m.plan->desc, strategy, chain, key, S_INIT, col);
   } else {
     const double v = m.v_scale * z(0);
     col[0] = v;
@@ -1981,6 +1991,8 @@ void finish_twin_layout(Model& m) {
                     (LOG_2 - LOG_PI - log_d(m.tau_scale));
   } else if (m.kind == KIND_LOGISTIC) {
     m.total_const = (double)m.D * (-log_d(m.scale) - HALF_LOG_2PI);
+  } else if (m.kind == KIND_PLAN) {
+    m.total_const = 0.0;
   } else {
     m.total_const = -(double)m.dim * HALF_LOG_2PI + (-log_d(m.v_scale) - HALF_LOG_2PI);
   }
@@ -2062,6 +2074,29 @@ void* orc_model_hier(const float* y, const int32_t* offsets, const double* sigma
   m->P = J + 2;
   finish_twin_layout(*m);
   return m;
+}
+
+// a general plan; NULL (with the reason in orc_plan_error) when the audit fails
+static std::string g_plan_error;
+void* orc_model_plan(const ut_plan_desc* desc) {
+  auto data = plan_oracle::copy_plan(*desc);
+  const int rc = plan_audit(&data->desc, data->audit);
+  if (rc != UT_OK) {
+    g_plan_error = data->audit.error;
+    return nullptr;
+  }
+  auto* m = new Model();
+  m->kind = KIND_PLAN;
+  m->P = desc->num_params;
+  m->plan = data;
+  finish_twin_layout(*m);
+  return m;
+}
+const char* orc_plan_error(void) { return g_plan_error.c_str(); }
+// the prior draw of a plan column (the samplers' UT_INIT_PRIOR start)
+void orc_plan_prior(void* h, int32_t strategy, uint32_t chain, uint32_t key, double* col) {
+  const Model& m = *(Model*)h;
+  plan_prior_draw(m.plan->desc, strategy, chain, key, S_INIT, col);
 }
 
 void* orc_model_funnel(int32_t dim, double v_scale) {

@@ -9,11 +9,13 @@ from .export import (parameter_names, pointwise_loglikelihood, posterior_array, 
                      write_crossppl_run)
 from .flow import AffineCouplingFlow, FullRankGaussian
 from .inference import HMCChains, ProgressInfo, batched_chees, batched_hmc, batched_meads, batched_nuts
+from .plan import Expr, PlanBuilder, PlanModel, VectorLatent
 from .models import (GaussianProcessPoisson, HierarchicalNormalNC, LogisticRegression, LogisticRegressionBF16,
                      NealsFunnel, TeaModel)
 from .vi import ADVIResult, batched_advi, variational_covariance, variational_mean, variational_samples
 
 __all__ = [
+    "Expr", "PlanBuilder", "PlanModel", "VectorLatent",
     "ADVIResult", "AffineCouplingFlow", "FullRankGaussian", "batched_advi", "variational_covariance", "variational_mean",
     "variational_samples", "parameter_names", "pointwise_loglikelihood", "posterior_array", "to_arviz_dict", "write_crossppl_run",
     "DeviceBatchedWorkspace", "GaussianProcessPoisson", "DeviceError", "HMCChains", "HierarchicalNormalNC", "LogisticRegression",

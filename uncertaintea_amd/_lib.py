@@ -23,13 +23,25 @@ UT_FLOAT32 = 0
 UT_FLOAT64 = 1
 UT_INIT_PRIOR = 0
 UT_INIT_UNIFORM = 1
+# general plans (include/uncertaintea_hip.h)
+UT_STEP_CHOICE, UT_STEP_DETERMINISTIC, UT_STEP_LOOP = 1, 2, 3
+(UT_FAMILY_NORMAL, UT_FAMILY_NONCENTERED_NORMAL, UT_FAMILY_LOGNORMAL, UT_FAMILY_HALFNORMAL, UT_FAMILY_HALFCAUCHY,
+ UT_FAMILY_EXPONENTIAL, UT_FAMILY_CAUCHY, UT_FAMILY_LAPLACE, UT_FAMILY_LOGISTIC, UT_FAMILY_STUDENTT, UT_FAMILY_GAMMA,
+ UT_FAMILY_BERNOULLI, UT_FAMILY_BERNOULLI_LOGIT, UT_FAMILY_POISSON, UT_FAMILY_DIAG_NORMAL,
+ UT_FAMILY_BERNOULLI_LOGIT_GLM) = range(1, 17)
+UT_TRANSFORM_IDENTITY, UT_TRANSFORM_LOG, UT_TRANSFORM_LOGIT = 0, 1, 2
+(UT_OP_LITERAL, UT_OP_SLOT, UT_OP_PARAM, UT_OP_OBS, UT_OP_ADD, UT_OP_SUB, UT_OP_MUL, UT_OP_DIV, UT_OP_NEG, UT_OP_EXP,
+ UT_OP_LOG, UT_OP_LOG1P, UT_OP_SQRT, UT_OP_POW, UT_OP_ABS, UT_OP_TANH, UT_OP_LOGISTIC) = range(1, 18)
+UT_PLAN_GENERIC, UT_PLAN_MATRIX_CORES = 1, 2
+(UT_ENGINE_INTERPRETER, UT_ENGINE_LOGISTIC, UT_ENGINE_HIER, UT_ENGINE_FUNNEL, UT_ENGINE_GLM_MATRIX_CORES,
+ UT_ENGINE_GP) = range(6)
 
 # names exported by the C ABI (checked by the load test)
 EXPORTED = (
     "ut_last_error", "ut_abi_version", "ut_device_available",
     "ut_plan_logistic_glm", "ut_plan_logistic_glm_bf16", "ut_plan_hier_normal_nc", "ut_plan_funnel",
     "ut_plan_gp_poisson", "ut_gp_factor",
-    "ut_plan_destroy", "ut_plan_num_params",
+    "ut_plan_destroy", "ut_plan_num_params", "ut_plan_validate", "ut_plan_create", "ut_plan_engine",
     "ut_workspace_create", "ut_workspace_destroy",
     "ut_logjoint", "ut_logjoint_gradient", "ut_noalloc_scratch_bytes", "ut_logjoint_gradient_noalloc",
     "ut_sampler_opts_default", "ut_nuts_run", "ut_hmc_run", "ut_chees_opts_default", "ut_chees_run",
@@ -39,6 +51,26 @@ EXPORTED = (
     "ut_flow_elbo_grad", "ut_initial_params",
     "ut_philox4x32", "ut_philox_fill", "ut_host_alloc", "ut_host_free", "ut_gemm_bf16",
 )
+
+
+class PlanExpr(ct.Structure):
+    _fields_ = [("op", ct.c_int32), ("a", ct.c_int32), ("b", ct.c_int32), ("index", ct.c_int32),
+                ("value", ct.c_double)]
+
+
+class PlanStep(ct.Structure):
+    _fields_ = [("kind", ct.c_int32), ("family", ct.c_int32), ("value_source", ct.c_int32), ("transform", ct.c_int32),
+                ("binding_slot", ct.c_int32), ("args", ct.c_int32 * 3), ("dimension", ct.c_int32),
+                ("coef_source", ct.c_int32), ("value_offset", ct.c_int32), ("loop_id", ct.c_int32),
+                ("iterator_slot", ct.c_int32), ("body_length", ct.c_int32), ("stride", ct.c_int32)]
+
+
+class PlanDesc(ct.Structure):
+    _fields_ = [("num_params", ct.c_int32), ("num_slots", ct.c_int32), ("num_loops", ct.c_int32),
+                ("num_steps", ct.c_int32), ("steps", ct.POINTER(PlanStep)), ("num_exprs", ct.c_int32),
+                ("exprs", ct.POINTER(PlanExpr)), ("num_observed", ct.c_int64), ("observed", ct.POINTER(ct.c_double)),
+                ("trip_counts", ct.POINTER(ct.c_int32)), ("loop_starts", ct.POINTER(ct.c_int32)),
+                ("flags", ct.c_int32)]
 
 
 class SamplerOpts(ct.Structure):
@@ -158,6 +190,9 @@ def lib() -> ct.CDLL:
         "ut_gp_factor": (ct.c_int, [vp, vp, i32, vp, vp, vp]),
         "ut_plan_destroy": (ct.c_int, [vp]),
         "ut_plan_num_params": (ct.c_int, [vp, ct.POINTER(i32)]),
+        "ut_plan_validate": (ct.c_int, [ct.POINTER(PlanDesc)]),
+        "ut_plan_create": (ct.c_int, [ct.POINTER(PlanDesc), pp]),
+        "ut_plan_engine": (ct.c_int, [vp, ct.POINTER(i32)]),
         "ut_workspace_create": (ct.c_int, [vp, i32, i32, pp]),
         "ut_workspace_destroy": (ct.c_int, [vp]),
         "ut_logjoint": (ct.c_int, [vp, vp, vp]),

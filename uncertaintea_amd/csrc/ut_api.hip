@@ -30,6 +30,8 @@
 #include "ut_lockstep.h"
 #include "ut_math.h"
 #include "ut_meads.h"
+#include "ut_mfma.h"
+#include "ut_plan_audit.h"
 
 using namespace ut;
 
@@ -86,8 +88,31 @@ constexpr double LOG_2 = 0.6931471805599453;
 }  // namespace
 
 // ---------------------------------------------------------------------------- plan
+// Host copy of a general plan description (the arrays the description points at).
+struct HostPlan {
+  std::vector<ut_plan_step> steps;
+  std::vector<ut_plan_expr> exprs;
+  std::vector<double> observed;
+  std::vector<int32_t> trips, starts;
+  ut_plan_desc desc{};
+  void copy(const ut_plan_desc& d) {
+    steps.assign(d.steps, d.steps + d.num_steps);
+    exprs.assign(d.exprs, d.exprs + d.num_exprs);
+    observed.assign(d.observed, d.observed + d.num_observed);
+    trips.assign(d.trip_counts, d.trip_counts + d.num_loops);
+    starts.assign(d.loop_starts, d.loop_starts + d.num_loops);
+    desc = d;
+    desc.steps = steps.data();
+    desc.exprs = exprs.data();
+    desc.observed = observed.data();
+    desc.trip_counts = trips.data();
+    desc.loop_starts = starts.data();
+  }
+};
+
 struct ut_plan {
   DevModelDesc dev;
+  HostPlan host;  // general plans run by the interpreter
   // matrix-core GLM (MODEL_GLM_MFMA): evaluated for all chains at once
   GlmPlan glm;
   // latent GP Poisson (MODEL_GP): evaluated for all chains at once
@@ -422,6 +447,272 @@ int ut_plan_funnel(int32_t dim, double v_scale, ut_plan** out) {
   m.total_const = -(double)dim * HALF_LOG_2PI + (-log_d(v_scale) - HALF_LOG_2PI);
   plan->P = m.P;
   *out = plan;
+  return UT_OK;
+}
+
+// ------------------------------------------------------------- general plans
+static int plan_route(const ut_plan_desc& d, const PlanAudit& a, int* engine);
+
+int ut_plan_validate(const ut_plan_desc* desc) {
+  PlanAudit a;
+  const int rc = plan_audit(desc, a);
+  if (rc != UT_OK) return fail(rc, a.error);
+  int engine = 0;
+  return plan_route(*desc, a, &engine);
+}
+
+}  // extern "C"
+
+namespace {
+// Recognition of the plans whose density a fused kernel computes: the caller's
+// description is matched node by node, and the data are pulled out of the staged
+// rows.  A plan that does not match exactly runs on the interpreter.
+struct PlanMatch {
+  const ut_plan_desc& d;
+  bool in(int i) const { return i >= 0 && i < d.num_exprs; }
+  bool lit(int i, double* v = nullptr) const {
+    if (!in(i) || d.exprs[i].op != UT_OP_LITERAL) return false;
+    if (v) *v = d.exprs[i].value;
+    return true;
+  }
+  bool lit_is(int i, double v) const {
+    double x;
+    return lit(i, &x) && x == v;
+  }
+  bool slot(int i, int s) const { return in(i) && d.exprs[i].op == UT_OP_SLOT && d.exprs[i].index == s; }
+  bool obs(int i, int k) const { return in(i) && d.exprs[i].op == UT_OP_OBS && d.exprs[i].index == k; }
+  bool op(int i, int o) const { return in(i) && d.exprs[i].op == o; }
+  const ut_plan_expr& e(int i) const { return d.exprs[i]; }
+  bool choice(const ut_plan_step& st, int fam, int vs, int tr) const {
+    return st.kind == UT_STEP_CHOICE && st.family == fam && st.value_source == vs && st.transform == tr;
+  }
+};
+
+// steps: normal(0, v_scale) latent v; diag normal of x_1..x_dim at (0, exp(v / 2))
+int match_funnel(const ut_plan_desc& d, int* dim, double* v_scale) {
+  const PlanMatch m{d};
+  if (d.num_steps != 2 || d.num_observed != 0) return 0;
+  const ut_plan_step &s0 = d.steps[0], &s1 = d.steps[1];
+  if (!m.choice(s0, UT_FAMILY_NORMAL, 1, UT_TRANSFORM_IDENTITY) || s0.binding_slot < 1) return 0;
+  if (!m.lit_is(s0.args[0], 0.0) || !m.lit(s0.args[1], v_scale) || !(*v_scale > 0)) return 0;
+  if (!m.choice(s1, UT_FAMILY_DIAG_NORMAL, 2, UT_TRANSFORM_IDENTITY) || s1.dimension + 1 != d.num_params) return 0;
+  if (!m.lit_is(s1.args[0], 0.0) || !m.op(s1.args[1], UT_OP_EXP)) return 0;
+  const int x = m.e(s1.args[1]).a, b = s0.binding_slot;
+  const bool half = (m.op(x, UT_OP_MUL) && ((m.slot(m.e(x).a, b) && m.lit_is(m.e(x).b, 0.5)) ||
+                                            (m.lit_is(m.e(x).a, 0.5) && m.slot(m.e(x).b, b)))) ||
+                    (m.op(x, UT_OP_DIV) && m.slot(m.e(x).a, b) && m.lit_is(m.e(x).b, 2.0));
+  if (!half) return 0;
+  *dim = s1.dimension;
+  return 1;
+}
+
+// steps: diag normal of beta (loc, scale); loop over rows: bernoullilogit GLM of
+// the staged covariates (intercept 0) and outcome
+int match_logistic(const ut_plan_desc& d, std::vector<float>& x, std::vector<float>& y, double* loc, double* scale) {
+  const PlanMatch m{d};
+  if (d.num_steps != 3) return 0;
+  const ut_plan_step &s0 = d.steps[0], &s1 = d.steps[1], &s2 = d.steps[2];
+  const int D = d.num_params;
+  if (!m.choice(s0, UT_FAMILY_DIAG_NORMAL, 1, UT_TRANSFORM_IDENTITY) || s0.dimension != D) return 0;
+  if (!m.lit(s0.args[0], loc) || !m.lit(s0.args[1], scale) || !(*scale > 0)) return 0;
+  if (s1.kind != UT_STEP_LOOP || s1.body_length != 1 || s1.stride != D + 1) return 0;
+  if (!m.choice(s2, UT_FAMILY_BERNOULLI_LOGIT_GLM, 0, UT_TRANSFORM_IDENTITY) || s2.coef_source != 1 ||
+      s2.dimension != D || s2.value_offset != 0 || !m.lit_is(s2.args[0], 0.0))
+    return 0;
+  const long long N = d.trip_counts[s1.loop_id - 1];
+  if (N < 1) return 0;
+  x.resize((size_t)N * D);
+  y.resize((size_t)N);
+  for (long long i = 0; i < N; ++i) {
+    const double* row = d.observed + i * (D + 1);
+    for (int k = 0; k < D; ++k) x[(size_t)i * D + k] = (float)row[k];
+    if (!(row[D] == 0.0 || row[D] == 1.0)) return 0;
+    y[(size_t)i] = (float)row[D];
+  }
+  return 1;
+}
+
+// steps: mu ~ normal(m, s); tau ~ halfcauchy(t) (log); theta_raw ~ diag normal(0, 1);
+// loop over observations (group, sigma, y): y ~ normal(mu + tau theta_raw[group], sigma)
+int match_hier(const ut_plan_desc& d, std::vector<float>& yv, std::vector<int32_t>& off, std::vector<double>& sig,
+               double* mloc, double* mscale, double* tscale) {
+  const PlanMatch m{d};
+  if (d.num_steps != 5) return 0;
+  const ut_plan_step &s0 = d.steps[0], &s1 = d.steps[1], &s2 = d.steps[2], &s3 = d.steps[3], &s4 = d.steps[4];
+  const int J = d.num_params - 2;
+  if (J < 1 || J > 64) return 0;
+  if (!m.choice(s0, UT_FAMILY_NORMAL, 1, UT_TRANSFORM_IDENTITY) || s0.binding_slot < 1) return 0;
+  if (!m.lit(s0.args[0], mloc) || !m.lit(s0.args[1], mscale) || !(*mscale > 0)) return 0;
+  if (!m.choice(s1, UT_FAMILY_HALFCAUCHY, 2, UT_TRANSFORM_LOG) || s1.binding_slot < 1) return 0;
+  if (!m.lit(s1.args[0], tscale) || !(*tscale > 0)) return 0;
+  if (!m.choice(s2, UT_FAMILY_DIAG_NORMAL, 3, UT_TRANSFORM_IDENTITY) || s2.dimension != J) return 0;
+  if (!m.lit_is(s2.args[0], 0.0) || !m.lit_is(s2.args[1], 1.0)) return 0;
+  if (s3.kind != UT_STEP_LOOP || s3.body_length != 1 || s3.stride != 3) return 0;
+  if (!m.choice(s4, UT_FAMILY_NORMAL, 0, UT_TRANSFORM_IDENTITY) || s4.value_offset != 2 || !m.obs(s4.args[1], 1))
+    return 0;
+  const int mu = s4.args[0], b1 = s0.binding_slot, b2 = s1.binding_slot;
+  if (!m.op(mu, UT_OP_ADD) || !m.slot(m.e(mu).a, b1) || !m.op(m.e(mu).b, UT_OP_MUL)) return 0;
+  const ut_plan_expr& mul = m.e(m.e(mu).b);
+  int pnode = -1;
+  if (m.slot(mul.a, b2) && m.op(mul.b, UT_OP_PARAM)) pnode = mul.b;
+  else if (m.op(mul.a, UT_OP_PARAM) && m.slot(mul.b, b2)) pnode = mul.a;
+  if (pnode < 0 || m.e(pnode).index != 3 || !m.obs(m.e(pnode).a, 0)) return 0;
+  const long long N = d.trip_counts[s3.loop_id - 1];
+  off.assign((size_t)J + 1, 0);
+  sig.assign((size_t)J, 0.0);
+  yv.resize((size_t)N);
+  std::vector<int> seen(J, 0);
+  int last = 0;
+  for (long long i = 0; i < N; ++i) {
+    const double* row = d.observed + 3 * i;
+    const int j = (int)row[0];
+    if ((double)j != row[0] || j < last || j >= J || !(row[1] > 0)) return 0;
+    if (seen[j] && sig[j] != row[1]) return 0;
+    seen[j] = 1;
+    sig[j] = row[1];
+    last = j;
+    ++off[(size_t)j + 1];
+    yv[(size_t)i] = (float)row[2];
+  }
+  for (int j = 0; j < J; ++j) {
+    if (!seen[j]) sig[j] = 1.0;  // empty groups: any positive scale
+    off[(size_t)j + 1] += off[(size_t)j];
+  }
+  return 1;
+}
+
+template <class T>
+int stage_plan_array(ut_plan* plan, const T** dst, const std::vector<T>& v) {
+  if (v.empty()) {
+    *dst = nullptr;
+    return UT_OK;
+  }
+  return plan->stage(dst, v);
+}
+}  // namespace
+
+// The engine that serves a validated plan: a fused family's kernels when the plan
+// matches one (and the flags allow it), else the interpreter within its limits.
+static int plan_route(const ut_plan_desc& d, const PlanAudit& a, int* engine) {
+  if (!(d.flags & UT_PLAN_GENERIC)) {
+    int dim = 0;
+    double v = 0, loc = 0, scale = 0, t = 0;
+    std::vector<float> x, y;
+    std::vector<int32_t> off;
+    std::vector<double> sig;
+    if (match_funnel(d, &dim, &v)) {
+      *engine = UT_ENGINE_FUNNEL;
+      return UT_OK;
+    }
+    if (match_logistic(d, x, y, &loc, &scale)) {
+      if ((d.flags & UT_PLAN_MATRIX_CORES) && d.num_params == 256 && y.size() % 32 == 0) {
+        *engine = UT_ENGINE_GLM_MATRIX_CORES;
+        return UT_OK;
+      }
+      if (k_slots(d.num_params) > 0) {
+        *engine = UT_ENGINE_LOGISTIC;
+        return UT_OK;
+      }
+    }
+    if (match_hier(d, y, off, sig, &loc, &scale, &t)) {
+      *engine = UT_ENGINE_HIER;
+      return UT_OK;
+    }
+  }
+  *engine = UT_ENGINE_INTERPRETER;
+  if (k_slots(d.num_params) < 0) return fail(UT_E_UNSUPPORTED, "ut_plan: the interpreter takes at most 256 parameters");
+  if (a.has_loop_param_reads && d.num_params > PLAN_MAX_LOOP_PARAMS)
+    return fail(UT_E_UNSUPPORTED, "ut_plan: loop bodies read parameters: the interpreter takes at most 128 parameters");
+  return UT_OK;
+}
+
+extern "C" {
+
+int ut_plan_create(const ut_plan_desc* desc, ut_plan** out) {
+  if (!out) return fail(UT_E_ARG, "ut_plan_create: null pointer");
+  *out = nullptr;
+  PlanAudit a;
+  int rc = plan_audit(desc, a);
+  if (rc != UT_OK) return fail(rc, a.error);
+  const ut_plan_desc& d = *desc;
+  if (!(d.flags & UT_PLAN_GENERIC)) {
+    int dim = 0;
+    double v_scale = 0, loc = 0, scale = 0, tscale = 0;
+    std::vector<float> x, y;
+    std::vector<int32_t> off;
+    std::vector<double> sig;
+    if (match_funnel(d, &dim, &v_scale)) {
+      rc = ut_plan_funnel(dim, v_scale, out);
+      return rc;
+    }
+    if (match_logistic(d, x, y, &loc, &scale)) {
+      const int D = d.num_params;
+      const long long N = (long long)y.size();
+      if ((d.flags & UT_PLAN_MATRIX_CORES) && D == 256 && N % 32 == 0) {
+        std::vector<uint16_t> xb(x.size());
+        for (size_t i = 0; i < x.size(); ++i) xb[i] = f2bf(x[i]);
+        rc = ut_plan_logistic_glm_bf16(xb.data(), y.data(), N, D, loc, scale, out);
+          return rc;
+      }
+      if (k_slots(D) > 0) {
+        rc = ut_plan_logistic_glm(x.data(), y.data(), N, D, loc, scale, out);
+          return rc;
+      }
+    }
+    if (match_hier(d, y, off, sig, &loc, &scale, &tscale)) {
+      rc = ut_plan_hier_normal_nc(y.data(), off.data(), sig.data(), (int32_t)sig.size(), loc, scale, tscale, out);
+      return rc;
+    }
+  }
+  // ---- the interpreter
+  int engine = 0;
+  if ((rc = plan_route(d, a, &engine))) return rc;
+  const int K = k_slots(d.num_params);
+  if (!ut_device_available()) return fail(UT_E_HIP, "ut_plan_create: no HIP device visible");
+  auto* plan = new ut_plan();
+  plan->host.copy(d);
+  plan->P = d.num_params;
+  DevModelDesc& m = plan->dev;
+  m.kind = MODEL_PLAN;
+  m.P = d.num_params;
+  m.K = K;
+  m.host_plan = &plan->host.desc;
+  PlanView& v = m.plan;
+  v.num_steps = d.num_steps;
+  v.num_params = d.num_params;
+  v.num_slots = d.num_slots;
+  v.max_nodes = a.max_nodes;
+  v.loop_params = a.loop_params;
+  std::vector<float> obs_f(plan->host.observed.begin(), plan->host.observed.end());
+  std::vector<int> trip(plan->host.trips.begin(), plan->host.trips.end());
+  std::vector<int> start(plan->host.starts.begin(), plan->host.starts.end());
+  std::vector<int> slot_local(a.slot_local.begin(), a.slot_local.end());
+  if (slot_local.empty()) slot_local.push_back(0);
+  if ((rc = stage_plan_array(plan, &v.steps, plan->host.steps)) ||
+      (rc = stage_plan_array(plan, &v.exprs, plan->host.exprs)) ||
+      (rc = stage_plan_array(plan, &v.first_node, a.first_node)) ||
+      (rc = stage_plan_array(plan, &v.obs_f, obs_f)) ||
+      (rc = stage_plan_array(plan, &v.obs_d, plan->host.observed)) ||
+      (rc = stage_plan_array(plan, &v.trip, trip)) || (rc = stage_plan_array(plan, &v.start, start)) ||
+      (rc = stage_plan_array(plan, &v.cursor, a.cursor)) || (rc = stage_plan_array(plan, &v.loop_info, a.loop_info)) ||
+      (rc = stage_plan_array(plan, &v.lists, a.lists)) || (rc = stage_plan_array(plan, &v.slot_local, slot_local))) {
+    delete plan;
+    return rc;
+  }
+  *out = plan;
+  return UT_OK;
+}
+
+int ut_plan_engine(const ut_plan* plan, int32_t* engine) {
+  if (!plan || !engine) return fail(UT_E_ARG, "ut_plan_engine: null pointer");
+  switch (plan->dev.kind) {
+    case MODEL_LOGISTIC: *engine = UT_ENGINE_LOGISTIC; break;
+    case MODEL_HIER: *engine = UT_ENGINE_HIER; break;
+    case MODEL_FUNNEL: *engine = UT_ENGINE_FUNNEL; break;
+    case MODEL_GLM_MFMA: *engine = UT_ENGINE_GLM_MATRIX_CORES; break;
+    case MODEL_GP: *engine = UT_ENGINE_GP; break;
+    default: *engine = UT_ENGINE_INTERPRETER;
+  }
   return UT_OK;
 }
 
@@ -767,6 +1058,8 @@ void draw_initial_column(const DevModelDesc& m, int strategy, uint32_t chain, ui
     for (int k = 0; k < m.D; ++k) col[k] = m.loc + m.scale * z(k);
     col[m.D] = log_d(m.v_scale * std::fabs(z(m.D)));
     for (int i = 0; i < (int)m.n; ++i) col[m.D + 1 + i] = z(m.D + 1 + i);
+  } else if (m.kind == MODEL_PLAN) {
+    plan_prior_draw(*m.host_plan, strategy, chain, key, STREAM_INIT, col);
   } else if (m.kind == MODEL_FUNNEL) {
     const double v = m.v_scale * z(0);
     col[0] = v;

@@ -4,12 +4,15 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "ut_plan.h"
+
 namespace ut {
 
 // MODEL_GLM_MFMA has no wavefront-per-chain evaluator: its density and gradient run
 // for all chains at once on the matrix cores (ut_glm.h).
 // MODEL_GP (latent GP Poisson, ut_gp.h) likewise evaluates all chains at once.
-enum ModelKind { MODEL_LOGISTIC = 1, MODEL_HIER = 2, MODEL_FUNNEL = 3, MODEL_GLM_MFMA = 4, MODEL_GP = 5 };
+// MODEL_PLAN is the general static execution plan, run by the plan interpreter.
+enum ModelKind { MODEL_LOGISTIC = 1, MODEL_HIER = 2, MODEL_FUNNEL = 3, MODEL_GLM_MFMA = 4, MODEL_GP = 5, MODEL_PLAN = 6 };
 
 // Philox stream ids.  NUTS uses 0..3 exactly as the reference's persistent kernel
 // (src/device/persistent_nuts.jl:70-73); the remaining purposes get their own ids.
@@ -57,6 +60,10 @@ struct DevModelDesc {
   int dim = 0;
   double v_scale = 1;
   double total_const = 0;
+  // general plan (device arrays owned by the plan; the host description for the
+  // samplers' prior draws)
+  PlanView plan;
+  const ut_plan_desc* host_plan = nullptr;
 };
 
 // Device pointers of the chain state (api order, P x C column-major).

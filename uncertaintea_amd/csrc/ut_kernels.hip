@@ -38,6 +38,26 @@ static HierModel<NMAX, DENSE> make_hier(const DevModelDesc& d) {
   return m;
 }
 
+template <int K>
+static GenericModel<K> make_generic(const DevModelDesc& d) {
+  GenericModel<K> m;
+  m.pv = d.plan;
+  m.total_const = 0.0;
+  return m;
+}
+
+// dynamic LDS of a launch: `base` bytes of the kernel's own, then the model's
+// region; kernels whose dynamic LDS exceeds 64 KB are allowed up to 160 KB once
+template <class T, class M>
+static size_t model_lds(M& m, size_t base, const void* kernel) {
+  const size_t mb = m.template lds_bytes<T>();
+  if (mb == 0) return base;
+  m.set_lds_offset(base);
+  const size_t total = (base + 15) / 16 * 16 + mb;
+  if (total > 65536) (void)hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+  return total;
+}
+
 template <class F>
 static hipError_t dispatch(const DevModelDesc& d, F&& f) {
   switch (d.kind) {
@@ -61,6 +81,11 @@ static hipError_t dispatch(const DevModelDesc& d, F&& f) {
       if (d.K == 1) return f(make_funnel<1>(d));
       if (d.K == 2) return f(make_funnel<2>(d));
       if (d.K == 4) return f(make_funnel<4>(d));
+      return hipErrorInvalidValue;
+    case MODEL_PLAN:
+      if (d.K == 1) return f(make_generic<1>(d));
+      if (d.K == 2) return f(make_generic<2>(d));
+      if (d.K == 4) return f(make_generic<4>(d));
       return hipErrorInvalidValue;
     default:
       return hipErrorInvalidValue;
@@ -900,7 +925,8 @@ __global__ void k_philox_fill(uint32_t chain, uint32_t iteration, uint32_t strea
 hipError_t launch_density_f32(const DevModelDesc& d, const float* params, float* values,
                               float* grads, int C, hipStream_t stream) {
   return dispatch(d, [&](auto m) {
-    hipLaunchKernelGGL((k_density<decltype(m), float>), dim3(C), dim3(WAVE), 0, stream, m, params,
+    const size_t lds = model_lds<float>(m, 0, (const void*)k_density<decltype(m), float>);
+    hipLaunchKernelGGL((k_density<decltype(m), float>), dim3(C), dim3(WAVE), lds, stream, m, params,
                        values, grads, d.P);
     return hipGetLastError();
   });
@@ -909,7 +935,8 @@ hipError_t launch_density_f32(const DevModelDesc& d, const float* params, float*
 hipError_t launch_density_f64(const DevModelDesc& d, const double* params, double* values,
                               double* grads, int C, hipStream_t stream) {
   return dispatch(d, [&](auto m) {
-    hipLaunchKernelGGL((k_density<decltype(m), double>), dim3(C), dim3(WAVE), 0, stream, m,
+    const size_t lds = model_lds<double>(m, 0, (const void*)k_density<decltype(m), double>);
+    hipLaunchKernelGGL((k_density<decltype(m), double>), dim3(C), dim3(WAVE), lds, stream, m,
                        params, values, grads, d.P);
     return hipGetLastError();
   });
@@ -919,7 +946,8 @@ hipError_t launch_nuts(const DevModelDesc& d, const ChainState& st, const Sample
                        int it_begin, int it_end, hipStream_t stream) {
   return dispatch(d, [&](auto m) {
     constexpr int K = decltype(m)::K;
-    const size_t lds = (size_t)(sp.max_depth + 1) * 2 * K * WAVE * sizeof(float);
+    const size_t ck = (size_t)(sp.max_depth + 1) * 2 * K * WAVE * sizeof(float);
+    const size_t lds = model_lds<float>(m, ck, (const void*)k_nuts<decltype(m)>);
     hipLaunchKernelGGL((k_nuts<decltype(m)>), dim3(sp.C), dim3(WAVE), lds, stream, m, st, sp,
                        it_begin, it_end);
     return hipGetLastError();
@@ -929,7 +957,8 @@ hipError_t launch_nuts(const DevModelDesc& d, const ChainState& st, const Sample
 hipError_t launch_hmc(const DevModelDesc& d, const ChainState& st, const SamplerParams& sp,
                       int it_begin, int it_end, hipStream_t stream) {
   return dispatch(d, [&](auto m) {
-    hipLaunchKernelGGL((k_hmc<decltype(m)>), dim3(sp.C), dim3(WAVE), 0, stream, m, st, sp,
+    const size_t lds = model_lds<float>(m, 0, (const void*)k_hmc<decltype(m)>);
+    hipLaunchKernelGGL((k_hmc<decltype(m)>), dim3(sp.C), dim3(WAVE), lds, stream, m, st, sp,
                        it_begin, it_end);
     return hipGetLastError();
   });
@@ -938,7 +967,8 @@ hipError_t launch_hmc(const DevModelDesc& d, const ChainState& st, const Sampler
 hipError_t launch_step_search(const DevModelDesc& d, const ChainState& st, int C,
                               uint32_t itc, int do_search, double target, hipStream_t stream) {
   return dispatch(d, [&](auto m) {
-    hipLaunchKernelGGL((k_step_search<decltype(m)>), dim3(C), dim3(WAVE), 0, stream, m, st, d.P,
+    const size_t lds = model_lds<float>(m, 0, (const void*)k_step_search<decltype(m)>);
+    hipLaunchKernelGGL((k_step_search<decltype(m)>), dim3(C), dim3(WAVE), lds, stream, m, st, d.P,
                        itc, do_search, target);
     return hipGetLastError();
   });

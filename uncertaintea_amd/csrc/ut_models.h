@@ -15,6 +15,7 @@
 // order reproduces the kernels bit for bit.
 #pragma once
 #include "ut_math.h"
+#include "ut_plan.h"
 
 namespace ut {
 
@@ -93,6 +94,22 @@ template <> struct Pick<double> {
   __device__ static double get(const float*, const double* d, int i) { return d[i]; }
 };
 
+// Models with LDS of their own (the plan interpreter) take a byte offset into the
+// kernel's dynamic LDS; the fused models use none.
+struct NoModelLds {
+  template <class T>
+  size_t lds_bytes() const { return 0; }
+  void set_lds_offset(size_t) {}
+};
+
+// LDS written by one lane and read by another lane of the same wavefront: order
+// the accesses (a wavefront's LDS operations complete in program order).
+__device__ inline void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // ----------------------------------------------------------------------------------
 // Non-centred hierarchical normal.  Internal order: slot 0 of lane j is theta_raw_j
 // (api 2 + j) for j < J; slot 1 of lane 0 is mu (api 0), of lane 1 is log tau (api 1).
@@ -107,7 +124,7 @@ template <> struct Pick<double> {
 // hold exactly NMAX observations and carry no per-observation masks.  The arithmetic
 // order is the same in every instance.
 template <int NMAX, bool DENSE = false>
-struct HierModel {
+struct HierModel : NoModelLds {
   static constexpr int K = 2;
   const float* y_f;
   const double* y_d;
@@ -216,7 +233,7 @@ struct HierModel {
 // 64 lanes reads 64 consecutive words of each feature.  The D gradient partials are
 // folded with one butterfly each and lane d keeps feature d's total.
 template <int KK>
-struct LogisticModel {
+struct LogisticModel : NoModelLds {
   static constexpr int K = KK;
   static constexpr int DMAX = KK * WAVE;
   const float* xT_f;
@@ -302,7 +319,7 @@ struct LogisticModel {
 //   logjoint = -S/2 - dim (v/2 + log(2 pi)/2) + log N(v; 0, v_scale),  S = sum s_i,
 //   d/dx_i = -x_i e^{-v},  d/dv = S/2 - dim/2 - v / v_scale^2.
 template <int KK>
-struct FunnelModel {
+struct FunnelModel : NoModelLds {
   static constexpr int K = KK;
   int dim;
   double v_scale, total_const;
@@ -346,6 +363,380 @@ struct FunnelModel {
       fin = fin && t_isfinite(g[k]);
     }
     ok = wave_all(fin && t_isfinite(total));
+    return total;
+  }
+};
+
+// ----------------------------------------------------------------------------------
+// The general static execution plan (ut_plan_desc), interpreted by one wavefront per
+// chain.  Parameter row i lives in lane i % 64, register slot i / 64.
+//
+// eval() runs the plan forward step by step and differentiates it in reverse:
+//  1. Top-level steps run uniformly (every lane computes the same values; uniform
+//     slot values and adjoints sit in LDS).  A LOOP runs its iterations 64 at a
+//     time, iteration t on lane t % 64 in increasing t: each iteration runs the
+//     body forward (per-lane slot values), then backward, accumulating per-lane
+//     adjoints of the slots the body reads from outside and per-lane partials of
+//     the parameters it reads (PARAM nodes, GLM coefficients).  The loop's log
+//     density is the wave_sum of the per-lane partial sums, added to the running
+//     total at the loop's place; the outer-slot adjoints are wave_summed into the
+//     uniform adjoints.
+//  2. The parameter partials of the loop bodies are wave_summed into the gradient
+//     (rows in increasing order).
+//  3. The top-level steps other than loops are revisited in reverse order: each
+//     recomputes its arguments, adds its own partials to the gradient and pushes
+//     the adjoints of its arguments back through their expressions.
+// A top-level DIAG_NORMAL scores its rows on the lanes that hold them and
+// wave_sums the partial sums.  The twin of this order is in the oracle.
+template <int KK>
+struct GenericModel {
+  static constexpr int K = KK;
+  PlanView pv;
+  size_t lds_offset = 0;
+  double total_const = 0.0;
+
+  template <class T>
+  size_t lds_bytes() const { return plan_lds_bytes<T>(pv); }
+  void set_lds_offset(size_t off) { lds_offset = (off + 15) / 16 * 16; }
+
+  template <class T>
+  struct Lane {
+    T* su;  // [S] uniform slot values
+    T* au;  // [S] uniform slot adjoints
+    T* sl;  // [S][64] per-lane slot values (loop bodies)
+    T* al;  // [S][64] per-lane slot adjoints
+    T* nv;  // [N][64] expression node values
+    T* na;  // [N][64] expression node adjoints
+    T* ap;  // [PL][64] per-lane parameter partials of loop bodies
+    T* qu;  // [P] parameters (uniform copy)
+  };
+
+  template <class T>
+  __device__ void prepare(Lane<T>& ld) const {
+    extern __shared__ __attribute__((aligned(16))) unsigned char ut_model_lds[];
+    T* p = (T*)(ut_model_lds + lds_offset);
+    const int S = pv.num_slots, N = pv.max_nodes, PL = pv.loop_params;
+    ld.su = p; p += S;
+    ld.au = p; p += S;
+    ld.sl = p; p += S * WAVE;
+    ld.al = p; p += S * WAVE;
+    ld.nv = p; p += N * WAVE;
+    ld.na = p; p += N * WAVE;
+    ld.ap = p; p += PL * WAVE;
+    ld.qu = p;
+  }
+
+  __device__ int api(int lane, int slot) const {
+    const int i = slot * WAVE + lane;
+    return i < pv.num_params ? i : -1;
+  }
+
+  // ---- register access by a uniform row
+  template <class T>
+  __device__ static void add_row(T (&g)[K], int row, T v) {
+    const int lane = threadIdx.x & (WAVE - 1);
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+      if (row == k * WAVE + lane) g[k] = g[k] + v;
+  }
+
+  // ---- expressions.  `lp` selects the loop-body context (per-lane slots, OBS
+  // rows at `cur`, parameter partials to ap); otherwise the top level.
+  template <class T>
+  __device__ T slot_read(const Lane<T>& ld, int s, bool lp) const {
+    const int lane = threadIdx.x & (WAVE - 1);
+    return (lp && pv.slot_local[s]) ? ld.sl[s * WAVE + lane] : ld.su[s];
+  }
+
+  template <class T>
+  __device__ T expr_forward(const Lane<T>& ld, int root, bool lp, long long cur) const {
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int first = pv.first_node[root];
+    for (int n = first; n <= root; ++n) {
+      const ut_plan_expr e = pv.exprs[n];
+      T v;
+      if (e.op == UT_OP_LITERAL) {
+        v = (T)e.value;
+      } else if (e.op == UT_OP_SLOT) {
+        v = slot_read(ld, e.index - 1, lp);
+      } else if (e.op == UT_OP_OBS) {
+        v = PObs<T>::get(pv, cur + e.index);
+      } else if (e.op == UT_OP_PARAM) {
+        const int off = e.a >= 0 ? (int)(ld.nv[(e.a - first) * WAVE + lane] + T(0.5)) : 0;
+        v = ld.qu[e.index - 1 + off];
+      } else {
+        const T va = ld.nv[(e.a - first) * WAVE + lane];
+        const T vb = plan_op_binary(e.op) ? ld.nv[(e.b - first) * WAVE + lane] : T(0);
+        v = plan_node_forward<T>(e.op, va, vb);
+      }
+      ld.nv[(n - first) * WAVE + lane] = v;
+    }
+    return ld.nv[(root - first) * WAVE + lane];
+  }
+
+  template <class T>
+  __device__ void expr_backward(const Lane<T>& ld, int root, T seed, bool lp, long long cur, T (&g)[K]) const {
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int first = pv.first_node[root];
+    expr_forward(ld, root, lp, cur);
+    for (int n = first; n <= root; ++n) ld.na[(n - first) * WAVE + lane] = T(0);
+    ld.na[(root - first) * WAVE + lane] = seed;
+    for (int n = root; n >= first; --n) {
+      const ut_plan_expr e = pv.exprs[n];
+      const T adj = ld.na[(n - first) * WAVE + lane];
+      if (e.op == UT_OP_LITERAL || e.op == UT_OP_OBS) continue;
+      if (e.op == UT_OP_SLOT) {
+        const int sidx = e.index - 1;
+        if (lp) ld.al[sidx * WAVE + lane] = ld.al[sidx * WAVE + lane] + adj;
+        else ld.au[sidx] = ld.au[sidx] + adj;
+        continue;
+      }
+      if (e.op == UT_OP_PARAM) {
+        const int off = e.a >= 0 ? (int)(ld.nv[(e.a - first) * WAVE + lane] + T(0.5)) : 0;
+        const int row = e.index - 1 + off;
+        if (lp) ld.ap[row * WAVE + lane] = ld.ap[row * WAVE + lane] + adj;
+        else add_row(g, row, adj);
+        continue;  // the offset is data: no adjoint
+      }
+      const T va = ld.nv[(e.a - first) * WAVE + lane];
+      const bool bin = plan_op_binary(e.op);
+      const T vb = bin ? ld.nv[(e.b - first) * WAVE + lane] : T(0);
+      const T v = ld.nv[(n - first) * WAVE + lane];
+      T da, db;
+      plan_node_partials<T>(e.op, va, vb, v, da, db);
+      ld.na[(e.a - first) * WAVE + lane] = ld.na[(e.a - first) * WAVE + lane] + adj * da;
+      if (bin) ld.na[(e.b - first) * WAVE + lane] = ld.na[(e.b - first) * WAVE + lane] + adj * db;
+    }
+  }
+
+  template <class T>
+  __device__ void args_forward(const Lane<T>& ld, const ut_plan_step& st, bool lp, long long cur, T (&a)[3]) const {
+    const int na = plan_family_arity(st.family);
+#pragma unroll
+    for (int j = 0; j < 3; ++j) a[j] = j < na ? expr_forward(ld, st.args[j], lp, cur) : T(0);
+  }
+
+  // ---- one loop: forward and backward of every iteration
+  template <class T>
+  __device__ T run_loop(const Lane<T>& ld, int si, T (&g)[K]) const {
+    const int lane = threadIdx.x & (WAVE - 1);
+    const ut_plan_step lst = pv.steps[si];
+    const int count = pv.trip[lst.loop_id - 1], start = pv.start[lst.loop_id - 1];
+    const long long base_row = pv.cursor[si];
+    const int* info = pv.loop_info + 4 * si;
+    const int body0 = si + 1, body1 = si + lst.body_length;
+    for (int i = 0; i < info[1]; ++i) ld.al[pv.lists[info[0] + i] * WAVE + lane] = T(0);
+    T lpl = T(0);
+    for (int t0 = 0; t0 < count; t0 += WAVE) {
+      const int t = t0 + lane;
+      if (t < count) {
+        const long long cur = base_row + (long long)t * lst.stride;
+        if (lst.iterator_slot > 0) ld.sl[(lst.iterator_slot - 1) * WAVE + lane] = (T)(start + t);
+        // forward
+        for (int bi = body0; bi <= body1; ++bi) {
+          const ut_plan_step st = pv.steps[bi];
+          if (st.kind == UT_STEP_DETERMINISTIC) {
+            ld.sl[(st.binding_slot - 1) * WAVE + lane] = expr_forward(ld, st.args[0], true, cur);
+            continue;
+          }
+          if (st.family == UT_FAMILY_BERNOULLI_LOGIT_GLM) {
+            T eta = expr_forward(ld, st.args[0], true, cur);
+            for (int d = 0; d < st.dimension; ++d)
+              eta = eta + ld.qu[st.coef_source - 1 + d] * PObs<T>::get(pv, cur + st.value_offset + d);
+            const T y = PObs<T>::get(pv, cur + st.value_offset + st.dimension);
+            const FamilyOut<T> fo = plan_family<T>(UT_FAMILY_BERNOULLI_LOGIT, y, &eta);
+            lpl = lpl + fo.lp;
+            continue;
+          }
+          T a[3];
+          args_forward(ld, st, true, cur, a);
+          const T x = PObs<T>::get(pv, cur + st.value_offset);
+          const FamilyOut<T> fo = plan_family<T>(st.family, x, a);
+          lpl = lpl + fo.lp;
+          if (st.binding_slot > 0) ld.sl[(st.binding_slot - 1) * WAVE + lane] = x;
+        }
+        // backward: the body's own slots start at zero adjoint every iteration
+        for (int i = 0; i < info[3]; ++i) ld.al[pv.lists[info[2] + i] * WAVE + lane] = T(0);
+        for (int bi = body1; bi >= body0; --bi) {
+          const ut_plan_step st = pv.steps[bi];
+          if (st.kind == UT_STEP_DETERMINISTIC) {
+            const T adj = ld.al[(st.binding_slot - 1) * WAVE + lane];
+            expr_backward(ld, st.args[0], adj, true, cur, g);
+            continue;
+          }
+          if (st.family == UT_FAMILY_BERNOULLI_LOGIT_GLM) {
+            T eta = expr_forward(ld, st.args[0], true, cur);
+            for (int d = 0; d < st.dimension; ++d)
+              eta = eta + ld.qu[st.coef_source - 1 + d] * PObs<T>::get(pv, cur + st.value_offset + d);
+            const T y = PObs<T>::get(pv, cur + st.value_offset + st.dimension);
+            const FamilyOut<T> fo = plan_family<T>(UT_FAMILY_BERNOULLI_LOGIT, y, &eta);
+            const T r = fo.da[0];
+            for (int d = 0; d < st.dimension; ++d) {
+              const int row = st.coef_source - 1 + d;
+              ld.ap[row * WAVE + lane] = ld.ap[row * WAVE + lane] + r * PObs<T>::get(pv, cur + st.value_offset + d);
+            }
+            expr_backward(ld, st.args[0], r, true, cur, g);
+            continue;
+          }
+          T a[3];
+          args_forward(ld, st, true, cur, a);
+          const T x = PObs<T>::get(pv, cur + st.value_offset);
+          const FamilyOut<T> fo = plan_family<T>(st.family, x, a);
+          const int na = plan_family_arity(st.family);
+          for (int j = 0; j < na; ++j) expr_backward(ld, st.args[j], fo.da[j], true, cur, g);
+        }
+      }
+    }
+    for (int i = 0; i < info[1]; ++i) {
+      const int sidx = pv.lists[info[0] + i];
+      const T tot = wave_sum(ld.al[sidx * WAVE + lane]);
+      ld.au[sidx] = ld.au[sidx] + tot;
+    }
+    return wave_sum(lpl);
+  }
+
+  // ---- a top-level DIAG_NORMAL: lanes score the rows they hold
+  template <class T>
+  __device__ T diag_forward(const Lane<T>& ld, const ut_plan_step& st, int si, const T (&q)[K], T mu, T sig) const {
+    const int lane = threadIdx.x & (WAVE - 1);
+    T part = T(0);
+    const T a[3] = {mu, sig, T(0)};
+    if (st.value_source > 0) {
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const int r = k * WAVE + lane;
+        if (r >= st.value_source - 1 && r < st.value_source - 1 + st.dimension)
+          part = part + plan_family<T>(UT_FAMILY_NORMAL, q[k], a).lp;
+      }
+    } else {
+      for (int d = lane; d < st.dimension; d += WAVE)
+        part = part + plan_family<T>(UT_FAMILY_NORMAL, PObs<T>::get(pv, pv.cursor[si] + d), a).lp;
+    }
+    return wave_sum(part);
+  }
+
+  template <class T>
+  __device__ T eval(const Lane<T>& ld, const T (&q)[K], T (&g)[K], bool& ok) const {
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int S = pv.num_slots, P = pv.num_params, PL = pv.loop_params;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      g[k] = T(0);
+      const int r = k * WAVE + lane;
+      if (r < P) ld.qu[r] = q[k];
+    }
+    for (int sidx = lane; sidx < S; sidx += WAVE) ld.au[sidx] = T(0);
+    for (int r = 0; r < PL; ++r) ld.ap[r * WAVE + lane] = T(0);
+    wave_lds_sync();
+    T total = T(0);
+    // 1. forward (loops run forward and backward)
+    for (int si = 0; si < pv.num_steps; ++si) {
+      const ut_plan_step st = pv.steps[si];
+      if (st.kind == UT_STEP_LOOP) {
+        total = total + run_loop(ld, si, g);
+        si += st.body_length;
+        continue;
+      }
+      if (st.kind == UT_STEP_DETERMINISTIC) {
+        const T v = expr_forward(ld, st.args[0], false, 0);
+        ld.su[st.binding_slot - 1] = v;
+        continue;
+      }
+      T a[3];
+      args_forward(ld, st, false, 0, a);
+      if (st.family == UT_FAMILY_DIAG_NORMAL) {
+        total = total + diag_forward(ld, st, si, q, a[0], a[1]);
+        continue;
+      }
+      T x, lad = T(0);
+      FamilyOut<T> fo;
+      if (st.family == UT_FAMILY_NONCENTERED_NORMAL) {
+        const T z = ld.qu[st.value_source - 1];
+        const T std_args[3] = {T(0), T(1), T(0)};
+        fo = plan_family<T>(UT_FAMILY_NORMAL, z, std_args);
+        x = a[0] + a[1] * z;
+      } else {
+        if (st.value_source > 0) {
+          const TransformOut<T> tr = plan_transform<T>(st.transform, ld.qu[st.value_source - 1]);
+          x = tr.c;
+          lad = tr.lad;
+        } else {
+          x = PObs<T>::get(pv, pv.cursor[si]);
+        }
+        fo = plan_family<T>(st.family, x, a);
+      }
+      total = total + (fo.lp + lad);
+      if (st.binding_slot > 0) ld.su[st.binding_slot - 1] = x;
+    }
+    // 2. parameter partials of the loop bodies
+    for (int r = 0; r < PL; ++r) {
+      const T tot = wave_sum(ld.ap[r * WAVE + lane]);
+      add_row(g, r, tot);
+    }
+    // 3. top-level steps in reverse
+    for (int si = pv.num_steps - 1; si >= 0; --si) {
+      const ut_plan_step st = pv.steps[si];
+      if (st.kind == UT_STEP_LOOP) continue;
+      // a body step (inside a loop) is skipped: its loop ran it
+      if (pv.cursor[si] == -2) continue;
+      if (st.kind == UT_STEP_DETERMINISTIC) {
+        const T adj = ld.au[st.binding_slot - 1];
+        expr_backward(ld, st.args[0], adj, false, 0, g);
+        continue;
+      }
+      T a[3];
+      args_forward(ld, st, false, 0, a);
+      const int na = plan_family_arity(st.family);
+      T seed[3] = {T(0), T(0), T(0)};
+      if (st.family == UT_FAMILY_DIAG_NORMAL) {
+        T pmu = T(0), psg = T(0);
+        const T fa[3] = {a[0], a[1], T(0)};
+        if (st.value_source > 0) {
+#pragma unroll
+          for (int k = 0; k < K; ++k) {
+            const int r = k * WAVE + lane;
+            if (r >= st.value_source - 1 && r < st.value_source - 1 + st.dimension) {
+              const FamilyOut<T> fo = plan_family<T>(UT_FAMILY_NORMAL, q[k], fa);
+              g[k] = g[k] + fo.dx;
+              pmu = pmu + fo.da[0];
+              psg = psg + fo.da[1];
+            }
+          }
+        } else {
+          for (int d = lane; d < st.dimension; d += WAVE) {
+            const FamilyOut<T> fo = plan_family<T>(UT_FAMILY_NORMAL, PObs<T>::get(pv, pv.cursor[si] + d), fa);
+            pmu = pmu + fo.da[0];
+            psg = psg + fo.da[1];
+          }
+        }
+        seed[0] = wave_sum(pmu);
+        seed[1] = wave_sum(psg);
+      } else if (st.family == UT_FAMILY_NONCENTERED_NORMAL) {
+        const T z = ld.qu[st.value_source - 1];
+        const T std_args[3] = {T(0), T(1), T(0)};
+        const FamilyOut<T> fo = plan_family<T>(UT_FAMILY_NORMAL, z, std_args);
+        const T adj = st.binding_slot > 0 ? ld.au[st.binding_slot - 1] : T(0);
+        add_row(g, st.value_source - 1, fo.dx + adj * a[1]);
+        seed[0] = adj;
+        seed[1] = adj * z;
+      } else {
+        if (st.value_source > 0) {
+          const TransformOut<T> tr = plan_transform<T>(st.transform, ld.qu[st.value_source - 1]);
+          const FamilyOut<T> fo = plan_family<T>(st.family, tr.c, a);
+          const T adj_x = st.binding_slot > 0 ? fo.dx + ld.au[st.binding_slot - 1] : fo.dx;
+          add_row(g, st.value_source - 1, adj_x * tr.dc + tr.dlad);
+          for (int j = 0; j < 3; ++j) seed[j] = fo.da[j];
+        } else {
+          const FamilyOut<T> fo = plan_family<T>(st.family, PObs<T>::get(pv, pv.cursor[si]), a);
+          for (int j = 0; j < 3; ++j) seed[j] = fo.da[j];
+        }
+      }
+      for (int j = 0; j < na; ++j) expr_backward(ld, st.args[j], seed[j], false, 0, g);
+    }
+    bool fin = t_isfinite(total);
+#pragma unroll
+    for (int k = 0; k < K; ++k) fin = fin && t_isfinite(g[k]);
+    ok = wave_all(fin);
     return total;
   }
 };
