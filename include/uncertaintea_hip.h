@@ -371,6 +371,18 @@ typedef struct ut_chain_out {
  * (reference _batched_nuts_device_per_chain!, api_batched_nuts.jl:761-980).    */
 int ut_nuts_run(ut_workspace* ws, const ut_sampler_opts* opts, ut_chain_out* out);
 
+/* NUTS with tree_strategy = masked (reference src/device/nuts_kernels.jl:61-580,
+ * api_batched_nuts.jl:44-47, 95-144): the same multinomial NUTS transition and the
+ * same per-chain adaptation as ut_nuts_run, but every chain advances one leaf of its
+ * own tree per step and the gradient of all chains comes from one batched call
+ * (each leaf's gradient launch for all chains, chains that end an iteration start
+ * their next one at once, chains past a launch range are masked).  It therefore
+ * runs every plan, including the matrix-core GLM, the GP and workspaces with a flow
+ * attached (NUTS in the flow's base space, draws returned mapped through it).  For
+ * the wavefront-per-chain plans its draws equal ut_nuts_run's bit for bit.  The
+ * per-chain step search runs in lock step the same way.                      */
+int ut_nuts_masked_run(ut_workspace* ws, const ut_sampler_opts* opts, ut_chain_out* out);
+
 /* Fixed-length HMC with per-chain step and pooled diagonal mass
  * (reference _run_device_batched_hmc, src/device/hmc_kernels.jl:306-723).      */
 int ut_hmc_run(ut_workspace* ws, const ut_sampler_opts* opts, ut_chain_out* out);

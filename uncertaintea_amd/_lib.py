@@ -44,7 +44,7 @@ EXPORTED = (
     "ut_plan_destroy", "ut_plan_num_params", "ut_plan_validate", "ut_plan_create", "ut_plan_engine",
     "ut_workspace_create", "ut_workspace_destroy",
     "ut_logjoint", "ut_logjoint_gradient", "ut_noalloc_scratch_bytes", "ut_logjoint_gradient_noalloc",
-    "ut_sampler_opts_default", "ut_nuts_run", "ut_hmc_run", "ut_chees_opts_default", "ut_chees_run",
+    "ut_sampler_opts_default", "ut_nuts_run", "ut_nuts_masked_run", "ut_hmc_run", "ut_chees_opts_default", "ut_chees_run",
     "ut_hmc_shared_run", "ut_meads_opts_default", "ut_meads_run", "ut_flow_create", "ut_flow_create_fullrank", "ut_pointwise_loglik_size", "ut_pointwise_loglik", "ut_flow_destroy", "ut_flow_num_weights", "ut_flow_layer_layout",
     "ut_flow_get_weights", "ut_flow_set_weights", "ut_flow_set_base", "ut_flow_train", "ut_flow_forward",
     "ut_flow_inverse", "ut_flow_vjp", "ut_flow_loss_grad", "ut_workspace_set_flow", "ut_flow_fit_elbo",
@@ -201,6 +201,7 @@ def lib() -> ct.CDLL:
         "ut_logjoint_gradient_noalloc": (ct.c_int, [vp, i32, vp, vp, vp, vp]),
         "ut_sampler_opts_default": (None, [ct.POINTER(SamplerOpts)]),
         "ut_nuts_run": (ct.c_int, [vp, ct.POINTER(SamplerOpts), ct.POINTER(ChainOut)]),
+        "ut_nuts_masked_run": (ct.c_int, [vp, ct.POINTER(SamplerOpts), ct.POINTER(ChainOut)]),
         "ut_hmc_run": (ct.c_int, [vp, ct.POINTER(SamplerOpts), ct.POINTER(ChainOut)]),
         "ut_chees_opts_default": (None, [ct.POINTER(CheesOpts)]),
         "ut_chees_run": (ct.c_int, [vp, ct.POINTER(CheesOpts), ct.POINTER(ChainOut), vp]),

@@ -1229,16 +1229,22 @@ int init_chains(ut_workspace* ws, const ut_sampler_opts* o, uint32_t rs) {
   }
 }
 
-int run_sampler(ut_workspace* ws, const ut_sampler_opts* o, ut_chain_out* out, bool nuts) {
+// masked: NUTS with tree_strategy = masked -- every chain advances one leaf of its
+// tree per step, the gradient of all chains from one batched call (any plan, a
+// flow included); the adaptation is the persistent path's, step for step.
+int run_sampler(ut_workspace* ws, const ut_sampler_opts* o, ut_chain_out* out, bool nuts, bool masked = false) {
   int rc = validate_opts(ws, o, out, nuts);
   if (rc) return rc;
-  if (ws->plan->dev.kind == MODEL_GLM_MFMA || ws->plan->dev.kind == MODEL_GP)
+  if (!masked && (ws->plan->dev.kind == MODEL_GLM_MFMA || ws->plan->dev.kind == MODEL_GP))
     return fail(UT_E_UNSUPPORTED,
                 "this plan evaluates all chains at once; sample it with the lock-step samplers "
-                "(ut_chees_run, ut_hmc_shared_run, ut_meads_run)");
-  if (ws->flow)
+                "(ut_nuts_masked_run, ut_chees_run, ut_hmc_shared_run, ut_meads_run)");
+  if (!masked && ws->flow)
     return fail(UT_E_UNSUPPORTED,
-                "a flow preconditions the lock-step samplers (ut_chees_run, ut_hmc_shared_run); detach it first");
+                "a flow preconditions the lock-step samplers (ut_nuts_masked_run, ut_chees_run, ut_hmc_shared_run); "
+                "detach it first");
+  if (masked && masked_slots(ws->plan->dev) < 0)
+    return fail(UT_E_UNSUPPORTED, "masked NUTS: more than 1024 parameters");
   const double t_start = now_seconds();
   const int P = ws->P, C = ws->C, S = o->num_samples, W = o->num_warmup;
   const DevModelDesc& m = ws->plan->dev;
@@ -1254,9 +1260,76 @@ int run_sampler(ut_workspace* ws, const ut_sampler_opts* o, ut_chain_out* out, b
   std::vector<double> step_h(C, o->step_size);
   UT_HIP(hipMemcpyAsync(ws->st.step, step_h.data(), (size_t)C * 8, hipMemcpyHostToDevice, stream));
   const int do_search = (o->find_reasonable_step_size || (W > 0 && o->adapt_step_size)) ? 1 : 0;
-  UT_HIP(launch_step_search(m, ws->st, C, rs, do_search, o->target_accept, stream));
 
   DeviceBuffers buf;
+  // ---- masked NUTS state and its lock-step drivers
+  MnState ms;
+  int* h_active = nullptr;
+  struct PinnedInts {
+    int* p = nullptr;
+    ~PinnedInts() {
+      if (p) (void)hipHostFree(p);
+    }
+  } pinned;
+  const int mk = masked ? masked_slots(m) : 0;
+  if (masked) {
+    UT_HIP(buf.alloc(&ms.vec, (size_t)C * MN_NV * mk * WAVE));
+    UT_HIP(buf.alloc(&ms.ck, (size_t)C * (o->max_tree_depth + 1) * 2 * mk * WAVE));
+    UT_HIP(buf.alloc(&ms.ch, (size_t)C));
+    UT_HIP(buf.alloc(&ms.active, 1));
+    UT_HIP(buf.alloc(&ms.wq, (size_t)C * P));
+    float *wg = nullptr, *wlj = nullptr;
+    UT_HIP(buf.alloc(&wg, (size_t)C * P));
+    UT_HIP(buf.alloc(&wlj, (size_t)C));
+    ms.wg = wg;
+    ms.wlj = wlj;
+    UT_HIP(hipMemsetAsync(ms.vec, 0, (size_t)C * MN_NV * mk * WAVE * sizeof(float), stream));
+    UT_HIP(hipMemsetAsync(ms.wq, 0, (size_t)C * P * sizeof(float), stream));
+    UT_HIP(hipHostMalloc((void**)&pinned.p, 2 * sizeof(int), hipHostMallocDefault));
+    h_active = pinned.p;
+  }
+  // lock-step loop: steps until no chain is running; the count of running chains
+  // is read back one step behind, so the loop never waits on the device
+  auto lockstep_until_idle = [&](auto&& step) -> int {
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    UT_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+    UT_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+    struct EvRelease {
+      hipEvent_t* e;
+      ~EvRelease() {
+        (void)hipEventDestroy(e[0]);
+        (void)hipEventDestroy(e[1]);
+      }
+    } release{ev};
+    for (long long i = 0;; ++i) {
+      UT_HIP(hipMemsetAsync(ms.active, 0, sizeof(int), stream));
+      int r = step();
+      if (r) return r;
+      UT_HIP(hipMemcpyAsync(h_active + (i & 1), ms.active, sizeof(int), hipMemcpyDeviceToHost, stream));
+      UT_HIP(hipEventRecord(ev[i & 1], stream));
+      if (i > 0) {
+        UT_HIP(hipEventSynchronize(ev[(i - 1) & 1]));
+        if (h_active[(i - 1) & 1] == 0) break;
+      }
+    }
+    return UT_OK;
+  };
+  auto search = [&](uint32_t itc, int do_it) -> int {
+    if (!masked) {
+      UT_HIP(launch_step_search(m, ws->st, C, itc, do_it, o->target_accept, stream));
+      return UT_OK;
+    }
+    UT_HIP(hipMemsetAsync(ms.active, 0, sizeof(int), stream));
+    UT_HIP(launch_mn_search_begin(m, ws->st, ms, C, P, itc, do_it, stream));
+    if (!do_it) return UT_OK;
+    UT_HIP(batched_grad(ws, ms.wq, (float*)ms.wlj, (float*)ms.wg, true));
+    return lockstep_until_idle([&]() -> int {
+      UT_HIP(launch_mn_search_step(m, ws->st, ms, C, P, itc, stream));
+      UT_HIP(batched_grad(ws, ms.wq, (float*)ms.wlj, (float*)ms.wg, true));
+      return UT_OK;
+    });
+  };
+  if ((rc = search(rs, do_search))) return rc;
   SamplerParams sp;
   sp.run_seed = rs;
   sp.P = P;
@@ -1283,9 +1356,20 @@ int run_sampler(ut_workspace* ws, const ut_sampler_opts* o, ut_chain_out* out, b
   UT_HIP(buf.alloc(&sp.o_depth, (size_t)S * C));
   UT_HIP(buf.alloc(&sp.o_steps, (size_t)S * C));
 
-  // a warmup window runs as one launch (nothing is copied out during warmup)
-  auto run_range = [&](int a, int b) -> hipError_t {
-    return nuts ? launch_nuts(m, ws->st, sp, a, b, stream) : launch_hmc(m, ws->st, sp, a, b, stream);
+  // a warmup window runs as one launch (nothing is copied out during warmup); masked
+  // NUTS runs it in lock-step leaves
+  auto masked_range = [&](int a, int b) -> int {
+    UT_HIP(launch_mn_reset(ms, C, a, stream));
+    return lockstep_until_idle([&]() -> int {
+      UT_HIP(launch_mn_step(m, ws->st, sp, ms, b, stream));
+      UT_HIP(batched_grad(ws, ms.wq, (float*)ms.wlj, (float*)ms.wg, true));
+      return UT_OK;
+    });
+  };
+  auto run_range = [&](int a, int b) -> int {
+    if (masked) return masked_range(a, b);
+    UT_HIP(nuts ? launch_nuts(m, ws->st, sp, a, b, stream) : launch_hmc(m, ws->st, sp, a, b, stream));
+    return UT_OK;
   };
 
   // ---- warmup: one launch range per mass window, device replay at each window end
@@ -1310,13 +1394,13 @@ int run_sampler(ut_workspace* ws, const ut_sampler_opts* o, ut_chain_out* out, b
     for (int a = it; a < b;) {
       const int e = prog.next_stop(a - 1, b, W);
       if (e == b) break;
-      UT_HIP(run_range(a, e));
+      if ((rc = run_range(a, e))) return rc;
       double st = 0.0;
       UT_HIP(report_step(&st));
       UT_HIP(prog.warmup(sp, C, e, W, st, stream));
       it = a = e + 1;
     }
-    UT_HIP(run_range(it, b));
+    if ((rc = run_range(it, b))) return rc;
     if (prog.due(b, W)) {
       double st = 0.0;
       UT_HIP(report_step(&st));
@@ -1330,7 +1414,7 @@ int run_sampler(ut_workspace* ws, const ut_sampler_opts* o, ut_chain_out* out, b
                                 o->mass_matrix_regularization, o->mass_matrix_min_samples, ws->d_im,
                                 ws->st.im_f, ws->d_summary, stream));
       if (o->adapt_step_size && b < W)
-        UT_HIP(launch_step_search(m, ws->st, C, rs + (uint32_t)b, 1, o->target_accept, stream));
+        if ((rc = search(rs + (uint32_t)b, 1))) return rc;
       ++window;
     }
     it = b + 1;
@@ -1349,9 +1433,7 @@ int run_sampler(ut_workspace* ws, const ut_sampler_opts* o, ut_chain_out* out, b
   const int chunk = std::max(kChunk, (S + kSampleLaunches - 1) / kSampleLaunches);
   for (int s0 = 0, s1 = 0; s0 < S; s0 = s1) {
     s1 = prog.next_stop(s0, std::min(S, s0 + chunk), S);
-    hipError_t e = nuts ? launch_nuts(m, ws->st, sp, W + 1 + s0, W + s1, stream)
-                        : launch_hmc(m, ws->st, sp, W + 1 + s0, W + s1, stream);
-    UT_HIP(e);
+    if ((rc = run_range(W + 1 + s0, W + s1))) return rc;
     hipEvent_t ev;
     UT_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
     done.push_back(ev);
@@ -1364,8 +1446,11 @@ int run_sampler(ut_workspace* ws, const ut_sampler_opts* o, ut_chain_out* out, b
                               pitch, (size_t)(s1 - s0) * elem, (size_t)C, hipMemcpyDeviceToHost,
                               ws->copy_stream);
     };
-    UT_HIP(copy2d(out->samples, sp.draws, (size_t)P * 4));
-    UT_HIP(copy2d(out->logjoint, sp.o_lj, 8));
+    // with a flow the draws (base space) and their logjoint are mapped after sampling
+    if (!ws->flow) {
+      UT_HIP(copy2d(out->samples, sp.draws, (size_t)P * 4));
+      UT_HIP(copy2d(out->logjoint, sp.o_lj, 8));
+    }
     UT_HIP(copy2d(out->accept_stat, sp.o_acc, 8));
     UT_HIP(copy2d(out->energy, sp.o_energy, 8));
     UT_HIP(copy2d(out->energy_error, sp.o_err, 8));
@@ -1380,6 +1465,19 @@ int run_sampler(ut_workspace* ws, const ut_sampler_opts* o, ut_chain_out* out, b
     }
   }
   UT_HIP(hipEventRecord(ev1, stream));
+  if (ws->flow && S > 0) {
+    // draws back to the model's space, C rows at a time; logjoint values lose the
+    // log-determinant
+    Flow& f = ws->flow->f;
+    const size_t SC = (size_t)S * C;
+    for (size_t r = 0; r < SC; r += (size_t)C) {
+      float* rows = sp.draws + r * P;
+      UT_HIP(flow_forward(f, rows, rows, nullptr, C, stream));
+      UT_HIP(flow_sub_logdet(sp.o_lj + r, f.sc.logdet, C, stream));
+    }
+    UT_HIP(hipMemcpyAsync(out->samples, sp.draws, SC * P * 4, hipMemcpyDeviceToHost, stream));
+    if (out->logjoint) UT_HIP(hipMemcpyAsync(out->logjoint, sp.o_lj, SC * 8, hipMemcpyDeviceToHost, stream));
+  }
   unsigned long long evals = 0;
   UT_HIP(hipMemcpyAsync(&evals, sp.eval_counter, sizeof(evals), hipMemcpyDeviceToHost, stream));
   UT_HIP(hipStreamSynchronize(stream));
@@ -1926,6 +2024,12 @@ int ut_nuts_run(ut_workspace* ws, const ut_sampler_opts* opts, ut_chain_out* out
 
 int ut_hmc_run(ut_workspace* ws, const ut_sampler_opts* opts, ut_chain_out* out) {
   return run_sampler(ws, opts, out, false);
+}
+
+int ut_nuts_masked_run(ut_workspace* ws, const ut_sampler_opts* opts, ut_chain_out* out) {
+  if (ws && ws->flow && ws->C % 128 != 0)
+    return fail(UT_E_DIM, "masked NUTS with a flow: num_chains must be a multiple of 128");
+  return run_sampler(ws, opts, out, true, true);
 }
 
 }  // extern "C"

@@ -123,6 +123,45 @@ hipError_t launch_mass_replay(const float* hist_pos, const double* hist_acc,
                               int window_length, int use_accepted, double regularization,
                               int min_samples, double* im_d, float* im_f, double* summary,
                               hipStream_t stream);
+// ---- masked NUTS (tree_strategy = masked): every chain advances one leaf of its
+// own tree per step and the gradient of all chains comes from one batched call.
+struct MnChain {
+  double H0, cont_w, accept_sum, prop_energy, prop_error, sub_w, sub_asum, sub_energy, sub_error, step;
+  float ljL, ljR, ljP, ljS, cur_lj, lj0, lj;
+  int accept_count, depth, steps, sub_int, sub_acount, leaf, forward, divergent, turning, sub_div, sub_turn;
+  int phase, it;
+  uint32_t leaf_counter;
+  // per-chain step search
+  double search_s, search_next, search_dir, search_H0;
+  int search_t, search_valid;
+};
+struct MnState {
+  float* vec = nullptr;   // C x MN_NV x K x 64 (tree vectors)
+  float* ck = nullptr;    // C x (max_depth + 1) x 2 x K x 64 (dyadic checkpoints)
+  MnChain* ch = nullptr;  // C
+  int* active = nullptr;  // chains not finished after a step
+  float* wq = nullptr;    // C x P: rows whose gradient the next batched call computes
+  const float* wg = nullptr;
+  const float* wlj = nullptr;
+};
+// Number of tree vectors per chain and the layout used for plans without a
+// wavefront-per-chain model (parameter a in lane a % 64, slot a / 64).
+constexpr int MN_NV = 15;
+int masked_slots(const DevModelDesc& d);
+// Range [it_begin, it_end] of NUTS iterations: mark every chain to start at it_begin.
+hipError_t launch_mn_reset(const MnState& ms, int C, int it_begin, hipStream_t stream);
+// One step: consume the gradient of the last rows written (when pending), advance
+// each chain's tree by one leaf (finishing and starting iterations as they end),
+// write the next rows to ms.wq and count the chains still running.
+hipError_t launch_mn_step(const DevModelDesc& d, const ChainState& st, const SamplerParams& sp, const MnState& ms,
+                          int it_end, hipStream_t stream);
+// Per-chain step search in lock step: start (do_search 0 only resets the dual
+// averaging) and one trial step per call; done when no chain is searching.
+hipError_t launch_mn_search_begin(const DevModelDesc& d, const ChainState& st, const MnState& ms, int C, int P,
+                                  uint32_t itc, int do_search, hipStream_t stream);
+hipError_t launch_mn_search_step(const DevModelDesc& d, const ChainState& st, const MnState& ms, int C, int P,
+                                 uint32_t itc, hipStream_t stream);
+
 hipError_t launch_philox_fill(uint32_t chain, uint32_t iteration, uint32_t stream_id,
                               uint32_t first, int64_t n, uint32_t* blocks, float* uf,
                               float* nf, double* ud, hipStream_t stream);

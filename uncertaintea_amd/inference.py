@@ -204,11 +204,12 @@ def _run(kind: str, model: TeaModel, num_chains: int, opts, initial_params,
         trace = np.zeros(int(base.num_warmup), dtype=np.float64)
         L.check(L.lib().ut_chees_run(ws.handle, ct.byref(opts), ct.byref(out), L.ptr(trace)))
     else:
-        fn = {"nuts": L.lib().ut_nuts_run, "hmc": L.lib().ut_hmc_run, "hmc_shared": L.lib().ut_hmc_shared_run}[kind]
+        fn = {"nuts": L.lib().ut_nuts_run, "nuts_masked": L.lib().ut_nuts_masked_run, "hmc": L.lib().ut_hmc_run,
+              "hmc_shared": L.lib().ut_hmc_shared_run}[kind]
         L.check(fn(ws.handle, ct.byref(opts), ct.byref(out)))
     if cb is not None:
         cb.reraise()
-    kind = "hmc" if kind == "hmc_shared" else kind
+    kind = {"hmc_shared": "hmc", "nuts_masked": "nuts"}.get(kind, kind)
     return HMCChains(kind, model, samples, lj, acc, en, err, accepted.view(bool), div.view(bool), depth,
                      steps, step, im, max_tree_depth, num_leapfrog_steps, target_accept, out.warmup_seconds,
                      out.sampling_seconds, int(out.sampling_gradient_evals), out.sampling_kernel_seconds,
@@ -222,11 +223,17 @@ def batched_nuts(model: TeaModel, args=(), constraints=None, *, num_chains: int,
                  find_reasonable_step_size: bool = False, divergence_threshold: float = 1000.0,
                  mass_matrix_regularization: float = 1e-3, mass_matrix_min_samples: int = 10,
                  tree_strategy: str = "persistent", backend: str = "hip", precision=np.float32,
-                 seed: int = 0, callback=None, callback_every: int = 10) -> HMCChains:
-    """Many NUTS chains, each building its whole tree in one persistent kernel."""
+                 seed: int = 0, callback=None, callback_every: int = 10, flow=None) -> HMCChains:
+    """Many NUTS chains.  tree_strategy='persistent': each chain builds its whole tree
+    in one kernel (wavefront-per-chain plans).  'masked': every chain advances one
+    leaf per step with the gradient of all chains in one batched call -- any plan,
+    the matrix-core GLM and the GP included, and a `flow` preconditioner (NUTS in its
+    base space); the same draws as 'persistent' where both run."""
     model = model.condition(args, constraints)
-    if tree_strategy != "persistent":
-        raise ValueError(f"the gfx950 backend implements tree_strategy='persistent', got {tree_strategy!r}")
+    if tree_strategy not in ("persistent", "masked"):
+        raise ValueError(f"batched_nuts tree_strategy must be 'persistent' or 'masked', got {tree_strategy!r}")
+    if flow is not None and tree_strategy != "masked":
+        raise ValueError("a flow preconditions NUTS with tree_strategy='masked' (leaves in lock step)")
     if backend != "hip":
         raise ValueError(f"backend must be 'hip', got {backend!r}")
     if per_chain_adaptation is False:
@@ -236,8 +243,8 @@ def batched_nuts(model: TeaModel, args=(), constraints=None, *, num_chains: int,
     o = _opts(num_samples, num_warmup, step_size, target_accept, divergence_threshold, mass_matrix_regularization,
               mass_matrix_min_samples, adapt_step_size, adapt_mass_matrix, find_reasonable_step_size, init_strategy,
               init_max_retries, seed, max_tree_depth=max_tree_depth)
-    return _run("nuts", model, num_chains, o, initial_params, max_tree_depth, 0, target_accept, callback=callback,
-                callback_every=callback_every)
+    return _run("nuts_masked" if tree_strategy == "masked" else "nuts", model, num_chains, o, initial_params,
+                max_tree_depth, 0, target_accept, flow=flow, callback=callback, callback_every=callback_every)
 
 
 def batched_hmc(model: TeaModel, args=(), constraints=None, *, num_chains: int, num_samples: int,
