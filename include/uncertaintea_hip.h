@@ -169,8 +169,13 @@ int ut_plan_funnel(int32_t dim, double v_scale, ut_plan** out);
  * UT_OP_OBS row `index` of the iteration's block (loop bodies only: covariates,
  * group indices, known scales); arithmetic ops on nodes a (and b).
  *
- * Families and transforms follow src/device/math.jl:50-235, 1137-1159: out of
- * support scores -Inf, never an error.  Gradients are reverse mode through the
+ * Families and transforms follow src/device/math.jl:50-293, 1137-1159: out of
+ * support scores -Inf, never an error; the positive families with a log(x) or 1/x
+ * term (lognormal, gamma, frechet, rayleigh, inversegaussian, inversegamma,
+ * weibull) score x below the smallest normal number of the workspace precision
+ * as outside the support (math.jl:53-66).  The count families (poisson,
+ * binomial, geometric, negativebinomial, and bernoulli's 0/1) are observed only;
+ * binomial's trial count carries no derivative.  Gradients are reverse mode through the
  * plan (per step and per loop iteration), fp32 or fp64 as the workspace.
  *
  * flags: UT_PLAN_GENERIC always runs the plan interpreter; otherwise the plans of
@@ -196,7 +201,17 @@ enum {
   UT_FAMILY_BERNOULLI_LOGIT = 13,
   UT_FAMILY_POISSON = 14,
   UT_FAMILY_DIAG_NORMAL = 15,
-  UT_FAMILY_BERNOULLI_LOGIT_GLM = 16
+  UT_FAMILY_BERNOULLI_LOGIT_GLM = 16,
+  UT_FAMILY_GUMBEL = 17,          /* (mu, scale) */
+  UT_FAMILY_FRECHET = 18,         /* (shape, scale) */
+  UT_FAMILY_RAYLEIGH = 19,        /* (scale) */
+  UT_FAMILY_INVERSEGAUSSIAN = 20, /* (mu, lambda) */
+  UT_FAMILY_BETA = 21,            /* (alpha, beta) */
+  UT_FAMILY_INVERSEGAMMA = 22,    /* (shape, scale) */
+  UT_FAMILY_WEIBULL = 23,         /* (shape, scale) */
+  UT_FAMILY_BINOMIAL = 24,        /* (trials, p), observed only */
+  UT_FAMILY_GEOMETRIC = 25,       /* (p), observed only */
+  UT_FAMILY_NEGATIVEBINOMIAL = 26 /* (successes, p), observed only */
 };
 enum { UT_TRANSFORM_IDENTITY = 0, UT_TRANSFORM_LOG = 1, UT_TRANSFORM_LOGIT = 2 };
 enum {

@@ -34,6 +34,8 @@ def _load():
         "orc_model_funnel": (vp, [i32, dbl]),
         "orc_model_gp": (vp, [vp, vp, i32, i32, dbl, dbl, dbl, dbl]),
         "orc_gp_eval": (None, [vp, vp, i32, vp, vp, vp]),
+        "orc_step_search": (None, [vp, i32, vp, vp, vp, u32, vp]),
+        "orc_nuts_subtree": (None, [vp, vp, vp, vp, dbl, i32, i32, vp, vp, vp, vp, vp]),
         "orc_model_free": (None, [vp]),
         "orc_model_plan": (vp, [vp]),
         "orc_plan_error": (ct.c_char_p, []),
@@ -50,6 +52,7 @@ def _load():
         "orc_chees_run": (ct.c_int, [vp, i32, vp, i32, i32, vp, vp, vp, i32]),
         "orc_meads_run": (ct.c_int, [vp, i32, vp, i32, i32, vp, vp, vp]),
         "orc_meads_max_eig": (dbl, [vp, i32, i32]),
+        "orc_meads_slice_drift": (dbl, [dbl, dbl]),
         "orc_flow_forward": (None, [i32, i32, i32, dbl, i32, vp, vp, i64, vp, vp, vp]),
         "orc_flow_inverse": (None, [i32, i32, i32, dbl, i32, vp, vp, i64, vp, vp]),
         "orc_flow_loss_grad": (dbl, [i32, i32, i32, dbl, i32, vp, vp, i64, vp]),
@@ -174,6 +177,31 @@ class OracleModel:
         g = np.empty((C, self.P))
         _load().orc_gp_eval(self.h, _p(p), C, _p(lj), _p(g), _p(ld))
         return lj, g.T, ld
+
+    def step_search(self, params, inverse_mass, anchors, itc):
+        """binary64 per-chain reasonable step-size search from P x C params with per-chain
+        anchors at Philox iteration coordinate itc."""
+        q = np.ascontiguousarray(np.asarray(params, dtype=np.float64).T)
+        C = q.shape[0]
+        a = np.ascontiguousarray(np.broadcast_to(np.asarray(anchors, dtype=np.float64), (C,)))
+        im = np.ascontiguousarray(inverse_mass, dtype=np.float64)
+        out = np.empty(C)
+        _load().orc_step_search(self.h, C, _p(q), _p(im), _p(a), int(itc) & 0xFFFFFFFF, _p(out))
+        return out
+
+    def nuts_subtree(self, q, p, inv_mass, step, direction, depth):
+        """binary64 NUTS subtree of 2**depth leaves from (q, p) with the checkpoint U-turn test.
+        Returns (steps, turning, divergent, traj_q, traj_p) with the full leapfrog trajectory
+        traj_* of shape (2**depth, P) in leaf order."""
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        p = np.ascontiguousarray(p, dtype=np.float64)
+        im = np.ascontiguousarray(inv_mass, dtype=np.float64)
+        n = 1 << int(depth)
+        tq, tp = np.empty((n, self.P)), np.empty((n, self.P))
+        st, tu, dv = ct.c_int32(), ct.c_int32(), ct.c_int32()
+        _load().orc_nuts_subtree(self.h, _p(q), _p(p), _p(im), float(step), int(direction), int(depth),
+                                 ct.byref(st), ct.byref(tu), ct.byref(dv), _p(tq), _p(tp))
+        return st.value, bool(tu.value), bool(dv.value), tq, tp
 
     def logjoint_grad_twin(self, params, precision=np.float32):
         """Kernel-exact values/gradients (float32 or float64) for P x C params."""
@@ -455,6 +483,11 @@ def iwae_weights_and_bound(logw, valid, group_size):
     if used == 0:
         raise ValueError("batched_advi encountered only non-finite importance weights")
     return w / used, bound / used
+
+
+def meads_slice_drift(u: float, delta: float) -> float:
+    """The oracle's MEADS persistent-slice drift of slice value u by delta."""
+    return float(_load().orc_meads_slice_drift(float(u), float(delta)))
 
 
 def meads_max_eig(columns):

@@ -526,6 +526,55 @@ struct TransOut {
 
 // ------------------------------------------------------------------ NUTS
 // One chain's multinomial NUTS transition; vectors have pol.n() entries.
+// One NUTS subtree of nleaves = 2^d leapfrog leaves grown from (cq, p, wg) in one direction, with
+// the U-turn test of every dyadic sub-subtree evaluated from checkpoints instead of recursion:
+// leaf li (0-based) that is even stores its (q, p) in slot popcount(li); an odd leaf closes the
+// 2^k-leaf subtrees for k = 1..ctz(~li), whose leftmost leaf li - 2^k + 1 sits in slot
+// popcount(li - 2^k + 1). This visits the same subtrees, in the same order, as the recursive
+// build_tree that test/uncertaintea/core/nuts_uturn_turning.jl:25-55 restates; the reference's own
+// _build_nuts_subtree (src/inference/nuts/tree_dynamics.jl:1-93) keeps the same checkpoints. leaf(lj) runs the
+// per-leaf energy/multinomial work and returns false for a divergent leaf.
+template <class Pol, class Leaf>
+void dyadic_subtree(const Pol& pol, const typename Pol::T* im, typename Pol::T ss, bool fwd, int nleaves,
+                    typename Pol::T* cq, typename Pol::T* p, typename Pol::T* wg, typename Pol::T& cur_lj,
+                    typename Pol::T* ck, int& sub_int, bool& sub_div, bool& sub_turn, Leaf&& leaf) {
+  using T = typename Pol::T;
+  const int n = pol.n();
+  const T half = ss / T(2);
+  auto cp = [n](T* d, const T* s) { std::memcpy(d, s, sizeof(T) * n); };
+  for (int l = 0; l < nleaves; ++l) {
+    for (int i = 0; i < n; ++i) {
+      p[i] = p[i] + half * wg[i];
+      cq[i] = cq[i] + ss * (im[i] * p[i]);
+    }
+    bool ok;
+    const T ljc = pol.eval(cq, wg, ok);
+    if (!ok) { sub_div = true; return; }
+    for (int i = 0; i < n; ++i) p[i] = p[i] + half * wg[i];
+    cur_lj = ljc;
+    sub_int += 1;
+    if (!leaf(ljc)) { sub_div = true; return; }
+    const int li = sub_int - 1;
+    if ((li & 1) == 0) {
+      const int slot = __builtin_popcount((unsigned)li);
+      cp(&ck[(size_t)(slot * 2 + 0) * n], cq);
+      cp(&ck[(size_t)(slot * 2 + 1) * n], p);
+    } else {
+      const int nk = __builtin_ctz(~(unsigned)li);
+      bool turned = false;
+      for (int k = 1; k <= nk && !turned; ++k) {
+        const int bs = li - (1 << k) + 1;
+        const int slot = __builtin_popcount((unsigned)bs);
+        const T* cpos = &ck[(size_t)(slot * 2 + 0) * n];
+        const T* cmom = &ck[(size_t)(slot * 2 + 1) * n];
+        if (fwd) turned = pol.turning(cq, cpos, cmom, p, im);
+        else turned = pol.turning(cpos, cq, p, cmom, im);
+      }
+      if (turned) { sub_turn = true; return; }
+    }
+  }
+}
+
 template <class Pol>
 void nuts_transition(const Pol& pol, const typename Pol::T* im, typename Pol::T h, uint32_t chain,
                      uint32_t itc, int max_depth, double div_thr, typename Pol::T* q,
@@ -608,7 +657,6 @@ void nuts_transition(const Pol& pol, const typename Pol::T* im, typename Pol::T 
     const float ud = rand_uniform_f(chain, itc, S_DIR, round);
     const bool fwd = !(ud < 0.5f);
     const T ss = fwd ? h : -h;
-    const T half = ss / T(2);
     if (fwd) { cp(cq, Rq); cp(p, Rp); cp(wg, Rg); }
     else { cp(cq, Lq); cp(p, Lp); cp(wg, Lg); }
     T cur_lj = fwd ? ljR : ljL;
@@ -616,52 +664,26 @@ void nuts_transition(const Pol& pol, const typename Pol::T* im, typename Pol::T 
     int sub_int = 0, sub_acount = 0;
     bool sub_div = false, sub_turn = false;
     const int nleaves = 1 << depth;
-    for (int leaf = 0; leaf < nleaves; ++leaf) {
-      for (int i = 0; i < n; ++i) {
-        p[i] = p[i] + half * wg[i];
-        cq[i] = cq[i] + ss * (im[i] * p[i]);
-      }
-      bool ok;
-      const T ljc = pol.eval(cq, wg, ok);
-      if (!ok) { sub_div = true; break; }
-      for (int i = 0; i < n; ++i) p[i] = p[i] + half * wg[i];
-      cur_lj = ljc;
-      sub_int += 1;
-      const double pe = pol.kinetic(p, im) - (double)ljc;
-      const double delta = pe - H0;
-      if (!std::isfinite(delta) || delta > div_thr) { sub_div = true; break; }
-      const double e = exp_d(std::fmin(0.0, -delta));
-      sub_asum += e < 1.0 ? e : 1.0;
-      sub_acount += 1;
-      const double cand = -pe;
-      const double comb = logaddexp_d(sub_w, cand);
-      const double ul = rand_uniform_d(chain, itc, S_LEAF, leaf_counter);
-      if (!std::isfinite(sub_w) || log_d(ul) < cand - comb) {
-        cp(sq, cq); cp(sg, wg); ljS = ljc;
-        sub_energy = pe;
-        sub_error = delta;
-      }
-      sub_w = comb;
-      leaf_counter += 1;
-      const int li = sub_int - 1;
-      if ((li & 1) == 0) {
-        const int slot = __builtin_popcount((unsigned)li);
-        cp(&ck[(size_t)(slot * 2 + 0) * n], cq);
-        cp(&ck[(size_t)(slot * 2 + 1) * n], p);
-      } else {
-        const int nk = __builtin_ctz(~(unsigned)li);
-        bool turned = false;
-        for (int k = 1; k <= nk && !turned; ++k) {
-          const int bs = li - (1 << k) + 1;
-          const int slot = __builtin_popcount((unsigned)bs);
-          const T* cpos = &ck[(size_t)(slot * 2 + 0) * n];
-          const T* cmom = &ck[(size_t)(slot * 2 + 1) * n];
-          if (fwd) turned = pol.turning(cq, cpos, cmom, p, im);
-          else turned = pol.turning(cpos, cq, p, cmom, im);
-        }
-        if (turned) { sub_turn = true; break; }
-      }
-    }
+    dyadic_subtree(pol, im, ss, fwd, nleaves, cq, p, wg, cur_lj, ck.data(), sub_int, sub_div, sub_turn,
+                   [&](T ljc) {
+                     const double pe = pol.kinetic(p, im) - (double)ljc;
+                     const double delta = pe - H0;
+                     if (!std::isfinite(delta) || delta > div_thr) return false;
+                     const double e = exp_d(std::fmin(0.0, -delta));
+                     sub_asum += e < 1.0 ? e : 1.0;
+                     sub_acount += 1;
+                     const double cand = -pe;
+                     const double comb = logaddexp_d(sub_w, cand);
+                     const double ul = rand_uniform_d(chain, itc, S_LEAF, leaf_counter);
+                     if (!std::isfinite(sub_w) || log_d(ul) < cand - comb) {
+                       cp(sq, cq); cp(sg, wg); ljS = ljc;
+                       sub_energy = pe;
+                       sub_error = delta;
+                     }
+                     sub_w = comb;
+                     leaf_counter += 1;
+                     return true;
+                   });
     depth += 1;
     if (sub_int == 0) { divergent = sub_div; break; }
     if (sub_turn || sub_div) {
@@ -896,6 +918,44 @@ double mass_weight(const RunVar& st, bool accepted, double ap, bool divergent) {
   return 1.0 + (rej - 1.0) * prog;
 }
 
+// Reasonable step-size search of one chain from step s: one leapfrog step with the momentum of
+// stream S_SEARCH at (chain, itc), then doubling (or halving) until the log accept ratio first
+// crosses log 0.5, at most 20 trials, clamped to [1e-8, 1e3] (adaptation.jl:516-571 per chain;
+// the batched per-chain form :438-514 and the window-end re-search call it with each chain's
+// current step as the anchor).
+template <class Pol>
+double reasonable_step(const Pol& pol, const typename Pol::T* im, const typename Pol::T* q,
+                       const typename Pol::T* g, typename Pol::T lj, uint32_t c, uint32_t itc, double s) {
+  using T = typename Pol::T;
+  const int n = pol.n();
+  const double log_target = -0.6931471805599453;
+  auto trial = [&](double st) {
+    std::vector<T> tq(q, q + n), tg(g, g + n);
+    T tlj = lj;
+    TransOut to;
+    double lar;
+    hmc_transition(pol, im, (T)st, c, itc, 1, 0.0, S_SEARCH, 0xffffffffu, tq.data(), tg.data(), tlj, to, &lar);
+    return lar;
+  };
+  const double lar0 = trial(s);
+  const double dir = lar0 > log_target ? 1.0 : -1.0;
+  for (int t = 0; t < 20; ++t) {
+    const double next = s * (dir > 0.0 ? 2.0 : 0.5);
+    if (next < 1e-8 || next > 1e3) break;
+    const double lar = trial(next);
+    s = next;
+    if ((dir > 0.0 && lar <= log_target) || (dir < 0.0 && lar >= log_target)) break;
+  }
+  return s < 1e-8 ? 1e-8 : (s > 1e3 ? 1e3 : s);
+}
+
+// persistent-slice drift ((u + 1 + delta) mod 2) - 1 with a floored mod, in [-1, 1)
+// (api_batched_meads.jl:141)
+double meads_slice_drift(double u, double delta) {
+  const double x = u + 1.0 + delta;
+  return (x - 2.0 * std::floor(x / 2.0)) - 1.0;
+}
+
 uint32_t run_seed_of(uint64_t seed) {
   return philox4x32((uint32_t)seed, (uint32_t)(seed >> 32), 0x5EEDu, 0u, 0u, 0u).w[0];
 }
@@ -980,28 +1040,7 @@ int run_sampler(const Pol& pol, const ut_sampler_opts* o, int C, bool nuts, int 
 #pragma omp parallel for schedule(dynamic, 1)
     for (int c = 0; c < C; ++c) {
       double s = step[c];
-      if (do_search) {
-        const double log_target = -0.6931471805599453;
-        auto trial = [&](double st) {
-          std::vector<T> tq(&q[(size_t)c * n], &q[(size_t)c * n] + n), tg(&g[(size_t)c * n], &g[(size_t)c * n] + n);
-          T tlj = lj[c];
-          TransOut to;
-          double lar;
-          hmc_transition(pol, im.data(), (T)st, (uint32_t)c, itc, 1, 0.0, S_SEARCH, 0xffffffffu,
-                         tq.data(), tg.data(), tlj, to, &lar);
-          return lar;
-        };
-        const double lar0 = trial(s);
-        const double dir = lar0 > log_target ? 1.0 : -1.0;
-        for (int t = 0; t < 20; ++t) {
-          const double next = s * (dir > 0.0 ? 2.0 : 0.5);
-          if (next < 1e-8 || next > 1e3) break;
-          const double lar = trial(next);
-          s = next;
-          if ((dir > 0.0 && lar <= log_target) || (dir < 0.0 && lar >= log_target)) break;
-        }
-        s = s < 1e-8 ? 1e-8 : (s > 1e3 ? 1e3 : s);
-      }
+      if (do_search) s = reasonable_step(pol, im.data(), &q[(size_t)c * n], &g[(size_t)c * n], lj[c], (uint32_t)c, itc, s);
       step[c] = s;
       dual[c] = dual_init(s);
     }
@@ -1702,8 +1741,7 @@ int run_meads(const Pol& pol, const ut_meads_opts* mo, int C, int threads, void*
       div[c] = d;
       acc[c] = false;
       if (f == skip) continue;
-      const double x = slice[c] + 1.0 + delta;
-      const double drifted = (x - 2.0 * std::floor(x / 2.0)) - 1.0;
+      const double drifted = meads_slice_drift(slice[c], delta);
       const bool moved = ok && log_d(std::fabs(drifted)) <= lar;
       acc[c] = moved;
       for (int a = 0; a < P; ++a) {
@@ -2127,6 +2165,60 @@ void* orc_model_gp(const double* inputs, const int32_t* counts, int32_t n, int32
   return m;
 }
 
+// One binary64 subtree of 2^depth leaves from (q, p) with the checkpoint U-turn test of
+// nuts_transition. Writes the leaves integrated before the subtree stopped and whether it turned,
+// and, independently, the full 2^depth-leaf leapfrog trajectory (traj_q, traj_p: 2^depth x P,
+// leaf order) so that a caller can rebuild the recursive tree over the same points.
+void orc_nuts_subtree(void* h, const double* q_in, const double* p_in, const double* im, double step,
+                      int32_t direction, int32_t depth, int32_t* steps, int32_t* turning, int32_t* divergent,
+                      double* traj_q, double* traj_p) {
+  const Model& m = *(const Model*)h;
+  RefF64 pol{&m};
+  const int n = m.P;
+  const bool fwd = direction > 0;
+  const double ss = fwd ? step : -step;
+  std::vector<double> q(q_in, q_in + n), p(p_in, p_in + n), g(n);
+  bool ok;
+  double lj = pol.eval(q.data(), g.data(), ok);
+  std::vector<double> ck((size_t)(depth + 2) * 2 * n);
+  int sub_int = 0;
+  bool sub_div = false, sub_turn = false;
+  std::vector<double> cq = q, cpm = p, cg = g;
+  double cur = lj;
+  dyadic_subtree(pol, im, ss, fwd, 1 << depth, cq.data(), cpm.data(), cg.data(), cur, ck.data(), sub_int,
+                 sub_div, sub_turn, [](double) { return true; });
+  *steps = sub_int;
+  *turning = sub_turn ? 1 : 0;
+  *divergent = sub_div ? 1 : 0;
+  cq = q; cpm = p; cg = g;
+  const double half = ss / 2.0;
+  for (int l = 0; l < (1 << depth); ++l) {
+    for (int i = 0; i < n; ++i) {
+      cpm[i] = cpm[i] + half * cg[i];
+      cq[i] = cq[i] + ss * (im[i] * cpm[i]);
+    }
+    pol.eval(cq.data(), cg.data(), ok);
+    for (int i = 0; i < n; ++i) cpm[i] = cpm[i] + half * cg[i];
+    std::memcpy(traj_q + (size_t)l * n, cq.data(), sizeof(double) * n);
+    std::memcpy(traj_p + (size_t)l * n, cpm.data(), sizeof(double) * n);
+  }
+}
+
+// binary64 per-chain reasonable step-size search (the samplers' initial and window-end search):
+// params C x P, inverse_mass P, anchors C, Philox iteration coordinate itc; writes C steps.
+void orc_step_search(void* h, int32_t C, const double* params, const double* inverse_mass, const double* anchors,
+                     uint32_t itc, double* steps) {
+  const Model& m = *(const Model*)h;
+  RefF64 pol{&m};
+  const int P = m.P;
+  for (int c = 0; c < C; ++c) {
+    std::vector<double> q(params + (size_t)c * P, params + (size_t)(c + 1) * P), g(P);
+    bool ok;
+    const double lj = pol.eval(q.data(), g.data(), ok);
+    steps[c] = reasonable_step(pol, inverse_mass, q.data(), g.data(), lj, (uint32_t)c, itc, anchors[c]);
+  }
+}
+
 // binary64 GP log density and gradient with the log-determinant of K (C chains)
 void orc_gp_eval(void* h, const double* params, int32_t C, double* lj, double* grad, double* logdet) {
   const Model& m = *(Model*)h;
@@ -2217,6 +2309,7 @@ int orc_meads_run(void* h, int32_t mode, const ut_meads_opts* mo, int32_t C, int
   return run_meads(RefF64{m}, mo, C, threads, samples, out, trace);
 }
 // the trace-ratio eigenvalue estimate on an n x P member-major ensemble
+double orc_meads_slice_drift(double u, double delta) { return meads_slice_drift(u, delta); }
 double orc_meads_max_eig(const double* x, int32_t n, int32_t P) {
   std::vector<double> v(x, x + (size_t)n * P);
   return meads_max_eig(v, n, P);

@@ -96,8 +96,9 @@ inline Du dlog1p_exp(const Du& eta) { return eta.v > 0 ? eta + dlog1p(dexp(-eta)
 inline Du neg_inf(int P) { return cst(-INFINITY, P); }
 
 constexpr double C_HALF_LOG_2PI = 0.9189385332046727, C_LOG_PI = 1.1447298858494002, C_LOG_2 = 0.6931471805599453;
+constexpr double DMIN = 2.2250738585072014e-308;  // floatmin(Float64)
 
-// the reference's device log densities (src/device/math.jl:88-235)
+// the reference's device log densities (src/device/math.jl:88-293)
 inline Du ref_family(int fam, const Du& x, const Du* a, int P) {
   switch (fam) {
     case UT_FAMILY_NORMAL: {
@@ -151,6 +152,60 @@ inline Du ref_family(int fam, const Du& x, const Du* a, int P) {
       const double k = std::round(x.v);
       if (!(x.v >= 0 && x.v == k)) return neg_inf(P);
       return (k * dlog(a[0]) - a[0]) - cst(std::lgamma(k + 1.0), P);
+    }
+    case UT_FAMILY_GUMBEL: {
+      const Du z = (x - a[0]) / a[1];
+      return ((-dlog(a[1])) - z) - dexp(-z);
+    }
+    case UT_FAMILY_FRECHET: {
+      if (!(x.v >= DMIN)) return neg_inf(P);
+      const Du lz = dlog(x) - dlog(a[1]);
+      return ((dlog(a[0]) - dlog(a[1])) - (a[0] + 1.0) * lz) - dexp(-(a[0] * lz));
+    }
+    case UT_FAMILY_RAYLEIGH:
+      if (!(x.v >= DMIN)) return neg_inf(P);
+      return (dlog(x) - 2.0 * dlog(a[0])) - (x * x) / (2.0 * (a[0] * a[0]));
+    case UT_FAMILY_INVERSEGAUSSIAN: {
+      if (!(x.v >= DMIN)) return neg_inf(P);
+      const Du d = x - a[0];
+      return ((0.5 * dlog(a[1]) - C_HALF_LOG_2PI) - 1.5 * dlog(x)) - (a[1] * (d * d)) / (2.0 * (a[0] * a[0]) * x);
+    }
+    case UT_FAMILY_BETA: {
+      if (!(x.v > 0.0 && x.v < 1.0)) return neg_inf(P);
+      const Du lb = (dlgamma(a[0]) + dlgamma(a[1])) - dlgamma(a[0] + a[1]);
+      return ((a[0] - 1.0) * dlog(x) + (a[1] - 1.0) * dlog1p(-x)) - lb;
+    }
+    case UT_FAMILY_INVERSEGAMMA:
+      if (!(x.v >= DMIN)) return neg_inf(P);
+      return ((a[0] * dlog(a[1]) - dlgamma(a[0])) - (a[0] + 1.0) * dlog(x)) - a[1] / x;
+    case UT_FAMILY_WEIBULL: {
+      if (!(x.v >= DMIN)) {
+        if (!(x.v == 0.0)) return neg_inf(P);
+        if (a[0].v < 1.0) return cst(INFINITY, P);
+        if (a[0].v == 1.0) return -dlog(a[1]);
+        return neg_inf(P);
+      }
+      return ((dlog(a[0]) + (a[0] - 1.0) * dlog(x)) - a[0] * dlog(a[1])) - dexp(a[0] * (dlog(x) - dlog(a[1])));
+    }
+    case UT_FAMILY_BINOMIAL: {
+      const double k = std::round(x.v), n = std::round(a[0].v);
+      if (!(x.v >= 0 && x.v == k && a[0].v >= 0 && a[0].v == n && k <= n)) return neg_inf(P);
+      Du out = cst(std::lgamma(n + 1.0) - std::lgamma(k + 1.0) - std::lgamma(n - k + 1.0), P);
+      if (k > 0) out = out + k * dlog(a[1]);
+      if (k < n) out = out + (n - k) * dlog1p(-a[1]);
+      return out;
+    }
+    case UT_FAMILY_GEOMETRIC: {
+      const double k = std::round(x.v);
+      if (!(x.v >= 0 && x.v == k)) return neg_inf(P);
+      return k > 0 ? dlog(a[0]) + k * dlog1p(-a[0]) : dlog(a[0]);
+    }
+    case UT_FAMILY_NEGATIVEBINOMIAL: {
+      const double k = std::round(x.v);
+      if (!(x.v >= 0 && x.v == k)) return neg_inf(P);
+      Du out = ((dlgamma(a[0] + k) - dlgamma(a[0])) - cst(std::lgamma(k + 1.0), P)) + a[0] * dlog(a[1]);
+      if (k > 0) out = out + k * dlog1p(-a[1]);
+      return out;
     }
   }
   return cst(NAN, P);

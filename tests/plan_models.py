@@ -107,6 +107,43 @@ def kitchen_sink(n=40, seed=11):
     return b.build("kitchen_sink"), dict(x=x, bern=bern, cnt=cnt, pos=pos, heavy=heavy)
 
 
+def kitchen_sink_more(n=40, seed=12):
+    """The families beyond the first kitchen sink: gumbel, frechet, rayleigh,
+    inversegaussian, beta, inversegamma and weibull as latents (identity, log and logit
+    transforms) and observed, and the count families binomial, geometric and
+    negativebinomial observed in a loop and at top level."""
+    rng = np.random.default_rng(seed)
+    x = rng.normal(size=n)
+    pos = rng.gamma(2.0, 1.0, size=n)
+    unit = rng.beta(2.0, 2.0, size=n)
+    trials = rng.integers(1, 11, size=n).astype(float)
+    succ = rng.binomial(trials.astype(int), 0.4).astype(float)
+    cnt = rng.poisson(2.0, size=n).astype(float)
+    b = ut.PlanBuilder()
+    gb = b.latent("gumbel", 0.3, 1.2, name="gb")
+    fr = b.latent("frechet", 3.0, 1.5, transform="log", name="fr")
+    ry = b.latent("rayleigh", 1.5, transform="log", name="ry")
+    ig = b.latent("inversegaussian", 1.0, 2.0, transform="log", name="ig")
+    bt = b.latent("beta", 2.0, 3.0, transform="logit", name="bt")
+    ivg = b.latent("inversegamma", 3.0, 2.0, transform="log", name="ivg")
+    wb = b.latent("weibull", 2.0, 1.5, transform="log", name="wb")
+    b.observe("binomial", 12.0, bt, value=5.0)
+    b.observe("geometric", PL.logistic(gb), value=0.0)
+    b.observe("negativebinomial", 3.0, bt, value=4.0)
+    with b.loop(np.column_stack([x, pos, unit, trials, succ, cnt]), name="obs") as it:
+        it.observe("gumbel", gb, ry + 1.0, value=it.col(0))
+        it.observe("frechet", fr + 1.0, ig, value=it.col(1))
+        it.observe("rayleigh", wb, value=it.col(1))
+        it.observe("inversegaussian", ivg, fr * 2.0, value=it.col(1))
+        it.observe("beta", bt * 4.0 + 0.5, ry + 1.0, value=it.col(2))
+        it.observe("inversegamma", ig + 1.0, wb, value=it.col(1))
+        it.observe("weibull", fr, ivg, value=it.col(1))
+        it.observe("binomial", it.col(3), PL.logistic(gb * 0.3 + it.col(0) * 0.2), value=it.col(4))
+        it.observe("geometric", bt, value=it.col(5))
+        it.observe("negativebinomial", ry + 0.5, bt, value=it.col(5))
+    return b.build("kitchen_sink_more"), dict(x=x, pos=pos, unit=unit, trials=trials, succ=succ, cnt=cnt)
+
+
 def glm_large(n=8192, d=8, seed=5):
     """A logistic regression with an intercept (not the fused family): N rows for the
     observation-strided interpreter under NUTS."""

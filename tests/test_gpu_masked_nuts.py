@@ -6,7 +6,8 @@ chains, gradients from the batched density kernel, the lock-step step search --
 reproduces the persistent kernel's runs bit for bit: draws, tree depths, steps,
 divergences, accept statistics, energies, step sizes and adapted mass.  That pins
 the state-machine restatement of the transition against the persistent kernel,
-which the oracle pins.
+which the oracle pins.  The conjugate-Gaussian gate and the two-parameter deep-tree
+check of the reference's persistent-NUTS tests run on both strategies.
 
 Statistical (the batched-only plans, where no kernel-order twin exists): masked
 NUTS on the matrix-core GLM agrees with lock-step ChEES on the same data, and on the
@@ -64,6 +65,53 @@ def test_masked_equals_persistent_bitexact(name, C, W, Sn):
         assert np.array_equal(getattr(a, f), getattr(b, f)), (name, f)
     assert a.sampling_gradient_evals == b.sampling_gradient_evals
     say(f"  mean depth {a.tree_depths.mean():.2f}")
+
+
+def _conjugate(y=0.3):
+    b = ut.PlanBuilder()
+    mu = b.latent("normal", 0.0, 1.0, name="mu")
+    b.observe("normal", mu, 1.0, value=y, name="y")
+    return b.build("dpnuts_conjugate_gauss")
+
+
+def _two_param(y=0.7):
+    b = ut.PlanBuilder()
+    mu = b.latent("normal", 0.0, 1.0, name="mu")
+    ls = b.latent("normal", 0.0, 0.5, name="log_sigma")
+    b.observe("normal", mu, ut.plan.exp(ls), value=y, name="y")
+    return b.build("dpnuts_two_param")
+
+
+def test_conjugate_gaussian_gate_persistent_and_masked():
+    """device_persistent_nuts.jl:57-72 on the device: 64 chains, 500 warmup + 1000 draws of
+    mu ~ N(0, 1), y ~ N(mu, 1) at y = 0.3 -- posterior mean 0.15 and sd sqrt(0.5) within 0.03,
+    R-hat < 1.01, no divergences -- for the persistent kernel and the masked sampler, which
+    agree bit for bit."""
+    pm = _conjugate()
+    a = ut.batched_nuts(pm, num_chains=64, num_samples=1000, num_warmup=500, seed=511)
+    b = ut.batched_nuts(pm, num_chains=64, num_samples=1000, num_warmup=500, seed=511, tree_strategy="masked")
+    assert np.array_equal(a.unconstrained_samples.view(np.uint32), b.unconstrained_samples.view(np.uint32))
+    d = a.unconstrained_samples[..., 0].astype(np.float64)
+    say(f"  conjugate mean {d.mean():.4f} sd {d.std(ddof=1):.4f} rhat {float(np.max(D.rhat(a.unconstrained_samples))):.4f}")
+    assert np.all(np.isfinite(d))
+    assert abs(d.mean() - 0.15) < 0.03 and abs(d.std(ddof=1) - np.sqrt(0.5)) < 0.03
+    assert float(np.max(np.asarray(D.rhat(a.unconstrained_samples)))) < 1.01
+    assert not a.divergent.any()
+
+
+def test_two_parameter_model_deep_trees_device():
+    """device_persistent_nuts.jl:99-120 on the device: depth >= 3 reached, R-hat < 1.01,
+    divergence rate < 1 %, persistent and masked bit-identical."""
+    pm = _two_param()
+    a = ut.batched_nuts(pm, num_chains=64, num_samples=1000, num_warmup=500, seed=2)
+    b = ut.batched_nuts(pm, num_chains=64, num_samples=1000, num_warmup=500, seed=2, tree_strategy="masked")
+    assert np.array_equal(a.unconstrained_samples.view(np.uint32), b.unconstrained_samples.view(np.uint32))
+    assert np.array_equal(a.tree_depths, b.tree_depths)
+    say(f"  two-param max depth {a.tree_depths.max()} div {a.divergence_rate():.4f}")
+    assert np.all(np.isfinite(a.unconstrained_samples))
+    assert float(np.max(np.asarray(D.rhat(a.unconstrained_samples)))) < 1.01
+    assert a.divergence_rate() < 0.01
+    assert a.tree_depths.max() >= 3
 
 
 def _zmeans(a, b):

@@ -6,7 +6,7 @@
   their dedicated kernels: the same engine and bit-identical densities and gradients
   as the ut_plan_* constructors.
 * The same descriptions forced onto the plan interpreter, and non-configuration
-  models (centred eight schools, linear regression, a model with every family,
+  models (centred eight schools, linear regression, two models with every family,
   transform and op, a logistic regression with an intercept at N = 8192): density
   and gradient bit-exact against the oracle's kernel-order twin (binary32 and
   binary64), and within 1e-5 (binary32) of the oracle's binary64 reference-semantics
@@ -93,7 +93,7 @@ def _interpreted(name):
 
 
 @pytest.mark.parametrize("name", ["hier_generic", "logistic_generic", "funnel_generic", "eight_schools_centered",
-                                  "linear_regression", "kitchen_sink", "glm_large"])
+                                  "linear_regression", "kitchen_sink", "kitchen_sink_more", "glm_large"])
 def test_interpreter_density_against_twin_and_reference(name):
     say("plan interpreter density", name)
     pm = _interpreted(name)
@@ -158,3 +158,39 @@ def test_eight_schools_posterior_nuts_interpreter():
         f"div {r.divergence_rate():.4f}")
     assert 3.0 < c[:, :, 0].mean() < 5.8
     assert 4.5 < c[:, :, 2].mean() < 7.5
+
+
+def test_float32_saturation_edges_on_the_device():
+    """The binary32 saturation edges (device_float32_saturation.jl:78-160) on the device: the
+    guarded positive families score -Inf with a finite gradient in the subnormal band of
+    exp(theta) and past underflow, finite just outside it; exponential stays finite; the beta
+    latent's logistic cliff rejects past theta = 18. Bit-exact against the twin at every point."""
+    thetas = np.array([-120.0, -100.0, -90.0, -87.0, -80.0, -10.0, 0.5], dtype=np.float32)
+    fams = {"gamma": (2.0, 1.0), "lognormal": (0.0, 1.0), "weibull": (2.0, 1.5), "inversegamma": (3.0, 2.0),
+            "frechet": (2.0, 1.5), "rayleigh": (1.5,), "inversegaussian": (1.0, 2.0), "exponential": (1.5,)}
+    for fam, args in fams.items():
+        say("saturation", fam)
+        b = ut.PlanBuilder()
+        b.latent(fam, *args, transform="log", name="x")
+        pm = b.build(f"dfs_{fam}")
+        q = thetas.reshape(1, -1)
+        v, g = ut.device_batched_logjoint_gradient(pm, q)
+        tv, tg = B.OracleModel(pm).logjoint_grad_twin(q)
+        assert _bits_equal(v, tv) and _bits_equal(g, tg), fam
+        assert not np.isnan(g).any()
+        if fam == "exponential":
+            assert np.isfinite(v).all()
+        else:
+            # theta = -80 is finite where the density has no exp(-k log x) or 1/x blow-up
+            # (device_float32_saturation.jl:90); frechet's exp(-k log z) overflows binary32 there
+            first_finite = 4 if fam in ("gamma", "lognormal", "weibull", "rayleigh") else 5
+            assert np.all(v[:3] == -np.inf) and np.isfinite(v[first_finite:]).all(), (fam, v)
+    b = ut.PlanBuilder()
+    p = b.latent("beta", 2.0, 2.0, transform="logit", name="p")
+    b.observe("bernoulli", p, value=1.0, name="y")
+    pm = b.build("dfs_beta_bern")
+    q = np.array([[-1.5, 0.3, 2.0, 13.0, 16.0, 18.0, 20.0, 30.0]], dtype=np.float32)
+    v, g = ut.device_batched_logjoint_gradient(pm, q)
+    tv, tg = B.OracleModel(pm).logjoint_grad_twin(q)
+    assert _bits_equal(v, tv) and _bits_equal(g, tg)
+    assert np.isfinite(v[:5]).all() and np.all(v[5:] == -np.inf) and not np.isnan(g).any()

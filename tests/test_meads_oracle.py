@@ -1,7 +1,7 @@
 """The oracle's MEADS restatement (CPU): the reference's known answers for the
 trace-ratio eigenvalue estimate and the slice drift
 (test/uncertaintea/core/batched_meads.jl:32-57), the estimate against numpy's
-eigenvalues where the ratio is exact, and posterior recovery on a logistic
+eigenvalues where the ratio is exact, the drift as a rotation of [-1, 1), and posterior recovery on a logistic
 regression against the oracle's NUTS."""
 import numpy as np
 import pytest
@@ -36,17 +36,28 @@ def test_max_eig_is_the_trace_ratio():
     assert B.meads_max_eig(Q[:, :5] * 3.0) == pytest.approx(0.0, abs=1e-12)
 
 
-def _drift(u, d):  # restates api_batched_meads.jl:141 with floored mod
-    x = u + 1.0 + d
-    return (x - 2.0 * np.floor(x / 2.0)) - 1.0
-
-
 def test_slice_drift_known_answers():
-    assert _drift(0.9, 0.2) == pytest.approx(-0.9, abs=1e-12)
-    assert _drift(-0.9, 0.0) == pytest.approx(-0.9, abs=1e-12)
+    # batched_meads.jl:48-55 against the oracle's drift function itself
+    assert B.meads_slice_drift(0.9, 0.2) == pytest.approx(-0.9, abs=1e-12)
+    assert B.meads_slice_drift(-0.9, 0.0) == pytest.approx(-0.9, abs=1e-12)
     for u in (-0.99, -0.5, 0.0, 0.4, 0.999):
         for d in (0.0, 0.1, 0.5, 0.9):
-            assert -1.0 <= _drift(u, d) < 1.0
+            assert -1.0 <= B.meads_slice_drift(u, d) < 1.0
+
+
+def test_slice_drift_is_a_rotation_of_the_circle():
+    # mod 2 arithmetic: drifting by d1 then d2 equals drifting by d1 + d2 (mod 2),
+    # drifting by 2 is the identity and the map is a translation on [-1, 1)
+    rng = np.random.default_rng(8)
+    for _ in range(200):
+        u = rng.uniform(-1.0, 1.0)
+        d1, d2 = rng.uniform(0.0, 1.0, size=2)
+        two = B.meads_slice_drift(B.meads_slice_drift(u, d1), d2)
+        one = B.meads_slice_drift(u, d1 + d2)
+        assert min(abs(two - one), 2.0 - abs(two - one)) < 1e-12
+        assert B.meads_slice_drift(u, 2.0) == pytest.approx(u, abs=1e-12)
+        t = B.meads_slice_drift(u, d1)
+        assert min(abs((t - u) % 2.0 - d1), 2.0 - abs((t - u) % 2.0 - d1)) < 1e-12
 
 
 def test_meads_posterior_matches_nuts():

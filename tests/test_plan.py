@@ -107,6 +107,46 @@ def test_plan_oracle_against_scipy_every_family_transform_and_op():
     assert _rel(l32, lj) <= 1e-6 and _rel(g32, g) <= 1e-5
 
 
+def _kitchen_sink_more_scipy(q, d):
+    """kitchen_sink_more's logjoint with scipy.stats (unconstrained q)."""
+    gb, fr, ry, ig, ivg, wb = q[0], np.exp(q[1]), np.exp(q[2]), np.exp(q[3]), np.exp(q[5]), np.exp(q[6])
+    bt = 1 / (1 + np.exp(-q[4]))
+    lp = stats.gumbel_r.logpdf(gb, 0.3, 1.2)
+    lp += stats.invweibull.logpdf(fr, 3.0, scale=1.5) + q[1]
+    lp += stats.rayleigh.logpdf(ry, scale=1.5) + q[2]
+    lp += stats.invgauss.logpdf(ig, 1.0 / 2.0, scale=2.0) + q[3]
+    lp += stats.beta.logpdf(bt, 2.0, 3.0) + np.log(bt * (1 - bt))
+    lp += stats.invgamma.logpdf(ivg, 3.0, scale=2.0) + q[5]
+    lp += stats.weibull_min.logpdf(wb, 2.0, scale=1.5) + q[6]
+    lp += stats.binom.logpmf(5, 12, bt) + stats.geom.logpmf(1, 1 / (1 + np.exp(-gb)))
+    lp += stats.nbinom.logpmf(4, 3.0, bt)
+    x, pos, unit, trials, succ, cnt = (d[k] for k in ("x", "pos", "unit", "trials", "succ", "cnt"))
+    lp += stats.gumbel_r.logpdf(x, gb, ry + 1.0).sum()
+    lp += stats.invweibull.logpdf(pos, fr + 1.0, scale=ig).sum()
+    lp += stats.rayleigh.logpdf(pos, scale=wb).sum()
+    lp += stats.invgauss.logpdf(pos, ivg / (fr * 2.0), scale=fr * 2.0).sum()
+    lp += stats.beta.logpdf(unit, bt * 4.0 + 0.5, ry + 1.0).sum()
+    lp += stats.invgamma.logpdf(pos, ig + 1.0, scale=wb).sum()
+    lp += stats.weibull_min.logpdf(pos, fr, scale=ivg).sum()
+    lp += stats.binom.logpmf(succ, trials, 1 / (1 + np.exp(-(gb * 0.3 + x * 0.2)))).sum()
+    lp += stats.geom.logpmf(cnt + 1, bt).sum() + stats.nbinom.logpmf(cnt, ry + 0.5, bt).sum()
+    return lp
+
+
+def test_plan_oracle_against_scipy_more_families():
+    pm, d = PMS.kitchen_sink_more()
+    o = B.OracleModel(pm)
+    q = np.random.default_rng(5).normal(size=(pm.num_params, 6)) * 0.4
+    lj, g = o.logjoint_grad_f64(q)
+    ref = np.array([_kitchen_sink_more_scipy(q[:, c], d) for c in range(q.shape[1])])
+    assert _rel(lj, ref) <= 1e-11, (lj, ref)
+    assert _rel(g, _fd(o, q)) <= 1e-6
+    lt, gt = o.logjoint_grad_twin(q, precision=np.float64)
+    assert _rel(lt, lj) <= 1e-12 and _rel(gt, g) <= 1e-9
+    l32, g32 = o.logjoint_grad_twin(q.astype(np.float32))
+    assert _rel(l32, lj) <= 1e-6 and _rel(g32, g) <= 1e-5
+
+
 @pytest.mark.parametrize("name", ["eight_schools_centered", "linear_regression"])
 def test_plan_oracle_against_scipy_regressions(name):
     pm, data = getattr(PMS, name)()

@@ -28,7 +28,9 @@ UT_STEP_CHOICE, UT_STEP_DETERMINISTIC, UT_STEP_LOOP = 1, 2, 3
 (UT_FAMILY_NORMAL, UT_FAMILY_NONCENTERED_NORMAL, UT_FAMILY_LOGNORMAL, UT_FAMILY_HALFNORMAL, UT_FAMILY_HALFCAUCHY,
  UT_FAMILY_EXPONENTIAL, UT_FAMILY_CAUCHY, UT_FAMILY_LAPLACE, UT_FAMILY_LOGISTIC, UT_FAMILY_STUDENTT, UT_FAMILY_GAMMA,
  UT_FAMILY_BERNOULLI, UT_FAMILY_BERNOULLI_LOGIT, UT_FAMILY_POISSON, UT_FAMILY_DIAG_NORMAL,
- UT_FAMILY_BERNOULLI_LOGIT_GLM) = range(1, 17)
+ UT_FAMILY_BERNOULLI_LOGIT_GLM, UT_FAMILY_GUMBEL, UT_FAMILY_FRECHET, UT_FAMILY_RAYLEIGH, UT_FAMILY_INVERSEGAUSSIAN,
+ UT_FAMILY_BETA, UT_FAMILY_INVERSEGAMMA, UT_FAMILY_WEIBULL, UT_FAMILY_BINOMIAL, UT_FAMILY_GEOMETRIC,
+ UT_FAMILY_NEGATIVEBINOMIAL) = range(1, 27)
 UT_TRANSFORM_IDENTITY, UT_TRANSFORM_LOG, UT_TRANSFORM_LOGIT = 0, 1, 2
 (UT_OP_LITERAL, UT_OP_SLOT, UT_OP_PARAM, UT_OP_OBS, UT_OP_ADD, UT_OP_SUB, UT_OP_MUL, UT_OP_DIV, UT_OP_NEG, UT_OP_EXP,
  UT_OP_LOG, UT_OP_LOG1P, UT_OP_SQRT, UT_OP_POW, UT_OP_ABS, UT_OP_TANH, UT_OP_LOGISTIC) = range(1, 18)

@@ -142,6 +142,12 @@ constexpr double PLAN_HALF_LOG_2PI = 0.9189385332046727;
 constexpr double PLAN_LOG_PI = 1.1447298858494002;
 constexpr double PLAN_LOG_2 = 0.6931471805599453;
 
+// exact non-negative integer (the count families' support; |x| < 2^52 holds for counts)
+template <class T> UT_HD bool plan_count_ok(T x) {
+  if (!(x >= T(0) && PM<T>::finite(x))) return false;
+  return (T)(long long)x == x;
+}
+
 // log density of `x` and its partials in x and the arguments a[0..2]
 template <class T> UT_HD FamilyOut<T> plan_family(int fam, T x, const T* a) {
   FamilyOut<T> o;
@@ -257,15 +263,123 @@ template <class T> UT_HD FamilyOut<T> plan_family(int fam, T x, const T* a) {
     case UT_FAMILY_POISSON: {  // (lambda); counts are exact non-negative integers
       const T lam = a[0];
       const T k = x;
-      bool integral = x >= T(0) && PM<T>::finite(x);
-      if (integral) {
-        // x is integral when it equals its truncation (|x| < 2^52 holds for counts)
-        const T tr = (T)(long long)x;
-        integral = tr == x;
-      }
-      if (!integral) return fam_outside<T>();
+      if (!plan_count_ok(x)) return fam_outside<T>();
       o.lp = (k * PM<T>::log(lam) - lam) - t_lgamma(k + T(1));
       o.da[0] = k / lam - T(1);
+      return o;
+    }
+    case UT_FAMILY_GUMBEL: {  // (mu, scale)
+      const T s = a[1], z = (x - a[0]) / s, ez = PM<T>::exp(-z);
+      o.lp = (-PM<T>::log(s) - z) - ez;
+      o.dx = (ez - T(1)) / s;
+      o.da[0] = -o.dx;
+      o.da[1] = (z * (T(1) - ez) - T(1)) / s;
+      return o;
+    }
+    case UT_FAMILY_FRECHET: {  // (shape, scale)
+      if (!(x >= PM<T>::tiny())) return fam_outside<T>();
+      const T k = a[0], s = a[1], ls = PM<T>::log(s), lz = PM<T>::log(x) - ls, t = PM<T>::exp(-k * lz);
+      o.lp = ((PM<T>::log(k) - ls) - (T(1) + k) * lz) - t;
+      o.dx = (k * t - (T(1) + k)) / x;
+      o.da[0] = T(1) / k - lz * (T(1) - t);
+      o.da[1] = k * (T(1) - t) / s;
+      return o;
+    }
+    case UT_FAMILY_RAYLEIGH: {  // (scale)
+      if (!(x >= PM<T>::tiny())) return fam_outside<T>();
+      const T s = a[0], s2 = s * s;
+      o.lp = (PM<T>::log(x) - T(2) * PM<T>::log(s)) - x * x / (T(2) * s2);
+      o.dx = T(1) / x - x / s2;
+      o.da[0] = (x * x / s2 - T(2)) / s;
+      return o;
+    }
+    case UT_FAMILY_INVERSEGAUSSIAN: {  // (mu, lambda)
+      if (!(x >= PM<T>::tiny())) return fam_outside<T>();
+      const T mu = a[0], lam = a[1], d = x - mu, mu2 = mu * mu;
+      o.lp = ((T(0.5) * PM<T>::log(lam) - T(PLAN_HALF_LOG_2PI)) - T(1.5) * PM<T>::log(x)) -
+             lam * d * d / (T(2) * mu2 * x);
+      o.dx = -T(1.5) / x - lam * d * (x + mu) / (T(2) * mu2 * x * x);
+      o.da[0] = lam * d / (mu2 * mu);
+      o.da[1] = T(0.5) / lam - d * d / (T(2) * mu2 * x);
+      return o;
+    }
+    case UT_FAMILY_BETA: {  // (alpha, beta)
+      if (!(x > T(0) && x < T(1))) return fam_outside<T>();
+      const T al = a[0], be = a[1], lx = PM<T>::log(x), l1x = PM<T>::log1p(-x);
+      const T lbeta = (t_lgamma(al) + t_lgamma(be)) - t_lgamma(al + be);
+      o.lp = ((al - T(1)) * lx + (be - T(1)) * l1x) - lbeta;
+      o.dx = (al - T(1)) / x - (be - T(1)) / (T(1) - x);
+      const T dab = t_digamma(al + be);
+      o.da[0] = (lx - t_digamma(al)) + dab;
+      o.da[1] = (l1x - t_digamma(be)) + dab;
+      return o;
+    }
+    case UT_FAMILY_INVERSEGAMMA: {  // (shape, scale)
+      if (!(x >= PM<T>::tiny())) return fam_outside<T>();
+      const T k = a[0], s = a[1], lx = PM<T>::log(x), ls = PM<T>::log(s);
+      o.lp = ((k * ls - t_lgamma(k)) - (k + T(1)) * lx) - s / x;
+      o.dx = -(k + T(1)) / x + s / (x * x);
+      o.da[0] = (ls - t_digamma(k)) - lx;
+      o.da[1] = k / s - T(1) / x;
+      return o;
+    }
+    case UT_FAMILY_WEIBULL: {  // (shape, scale)
+      const T k = a[0], s = a[1], ls = PM<T>::log(s);
+      if (!(x >= PM<T>::tiny())) {
+        if (!(x == T(0))) return fam_outside<T>();
+        // density at 0: +Inf below shape 1, 1/scale at shape 1, 0 above (math.jl:181-186)
+        if (k < T(1)) {
+          o.lp = PM<T>::inf();
+        } else if (k == T(1)) {
+          o.lp = -ls;
+          o.da[1] = -T(1) / s;
+        } else {
+          return fam_outside<T>();
+        }
+        return o;
+      }
+      const T lx = PM<T>::log(x), lr = lx - ls, w = PM<T>::exp(k * lr);
+      o.lp = ((PM<T>::log(k) + (k - T(1)) * lx) - k * ls) - w;
+      o.dx = ((k - T(1)) - k * w) / x;
+      o.da[0] = (T(1) / k + lr) - w * lr;
+      o.da[1] = k * (w - T(1)) / s;
+      return o;
+    }
+    case UT_FAMILY_BINOMIAL: {  // (trials, p); the count and the trials are exact integers
+      const T n = a[0], p = a[1], k = x;
+      if (!plan_count_ok(x) || !plan_count_ok(n) || !(k <= n)) return fam_outside<T>();
+      o.lp = (t_lgamma(n + T(1)) - t_lgamma(k + T(1))) - t_lgamma((n - k) + T(1));
+      if (k > T(0)) {
+        o.lp = o.lp + k * PM<T>::log(p);
+        o.da[1] = k / p;
+      }
+      if (k < n) {
+        o.lp = o.lp + (n - k) * PM<T>::log1p(-p);
+        o.da[1] = o.da[1] - (n - k) / (T(1) - p);
+      }
+      return o;  // no derivative in the integer trial count
+    }
+    case UT_FAMILY_GEOMETRIC: {  // (p): failures before the first success
+      const T p = a[0], k = x;
+      if (!plan_count_ok(x)) return fam_outside<T>();
+      o.lp = PM<T>::log(p);
+      o.da[0] = T(1) / p;
+      if (k > T(0)) {
+        o.lp = o.lp + k * PM<T>::log1p(-p);
+        o.da[0] = o.da[0] - k / (T(1) - p);
+      }
+      return o;
+    }
+    case UT_FAMILY_NEGATIVEBINOMIAL: {  // (successes r, p)
+      const T r = a[0], p = a[1], k = x;
+      if (!plan_count_ok(x)) return fam_outside<T>();
+      o.lp = (((t_lgamma(k + r) - t_lgamma(r)) - t_lgamma(k + T(1))) + r * PM<T>::log(p));
+      o.da[0] = (t_digamma(k + r) - t_digamma(r)) + PM<T>::log(p);
+      o.da[1] = r / p;
+      if (k > T(0)) {
+        o.lp = o.lp + k * PM<T>::log1p(-p);
+        o.da[1] = o.da[1] - k / (T(1) - p);
+      }
       return o;
     }
   }
@@ -283,6 +397,8 @@ UT_HD int plan_family_arity(int fam) {
     case UT_FAMILY_BERNOULLI_LOGIT:
     case UT_FAMILY_POISSON:
     case UT_FAMILY_BERNOULLI_LOGIT_GLM:
+    case UT_FAMILY_RAYLEIGH:
+    case UT_FAMILY_GEOMETRIC:
       return 1;
     case UT_FAMILY_STUDENTT:
       return 3;

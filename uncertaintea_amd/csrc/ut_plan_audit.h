@@ -19,6 +19,12 @@
 
 namespace ut {
 
+// families whose support is discrete: observed only
+inline bool plan_family_discrete(int fam) {
+  return fam == UT_FAMILY_BERNOULLI || fam == UT_FAMILY_BERNOULLI_LOGIT || fam == UT_FAMILY_POISSON ||
+         fam == UT_FAMILY_BINOMIAL || fam == UT_FAMILY_GEOMETRIC || fam == UT_FAMILY_NEGATIVEBINOMIAL;
+}
+
 struct PlanAudit {
   std::vector<int> first_node;  // per node, the first node of the expression rooted there (-1 unused)
   std::vector<int> cursor;      // per step (see PlanView)
@@ -236,7 +242,7 @@ inline int plan_audit(const ut_plan_desc* dp, PlanAudit& a) {
                                              ": latent choices inside loops are not lowerable");
           if (b.family == UT_FAMILY_DIAG_NORMAL || b.family == UT_FAMILY_NONCENTERED_NORMAL)
             return bad(UT_E_UNSUPPORTED, "step " + std::to_string(bi) + ": family not valid in a loop body");
-          if (b.family < UT_FAMILY_NORMAL || b.family > UT_FAMILY_BERNOULLI_LOGIT_GLM)
+          if (b.family < UT_FAMILY_NORMAL || b.family > UT_FAMILY_NEGATIVEBINOMIAL)
             return bad(UT_E_UNSUPPORTED, "step " + std::to_string(bi) + ": unknown family");
           const int na = plan_family_arity(b.family);
           for (int j = 0; j < na; ++j)
@@ -307,7 +313,7 @@ inline int plan_audit(const ut_plan_desc* dp, PlanAudit& a) {
     }
     if (st.kind != UT_STEP_CHOICE) return bad(UT_E_ARG, "step " + std::to_string(si) + ": unknown step kind");
     const int fam = st.family;
-    if (fam < UT_FAMILY_NORMAL || fam > UT_FAMILY_BERNOULLI_LOGIT_GLM)
+    if (fam < UT_FAMILY_NORMAL || fam > UT_FAMILY_NEGATIVEBINOMIAL)
       return bad(UT_E_UNSUPPORTED, "step " + std::to_string(si) + ": unknown family");
     if (fam == UT_FAMILY_BERNOULLI_LOGIT_GLM)
       return bad(UT_E_UNSUPPORTED, "step " + std::to_string(si) + ": the GLM step belongs in a loop body");
@@ -317,7 +323,7 @@ inline int plan_audit(const ut_plan_desc* dp, PlanAudit& a) {
     for (int j = 0; j < na; ++j)
       if ((rc = check_expr(st.args[j], si, false, &reads, nullptr))) return rc;
     if ((rc = check_reads(reads, si, -1))) return rc;
-    const bool discrete = fam == UT_FAMILY_BERNOULLI || fam == UT_FAMILY_BERNOULLI_LOGIT || fam == UT_FAMILY_POISSON;
+    const bool discrete = plan_family_discrete(fam);
     if (st.value_source > 0) {
       if (discrete) return bad(UT_E_UNSUPPORTED, "step " + std::to_string(si) + ": discrete latents are not lowerable");
       if (fam == UT_FAMILY_DIAG_NORMAL) {
@@ -438,6 +444,20 @@ inline void plan_prior_draw(const ut_plan_desc& d, int strategy, uint32_t chain,
         x = a[0] + a[1] * log_d(v / (1.0 - v));
         break;
       }
+      case UT_FAMILY_GUMBEL: {
+        double v = u(r);
+        v = v < 1e-300 ? 1e-300 : v;
+        x = a[0] - a[1] * log_d(-log_d(v));
+        break;
+      }
+      case UT_FAMILY_FRECHET: {
+        double v = u(r);
+        v = v < 1e-300 ? 1e-300 : v;
+        x = a[1] * exp_d(-log_d(-log_d(v)) / a[0]);
+        break;
+      }
+      case UT_FAMILY_RAYLEIGH: x = a[0] * sqrt_d(-2.0 * log_d(1.0 - u(r))); break;
+      case UT_FAMILY_WEIBULL: x = a[1] * exp_d(log_d(-log_d(1.0 - u(r))) / a[0]); break;
       default: unconstrained_uniform = true;
     }
     double uu;

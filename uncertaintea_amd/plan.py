@@ -41,7 +41,12 @@ FAMILIES = {
     "cauchy": (L.UT_FAMILY_CAUCHY, 2), "laplace": (L.UT_FAMILY_LAPLACE, 2), "logistic": (L.UT_FAMILY_LOGISTIC, 2),
     "studentt": (L.UT_FAMILY_STUDENTT, 3), "gamma": (L.UT_FAMILY_GAMMA, 2),
     "bernoulli": (L.UT_FAMILY_BERNOULLI, 1), "bernoullilogit": (L.UT_FAMILY_BERNOULLI_LOGIT, 1),
-    "poisson": (L.UT_FAMILY_POISSON, 1),
+    "poisson": (L.UT_FAMILY_POISSON, 1), "gumbel": (L.UT_FAMILY_GUMBEL, 2),
+    "frechet": (L.UT_FAMILY_FRECHET, 2), "rayleigh": (L.UT_FAMILY_RAYLEIGH, 1),
+    "inversegaussian": (L.UT_FAMILY_INVERSEGAUSSIAN, 2), "beta": (L.UT_FAMILY_BETA, 2),
+    "inversegamma": (L.UT_FAMILY_INVERSEGAMMA, 2), "weibull": (L.UT_FAMILY_WEIBULL, 2),
+    "binomial": (L.UT_FAMILY_BINOMIAL, 2), "geometric": (L.UT_FAMILY_GEOMETRIC, 1),
+    "negativebinomial": (L.UT_FAMILY_NEGATIVEBINOMIAL, 2),
 }
 TRANSFORMS = {"identity": L.UT_TRANSFORM_IDENTITY, "log": L.UT_TRANSFORM_LOG, "logit": L.UT_TRANSFORM_LOGIT}
 ENGINES = {L.UT_ENGINE_INTERPRETER: "interpreter", L.UT_ENGINE_LOGISTIC: "logistic_glm",
