@@ -6,7 +6,9 @@ against the oracle's binary32 twin.  Tolerance: the matrix-core GLM logjoint and
 gradient against the binary64 reference formulas on the same bf16 design matrix,
 errors printed and bounded: logjoint |d| <= 1e-6 |lj|; gradient normwise
 |d| <= 1e-5 |g| and componentwise |d_d| <= 2e-6 (1 + sum_i |x_id|).  Measured on
-MI355X: 2.5e-7, 1.7e-6 and 3.7e-7."""
+MI355X: 2.5e-7, 1.7e-6 and 3.7e-7.  The same bounds hold at configuration 3's own
+shape (N = 131072, D = 256, 4096 chains through the split-K layout), checked on a
+128-chain slice from both ends of the batch against the binary64 oracle."""
 import sys
 
 import numpy as np
@@ -91,6 +93,23 @@ def test_glm_mfma_density_accuracy(n, C):
     assert np.linalg.norm(g - rg) <= 1e-5 * np.linalg.norm(rg)
     # logjoint-only call agrees
     assert np.array_equal(ut.device_batched_logjoint(model, params), v)
+
+
+def test_glm_mfma_density_cfg3_full_shape():
+    say("glm mfma density cfg3 N=131072 D=256 C=4096")
+    model = _glm(131072)
+    om = B.OracleModel(model)
+    params = np.random.default_rng(9).normal(size=(256, 4096)).astype(np.float32)
+    v, g = ut.device_batched_logjoint_gradient(model, params)
+    cols = np.r_[0:64, 4032:4096]
+    rv, rg = om.logjoint_grad_f64(params[:, cols].astype(np.float64))
+    lj_rel = np.abs(v[cols] - rv) / np.abs(rv)
+    scale = np.abs(model.x).sum(axis=0)[:, None] + 1.0
+    g_err = np.abs(g[:, cols] - rg) / scale
+    nw = np.linalg.norm(g[:, cols] - rg) / np.linalg.norm(rg)
+    say(f"  logjoint rel err max {lj_rel.max():.3e}; gradient err / sum|x| max {g_err.max():.3e}; normwise {nw:.3e}")
+    assert np.all(np.isfinite(v)) and np.all(np.isfinite(g))
+    assert lj_rel.max() <= 1e-6 and g_err.max() <= 2e-6 and nw <= 1e-5
 
 
 def test_glm_mfma_argument_errors():
