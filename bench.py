@@ -24,8 +24,11 @@ unit 100 warmup iterations, so the driver's `--steps 20 --warmup 5` samples 4000
 per chain after 500 warmup iterations.  `value` is the flow run's min over parameters
 of rank-normalised bulk ESS (constrained space) divided by its sampling wall time
 (chain state resident in HBM; the time includes copying the draws back), and is null
-unless every parameter has split R-hat < 1.01 (the reference harness's gate,
-bench/crossppl/python/analyze.py:76-97); `ungated_value` keeps the raw quotient.
+unless the run passes the reference harness's gate (bench/crossppl/python/analyze.py:
+51-97): every parameter's rank R-hat < 1.01 and its mean and 5 % / 95 % quantiles within
+4 MCSE of the reference -- here the funnel's exact law, so z is against the run's own
+MCSE -- with the harness's ArviZ 0.23.4 estimators (bulk / tail ESS, quantile MCSE);
+`ungated_value` keeps the raw quotient.
 The plain run is reported beside it under the same gate.
 
 Also reported: gradient evaluations/s; the roofline of the dominant kernel pair (the
@@ -68,6 +71,7 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "min bulk-ESS/sec at R̂<1.01, plain vs flow-preconditioned HMC, 1 MI355X"
 RHAT_MAX = 1.01
+Z_MAX = 4.0  # analyze.py's moment gate: combined-MCSE z of mean, q05 and q95
 PEAK_FP32_TFLOPS = 157.3
 PEAK_FP64_TFLOPS = 78.6
 PEAK_BF16_TFLOPS = 2500.0
@@ -113,6 +117,12 @@ def cpu_cores() -> int:
     return max(1, cpu_info()["affinity_cores"])
 
 
+def harness_rhat_ess(D, draws):
+    """Rank R-hat and bulk ESS per parameter with the harness's ArviZ 0.23.4 estimators."""
+    st = D.harness_stats(draws)
+    return st["rhat"], st["ess_bulk"]
+
+
 def moments_vs(D, a, b) -> dict:
     """Posterior means and variances of draws `a` against `b` (chains x draws x params)
     in units of their combined Monte Carlo standard errors: z = (m_a - m_b) / sqrt(se_a^2
@@ -141,6 +151,21 @@ def moments_vs_exact(D, a, mean, var) -> dict:
     zv = np.abs(dev.mean(axis=(0, 1)) - var) / np.asarray(D.mcse_mean(dev))
     return {"max_abs_z_mean": float(np.nanmax(zm)), "max_abs_z_var": float(np.nanmax(zv)),
             "within_mcse": bool(np.nanmax(zm) < 4.0 and np.nanmax(zv) < 4.0)}
+
+
+def funnel_exact_stats(dim: int, v_scale: float) -> dict:
+    """Mean, 5 % and 95 % quantiles of (v, x_1..x_dim) under Neal's funnel: v ~ N(0, s^2) and
+    x_i | v ~ N(0, e^v); the x_i quantile q_p solves E_v[Phi(q e^(-v/2))] = p (quadrature)."""
+    from scipy import integrate, optimize, stats
+
+    def cdf(q):
+        f = lambda v: stats.norm.cdf(q * np.exp(-0.5 * v)) * stats.norm.pdf(v, 0.0, v_scale)  # noqa: E731
+        return integrate.quad(f, -12 * v_scale, 12 * v_scale, limit=400, epsabs=1e-12)[0]
+    qx = {p: optimize.brentq(lambda q: cdf(q) - p, -1e4, 1e4, xtol=1e-12) for p in (0.05, 0.95)}
+    zq = stats.norm.ppf(0.95)
+    return {"mean": np.zeros(dim + 1),
+            "q05": np.concatenate([[-zq * v_scale], np.full(dim, qx[0.05])]),
+            "q95": np.concatenate([[zq * v_scale], np.full(dim, qx[0.95])])}
 
 
 def _funnel_standardised(draws):
@@ -243,13 +268,11 @@ def bench_hier_nuts(args) -> int:
                           max_tree_depth=10, seed=args.seed)
     wall = time.perf_counter() - t0
     log(f"bench: sampler done warmup {res.warmup_seconds:.3f}s sampling {res.sampling_seconds:.3f}s wall {wall:.3f}s")
-    rh, eb = D.rhat_and_ess_bulk(res.constrained_samples)
-    min_ess = float(np.nanmin(eb))
-    rhat_max = float(np.nanmax(rh))
+    st = D.harness_stats(res.constrained_samples)
+    min_ess = float(np.nanmin(st["ess_bulk"]))
+    rhat_max = float(np.nanmax(st["rhat"]))
     raw = min_ess / res.sampling_seconds
-    value = gated(raw, rhat_max)
-    log(f"bench: rhat max {rhat_max:.4f} min bulk ESS {min_ess:.1f} -> {raw:.1f} ESS/s "
-        f"({'gated in' if value is not None else 'R-hat gate failed: value null'})")
+    log(f"bench: rhat max {rhat_max:.4f} min bulk ESS {min_ess:.1f} -> {raw:.1f} ESS/s")
 
     evals = res.sampling_gradient_evals
     flops = evals * (FLOPS_PER_GRADIENT + FLOPS_PER_LEAF_EXTRA)
@@ -284,8 +307,8 @@ def bench_hier_nuts(args) -> int:
         orc = om.run("nuts", "f64", args.chains, num_samples=args.steps, num_warmup=args.warmup, max_tree_depth=10,
                      seed=args.seed, threads=cores)
         t_full = time.perf_counter() - t0
-        crh, ceb = D.rhat_and_ess_bulk(model.constrain(orc["samples"]))
-        c_ess, c_rhat = float(np.nanmin(ceb)), float(np.nanmax(crh))
+        cst = D.harness_stats(model.constrain(orc["samples"]))
+        c_ess, c_rhat = float(np.nanmin(cst["ess_bulk"])), float(np.nanmax(cst["rhat"]))
         c_raw = c_ess / orc["sampling_seconds"]
         # one thread: a bounded sample of sampling iterations from the GPU run's adapted state
         pos = res.unconstrained_samples[:, -1, :].astype(np.float64).T.copy()
@@ -310,9 +333,18 @@ def bench_hier_nuts(args) -> int:
         log(f"bench: cpu {cores} threads: warmup {orc['warmup_seconds']:.1f}s sampling {orc['sampling_seconds']:.1f}s "
             f"rhat {c_rhat:.4f} -> {c_raw:.1f} ESS/s; 1 thread {s1:.3f} s/iter")
 
+    # the harness gate: R-hat, and mean / q05 / q95 within 4 combined MCSE of the reference
+    # path's run (the CPU oracle's full run) when it was made
+    ok, fails, zmax = D.harness_gate(st, cst if cpu is not None else None, RHAT_MAX, Z_MAX)
+    value = raw if ok else None
+    log(f"bench: gate {'passed' if ok else 'failed: ' + '; '.join(fails[:3])} (max z vs the CPU run {zmax:.2f})")
+    gate = {"passed": ok, "max_z_vs_cpu_run": zmax if cpu is not None else None, "failures": fails[:8],
+            "rule": f"rank R-hat < {RHAT_MAX}" + (f"; mean, q05, q95 within {Z_MAX} combined MCSE of the CPU "
+                                                 f"reference run" if cpu is not None else "")
+                    + " (analyze.py:76-97; ArviZ 0.23.4 estimators)"}
     line = {
         "metric": METRIC, "value": value, "unit": "ESS/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": 1e3 * res.sampling_seconds / args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * res.sampling_seconds / args.steps, "gate": gate,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (Philox4x32-10, seed %d)" % args.seed,
         "config": {"workload": "hierarchical normal non-centred J=64x32, batched_nuts persistent, 1024 chains, "
@@ -342,21 +374,32 @@ def bench_funnel_flow(args) -> int:
     dim, L, H, layers = 100, 10, 256, 6
     model = ut.NealsFunnel(dim, 3.0)
 
+    exact = funnel_exact_stats(dim, 3.0)
+
     def summarise(res, label):
-        rh, eb = D.rhat_and_ess_bulk(res.constrained_samples)
+        draws = res.constrained_samples
+        st = D.harness_stats(draws)
+        ok, fails, zmax = D.harness_gate(st, exact, RHAT_MAX, Z_MAX)
+        eb = st["ess_bulk"]
         raw = float(np.nanmin(eb)) / res.sampling_seconds
-        rmax = float(np.nanmax(rh))
-        out = {"min_bulk_ess": float(np.nanmin(eb)), "rhat_max": rmax, "rhat_ok": rmax < RHAT_MAX,
+        rmax = float(np.nanmax(st["rhat"]))
+        _, eb_jl = D.rhat_and_ess_bulk(draws)
+        out = {"min_bulk_ess": float(np.nanmin(eb)), "min_tail_ess": float(np.nanmin(st["ess_tail"])),
+               "rhat_max": rmax, "rhat_ok": rmax < RHAT_MAX,
+               "gate": {"passed": ok, "max_z_vs_exact": zmax, "failures": fails[:8], "num_failures": len(fails),
+                        "rule": f"rank R-hat < {RHAT_MAX} and mean, q05, q95 of every parameter within {Z_MAX} "
+                                f"MCSE of the funnel's exact law (analyze.py:76-97; ArviZ 0.23.4 estimators)"},
+               "min_bulk_ess_diagnostics_jl": float(np.nanmin(eb_jl)),
                "ess_bulk_v": float(eb[0]), "sampling_seconds": res.sampling_seconds,
-               "warmup_seconds": res.warmup_seconds, "value": gated(raw, rmax), "ungated_value": raw,
+               "warmup_seconds": res.warmup_seconds, "value": raw if ok else None, "ungated_value": raw,
                "num_samples": res.num_samples,
                "gradient_evals_per_s": res.sampling_gradient_evals / res.sampling_seconds,
                "divergence_rate": res.divergence_rate(), "step_size": float(np.median(res.step_size)),
                "v_mean": float(res.unconstrained_samples[:, :, 0].mean()),
                "v_sd": float(res.unconstrained_samples[:, :, 0].std())}
-        log(f"bench: {label}: min ESS {out['min_bulk_ess']:.1f} rhat {rmax:.4f} -> {raw:.1f} ESS/s "
-            f"({'gated in' if out['value'] is not None else 'fails the R-hat gate'}); v mean {out['v_mean']:.3f} "
-            f"sd {out['v_sd']:.3f}; div {out['divergence_rate']:.4f}")
+        log(f"bench: {label}: min bulk ESS {out['min_bulk_ess']:.1f} rhat {rmax:.4f} max z {zmax:.2f} -> "
+            f"{raw:.1f} ESS/s ({'gated in' if ok else 'fails the gate: ' + '; '.join(fails[:3])}); v mean "
+            f"{out['v_mean']:.3f} sd {out['v_sd']:.3f}; div {out['divergence_rate']:.4f}")
         return out
 
     log(f"bench: funnel D={dim} plain batched_hmc chains={chains} warmup={W_plain} draws={TRAIN_DRAWS}")
@@ -582,7 +625,7 @@ def bench_gp_meads(args) -> int:
     init[2] += np.log(0.674)
 
     def summarise(res, label):
-        rh, eb = D.rhat_and_ess_bulk(res.constrained_samples)
+        rh, eb = harness_rhat_ess(D, res.constrained_samples)
         out = {"min_bulk_ess": float(np.nanmin(eb)), "rhat_max": float(np.nanmax(rh)),
                "ess_bulk_hyper": [float(v) for v in eb[:3]], "sampling_seconds": res.sampling_seconds,
                "warmup_seconds": res.warmup_seconds, "value": float(np.nanmin(eb)) / res.sampling_seconds,
@@ -686,7 +729,7 @@ def bench_glm_chees(args) -> int:
                            forward_velocity=args.forward_velocity)
     log(f"bench: sampler done warmup {res.warmup_seconds:.3f}s sampling {res.sampling_seconds:.3f}s "
         f"T {res.trajectory_trace[-1] if warmup else 0:.4f} eps {res.step_size[0]:.5f}")
-    rh, eb = D.rhat_and_ess_bulk(res.constrained_samples)
+    rh, eb = harness_rhat_ess(D, res.constrained_samples)
     min_ess, rhat_max = float(np.nanmin(eb)), float(np.nanmax(rh))
     raw = min_ess / res.sampling_seconds
     value = gated(raw, rhat_max)
@@ -751,7 +794,7 @@ def bench_glm_chees(args) -> int:
         log(f"bench: flow trained in {t_train:.2f}s, loss {trace[:20].mean():.2f} -> {trace[-100:].mean():.2f}")
         fres = ut.batched_chees(model, num_chains=chains, num_samples=steps, num_warmup=warmup, seed=args.seed + 1,
                                 forward_velocity=args.forward_velocity, flow=flow)
-        frh, feb = D.rhat_and_ess_bulk(fres.constrained_samples)
+        frh, feb = harness_rhat_ess(D, fres.constrained_samples)
         fmin = float(np.nanmin(feb))
         fval_raw = fmin / fres.sampling_seconds
         fval = gated(fval_raw, float(np.nanmax(frh)))

@@ -189,3 +189,249 @@ def mcse_mean(draws) -> np.ndarray:
     d = _t(draws)
     sd = d.reshape((-1,) + tuple(d.shape[2:])).std(dim=0, unbiased=True)
     return _np(sd) / np.sqrt(ess(draws))
+
+
+# --------------------------------------------------------------------------------------
+# The cross-PPL harness's statistics (bench/crossppl/python/analyze.py:51-97), which call
+# ArviZ 0.23.4 (the version bench/crossppl/python/uv.lock pins).  Restated from ArviZ's
+# published estimators (Vehtari et al. 2021, arviz/stats/diagnostics.py), which differ
+# from the reference's diagnostics.jl in four places: ties get average ranks; a split
+# drops the middle draw of an odd-length chain; the autocorrelation sum stops at lag
+# n - 3, adds the even term of the first non-positive pair, and is not clipped to 1; and
+# ESS may exceed the draw count (tau >= 1 / log10(C S) instead of tau >= 1).
+
+def _arviz_split(d: torch.Tensor) -> torch.Tensor:
+    half = d.shape[1] // 2
+    return torch.cat([d[:, :half], d[:, d.shape[1] - half:]], dim=0)
+
+
+def _average_ranks(flat: torch.Tensor) -> torch.Tensor:
+    """Ranks 1..S of each column with ties given their average rank (scipy 'average')."""
+    S = flat.shape[0]
+    srt, order = torch.sort(flat, dim=0, stable=True)
+    out = torch.empty_like(flat)
+    pos = torch.arange(1, S + 1, device=flat.device, dtype=flat.dtype)
+    for j in range(flat.shape[1]):
+        col = srt[:, j]
+        new = torch.ones(S, dtype=torch.bool, device=flat.device)
+        new[1:] = col[1:] != col[:-1]
+        gid = torch.cumsum(new.to(torch.int64), dim=0) - 1
+        ng = int(gid[-1].item()) + 1
+        first = torch.zeros(ng, dtype=flat.dtype, device=flat.device).scatter_reduce(
+            0, gid, pos, reduce="amin", include_self=False)
+        last = torch.zeros(ng, dtype=flat.dtype, device=flat.device).scatter_reduce(
+            0, gid, pos, reduce="amax", include_self=False)
+        out[order[:, j], j] = (0.5 * (first + last))[gid]
+    return out
+
+
+def _arviz_z_scale(sp: torch.Tensor) -> torch.Tensor:
+    shp = sp.shape
+    flat = sp.reshape(shp[0] * shp[1], -1)
+    S = flat.shape[0]
+    r = (_average_ranks(flat) - 0.375) / (S + 0.25)
+    return (np.sqrt(2.0) * torch.special.erfinv(2.0 * r - 1.0)).reshape(shp)
+
+
+def _arviz_ess(sp: torch.Tensor) -> np.ndarray:
+    """ArviZ `_ess` of (chains, draws, params) columns."""
+    m, n = sp.shape[0], sp.shape[1]
+    total = m * n
+    flat = sp.reshape(m, n, -1)
+    P = flat.shape[2]
+    rng = (flat.amax(dim=(0, 1)) - flat.amin(dim=(0, 1)))
+    x = flat - flat.mean(dim=1, keepdim=True)
+    nfft = 1 << int(np.ceil(np.log2(2 * n)))
+    f = torch.fft.rfft(x, n=nfft, dim=1)
+    acov = (torch.fft.irfft(f * torch.conj(f), n=nfft, dim=1)[:, :n] / n).mean(dim=0)  # (n, P)
+    mean_var = acov[0] * n / (n - 1.0)
+    var_plus = mean_var * (n - 1.0) / n
+    if m > 1:
+        var_plus = var_plus + flat.mean(dim=1).var(dim=0, unbiased=True)
+    rho_all = _np(1.0 - (mean_var.unsqueeze(0) - acov) / var_plus.unsqueeze(0))
+    rng = _np(rng)
+    out = np.empty(P)
+    for j in range(P):
+        if not np.isfinite(rng[j]):
+            out[j] = np.nan
+            continue
+        if rng[j] < np.finfo(float).resolution:
+            out[j] = float(total)
+            continue
+        rho = np.zeros(n)
+        rho_all_j = rho_all[:, j]
+        even, odd = 1.0, rho_all_j[1] if n > 1 else 0.0
+        rho[0] = even
+        if n > 1:
+            rho[1] = odd
+        t = 1
+        while t < n - 3 and (even + odd) > 0.0:
+            even, odd = rho_all_j[t + 1], rho_all_j[t + 2]
+            if even + odd >= 0:
+                rho[t + 1], rho[t + 2] = even, odd
+            t += 2
+        max_t = t - 2
+        if even > 0:
+            rho[max_t + 1] = even
+        t = 1
+        while t <= max_t - 2:
+            if rho[t + 1] + rho[t + 2] > rho[t - 1] + rho[t]:
+                rho[t + 1] = (rho[t - 1] + rho[t]) / 2.0
+                rho[t + 2] = rho[t + 1]
+            t += 2
+        tau = -1.0 + 2.0 * np.sum(rho[:max_t + 1]) + np.sum(rho[max_t + 1:max_t + 2])
+        tau = max(tau, 1.0 / np.log10(total))
+        out[j] = np.nan if np.isnan(rho).any() else total / tau
+    return out
+
+
+def _arviz_cols(draws) -> torch.Tensor:
+    d = _t(draws)
+    return d.unsqueeze(-1) if d.ndim == 2 else d
+
+
+def _squeeze(draws, v):
+    return v[0] if np.asarray(draws).ndim == 2 else v
+
+
+@_per_param_chunks
+def arviz_ess_bulk(draws) -> np.ndarray:
+    """ArviZ bulk ESS (az.ess(method='bulk')) per parameter of (C, S[, P]) draws."""
+    return _squeeze(draws, _arviz_ess(_arviz_z_scale(_arviz_split(_arviz_cols(draws)))))
+
+
+@_per_param_chunks
+def arviz_ess_mean(draws) -> np.ndarray:
+    return _squeeze(draws, _arviz_ess(_arviz_split(_arviz_cols(draws))))
+
+
+def _quantile_cols(flat: torch.Tensor, q: float) -> torch.Tensor:
+    # numpy's default (linear) quantile, column-wise
+    srt = torch.sort(flat, dim=0).values
+    pos = q * (flat.shape[0] - 1)
+    lo = int(np.floor(pos))
+    hi = min(lo + 1, flat.shape[0] - 1)
+    return srt[lo] + (pos - lo) * (srt[hi] - srt[lo])
+
+
+@_per_param_chunks
+def arviz_ess_quantile(draws, prob: float) -> np.ndarray:
+    d = _arviz_cols(draws)
+    qv = _quantile_cols(d.reshape(-1, d.shape[2]), prob)
+    return _squeeze(draws, _arviz_ess(_arviz_split((d <= qv).to(torch.float64))))
+
+
+def arviz_ess_tail(draws) -> np.ndarray:
+    """ArviZ tail ESS (az.ess(method='tail')): min of the 5 % and 95 % quantile-indicator ESS."""
+    return np.minimum(arviz_ess_quantile(draws, 0.05), arviz_ess_quantile(draws, 0.95))
+
+
+def _arviz_rhat_cols(sp: torch.Tensor) -> torch.Tensor:
+    n = sp.shape[1]
+    chain_mean = sp.mean(dim=1)
+    between = n * chain_mean.var(dim=0, unbiased=True)
+    within = sp.var(dim=1, unbiased=True).mean(dim=0)
+    return torch.sqrt((between / within + n - 1) / n)
+
+
+@_per_param_chunks
+def arviz_rhat(draws) -> np.ndarray:
+    """ArviZ rank R-hat (az.rhat(method='rank')): max of the bulk and folded split R-hat."""
+    d = _arviz_cols(draws)
+    bulk = _arviz_rhat_cols(_arviz_z_scale(_arviz_split(d)))
+    flat = d.reshape(-1, d.shape[2])
+    srt = torch.sort(flat, dim=0).values
+    S = flat.shape[0]
+    med = 0.5 * (srt[(S - 1) // 2] + srt[S // 2])
+    tail = _arviz_rhat_cols(_arviz_z_scale(_arviz_split(torch.abs(d - med))))
+    return _squeeze(draws, _np(torch.maximum(bulk, tail)))
+
+
+@_per_param_chunks
+def arviz_mcse_mean(draws) -> np.ndarray:
+    d = _arviz_cols(draws)
+    sd = _np(d.reshape(-1, d.shape[2]).std(dim=0, unbiased=True))
+    return _squeeze(draws, sd / np.sqrt(_arviz_ess(_arviz_split(d))))
+
+
+@_per_param_chunks
+def arviz_mcse_quantile(draws, prob: float) -> np.ndarray:
+    """ArviZ quantile MCSE: half the width of the order-statistic interval whose Beta
+    posterior (ESS of the quantile indicator) covers +-1 sd around `prob`."""
+    from scipy import stats
+    d = _arviz_cols(draws)
+    ess_q = np.atleast_1d(arviz_ess_quantile(draws, prob))
+    srt = _np(torch.sort(d.reshape(-1, d.shape[2]), dim=0).values)
+    size = srt.shape[0]
+    out = np.empty(srt.shape[1])
+    for j in range(srt.shape[1]):
+        a = stats.beta.ppf([0.1586553, 0.8413447], ess_q[j] * prob + 1, ess_q[j] * (1 - prob) + 1)
+        th1 = srt[int(np.floor(np.nanmax((a[0] * size, 0)))), j]
+        th2 = srt[int(np.ceil(np.nanmin((a[1] * size, size - 1)))), j]
+        out[j] = (th2 - th1) / 2
+    return _squeeze(draws, out)
+
+
+@_per_param_chunks
+def _harness_arrays(draws):
+    d = _arviz_cols(draws)
+    P = d.shape[2]
+    flat = d.reshape(-1, P)
+    size = flat.shape[0]
+    srt = torch.sort(flat, dim=0).values
+
+    def quant(q):
+        pos = q * (size - 1)
+        lo = int(np.floor(pos))
+        return srt[lo] + (pos - lo) * (srt[min(lo + 1, size - 1)] - srt[lo])
+    q05, q95 = quant(0.05), quant(0.95)
+    med = 0.5 * (srt[(size - 1) // 2] + srt[size // 2])
+    sp = _arviz_split(d)
+    z = _arviz_z_scale(sp)
+    ess_bulk = _arviz_ess(z)
+    r_bulk = _arviz_rhat_cols(z)
+    del z
+    r_tail = _arviz_rhat_cols(_arviz_z_scale(_arviz_split(torch.abs(d - med))))
+    ess_mean = _arviz_ess(sp)
+    ess_q = [_arviz_ess(_arviz_split((d <= q).to(torch.float64))) for q in (q05, q95)]
+    sd = _np(flat.std(dim=0, unbiased=True))
+    srt_np = _np(srt)
+    from scipy import stats
+    mcse_q = []
+    for prob, e in zip((0.05, 0.95), ess_q):
+        m = np.empty(P)
+        for j in range(P):
+            a = stats.beta.ppf([0.1586553, 0.8413447], e[j] * prob + 1, e[j] * (1 - prob) + 1)
+            th1 = srt_np[int(np.floor(np.nanmax((a[0] * size, 0)))), j]
+            th2 = srt_np[int(np.ceil(np.nanmin((a[1] * size, size - 1)))), j]
+            m[j] = (th2 - th1) / 2
+        mcse_q.append(m)
+    return (_np(flat.mean(dim=0)), sd, _np(q05), _np(q95), _np(torch.maximum(r_bulk, r_tail)), ess_bulk,
+            np.minimum(ess_q[0], ess_q[1]), sd / np.sqrt(ess_mean), mcse_q[0], mcse_q[1])
+
+
+def harness_stats(draws) -> dict:
+    """analyze.py:51-72 per parameter of (C, S, P) draws: mean, sd, 5 % / 95 % quantiles,
+    rank R-hat, bulk / tail ESS and the MCSE of the mean and of both quantiles (one sort
+    and one rank transform shared per parameter chunk)."""
+    keys = ("mean", "sd", "q05", "q95", "rhat", "ess_bulk", "ess_tail", "mcse_mean", "mcse_q05", "mcse_q95")
+    return dict(zip(keys, _harness_arrays(np.asarray(draws))))
+
+
+def harness_gate(stats: dict, ref: dict | None, rhat_max: float = 1.01, z_max: float = 4.0):
+    """analyze.py:76-97: every rank R-hat below `rhat_max`, and (against a reference's
+    statistics) mean, q05 and q95 within `z_max` combined MCSE.  A reference with zero MCSE
+    (exact values) gives z against the run's own MCSE.  Returns (passed, failures, max z)."""
+    failures = []
+    for p in np.nonzero(~(np.asarray(stats["rhat"]) < rhat_max))[0]:
+        failures.append(f"param {p}: rhat={float(stats['rhat'][p]):.4f}")
+    zmax = 0.0
+    if ref is not None:
+        for key, mkey in (("mean", "mcse_mean"), ("q05", "mcse_q05"), ("q95", "mcse_q95")):
+            denom = np.sqrt(np.asarray(stats[mkey]) ** 2 + np.asarray(ref.get(mkey, 0.0)) ** 2)
+            z = np.where(denom > 0, np.abs(np.asarray(stats[key]) - np.asarray(ref[key])) / np.where(denom > 0, denom, 1),
+                         0.0)
+            zmax = max(zmax, float(np.nanmax(z)))
+            for p in np.nonzero(z >= z_max)[0]:
+                failures.append(f"param {p}: {key} z={float(z[p]):.1f}")
+    return not failures, failures, zmax
