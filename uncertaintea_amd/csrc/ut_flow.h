@@ -30,6 +30,12 @@ struct FlowLayer {
   uint16_t* W1Tb = nullptr;  // 128 x H
   uint16_t* W2b = nullptr;   // 128 x H
   uint16_t* W2Tb = nullptr;  // H x 128
+  // the same four matrices in the all-layer kernels' fragment order (the B-operand
+  // pieces of one wavefront contiguous), filled when the layer's widths are the fused ones
+  uint16_t* W1f = nullptr;
+  uint16_t* W2f = nullptr;
+  uint16_t* W2Tf = nullptr;
+  uint16_t* W1Tf = nullptr;
 };
 
 // Device descriptors of the layers served by the LDS-staged kernels (rebuilt whenever
@@ -43,6 +49,8 @@ struct FusedFwdLayer {
   float* s;   // B x na
   float* xs;  // B x na
   int np, na, par;
+  const uint16_t* W1f;  // W1b / W2b in fragment order (the all-layer kernels)
+  const uint16_t* W2f;
 };
 struct FusedBwdLayer {
   const uint16_t* W2Tb;
@@ -51,6 +59,8 @@ struct FusedBwdLayer {
   const float* s;
   const float* xs;
   int np, na, par;
+  const uint16_t* W2Tf;  // W2Tb / W1Tb in fragment order
+  const uint16_t* W1Tf;
 };
 
 // Per-batch activations kept for the backward pass.

@@ -523,25 +523,289 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
   layer_bwd(g, P, ly, nullptr, c, v1, v2, lsh);
 }
 
+// ---- all-layer kernels: rows resident in LDS, weights in registers -------------------
+// The sampling path runs the whole flow in one launch per direction.  A workgroup keeps
+// its rows (fp32, row stride P) in LDS from the first layer to the last, so a layer
+// reads and writes no row through global memory, and each wavefront holds the B operand
+// of exactly the MFMA tiles it computes in registers, loaded straight from global (L2),
+// so no weight passes through LDS.  Each matrix's fragments for the next layer are
+// requested as soon as this layer's GEMM with that matrix is done (half a layer ahead,
+// one register set per matrix); the LDS operands of each step are loaded as one batch
+// before its arithmetic, so a wavefront waits on LDS once per batch.  The arithmetic, its
+// order and the activations written for the backward are those of layer_fwd / layer_bwd
+// (and so of the per-step kernels), bit for bit.
+
+// Workgroup barrier for LDS traffic only: this wavefront's LDS operations complete, then
+// s_barrier.  __syncthreads() also makes global memory coherent across the workgroup,
+// which waits for every global load and store in flight (the next layer's weights, the
+// activations just written); the resident kernels exchange data between wavefronts only
+// through LDS, so their global traffic keeps flying across the barriers.
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+// v_i <- sum over the wavefront in the xor-butterfly order (partners i ^ 32, i ^ 16, ...,
+// i ^ 1) for RW values at once, every lane ending with the same bits.  The 32 and 16
+// steps exchange through ds_bpermute; from there on a lane's value depends only on i mod
+// 16 (then mod 8, ...), so a rotation by 8 (then 4) inside the 16-lane DPP row reads the
+// same value as lane i ^ 8 (i ^ 4), and quad permutations give i ^ 2 and i ^ 1 exactly.
+template <int CTRL>
+__device__ inline float flow_dpp(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), CTRL, 0xF, 0xF, true));
+}
+template <int N>
+__device__ inline void xor_sum_rows(float (&v)[N]) {
+#pragma unroll
+  for (int m = 32; m >= 16; m >>= 1)
+#pragma unroll
+    for (int i = 0; i < N; ++i) v[i] += __shfl_xor(v[i], m);
+#pragma unroll
+  for (int i = 0; i < N; ++i) v[i] += flow_dpp<0x128>(v[i]);  // row_ror:8
+#pragma unroll
+  for (int i = 0; i < N; ++i) v[i] += flow_dpp<0x124>(v[i]);  // row_ror:4
+#pragma unroll
+  for (int i = 0; i < N; ++i) v[i] += flow_dpp<0x4E>(v[i]);   // quad_perm [2,3,0,1]
+#pragma unroll
+  for (int i = 0; i < N; ++i) v[i] += flow_dpp<0xB1>(v[i]);   // quad_perm [1,0,3,2]
+}
+
+constexpr int KF1 = LW / 16;  // 16-deep k-steps of the first GEMM (K = LW)
+constexpr int KF2 = LH / 16;  // and of the second (K = LH)
+// A workgroup serves RL = 32 or 16 rows.  The MFMA tiles stay 32 rows tall either way
+// (so every accumulator sums k exactly as the per-step kernels do); with RL = 16 the
+// A-operand rows 16..31 hold whatever the LDS holds and the result rows they produce are
+// never read (row i of a product depends on row i of A only), and each thread keeps the
+// first NR = RL / 2 accumulator registers of a tile, the ones holding rows < RL.  Sixteen
+// rows per workgroup put 4096 rows on 256 workgroups, one per CU.
+constexpr int FLOW_MAX_FUSED_P = LW + LW / 2;  // the fused layers' widths bound P (np <= 128, na <= 64)
+template <int RL>
+inline size_t res_lds_bytes(int P) {
+  return (size_t)RL * P * sizeof(float) + (size_t)LR * LHP * sizeof(bf16_t) + (size_t)RL * OSP * sizeof(float);
+}
+// the fused kernels' activation fragment order (hfrag) for RL-row workgroups
+template <int RL>
+__device__ inline long long hfrag_rl(int blk, int wave, int t, int rq, int lane) {
+  return ((((long long)blk * 4 + wave) * 2 + t) * (RL / 8) + rq) * 256 + lane * 4;
+}
+
+// B-operand fragments for this wavefront, 16-byte K-contiguous pieces (row l32 of the
+// tile, k = 16 kk + 8 half .. + 7) of a K-contiguous weight matrix: the first GEMM's two
+// 32-column tiles 32 wave + 128 t (K = LW, row stride LW) and the second GEMM's one tile
+// 32 wave (K = LH, row stride LH)
+// The matrices are read from copies in fragment order (k_frag_order1 / 2), where the
+// pieces of one load instruction -- one (wavefront, tile, k-step) for all 64 lanes -- are
+// 1 KiB contiguous: fully coalesced loads.
+__device__ inline long long frag_index1(int wave, int t, int kk, int lane) {
+  return ((((long long)wave * 2 + t) * KF1 + kk) * 64 + lane) * 8;
+}
+__device__ inline long long frag_index2(int wave, int kk, int lane) {
+  return (((long long)wave * KF2 + kk) * 64 + lane) * 8;
+}
+__device__ inline void frag_load1(const bf16_t* m, bf16x8 (&f)[2][KF1]) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const UT_GLOBAL bf16_t* g = glb(m);
+#pragma unroll
+  for (int t = 0; t < 2; ++t)
+#pragma unroll
+    for (int kk = 0; kk < KF1; ++kk) f[t][kk] = *(const UT_GLOBAL bf16x8*)(g + frag_index1(wave, t, kk, lane));
+}
+__device__ inline void frag_load2(const bf16_t* m, bf16x8 (&f)[1][KF2]) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const UT_GLOBAL bf16_t* g = glb(m);
+#pragma unroll
+  for (int kk = 0; kk < KF2; ++kk) f[0][kk] = *(const UT_GLOBAL bf16x8*)(g + frag_index2(wave, kk, lane));
+}
+
+// fragment-order copies: piece (wave, t, kk, lane) of a 256 x 128 matrix (row stride 128)
+// is row 32 wave + 128 t + lane % 32, columns 16 kk + 8 (lane / 32) .. + 7; of a 128 x 256
+// matrix (row stride 256) row 32 wave + lane % 32, the same columns
+__global__ void k_frag_order1(const bf16_t* __restrict__ src, bf16_t* __restrict__ dst) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;  // one 8-element piece
+  if (p >= LH * LW / 8) return;
+  const int lane = p & 63, kk = (p >> 6) % KF1, t = ((p >> 6) / KF1) & 1, wave = ((p >> 6) / KF1) >> 1;
+  const bf16_t* s = src + (long long)(32 * wave + 128 * t + (lane & 31)) * LW + 16 * kk + 8 * (lane >> 5);
+#pragma unroll
+  for (int e = 0; e < 8; ++e) dst[(long long)p * 8 + e] = s[e];
+}
+__global__ void k_frag_order2(const bf16_t* __restrict__ src, bf16_t* __restrict__ dst) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= LW * LH / 8) return;
+  const int lane = p & 63, kk = (p >> 6) % KF2, wave = (p >> 6) / KF2;
+  const bf16_t* s = src + (long long)(32 * wave + (lane & 31)) * LH + 16 * kk + 8 * (lane >> 5);
+#pragma unroll
+  for (int e = 0; e < 8; ++e) dst[(long long)p * 8 + e] = s[e];
+}
+
+// acc[t] = A[32 x 16 KS] (LDS, row stride lda) * (tile t of the wavefront)^T, k in the
+// order of lds_gemm; the A fragments are read in batches of 8
+template <int NT, int KS>
+__device__ inline void reg_gemm(const bf16_t* As, int lda, const bf16x8 (&B)[NT][KS], f32x16 (&acc)[NT]) {
+  const int lane = threadIdx.x & 63, l32 = lane & 31, half = lane >> 5;
+#pragma unroll
+  for (int t = 0; t < NT; ++t)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[t][r] = 0.0f;
+#pragma unroll
+  for (int k0 = 0; k0 < KS; k0 += 8) {
+    bf16x8 a[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) a[j] = *(const bf16x8*)(As + l32 * lda + 16 * (k0 + j) + 8 * half);
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+#pragma unroll
+      for (int t = 0; t < NT; ++t) acc[t] = mfma32(a[j], B[t][k0 + j], acc[t]);
+  }
+}
+
+// the forward layer's biases for this thread
+struct FwdBias {
+  float b1[2], b2s, b2t;
+};
+__device__ inline void fbias_load(const FusedFwdLayer& ly, FwdBias& b) {
+  const int lane = threadIdx.x & 63, l32 = lane & 31, wave = threadIdx.x >> 6;
+  const UT_GLOBAL float* b1 = glb(ly.b1);
+  const UT_GLOBAL float* b2 = glb(ly.b2);
+#pragma unroll
+  for (int t = 0; t < 2; ++t) b.b1[t] = b1[32 * wave + 128 * t + l32];
+  const bool act = lane < ly.na;
+  b.b2s = act ? b2[lane] : 0.0f;
+  b.b2t = act ? b2[ly.na + lane] : 0.0f;
+}
+
+// One forward layer on the resident rows Xs (RL x P); ldv[rr] is the log-determinant of
+// row wave * RW + rr.  W1 / W2 / bs hold this layer's operands on entry and, when `next`
+// is given, the next layer's on exit (requested as each is consumed).  With `gathered`
+// R already holds the layer's bf16 passive inputs: the masks alternate, so the next
+// layer's passive coordinates are this layer's active ones (np of the next layer = na),
+// and the coupling writes them into R as it produces them.
+template <int RL>
+__device__ __forceinline__ void layer_fwd_res(float* Xs, int P, const FusedFwdLayer& ly, const FusedFwdLayer& nx,
+                                              bool next, bool gathered, bf16x8 (&W1)[2][KF1], bf16x8 (&W2)[1][KF2],
+                                              FwdBias& bs, float c, float (&ldv)[RL / 4], bf16_t* R, float* Os,
+                                              long long r0) {
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, l32 = lane & 31;
+  constexpr int G = RL * LW / 256;
+  constexpr int RW = RL / 4;
+  constexpr int NR = RL / 2;
+  const int np = ly.np, na = ly.na, par = ly.par;
+  UT_GLOBAL float* hout = glb(ly.h);
+  UT_GLOBAL float* sout = glb(ly.s);
+  UT_GLOBAL float* xsout = glb(ly.xs);
+  const bool act = lane < na;
+  const int kq = tid & (LW - 1);
+  const int pk = 2 * kq + par;
+  const int ak = 2 * lane + 1 - par;
+  if (!gathered) {
+    float xv[G];
+#pragma unroll
+    for (int i = 0; i < G; ++i) {
+      const int r = (tid + 256 * i) / LW;
+      xv[i] = Xs[r * P + min(pk, P - 1)];  // read in bounds for every lane, selected below
+    }
+#pragma unroll
+    for (int i = 0; i < G; ++i) {
+      const int r = (tid + 256 * i) / LW;
+      R[r * LWP + kq] = kq < np ? f2bf(xv[i]) : (bf16_t)0;
+    }
+    lds_barrier();
+  }
+  f32x16 acc[2];
+  reg_gemm<2, KF1>(R, LWP, W1, acc);
+  const FwdBias cb = bs;
+  __builtin_amdgcn_sched_barrier(0);
+  if (next) {
+    frag_load1(nx.W1f, W1);
+    fbias_load(nx, bs);
+  }
+  lds_barrier();  // every wavefront is done with the passive inputs; R now takes h
+#pragma unroll
+  for (int t = 0; t < 2; ++t)
+#pragma unroll
+    for (int rq = 0; rq < NR / 4; ++rq) {
+      f32x4 hv;
+#pragma unroll
+      for (int r3 = 0; r3 < 4; ++r3) {
+        const int r = 4 * rq + r3;
+        const int row = mfma32_row(lane, r), col = 32 * wave + 128 * t + l32;
+        const float v = flow_tanh(acc[t][r] + cb.b1[t]);
+        hv[r3] = v;
+        R[row * LHP + col] = f2bf(v);
+      }
+      *(UT_GLOBAL f32x4*)(hout + hfrag_rl<RL>(blockIdx.x, wave, t, rq, lane)) = hv;
+    }
+  lds_barrier();
+  f32x16 acc2[1];
+  reg_gemm<1, KF2>(R, LHP, W2, acc2);
+  __builtin_amdgcn_sched_barrier(0);
+  if (next) frag_load2(nx.W2f, W2);
+#pragma unroll
+  for (int r = 0; r < NR; ++r) Os[mfma32_row(lane, r) * OSP + 32 * wave + l32] = acc2[0][r];
+  lds_barrier();
+  float os[RW], ot[RW], xa[RW];
+#pragma unroll
+  for (int rr = 0; rr < RW; ++rr) {
+    const int row = wave * RW + rr;
+    os[rr] = Os[row * OSP + lane];  // lane + na < OSP: in bounds for every lane
+    ot[rr] = Os[row * OSP + na + lane];
+    xa[rr] = Xs[row * P + min(ak, P - 1)];
+  }
+  float sum[RW], sv[RW], xs[RW];
+#pragma unroll
+  for (int rr = 0; rr < RW; ++rr) {
+    sv[rr] = c * flow_tanh((os[rr] + cb.b2s) * (1.0f / c));
+    xs[rr] = xa[rr] * expf(sv[rr]);
+    sum[rr] = 0.0f + (act ? sv[rr] : 0.0f);
+  }
+#pragma unroll
+  for (int rr = 0; rr < RW; ++rr) {
+    const int row = wave * RW + rr;
+    const long long b = r0 + row;
+    const float xn = xs[rr] + (ot[rr] + cb.b2t);
+    if (act) {
+      Xs[row * P + ak] = xn;
+      sout[b * na + lane] = sv[rr];
+      xsout[b * na + lane] = xs[rr];
+    }
+    if (next) {  // the next layer's passive input j = lane (zero beyond its np = na)
+      R[row * LWP + lane] = act ? f2bf(xn) : (bf16_t)0;
+      R[row * LWP + 64 + lane] = (bf16_t)0;
+    }
+  }
+  xor_sum_rows<RW>(sum);
+#pragma unroll
+  for (int rr = 0; rr < RW; ++rr) ldv[rr] = ldv[rr] + sum[rr];
+  lds_barrier();  // the active coordinates are written before the next layer gathers
+}
+
 // The whole forward in one launch: the elementwise base (x = mu + e^ls z and the
-// log-determinant start sum ls, in the order of k_base_fwd / k_logdet_init), then all
-// L coupling layers; a workgroup carries its rows through every layer (rows are
-// independent), with the next layer's weights in flight during each layer.
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_flow_fwd_all(
-    const float* __restrict__ z, const float* __restrict__ prm, float* __restrict__ x, int P,
-    const FusedFwdLayer* __restrict__ layers, int L, float c, float* __restrict__ logdet) {
-  extern __shared__ __attribute__((aligned(16))) bf16_t lsh[];
-  u32x4 v1[NSTAGE], v2[NSTAGE];
-  load_weights(layers[0].W1b, layers[0].W2b, v1, v2);
-  const long long r0 = (long long)blockIdx.x * LR;
+// log-determinant start sum ls, in the order of k_base_fwd / k_logdet_init), then all L
+// coupling layers on the resident rows; the rows go to x (and q when given), the
+// log-determinants to logdet (and logdet_out when given).
+// LT > 0 fixes the layer count at compile time (L == LT) and unrolls the layers, so the
+// waits for the operands requested half a layer ahead are counted exactly; LT == 0 runs
+// any L in a loop, whose back edge has to wait for every request in flight.
+template <int RL, int LT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RL == 8 ? 2 : 1, RL == 8 ? 2 : 1))) void k_flow_fwd_all(
+    const float* __restrict__ z, const float* __restrict__ prm, float* __restrict__ x, float* __restrict__ q, int P,
+    const FusedFwdLayer* __restrict__ layers, int L_, float c, float* __restrict__ logdet,
+    float* __restrict__ logdet_out) {
+  const int L = LT > 0 ? LT : L_;
+  extern __shared__ __attribute__((aligned(16))) float lsf[];
+  float* Xs = lsf;
+  bf16_t* R = (bf16_t*)(lsf + RL * P);
+  float* Os = (float*)(R + LR * LHP);
+  constexpr int RW = RL / 4;
+  bf16x8 W1[2][KF1], W2[1][KF2];
+  FwdBias bs;
+  frag_load1(layers[0].W1f, W1);
+  frag_load2(layers[0].W2f, W2);
+  fbias_load(layers[0], bs);
+  const long long r0 = (long long)blockIdx.x * RL;
   const int tid = threadIdx.x;
-  // loads in batches of four elements per thread (a plain loop would wait on each)
-  for (int e0 = tid; e0 < LR * P; e0 += 4 * 256) {
+  for (int e0 = tid; e0 < RL * P; e0 += 4 * 256) {
     float zv[4], mv[4], lv[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       const int e = e0 + 256 * k;
-      const bool in = e < LR * P;
+      const bool in = e < RL * P;
       const int cidx = in ? e % P : 0;
       zv[k] = in ? z[r0 * P + e] : 0.0f;
       mv[k] = prm[cidx];
@@ -550,11 +814,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       const int e = e0 + 256 * k;
-      if (e < LR * P) x[r0 * P + e] = mv[k] + expf(lv[k]) * zv[k];
+      if (e < RL * P) Xs[e] = mv[k] + expf(lv[k]) * zv[k];
     }
   }
-  if ((tid & 63) == 0) {  // lane 0 of each wavefront: the log-determinant start of its rows
-    // sequential sum (the order of k_logdet_init), loads eight at a time
+  float ldv[RW];
+  {
+    // the log-determinant start (every lane computes it; lane 0's is stored), sequential
+    // sum in the order of k_logdet_init, loads eight at a time
     float sl = 0.0f;
     int i = 0;
     for (; i + 8 <= P; i += 8) {
@@ -565,46 +831,219 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
       for (int k = 0; k < 8; ++k) sl += t[k];
     }
     for (; i < P; ++i) sl += prm[P + i];
-    for (int rr = 0; rr < LR / 4; ++rr) logdet[r0 + (tid >> 6) * (LR / 4) + rr] = sl;
+#pragma unroll
+    for (int rr = 0; rr < RW; ++rr) ldv[rr] = sl;
   }
-  __syncthreads();  // the base rows are written
-  for (int l = 0; l < L; ++l) {
+  lds_barrier();  // the base rows are in LDS
+#pragma unroll
+  for (int l = 0; l < (LT > 0 ? LT : L); ++l) {
+    // both descriptors through scalar loads: the next layer's pointers and widths are
+    // then uniform registers, so requesting its operands waits on no vector memory
     const FusedFwdLayer ly = layers[l];
-    layer_fwd(x, P, ly, l + 1 < L ? &layers[l + 1] : nullptr, c, logdet, v1, v2, lsh);
-    __syncthreads();  // the layer's rows and log-determinants are written; the LDS is free
+    const FusedFwdLayer nx = layers[l + 1 < L ? l + 1 : l];
+    layer_fwd_res<RL>(Xs, P, ly, nx, l + 1 < L, l > 0, W1, W2, bs, c, ldv, R, Os, r0);
+  }
+  for (int e = tid; e < RL * P; e += 256) {
+    const float v = Xs[e];
+    x[r0 * P + e] = v;
+    if (q) q[r0 * P + e] = v;
+  }
+  if ((tid & 63) == 0)
+#pragma unroll
+    for (int rr = 0; rr < RW; ++rr) {
+      const long long b = r0 + (tid >> 6) * RW + rr;
+      logdet[b] = ldv[rr];
+      if (logdet_out) logdet_out[b] = ldv[rr];
+    }
+}
+
+// What one backward layer reads besides its weights, for this thread: the scales and
+// scaled inputs of its rows, and the tanh activations of its accumulator positions
+template <int RL>
+struct BwdRows {
+  float s[RL / 4], xs[RL / 4];
+};
+template <int RL>
+__device__ inline void brows_load(const FusedBwdLayer& ly, long long r0, BwdRows<RL>& a) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  constexpr int RW = RL / 4;
+  const bool act = lane < ly.na;
+  const UT_GLOBAL float* sin_ = glb(ly.s);
+  const UT_GLOBAL float* xsin = glb(ly.xs);
+#pragma unroll
+  for (int rr = 0; rr < RW; ++rr) {
+    const long long b = r0 + wave * RW + rr;
+    a.s[rr] = act ? sin_[b * ly.na + lane] : 0.0f;
+    a.xs[rr] = act ? xsin[b * ly.na + lane] : 0.0f;
+  }
+}
+template <int RL>
+__device__ inline void bh_load(const FusedBwdLayer& ly, f32x4 (&h)[2][RL / 8]) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const UT_GLOBAL float* hin = glb(ly.h);
+#pragma unroll
+  for (int t = 0; t < 2; ++t)
+#pragma unroll
+    for (int rq = 0; rq < RL / 8; ++rq)
+      h[t][rq] = *(const UT_GLOBAL f32x4*)(hin + hfrag_rl<RL>(blockIdx.x, wave, t, rq, lane));
+}
+
+// One backward layer on the resident gradient rows Gs (RL x P); the operands are
+// replaced by the next (lower) layer's as they are consumed, as in layer_fwd_res.
+template <int RL>
+__device__ __forceinline__ void layer_bwd_res(float* Gs, int P, const FusedBwdLayer& ly, const FusedBwdLayer& nx,
+                                              bool next, bf16x8 (&W2T)[2][KF1], bf16x8 (&W1T)[1][KF2],
+                                              BwdRows<RL>& av, f32x4 (&h)[2][RL / 8], float c, bf16_t* R,
+                                              long long r0) {
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, l32 = lane & 31;
+  constexpr int RW = RL / 4;
+  constexpr int NR = RL / 2;
+  const int np = ly.np, na = ly.na, par = ly.par;
+  const bool act = lane < na;
+  const int pcol = 32 * wave + l32;
+  const int ak = 2 * lane + 1 - par;
+  const int pk = 2 * pcol + par;
+  float gv[RW];
+#pragma unroll
+  for (int rr = 0; rr < RW; ++rr) gv[rr] = Gs[(wave * RW + rr) * P + min(ak, P - 1)];  // selected by act below
+  const BwdRows<RL> cr = av;
+  __builtin_amdgcn_sched_barrier(0);
+  if (next) brows_load<RL>(nx, r0, av);
+#pragma unroll
+  for (int rr = 0; rr < RW; ++rr) {
+    const int row = wave * RW + rr;
+    if (act) {
+      const float ga = gv[rr];
+      const float gs = ga * cr.xs[rr] + 1.0f;
+      const float r = cr.s[rr] * (1.0f / c);
+      const float gos = gs * (1.0f - r * r);
+      R[row * LWP + lane] = f2bf(gos);
+      R[row * LWP + na + lane] = f2bf(ga);
+      Gs[row * P + ak] = ga * expf(cr.s[rr]);
+    }
+    for (int j = 2 * na + lane; j < LW; j += 64) R[row * LWP + j] = (bf16_t)0;
+  }
+  lds_barrier();
+  f32x16 acc[2];
+  reg_gemm<2, KF1>(R, LWP, W2T, acc);
+  __builtin_amdgcn_sched_barrier(0);
+  if (next) frag_load1(nx.W2Tf, W2T);
+  lds_barrier();  // every wavefront is done with dO; R now takes dU
+#pragma unroll
+  for (int t = 0; t < 2; ++t)
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      const int row = mfma32_row(lane, r), col = 32 * wave + 128 * t + l32;
+      const float hv = h[t][r / 4][r % 4];
+      R[row * LHP + col] = f2bf(acc[t][r] * (1.0f - hv * hv));
+    }
+  __builtin_amdgcn_sched_barrier(0);
+  if (next) bh_load<RL>(nx, h);
+  lds_barrier();
+  f32x16 acc2[1];
+  reg_gemm<1, KF2>(R, LHP, W1T, acc2);
+  __builtin_amdgcn_sched_barrier(0);
+  if (next) frag_load2(nx.W1Tf, W1T);
+  {
+    float gp[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) gp[r] = Gs[mfma32_row(lane, r) * P + min(pk, P - 1)];
+    if (pcol < np)
+#pragma unroll
+      for (int r = 0; r < NR; ++r) Gs[mfma32_row(lane, r) * P + pk] = gp[r] + acc2[0][r];
+  }
+  lds_barrier();  // the passive gradients are written; R is free
+}
+
+// The whole backward in one launch: the gradient rows g_q enter LDS once, all L coupling
+// layers run in reverse on them, then the base's g_z = g e^ls (as k_base_fwd_bwd) is
+// written straight to the output (g_z may alias g_q: a workgroup reads its rows before
+// it writes them).
+template <int RL, int LT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RL == 8 ? 2 : 1, RL == 8 ? 2 : 1))) void k_flow_bwd_all(
+    const float* g_q, const float* __restrict__ prm, float* gz, int P, const FusedBwdLayer* __restrict__ layers,
+    int L_, float c) {
+  const int L = LT > 0 ? LT : L_;
+  extern __shared__ __attribute__((aligned(16))) float lsf[];
+  float* Gs = lsf;
+  bf16_t* R = (bf16_t*)(lsf + RL * P);
+  const long long r0 = (long long)blockIdx.x * RL;
+  const int tid = threadIdx.x;
+  bf16x8 W2T[2][KF1], W1T[1][KF2];
+  BwdRows<RL> av;
+  f32x4 h[2][RL / 8];
+  frag_load1(layers[L - 1].W2Tf, W2T);
+  frag_load2(layers[L - 1].W1Tf, W1T);
+  brows_load<RL>(layers[L - 1], r0, av);
+  bh_load<RL>(layers[L - 1], h);
+  for (int e = tid; e < RL * P; e += 256) Gs[e] = g_q[r0 * P + e];
+  lds_barrier();
+#pragma unroll
+  for (int l = (LT > 0 ? LT : L) - 1; l >= 0; --l) {
+    const FusedBwdLayer ly = layers[l];
+    const FusedBwdLayer nx = layers[l > 0 ? l - 1 : l];
+    layer_bwd_res<RL>(Gs, P, ly, nx, l > 0, W2T, W1T, av, h, c, R, r0);
+  }
+  for (int e0 = tid; e0 < RL * P; e0 += 4 * 256) {
+    float lv[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int e = e0 + 256 * k;
+      lv[k] = prm[P + (e < RL * P ? e % P : 0)];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int e = e0 + 256 * k;
+      if (e < RL * P) gz[r0 * P + e] = Gs[e] * expf(lv[k]);
+    }
   }
 }
 
-// The whole backward in one launch: all L coupling layers in reverse, then the base's
-// g_z = g e^ls (as k_base_fwd_bwd) written straight to the output.
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_flow_bwd_all(
-    float* __restrict__ g, const float* __restrict__ prm, float* gz, int P,
-    const FusedBwdLayer* __restrict__ layers, int L, float c) {
-  extern __shared__ __attribute__((aligned(16))) bf16_t lsh[];
-  u32x4 v1[NSTAGE], v2[NSTAGE];
-  load_weights(layers[L - 1].W2Tb, layers[L - 1].W1Tb, v1, v2);
-  for (int l = L - 1; l >= 0; --l) {
-    const FusedBwdLayer ly = layers[l];
-    layer_bwd(g, P, ly, l > 0 ? &layers[l - 1] : nullptr, c, v1, v2, lsh);
-    __syncthreads();  // the layer's gradients are written; the LDS is free
-  }
-  const long long r0 = (long long)blockIdx.x * LR;
-  for (int e0 = threadIdx.x; e0 < LR * P; e0 += 4 * 256) {
-    float gv[4], lv[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const int e = e0 + 256 * k;
-      const bool in = e < LR * P;
-      gv[k] = in ? g[r0 * P + e] : 0.0f;
-      lv[k] = prm[P + (in ? e % P : 0)];
-    }
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const int e = e0 + 256 * k;
-      if (e < LR * P) gz[r0 * P + e] = gv[k] * expf(lv[k]);
-    }
+// rows per workgroup of the all-layer kernels: 16 while 32 would leave CUs idle (fewer
+// than 256 workgroups), else 32 (half the weight traffic per row)
+static int resident_rows(int B) {
+  static const int forced = [] {
+    const char* e = std::getenv("UT_FLOW_ROWS");  // 8, 16 or 32 rows per workgroup (comparison runs)
+    const int v = e ? std::atoi(e) : 0;
+    return (v == 8 || v == 16 || v == 32) ? v : 0;
+  }();
+  if (forced) return forced;
+  return B / 32 < 256 ? 16 : 32;
+}
+
+// the all-layer kernel instances: layer counts 2, 4, 6 and 8 unrolled, others in a loop
+typedef void (*FwdAllFn)(const float*, const float*, float*, float*, int, const FusedFwdLayer*, int, float, float*,
+                         float*);
+typedef void (*BwdAllFn)(const float*, const float*, float*, int, const FusedBwdLayer*, int, float);
+template <int RL>
+static FwdAllFn fwd_all_rl(int L) {
+  switch (L) {
+    case 2: return k_flow_fwd_all<RL, 2>;
+    case 4: return k_flow_fwd_all<RL, 4>;
+    case 6: return k_flow_fwd_all<RL, 6>;
+    case 8: return k_flow_fwd_all<RL, 8>;
+    default: return k_flow_fwd_all<RL, 0>;
   }
 }
+template <int RL>
+static BwdAllFn bwd_all_rl(int L) {
+  switch (L) {
+    case 2: return k_flow_bwd_all<RL, 2>;
+    case 4: return k_flow_bwd_all<RL, 4>;
+    case 6: return k_flow_bwd_all<RL, 6>;
+    case 8: return k_flow_bwd_all<RL, 8>;
+    default: return k_flow_bwd_all<RL, 0>;
+  }
+}
+static FwdAllFn fwd_all_kernel(int rl, int L) {
+  return rl == 8 ? fwd_all_rl<8>(L) : rl == 16 ? fwd_all_rl<16>(L) : fwd_all_rl<32>(L);
+}
+static BwdAllFn bwd_all_kernel(int rl, int L) {
+  return rl == 8 ? bwd_all_rl<8>(L) : rl == 16 ? bwd_all_rl<16>(L) : bwd_all_rl<32>(L);
+}
+template <int RL>
+static size_t res_lds(int P) { return res_lds_bytes<RL>(P); }
+static size_t res_lds_rl(int rl, int P) { return rl == 8 ? res_lds<8>(P) : rl == 16 ? res_lds<16>(P) : res_lds<32>(P); }
 
 // column sums in two passes (fixed order): COLSUM_CHUNKS row chunks, then chunks
 constexpr int COLSUM_CHUNKS = 256;  // 256 workgroups in the first pass (16 rows each at 4096 rows)
@@ -995,10 +1434,14 @@ hipError_t flow_alloc(Flow& f, int P, int L, int H, bool factor) {
     if (!lds_set) {
       (void)hipFuncSetAttribute((const void*)k_layer_fwd, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LAYER_LDS);
       (void)hipFuncSetAttribute((const void*)k_layer_bwd, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LAYER_LDS);
-      (void)hipFuncSetAttribute((const void*)k_flow_fwd_all, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)LAYER_LDS);
-      (void)hipFuncSetAttribute((const void*)k_flow_bwd_all, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)LAYER_LDS);
+      for (int rl : {8, 16, 32})
+        for (int l : {0, 2, 4, 6, 8}) {
+          const int bytes = (int)res_lds_rl(rl, FLOW_MAX_FUSED_P);
+          (void)hipFuncSetAttribute((const void*)fwd_all_kernel(rl, l), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    bytes);
+          (void)hipFuncSetAttribute((const void*)bwd_all_kernel(rl, l), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    bytes);
+        }
       lds_set = true;
     }
   }
@@ -1046,6 +1489,10 @@ hipError_t flow_alloc(Flow& f, int P, int L, int H, bool factor) {
     if (e == hipSuccess) e = own(f, &ly.W1Tb, (size_t)H * kp);
     if (e == hipSuccess) e = own(f, &ly.W2b, (size_t)H * op);
     if (e == hipSuccess) e = own(f, &ly.W2Tb, (size_t)H * op);
+    if (e == hipSuccess) e = own(f, &ly.W1f, (size_t)H * kp);
+    if (e == hipSuccess) e = own(f, &ly.W2f, (size_t)H * op);
+    if (e == hipSuccess) e = own(f, &ly.W2Tf, (size_t)H * op);
+    if (e == hipSuccess) e = own(f, &ly.W1Tf, (size_t)H * kp);
   }
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&f.stream, hipStreamNonBlocking);
   return e;
@@ -1104,8 +1551,9 @@ hipError_t flow_scratch(Flow& f, int B) {
     std::vector<FusedBwdLayer> bw(f.L);
     for (int l = 0; l < f.L; ++l) {
       const FlowLayer& ly = f.layers[l];
-      fw[l] = {ly.W1b, f.params + ly.ob1, ly.W2b, f.params + ly.ob2, s.h[l], s.s[l], s.xa[l], ly.np, ly.na, l & 1};
-      bw[l] = {ly.W2Tb, ly.W1Tb, s.h[l], s.s[l], s.xa[l], ly.np, ly.na, l & 1};
+      fw[l] = {ly.W1b, f.params + ly.ob1, ly.W2b, f.params + ly.ob2, s.h[l], s.s[l], s.xa[l], ly.np, ly.na, l & 1,
+               ly.W1f, ly.W2f};
+      bw[l] = {ly.W2Tb, ly.W1Tb, s.h[l], s.s[l], s.xa[l], ly.np, ly.na, l & 1, ly.W2Tf, ly.W1Tf};
     }
     e = hipMemcpy(f.dfwd, fw.data(), fw.size() * sizeof(FusedFwdLayer), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(f.dbwd, bw.data(), bw.size() * sizeof(FusedBwdLayer), hipMemcpyHostToDevice);
@@ -1114,12 +1562,20 @@ hipError_t flow_scratch(Flow& f, int B) {
 }
 
 hipError_t flow_sync_weights(Flow& f, hipStream_t st) {
-  for (FlowLayer& ly : f.layers) {
+  for (int l = 0; l < f.L; ++l) {
+    FlowLayer& ly = f.layers[l];
     const int kp = round_up(ly.np, FLOW_PAD), op = round_up(2 * ly.na, FLOW_PAD);
     hipLaunchKernelGGL(k_weights_bf16, dim3(nblk((long long)f.H * kp)), dim3(T256), 0, st, f.params + ly.oW1, f.H,
                        kp, ly.W1b, ly.W1Tb);
     hipLaunchKernelGGL(k_weights_bf16, dim3(nblk((long long)op * f.H)), dim3(T256), 0, st, f.params + ly.oW2, op,
                        f.H, ly.W2b, ly.W2Tb);
+    if (f.H == LH && kp == LW && op == LW) {
+      const unsigned nb = nblk((long long)LH * LW / 8);
+      hipLaunchKernelGGL(k_frag_order1, dim3(nb), dim3(T256), 0, st, ly.W1b, ly.W1f);
+      hipLaunchKernelGGL(k_frag_order2, dim3(nb), dim3(T256), 0, st, ly.W2b, ly.W2f);
+      hipLaunchKernelGGL(k_frag_order1, dim3(nb), dim3(T256), 0, st, ly.W2Tb, ly.W2Tf);
+      hipLaunchKernelGGL(k_frag_order2, dim3(nb), dim3(T256), 0, st, ly.W1Tb, ly.W1Tf);
+    }
   }
   return hipGetLastError();
 }
@@ -1130,6 +1586,7 @@ static void colsum(const float* M, int B, int n, float* out, float* part, hipStr
 }
 
 // the LDS-staged layer kernels serve H = 256 with passive and output widths of 128
+
 static bool fused_layer(const Flow& f, int l) {
   const FlowLayer& ly = f.layers[l];
   return f.fused && f.H == LH && round_up(ly.np, FLOW_PAD) == LW && round_up(2 * ly.na, FLOW_PAD) == LW;
@@ -1187,8 +1644,14 @@ static hipError_t forward_impl(Flow& f, const float* z, float* q, float* logdet,
     all = all && s.fwd_fused[l];
   }
   if (all) {
-    hipLaunchKernelGGL(k_flow_fwd_all, dim3(B / LR), dim3(T256), LAYER_LDS, st, z, f.params, s.x, f.P, f.dfwd, f.L,
-                       f.clamp, s.logdet);
+    // the rows go to s.x and the log-determinants to s.logdet (read by the model gradient
+    // and the log-density) and also straight to q / logdet when given
+    float* q2 = (q && q != s.x) ? q : nullptr;
+    float* l2 = (logdet && logdet != s.logdet) ? logdet : nullptr;
+    const int rl = resident_rows(B);
+    hipLaunchKernelGGL(fwd_all_kernel(rl, f.L), dim3(B / rl), dim3(T256), res_lds_rl(rl, f.P), st, z, f.params, s.x,
+                       q2, f.P, f.dfwd, f.L, f.clamp, s.logdet, l2);
+    return hipGetLastError();
   } else {
     hipLaunchKernelGGL(k_logdet_init, dim3(nblk(B)), dim3(T256), 0, st, f.params + f.P, f.P, 1.0f, s.logdet, B);
     hipLaunchKernelGGL(k_base_fwd, dim3(nblk(n)), dim3(T256), 0, st, z, f.params, f.P, s.x, n);
@@ -1231,18 +1694,20 @@ hipError_t flow_backward(Flow& f, const float* g_q, float* g_z, int B, hipStream
   if (B % FLOW_PAD || B > f.sc.B) return hipErrorInvalidValue;
   FlowScratch& s = f.sc;
   const long long n = (long long)B * f.P;
+  bool all = f.L > 0;
+  for (int l = 0; l < f.L; ++l) all = all && s.fwd_fused[l];
+  if (all) {
+    // the gradient rows are read from g_q once; base backward fused in, written to g_z
+    const int rl = resident_rows(B);
+    hipLaunchKernelGGL(bwd_all_kernel(rl, f.L), dim3(B / rl), dim3(T256), res_lds_rl(rl, f.P), st, g_q, f.params,
+                       g_z, f.P, f.dbwd, f.L, f.clamp);
+    return hipGetLastError();
+  }
   if (g_q != s.g) {
     hipError_t e = hipMemcpyAsync(s.g, g_q, n * 4, hipMemcpyDeviceToDevice, st);
     if (e != hipSuccess) return e;
   }
-  bool all = f.L > 0;
-  for (int l = 0; l < f.L; ++l) all = all && s.fwd_fused[l];
-  if (all) {
-    // base backward fused in, written straight to g_z
-    hipLaunchKernelGGL(k_flow_bwd_all, dim3(B / LR), dim3(T256), LAYER_LDS, st, s.g, f.params, g_z, f.P, f.dbwd,
-                       f.L, f.clamp);
-    return hipGetLastError();
-  } else {
+  {
     for (int l = f.L - 1; l >= 0; --l) {
       FlowLayer& ly = f.layers[l];
       const int op = round_up(2 * ly.na, FLOW_PAD);

@@ -21,8 +21,13 @@ __device__ inline f32x16 mfma32(const bf16x8& a, const bf16x8& b, const f32x16& 
   return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
 }
 
-// Round-to-nearest-even float -> bfloat16 (NaN kept quiet).
+// Round-to-nearest-even float -> bfloat16 (NaN kept quiet).  On the device this is
+// gfx950's conversion instruction (v_cvt_pk_bf16_f32), which gives these bits for every
+// one of the 2^32 inputs (NaN quieting included); the host restates it.
 __host__ __device__ inline bf16_t f2bf(float f) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_bit_cast(bf16_t, (__bf16)f);
+#endif
   uint32_t u = __builtin_bit_cast(uint32_t, f);
   if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x007fffffu)) return (bf16_t)((u >> 16) | 0x40);
   u += 0x7fffu + ((u >> 16) & 1u);
