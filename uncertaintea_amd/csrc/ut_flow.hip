@@ -999,6 +999,319 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RL == 8 ? 2
   }
 }
 
+// ---- the 8-wavefront variant (16 rows, 512 threads) ----------------------------------
+// Two wavefronts per SIMD hide each other's LDS, MFMA and memory latencies.  Wavefront w
+// owns first-GEMM tile w (columns 32 w .. 32 w + 31 of H = 256: tile (w % 4, w / 4) of the
+// 4-wavefront kernels, so the same fragment-order copies serve) and, for w < 4, the second
+// GEMM's tile w; the coupling runs two rows per wavefront.  Same arithmetic, same order.
+constexpr int W8 = 8;
+constexpr int RL8 = 16;
+constexpr int RW8 = RL8 / W8;  // coupling rows per wavefront
+constexpr int NR8 = RL8 / 2;   // accumulator registers holding rows < 16
+
+// the first GEMM's tile of wavefront w in fragment order (k_frag_order1's layout)
+__device__ inline void frag_load1_w8(const bf16_t* m, bf16x8 (&f)[1][KF1]) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const UT_GLOBAL bf16_t* g = glb(m);
+#pragma unroll
+  for (int kk = 0; kk < KF1; ++kk) f[0][kk] = *(const UT_GLOBAL bf16x8*)(g + frag_index1(w & 3, w >> 2, kk, lane));
+}
+__device__ inline void frag_load2_w8(const bf16_t* m, bf16x8 (&f)[1][KF2]) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (w >= 4) return;
+  const UT_GLOBAL bf16_t* g = glb(m);
+#pragma unroll
+  for (int kk = 0; kk < KF2; ++kk) f[0][kk] = *(const UT_GLOBAL bf16x8*)(g + frag_index2(w, kk, lane));
+}
+// activations of accumulator register r = 4 rq + r3 (rq < 2) of wavefront w, lane
+__device__ inline long long hfrag8(int blk, int w, int rq, int lane) {
+  return ((((long long)blk * W8 + w) * 2 + rq) * 64 + lane) * 4;
+}
+
+struct FwdBias8 {
+  float b1, b2s, b2t;
+};
+__device__ inline void fbias_load8(const FusedFwdLayer& ly, FwdBias8& b) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  b.b1 = glb(ly.b1)[32 * w + (lane & 31)];
+  const bool act = lane < ly.na;
+  b.b2s = act ? glb(ly.b2)[lane] : 0.0f;
+  b.b2t = act ? glb(ly.b2)[ly.na + lane] : 0.0f;
+}
+
+__device__ __forceinline__ void layer_fwd_w8(float* Xs, int P, const FusedFwdLayer& ly, const FusedFwdLayer& nx,
+                                             bool next, bool gathered, bf16x8 (&W1)[1][KF1], bf16x8 (&W2)[1][KF2],
+                                             FwdBias8& bs, float c, float (&ldv)[RW8], bf16_t* R, float* Os,
+                                             long long r0) {
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63, l32 = lane & 31;
+  const int np = ly.np, na = ly.na, par = ly.par;
+  UT_GLOBAL float* hout = glb(ly.h);
+  UT_GLOBAL float* sout = glb(ly.s);
+  UT_GLOBAL float* xsout = glb(ly.xs);
+  const bool act = lane < na;
+  const int ak = 2 * lane + 1 - par;
+  if (!gathered) {
+    constexpr int G = RL8 * LW / 512;
+    const int kq = tid & (LW - 1);
+    const int pk = 2 * kq + par;
+    float xv[G];
+#pragma unroll
+    for (int i = 0; i < G; ++i) xv[i] = Xs[((tid + 512 * i) / LW) * P + min(pk, P - 1)];
+#pragma unroll
+    for (int i = 0; i < G; ++i) R[((tid + 512 * i) / LW) * LWP + kq] = kq < np ? f2bf(xv[i]) : (bf16_t)0;
+    lds_barrier();
+  }
+  f32x16 acc[1];
+  reg_gemm<1, KF1>(R, LWP, W1, acc);
+  const FwdBias8 cb = bs;
+  __builtin_amdgcn_sched_barrier(0);
+  if (next) {
+    frag_load1_w8(nx.W1f, W1);
+    fbias_load8(nx, bs);
+  }
+  lds_barrier();  // every wavefront is done with the passive inputs; R now takes h
+#pragma unroll
+  for (int rq = 0; rq < NR8 / 4; ++rq) {
+    f32x4 hv;
+#pragma unroll
+    for (int r3 = 0; r3 < 4; ++r3) {
+      const int r = 4 * rq + r3;
+      const float v = flow_tanh(acc[0][r] + cb.b1);
+      hv[r3] = v;
+      R[mfma32_row(lane, r) * LHP + 32 * w + l32] = f2bf(v);
+    }
+    *(UT_GLOBAL f32x4*)(hout + hfrag8(blockIdx.x, w, rq, lane)) = hv;
+  }
+  lds_barrier();
+  if (w < 4) {
+    f32x16 acc2[1];
+    reg_gemm<1, KF2>(R, LHP, W2, acc2);
+#pragma unroll
+    for (int r = 0; r < NR8; ++r) Os[mfma32_row(lane, r) * OSP + 32 * w + l32] = acc2[0][r];
+  }
+  __builtin_amdgcn_sched_barrier(0);
+  if (next) frag_load2_w8(nx.W2f, W2);
+  lds_barrier();
+  float os[RW8], ot[RW8], xa[RW8];
+#pragma unroll
+  for (int rr = 0; rr < RW8; ++rr) {
+    const int row = w * RW8 + rr;
+    os[rr] = Os[row * OSP + lane];
+    ot[rr] = Os[row * OSP + na + lane];
+    xa[rr] = Xs[row * P + min(ak, P - 1)];
+  }
+  float sum[RW8];
+#pragma unroll
+  for (int rr = 0; rr < RW8; ++rr) {
+    const int row = w * RW8 + rr;
+    const long long b = r0 + row;
+    const float sv = c * flow_tanh((os[rr] + cb.b2s) * (1.0f / c));
+    const float xs = xa[rr] * expf(sv);
+    const float xn = xs + (ot[rr] + cb.b2t);
+    sum[rr] = 0.0f + (act ? sv : 0.0f);
+    if (act) {
+      Xs[row * P + ak] = xn;
+      sout[b * na + lane] = sv;
+      xsout[b * na + lane] = xs;
+    }
+    if (next) {
+      R[row * LWP + lane] = act ? f2bf(xn) : (bf16_t)0;
+      R[row * LWP + 64 + lane] = (bf16_t)0;
+    }
+  }
+  xor_sum_rows<RW8>(sum);
+#pragma unroll
+  for (int rr = 0; rr < RW8; ++rr) ldv[rr] = ldv[rr] + sum[rr];
+  lds_barrier();
+}
+
+template <int LT>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_flow_fwd_w8(
+    const float* __restrict__ z, const float* __restrict__ prm, float* __restrict__ x, float* __restrict__ q, int P,
+    const FusedFwdLayer* __restrict__ layers, int L_, float c, float* __restrict__ logdet,
+    float* __restrict__ logdet_out) {
+  const int L = LT > 0 ? LT : L_;
+  extern __shared__ __attribute__((aligned(16))) float lsf[];
+  float* Xs = lsf;
+  bf16_t* R = (bf16_t*)(lsf + RL8 * P);
+  float* Os = (float*)(R + LR * LHP);
+  bf16x8 W1[1][KF1], W2[1][KF2];
+  FwdBias8 bs;
+  frag_load1_w8(layers[0].W1f, W1);
+  frag_load2_w8(layers[0].W2f, W2);
+  fbias_load8(layers[0], bs);
+  const long long r0 = (long long)blockIdx.x * RL8;
+  const int tid = threadIdx.x;
+  for (int e0 = tid; e0 < RL8 * P; e0 += 4 * 512) {
+    float zv[4], mv[4], lv[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int e = e0 + 512 * k;
+      const bool in = e < RL8 * P;
+      const int cidx = in ? e % P : 0;
+      zv[k] = in ? z[r0 * P + e] : 0.0f;
+      mv[k] = prm[cidx];
+      lv[k] = prm[P + cidx];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int e = e0 + 512 * k;
+      if (e < RL8 * P) Xs[e] = mv[k] + expf(lv[k]) * zv[k];
+    }
+  }
+  float ldv[RW8];
+  {
+    float sl = 0.0f;
+    int i = 0;
+    for (; i + 8 <= P; i += 8) {
+      float t[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) t[k] = prm[P + i + k];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) sl += t[k];
+    }
+    for (; i < P; ++i) sl += prm[P + i];
+#pragma unroll
+    for (int rr = 0; rr < RW8; ++rr) ldv[rr] = sl;
+  }
+  lds_barrier();
+#pragma unroll
+  for (int l = 0; l < (LT > 0 ? LT : L); ++l) {
+    const FusedFwdLayer ly = layers[l];
+    const FusedFwdLayer nx = layers[l + 1 < L ? l + 1 : l];
+    layer_fwd_w8(Xs, P, ly, nx, l + 1 < L, l > 0, W1, W2, bs, c, ldv, R, Os, r0);
+  }
+  for (int e = tid; e < RL8 * P; e += 512) {
+    const float v = Xs[e];
+    x[r0 * P + e] = v;
+    if (q) q[r0 * P + e] = v;
+  }
+  if ((tid & 63) == 0)
+#pragma unroll
+    for (int rr = 0; rr < RW8; ++rr) {
+      const long long b = r0 + (tid >> 6) * RW8 + rr;
+      logdet[b] = ldv[rr];
+      if (logdet_out) logdet_out[b] = ldv[rr];
+    }
+}
+
+struct BwdRows8 {
+  float s[RW8], xs[RW8];
+};
+__device__ inline void brows_load8(const FusedBwdLayer& ly, long long r0, BwdRows8& a) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const bool act = lane < ly.na;
+#pragma unroll
+  for (int rr = 0; rr < RW8; ++rr) {
+    const long long b = r0 + w * RW8 + rr;
+    a.s[rr] = act ? glb(ly.s)[b * ly.na + lane] : 0.0f;
+    a.xs[rr] = act ? glb(ly.xs)[b * ly.na + lane] : 0.0f;
+  }
+}
+__device__ inline void bh_load8(const FusedBwdLayer& ly, f32x4 (&h)[NR8 / 4]) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int rq = 0; rq < NR8 / 4; ++rq) h[rq] = *(const UT_GLOBAL f32x4*)(glb(ly.h) + hfrag8(blockIdx.x, w, rq, lane));
+}
+
+__device__ __forceinline__ void layer_bwd_w8(float* Gs, int P, const FusedBwdLayer& ly, const FusedBwdLayer& nx,
+                                             bool next, bf16x8 (&W2T)[1][KF1], bf16x8 (&W1T)[1][KF2], BwdRows8& av,
+                                             f32x4 (&h)[NR8 / 4], float c, bf16_t* R, long long r0) {
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63, l32 = lane & 31;
+  const int np = ly.np, na = ly.na, par = ly.par;
+  const bool act = lane < na;
+  const int ak = 2 * lane + 1 - par;
+  float gv[RW8];
+#pragma unroll
+  for (int rr = 0; rr < RW8; ++rr) gv[rr] = Gs[(w * RW8 + rr) * P + min(ak, P - 1)];
+  const BwdRows8 cr = av;
+  __builtin_amdgcn_sched_barrier(0);
+  if (next) brows_load8(nx, r0, av);
+#pragma unroll
+  for (int rr = 0; rr < RW8; ++rr) {
+    const int row = w * RW8 + rr;
+    if (act) {
+      const float ga = gv[rr];
+      const float gs = ga * cr.xs[rr] + 1.0f;
+      const float r = cr.s[rr] * (1.0f / c);
+      const float gos = gs * (1.0f - r * r);
+      R[row * LWP + lane] = f2bf(gos);
+      R[row * LWP + na + lane] = f2bf(ga);
+      Gs[row * P + ak] = ga * expf(cr.s[rr]);
+    }
+    for (int j = 2 * na + lane; j < LW; j += 64) R[row * LWP + j] = (bf16_t)0;
+  }
+  lds_barrier();
+  f32x16 acc[1];
+  reg_gemm<1, KF1>(R, LWP, W2T, acc);
+  __builtin_amdgcn_sched_barrier(0);
+  if (next) frag_load1_w8(nx.W2Tf, W2T);
+  lds_barrier();  // every wavefront is done with dO; R now takes dU
+#pragma unroll
+  for (int r = 0; r < NR8; ++r) {
+    const float hv = h[r / 4][r % 4];
+    R[mfma32_row(lane, r) * LHP + 32 * w + l32] = f2bf(acc[0][r] * (1.0f - hv * hv));
+  }
+  __builtin_amdgcn_sched_barrier(0);
+  if (next) bh_load8(nx, h);
+  lds_barrier();
+  if (w < 4) {
+    f32x16 acc2[1];
+    reg_gemm<1, KF2>(R, LHP, W1T, acc2);
+    const int pcol = 32 * w + l32;
+    const int pk = 2 * pcol + par;
+    float gp[NR8];
+#pragma unroll
+    for (int r = 0; r < NR8; ++r) gp[r] = Gs[mfma32_row(lane, r) * P + min(pk, P - 1)];
+    if (pcol < np)
+#pragma unroll
+      for (int r = 0; r < NR8; ++r) Gs[mfma32_row(lane, r) * P + pk] = gp[r] + acc2[0][r];
+  }
+  __builtin_amdgcn_sched_barrier(0);
+  if (next) frag_load2_w8(nx.W1Tf, W1T);
+  lds_barrier();
+}
+
+template <int LT>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_flow_bwd_w8(
+    const float* g_q, const float* __restrict__ prm, float* gz, int P, const FusedBwdLayer* __restrict__ layers,
+    int L_, float c) {
+  const int L = LT > 0 ? LT : L_;
+  extern __shared__ __attribute__((aligned(16))) float lsf[];
+  float* Gs = lsf;
+  bf16_t* R = (bf16_t*)(lsf + RL8 * P);
+  const long long r0 = (long long)blockIdx.x * RL8;
+  const int tid = threadIdx.x;
+  bf16x8 W2T[1][KF1], W1T[1][KF2];
+  BwdRows8 av;
+  f32x4 h[NR8 / 4];
+  frag_load1_w8(layers[L - 1].W2Tf, W2T);
+  frag_load2_w8(layers[L - 1].W1Tf, W1T);
+  brows_load8(layers[L - 1], r0, av);
+  bh_load8(layers[L - 1], h);
+  for (int e = tid; e < RL8 * P; e += 512) Gs[e] = g_q[r0 * P + e];
+  lds_barrier();
+#pragma unroll
+  for (int l = (LT > 0 ? LT : L) - 1; l >= 0; --l) {
+    const FusedBwdLayer ly = layers[l];
+    const FusedBwdLayer nx = layers[l > 0 ? l - 1 : l];
+    layer_bwd_w8(Gs, P, ly, nx, l > 0, W2T, W1T, av, h, c, R, r0);
+  }
+  for (int e0 = tid; e0 < RL8 * P; e0 += 4 * 512) {
+    float lv[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int e = e0 + 512 * k;
+      lv[k] = prm[P + (e < RL8 * P ? e % P : 0)];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int e = e0 + 512 * k;
+      if (e < RL8 * P) gz[r0 * P + e] = Gs[e] * expf(lv[k]);
+    }
+  }
+}
+
 // rows per workgroup of the all-layer kernels: 16 while 32 would leave CUs idle (fewer
 // than 256 workgroups), else 32 (half the weight traffic per row)
 static int resident_rows(int B) {
@@ -1034,6 +1347,32 @@ static BwdAllFn bwd_all_rl(int L) {
     case 8: return k_flow_bwd_all<RL, 8>;
     default: return k_flow_bwd_all<RL, 0>;
   }
+}
+static FwdAllFn fwd_w8_kernel(int L) {
+  switch (L) {
+    case 2: return k_flow_fwd_w8<2>;
+    case 4: return k_flow_fwd_w8<4>;
+    case 6: return k_flow_fwd_w8<6>;
+    case 8: return k_flow_fwd_w8<8>;
+    default: return k_flow_fwd_w8<0>;
+  }
+}
+static BwdAllFn bwd_w8_kernel(int L) {
+  switch (L) {
+    case 2: return k_flow_bwd_w8<2>;
+    case 4: return k_flow_bwd_w8<4>;
+    case 6: return k_flow_bwd_w8<6>;
+    case 8: return k_flow_bwd_w8<8>;
+    default: return k_flow_bwd_w8<0>;
+  }
+}
+// 16-row workgroups run the 8-wavefront kernels unless UT_FLOW_WAVES=4 (comparison runs)
+static bool resident_w8(int rl) {
+  static const bool four = [] {
+    const char* e = std::getenv("UT_FLOW_WAVES");
+    return e && std::strcmp(e, "4") == 0;
+  }();
+  return rl == 16 && !four;
 }
 static FwdAllFn fwd_all_kernel(int rl, int L) {
   return rl == 8 ? fwd_all_rl<8>(L) : rl == 16 ? fwd_all_rl<16>(L) : fwd_all_rl<32>(L);
@@ -1649,8 +1988,9 @@ static hipError_t forward_impl(Flow& f, const float* z, float* q, float* logdet,
     float* q2 = (q && q != s.x) ? q : nullptr;
     float* l2 = (logdet && logdet != s.logdet) ? logdet : nullptr;
     const int rl = resident_rows(B);
-    hipLaunchKernelGGL(fwd_all_kernel(rl, f.L), dim3(B / rl), dim3(T256), res_lds_rl(rl, f.P), st, z, f.params, s.x,
-                       q2, f.P, f.dfwd, f.L, f.clamp, s.logdet, l2);
+    const bool w8 = resident_w8(rl);
+    hipLaunchKernelGGL(w8 ? fwd_w8_kernel(f.L) : fwd_all_kernel(rl, f.L), dim3(B / rl), dim3(w8 ? 512 : T256),
+                       res_lds_rl(rl, f.P), st, z, f.params, s.x, q2, f.P, f.dfwd, f.L, f.clamp, s.logdet, l2);
     return hipGetLastError();
   } else {
     hipLaunchKernelGGL(k_logdet_init, dim3(nblk(B)), dim3(T256), 0, st, f.params + f.P, f.P, 1.0f, s.logdet, B);
@@ -1699,8 +2039,9 @@ hipError_t flow_backward(Flow& f, const float* g_q, float* g_z, int B, hipStream
   if (all) {
     // the gradient rows are read from g_q once; base backward fused in, written to g_z
     const int rl = resident_rows(B);
-    hipLaunchKernelGGL(bwd_all_kernel(rl, f.L), dim3(B / rl), dim3(T256), res_lds_rl(rl, f.P), st, g_q, f.params,
-                       g_z, f.P, f.dbwd, f.L, f.clamp);
+    const bool w8 = resident_w8(rl);
+    hipLaunchKernelGGL(w8 ? bwd_w8_kernel(f.L) : bwd_all_kernel(rl, f.L), dim3(B / rl), dim3(w8 ? 512 : T256),
+                       res_lds_rl(rl, f.P), st, g_q, f.params, g_z, f.P, f.dbwd, f.L, f.clamp);
     return hipGetLastError();
   }
   if (g_q != s.g) {
