@@ -32,9 +32,11 @@ MCSE -- with the harness's ArviZ 0.23.4 estimators (bulk / tail ESS, quantile MC
 The plain run is reported beside it under the same gate.
 
 Also reported: gradient evaluations/s; the roofline of the dominant kernel pair (the
-flow's all-layer forward and backward launches, bf16 MFMA) from HIP-event device time
-over the timed region, with the HBM bytes per launch from the committed counter runs
-of the same 4096-row launch shape; and the CPU baseline -- the oracle's binary64 plain
+flow's all-layer forward and backward launches, bf16 MFMA) from HIP-event device time of
+back-to-back launches at the sampling launch shape (4096 rows) after the flow run, beside
+the sampling run's device time per batched gradient (flow pair + model gradient +
+leapfrog); the HBM bytes per launch when a committed counter record of these kernels
+exists; and the CPU baseline -- the oracle's binary64 plain
 HMC on every host core in the affinity mask (and on one thread), timed on a bounded
 sample of iterations, giving CPU s/iteration and a projected plain ESS/s.
 
@@ -460,21 +462,31 @@ def bench_funnel_flow(args) -> int:
         flops_grad += 2 * 2.0 * (npass * H + H * 2 * nact)
     ksec = fres.sampling_kernel_seconds
     evals = fres.sampling_gradient_evals
-    achieved = evals * flops_grad / ksec / 1e12
-    traffic = None
-    pmc = ROOT / "profiles" / "k_flow_cfg4_pmc.json"
-    if pmc.exists():  # HBM bytes of one forward + one backward launch at 4096 rows (the launch shape timed)
+    # the dominant kernels' own device time: back-to-back launches of the flow's forward
+    # and backward at the sampling launch shape (all chains' rows), HIP events on the
+    # flow's stream
+    fwd_us, bwd_us = flow.kernel_times(chains, reps=200)
+    achieved = chains * flops_grad / ((fwd_us + bwd_us) * 1e-6) / 1e12
+    traffic, traffic_note = None, None
+    pmc = ROOT / "profiles" / "k_flow_w8_cfg4_pmc.json"
+    if pmc.exists():  # HBM bytes of one forward + one backward launch at 4096 rows
         rec = json.loads(pmc.read_text())
-        traffic = sum(rec[k]["traffic_bytes_per_launch"] for k in ("k_flow_fwd_all", "k_flow_bwd_all"))
+        traffic = sum(rec[k]["traffic_bytes_per_launch"] for k in ("k_flow_fwd_w8", "k_flow_bwd_w8"))
+    else:
+        traffic_note = "no FETCH_SIZE / WRITE_SIZE record for these kernels (see the profiles' notes)"
     roofline = {"bound": "mfma", "compute": "bf16 MFMA conditioner GEMMs, fp32 accumulation (latency-bound at "
                                            "4096 rows)",
                 "achieved": achieved, "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
-                "frac": achieved / PEAK_BF16_TFLOPS, "traffic": traffic,
-                "kernel": "k_flow_fwd_all + k_flow_bwd_all (one launch of each per batched gradient of 4096 rows)",
-                "timing": "HIP events around every sampling launch of the flow run on the workspace stream "
-                          "(the per-gradient time includes the model gradient and the lock-step leapfrog kernels)",
-                "us_per_batched_gradient": 1e6 * ksec / max(1, evals // chains),
-                "flops_per_chain_gradient": flops_grad}
+                "frac": achieved / PEAK_BF16_TFLOPS, "traffic": traffic, "traffic_note": traffic_note,
+                "kernel": "k_flow_fwd_w8 + k_flow_bwd_w8 (one launch of each per batched gradient of all chains)",
+                "timing": "HIP events around 200 back-to-back launches of each direction on the flow's stream, "
+                          f"{chains} device-resident rows, after the flow run",
+                "forward_us": fwd_us, "backward_us": bwd_us,
+                "flops_per_launch_pair": chains * flops_grad, "flops_per_chain_gradient": flops_grad,
+                "us_per_batched_gradient_in_sampling": 1e6 * ksec / max(1, evals // chains),
+                "per_gradient_note": "the sampling run's device time per batched gradient (HIP events around the "
+                                     "timed region on the workspace stream): flow pair + model gradient + lock-step "
+                                     "leapfrog kernels"}
     cpu = None
     if not args.no_cpu_baseline:
         from oracle import binding as B

@@ -517,6 +517,12 @@ int ut_flow_forward(ut_flow* flow, const float* z, int64_t n, float* q, float* l
 int ut_flow_inverse(ut_flow* flow, const float* q, int64_t n, float* z, float* logdet_inv);
 /* g_z = (dq/dz)^T g_q + d log|det df/dz| / dz (the sampler's gradient path)       */
 int ut_flow_vjp(ut_flow* flow, const float* z, int64_t n, const float* g_q, float* g_z);
+/* Device time of the sampling path's flow kernels: `reps` back-to-back forward
+ * launches, then `reps` backward launches, over n device-resident rows of N(0, I)
+ * draws, timed with HIP events on the flow's stream; writes the mean microseconds
+ * per launch of each direction.  A measurement utility with no counterpart in the
+ * reference (the benchmark's roofline of the flow kernels).                        */
+int ut_flow_kernel_times(ut_flow* flow, int64_t n, int32_t reps, double* forward_us, double* backward_us);
 /* mean over rows of 0.5|f^-1(q)|^2 - log|det df^-1/dq| + P/2 log 2 pi and its
  * gradient over the flat weights (the training objective)                       */
 int ut_flow_loss_grad(ut_flow* flow, const float* q, int64_t n, double* loss, float* grad);

@@ -49,7 +49,7 @@ EXPORTED = (
     "ut_sampler_opts_default", "ut_nuts_run", "ut_nuts_masked_run", "ut_hmc_run", "ut_chees_opts_default", "ut_chees_run",
     "ut_hmc_shared_run", "ut_meads_opts_default", "ut_meads_run", "ut_flow_create", "ut_flow_create_fullrank", "ut_pointwise_loglik_size", "ut_pointwise_loglik", "ut_flow_destroy", "ut_flow_num_weights", "ut_flow_layer_layout",
     "ut_flow_get_weights", "ut_flow_set_weights", "ut_flow_set_base", "ut_flow_train", "ut_flow_forward",
-    "ut_flow_inverse", "ut_flow_vjp", "ut_flow_loss_grad", "ut_workspace_set_flow", "ut_flow_fit_elbo",
+    "ut_flow_inverse", "ut_flow_vjp", "ut_flow_kernel_times", "ut_flow_loss_grad", "ut_workspace_set_flow", "ut_flow_fit_elbo",
     "ut_flow_elbo_grad", "ut_initial_params",
     "ut_philox4x32", "ut_philox_fill", "ut_host_alloc", "ut_host_free", "ut_gemm_bf16",
 )
@@ -224,6 +224,7 @@ def lib() -> ct.CDLL:
         "ut_flow_forward": (ct.c_int, [vp, vp, i64, vp, vp]),
         "ut_flow_inverse": (ct.c_int, [vp, vp, i64, vp, vp]),
         "ut_flow_vjp": (ct.c_int, [vp, vp, i64, vp, vp]),
+        "ut_flow_kernel_times": (ct.c_int, [vp, i64, i32, vp, vp]),
         "ut_flow_loss_grad": (ct.c_int, [vp, vp, i64, ct.POINTER(dbl), vp]),
         "ut_workspace_set_flow": (ct.c_int, [vp, vp]),
         "ut_flow_fit_elbo": (ct.c_int, [vp, vp, i32, dbl, dbl, dbl, dbl, dbl, i32, ct.c_uint64, vp, vp, vp, vp,

@@ -2248,6 +2248,38 @@ int ut_flow_forward(ut_flow* flow, const float* z, int64_t n, float* q, float* l
   return UT_OK;
 }
 
+int ut_flow_kernel_times(ut_flow* flow, int64_t n, int32_t reps, double* forward_us, double* backward_us) {
+  int rc = flow_rows_check(flow, n, "ut_flow_kernel_times");
+  if (rc) return rc;
+  if (reps < 1 || !forward_us || !backward_us) return fail(UT_E_ARG, "ut_flow_kernel_times: reps >= 1 and outputs");
+  Flow& f = flow->f;
+  UT_HIP(flow_scratch(f, (int)n));
+  DeviceBuffers buf;
+  float *dz = nullptr, *dg = nullptr;
+  UT_HIP(buf.alloc(&dz, (size_t)n * f.P));
+  UT_HIP(buf.alloc(&dg, (size_t)n * f.P));
+  UT_HIP(flow_normal_fill(dz, (int)n, f.P, 0x7f10u, 0u, f.stream));
+  UT_HIP(flow_normal_fill(dg, (int)n, f.P, 0x7f10u, 1u, f.stream));
+  hipEvent_t ev[3];
+  for (auto& e : ev) UT_HIP(hipEventCreate(&e));
+  UT_HIP(flow_forward(f, dz, nullptr, nullptr, (int)n, f.stream));  // warm
+  UT_HIP(flow_backward(f, dg, dg, (int)n, f.stream));
+  UT_HIP(hipEventRecord(ev[0], f.stream));
+  for (int r = 0; r < reps; ++r) UT_HIP(flow_forward(f, dz, nullptr, nullptr, (int)n, f.stream));
+  UT_HIP(hipEventRecord(ev[1], f.stream));
+  // the backward reads the last forward's activations and writes g_z over its input
+  for (int r = 0; r < reps; ++r) UT_HIP(flow_backward(f, dg, dg, (int)n, f.stream));
+  UT_HIP(hipEventRecord(ev[2], f.stream));
+  UT_HIP(hipEventSynchronize(ev[2]));
+  float ms_f = 0.0f, ms_b = 0.0f;
+  UT_HIP(hipEventElapsedTime(&ms_f, ev[0], ev[1]));
+  UT_HIP(hipEventElapsedTime(&ms_b, ev[1], ev[2]));
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  *forward_us = 1e3 * ms_f / reps;
+  *backward_us = 1e3 * ms_b / reps;
+  return UT_OK;
+}
+
 int ut_flow_inverse(ut_flow* flow, const float* q, int64_t n, float* z, float* logdet_inv) {
   int rc = flow_rows_check(flow, n, "ut_flow_inverse");
   if (rc) return rc;

@@ -590,7 +590,7 @@ __device__ inline long long hfrag_rl(int blk, int wave, int t, int rq, int lane)
 // tile, k = 16 kk + 8 half .. + 7) of a K-contiguous weight matrix: the first GEMM's two
 // 32-column tiles 32 wave + 128 t (K = LW, row stride LW) and the second GEMM's one tile
 // 32 wave (K = LH, row stride LH)
-// The matrices are read from copies in fragment order (k_frag_order1 / 2), where the
+// The matrices are read from copies in fragment order (written by k_weights_bf16), where the
 // pieces of one load instruction -- one (wavefront, tile, k-step) for all 64 lanes -- are
 // 1 KiB contiguous: fully coalesced loads.
 __device__ inline long long frag_index1(int wave, int t, int kk, int lane) {
@@ -612,26 +612,6 @@ __device__ inline void frag_load2(const bf16_t* m, bf16x8 (&f)[1][KF2]) {
   const UT_GLOBAL bf16_t* g = glb(m);
 #pragma unroll
   for (int kk = 0; kk < KF2; ++kk) f[0][kk] = *(const UT_GLOBAL bf16x8*)(g + frag_index2(wave, kk, lane));
-}
-
-// fragment-order copies: piece (wave, t, kk, lane) of a 256 x 128 matrix (row stride 128)
-// is row 32 wave + 128 t + lane % 32, columns 16 kk + 8 (lane / 32) .. + 7; of a 128 x 256
-// matrix (row stride 256) row 32 wave + lane % 32, the same columns
-__global__ void k_frag_order1(const bf16_t* __restrict__ src, bf16_t* __restrict__ dst) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;  // one 8-element piece
-  if (p >= LH * LW / 8) return;
-  const int lane = p & 63, kk = (p >> 6) % KF1, t = ((p >> 6) / KF1) & 1, wave = ((p >> 6) / KF1) >> 1;
-  const bf16_t* s = src + (long long)(32 * wave + 128 * t + (lane & 31)) * LW + 16 * kk + 8 * (lane >> 5);
-#pragma unroll
-  for (int e = 0; e < 8; ++e) dst[(long long)p * 8 + e] = s[e];
-}
-__global__ void k_frag_order2(const bf16_t* __restrict__ src, bf16_t* __restrict__ dst) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= LW * LH / 8) return;
-  const int lane = p & 63, kk = (p >> 6) % KF2, wave = (p >> 6) / KF2;
-  const bf16_t* s = src + (long long)(32 * wave + (lane & 31)) * LH + 16 * kk + 8 * (lane >> 5);
-#pragma unroll
-  for (int e = 0; e < 8; ++e) dst[(long long)p * 8 + e] = s[e];
 }
 
 // acc[t] = A[32 x 16 KS] (LDS, row stride lda) * (tile t of the wavefront)^T, k in the
@@ -1009,7 +989,7 @@ constexpr int RL8 = 16;
 constexpr int RW8 = RL8 / W8;  // coupling rows per wavefront
 constexpr int NR8 = RL8 / 2;   // accumulator registers holding rows < 16
 
-// the first GEMM's tile of wavefront w in fragment order (k_frag_order1's layout)
+// the first GEMM's tile of wavefront w in fragment order (frag_index1's layout)
 __device__ inline void frag_load1_w8(const bf16_t* m, bf16x8 (&f)[1][KF1]) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const UT_GLOBAL bf16_t* g = glb(m);
@@ -1415,14 +1395,28 @@ __global__ void k_transpose_bf16(const bf16_t* __restrict__ in, int R, int Cc, b
   for (int k = ty; k < 32; k += 8) out[(long long)(c0 + k) * R + r0 + tx] = tile[tx][k];
 }
 
+// position of element (r, c) of a 256 x 128 (order 1) or 128 x 256 (order 2) matrix in
+// the all-layer kernels' fragment order (frag_index1 / frag_index2 plus the element)
+__device__ inline long long frag_pos(int r, int c, int rows) {
+  const int kk = c / 16, lane = (r & 31) + 32 * ((c & 15) >> 3), e = c & 7;
+  if (rows == LH) return frag_index1((r & 127) >> 5, r >> 7, kk, lane) + e;
+  return frag_index2(r >> 5, kk, lane) + e;
+}
+
+// bf16 copies of the R x Cc fp32 matrix W: row-major, transposed and, when given (the
+// fused widths: one of R, Cc is 256 and the other 128), both in fragment order
 __global__ void k_weights_bf16(const float* __restrict__ W, int R, int Cc, bf16_t* __restrict__ Wb,
-                               bf16_t* __restrict__ WTb) {
+                               bf16_t* __restrict__ WTb, bf16_t* __restrict__ Wf, bf16_t* __restrict__ WTf) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= (long long)R * Cc) return;
   const int r = (int)(i / Cc), c = (int)(i % Cc);
   const bf16_t v = f2bf(W[i]);
   Wb[i] = v;
   WTb[(long long)c * R + r] = v;
+  if (Wf) {
+    Wf[frag_pos(r, c, R)] = v;
+    WTf[frag_pos(c, r, Cc)] = v;
+  }
 }
 
 __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
@@ -1904,17 +1898,11 @@ hipError_t flow_sync_weights(Flow& f, hipStream_t st) {
   for (int l = 0; l < f.L; ++l) {
     FlowLayer& ly = f.layers[l];
     const int kp = round_up(ly.np, FLOW_PAD), op = round_up(2 * ly.na, FLOW_PAD);
+    const bool frag = f.H == LH && kp == LW && op == LW;  // the all-layer kernels' widths
     hipLaunchKernelGGL(k_weights_bf16, dim3(nblk((long long)f.H * kp)), dim3(T256), 0, st, f.params + ly.oW1, f.H,
-                       kp, ly.W1b, ly.W1Tb);
+                       kp, ly.W1b, ly.W1Tb, frag ? ly.W1f : nullptr, frag ? ly.W1Tf : nullptr);
     hipLaunchKernelGGL(k_weights_bf16, dim3(nblk((long long)op * f.H)), dim3(T256), 0, st, f.params + ly.oW2, op,
-                       f.H, ly.W2b, ly.W2Tb);
-    if (f.H == LH && kp == LW && op == LW) {
-      const unsigned nb = nblk((long long)LH * LW / 8);
-      hipLaunchKernelGGL(k_frag_order1, dim3(nb), dim3(T256), 0, st, ly.W1b, ly.W1f);
-      hipLaunchKernelGGL(k_frag_order2, dim3(nb), dim3(T256), 0, st, ly.W2b, ly.W2f);
-      hipLaunchKernelGGL(k_frag_order1, dim3(nb), dim3(T256), 0, st, ly.W2Tb, ly.W2Tf);
-      hipLaunchKernelGGL(k_frag_order2, dim3(nb), dim3(T256), 0, st, ly.W1Tb, ly.W1Tf);
-    }
+                       f.H, ly.W2b, ly.W2Tb, frag ? ly.W2f : nullptr, frag ? ly.W2Tf : nullptr);
   }
   return hipGetLastError();
 }

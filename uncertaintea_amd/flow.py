@@ -180,6 +180,13 @@ class AffineCouplingFlow:
         L.check(L.lib().ut_flow_vjp(self._h, L.ptr(zp), zp.shape[0], L.ptr(gp), L.ptr(gz)))
         return gz[:n]
 
+    def kernel_times(self, rows: int, reps: int = 100):
+        """Mean device microseconds per launch of the sampling path's flow forward and
+        backward over `rows` device-resident rows (HIP events on the flow's stream)."""
+        f, b = ct.c_double(), ct.c_double()
+        L.check(L.lib().ut_flow_kernel_times(self._h, int(rows), int(reps), ct.byref(f), ct.byref(b)))
+        return f.value, b.value
+
     def loss_grad(self, q):
         """(mean negative log-likelihood, its gradient over the flat weights); rows
         must be a multiple of 128."""
