@@ -749,13 +749,32 @@ def bench_glm_chees(args) -> int:
     evals = res.sampling_gradient_evals
     flops = evals * 4.0 * N * Dm
     ksec = res.sampling_kernel_seconds
-    achieved = flops / ksec / 1e12 if ksec > 0 else 0.0
+    sampling_tflops = flops / ksec / 1e12 if ksec > 0 else 0.0
     mean_steps = float(res.integration_steps.mean())
+    # the batch gradient's own device time: HIP events around back-to-back launches of
+    # the samplers' gradient (k_split_bf16 + k_glm_grad + k_glm_reduce) on the workspace
+    # stream, at the run's last draws
+    ws = ut.DeviceBatchedWorkspace(model, chains)
+    last = res.unconstrained_samples[:, -1, :].T
+    grad_ms = {nl: 1e-3 * ws.gradient_kernel_time(last, reps=20, need_logjoint=nl) for nl in (False, True)}
+    del ws
+    fl_batch = 4.0 * N * Dm * chains
+    achieved = fl_batch / (grad_ms[False] * 1e-3) / 1e12
+    traffic = None
+    pmc = ROOT / "profiles" / "k_glm_grad_cfg3_pmc.json"
+    if pmc.exists():
+        traffic = json.loads(pmc.read_text()).get("traffic_bytes_per_launch")
     roofline = {"bound": "mfma", "compute": "bf16 MFMA, fp32 accumulation (hi+lo operand pairs: 2x issued)",
                 "achieved": achieved, "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
-                "frac": achieved / PEAK_BF16_TFLOPS, "traffic": None, "kernel": "k_glm_grad",
-                "timing": "HIP events around all sampling launches (gradient + trajectory kernels)",
-                "flops_per_batch_gradient": 4.0 * N * Dm * chains}
+                "frac": achieved / PEAK_BF16_TFLOPS, "traffic": traffic, "kernel": "k_glm_grad",
+                "timing": "HIP events around 20 back-to-back batch gradients of all chains (k_split_bf16 + "
+                          "k_glm_grad<false> + k_glm_reduce) on the workspace stream, at the run's last draws "
+                          "(ut_gradient_kernel_time)",
+                "ms_per_batch_gradient": grad_ms[False], "ms_per_batch_gradient_with_logjoint": grad_ms[True],
+                "flops_per_batch_gradient": fl_batch,
+                "sampling_tflops": sampling_tflops,
+                "sampling_note": "algorithmic flops of all sampling gradients over the device time of every "
+                                 "sampling launch (gradient + trajectory kernels)"}
     cpu = None
     if not args.no_cpu_baseline:
         from oracle import binding as B

@@ -289,6 +289,14 @@ int ut_workspace_destroy(ut_workspace* ws);
 int ut_logjoint(ut_workspace* ws, const void* params, void* values);
 int ut_logjoint_gradient(ut_workspace* ws, const void* params, void* values,
                          void* gradients);
+/* Device time of the samplers' batched gradient: copies params (P x C, float32) to
+ * the workspace, then times `reps` back-to-back calls of the gradient engine the
+ * samplers use (the plan's kernels, plus the flow when one is attached; with the
+ * log density only when need_logjoint != 0) with HIP events on the workspace's
+ * stream, and writes the mean microseconds per call.  A measurement utility with
+ * no counterpart in the reference (the benchmark's roofline of the gradient).    */
+int ut_gradient_kernel_time(ut_workspace* ws, const void* params, int32_t reps, int32_t need_logjoint,
+                            double* us_per_call);
 
 /* Allocation-free density and gradient.  No counterpart in the reference: this is
  * for callers that cannot allocate device memory (code under HIP graph capture,

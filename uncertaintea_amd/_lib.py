@@ -45,7 +45,8 @@ EXPORTED = (
     "ut_plan_gp_poisson", "ut_gp_factor",
     "ut_plan_destroy", "ut_plan_num_params", "ut_plan_validate", "ut_plan_create", "ut_plan_engine",
     "ut_workspace_create", "ut_workspace_destroy",
-    "ut_logjoint", "ut_logjoint_gradient", "ut_noalloc_scratch_bytes", "ut_logjoint_gradient_noalloc",
+    "ut_logjoint", "ut_logjoint_gradient", "ut_gradient_kernel_time", "ut_noalloc_scratch_bytes",
+    "ut_logjoint_gradient_noalloc",
     "ut_sampler_opts_default", "ut_nuts_run", "ut_nuts_masked_run", "ut_hmc_run", "ut_chees_opts_default", "ut_chees_run",
     "ut_hmc_shared_run", "ut_meads_opts_default", "ut_meads_run", "ut_flow_create", "ut_flow_create_fullrank", "ut_pointwise_loglik_size", "ut_pointwise_loglik", "ut_flow_destroy", "ut_flow_num_weights", "ut_flow_layer_layout",
     "ut_flow_get_weights", "ut_flow_set_weights", "ut_flow_set_base", "ut_flow_train", "ut_flow_forward",
@@ -199,6 +200,7 @@ def lib() -> ct.CDLL:
         "ut_workspace_destroy": (ct.c_int, [vp]),
         "ut_logjoint": (ct.c_int, [vp, vp, vp]),
         "ut_logjoint_gradient": (ct.c_int, [vp, vp, vp, vp]),
+        "ut_gradient_kernel_time": (ct.c_int, [vp, vp, i32, i32, vp]),
         "ut_noalloc_scratch_bytes": (i64, [vp, i32]),
         "ut_logjoint_gradient_noalloc": (ct.c_int, [vp, i32, vp, vp, vp, vp]),
         "ut_sampler_opts_default": (None, [ct.POINTER(SamplerOpts)]),

@@ -296,9 +296,6 @@ int ut_plan_logistic_glm_bf16(const uint16_t* x_bf16, const float* y, int64_t n,
       return fail(UT_E_ARG, "ut_plan_logistic_glm_bf16: y must hold 0/1 outcomes");
   auto* plan = new ut_plan();
   std::vector<uint16_t> X(x_bf16, x_bf16 + (size_t)n * d);
-  std::vector<uint16_t> Xt((size_t)n * d);
-  for (int64_t i = 0; i < n; ++i)
-    for (int j = 0; j < d; ++j) Xt[(size_t)j * n + i] = X[(size_t)i * d + j];
   std::vector<float> y_f(y, y + n);
   DevModelDesc& m = plan->dev;
   m.kind = MODEL_GLM_MFMA;
@@ -315,7 +312,7 @@ int ut_plan_logistic_glm_bf16(const uint16_t* x_bf16, const float* y, int64_t n,
   g.scale = prior_scale;
   g.total_const = m.total_const;
   int rc;
-  if ((rc = plan->stage(&g.X, X)) || (rc = plan->stage(&g.Xt, Xt)) || (rc = plan->stage(&g.y, y_f))) {
+  if ((rc = plan->stage(&g.X, X)) || (rc = plan->stage(&g.y, y_f))) {
     delete plan;
     return rc;
   }
@@ -873,6 +870,31 @@ int ut_logjoint(ut_workspace* ws, const void* params, void* values) {
 int ut_logjoint_gradient(ut_workspace* ws, const void* params, void* values, void* gradients) {
   if (!gradients) return fail(UT_E_ARG, "ut_logjoint_gradient: null gradients");
   return density_call(ws, params, values, gradients);
+}
+
+int ut_gradient_kernel_time(ut_workspace* ws, const void* params, int32_t reps, int32_t need_logjoint,
+                            double* us_per_call) {
+  if (!ws || !params || !us_per_call) return fail(UT_E_ARG, "ut_gradient_kernel_time: null pointer");
+  if (reps < 1) return fail(UT_E_ARG, "ut_gradient_kernel_time: reps must be >= 1");
+  if (ws->precision != UT_FLOAT32) return fail(UT_E_PRECISION, "ut_gradient_kernel_time: float32 workspaces only");
+  const size_t PC = (size_t)ws->P * ws->C;
+  UT_HIP(hipMemcpyAsync(ws->d_params, params, PC * 4, hipMemcpyHostToDevice, ws->stream));
+  const float* q = (const float*)ws->d_params;
+  float* lj = (float*)ws->d_values;
+  float* g = (float*)ws->d_grads;
+  const bool nl = need_logjoint != 0;
+  hipEvent_t ev[2];
+  for (auto& e : ev) UT_HIP(hipEventCreate(&e));
+  UT_HIP(batched_grad(ws, q, lj, g, nl));  // warm (and capture, when graphs are used)
+  UT_HIP(hipEventRecord(ev[0], ws->stream));
+  for (int r = 0; r < reps; ++r) UT_HIP(batched_grad(ws, q, lj, g, nl));
+  UT_HIP(hipEventRecord(ev[1], ws->stream));
+  UT_HIP(hipEventSynchronize(ev[1]));
+  float ms = 0.0f;
+  UT_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  *us_per_call = 1e3 * ms / reps;
+  return UT_OK;
 }
 
 int64_t ut_noalloc_scratch_bytes(const ut_plan* plan, int32_t num_chains) {
@@ -2481,17 +2503,26 @@ __global__ __launch_bounds__(256) void k_ll_logistic(const double* __restrict__ 
   out[t] = yv[i] * eta - softplus;
 }
 
-// the same for the bf16 plan: X is the plan's bf16 matrix (transposed, Xt[d n + i]),
-// widened exactly, as the matrix-core gradient evaluates it
+// the same for the bf16 plan: X is the plan's bf16 matrix (row-major, X[i D + d]),
+// widened exactly, as the matrix-core gradient evaluates it; 16-byte row chunks
 __global__ __launch_bounds__(256) void k_ll_glm_bf16(const double* __restrict__ draws, long long S, int D,
-                                                     const uint16_t* __restrict__ Xt, const float* __restrict__ yv,
+                                                     const uint16_t* __restrict__ X, const float* __restrict__ yv,
                                                      long long n, double* __restrict__ out) {
   const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
   if (t >= S * n) return;
   const long long s = t / n, i = t - s * n;
   const double* b = draws + s * D;
+  const uint4* xr = (const uint4*)(X + i * D);
   double eta = 0.0;
-  for (int d = 0; d < D; ++d) eta += (double)__uint_as_float((uint32_t)Xt[(long long)d * n + i] << 16) * b[d];
+  for (int d8 = 0; d8 < D / 8; ++d8) {
+    const uint4 v = xr[d8];
+    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      eta += (double)__uint_as_float(w[k] << 16) * b[8 * d8 + 2 * k];
+      eta += (double)__uint_as_float(w[k] & 0xffff0000u) * b[8 * d8 + 2 * k + 1];
+    }
+  }
   const double softplus = eta > 0.0 ? eta + log1p(exp(-eta)) : log1p(exp(eta));
   out[t] = (double)yv[i] * eta - softplus;
 }
@@ -2624,7 +2655,7 @@ int ut_pointwise_loglik(const ut_plan* plan, const double* draws, int64_t num_dr
     UT_HIP(hipMemcpy(dd, draws + s0 * P, (size_t)S * P * 8, hipMemcpyHostToDevice));
     const unsigned blocks = (unsigned)((S * n + 255) / 256);
     if (m.kind == MODEL_GLM_MFMA)
-      hipLaunchKernelGGL(k_ll_glm_bf16, dim3(blocks), dim3(256), 0, 0, dd, (long long)S, plan->glm.D, plan->glm.Xt,
+      hipLaunchKernelGGL(k_ll_glm_bf16, dim3(blocks), dim3(256), 0, 0, dd, (long long)S, plan->glm.D, plan->glm.X,
                          plan->glm.y, (long long)n, dout);
     else if (m.kind == MODEL_LOGISTIC)
       hipLaunchKernelGGL(k_ll_logistic, dim3(blocks), dim3(256), 0, 0, dd, (long long)S, m.D, m.xT_d, m.yv_d,

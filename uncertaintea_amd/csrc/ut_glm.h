@@ -6,10 +6,9 @@
 namespace ut {
 
 // Data of a logistic GLM staged for the matrix cores: X (N x D, bf16, row-major),
-// its transpose Xt (D x N), outcomes y (fp32), normal(loc, scale) prior.
+// outcomes y (fp32), normal(loc, scale) prior.
 struct GlmPlan {
   const uint16_t* X = nullptr;
-  const uint16_t* Xt = nullptr;
   const float* y = nullptr;
   int N = 0;
   int D = 0;

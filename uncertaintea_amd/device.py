@@ -47,6 +47,17 @@ class DeviceBatchedWorkspace:
             self._destroy(self._h)
             self._h = None
 
+    def gradient_kernel_time(self, params, reps: int = 20, need_logjoint: bool = False) -> float:
+        """Mean device microseconds per batched gradient of all chains at `params`
+        (P x C) over `reps` back-to-back calls (HIP events on the workspace stream)."""
+        if self.dtype != np.float32:
+            raise ValueError("gradient_kernel_time: float32 workspaces only")
+        p = self._params(params)
+        us = ct.c_double()
+        L.check(L.lib().ut_gradient_kernel_time(self._h, L.ptr(p), int(reps), int(bool(need_logjoint)),
+                                                ct.byref(us)))
+        return us.value
+
     def _params(self, params) -> np.ndarray:
         a = np.asarray(params)
         if a.ndim != 2:
