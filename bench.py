@@ -56,6 +56,10 @@ batched_meads with 1024 chains, without and with a flow; CPU baseline projected.
 
 `--workload advi`: batched_advi on the hierarchical model (P = 66, 1024 particles);
 `value` is Adam iterations/s.
+
+`--workload flow_kernels`: the funnel preconditioner's sampling-path kernel pair alone (4096
+rows; `--steps` = back-to-back launches per direction), for profiling those kernels by
+themselves.
 """
 from __future__ import annotations
 
@@ -194,7 +198,7 @@ def _cpu_hmc_seconds_per_iteration(om, chains, L, step, seconds, threads, seed, 
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=("funnel_flow", "hier_nuts", "glm_chees", "gp_meads", "advi"),
+    ap.add_argument("--workload", choices=("funnel_flow", "hier_nuts", "glm_chees", "gp_meads", "advi", "flow_kernels"),
                     default="funnel_flow")
     ap.add_argument("--guide", choices=("meanfield", "fullrank", "flow"), default="fullrank",
                     help="advi: the variational guide")
@@ -251,7 +255,7 @@ def main() -> int:
               "(chains shard across GPUs without communication; see the design notes)", file=sys.stderr)
         return 2
     return {"funnel_flow": bench_funnel_flow, "hier_nuts": bench_hier_nuts, "glm_chees": bench_glm_chees,
-            "gp_meads": bench_gp_meads, "advi": bench_advi}[args.workload](args)
+            "gp_meads": bench_gp_meads, "advi": bench_advi, "flow_kernels": bench_flow_kernels}[args.workload](args)
 
 
 def bench_hier_nuts(args) -> int:
@@ -363,6 +367,63 @@ def bench_hier_nuts(args) -> int:
     return 0
 
 
+FLOW16_PMC = "k_flow16_cfg4_pmc.json"
+
+
+def flow_roofline(flow, rows, reps=200) -> dict:
+    """Roofline of the flow's sampling-path kernel pair (one forward and one backward launch per
+    batched gradient of `rows` rows): algorithmic conditioner-GEMM flops over the kernels' own
+    device time (HIP events around `reps` back-to-back launches of each direction on the flow's
+    stream), and their HBM bytes when a committed FETCH_SIZE / WRITE_SIZE record exists."""
+    H = flow.hidden
+    # conditioner GEMM flops per chain-gradient: forward 2 GEMMs + backward 2 GEMMs per layer
+    flops_grad = 0.0
+    for npass, nact, _ in flow.layout():
+        flops_grad += 2 * 2.0 * (npass * H + H * 2 * nact)
+    fwd_us, bwd_us = flow.kernel_times(rows, reps=reps)
+    achieved = rows * flops_grad / ((fwd_us + bwd_us) * 1e-6) / 1e12
+    traffic, traffic_note = None, None
+    pmc = ROOT / "profiles" / FLOW16_PMC
+    if pmc.exists():  # HBM bytes of one forward + one backward launch at 4096 rows
+        rec = json.loads(pmc.read_text())
+        traffic = rec["traffic_bytes_per_launch_pair"]
+        traffic_note = rec.get("note")
+    else:
+        traffic_note = "no FETCH_SIZE / WRITE_SIZE record for these kernels"
+    return {"bound": "mfma", "compute": f"bf16 MFMA conditioner GEMMs, fp32 accumulation (latency-bound at {rows} rows)",
+            "achieved": achieved, "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
+            "frac": achieved / PEAK_BF16_TFLOPS, "traffic": traffic, "traffic_note": traffic_note,
+            "kernel": "k_flow16_fwd + k_flow16_bwd (one launch of each per batched gradient of all chains)",
+            "timing": f"HIP events around {reps} back-to-back launches of each direction on the flow's stream, "
+                      f"{rows} device-resident rows",
+            "forward_us": fwd_us, "backward_us": bwd_us,
+            "flops_per_launch_pair": rows * flops_grad, "flops_per_chain_gradient": flops_grad}
+
+
+def bench_flow_kernels(args) -> int:
+    """Micro-workload: the funnel preconditioner's sampling-path kernel pair alone (P = 101, 6
+    coupling layers of width 256, 4096 rows of N(0, I) draws, weights at a small random
+    perturbation of the identity flow), for profiling the dominant kernels by themselves."""
+    import uncertaintea_amd as ut
+    rows = 4096 if args.chains is None else args.chains
+    flow = ut.AffineCouplingFlow(101, 6, 256, seed=args.seed)
+    rng = np.random.default_rng(args.seed)
+    w = flow.weights.copy()
+    w += (rng.normal(size=w.size) * 0.02).astype(np.float32)
+    flow.weights = w
+    reps = 200 if args.steps is None else args.steps
+    roof = flow_roofline(flow, rows, reps=reps)
+    line = {"metric": "flow kernel pair throughput (rows/s through forward + backward)",
+            "value": rows / ((roof["forward_us"] + roof["backward_us"]) * 1e-6), "unit": "rows/s", "n_gpus": 1,
+            "steps": reps, "warmup": 1, "ms_per_step": 1e-3 * (roof["forward_us"] + roof["backward_us"]),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16 (MFMA operands)",
+            "data": "synthetic (N(0, I) rows)",
+            "config": {"workload": "flow kernel pair, P=101, 6 coupling layers, H=256", "rows": rows},
+            "roofline": roof, "cpu_baseline": None}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def bench_funnel_flow(args) -> int:
     import uncertaintea_amd as ut
     from uncertaintea_amd import diagnostics as D
@@ -456,37 +517,14 @@ def bench_funnel_flow(args) -> int:
                               target_accept=args.flow_target_accept)
     flow_s = summarise(fres, "flow")
 
-    # conditioner GEMM flops per chain-gradient: forward 2 GEMMs + backward 2 GEMMs per layer
-    flops_grad = 0.0
-    for npass, nact, _ in flow.layout():
-        flops_grad += 2 * 2.0 * (npass * H + H * 2 * nact)
+    roofline = flow_roofline(flow, chains)
     ksec = fres.sampling_kernel_seconds
     evals = fres.sampling_gradient_evals
-    # the dominant kernels' own device time: back-to-back launches of the flow's forward
-    # and backward at the sampling launch shape (all chains' rows), HIP events on the
-    # flow's stream
-    fwd_us, bwd_us = flow.kernel_times(chains, reps=200)
-    achieved = chains * flops_grad / ((fwd_us + bwd_us) * 1e-6) / 1e12
-    traffic, traffic_note = None, None
-    pmc = ROOT / "profiles" / "k_flow_w8_cfg4_pmc.json"
-    if pmc.exists():  # HBM bytes of one forward + one backward launch at 4096 rows
-        rec = json.loads(pmc.read_text())
-        traffic = sum(rec[k]["traffic_bytes_per_launch"] for k in ("k_flow_fwd_w8", "k_flow_bwd_w8"))
-    else:
-        traffic_note = "no FETCH_SIZE / WRITE_SIZE record for these kernels (see the profiles' notes)"
-    roofline = {"bound": "mfma", "compute": "bf16 MFMA conditioner GEMMs, fp32 accumulation (latency-bound at "
-                                           "4096 rows)",
-                "achieved": achieved, "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
-                "frac": achieved / PEAK_BF16_TFLOPS, "traffic": traffic, "traffic_note": traffic_note,
-                "kernel": "k_flow_fwd_w8 + k_flow_bwd_w8 (one launch of each per batched gradient of all chains)",
-                "timing": "HIP events around 200 back-to-back launches of each direction on the flow's stream, "
-                          f"{chains} device-resident rows, after the flow run",
-                "forward_us": fwd_us, "backward_us": bwd_us,
-                "flops_per_launch_pair": chains * flops_grad, "flops_per_chain_gradient": flops_grad,
-                "us_per_batched_gradient_in_sampling": 1e6 * ksec / max(1, evals // chains),
-                "per_gradient_note": "the sampling run's device time per batched gradient (HIP events around the "
-                                     "timed region on the workspace stream): flow pair + model gradient + lock-step "
-                                     "leapfrog kernels"}
+    roofline.update({
+        "us_per_batched_gradient_in_sampling": 1e6 * ksec / max(1, evals // chains),
+        "per_gradient_note": "the sampling run's device time per batched gradient (HIP events around the timed "
+                             "region on the workspace stream): flow pair + model gradient + lock-step leapfrog "
+                             "kernels"})
     cpu = None
     if not args.no_cpu_baseline:
         from oracle import binding as B

@@ -5,8 +5,9 @@ gradient against the oracle's binary64 flow evaluated with the same bf16-rounded
 matrix-core operands.  The device's fp32 intermediates differ from binary64 by
 ~1e-7, which now and then moves an operand across a bf16 rounding boundary; such
 flips change that operand by 2^-9 relative and propagate through later layers.
-So positions and log-determinants are bounded at the 99th percentile
-(|d| <= 1e-5 (1 + |ref|), the arithmetic) and at the maximum (2e-3, the flips);
+So a row whose operands saw no flip agrees to the arithmetic (|d| <= 1e-5 (1 + |ref|)), which
+at least 90 % of the rows must do for positions (their row maximum) and log-determinants,
+and every value within 2e-3 (the flips);
 gradients normwise (|d|_2 <= 2e-3 |ref|_2).  Errors against the unrounded
 binary64 flow are printed.  Bit-exact: shared-adaptation lock-step HMC against
 the oracle's binary32 twin.  Statistical: a flow trained on exact funnel draws maps
@@ -59,9 +60,13 @@ def _funnel_draws(n, dim, seed=1):
     return np.concatenate([v[:, None], x], axis=1).astype(np.float32)
 
 
-def test_flow_kernels_against_oracle():
-    say("flow kernels vs oracle")
-    P, Lyr, H = 101, 6, 256
+@pytest.mark.parametrize("P", [101, 2, 128, 129])
+def test_flow_kernels_against_oracle(P):
+    """P = 101, 2 and 128 run the sampling path's 16x16x32-MFMA all-layer kernels (every layer
+    with at most 64 passive and active coordinates; 128 is the widest), P = 129 the older
+    all-layer kernels; the inverse and the training gradient run the per-step kernels."""
+    say("flow kernels vs oracle", P)
+    Lyr, H = 6, 256
     flow = ut.AffineCouplingFlow(P, Lyr, H, scale_clamp=5.0, seed=3)
     flow.weights = _random_weights(flow, 4, scale=0.2)
     o16 = B.OracleFlow.of(flow, bf16=True)
@@ -76,8 +81,12 @@ def test_flow_kernels_against_oracle():
     say(f"  forward: vs bf16-operand oracle q p99 {np.quantile(d_q, 0.99):.2e} max {d_q.max():.2e}, logdet p99 "
         f"{np.quantile(d_ld, 0.99):.2e} max {d_ld.max():.2e}; vs binary64 q max "
         f"{np.max(np.abs(q - fq) / (1 + np.abs(fq))):.2e} logdet max {np.max(np.abs(ld - fld)):.2e}")
-    assert np.median(d_q) <= 1e-6 and np.quantile(d_q, 0.99) <= 1e-5 and d_q.max() <= 2e-3
-    assert np.quantile(d_ld, 0.99) <= 1e-5 and d_ld.max() <= 2e-3
+    # a row's values depend on ~2000 bf16-rounded operands of that row, so a few percent of
+    # the rows carry a flip (up to ~1e-3 on the row's later coordinates and its
+    # log-determinant): at least 90 % of the rows must agree to the arithmetic (1e-5), all
+    # of them to 2e-3
+    assert np.median(d_q) <= 1e-6 and np.quantile(d_q.max(axis=1), 0.9) <= 1e-5 and d_q.max() <= 2e-3
+    assert np.quantile(d_ld, 0.9) <= 1e-5 and d_ld.max() <= 2e-3
     zi, ldi = flow.inverse(q)
     rz, rldi = o16.inverse(q)
     d_z = np.abs(zi - rz) / (1 + np.abs(rz))
@@ -197,11 +206,14 @@ def test_hmc_shared_bitexact_against_twin():
 
 @pytest.mark.parametrize("P", [101, 2, 300, 129])
 def test_fused_layers_match_unfused(monkeypatch, P):
-    """The LDS-staged coupling-layer kernels of the sampling path (passive and output
-    widths of 128) against the per-step kernels: forward values, log-determinants and
-    the sampler's VJP bit for bit.  P = 101 and 2 run every layer in one launch per
-    direction; P = 129 mixes per-layer LDS-staged kernels (even layers) with per-step
-    ones (odd layers, 2 na = 130); P = 300 exercises the per-step fallback."""
+    """The sampling path's all-layer kernels against the per-step kernels (UT_FLOW_FUSED=0).
+    P = 101 and 2 run the 16x16x32-MFMA kernels, whose tiles sum in another order: forward
+    values, log-determinants and the sampler's VJP agree to the flow tolerance (fp32 rounding
+    and the occasional bf16 flip of an operand: 99th percentile <= 1e-5 (1 + |ref|), maximum
+    2e-3; VJP normwise 2e-3), and an HMC run on the funnel with each path samples the same law
+    (v's mean and sd within 5 combined MCSE).  P = 129 mixes per-layer LDS-staged kernels (even
+    layers) with per-step ones (odd layers, 2 na = 130) and P = 300 exercises the per-step
+    fallback: those match bit for bit."""
     say("fused vs unfused flow layers", P)
     flows = {}
     for mode in ("0", "1"):
@@ -215,17 +227,30 @@ def test_fused_layers_match_unfused(monkeypatch, P):
     gq = rng.normal(size=(256, P)).astype(np.float32)
     q0, l0 = flows["0"].forward(z)
     q1, l1 = flows["1"].forward(z)
-    assert np.array_equal(q0.view(np.uint32), q1.view(np.uint32))
-    assert np.array_equal(l0.view(np.uint32), l1.view(np.uint32))
     g0 = flows["0"].vjp(z, gq)
     g1 = flows["1"].vjp(z, gq)
-    assert np.array_equal(g0.view(np.uint32), g1.view(np.uint32))
+    if P in (300, 129):
+        assert np.array_equal(q0.view(np.uint32), q1.view(np.uint32))
+        assert np.array_equal(l0.view(np.uint32), l1.view(np.uint32))
+        assert np.array_equal(g0.view(np.uint32), g1.view(np.uint32))
+        return
+    d_q = np.abs(q1 - q0) / (1 + np.abs(q0))
+    d_l = np.abs(l1 - l0) / (1 + np.abs(l0))
+    e_g = np.linalg.norm(g1 - g0) / np.linalg.norm(g0)
+    say(f"  q p99 {np.quantile(d_q, 0.99):.2e} max {d_q.max():.2e}; logdet max {d_l.max():.2e}; vjp {e_g:.2e}")
+    assert np.quantile(d_q, 0.99) <= 1e-5 and d_q.max() <= 2e-3
+    assert np.quantile(d_l, 0.99) <= 1e-5 and d_l.max() <= 2e-3
+    assert e_g <= 2e-3
     if P == 101:
         model = M.NealsFunnel(100, 3.0)
-        runs = [ut.batched_hmc(model, num_chains=128, num_samples=10, num_warmup=20, num_leapfrog_steps=5,
+        runs = [ut.batched_hmc(model, num_chains=256, num_samples=300, num_warmup=200, num_leapfrog_steps=5,
                                per_chain_adaptation=False, flow=f, seed=23) for f in (flows["0"], flows["1"])]
-        assert np.array_equal(runs[0].unconstrained_samples.view(np.uint32),
-                              runs[1].unconstrained_samples.view(np.uint32))
+        v = [r.unconstrained_samples[:, :, :1].astype(np.float64) for r in runs]
+        for stat in (lambda a: a, lambda a: (a - a.mean()) ** 2):
+            x0, x1 = stat(v[0]), stat(v[1])
+            z_ = abs(x0.mean() - x1.mean()) / np.sqrt(float(D.mcse_mean(x0)[0]) ** 2 + float(D.mcse_mean(x1)[0]) ** 2)
+            say(f"  HMC v statistic, per-step vs all-layer kernels: z {z_:.2f}")
+            assert z_ < 5.0
 
 
 def test_flow_graph_replay_matches_eager(monkeypatch):

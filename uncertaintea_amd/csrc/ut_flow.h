@@ -63,6 +63,22 @@ struct FusedBwdLayer {
   const uint16_t* W1Tf;
 };
 
+// Layer descriptor of the 16x16x32-MFMA all-layer kernels (ut_flow16.hip): fragment-order
+// weight copies (that kernel's layout), biases, and the per-batch values the forward keeps for
+// the backward (s and x_a e^s, B x na fp32; the bf16 passive inputs in fragment order, B x 64).
+struct Flow16Layer {
+  const uint16_t* W1f;
+  const uint16_t* W2f;
+  const uint16_t* W2Tf;
+  const uint16_t* W1Tf;
+  const float* b1;
+  const float* b2;
+  float* s;
+  float* xs;
+  uint16_t* xpg;
+  int np, na, par;
+};
+
 // Per-batch activations kept for the backward pass.
 struct FlowScratch {
   int B = 0;
@@ -87,6 +103,9 @@ struct FlowScratch {
   // per layer: the last forward ran the LDS-staged kernel (its h is in that kernel's
   // private fragment order, readable only by the matching backward kernel)
   std::vector<char> fwd_fused;
+  // the last forward ran the 16x16x32-MFMA all-layer kernel (its saved values are readable
+  // only by the matching backward kernel)
+  bool fwd16 = false;
 };
 
 struct Flow {
@@ -105,6 +124,9 @@ struct Flow {
   unsigned gen = 0;         // bumped whenever the scratch buffers move (captured graphs go stale)
   FusedFwdLayer* dfwd = nullptr;  // device, L descriptors
   FusedBwdLayer* dbwd = nullptr;
+  Flow16Layer* d16 = nullptr;     // device, L descriptors of the 16x16x32-MFMA kernels
+  bool m16 = false;               // the sampling path runs those kernels (flow16_eligible)
+  float* ldbase = nullptr;        // device: sum of the base log-scales, refreshed with the weights
   bool fused = true;        // LDS-staged layer kernels on the sampling path where the widths allow
   // full-rank Gaussian guide (no coupling layers): q = mu + e^ls z + F z with F the
   // strictly lower P x P factor stored row-major at params + ofac
@@ -113,6 +135,18 @@ struct Flow {
   double* fpart = nullptr;  // fsplits x P x P split partials of F's gradient
   int fsplits = 0;          // FACTOR_SPLITS, fewer for large P (the partials stay <= 128 MiB)
 };
+
+// The sampling path's 16x16x32-MFMA all-layer kernels (ut_flow16.hip): eligible when every
+// coupling layer has 1..64 passive and <= 64 active coordinates and H = 256.
+bool flow16_eligible(const Flow& f);
+// fragment-order weight copies of those kernels and the base log-determinant
+hipError_t flow16_sync_weights(Flow& f, hipStream_t st);
+// q = f(z) of B rows (B a multiple of 16) into x (and q when not null), log-determinants into
+// logdet (and logdet_out when not null); keeps the backward's values in the descriptors' buffers
+hipError_t flow16_forward(Flow& f, const float* z, float* x, float* q, float* logdet, float* logdet_out, int B,
+                          hipStream_t st);
+// g_z = (dq/dz)^T g_q + d logdet / dz for the rows of the last flow16_forward
+hipError_t flow16_backward(Flow& f, const float* g_q, float* g_z, int B, hipStream_t st);
 
 // factor = true: the full-rank guide (L must be 0), mu, ls and the strictly lower
 // factor as its weights

@@ -1806,6 +1806,8 @@ hipError_t flow_alloc(Flow& f, int P, int L, int H, bool factor) {
   if (e == hipSuccess && factor) e = own(f, &f.fpart, (size_t)f.fsplits * P * P);
   if (e == hipSuccess && L > 0) e = own(f, &f.dfwd, (size_t)L);
   if (e == hipSuccess && L > 0) e = own(f, &f.dbwd, (size_t)L);
+  if (e == hipSuccess && L > 0) e = own(f, &f.d16, (size_t)L);
+  if (e == hipSuccess) e = own(f, &f.ldbase, (size_t)1);
   if (e == hipSuccess) e = own(f, &f.grads, (size_t)off);
   if (e == hipSuccess) e = own(f, &f.adam_m, (size_t)off);
   if (e == hipSuccess) e = own(f, &f.adam_v, (size_t)off);
@@ -1827,6 +1829,7 @@ hipError_t flow_alloc(Flow& f, int P, int L, int H, bool factor) {
     if (e == hipSuccess) e = own(f, &ly.W2Tf, (size_t)H * op);
     if (e == hipSuccess) e = own(f, &ly.W1Tf, (size_t)H * kp);
   }
+  f.m16 = flow16_eligible(f);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&f.stream, hipStreamNonBlocking);
   return e;
 }
@@ -1862,8 +1865,9 @@ hipError_t flow_scratch(Flow& f, int B) {
     e = own(f, &s.xp[l], (size_t)B * round_up(ly.np, FLOW_PAD));
     if (e == hipSuccess) e = own(f, &s.h[l], (size_t)B * H);
     if (e == hipSuccess) e = own(f, &s.hb[l], (size_t)B * H);
-    if (e == hipSuccess) e = own(f, &s.s[l], (size_t)B * std::max(ly.na, 1));
-    if (e == hipSuccess) e = own(f, &s.xa[l], (size_t)B * std::max(ly.na, 1));
+    // row stride na on the per-step path, 64 in the 16x16x32-MFMA kernels (unconditional stores)
+    if (e == hipSuccess) e = own(f, &s.s[l], (size_t)B * std::max(ly.na, 64));
+    if (e == hipSuccess) e = own(f, &s.xa[l], (size_t)B * std::max(ly.na, 64));
   }
   const size_t wide = std::max<size_t>(H, (size_t)std::max(kpm, opm));
   if (e == hipSuccess) e = own(f, &s.O, (size_t)B * opm);
@@ -1890,6 +1894,13 @@ hipError_t flow_scratch(Flow& f, int B) {
     }
     e = hipMemcpy(f.dfwd, fw.data(), fw.size() * sizeof(FusedFwdLayer), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(f.dbwd, bw.data(), bw.size() * sizeof(FusedBwdLayer), hipMemcpyHostToDevice);
+    std::vector<Flow16Layer> d16(f.L);
+    for (int l = 0; l < f.L; ++l) {
+      const FlowLayer& ly = f.layers[l];
+      d16[l] = {ly.W1f, ly.W2f, ly.W2Tf, ly.W1Tf, f.params + ly.ob1, f.params + ly.ob2, s.s[l], s.xa[l], s.xp[l],
+                ly.np, ly.na, l & 1};
+    }
+    if (e == hipSuccess) e = hipMemcpy(f.d16, d16.data(), d16.size() * sizeof(Flow16Layer), hipMemcpyHostToDevice);
   }
   return e;
 }
@@ -1898,12 +1909,14 @@ hipError_t flow_sync_weights(Flow& f, hipStream_t st) {
   for (int l = 0; l < f.L; ++l) {
     FlowLayer& ly = f.layers[l];
     const int kp = round_up(ly.np, FLOW_PAD), op = round_up(2 * ly.na, FLOW_PAD);
-    const bool frag = f.H == LH && kp == LW && op == LW;  // the all-layer kernels' widths
+    // the all-layer kernels' widths; the 16x16x32 kernels write their own fragment order
+    const bool frag = f.H == LH && kp == LW && op == LW && !f.m16;
     hipLaunchKernelGGL(k_weights_bf16, dim3(nblk((long long)f.H * kp)), dim3(T256), 0, st, f.params + ly.oW1, f.H,
                        kp, ly.W1b, ly.W1Tb, frag ? ly.W1f : nullptr, frag ? ly.W1Tf : nullptr);
     hipLaunchKernelGGL(k_weights_bf16, dim3(nblk((long long)op * f.H)), dim3(T256), 0, st, f.params + ly.oW2, op,
                        f.H, ly.W2b, ly.W2Tb, frag ? ly.W2f : nullptr, frag ? ly.W2Tf : nullptr);
   }
+  if (f.m16) return flow16_sync_weights(f, st);
   return hipGetLastError();
 }
 
@@ -1970,11 +1983,16 @@ static hipError_t forward_impl(Flow& f, const float* z, float* q, float* logdet,
     s.fwd_fused[l] = allow_fused && fused_layer(f, l);
     all = all && s.fwd_fused[l];
   }
+  s.fwd16 = false;
   if (all) {
     // the rows go to s.x and the log-determinants to s.logdet (read by the model gradient
     // and the log-density) and also straight to q / logdet when given
     float* q2 = (q && q != s.x) ? q : nullptr;
     float* l2 = (logdet && logdet != s.logdet) ? logdet : nullptr;
+    if (f.m16) {
+      s.fwd16 = true;
+      return flow16_forward(f, z, s.x, q2, s.logdet, l2, B, st);
+    }
     const int rl = resident_rows(B);
     const bool w8 = resident_w8(rl);
     hipLaunchKernelGGL(w8 ? fwd_w8_kernel(f.L) : fwd_all_kernel(rl, f.L), dim3(B / rl), dim3(w8 ? 512 : T256),
@@ -2022,6 +2040,7 @@ hipError_t flow_backward(Flow& f, const float* g_q, float* g_z, int B, hipStream
   if (B % FLOW_PAD || B > f.sc.B) return hipErrorInvalidValue;
   FlowScratch& s = f.sc;
   const long long n = (long long)B * f.P;
+  if (s.fwd16) return flow16_backward(f, g_q, g_z, B, st);
   bool all = f.L > 0;
   for (int l = 0; l < f.L; ++l) all = all && s.fwd_fused[l];
   if (all) {
@@ -2064,6 +2083,7 @@ hipError_t flow_backward(Flow& f, const float* g_q, float* g_z, int B, hipStream
 hipError_t flow_inverse(Flow& f, const float* q, float* z, float* logdet_inv, int B, hipStream_t st) {
   if (B % FLOW_PAD || B > f.sc.B) return hipErrorInvalidValue;
   FlowScratch& s = f.sc;
+  s.fwd16 = false;  // the saved values below are the inverse's
   const long long n = (long long)B * f.P;
   if (q != s.x) {
     hipError_t e = hipMemcpyAsync(s.x, q, n * 4, hipMemcpyDeviceToDevice, st);

@@ -1,0 +1,569 @@
+// ut_flow16.hip -- the sampling path's flow kernels on v_mfma_f32_16x16x32_bf16 for flows whose
+// coupling layers have at most 64 passive and 64 active coordinates (P <= 128) and H = 256
+// (the funnel's P = 101 preconditioner).
+//
+// One launch per direction carries a workgroup's 16 rows through every layer; eight
+// wavefronts, two per SIMD.  Every conditioner GEMM is computed transposed (weights as the A
+// operand, held in registers from fragment-order copies; the 16 rows as the N dimension), so a
+// lane's accumulator holds four consecutive hidden units (or outputs) of one row:
+//
+//   forward, layer l    U^T[256 x 16] = W1 X_p^T       (wavefront w: hidden 32 w .. 32 w + 31)
+//                       h = tanh(U + b1) -> LDS (bf16, 8-byte stores)
+//                       O^T[128 x 16] = W2' h^T         (wavefront w: outputs 16 w .. 16 w + 15)
+//                       coupling in registers: W2' interleaves W2's rows so that a lane holds
+//                       (s_j, t_j, s_j+1, t_j+1) of one row, j = 8 w + 2 (lane / 16)
+//   backward, layer l   coupling backward in registers -> dO (bf16) in the same interleaved order
+//                       dh^T = W2'^T dO^T, with U recomputed from the stored bf16 passive
+//                       inputs (exactly the forward's arithmetic), dU = dh (1 - h^2) -> LDS
+//                       dX_p^T[64 x 16] = W1^T dU^T, added to the passive coordinates
+//
+// The forward stores per layer only what the backward cannot recompute cheaply: the bf16
+// passive inputs (in the B-operand fragment order, 2 KB per workgroup), s and x_a e^s (fp32);
+// the hidden activations never leave the chip.  Two workgroup barriers per forward layer, three
+// per backward layer.  The padded passive width is 64 (two 32-deep k-steps), so no multiply runs
+// on the zero padding of the 128-wide parameter layout.  The log-determinant of the base
+// (sum of the log-scales) is summed once per weight update (k_ldbase16), each lane keeps its
+// coordinates' share of the coupling log-determinants across layers, and the row sums are
+// formed once at the end.
+//
+// Arithmetic: the same operations as the per-step kernels (bf16 operands, fp32 accumulation,
+// flow_tanh, the clamp c tanh(s / c)), summed in the orders of this kernel's tiles, so results
+// agree with the per-step path to fp32 rounding (and the occasional bf16 rounding flip of an
+// operand), not bit for bit.
+#include "ut_flow.h"
+#include "ut_mfma.h"
+
+namespace ut {
+
+namespace {
+
+#define UT_GLOBAL16 __attribute__((address_space(1)))
+template <typename T>
+__device__ inline UT_GLOBAL16 T* g16(T* p) {
+  return (UT_GLOBAL16 T*)p;
+}
+template <typename T>
+__device__ inline const UT_GLOBAL16 T* g16(const T* p) {
+  return (const UT_GLOBAL16 T*)p;
+}
+
+constexpr int R16 = 16;          // rows per workgroup
+constexpr int NW16 = 8;          // wavefronts per workgroup
+constexpr int THREADS16 = 64 * NW16;
+constexpr int HID16 = 256;       // hidden width
+constexpr int KP16 = 64;         // padded passive / active width
+constexpr int XPS16 = KP16 + 8;  // LDS row stride (bf16) of the passive-input tile
+constexpr int HS16 = HID16 + 8;  // of the hidden tile
+constexpr int OS16 = 2 * KP16 + 8;  // of the conditioner-output gradient tile
+// fragment-order weight copies, elements per layer
+constexpr int W1F16 = NW16 * 2 * 2 * 512;   // [wave][tile][k-step][lane][8]
+constexpr int W2F16 = NW16 * 8 * 512;       // [wave][k-step][lane][8]
+constexpr int W2TF16 = NW16 * 2 * 4 * 512;  // [wave][tile][k-step][lane][8]
+constexpr int W1TF16 = 4 * 8 * 512;         // [wave < 4][k-step][lane][8]
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ inline f32x4 mfma16(const bf16x8& a, const bf16x8& b, const f32x4& c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// s_waitcnt lgkmcnt(0) + s_barrier: the kernels exchange data between wavefronts only through
+// LDS, so global loads and stores keep flying across the barrier
+__device__ __forceinline__ void lds_barrier16() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+__device__ inline uint32_t pack2(float a, float b) {
+  return (uint32_t)f2bf(a) | ((uint32_t)f2bf(b) << 16);
+}
+
+// W2' row o (the interleaved output order s_0, t_0, s_1, t_1, ...) -> W2 row, or -1 past 2 na
+__device__ inline int w2_row(int o, int na) {
+  const int j = o >> 1;
+  return j >= na ? -1 : ((o & 1) ? na + j : j);
+}
+
+// fragment-order bf16 copies of one layer's weights (W1: H x kp, W2: op x H, fp32 masters)
+__global__ void k_frag16(const float* __restrict__ W1, int kp, const float* __restrict__ W2, int np, int na,
+                         bf16_t* __restrict__ W1f, bf16_t* __restrict__ W2f, bf16_t* __restrict__ W2Tf,
+                         bf16_t* __restrict__ W1Tf) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int e = i & 7, lane = (i >> 3) & 63, lr = lane & 15, lg = lane >> 4;
+  if (i < W1F16) {  // A operand of U^T: W1[hidden][k]
+    const int ks = (i >> 9) & 1, t = (i >> 10) & 1, w = i >> 11;
+    const int hid = 32 * w + 16 * t + lr, k = 32 * ks + 8 * lg + e;
+    W1f[i] = k < np ? f2bf(W1[(long long)hid * kp + k]) : (bf16_t)0;
+    return;
+  }
+  i -= W1F16;
+  if (i < W2F16) {  // A operand of O^T: W2'[o][hidden]
+    const int ks = (i >> 9) & 7, w = i >> 12;
+    const int r = w2_row(16 * w + lr, na), hid = 32 * ks + 8 * lg + e;
+    W2f[i] = r < 0 ? (bf16_t)0 : f2bf(W2[(long long)r * HID16 + hid]);
+    return;
+  }
+  i -= W2F16;
+  if (i < W2TF16) {  // A operand of dh^T: W2'^T[hidden][o]
+    const int ks = (i >> 9) & 3, t = (i >> 11) & 1, w = i >> 12;
+    const int hid = 32 * w + 16 * t + lr, r = w2_row(32 * ks + 8 * lg + e, na);
+    W2Tf[i] = r < 0 ? (bf16_t)0 : f2bf(W2[(long long)r * HID16 + hid]);
+    return;
+  }
+  i -= W2TF16;
+  if (i < W1TF16) {  // A operand of dX_p^T: W1^T[k][hidden]
+    const int ks = (i >> 9) & 7, w = i >> 12;
+    const int k = 16 * w + lr, hid = 32 * ks + 8 * lg + e;
+    W1Tf[i] = k < np ? f2bf(W1[(long long)hid * kp + k]) : (bf16_t)0;
+  }
+}
+
+// sum of the base log-scales in index order (the order of k_logdet_init)
+__global__ void k_ldbase16(const float* __restrict__ ls, int P, float* __restrict__ out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  float s = 0.0f;
+  for (int i = 0; i < P; ++i) s += ls[i];
+  out[0] = s;
+}
+
+struct Fwd16Regs {
+  bf16x8 a1[2][2];  // W1 tiles x k-steps
+  bf16x8 a2[8];     // W2' k-steps
+  float b1[2][4];
+  float b2s[2], b2t[2];
+};
+// the first GEMM's weights and biases of layer `ly`
+__device__ inline void fwd16_load1(const Flow16Layer& ly, Fwd16Regs& g) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, lg = lane >> 4;
+  const UT_GLOBAL16 bf16_t* W1f = g16(ly.W1f);
+#pragma unroll
+  for (int t = 0; t < 2; ++t)
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks)
+      g.a1[t][ks] = *(const UT_GLOBAL16 bf16x8*)(W1f + (((w * 2 + t) * 2 + ks) * 64 + lane) * 8);
+  const UT_GLOBAL16 float* b1 = g16(ly.b1);
+#pragma unroll
+  for (int t = 0; t < 2; ++t)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) g.b1[t][r] = b1[32 * w + 16 * t + 4 * lg + r];
+}
+// the second GEMM's weights and the coupling biases of this lane's coordinates
+__device__ inline void fwd16_load2(const Flow16Layer& ly, Fwd16Regs& g) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, lg = lane >> 4;
+  const UT_GLOBAL16 bf16_t* W2f = g16(ly.W2f);
+#pragma unroll
+  for (int ks = 0; ks < 8; ++ks) g.a2[ks] = *(const UT_GLOBAL16 bf16x8*)(W2f + ((w * 8 + ks) * 64 + lane) * 8);
+  const UT_GLOBAL16 float* b2 = g16(ly.b2);
+  const int j0 = 8 * w + 2 * lg;
+#pragma unroll
+  for (int e = 0; e < 2; ++e) {
+    const int j = min(j0 + e, ly.na - 1);  // the padded bias vector holds >= 2 na entries
+    g.b2s[e] = b2[j];
+    g.b2t[e] = b2[ly.na + j];
+  }
+}
+
+template <int LT>
+__global__ __launch_bounds__(THREADS16) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_flow16_fwd(
+    const float* __restrict__ z, const float* __restrict__ prm, const float* __restrict__ ldbase,
+    float* __restrict__ x, float* __restrict__ q, int P, const Flow16Layer* __restrict__ layers, int L_, float c,
+    float* __restrict__ logdet, float* __restrict__ logdet_out) {
+  const int L = LT > 0 ? LT : L_;
+  extern __shared__ __attribute__((aligned(16))) float lds16[];
+  float* Xs = lds16;                          // 16 x P fp32 rows
+  bf16_t* Xp = (bf16_t*)(Xs + R16 * P);       // 16 x XPS16 passive inputs (bf16)
+  bf16_t* Hs = Xp + R16 * XPS16;              // 16 x HS16 hidden activations (bf16)
+  float* Red = (float*)(Hs + R16 * HS16);     // 8 x 16 log-determinant partials
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63, lr = lane & 15, lg = lane >> 4;
+  const long long r0 = (long long)blockIdx.x * R16;
+  const int j0 = 8 * w + 2 * lg;  // this lane's coupling coordinates j0, j0 + 1 (row lr)
+  const int wu = __builtin_amdgcn_readfirstlane(w);  // the wavefront index as a scalar (uniform branches)
+  // weights in two register sets: layer l computes with one while layer l + 1's are in flight
+  Fwd16Regs ga, gb;
+  fwd16_load1(layers[0], ga);
+  fwd16_load2(layers[0], ga);
+  if (L > 1) {
+    fwd16_load1(layers[1], gb);
+    fwd16_load2(layers[1], gb);
+  }
+  // base: x = mu + e^ls z
+  for (int e0 = tid; e0 < R16 * P; e0 += 4 * THREADS16) {
+    float zv[4], mv[4], lv[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int e = min(e0 + THREADS16 * k, R16 * P - 1);
+      const int ci = e % P;
+      zv[k] = z[r0 * P + e];
+      mv[k] = prm[ci];
+      lv[k] = prm[P + ci];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int e = e0 + THREADS16 * k;
+      if (e < R16 * P) Xs[e] = mv[k] + expf(lv[k]) * zv[k];
+    }
+  }
+  lds_barrier16();
+  {  // layer 0's passive inputs
+    const int np0 = layers[0].np, par0 = layers[0].par;
+    for (int e = tid; e < R16 * KP16; e += THREADS16) {
+      const int row = e / KP16, k = e % KP16;
+      Xp[row * XPS16 + k] = k < np0 ? f2bf(Xs[row * P + 2 * k + par0]) : (bf16_t)0;
+    }
+  }
+  lds_barrier16();
+  float ldp = 0.0f;  // this lane's share of its row's coupling log-determinants
+  // one layer with weights `g`; afterwards `g` receives layer l + 2's weights
+  auto layer = [&](int l, Fwd16Regs& g) {
+    const Flow16Layer ly = layers[l];
+    const bool next2 = l + 2 < L;
+    const Flow16Layer& nx = layers[next2 ? l + 2 : l];
+    const int na = ly.na, par = ly.par;
+    // ---- U^T = W1 X_p^T: B operand = 8 passive inputs of row lr.  Every wavefront holds the
+    // same fragments; wavefront 0 stores them for the backward's recomputation of U
+    bf16x8 xb[2];
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks) xb[ks] = *(const bf16x8*)(Xp + lr * XPS16 + 32 * ks + 8 * lg);
+    if (wu == 0) {
+      UT_GLOBAL16 bf16_t* xpg = g16(ly.xpg);
+#pragma unroll
+      for (int ks = 0; ks < 2; ++ks)
+        *(UT_GLOBAL16 bf16x8*)(xpg + (((long long)blockIdx.x * 2 + ks) * 64 + lane) * 8) = xb[ks];
+    }
+    f32x4 u[2];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      u[t] = (f32x4){0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+      for (int ks = 0; ks < 2; ++ks) u[t] = mfma16(g.a1[t][ks], xb[ks], u[t]);
+    }
+    float cb1[2][4];
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) cb1[t][r] = g.b1[t][r];
+    __builtin_amdgcn_sched_barrier(0);
+    if (next2) fwd16_load1(nx, g);  // a layer and a half ahead
+    // ---- h = tanh(U + b1): hidden 32 w + 16 t + 4 lg .. + 3 of row lr
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      float hv[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) hv[r] = flow_tanh(u[t][r] + cb1[t][r]);
+      uint2 pk;
+      pk.x = pack2(hv[0], hv[1]);
+      pk.y = pack2(hv[2], hv[3]);
+      *(uint2*)(Hs + lr * HS16 + 32 * w + 16 * t + 4 * lg) = pk;
+    }
+    lds_barrier16();  // h complete; every wavefront is done with X_p
+    // ---- O^T = W2' h^T (outputs 16 w .. 16 w + 15: coordinates j0, j0 + 1) and the coupling
+    // x_a <- x_a e^s + t, on the wavefronts whose outputs exist
+    float xn[2] = {0.0f, 0.0f};
+    if (16 * wu < 2 * na) {
+      f32x4 o = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+      for (int k0 = 0; k0 < 8; k0 += 4) {  // B fragments four at a time (register pressure)
+        bf16x8 hb[4];
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks) hb[ks] = *(const bf16x8*)(Hs + lr * HS16 + 32 * (k0 + ks) + 8 * lg);
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks) o = mfma16(g.a2[k0 + ks], hb[ks], o);
+      }
+      const float cb2s[2] = {g.b2s[0], g.b2s[1]}, cb2t[2] = {g.b2t[0], g.b2t[1]};
+      __builtin_amdgcn_sched_barrier(0);
+      if (next2) fwd16_load2(nx, g);
+      UT_GLOBAL16 float* sg = g16(ly.s);
+      UT_GLOBAL16 float* xsg = g16(ly.xs);
+#pragma unroll
+      for (int e = 0; e < 2; ++e) {
+        const int j = j0 + e;
+        if (j < na) {
+          const int ak = 2 * j + 1 - par;
+          const float sv = c * flow_tanh((o[2 * e] + cb2s[e]) * (1.0f / c));
+          const float xs = Xs[lr * P + ak] * expf(sv);
+          xn[e] = xs + (o[2 * e + 1] + cb2t[e]);
+          Xs[lr * P + ak] = xn[e];
+          ldp += sv;
+          // saved for the backward (row stride 64)
+          sg[(r0 + lr) * KP16 + j] = sv;
+          xsg[(r0 + lr) * KP16 + j] = xs;
+        }
+      }
+    } else if (next2) {
+      fwd16_load2(nx, g);
+    }
+    // the next layer's passive inputs are this layer's active outputs (zero past na)
+    *(uint32_t*)(Xp + lr * XPS16 + j0) = pack2(xn[0], xn[1]);
+    lds_barrier16();
+  };
+  if (LT > 0) {
+#pragma unroll
+    for (int l = 0; l < LT; l += 2) {
+      layer(l, ga);
+      if (l + 1 < LT) layer(l + 1, gb);
+    }
+  } else {
+    for (int l = 0; l < L; l += 2) {
+      layer(l, ga);
+      if (l + 1 < L) layer(l + 1, gb);
+    }
+  }
+  // log-determinant: the base's, plus the row sums of the lanes' shares
+  ldp += __shfl_xor(ldp, 16);
+  ldp += __shfl_xor(ldp, 32);
+  if (lane < 16) Red[w * 16 + lr] = ldp;
+  lds_barrier16();
+  if (tid < R16) {
+    float s = ldbase[0];
+#pragma unroll
+    for (int ww = 0; ww < NW16; ++ww) s += Red[ww * 16 + tid];
+    logdet[r0 + tid] = s;
+    if (logdet_out) logdet_out[r0 + tid] = s;
+  }
+  for (int e = tid; e < R16 * P; e += THREADS16) {
+    const float v = Xs[e];
+    x[r0 * P + e] = v;
+    if (q) q[r0 * P + e] = v;
+  }
+}
+
+struct Bwd16Regs {
+  bf16x8 a1[2][2];   // W1 (recompute of U)
+  bf16x8 a2t[2][4];  // W2'^T tiles x k-steps
+  bf16x8 a1t[8];     // W1^T k-steps (wavefront w & 3's tile)
+  float b1[2][4];
+};
+__device__ inline void bwd16_load(const Flow16Layer& ly, Bwd16Regs& g) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, lg = lane >> 4;
+  const UT_GLOBAL16 bf16_t* W1f = g16(ly.W1f);
+  const UT_GLOBAL16 bf16_t* W2Tf = g16(ly.W2Tf);
+  const UT_GLOBAL16 bf16_t* W1Tf = g16(ly.W1Tf);
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks)
+      g.a1[t][ks] = *(const UT_GLOBAL16 bf16x8*)(W1f + (((w * 2 + t) * 2 + ks) * 64 + lane) * 8);
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks)
+      g.a2t[t][ks] = *(const UT_GLOBAL16 bf16x8*)(W2Tf + (((w * 2 + t) * 4 + ks) * 64 + lane) * 8);
+  }
+  if (w < 4)
+#pragma unroll
+    for (int ks = 0; ks < 8; ++ks) g.a1t[ks] = *(const UT_GLOBAL16 bf16x8*)(W1Tf + ((w * 8 + ks) * 64 + lane) * 8);
+  const UT_GLOBAL16 float* b1 = g16(ly.b1);
+#pragma unroll
+  for (int t = 0; t < 2; ++t)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) g.b1[t][r] = b1[32 * w + 16 * t + 4 * lg + r];
+}
+
+template <int LT>
+__global__ __launch_bounds__(THREADS16) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_flow16_bwd(
+    const float* g_q, const float* __restrict__ prm, float* gz, int P, const Flow16Layer* __restrict__ layers, int L_,
+    float c) {
+  const int L = LT > 0 ? LT : L_;
+  extern __shared__ __attribute__((aligned(16))) float lds16[];
+  float* Gs = lds16;                      // 16 x P fp32 gradient rows
+  bf16_t* dO = (bf16_t*)(Gs + R16 * P);   // 16 x OS16 conditioner-output gradient (interleaved order)
+  bf16_t* DU = dO + R16 * OS16;           // 16 x HS16 hidden pre-activation gradient
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63, lr = lane & 15, lg = lane >> 4;
+  const long long r0 = (long long)blockIdx.x * R16;
+  const int j0 = 8 * w + 2 * lg;
+  const int wu = __builtin_amdgcn_readfirstlane(w);  // the wavefront index as a scalar (uniform branches)
+  // weights in two register sets: a layer computes with one while the next layer's are in flight
+  Bwd16Regs ga, gb;
+  bwd16_load(layers[L - 1], ga);
+  if (L > 1) bwd16_load(layers[L - 2], gb);
+  // the forward's saved values of this lane's coordinates (s, x_a e^s; row stride 64) and the
+  // passive-input fragments of a layer, loaded one layer ahead
+  auto saved = [&](const Flow16Layer& ly, float (&sv)[2], float (&xsv)[2]) {
+    const UT_GLOBAL16 float* sg = g16(ly.s);
+    const UT_GLOBAL16 float* xsg = g16(ly.xs);
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      sv[e] = sg[(r0 + lr) * KP16 + j0 + e];
+      xsv[e] = xsg[(r0 + lr) * KP16 + j0 + e];
+    }
+  };
+  auto xfrags = [&](const Flow16Layer& ly, bf16x8 (&xb)[2]) {
+    const UT_GLOBAL16 bf16_t* xpg = g16(ly.xpg);
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks)
+      xb[ks] = *(const UT_GLOBAL16 bf16x8*)(xpg + (((long long)blockIdx.x * 2 + ks) * 64 + lane) * 8);
+  };
+  float sv[2], xsv[2];
+  bf16x8 xb[2];
+  saved(layers[L - 1], sv, xsv);
+  xfrags(layers[L - 1], xb);
+  for (int e = tid; e < R16 * P; e += THREADS16) Gs[e] = g_q[r0 * P + e];
+  lds_barrier16();
+  // one layer (l = L - 1 - li) with weights `g`; afterwards `g` receives layer l - 2's weights
+  auto layer = [&](int li, Bwd16Regs& g) {
+    const int l = L - 1 - li;
+    const Flow16Layer ly = layers[l];
+    const bool next = l > 0;
+    const Flow16Layer& nx = layers[next ? l - 1 : l];
+    const Flow16Layer& nx2 = layers[l > 1 ? l - 2 : l];
+    const int np = ly.np, na = ly.na, par = ly.par;
+    // ---- coupling backward: x_a <- x_a e^s + t, log-determinant + s, s = c tanh(s_raw / c)
+    float d[4];  // (ds_raw, dt) of j0, j0 + 1, zero past na
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      const int j = j0 + e;
+      const bool valid = j < na;
+      const int ak = valid ? 2 * j + 1 - par : 0;
+      const float ga = valid ? Gs[lr * P + ak] : 0.0f;
+      const float r = sv[e] * (1.0f / c);
+      d[2 * e] = valid ? (ga * xsv[e] + 1.0f) * (1.0f - r * r) : 0.0f;
+      d[2 * e + 1] = ga;
+      if (valid) Gs[lr * P + ak] = ga * expf(sv[e]);
+    }
+    {
+      uint2 pk;
+      pk.x = pack2(d[0], d[1]);
+      pk.y = pack2(d[2], d[3]);
+      *(uint2*)(dO + lr * OS16 + 16 * w + 4 * lg) = pk;
+    }
+    float sv_n[2], xsv_n[2];
+    if (next) saved(nx, sv_n, xsv_n);
+    lds_barrier16();
+    // ---- dh^T = W2'^T dO^T; U recomputed from the forward's passive inputs; dU = dh (1 - h^2)
+    bf16x8 ob[4];
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) ob[ks] = *(const bf16x8*)(dO + lr * OS16 + 32 * ks + 8 * lg);
+    f32x4 dh[2], u[2];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      dh[t] = (f32x4){0.0f, 0.0f, 0.0f, 0.0f};
+      u[t] = (f32x4){0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+      for (int ks = 0; ks < 4; ++ks) dh[t] = mfma16(g.a2t[t][ks], ob[ks], dh[t]);
+#pragma unroll
+      for (int ks = 0; ks < 2; ++ks) u[t] = mfma16(g.a1[t][ks], xb[ks], u[t]);
+    }
+    bf16x8 xb_n[2];
+    __builtin_amdgcn_sched_barrier(0);
+    if (next) xfrags(nx, xb_n);
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      float du[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const float hv = flow_tanh(u[t][r] + g.b1[t][r]);
+        du[r] = dh[t][r] * (1.0f - hv * hv);
+      }
+      uint2 pk;
+      pk.x = pack2(du[0], du[1]);
+      pk.y = pack2(du[2], du[3]);
+      *(uint2*)(DU + lr * HS16 + 32 * w + 16 * t + 4 * lg) = pk;
+    }
+    lds_barrier16();
+    // ---- dX_p^T = W1^T dU^T: passive coordinates 16 w + 4 lg .. + 3 of row lr
+    if (16 * wu < np) {
+      f32x4 gp = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+      for (int k0 = 0; k0 < 8; k0 += 4) {  // B fragments four at a time (register pressure)
+        bf16x8 ub[4];
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks) ub[ks] = *(const bf16x8*)(DU + lr * HS16 + 32 * (k0 + ks) + 8 * lg);
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks) gp = mfma16(g.a1t[k0 + ks], ub[ks], gp);
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int k = 16 * w + 4 * lg + r;
+        if (k < np) Gs[lr * P + 2 * k + par] += gp[r];
+      }
+    }
+    if (l > 1) bwd16_load(nx2, g);
+    lds_barrier16();
+    if (next) {
+#pragma unroll
+      for (int e = 0; e < 2; ++e) {
+        sv[e] = sv_n[e];
+        xsv[e] = xsv_n[e];
+        xb[e] = xb_n[e];
+      }
+    }
+  };
+  if (LT > 0) {
+#pragma unroll
+    for (int li = 0; li < LT; li += 2) {
+      layer(li, ga);
+      if (li + 1 < LT) layer(li + 1, gb);
+    }
+  } else {
+    for (int li = 0; li < L; li += 2) {
+      layer(li, ga);
+      if (li + 1 < L) layer(li + 1, gb);
+    }
+  }
+  // base backward: g_z = g_x e^ls
+  for (int e0 = tid; e0 < R16 * P; e0 += 4 * THREADS16) {
+    float lv[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) lv[k] = prm[P + min(e0 + THREADS16 * k, R16 * P - 1) % P];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int e = e0 + THREADS16 * k;
+      if (e < R16 * P) gz[r0 * P + e] = Gs[e] * expf(lv[k]);
+    }
+  }
+}
+
+typedef void (*Fwd16Fn)(const float*, const float*, const float*, float*, float*, int, const Flow16Layer*, int, float,
+                        float*, float*);
+typedef void (*Bwd16Fn)(const float*, const float*, float*, int, const Flow16Layer*, int, float);
+Fwd16Fn fwd16_kernel(int L) {
+  switch (L) {
+    case 2: return k_flow16_fwd<2>;
+    case 4: return k_flow16_fwd<4>;
+    case 6: return k_flow16_fwd<6>;
+    case 8: return k_flow16_fwd<8>;
+    default: return k_flow16_fwd<0>;
+  }
+}
+Bwd16Fn bwd16_kernel(int L) {
+  switch (L) {
+    case 2: return k_flow16_bwd<2>;
+    case 4: return k_flow16_bwd<4>;
+    case 6: return k_flow16_bwd<6>;
+    case 8: return k_flow16_bwd<8>;
+    default: return k_flow16_bwd<0>;
+  }
+}
+size_t fwd16_lds(int P) { return (size_t)R16 * P * 4 + (size_t)R16 * XPS16 * 2 + (size_t)R16 * HS16 * 2 + NW16 * 16 * 4; }
+size_t bwd16_lds(int P) { return (size_t)R16 * P * 4 + (size_t)R16 * OS16 * 2 + (size_t)R16 * HS16 * 2; }
+
+}  // namespace
+
+bool flow16_eligible(const Flow& f) {
+  if (!f.fused || f.factor || f.L < 1 || f.H != HID16 || f.P > 2 * KP16) return false;
+  for (const FlowLayer& ly : f.layers)
+    if (ly.np > KP16 || ly.na > KP16 || ly.np < 1 || ly.na < 1) return false;
+  return true;
+}
+
+hipError_t flow16_sync_weights(Flow& f, hipStream_t st) {
+  for (const FlowLayer& ly : f.layers) {
+    const int n = W1F16 + W2F16 + W2TF16 + W1TF16;
+    hipLaunchKernelGGL(k_frag16, dim3((n + 255) / 256), dim3(256), 0, st, f.params + ly.oW1, FLOW_PAD,
+                       f.params + ly.oW2, ly.np, ly.na, ly.W1f, ly.W2f, ly.W2Tf, ly.W1Tf);
+  }
+  hipLaunchKernelGGL(k_ldbase16, dim3(1), dim3(64), 0, st, f.params + f.P, f.P, f.ldbase);
+  return hipGetLastError();
+}
+
+hipError_t flow16_forward(Flow& f, const float* z, float* x, float* q, float* logdet, float* logdet_out, int B,
+                          hipStream_t st) {
+  if (B % R16) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(fwd16_kernel(f.L), dim3(B / R16), dim3(THREADS16), fwd16_lds(f.P), st, z, f.params, f.ldbase, x,
+                     q, f.P, f.d16, f.L, f.clamp, logdet, logdet_out);
+  return hipGetLastError();
+}
+
+hipError_t flow16_backward(Flow& f, const float* g_q, float* g_z, int B, hipStream_t st) {
+  if (B % R16) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(bwd16_kernel(f.L), dim3(B / R16), dim3(THREADS16), bwd16_lds(f.P), st, g_q, f.params, g_z, f.P,
+                     f.d16, f.L, f.clamp);
+  return hipGetLastError();
+}
+
+}  // namespace ut
