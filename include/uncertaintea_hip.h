@@ -279,6 +279,21 @@ int ut_plan_num_params(const ut_plan* plan, int32_t* num_params);
 int ut_workspace_create(const ut_plan* plan, int32_t num_chains, int32_t precision,
                         ut_workspace** out);
 int ut_workspace_destroy(ut_workspace* ws);
+/* Chain layout of the density kernel and the persistent samplers: wavefronts per
+ * chain.  0 (the default) chooses automatically: with few chains over many rows the
+ * logistic plan splits each chain's observations over a workgroup of 2, 4 or 8
+ * wavefronts (the observation-tiled gradient inside the persistent kernel that the
+ * reference defers, docs/persistent-nuts.md:406-414; its tiling rule for the
+ * batched gradient is src/device/workspace.jl:123-132); 1, 2, 4, 8 force a layout.
+ * The per-wavefront totals are added in wavefront order, so each layout is its own
+ * deterministic summation order.  waves > 1 on a plan other than the logistic one
+ * is UT_E_UNSUPPORTED.  A NUTS run whose checkpoint stacks (per wavefront) would
+ * exceed the LDS at its max_tree_depth fails with UT_E_ARG for an explicit layout
+ * and takes fewer wavefronts for the automatic one.  ut_workspace_chain_waves
+ * reports the layout in effect (after a sampler run: that run's).  No reference
+ * counterpart as an entry point.                                               */
+int ut_workspace_set_chain_waves(ut_workspace* ws, int32_t waves);
+int ut_workspace_chain_waves(const ut_workspace* ws, int32_t* waves);
 
 /* ---- density -------------------------------------------------------------------
  * params: P x C unconstrained parameters in the workspace precision (float or

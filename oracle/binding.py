@@ -30,6 +30,7 @@ def _load():
         "orc_math_f32": (None, [ct.c_int, vp, vp, i64]),
         "orc_math_f64": (None, [ct.c_int, vp, vp, i64]),
         "orc_model_logistic": (vp, [vp, vp, i64, i32, dbl, dbl]),
+        "orc_model_set_tiles": (i32, [vp, i32]),
         "orc_model_hier": (vp, [vp, vp, vp, i32, dbl, dbl, dbl]),
         "orc_model_funnel": (vp, [i32, dbl]),
         "orc_model_gp": (vp, [vp, vp, i32, i32, dbl, dbl, dbl, dbl]),
@@ -120,7 +121,7 @@ def math_f64(fn: str, x):
 class OracleModel:
     """Oracle twin of a uncertaintea_amd model (same data, same layout)."""
 
-    def __init__(self, model):
+    def __init__(self, model, waves_per_chain: int = 1):
         L = _load()
         self.model = model
         kind = type(model).__name__
@@ -147,6 +148,8 @@ class OracleModel:
         else:
             raise TypeError(kind)
         self.P = L.orc_model_num_params(self.h)
+        if waves_per_chain != 1 and L.orc_model_set_tiles(self.h, int(waves_per_chain)) != 0:
+            raise ValueError(f"waves_per_chain {waves_per_chain}: logistic models with 1, 2, 4 or 8 only")
 
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:

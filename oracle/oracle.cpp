@@ -64,6 +64,7 @@ struct Model {
   long long n = 0;
   int D = 0;
   double loc = 0, scale = 1;
+  int tiles = 1;  // wavefronts per chain (the workgroup-per-chain summation order)
   // hierarchical
   std::vector<float> hy;
   std::vector<int> offsets;
@@ -338,33 +339,48 @@ T twin_eval(const Model& m, const T* q, T* g, bool& ok) {
     return total;
   }
   if (m.kind == KIND_LOGISTIC) {
+    // W = m.tiles wavefronts per chain (ut_models.h TiledLogisticModel; W = 1 is
+    // LogisticModel): wavefront w, lane l takes rows 64 (w + W k) + l; wavefront 0's
+    // lanes add the prior terms; per-wavefront butterfly totals are added in
+    // wavefront order.
     const int DMAX = K * WAVE;
-    std::vector<T> beta(DMAX), gp((size_t)DMAX * WAVE, T(0)), lpl(WAVE, T(0));
+    const int NW = m.tiles;
+    std::vector<T> beta(DMAX), gp((size_t)DMAX * WAVE), lpl(WAVE);
     for (int d = 0; d < DMAX; ++d) beta[d] = q[(d / WAVE) * WAVE + d % WAVE];
-    for (int l = 0; l < WAVE; ++l) {
-      T lp = T(0);
-      T* gpl = &gp[(size_t)l * DMAX];
-      for (long long i = l; i < m.n; i += WAVE) {
-        T eta = T(0);
-        for (int d = 0; d < m.D; ++d) eta = eta + T(m.x[i * m.D + d]) * beta[d];
-        const T yi = T(m.y[i]);
-        lp = lp + (yi * eta - tl1pe<T>(eta));
-        const T r = yi - tlogistic<T>(eta);
-        for (int d = 0; d < m.D; ++d) gpl[d] = gpl[d] + T(m.x[i * m.D + d]) * r;
-      }
-      for (int k = 0; k < K; ++k) {
-        if (m.api(l, k) >= 0) {
-          const T z = (q[k * WAVE + l] - T(m.loc)) / T(m.scale);
-          lp = lp - (z * z) / T(2);
-        }
-      }
-      lpl[l] = lp;
-    }
     std::vector<T> gl((size_t)K * WAVE, T(0));
-    T col[WAVE];
-    for (int d = 0; d < m.D; ++d) {
-      for (int l = 0; l < WAVE; ++l) col[l] = gp[(size_t)l * DMAX + d];
-      gl[(d / WAVE) * WAVE + d % WAVE] = bfly(col);
+    T lp_total = T(0);
+    for (int w = 0; w < NW; ++w) {
+      std::fill(gp.begin(), gp.end(), T(0));
+      for (int l = 0; l < WAVE; ++l) {
+        T lp = T(0);
+        T* gpl = &gp[(size_t)l * DMAX];
+        for (long long i = (long long)w * WAVE + l; i < m.n; i += (long long)NW * WAVE) {
+          T eta = T(0);
+          for (int d = 0; d < m.D; ++d) eta = eta + T(m.x[i * m.D + d]) * beta[d];
+          const T yi = T(m.y[i]);
+          lp = lp + (yi * eta - tl1pe<T>(eta));
+          const T r = yi - tlogistic<T>(eta);
+          for (int d = 0; d < m.D; ++d) gpl[d] = gpl[d] + T(m.x[i * m.D + d]) * r;
+        }
+        if (w == 0) {
+          for (int k = 0; k < K; ++k) {
+            if (m.api(l, k) >= 0) {
+              const T z = (q[k * WAVE + l] - T(m.loc)) / T(m.scale);
+              lp = lp - (z * z) / T(2);
+            }
+          }
+        }
+        lpl[l] = lp;
+      }
+      T col[WAVE];
+      for (int d = 0; d < m.D; ++d) {
+        for (int l = 0; l < WAVE; ++l) col[l] = gp[(size_t)l * DMAX + d];
+        const T tot = bfly(col);
+        T& dst = gl[(d / WAVE) * WAVE + d % WAVE];
+        dst = w == 0 ? tot : dst + tot;
+      }
+      const T tl = bfly(lpl.data());
+      lp_total = w == 0 ? tl : lp_total + tl;
     }
     for (int k = 0; k < K; ++k)
       for (int l = 0; l < WAVE; ++l) {
@@ -377,7 +393,7 @@ T twin_eval(const Model& m, const T* q, T* g, bool& ok) {
         }
         fin = fin && tfin(g[i]);
       }
-    const T total = bfly(lpl.data()) + T(m.total_const);
+    const T total = lp_total + T(m.total_const);
     ok = fin && tfin(total);
     return total;
   }
@@ -2096,6 +2112,14 @@ void* orc_model_logistic(const float* x, const float* y, int64_t n, int32_t d, d
   m->scale = scale;
   finish_twin_layout(*m);
   return m;
+}
+
+// Wavefronts per chain of the logistic model's device layout (1, 2, 4, 8).
+int32_t orc_model_set_tiles(void* h, int32_t tiles) {
+  auto* m = (Model*)h;
+  if (m->kind != KIND_LOGISTIC || (tiles != 1 && tiles != 2 && tiles != 4 && tiles != 8)) return -1;
+  m->tiles = tiles;
+  return 0;
 }
 
 void* orc_model_hier(const float* y, const int32_t* offsets, const double* sigma, int32_t J,

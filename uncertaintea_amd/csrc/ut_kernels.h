@@ -56,6 +56,9 @@ struct DevModelDesc {
   long long n = 0;
   int D = 0;
   double loc = 0, scale = 1;
+  // wavefronts per chain in the persistent samplers and the density kernel (1, or
+  // 2, 4, 8 for the logistic model: one chain's rows split over its workgroup)
+  int waves = 1;
   // funnel
   int dim = 0;
   double v_scale = 1;
@@ -65,6 +68,20 @@ struct DevModelDesc {
   PlanView plan;
   const ut_plan_desc* host_plan = nullptr;
 };
+
+// LDS of one wavefront's NUTS checkpoint stack (positions and momenta of the
+// dyadic checkpoints, max_depth + 1 of each).
+inline size_t nuts_checkpoint_bytes(int K, int max_depth) {
+  return (size_t)(max_depth + 1) * 2 * K * WAVE * sizeof(float);
+}
+// LDS of the persistent NUTS kernel for a desc's model and chain layout (the
+// checkpoint stacks of every wavefront plus a workgroup-per-chain exchange area).
+inline size_t nuts_lds_bytes(const DevModelDesc& d, int max_depth) {
+  const size_t w = d.waves > 1 ? (size_t)d.waves : 1;
+  size_t b = w * nuts_checkpoint_bytes(d.K, max_depth);
+  if (d.waves > 1) b = (b + 15) / 16 * 16 + w * ((size_t)d.K * WAVE + 1) * sizeof(float);
+  return b;
+}
 
 // Device pointers of the chain state (api order, P x C column-major).
 struct ChainState {

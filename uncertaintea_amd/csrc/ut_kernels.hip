@@ -1,6 +1,8 @@
 // ut_kernels.hip -- gfx950 kernels of the batched density and batched MCMC path.
 //
-// One 64-lane wavefront (one workgroup) per chain.  The whole per-chain state of a
+// One 64-lane wavefront (one workgroup) per chain; the logistic model can also split
+// one chain's rows over the W wavefronts of its workgroup, each of which then runs the
+// same transition on the same values.  The whole per-chain state of a
 // sampler iteration -- position, momentum, gradient, the NUTS frontiers and proposal
 // -- lives in registers; the NUTS dyadic checkpoint stack lives in LDS.  Kernels run
 // a whole range of sampler iterations per launch, so within a range the host is
@@ -16,9 +18,16 @@
 namespace ut {
 
 // ------------------------------------------------------------------ model dispatch
-template <int K>
-static LogisticModel<K> make_logistic(const DevModelDesc& d) {
-  LogisticModel<K> m;
+template <int K, int DB = K * WAVE>
+static LogisticModel<K, DB> make_logistic(const DevModelDesc& d) {
+  LogisticModel<K, DB> m;
+  m.xT_f = d.xT_f; m.xT_d = d.xT_d; m.y_f = d.yv_f; m.y_d = d.yv_d;
+  m.n = d.n; m.D = d.D; m.loc = d.loc; m.scale = d.scale; m.total_const = d.total_const;
+  return m;
+}
+template <int K, int DB, int W>
+static TiledLogisticModel<K, DB, W> make_logistic_tiled(const DevModelDesc& d) {
+  TiledLogisticModel<K, DB, W> m;
   m.xT_f = d.xT_f; m.xT_d = d.xT_d; m.y_f = d.yv_f; m.y_d = d.yv_d;
   m.n = d.n; m.D = d.D; m.loc = d.loc; m.scale = d.scale; m.total_const = d.total_const;
   return m;
@@ -74,6 +83,8 @@ static hipError_t dispatch(const DevModelDesc& d, F&& f) {
       if (d.max_n <= 64) return f(make_hier<64>(d));
       return f(make_hier<0>(d));
     case MODEL_LOGISTIC:
+      // up to 32 covariates: register arrays of 32 and the next row prefetched
+      if (d.K == 1 && d.D <= 32) return f(make_logistic<1, 32>(d));
       if (d.K == 1) return f(make_logistic<1>(d));
       if (d.K == 2) return f(make_logistic<2>(d));
       if (d.K == 4) return f(make_logistic<4>(d));
@@ -91,6 +102,31 @@ static hipError_t dispatch(const DevModelDesc& d, F&& f) {
     default:
       return hipErrorInvalidValue;
   }
+}
+
+// Workgroup-per-chain instances (d.waves wavefronts per chain): the logistic model.
+template <int W, class F>
+static hipError_t dispatch_tiled_w(const DevModelDesc& d, F&& f) {
+  if (d.K == 1 && d.D <= 32) return f(make_logistic_tiled<1, 32, W>(d));
+  if (d.K == 1) return f(make_logistic_tiled<1, 64, W>(d));
+  if (d.K == 2) return f(make_logistic_tiled<2, 128, W>(d));
+  if (d.K == 4) return f(make_logistic_tiled<4, 256, W>(d));
+  return hipErrorInvalidValue;
+}
+template <class F>
+static hipError_t dispatch_tiled(const DevModelDesc& d, F&& f) {
+  if (d.kind != MODEL_LOGISTIC) return hipErrorInvalidValue;
+  switch (d.waves) {
+    case 2: return dispatch_tiled_w<2>(d, f);
+    case 4: return dispatch_tiled_w<4>(d, f);
+    case 8: return dispatch_tiled_w<8>(d, f);
+    default: return hipErrorInvalidValue;
+  }
+}
+// The persistent samplers and the density kernel take either.
+template <class F>
+static hipError_t dispatch_chain(const DevModelDesc& d, F&& f) {
+  return d.waves > 1 ? dispatch_tiled(d, f) : dispatch(d, f);
 }
 
 // A model bound to its per-lane data (prepared once per launch).
@@ -505,14 +541,15 @@ __device__ void hmc_transition(const M& m, const float (&im)[M::K], float h, uin
 
 // ------------------------------------------------------------------ kernels
 template <class M, class T>
-__global__ __launch_bounds__(64) void k_density(M model, const T* __restrict__ params,
-                                                T* __restrict__ values, T* __restrict__ grads,
-                                                int P) {
+__global__ __launch_bounds__(64 * WavesOf<M>::value) void k_density(M model, const T* __restrict__ params,
+                                                                    T* __restrict__ values,
+                                                                    T* __restrict__ grads, int P) {
   constexpr int K = M::K;
   typename M::template Lane<T> ld;
   model.prepare(ld);
   const Bound<M, T> m{model, ld};
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & (WAVE - 1);
+  const bool lead = threadIdx.x < WAVE;  // wavefront 0 writes the results
   const long long c = blockIdx.x;
   T q[K], g[K];
 #pragma unroll
@@ -522,14 +559,14 @@ __global__ __launch_bounds__(64) void k_density(M model, const T* __restrict__ p
   }
   bool ok;
   const T v = m.eval(q, g, ok);
-  if (grads) {
+  if (grads && lead) {
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const int a = m.api(lane, k);
       if (a >= 0) grads[c * P + a] = g[k];
     }
   }
-  if (lane == 0) values[c] = v;
+  if (threadIdx.x == 0) values[c] = v;
 }
 
 template <class M>
@@ -547,6 +584,7 @@ __device__ inline void load_chain(const M& m, const ChainState& st, int P, long 
 
 template <class M>
 __device__ inline void store_vec(const M& m, float* dst, const float (&v)[M::K]) {
+  if (threadIdx.x >= WAVE) return;  // wavefront 0 writes a workgroup-per-chain state
   const int lane = threadIdx.x & (WAVE - 1);
 #pragma unroll
   for (int k = 0; k < M::K; ++k) {
@@ -557,11 +595,13 @@ __device__ inline void store_vec(const M& m, float* dst, const float (&v)[M::K])
 
 // Runs iterations [it_begin, it_end] (1-based, inclusive) of batched NUTS for one
 // chain per workgroup: transition, accept, and -- during warmup -- the chain's
-// dual-averaging update and its warmup-history record for the pooled mass.
+// dual-averaging update and its warmup-history record for the pooled mass.  Each
+// wavefront of a workgroup-per-chain model keeps its own checkpoint stack.
 template <class M>
-__global__ __launch_bounds__(64) void k_nuts(M model, ChainState st, SamplerParams sp, int it_begin,
-                                             int it_end) {
-  extern __shared__ float ck[];
+__global__ __launch_bounds__(64 * WavesOf<M>::value) void k_nuts(M model, ChainState st, SamplerParams sp,
+                                                                 int it_begin, int it_end) {
+  extern __shared__ float ck_all[];
+  float* ck = ck_all + (size_t)(threadIdx.x >> 6) * ((sp.max_depth + 1) * 2 * M::K * WAVE);
   constexpr int K = M::K;
   typename M::template Lane<float> ld;
   model.prepare(ld);
@@ -628,8 +668,8 @@ __global__ __launch_bounds__(64) void k_nuts(M model, ChainState st, SamplerPara
 }
 
 template <class M>
-__global__ __launch_bounds__(64) void k_hmc(M model, ChainState st, SamplerParams sp, int it_begin,
-                                            int it_end) {
+__global__ __launch_bounds__(64 * WavesOf<M>::value) void k_hmc(M model, ChainState st, SamplerParams sp,
+                                                                int it_begin, int it_end) {
   constexpr int K = M::K;
   typename M::template Lane<float> ld;
   model.prepare(ld);
@@ -699,8 +739,9 @@ __global__ __launch_bounds__(64) void k_hmc(M model, ChainState st, SamplerParam
 // Afterwards the chain's dual-averaging state restarts from its step
 // (adaptation.jl:1-13).
 template <class M>
-__global__ __launch_bounds__(64) void k_step_search(M model, ChainState st, int P, uint32_t itc,
-                                                    int do_search, double target) {
+__global__ __launch_bounds__(64 * WavesOf<M>::value) void k_step_search(M model, ChainState st, int P,
+                                                                        uint32_t itc, int do_search,
+                                                                        double target) {
   typename M::template Lane<float> ld;
   model.prepare(ld);
   const Bound<M, float> m{model, ld};
@@ -925,31 +966,34 @@ __global__ void k_philox_fill(uint32_t chain, uint32_t iteration, uint32_t strea
 // ------------------------------------------------------------------ launchers
 hipError_t launch_density_f32(const DevModelDesc& d, const float* params, float* values,
                               float* grads, int C, hipStream_t stream) {
-  return dispatch(d, [&](auto m) {
+  return dispatch_chain(d, [&](auto m) {
     const size_t lds = model_lds<float>(m, 0, (const void*)k_density<decltype(m), float>);
-    hipLaunchKernelGGL((k_density<decltype(m), float>), dim3(C), dim3(WAVE), lds, stream, m, params,
-                       values, grads, d.P);
+    hipLaunchKernelGGL((k_density<decltype(m), float>), dim3(C), dim3(WAVE * WavesOf<decltype(m)>::value),
+                       lds, stream, m, params, values, grads, d.P);
     return hipGetLastError();
   });
 }
 
 hipError_t launch_density_f64(const DevModelDesc& d, const double* params, double* values,
                               double* grads, int C, hipStream_t stream) {
-  return dispatch(d, [&](auto m) {
+  return dispatch_chain(d, [&](auto m) {
     const size_t lds = model_lds<double>(m, 0, (const void*)k_density<decltype(m), double>);
-    hipLaunchKernelGGL((k_density<decltype(m), double>), dim3(C), dim3(WAVE), lds, stream, m,
-                       params, values, grads, d.P);
+    hipLaunchKernelGGL((k_density<decltype(m), double>), dim3(C), dim3(WAVE * WavesOf<decltype(m)>::value),
+                       lds, stream, m, params, values, grads, d.P);
     return hipGetLastError();
   });
 }
 
 hipError_t launch_nuts(const DevModelDesc& d, const ChainState& st, const SamplerParams& sp,
                        int it_begin, int it_end, hipStream_t stream) {
-  return dispatch(d, [&](auto m) {
+  return dispatch_chain(d, [&](auto m) {
     constexpr int K = decltype(m)::K;
-    const size_t ck = (size_t)(sp.max_depth + 1) * 2 * K * WAVE * sizeof(float);
+    constexpr int NW = WavesOf<decltype(m)>::value;
+    const size_t ck = (size_t)NW * nuts_checkpoint_bytes(K, sp.max_depth);
     const size_t lds = model_lds<float>(m, ck, (const void*)k_nuts<decltype(m)>);
-    hipLaunchKernelGGL((k_nuts<decltype(m)>), dim3(sp.C), dim3(WAVE), lds, stream, m, st, sp,
+    if (lds > 65536) (void)hipFuncSetAttribute((const void*)k_nuts<decltype(m)>,
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    hipLaunchKernelGGL((k_nuts<decltype(m)>), dim3(sp.C), dim3(WAVE * NW), lds, stream, m, st, sp,
                        it_begin, it_end);
     return hipGetLastError();
   });
@@ -957,10 +1001,10 @@ hipError_t launch_nuts(const DevModelDesc& d, const ChainState& st, const Sample
 
 hipError_t launch_hmc(const DevModelDesc& d, const ChainState& st, const SamplerParams& sp,
                       int it_begin, int it_end, hipStream_t stream) {
-  return dispatch(d, [&](auto m) {
+  return dispatch_chain(d, [&](auto m) {
     const size_t lds = model_lds<float>(m, 0, (const void*)k_hmc<decltype(m)>);
-    hipLaunchKernelGGL((k_hmc<decltype(m)>), dim3(sp.C), dim3(WAVE), lds, stream, m, st, sp,
-                       it_begin, it_end);
+    hipLaunchKernelGGL((k_hmc<decltype(m)>), dim3(sp.C), dim3(WAVE * WavesOf<decltype(m)>::value), lds,
+                       stream, m, st, sp, it_begin, it_end);
     return hipGetLastError();
   });
 }
@@ -1543,10 +1587,10 @@ hipError_t launch_mn_search_step(const DevModelDesc& d, const ChainState& st, co
 
 hipError_t launch_step_search(const DevModelDesc& d, const ChainState& st, int C,
                               uint32_t itc, int do_search, double target, hipStream_t stream) {
-  return dispatch(d, [&](auto m) {
+  return dispatch_chain(d, [&](auto m) {
     const size_t lds = model_lds<float>(m, 0, (const void*)k_step_search<decltype(m)>);
-    hipLaunchKernelGGL((k_step_search<decltype(m)>), dim3(C), dim3(WAVE), lds, stream, m, st, d.P,
-                       itc, do_search, target);
+    hipLaunchKernelGGL((k_step_search<decltype(m)>), dim3(C), dim3(WAVE * WavesOf<decltype(m)>::value), lds,
+                       stream, m, st, d.P, itc, do_search, target);
     return hipGetLastError();
   });
 }

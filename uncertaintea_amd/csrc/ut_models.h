@@ -88,10 +88,10 @@ template <> __device__ inline bool t_isfinite<double>(double x) { return isfinit
 // precision T reads the matching copy.
 template <class T> struct Pick;
 template <> struct Pick<float> {
-  __device__ static float get(const float* f, const double*, int i) { return f[i]; }
+  __device__ static float get(const float* f, const double*, long long i) { return f[i]; }
 };
 template <> struct Pick<double> {
-  __device__ static double get(const float*, const double* d, int i) { return d[i]; }
+  __device__ static double get(const float*, const double* d, long long i) { return d[i]; }
 };
 
 // Models with LDS of their own (the plan interpreter) take a byte offset into the
@@ -232,10 +232,84 @@ struct HierModel : NoModelLds {
 // i = l, l + 64, ...; X is stored feature-major xT[d * N + i] so a row step of the
 // 64 lanes reads 64 consecutive words of each feature.  The D gradient partials are
 // folded with one butterfly each and lane d keeps feature d's total.
-template <int KK>
+//
+// One lane's rows (logistic_rows): per row eta = ((0 + x_0 b_0) + x_1 b_1) + ...,
+// lp += y eta - log1p_exp(eta), r = y - logistic(eta), gp_d += x_d r.  DB (>= D) is
+// the compile-time bound of the register arrays.  The covariate loads are issued
+// unconditionally, all of a row (or of a 32-feature chunk) together, with the
+// feature index clamped to D - 1 and the arithmetic of features >= D skipped, so
+// their latencies overlap instead of one wait per feature; with DB <= 32 the next
+// row's covariates are loaded while the current row is scored (the row index
+// clamped to n - 1 on the last step).
+template <int DB, class T>
+__device__ inline void logistic_rows(const float* xT_f, const double* xT_d, const float* y_f,
+                                     const double* y_d, long long n, int D, long long first,
+                                     long long step, const T (&beta)[DB], T (&gp)[DB], T& lp) {
+  if constexpr (DB <= 32) {
+    T xv[DB];
+    T yv = T(0);
+    auto load = [&](long long r) {
+#pragma unroll
+      for (int d = 0; d < DB; ++d) xv[d] = Pick<T>::get(xT_f, xT_d, (long long)(d < D ? d : D - 1) * n + r);
+      yv = Pick<T>::get(y_f, y_d, r);
+    };
+    if (first < n) load(first);
+    for (long long i = first; i < n; i += step) {
+      T xc[DB];
+#pragma unroll
+      for (int d = 0; d < DB; ++d) xc[d] = xv[d];
+      const T yi = yv;
+      load(i + step < n ? i + step : n - 1);
+      T eta = T(0);
+#pragma unroll
+      for (int d = 0; d < DB; ++d)
+        if (d < D) eta = eta + xc[d] * beta[d];
+      lp = lp + (yi * eta - t_log1p_exp<T>(eta));
+      const T r = yi - t_logistic<T>(eta);
+#pragma unroll
+      for (int d = 0; d < DB; ++d)
+        if (d < D) gp[d] = gp[d] + xc[d] * r;
+    }
+  } else {
+    constexpr int CH = 32;
+    for (long long i = first; i < n; i += step) {
+      const T yi = Pick<T>::get(y_f, y_d, i);
+      T eta = T(0);
+#pragma unroll
+      for (int c = 0; c < DB; c += CH) {
+        if (c < D) {
+          T xv[CH];
+#pragma unroll
+          for (int j = 0; j < CH; ++j)
+            xv[j] = Pick<T>::get(xT_f, xT_d, (long long)(c + j < D ? c + j : D - 1) * n + i);
+#pragma unroll
+          for (int j = 0; j < CH; ++j)
+            if (c + j < D) eta = eta + xv[j] * beta[c + j];
+        }
+      }
+      lp = lp + (yi * eta - t_log1p_exp<T>(eta));
+      const T r = yi - t_logistic<T>(eta);
+#pragma unroll
+      for (int c = 0; c < DB; c += CH) {
+        if (c < D) {
+          T xv[CH];
+#pragma unroll
+          for (int j = 0; j < CH; ++j)
+            xv[j] = Pick<T>::get(xT_f, xT_d, (long long)(c + j < D ? c + j : D - 1) * n + i);
+#pragma unroll
+          for (int j = 0; j < CH; ++j)
+            if (c + j < D) gp[c + j] = gp[c + j] + xv[j] * r;
+        }
+      }
+    }
+  }
+}
+
+template <int KK, int DBB = KK * WAVE>
 struct LogisticModel : NoModelLds {
   static constexpr int K = KK;
-  static constexpr int DMAX = KK * WAVE;
+  static constexpr int DB = DBB;  // D <= DB <= 64 K
+  static_assert(DB <= KK * WAVE, "feature bound beyond the slots");
   const float* xT_f;
   const double* xT_d;
   const float* y_f;
@@ -257,26 +331,15 @@ struct LogisticModel : NoModelLds {
   template <class T>
   __device__ T eval(const Lane<T>&, const T (&q)[K], T (&g)[K], bool& ok) const {
     const int lane = threadIdx.x & (WAVE - 1);
-    T beta[DMAX];
-    T gp[DMAX];
+    T beta[DB];
+    T gp[DB];
 #pragma unroll
-    for (int d = 0; d < DMAX; ++d) {
+    for (int d = 0; d < DB; ++d) {
       beta[d] = wave_bcast(q[d / WAVE], d % WAVE);
       gp[d] = T(0);
     }
     T lp = T(0);
-    for (long long i = lane; i < n; i += WAVE) {
-      T eta = T(0);
-#pragma unroll
-      for (int d = 0; d < DMAX; ++d)
-        if (d < D) eta = eta + Pick<T>::get(xT_f, xT_d, (int)(d * n + i)) * beta[d];
-      const T yi = Pick<T>::get(y_f, y_d, (int)i);
-      lp = lp + (yi * eta - t_log1p_exp<T>(eta));
-      const T r = yi - t_logistic<T>(eta);
-#pragma unroll
-      for (int d = 0; d < DMAX; ++d)
-        if (d < D) gp[d] = gp[d] + Pick<T>::get(xT_f, xT_d, (int)(d * n + i)) * r;
-    }
+    logistic_rows<DB, T>(xT_f, xT_d, y_f, y_d, n, D, lane, WAVE, beta, gp, lp);
     // prior terms for this lane's slots
 #pragma unroll
     for (int k = 0; k < K; ++k) {
@@ -289,7 +352,7 @@ struct LogisticModel : NoModelLds {
 #pragma unroll
     for (int k = 0; k < K; ++k) gl[k] = T(0);
 #pragma unroll
-    for (int d = 0; d < DMAX; ++d) {
+    for (int d = 0; d < DB; ++d) {
       if (d < D) {
         const T tot = wave_sum(gp[d]);
         if (lane == d % WAVE) gl[d / WAVE] = tot;
@@ -307,6 +370,112 @@ struct LogisticModel : NoModelLds {
       fin = fin && t_isfinite(g[k]);
     }
     const T total = wave_sum(lp) + T(total_const);
+    ok = wave_all(fin && t_isfinite(total));
+    return total;
+  }
+};
+
+// ----------------------------------------------------------------------------------
+// Logistic regression with one chain's observations split over the W wavefronts of
+// its workgroup (W = 2, 4, 8), for few chains over many rows.  Wavefront w, lane l
+// scores rows i = 64 (w + W k) + l, k = 0, 1, ...; wavefront 0's lanes add the prior
+// terms of their slots to the partial log density after the rows, as LogisticModel
+// does.  Each wavefront folds its partials with wave_sum into one total per
+// feature (and one log-density total), writes them to LDS, and after a workgroup
+// barrier every wavefront adds the W totals in wavefront order
+// ((T_0 + T_1) + T_2) + ... and holds the same values; a second barrier frees the
+// exchange area for the next evaluation.  With W = 1 this is LogisticModel's
+// arithmetic.  The sampler kernels run the same transition in every wavefront on
+// the same values, so all of them reach eval() equally often; wavefront 0 writes
+// the results.
+template <class M, class = void>
+struct WavesOf {
+  static constexpr int value = 1;
+};
+template <class M>
+struct WavesOf<M, decltype((void)M::WAVES)> {
+  static constexpr int value = M::WAVES;
+};
+
+template <int KK, int DBB, int W>
+struct TiledLogisticModel : LogisticModel<KK, DBB> {
+  static constexpr int K = KK;
+  static constexpr int DB = DBB;
+  static constexpr int WAVES = W;
+  static constexpr int DMAX = KK * WAVE;
+  static constexpr int XS = DMAX + 1;  // per-wavefront exchange row: slot totals, log density
+  size_t lds_offset = 0;
+
+  template <class T>
+  size_t lds_bytes() const { return (size_t)W * XS * sizeof(T); }
+  void set_lds_offset(size_t off) { lds_offset = (off + 15) / 16 * 16; }
+
+  template <class T>
+  __device__ T eval(const typename LogisticModel<KK, DBB>::template Lane<T>&, const T (&q)[K], T (&g)[K],
+                    bool& ok) const {
+    extern __shared__ __attribute__((aligned(16))) unsigned char ut_model_lds[];
+    T* xs = (T*)(ut_model_lds + lds_offset);
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int D = this->D;
+    T beta[DB];
+    T gp[DB];
+#pragma unroll
+    for (int d = 0; d < DB; ++d) {
+      beta[d] = wave_bcast(q[d / WAVE], d % WAVE);
+      gp[d] = T(0);
+    }
+    T lp = T(0);
+    logistic_rows<DB, T>(this->xT_f, this->xT_d, this->y_f, this->y_d, this->n, D, (long long)wave * WAVE + lane,
+                         (long long)W * WAVE, beta, gp, lp);
+    if (wave == 0) {
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        if (this->api(lane, k) >= 0) {
+          const T z = (q[k] - T(this->loc)) / T(this->scale);
+          lp = lp - (z * z) / T(2);
+        }
+      }
+    }
+    T gl[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) gl[k] = T(0);
+#pragma unroll
+    for (int d = 0; d < DB; ++d) {
+      if (d < D) {
+        const T tot = wave_sum(gp[d]);
+        if (lane == d % WAVE) gl[d / WAVE] = tot;
+      }
+    }
+    const T tl = wave_sum(lp);
+    T* mine = xs + wave * XS;
+#pragma unroll
+    for (int k = 0; k < K; ++k) mine[k * WAVE + lane] = gl[k];
+    if (lane == 0) mine[DMAX] = tl;
+    __syncthreads();
+    T G[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) G[k] = xs[k * WAVE + lane];
+    T total = xs[DMAX];
+#pragma unroll
+    for (int w = 1; w < W; ++w) {
+#pragma unroll
+      for (int k = 0; k < K; ++k) G[k] = G[k] + xs[w * XS + k * WAVE + lane];
+      total = total + xs[w * XS + DMAX];
+    }
+    __syncthreads();
+    bool fin = true;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      if (this->api(lane, k) >= 0) {
+        const T z = (q[k] - T(this->loc)) / T(this->scale);
+        g[k] = G[k] - z / T(this->scale);
+      } else {
+        g[k] = T(0);
+      }
+      fin = fin && t_isfinite(g[k]);
+    }
+    total = total + T(this->total_const);
     ok = wave_all(fin && t_isfinite(total));
     return total;
   }

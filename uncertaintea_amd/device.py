@@ -28,7 +28,7 @@ class DeviceBatchedWorkspace:
     """Device buffers + one HIP stream for `num_chains` chains of one model
     (reference DeviceBatchedWorkspace, src/device/workspace.jl:71-162)."""
 
-    def __init__(self, model: TeaModel, num_chains: int, precision=np.float32):
+    def __init__(self, model: TeaModel, num_chains: int, precision=np.float32, waves_per_chain: int = 0):
         self.model = model
         self.num_chains = int(num_chains)
         self.dtype = _precision(precision)
@@ -37,6 +37,20 @@ class DeviceBatchedWorkspace:
         L.check(L.lib().ut_workspace_create(model.plan, self.num_chains, _PREC[self.dtype], ct.byref(h)))
         self._h = h
         self._destroy = L.lib().ut_workspace_destroy  # kept: module globals may be gone at exit
+        if waves_per_chain:
+            self.set_waves_per_chain(waves_per_chain)
+
+    def set_waves_per_chain(self, waves: int) -> None:
+        """Wavefronts per chain of the density kernel and the persistent samplers:
+        0 automatic, 1, or 2/4/8 (logistic plans: one chain's rows split over a
+        workgroup, each layout its own deterministic summation order)."""
+        L.check(L.lib().ut_workspace_set_chain_waves(self._h, int(waves)))
+
+    @property
+    def waves_per_chain(self) -> int:
+        w = ct.c_int32()
+        L.check(L.lib().ut_workspace_chain_waves(self._h, ct.byref(w)))
+        return w.value
 
     @property
     def handle(self):

@@ -153,7 +153,7 @@ def _opts(num_samples, num_warmup, step_size, target_accept, divergence_threshol
 
 def _run(kind: str, model: TeaModel, num_chains: int, opts, initial_params,
          max_tree_depth: int, num_leapfrog_steps: int, target_accept: float, flow=None, callback=None,
-         callback_every: int = 10) -> HMCChains:
+         callback_every: int = 10, waves_per_chain: int = 0) -> HMCChains:
     if num_chains <= 0:
         raise ValueError("num_chains must be positive")
     cb = None
@@ -162,7 +162,7 @@ def _run(kind: str, model: TeaModel, num_chains: int, opts, initial_params,
             raise ValueError("callback must be callable (it receives a ProgressInfo)")
         cb = _Callback(callback)
         cb.install(opts.base if kind == "chees" else opts, callback_every)
-    ws = DeviceBatchedWorkspace(model, num_chains, np.float32)
+    ws = DeviceBatchedWorkspace(model, num_chains, np.float32, waves_per_chain=waves_per_chain)
     if flow is not None:
         if flow.num_params != model.num_params:
             raise ValueError("DimensionMismatch: the flow and the model have different parameter counts")
@@ -223,12 +223,15 @@ def batched_nuts(model: TeaModel, args=(), constraints=None, *, num_chains: int,
                  find_reasonable_step_size: bool = False, divergence_threshold: float = 1000.0,
                  mass_matrix_regularization: float = 1e-3, mass_matrix_min_samples: int = 10,
                  tree_strategy: str = "persistent", backend: str = "hip", precision=np.float32,
-                 seed: int = 0, callback=None, callback_every: int = 10, flow=None) -> HMCChains:
+                 seed: int = 0, callback=None, callback_every: int = 10, flow=None,
+                 waves_per_chain: int = 0) -> HMCChains:
     """Many NUTS chains.  tree_strategy='persistent': each chain builds its whole tree
     in one kernel (wavefront-per-chain plans).  'masked': every chain advances one
     leaf per step with the gradient of all chains in one batched call -- any plan,
     the matrix-core GLM and the GP included, and a `flow` preconditioner (NUTS in its
-    base space); the same draws as 'persistent' where both run."""
+    base space); the same draws as 'persistent' where both run.  waves_per_chain: 0
+    chooses the chain layout automatically (few chains over many logistic rows: a
+    workgroup of 2, 4 or 8 wavefronts per chain), 1/2/4/8 forces it."""
     model = model.condition(args, constraints)
     if tree_strategy not in ("persistent", "masked"):
         raise ValueError(f"batched_nuts tree_strategy must be 'persistent' or 'masked', got {tree_strategy!r}")
@@ -244,7 +247,8 @@ def batched_nuts(model: TeaModel, args=(), constraints=None, *, num_chains: int,
               mass_matrix_min_samples, adapt_step_size, adapt_mass_matrix, find_reasonable_step_size, init_strategy,
               init_max_retries, seed, max_tree_depth=max_tree_depth)
     return _run("nuts_masked" if tree_strategy == "masked" else "nuts", model, num_chains, o, initial_params,
-                max_tree_depth, 0, target_accept, flow=flow, callback=callback, callback_every=callback_every)
+                max_tree_depth, 0, target_accept, flow=flow, callback=callback, callback_every=callback_every,
+                waves_per_chain=waves_per_chain)
 
 
 def batched_hmc(model: TeaModel, args=(), constraints=None, *, num_chains: int, num_samples: int,
@@ -254,14 +258,15 @@ def batched_hmc(model: TeaModel, args=(), constraints=None, *, num_chains: int, 
                 find_reasonable_step_size: bool = False, divergence_threshold: float = 1000.0,
                 mass_matrix_regularization: float = 1e-3, mass_matrix_min_samples: int = 10,
                 backend: str = "hip", precision=np.float32, seed: int = 0, flow=None, callback=None,
-                callback_every: int = 10) -> HMCChains:
+                callback_every: int = 10, waves_per_chain: int = 0) -> HMCChains:
     """Many fixed-length HMC chains, device-resident (momentum, accept and adaptation on the GPU).
 
     per_chain_adaptation (default True, as the reference): each chain adapts its own
     step size inside the persistent kernel.  False: the reference's shared mode -- all
     chains in lock step with one dual-averaged step size.  `flow` (an
     AffineCouplingFlow) preconditions the shared mode: chains run in the flow's base
-    space and draws are returned mapped through the flow."""
+    space and draws are returned mapped through the flow.  waves_per_chain: the chain
+    layout of the per-chain mode, as for batched_nuts."""
     model = model.condition(args, constraints)
     if backend != "hip":
         raise ValueError(f"backend must be 'hip', got {backend!r}")
@@ -274,7 +279,8 @@ def batched_hmc(model: TeaModel, args=(), constraints=None, *, num_chains: int, 
               mass_matrix_min_samples, adapt_step_size, adapt_mass_matrix, find_reasonable_step_size, init_strategy,
               init_max_retries, seed, num_leapfrog_steps=num_leapfrog_steps)
     return _run("hmc_shared" if shared else "hmc", model, num_chains, o, initial_params, 0, num_leapfrog_steps,
-                target_accept, flow=flow, callback=callback, callback_every=callback_every)
+                target_accept, flow=flow, callback=callback, callback_every=callback_every,
+                waves_per_chain=waves_per_chain)
 
 
 def batched_chees(model: TeaModel, args=(), constraints=None, *, num_chains: int, num_samples: int,
