@@ -50,7 +50,8 @@ def _run(cmd: list[str], verbose: bool) -> None:
 def build_library(verbose: bool = True) -> Path:
     """Compile csrc/*.hip for gfx950 and link libuncertaintea_hip.so."""
     BUILD.mkdir(parents=True, exist_ok=True)
-    sources = sorted(CSRC.glob("*.hip"))
+    # largest translation units first, so the longest compiles start at once
+    sources = sorted(CSRC.glob("*.hip"), key=lambda p: -p.stat().st_size)
     hipcc = _hipcc()
     objs = []
     cmds = []
@@ -58,7 +59,7 @@ def build_library(verbose: bool = True) -> Path:
         obj = BUILD / (src.stem + ".o")
         objs.append(obj)
         cmds.append([hipcc, *HIPCC_FLAGS, "-c", str(src), "-o", str(obj)])
-    with ThreadPoolExecutor(max_workers=min(len(cmds), 4)) as ex:
+    with ThreadPoolExecutor(max_workers=min(len(cmds), max(1, min(8, os.cpu_count() or 1)))) as ex:
         list(ex.map(lambda c: _run(c, verbose), cmds))
     tmp = LIB.with_suffix(".so.tmp")
     _run([hipcc, "--offload-arch=gfx950", "-shared", "-o", str(tmp), *map(str, objs)], verbose)
