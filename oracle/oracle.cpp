@@ -83,9 +83,8 @@ struct Model {
   // twin layout
   int K = 1;
   double total_const = 0;
-  std::vector<float> ty_f;      // hierarchical y[i*64 + j]
   std::vector<int> tnobs;
-  std::vector<double> tlane;    // [3][64], binary64 (binary32 kernels round each)
+  std::vector<double> tlane;    // [6][64], binary64 (binary32 kernels round each)
   int max_n = 0;
 
   int api(int lane, int slot) const {
@@ -286,9 +285,6 @@ template <class T> T tlogistic(T x);
 template <> float tlogistic<float>(float x) { return logistic_f(x); }
 template <> double tlogistic<double>(double x) { return logistic_d(x); }
 template <class T> bool tfin(T x) { return std::isfinite(x); }
-template <class T> T tfma(T a, T b, T c);
-template <> float tfma<float>(float a, float b, float c) { return fma_f(a, b, c); }
-template <> double tfma<double>(double a, double b, double c) { return fma_d(a, b, c); }
 
 template <class T>
 T twin_eval(const Model& m, const T* q, T* g, bool& ok) {
@@ -301,20 +297,15 @@ T twin_eval(const Model& m, const T* q, T* g, bool& ok) {
     const T tau = texp<T>(ltau);
     T gth[WAVE], gq[WAVE], lpl[WAVE];
     for (int l = 0; l < WAVE; ++l) {
+      // the group's observations through their sufficient statistics (ut_models.h HierModel)
       const T qr = q[l];
       const T theta = mu + tau * qr;
-      const int n = m.tnobs[l];
-      T sd = T(0), ss = T(0);
-      for (int i = 0; i < m.max_n; ++i) {
-        if (i < n) {
-          const T d = T(m.ty_f[i * WAVE + l]) - theta;
-          sd = sd + d;
-          ss = tfma<T>(d, d, ss);
-        }
-      }
       const T cj = T(m.tlane[l]), hinv = T(m.tlane[WAVE + l]), inv = T(m.tlane[2 * WAVE + l]);
+      const T ybar = T(m.tlane[3 * WAVE + l]), ss0 = T(m.tlane[4 * WAVE + l]), n = T(m.tlane[5 * WAVE + l]);
+      const T dlt = ybar - theta;
+      const T ss = ss0 + n * (dlt * dlt);
       T lp = cj - ss * hinv;
-      gth[l] = sd * inv;
+      gth[l] = (n * dlt) * inv;
       if (l >= m.J) {
         lp = T(0);
         gth[l] = T(0);
@@ -2029,17 +2020,28 @@ void finish_twin_layout(Model& m) {
   if (m.kind == KIND_HIER) {
     m.max_n = 0;
     m.tnobs.assign(WAVE, 0);
-    m.tlane.assign(3 * WAVE, 0.0);
+    // [6][64]: c_j, hinv_j, inv_j, then the group's sufficient statistics ybar_j,
+    // S_j = sum_i (y_ij - ybar_j)^2 (binary64, observation order, two passes), n_j
+    m.tlane.assign(6 * WAVE, 0.0);
     for (int j = 0; j < m.J; ++j) m.max_n = std::max(m.max_n, m.offsets[j + 1] - m.offsets[j]);
-    m.ty_f.assign((size_t)std::max(m.max_n, 1) * WAVE, 0.0f);
     for (int j = 0; j < m.J; ++j) {
       const int nj = m.offsets[j + 1] - m.offsets[j];
       m.tnobs[j] = nj;
-      for (int i = 0; i < nj; ++i) m.ty_f[(size_t)i * WAVE + j] = m.hy[m.offsets[j] + i];
+      double sy = 0.0;
+      for (int i = 0; i < nj; ++i) sy = sy + (double)m.hy[m.offsets[j] + i];
+      const double ybar = nj > 0 ? sy / (double)nj : 0.0;
+      double ss = 0.0;
+      for (int i = 0; i < nj; ++i) {
+        const double dv = (double)m.hy[m.offsets[j] + i] - ybar;
+        ss = ss + dv * dv;
+      }
       const double s = m.sigma[j];
       m.tlane[j] = -(double)nj * log_d(s) - (double)nj * HALF_LOG_2PI;
       m.tlane[WAVE + j] = 1.0 / (2.0 * s * s);
       m.tlane[2 * WAVE + j] = 1.0 / (s * s);
+      m.tlane[3 * WAVE + j] = ybar;
+      m.tlane[4 * WAVE + j] = ss;
+      m.tlane[5 * WAVE + j] = (double)nj;
     }
     m.total_const = (double)m.J * (-HALF_LOG_2PI) + (-log_d(m.mu_scale) - HALF_LOG_2PI) +
                     (LOG_2 - LOG_PI - log_d(m.tau_scale));

@@ -422,22 +422,36 @@ int ut_plan_hier_normal_nc(const float* y, const int32_t* group_offsets, const d
   std::vector<float> y_f((size_t)(max_n > 0 ? max_n : 1) * WAVE, 0.0f);
   std::vector<double> y_d(y_f.size(), 0.0);
   std::vector<int> nobs(WAVE, 0);
-  std::vector<float> lane_f(3 * WAVE, 0.0f);
-  std::vector<double> lane_d(3 * WAVE, 0.0);
+  // per-lane constants [6][64]: c_j, hinv_j, inv_j, then the group's sufficient
+  // statistics ybar_j, S_j = sum_i (y_ij - ybar_j)^2 and n_j (binary64, summed in
+  // observation order, two passes)
+  std::vector<float> lane_f(6 * WAVE, 0.0f);
+  std::vector<double> lane_d(6 * WAVE, 0.0);
   for (int j = 0; j < num_groups; ++j) {
     int nj = group_offsets[j + 1] - group_offsets[j];
     nobs[j] = nj;
+    double sy = 0.0;
     for (int i = 0; i < nj; ++i) {
       float v = y[group_offsets[j] + i];
       y_f[(size_t)i * WAVE + j] = v;
       y_d[(size_t)i * WAVE + j] = (double)v;
+      sy = sy + (double)v;
+    }
+    const double ybar = nj > 0 ? sy / (double)nj : 0.0;
+    double ss = 0.0;
+    for (int i = 0; i < nj; ++i) {
+      const double dv = (double)y[group_offsets[j] + i] - ybar;
+      ss = ss + dv * dv;
     }
     const double s = sigma[j];
     const double cj = -(double)nj * log_d(s) - (double)nj * HALF_LOG_2PI;
     const double hinv = 1.0 / (2.0 * s * s);
     const double inv = 1.0 / (s * s);
-    lane_d[j] = cj; lane_d[WAVE + j] = hinv; lane_d[2 * WAVE + j] = inv;
-    lane_f[j] = (float)cj; lane_f[WAVE + j] = (float)hinv; lane_f[2 * WAVE + j] = (float)inv;
+    const double row[6] = {cj, hinv, inv, ybar, ss, (double)nj};
+    for (int r = 0; r < 6; ++r) {
+      lane_d[r * WAVE + j] = row[r];
+      lane_f[r * WAVE + j] = (float)row[r];
+    }
   }
   DevModelDesc& m = plan->dev;
   m.kind = MODEL_HIER;
@@ -450,9 +464,6 @@ int ut_plan_hier_normal_nc(const float* y, const int32_t* group_offsets, const d
   m.tau_scale = tau_scale;
   m.inv_mu_scale = 1.0 / mu_scale;
   m.inv_tau_scale = 1.0 / tau_scale;
-  m.dense = 1;
-  for (int j = 0; j < num_groups; ++j)
-    if (group_offsets[j + 1] - group_offsets[j] != max_n) m.dense = 0;
   m.total_const = (double)num_groups * (-HALF_LOG_2PI) + (-log_d(mu_scale) - HALF_LOG_2PI) +
                   (LOG_2 - LOG_PI - log_d(tau_scale));
   int rc;

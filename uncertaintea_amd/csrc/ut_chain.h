@@ -44,11 +44,10 @@ static FunnelModel<K> make_funnel(const DevModelDesc& d) {
   m.dim = d.dim; m.v_scale = d.v_scale; m.total_const = d.total_const;
   return m;
 }
-template <int NMAX, bool DENSE = false>
-static HierModel<NMAX, DENSE> make_hier(const DevModelDesc& d) {
-  HierModel<NMAX, DENSE> m;
-  m.y_f = d.y_f; m.y_d = d.y_d; m.nobs = d.nobs; m.lane_f = d.lane_f; m.lane_d = d.lane_d;
-  m.J = d.J; m.max_n = d.max_n; m.mu_loc = d.mu_loc; m.mu_scale = d.mu_scale;
+static HierModel make_hier(const DevModelDesc& d) {
+  HierModel m;
+  m.lane_f = d.lane_f; m.lane_d = d.lane_d;
+  m.J = d.J; m.mu_loc = d.mu_loc; m.mu_scale = d.mu_scale;
   m.tau_scale = d.tau_scale; m.total_const = d.total_const;
   m.inv_mu_scale = d.inv_mu_scale; m.inv_tau_scale = d.inv_tau_scale;
   return m;
@@ -78,16 +77,7 @@ template <class F>
 static hipError_t dispatch(const DevModelDesc& d, F&& f) {
   switch (d.kind) {
     case MODEL_HIER:
-      // observations per group held in registers when they fit; equal group sizes of
-      // 16, 32 or 64 take the mask-free instances
-      if (d.dense && d.max_n == 16) return f(make_hier<16, true>(d));
-      if (d.dense && d.max_n == 32) return f(make_hier<32, true>(d));
-      if (d.dense && d.max_n == 64) return f(make_hier<64, true>(d));
-      if (d.max_n <= 8) return f(make_hier<8>(d));
-      if (d.max_n <= 16) return f(make_hier<16>(d));
-      if (d.max_n <= 32) return f(make_hier<32>(d));
-      if (d.max_n <= 64) return f(make_hier<64>(d));
-      return f(make_hier<0>(d));
+      return f(make_hier(d));
     case MODEL_LOGISTIC:
       // up to 32 covariates: register arrays of 32 and the next row prefetched
       if (d.K == 1 && d.D <= 32) return f(make_logistic<1, 32>(d));

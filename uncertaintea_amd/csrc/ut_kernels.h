@@ -45,7 +45,6 @@ struct DevModelDesc {
   const double* lane_d = nullptr;
   int J = 0;
   int max_n = 0;
-  int dense = 0;
   double mu_loc = 0, mu_scale = 1, tau_scale = 1;
   double inv_mu_scale = 1, inv_tau_scale = 1;
   // logistic

@@ -23,7 +23,12 @@ __device__ inline double window_progress(long long count, int wl) {
 
 // Each iteration's history is staged through LDS in blocks of `cb` chains (positions
 // as one contiguous copy, weights computed lane-parallel), so the sequential fold reads
-// LDS instead of waiting on a global load per step.
+// LDS instead of waiting on a global load per step.  RPT parameters per thread (1 when
+// P <= REPLAY_THREADS); the fold is latency-bound (one binary64 dependency chain per
+// parameter), so it avoids branches: the running weight sums are a branch-free loop,
+// the next chain's staged values are read while the current one is folded, and the
+// clip's divide and square root run only when some lane of the wavefront needs them.
+template <int RPT>
 __global__ __launch_bounds__(REPLAY_THREADS) void k_mass_replay(
     const float* __restrict__ hist_pos, const double* __restrict__ hist_acc,
     const uint8_t* __restrict__ hist_flags, int C, int P, int it_lo, int it_hi, int wl,
@@ -37,9 +42,9 @@ __global__ __launch_bounds__(REPLAY_THREADS) void k_mass_replay(
   double* sstate = (double*)(scnt + cb);   // running weight sum / square sum / count after the block
   float* sx = (float*)(sstate + 4);        // cb x P positions
   const int t = threadIdx.x;
-  double mean[REPLAY_PER_THREAD], m2[REPLAY_PER_THREAD];
+  double mean[RPT], m2[RPT];
 #pragma unroll
-  for (int j = 0; j < REPLAY_PER_THREAD; ++j) { mean[j] = 0.0; m2[j] = 0.0; }
+  for (int j = 0; j < RPT; ++j) { mean[j] = 0.0; m2[j] = 0.0; }
   long long count = 0;
   double wsum = 0.0, wsq = 0.0;
   for (int it = it_lo; it <= it_hi; ++it) {
@@ -48,7 +53,25 @@ __global__ __launch_bounds__(REPLAY_THREADS) void k_mass_replay(
       const int nb = C - c0 < cb ? C - c0 : cb;
       const long long row0 = (long long)(it - 1) * C + c0;
       __syncthreads();
-      for (int e = t; e < nb * P; e += blockDim.x) sx[e] = hist_pos[row0 * P + e];
+      {
+        // SU loads in flight per thread (indices clamped into the block), then the LDS stores
+        constexpr int SU = 16;
+        const int nbp = nb * P;
+        const float* src = hist_pos + row0 * P;
+        for (int e0 = t; e0 < nbp; e0 += (int)blockDim.x * SU) {
+          float v[SU];
+#pragma unroll
+          for (int u = 0; u < SU; ++u) {
+            const int e = e0 + u * (int)blockDim.x;
+            v[u] = src[e < nbp ? e : nbp - 1];
+          }
+#pragma unroll
+          for (int u = 0; u < SU; ++u) {
+            const int e = e0 + u * (int)blockDim.x;
+            if (e < nbp) sx[e] = v[u];
+          }
+        }
+      }
       for (int c = t; c < nb; c += blockDim.x) {
         const uint8_t fl = hist_flags[row0 + c];
         const double ap = hist_acc[row0 + c];
@@ -63,24 +86,39 @@ __global__ __launch_bounds__(REPLAY_THREADS) void k_mass_replay(
         sw[c] = w;
       }
       __syncthreads();
-      // the state's scalars in chain order: running count and weight sums (one thread;
-      // the weight sum's additions keep their order) ...
-      if (t == 0) {
+      // the state's scalars in chain order: running count and weight sums, by the first
+      // wavefront in groups of 64 chains: lane k holds chain c0 + k's weight, every lane
+      // runs the same sums in chain order on the weights read across lanes (the weight
+      // sum's additions keep their order; a zero weight leaves the sums unchanged), and
+      // lane k keeps the state before and after chain k's step ...
+      if (t < WAVE) {
         long long cnt = count;
         double ws = wsum, wq = wsq;
-        for (int c = 0; c < nb; ++c) {
-          const double w = sw[c];
-          scnt[c] = cnt;
-          if (w != 0.0) {
-            ws = ws + w;
-            sfrac[c] = ws;
-            wq += w * w;
-            cnt += 1;
+        for (int g0 = 0; g0 < nb; g0 += WAVE) {
+          const int c = g0 + t;
+          const double wl = sw[c < nb ? c : nb - 1];
+          const int m = nb - g0 < WAVE ? nb - g0 : WAVE;
+          long long my_cnt = 0;
+          double my_ws = 0.0;
+          for (int k = 0; k < m; ++k) {
+            const double w = readlane(wl, k);
+            my_cnt = t == k ? cnt : my_cnt;
+            const bool on = w != 0.0;
+            ws = on ? ws + w : ws;
+            wq = on ? wq + w * w : wq;
+            cnt += on ? 1 : 0;
+            my_ws = t == k ? ws : my_ws;
+          }
+          if (c < nb) {
+            scnt[c] = my_cnt;
+            sfrac[c] = my_ws;
           }
         }
-        sstate[0] = ws;
-        sstate[1] = wq;
-        sstate[2] = (double)cnt;
+        if (t == 0) {
+          sstate[0] = ws;
+          sstate[1] = wq;
+          sstate[2] = (double)cnt;
+        }
       }
       __syncthreads();
       // ... then every step's clip scale, divisor and weight fraction in parallel
@@ -95,37 +133,87 @@ __global__ __launch_bounds__(REPLAY_THREADS) void k_mass_replay(
         sfrac[c] = w / sfrac[c];
       }
       __syncthreads();
-      for (int c = 0; c < nb; ++c) {
-        const double w = sw[c];
-        if (w == 0.0) continue;
-        const double clip_scale = sclip[c];
-        const bool clip = clip_scale >= 0.0;
-        const double denom_c = sden[c];
-        const double frac = sfrac[c];
-        // |delta| <= bound  <=>  delta^2 denom_c <= clip_scale^2 max(m2, 1e-3 denom_c); with a
-        // 1e-12 relative margin the test below implies the exact comparison, so the divide
-        // and square root are only evaluated for samples near or beyond the bound
-        const double inside = clip_scale * clip_scale * (1.0 - 1e-12);
+      // software-pipelined: chain c + 1's staged values are read from LDS while chain c
+      // is folded, so the fold's dependency chain does not wait on LDS (the entries of
+      // zero-weight chains are read but not used; parameters >= P read position 0 and
+      // fold zeros, which no result uses)
+      double w_n = sw[0], cs_n = sclip[0], den_n = sden[0], fr_n = sfrac[0];
+      float x_n[RPT];
 #pragma unroll
-        for (int j = 0; j < REPLAY_PER_THREAD; ++j) {
-          const int p = t + j * REPLAY_THREADS;
-          if (p < P) {
-            double x = (double)sx[c * P + p];
-            if (clip) {
-              double delta = x - mean[j];
-              const double floor_m2 = 1e-3 * denom_c;
-              if (!(delta * delta * denom_c <= inside * (m2[j] > floor_m2 ? m2[j] : floor_m2))) {
-                const double var = m2[j] / denom_c;
-                const double bound = clip_scale * sqrt_d(var > 1e-3 ? var : 1e-3);
-                delta = delta > bound ? bound : (delta < -bound ? -bound : delta);
-              }
-              x = mean[j] + delta;
-            }
-            const double delta = x - mean[j];
-            mean[j] += frac * delta;
-            const double delta2 = x - mean[j];
-            m2[j] += (w * delta) * delta2;
+      for (int j = 0; j < RPT; ++j) {
+        const int p = t + j * REPLAY_THREADS;
+        const float v = sx[p < P ? p : 0];
+        x_n[j] = p < P ? v : 0.0f;
+      }
+      for (int c = 0; c < nb; ++c) {
+        const double w = w_n;
+        const double clip_scale = cs_n;
+        const double denom_c = den_n;
+        const double frac = fr_n;
+        double x[RPT];
+#pragma unroll
+        for (int j = 0; j < RPT; ++j) x[j] = (double)x_n[j];
+        {
+          const int cn = c + 1 < nb ? c + 1 : c;
+          w_n = sw[cn];
+          cs_n = sclip[cn];
+          den_n = sden[cn];
+          fr_n = sfrac[cn];
+#pragma unroll
+          for (int j = 0; j < RPT; ++j) {
+            const int p = t + j * REPLAY_THREADS;
+            const float v = sx[cn * P + (p < P ? p : 0)];
+            x_n[j] = p < P ? v : 0.0f;
           }
+        }
+        if (w == 0.0) continue;
+        // The step is computed for the unclipped deviation first, alongside the clip
+        // test, so that the test's latency overlaps the update; the rare steps whose
+        // deviation lies beyond the bound are then recomputed from the clipped one.
+        // |delta| <= bound  <=>  delta^2 denom_c <= clip_scale^2 max(m2, 1e-3 denom_c); with
+        // a 1e-12 relative margin the test below implies the exact comparison, so the
+        // divide and square root are only evaluated for samples near or beyond the bound.
+        const bool clip = clip_scale >= 0.0;
+        const double inside = clip_scale * clip_scale * (1.0 - 1e-12);
+        const double floor_m2 = 1e-3 * denom_c;
+        double dl[RPT], mn[RPT], mm[RPT];
+        bool in[RPT];
+#pragma unroll
+        for (int j = 0; j < RPT; ++j) dl[j] = 0.0;
+        bool out = false;
+#pragma unroll
+        for (int j = 0; j < RPT; ++j) {
+          double xf = x[j];
+          if (clip) {
+            dl[j] = x[j] - mean[j];
+            xf = mean[j] + dl[j];
+          }
+          const double delta = xf - mean[j];
+          mn[j] = mean[j] + frac * delta;
+          const double delta2 = xf - mn[j];
+          mm[j] = m2[j] + (w * delta) * delta2;
+          in[j] = !clip || dl[j] * dl[j] * denom_c <= inside * (m2[j] > floor_m2 ? m2[j] : floor_m2);
+          out = out || !in[j];
+        }
+        if (wave_any(out)) {
+#pragma unroll
+          for (int j = 0; j < RPT; ++j) {
+            if (!in[j]) {
+              const double var = m2[j] / denom_c;
+              const double bound = clip_scale * sqrt_d(var > 1e-3 ? var : 1e-3);
+              const double d = dl[j] > bound ? bound : (dl[j] < -bound ? -bound : dl[j]);
+              const double xf = mean[j] + d;
+              const double delta = xf - mean[j];
+              mn[j] = mean[j] + frac * delta;
+              const double delta2 = xf - mn[j];
+              mm[j] = m2[j] + (w * delta) * delta2;
+            }
+          }
+        }
+#pragma unroll
+        for (int j = 0; j < RPT; ++j) {
+          mean[j] = mn[j];
+          m2[j] = mm[j];
         }
       }
       wsum = sstate[0];
@@ -139,7 +227,7 @@ __global__ __launch_bounds__(REPLAY_THREADS) void k_mass_replay(
     const double denom = wsum - wsq / wsum;
     const double shrink = eff / (eff + 5.0);
 #pragma unroll
-    for (int j = 0; j < REPLAY_PER_THREAD; ++j) {
+    for (int j = 0; j < RPT; ++j) {
       const int p = t + j * REPLAY_THREADS;
       if (p < P) {
         double v;
@@ -763,9 +851,14 @@ hipError_t launch_mass_replay(const float* hist_pos, const double* hist_acc,
   // staged block within 64 KB of LDS: per chain 5 scalars of 8 bytes and P positions
   const int cb = std::max(1, std::min(256, (65536 - 64) / (4 * P + 40)));
   const size_t shmem = (size_t)cb * 40 + 32 + (size_t)cb * P * sizeof(float);
-  hipLaunchKernelGGL(k_mass_replay, dim3(1), dim3(REPLAY_THREADS), shmem, stream, hist_pos, hist_acc,
-                     hist_flags, C, P, it_lo, it_hi, wl, use_accepted, reg, min_samples, im_d,
-                     im_f, summary, cb);
+  if (P <= REPLAY_THREADS)
+    hipLaunchKernelGGL(k_mass_replay<1>, dim3(1), dim3(REPLAY_THREADS), shmem, stream, hist_pos, hist_acc,
+                       hist_flags, C, P, it_lo, it_hi, wl, use_accepted, reg, min_samples, im_d,
+                       im_f, summary, cb);
+  else
+    hipLaunchKernelGGL(k_mass_replay<REPLAY_PER_THREAD>, dim3(1), dim3(REPLAY_THREADS), shmem, stream, hist_pos,
+                       hist_acc, hist_flags, C, P, it_lo, it_hi, wl, use_accepted, reg, min_samples, im_d,
+                       im_f, summary, cb);
   return hipGetLastError();
 }
 

@@ -113,34 +113,27 @@ __device__ inline void wave_lds_sync() {
 // ----------------------------------------------------------------------------------
 // Non-centred hierarchical normal.  Internal order: slot 0 of lane j is theta_raw_j
 // (api 2 + j) for j < J; slot 1 of lane 0 is mu (api 0), of lane 1 is log tau (api 1).
-// Lane j scores group j's observations:
-//   sum_i log N(y_ij; theta_j, sigma_j) = c_j - hinv_j * sum_i (y_ij - theta_j)^2
+// Lane j scores group j's n_j observations through the group's sufficient statistics
+// (mean ybar_j and centred sum of squares S_j, formed in binary64 when the plan is
+// staged):
+//   sum_i (y_ij - theta_j)^2 = S_j + n_j (ybar_j - theta_j)^2,
+//   sum_i log N(y_ij; theta_j, sigma_j) = c_j - hinv_j [S_j + n_j (ybar_j - theta_j)^2]
 //   with c_j = -n_j log sigma_j - n_j log(2 pi)/2 and hinv_j = 1 / (2 sigma_j^2),
-//   d/dtheta_j = inv_j * sum_i (y_ij - theta_j), inv_j = 1 / sigma_j^2.
-// Observations are stored y[i * 64 + j] (zero padded to max_n rows) so that the lanes
-// of one row read consecutive words.  With NMAX > 0 every lane keeps its group's
-// observations in registers for the whole launch (prepare()); NMAX == 0 streams
-// them from memory on every evaluation.  DENSE instances serve plans whose groups all
-// hold exactly NMAX observations and carry no per-observation masks.  The arithmetic
-// order is the same in every instance.
-template <int NMAX, bool DENSE = false>
+//   d/dtheta_j = inv_j n_j (ybar_j - theta_j), inv_j = 1 / sigma_j^2,
+// an exact rewriting of the per-observation sums (and better conditioned than them:
+// no cancellation between observations), so one evaluation costs O(1) per lane
+// whatever the group sizes.  Empty groups and lanes >= J contribute zero.
 struct HierModel : NoModelLds {
   static constexpr int K = 2;
-  const float* y_f;
-  const double* y_d;
-  const int* nobs;         // [64], 0 for lanes >= J
-  const float* lane_f;     // [3][64]: c_j, hinv_j, inv_j
+  const float* lane_f;     // [6][64]: c_j, hinv_j, inv_j, ybar_j, S_j, n_j
   const double* lane_d;
   int J;
-  int max_n;
   double mu_loc, mu_scale, tau_scale, total_const;
   double inv_mu_scale, inv_tau_scale;
 
   template <class T>
   struct Lane {
-    T y[NMAX > 0 ? NMAX : 1];
-    T cj, hinv, inv;
-    int n;
+    T cj, hinv, inv, ybar, ss0, n;
   };
 
   __device__ int api(int lane, int slot) const {
@@ -151,15 +144,12 @@ struct HierModel : NoModelLds {
   template <class T>
   __device__ void prepare(Lane<T>& ld) const {
     const int lane = threadIdx.x & (WAVE - 1);
-    ld.n = nobs[lane];
-    if (NMAX > 0) {
-#pragma unroll
-      for (int i = 0; i < (NMAX > 0 ? NMAX : 1); ++i)
-        ld.y[i] = i < ld.n ? Pick<T>::get(y_f, y_d, i * WAVE + lane) : T(0);
-    }
     ld.cj = Pick<T>::get(lane_f, lane_d, lane);
     ld.hinv = Pick<T>::get(lane_f, lane_d, WAVE + lane);
     ld.inv = Pick<T>::get(lane_f, lane_d, 2 * WAVE + lane);
+    ld.ybar = Pick<T>::get(lane_f, lane_d, 3 * WAVE + lane);
+    ld.ss0 = Pick<T>::get(lane_f, lane_d, 4 * WAVE + lane);
+    ld.n = Pick<T>::get(lane_f, lane_d, 5 * WAVE + lane);
   }
 
   template <class T>
@@ -170,37 +160,10 @@ struct HierModel : NoModelLds {
     const T tau = t_exp<T>(ltau);
     const T qr = q[0];
     const T theta = mu + tau * qr;
-    const int n = ld.n;
-    T sd = T(0), ss = T(0);
-    if constexpr (NMAX > 0 && DENSE) {
-      // every group has exactly NMAX observations: no per-observation mask; lanes
-      // past the last group are zeroed after the loop
-#pragma unroll
-      for (int i = 0; i < (NMAX > 0 ? NMAX : 1); ++i) {
-        const T d = ld.y[i] - theta;
-        sd = sd + d;
-        ss = t_fma<T>(d, d, ss);
-      }
-    } else if (NMAX > 0) {
-#pragma unroll
-      for (int i = 0; i < (NMAX > 0 ? NMAX : 1); ++i) {
-        if (i < n) {
-          const T d = ld.y[i] - theta;
-          sd = sd + d;
-          ss = t_fma<T>(d, d, ss);
-        }
-      }
-    } else {
-      for (int i = 0; i < max_n; ++i) {
-        if (i < n) {
-          const T d = Pick<T>::get(y_f, y_d, i * WAVE + lane) - theta;
-          sd = sd + d;
-          ss = t_fma<T>(d, d, ss);
-        }
-      }
-    }
+    const T dlt = ld.ybar - theta;
+    const T ss = ld.ss0 + ld.n * (dlt * dlt);
     T lp = ld.cj - ss * ld.hinv;
-    T gth = sd * ld.inv;
+    T gth = (ld.n * dlt) * ld.inv;
     if (lane >= J) {
       lp = T(0);
       gth = T(0);
