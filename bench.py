@@ -60,6 +60,12 @@ batched_meads with 1024 chains, without and with a flow; CPU baseline projected.
 `--workload flow_kernels`: the funnel preconditioner's sampling-path kernel pair alone (4096
 rows; `--steps` = back-to-back launches per direction), for profiling those kernels by
 themselves.
+
+`--workload logistic_nuts`: few chains over many rows -- logistic regression N = 65536,
+D = 32 (fp32), batched_nuts with 64 chains, 500 + 500 iterations, each chain's rows split over
+a workgroup of 8 wavefronts (the automatic layout); also times the one-wavefront layout on a
+shorter run; CPU baseline projected from binary64 oracle NUTS iterations from the GPU run's
+adapted state.
 """
 from __future__ import annotations
 
@@ -198,7 +204,8 @@ def _cpu_hmc_seconds_per_iteration(om, chains, L, step, seconds, threads, seed, 
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=("funnel_flow", "hier_nuts", "glm_chees", "gp_meads", "advi", "flow_kernels"),
+    ap.add_argument("--workload", choices=("funnel_flow", "hier_nuts", "glm_chees", "gp_meads", "advi", "flow_kernels",
+                                           "logistic_nuts"),
                     default="funnel_flow")
     ap.add_argument("--guide", choices=("meanfield", "fullrank", "flow"), default="fullrank",
                     help="advi: the variational guide")
@@ -255,7 +262,104 @@ def main() -> int:
               "(chains shard across GPUs without communication; see the design notes)", file=sys.stderr)
         return 2
     return {"funnel_flow": bench_funnel_flow, "hier_nuts": bench_hier_nuts, "glm_chees": bench_glm_chees,
-            "gp_meads": bench_gp_meads, "advi": bench_advi, "flow_kernels": bench_flow_kernels}[args.workload](args)
+            "gp_meads": bench_gp_meads, "advi": bench_advi, "flow_kernels": bench_flow_kernels,
+            "logistic_nuts": bench_logistic_nuts}[args.workload](args)
+
+
+def bench_logistic_nuts(args) -> int:
+    """Few chains over many rows: persistent NUTS with each chain's rows split over the
+    wavefronts of its workgroup."""
+    import uncertaintea_amd as ut
+    from uncertaintea_amd import diagnostics as D
+    from uncertaintea_amd import synthetic as S
+    from uncertaintea_amd.device import DeviceBatchedWorkspace
+
+    N, Dd = 65536, 32
+    args.steps = 500 if args.steps is None else args.steps
+    args.warmup = 500 if args.warmup is None else args.warmup
+    args.chains = 64 if args.chains is None else args.chains
+    x, y, _ = S.logistic_data(N, Dd, seed=args.seed)
+    model = ut.LogisticRegression(x, y)
+    waves = DeviceBatchedWorkspace(model, args.chains).waves_per_chain
+    log(f"bench: batched_nuts logistic N={N} D={Dd} chains={args.chains} waves/chain={waves} "
+        f"warmup={args.warmup} steps={args.steps}")
+    t0 = time.perf_counter()
+    res = ut.batched_nuts(model, num_chains=args.chains, num_samples=args.steps, num_warmup=args.warmup,
+                          max_tree_depth=10, seed=args.seed)
+    wall = time.perf_counter() - t0
+    log(f"bench: sampler done warmup {res.warmup_seconds:.3f}s sampling {res.sampling_seconds:.3f}s wall {wall:.3f}s")
+    st = D.harness_stats(res.constrained_samples)
+    min_ess = float(np.nanmin(st["ess_bulk"]))
+    rhat_max = float(np.nanmax(st["rhat"]))
+    raw = min_ess / res.sampling_seconds
+    log(f"bench: rhat max {rhat_max:.4f} min bulk ESS {min_ess:.1f} -> {raw:.1f} ESS/s")
+    evals = res.sampling_gradient_evals
+    # per chain-gradient: 4 N D flops (X beta and X^T r) + ~6 N + 3 D (the logistic-regression
+    # unit of work); algorithmic bytes 4 N D + 4 N (X and y, fp32, from L2 after the first chain)
+    flops_per_grad = 4 * N * Dd + 6 * N + 3 * Dd
+    ksec = res.sampling_kernel_seconds
+    launches = max(res.sampling_launches, 1)
+    achieved = evals * flops_per_grad / ksec / 1e12 if ksec > 0 else 0.0
+    roofline = {"bound": "fp32-valu", "compute": "fp32 vector ALU and L2 reads of X (latency-bound: "
+                f"{args.chains} chains x {waves} wavefronts on 1024 SIMDs)",
+                "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
+                "traffic": None, "kernel": f"k_nuts<TiledLogisticModel<1, 32, {waves}>>" if waves > 1 else "k_nuts<LogisticModel<1, 32>>", "launches": res.sampling_launches,
+                "avg_launch_ms": 1e3 * ksec / launches, "flops_per_chain_gradient": flops_per_grad,
+                "l2_bytes_per_chain_gradient": 4 * N * Dd + 4 * N,
+                "timing": "HIP events around every sampling launch on the workspace stream"}
+    # the one-wavefront layout on a shorter run, for the speedup of the workgroup layout
+    short = min(100, args.steps)
+    r1 = ut.batched_nuts(model, num_chains=args.chains, num_samples=short, num_warmup=min(200, args.warmup),
+                         max_tree_depth=10, seed=args.seed, waves_per_chain=1)
+    one = {"waves_per_chain": 1, "sampling_seconds": r1.sampling_seconds, "num_samples": short,
+           "gradient_evals_per_s": r1.sampling_gradient_evals / r1.sampling_seconds,
+           "ms_per_gradient_eval_all_chains": 1e3 * r1.sampling_seconds * args.chains / max(1, r1.sampling_gradient_evals)}
+    mine = 1e3 * res.sampling_seconds * args.chains / max(1, evals)
+    log(f"bench: per leapfrog step of all chains {mine:.3f} ms with {waves} wavefronts per chain, "
+        f"{one['ms_per_gradient_eval_all_chains']:.3f} ms with one")
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import binding as B
+        om = B.OracleModel(model)
+        info = cpu_info()
+        cores = max(1, info["affinity_cores"])
+        pos = res.unconstrained_samples[:, -1, :].astype(np.float64).T.copy()
+        it = args.warmup + args.steps + 1
+        n, t, ev = 0, 0.0, 0
+        while t < args.cpu_seconds and n < 500:
+            ts = time.perf_counter()
+            e, pos = B.nuts_sample_from(om, "f64", pos, res.step_size, res.mass_matrix, it + n, 1, seed=args.seed,
+                                        threads=cores)
+            t += time.perf_counter() - ts
+            ev += e
+            n += 1
+        s_it = t / n
+        cpu = {"value": (min_ess / args.steps) / s_it, "unit": "ESS/s (projected, binary64 oracle NUTS)",
+               "cores": cores, "kind": "port", "cpu_model": info["cpu_model"], "host": info,
+               "sample": f"{n} sampling iterations x {args.chains} chains of binary64 oracle NUTS on {cores} threads "
+                         f"({t:.1f} s) from the GPU run's adapted state; projected = GPU run's bulk ESS per "
+                         f"iteration / CPU s per iteration",
+               "cpu_s_per_iteration": s_it, "cpu_gradient_evals_per_s": ev / t}
+        log(f"bench: cpu {cores} threads {s_it:.3f} s/iter -> projected {cpu['value']:.1f} ESS/s")
+
+    ok = rhat_max < RHAT_MAX
+    line = {
+        "metric": METRIC, "value": raw if ok else None, "unit": "ESS/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * res.sampling_seconds / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic (Philox4x32-10, seed %d)" % args.seed,
+        "config": {"workload": f"logistic regression N={N} D={Dd} fp32, batched_nuts persistent, {args.chains} chains, "
+                               f"max depth 10, {waves} wavefronts per chain (plain NUTS, no flow)",
+                   "chains": args.chains, "num_params": model.num_params, "waves_per_chain": waves},
+        "ungated_value": raw, "rhat_max": rhat_max, "rhat_ok": ok, "min_bulk_ess": min_ess,
+        "sampling_seconds": res.sampling_seconds, "warmup_seconds": res.warmup_seconds,
+        "gradient_evals_per_s": evals / res.sampling_seconds, "mean_tree_depth": float(res.tree_depths.mean()),
+        "ms_per_gradient_eval_all_chains": mine, "one_wavefront_per_chain": one,
+        "roofline": roofline, "cpu_baseline": cpu,
+        "vs_cpu": (raw / cpu["value"]) if cpu and cpu["value"] else None,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
 
 
 def bench_hier_nuts(args) -> int:

@@ -281,6 +281,9 @@ template <> double tlog1p<double>(double x) { return log1p_d(x); }
 template <class T> T tl1pe(T x);
 template <> float tl1pe<float>(float x) { return log1p_exp_f(x); }
 template <> double tl1pe<double>(double x) { return log1p_exp_d(x); }
+template <class T> T tfma(T a, T b, T c);
+template <> float tfma<float>(float a, float b, float c) { return fma_f(a, b, c); }
+template <> double tfma<double>(double a, double b, double c) { return fma_d(a, b, c); }
 template <class T> T tlogistic(T x);
 template <> float tlogistic<float>(float x) { return logistic_f(x); }
 template <> double tlogistic<double>(double x) { return logistic_d(x); }
@@ -347,11 +350,11 @@ T twin_eval(const Model& m, const T* q, T* g, bool& ok) {
         T* gpl = &gp[(size_t)l * DMAX];
         for (long long i = (long long)w * WAVE + l; i < m.n; i += (long long)NW * WAVE) {
           T eta = T(0);
-          for (int d = 0; d < m.D; ++d) eta = eta + T(m.x[i * m.D + d]) * beta[d];
+          for (int d = 0; d < m.D; ++d) eta = tfma<T>(T(m.x[i * m.D + d]), beta[d], eta);
           const T yi = T(m.y[i]);
           lp = lp + (yi * eta - tl1pe<T>(eta));
           const T r = yi - tlogistic<T>(eta);
-          for (int d = 0; d < m.D; ++d) gpl[d] = gpl[d] + T(m.x[i * m.D + d]) * r;
+          for (int d = 0; d < m.D; ++d) gpl[d] = tfma<T>(T(m.x[i * m.D + d]), r, gpl[d]);
         }
         if (w == 0) {
           for (int k = 0; k < K; ++k) {

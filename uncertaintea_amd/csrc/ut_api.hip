@@ -292,8 +292,10 @@ int ut_plan_logistic_glm(const float* x, const float* y, int64_t n, int32_t d, d
     if (!(y[i] == 0.0f || y[i] == 1.0f))
       return fail(UT_E_ARG, "ut_plan_logistic_glm: y must hold 0/1 outcomes");
   auto* plan = new ut_plan();
-  std::vector<float> xT_f((size_t)n * d);
-  std::vector<double> xT_d((size_t)n * d);
+  // feature-major, zero-padded to the model instance's feature bound
+  const int rows = logistic_feature_rows(d, k_slots(d));
+  std::vector<float> xT_f((size_t)n * rows, 0.0f);
+  std::vector<double> xT_d((size_t)n * rows, 0.0);
   for (int64_t i = 0; i < n; ++i)
     for (int j = 0; j < d; ++j) {
       xT_f[(size_t)j * n + i] = x[(size_t)i * d + j];

@@ -68,6 +68,11 @@ struct DevModelDesc {
   const ut_plan_desc* host_plan = nullptr;
 };
 
+// Feature rows of the logistic plan's staged X (feature-major, zero rows past D): the
+// register bound of the model instance that runs it (32 up to 32 covariates, else 64 per
+// slot), so the row loop needs no per-feature masks.
+inline int logistic_feature_rows(int D, int K) { return D <= 32 ? 32 : 64 * K; }
+
 // LDS of one wavefront's NUTS checkpoint stack (positions and momenta of the
 // dyadic checkpoints, max_depth + 1 of each).
 inline size_t nuts_checkpoint_bytes(int K, int max_depth) {

@@ -347,8 +347,17 @@ UT_HD float log1p_exp_f(float eta) {
 UT_HD double log1p_exp_d(double eta) {
   return eta > 0.0 ? eta + log1p_d(exp_d(-eta)) : log1p_d(exp_d(eta));
 }
-// logistic(eta) = exp(-log1p_exp(-eta)) (reference math.jl:224).
-UT_HD float logistic_f(float eta) { return exp_f(-log1p_exp_f(-eta)); }
-UT_HD double logistic_d(double eta) { return exp_d(-log1p_exp_d(-eta)); }
+// logistic(eta) = 1 / (1 + e^-eta), the reference's exp(-log1p_exp(-eta)) (math.jl:224),
+// evaluated from t = e^-|eta| as 1 / (1 + t) for eta > 0 and t / (1 + t) otherwise: one
+// exponential with the argument log1p_exp takes (a caller computing both shares it)
+// and no cancellation for either sign.
+UT_HD float logistic_f(float eta) {
+  const float t = exp_f(eta > 0.0f ? -eta : eta);
+  return eta > 0.0f ? 1.0f / (1.0f + t) : t / (1.0f + t);
+}
+UT_HD double logistic_d(double eta) {
+  const double t = exp_d(eta > 0.0 ? -eta : eta);
+  return eta > 0.0 ? 1.0 / (1.0 + t) : t / (1.0 + t);
+}
 
 }  // namespace ut

@@ -196,13 +196,14 @@ struct HierModel : NoModelLds {
 // 64 lanes reads 64 consecutive words of each feature.  The D gradient partials are
 // folded with one butterfly each and lane d keeps feature d's total.
 //
-// One lane's rows (logistic_rows): per row eta = ((0 + x_0 b_0) + x_1 b_1) + ...,
-// lp += y eta - log1p_exp(eta), r = y - logistic(eta), gp_d += x_d r.  DB (>= D) is
-// the compile-time bound of the register arrays.  The covariate loads are issued
-// unconditionally, all of a row (or of a 32-feature chunk) together, with the
-// feature index clamped to D - 1 and the arithmetic of features >= D skipped, so
-// their latencies overlap instead of one wait per feature; with DB <= 32 the next
-// row's covariates are loaded while the current row is scored (the row index
+// One lane's rows (logistic_rows): per row eta = fma(x_{D-1}, b_{D-1}, ... fma(x_0, b_0, 0)),
+// lp += y eta - log1p_exp(eta), r = y - logistic(eta), gp_d = fma(x_d, r, gp_d).  DB (>= D) is
+// the compile-time bound of the register arrays; the plan stages X with DB feature rows,
+// rows D.. zero, and the padded coefficients are zero, so the loop runs over all DB
+// features without masks (fma(0, 0, eta) leaves eta's value unchanged, up to the sign of
+// a zero eta, which no later step distinguishes).  The covariate loads of a row (or of a
+// 32-feature chunk) are issued together so their latencies overlap; with DB <= 32 the
+// next row's covariates are loaded while the current row is scored (the row index
 // clamped to n - 1 on the last step).
 template <int DB, class T>
 __device__ inline void logistic_rows(const float* xT_f, const double* xT_d, const float* y_f,
@@ -213,7 +214,7 @@ __device__ inline void logistic_rows(const float* xT_f, const double* xT_d, cons
     T yv = T(0);
     auto load = [&](long long r) {
 #pragma unroll
-      for (int d = 0; d < DB; ++d) xv[d] = Pick<T>::get(xT_f, xT_d, (long long)(d < D ? d : D - 1) * n + r);
+      for (int d = 0; d < DB; ++d) xv[d] = Pick<T>::get(xT_f, xT_d, (long long)d * n + r);
       yv = Pick<T>::get(y_f, y_d, r);
     };
     if (first < n) load(first);
@@ -225,13 +226,11 @@ __device__ inline void logistic_rows(const float* xT_f, const double* xT_d, cons
       load(i + step < n ? i + step : n - 1);
       T eta = T(0);
 #pragma unroll
-      for (int d = 0; d < DB; ++d)
-        if (d < D) eta = eta + xc[d] * beta[d];
+      for (int d = 0; d < DB; ++d) eta = t_fma<T>(xc[d], beta[d], eta);
       lp = lp + (yi * eta - t_log1p_exp<T>(eta));
       const T r = yi - t_logistic<T>(eta);
 #pragma unroll
-      for (int d = 0; d < DB; ++d)
-        if (d < D) gp[d] = gp[d] + xc[d] * r;
+      for (int d = 0; d < DB; ++d) gp[d] = t_fma<T>(xc[d], r, gp[d]);
     }
   } else {
     constexpr int CH = 32;
@@ -243,11 +242,9 @@ __device__ inline void logistic_rows(const float* xT_f, const double* xT_d, cons
         if (c < D) {
           T xv[CH];
 #pragma unroll
-          for (int j = 0; j < CH; ++j)
-            xv[j] = Pick<T>::get(xT_f, xT_d, (long long)(c + j < D ? c + j : D - 1) * n + i);
+          for (int j = 0; j < CH; ++j) xv[j] = Pick<T>::get(xT_f, xT_d, (long long)(c + j) * n + i);
 #pragma unroll
-          for (int j = 0; j < CH; ++j)
-            if (c + j < D) eta = eta + xv[j] * beta[c + j];
+          for (int j = 0; j < CH; ++j) eta = t_fma<T>(xv[j], beta[c + j], eta);
         }
       }
       lp = lp + (yi * eta - t_log1p_exp<T>(eta));
@@ -257,11 +254,9 @@ __device__ inline void logistic_rows(const float* xT_f, const double* xT_d, cons
         if (c < D) {
           T xv[CH];
 #pragma unroll
-          for (int j = 0; j < CH; ++j)
-            xv[j] = Pick<T>::get(xT_f, xT_d, (long long)(c + j < D ? c + j : D - 1) * n + i);
+          for (int j = 0; j < CH; ++j) xv[j] = Pick<T>::get(xT_f, xT_d, (long long)(c + j) * n + i);
 #pragma unroll
-          for (int j = 0; j < CH; ++j)
-            if (c + j < D) gp[c + j] = gp[c + j] + xv[j] * r;
+          for (int j = 0; j < CH; ++j) gp[c + j] = t_fma<T>(xv[j], r, gp[c + j]);
         }
       }
     }
