@@ -275,6 +275,31 @@ def test_flow_graph_replay_matches_eager(monkeypatch):
         assert np.array_equal(a.integration_steps, b.integration_steps)
 
 
+@pytest.mark.parametrize("dim,layers", [(10, 4), (100, 6)])
+def test_fused_leapfrog_step_matches_separate_kernel(monkeypatch, dim, layers):
+    """With a flow on the 16-row kernels the lock-step samplers take each leapfrog step
+    after the first in the flow backward's epilogue; with UT_FUSED_STEP=0 the step is
+    k_ls_step after the backward.  Same arithmetic: whole shared-HMC and ChEES runs must
+    agree bit for bit (the funnel at the benchmark's width too)."""
+    say("fused leapfrog step", dim, layers)
+    model = M.NealsFunnel(dim, 3.0)
+    flow = ut.AffineCouplingFlow(model.num_params, layers, 256, seed=5)
+    flow.weights = _random_weights(flow, layers, scale=0.2)
+    runs = {}
+    for mode in ("0", "1"):
+        monkeypatch.setenv("UT_FUSED_STEP", mode)
+        h = ut.batched_hmc(model, num_chains=256, num_samples=20, num_warmup=30, num_leapfrog_steps=6,
+                           per_chain_adaptation=False, flow=flow, seed=23)
+        c = ut.batched_chees(model, num_chains=256, num_samples=20, num_warmup=30, flow=flow, seed=24)
+        runs[mode] = (h, c)
+    for a, b in zip(runs["0"], runs["1"]):
+        assert np.array_equal(a.unconstrained_samples.view(np.uint32), b.unconstrained_samples.view(np.uint32))
+        assert np.array_equal(a.logjoint_values, b.logjoint_values)
+        assert np.array_equal(a.step_size, b.step_size)
+        assert np.array_equal(a.integration_steps, b.integration_steps)
+        assert np.array_equal(a.divergent, b.divergent)
+
+
 def test_flow_hmc_funnel_gains_over_plain():
     """NeuTra on Neal's funnel (dim 10): the same shared-adaptation HMC budget with
     and without a flow trained on exact funnel draws."""

@@ -169,12 +169,14 @@ struct ut_workspace {
     float* lj;
     float* grad;
     bool need_lj;
+    bool fwd_only;  // the flow forward and the model gradient only (the backward launched apart)
     const void* flow;
     unsigned gen;
     hipGraphExec_t exec;
   };
   std::vector<GradGraph> graphs;
   bool use_graphs = true;
+  bool fused_step = true;  // lock-step leapfrog kick and drift in the flow backward's epilogue
   void clear_graphs() {
     for (GradGraph& g : graphs) (void)hipGraphExecDestroy(g.exec);
     graphs.clear();
@@ -801,6 +803,9 @@ int ut_workspace_create(const ut_plan* plan, int32_t num_chains, int32_t precisi
     // UT_FLOW_GRAPHS=0 keeps the flow-wrapped gradient eager (for comparison runs)
     const char* g = std::getenv("UT_FLOW_GRAPHS");
     ws->use_graphs = !(g && std::strcmp(g, "0") == 0);
+    // UT_FUSED_STEP=0 keeps k_ls_step a kernel of its own (for comparison runs)
+    const char* fs = std::getenv("UT_FUSED_STEP");
+    ws->fused_step = !(fs && std::strcmp(fs, "0") == 0);
   }
   const size_t PC = (size_t)ws->P * num_chains;
   const size_t es = precision == UT_FLOAT32 ? 4 : 8;
@@ -877,7 +882,8 @@ static hipError_t batched_grad(ut_workspace* ws, const float* q, float* lj, floa
     }
   }
   for (const ut_workspace::GradGraph& g : ws->graphs)
-    if (g.q == q && g.lj == lj && g.grad == grad && g.need_lj == need_lj) return hipGraphLaunch(g.exec, ws->stream);
+    if (!g.fwd_only && g.q == q && g.lj == lj && g.grad == grad && g.need_lj == need_lj)
+      return hipGraphLaunch(g.exec, ws->stream);
   hipError_t e = hipStreamBeginCapture(ws->stream, hipStreamCaptureModeThreadLocal);
   if (e != hipSuccess) return e;
   const hipError_t ec = flow_grad_eager(ws, q, lj, grad, need_lj);
@@ -891,8 +897,70 @@ static hipError_t batched_grad(ut_workspace* ws, const float* q, float* lj, floa
   e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
   (void)hipGraphDestroy(graph);
   if (e != hipSuccess) return e;
-  ws->graphs.push_back({q, lj, grad, need_lj, &f, f.gen, exec});
+  ws->graphs.push_back({q, lj, grad, need_lj, false, &f, f.gen, exec});
   return hipGraphLaunch(exec, ws->stream);
+}
+
+// One leapfrog step of the lock-step samplers after the first: the batched gradient at
+// ls.wq, then the full kick and drift (k_ls_step).  With a flow on the 16-row kernels the
+// step runs in the flow backward's epilogue: the flow forward and the model gradient are
+// replayed from a HIP graph, and the backward (with the step size as an argument) is
+// launched after it; the arithmetic is k_ls_step's, so the draws are the same.
+static hipError_t batched_grad_step(ut_workspace* ws, const LockstepState& ls, int P, int C, float eps) {
+  if (!(ws->flow && ws->flow->f.m16 && ws->fused_step)) {
+    hipError_t e = batched_grad(ws, ls.wq, ls.wlj, ls.wg, false);
+    return e == hipSuccess ? launch_ls_step(ls, P, C, eps, ws->stream) : e;
+  }
+  Flow& f = ws->flow->f;
+  hipError_t e = hipSuccess;
+  auto forward = [&]() -> hipError_t {
+    hipError_t r = flow_forward(f, ls.wq, nullptr, nullptr, ws->C, ws->stream);
+    return r == hipSuccess ? model_grad(ws, f.sc.x, ls.wlj, f.sc.g, false) : r;
+  };
+  if (!ws->use_graphs) {
+    e = forward();
+  } else {
+    for (const ut_workspace::GradGraph& g : ws->graphs) {
+      if (g.flow != &f || g.gen != f.gen) {
+        ws->clear_graphs();
+        break;
+      }
+    }
+    const ut_workspace::GradGraph* hit = nullptr;
+    for (const ut_workspace::GradGraph& g : ws->graphs)
+      if (g.fwd_only && g.q == ls.wq && g.lj == ls.wlj) hit = &g;
+    if (hit) {
+      e = hipGraphLaunch(hit->exec, ws->stream);
+    } else {
+      e = hipStreamBeginCapture(ws->stream, hipStreamCaptureModeThreadLocal);
+      if (e != hipSuccess) return e;
+      const hipError_t ec = forward();
+      hipGraph_t graph = nullptr;
+      e = hipStreamEndCapture(ws->stream, &graph);
+      if (ec != hipSuccess || e != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return ec != hipSuccess ? ec : e;
+      }
+      hipGraphExec_t exec = nullptr;
+      e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (e != hipSuccess) return e;
+      ws->graphs.push_back({ls.wq, ls.wlj, nullptr, false, true, &f, f.gen, exec});
+      e = hipGraphLaunch(exec, ws->stream);
+    }
+  }
+  if (e != hipSuccess) return e;
+  if (!f.sc.fwd16) {  // the forward ran other kernels: their backward, then the step
+    e = flow_backward(f, f.sc.g, ls.wg, ws->C, ws->stream);
+    return e == hipSuccess ? launch_ls_step(ls, P, C, eps, ws->stream) : e;
+  }
+  Flow16Step stp;
+  stp.wq = ls.wq;
+  stp.wp = ls.wp;
+  stp.im_f = ls.im_f;
+  stp.valid = ls.valid;
+  stp.eps = eps;
+  return flow16_backward_step(f, f.sc.g, ls.wg, ws->C, stp, ws->stream);
 }
 
 static int density_call(ut_workspace* ws, const void* params, void* values, void* gradients) {
@@ -1795,10 +1863,7 @@ int run_lockstep(ut_workspace* ws, const ut_chees_opts* co, ut_chain_out* out, d
     const bool warm = it <= W;
     const float eps_f = (float)eps;
     UT_HIP(launch_ls_begin(ls, P, C, itc, STREAM_MOMENTUM, eps_f, (warm && adapt_traj) ? 1 : 0, stream));
-    for (int l = 1; l < L; ++l) {
-      UT_HIP(batched_grad(ws, ls.wq, ls.wlj, ls.wg, false));
-      UT_HIP(launch_ls_step(ls, P, C, eps_f, stream));
-    }
+    for (int l = 1; l < L; ++l) UT_HIP(batched_grad_step(ws, ls, P, C, eps_f));
     UT_HIP(batched_grad(ws, ls.wq, ls.wlj, ls.wg, true));
     UT_HIP(launch_ls_end(ls, rec, P, C, itc, eps_f, o->divergence_threshold, 0, stream));
     UT_HIP(launch_ls_record(ls, rec, sp, it, L, stream));

@@ -147,6 +147,21 @@ hipError_t flow16_forward(Flow& f, const float* z, float* x, float* q, float* lo
                           hipStream_t st);
 // g_z = (dq/dz)^T g_q + d logdet / dz for the rows of the last flow16_forward
 hipError_t flow16_backward(Flow& f, const float* g_q, float* g_z, int B, hipStream_t st);
+// A lock-step leapfrog's full kick and drift applied to the rows of a backward pass (the
+// lock-step samplers' k_ls_step, same arithmetic): chain-major positions and momenta,
+// the shared inverse mass and the per-chain validity flags.
+struct Flow16Step {
+  float* wq = nullptr;
+  float* wp = nullptr;
+  const float* im_f = nullptr;
+  uint8_t* valid = nullptr;
+  float eps = 0.0f;
+};
+// flow16_backward with the leapfrog step in its epilogue: g_z is written as by
+// flow16_backward, then for every row of a still-valid chain either the step is taken
+// (all of the row's g_z finite) or the chain is marked invalid.
+hipError_t flow16_backward_step(Flow& f, const float* g_q, float* g_z, int B, const Flow16Step& stp,
+                                hipStream_t st);
 
 // factor = true: the full-rank guide (L must be 0), mu, ls and the strictly lower
 // factor as its weights

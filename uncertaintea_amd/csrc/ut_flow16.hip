@@ -31,6 +31,7 @@
 // agree with the per-step path to fp32 rounding (and the occasional bf16 rounding flip of an
 // operand), not bit for bit.
 #include "ut_flow.h"
+#include "ut_math.h"
 #include "ut_mfma.h"
 
 namespace ut {
@@ -354,10 +355,10 @@ __device__ inline void bwd16_load(const Flow16Layer& ly, Bwd16Regs& g) {
     for (int r = 0; r < 4; ++r) g.b1[t][r] = b1[32 * w + 16 * t + 4 * lg + r];
 }
 
-template <int LT>
+template <int LT, bool STEP>
 __global__ __launch_bounds__(THREADS16) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_flow16_bwd(
     const float* g_q, const float* __restrict__ prm, float* gz, int P, const Flow16Layer* __restrict__ layers, int L_,
-    float c) {
+    float c, Flow16Step stp) {
   const int L = LT > 0 ? LT : L_;
   extern __shared__ __attribute__((aligned(16))) float lds16[];
   float* Gs = lds16;                      // 16 x P fp32 gradient rows
@@ -496,6 +497,11 @@ __global__ __launch_bounds__(THREADS16) __attribute__((amdgpu_waves_per_eu(2, 2)
     }
   }
   // base backward: g_z = g_x e^ls
+  __shared__ int row_ok[R16];
+  if (STEP) {
+    if (tid < R16) row_ok[tid] = 1;
+    __syncthreads();
+  }
   for (int e0 = tid; e0 < R16 * P; e0 += 4 * THREADS16) {
     float lv[4];
 #pragma unroll
@@ -503,14 +509,35 @@ __global__ __launch_bounds__(THREADS16) __attribute__((amdgpu_waves_per_eu(2, 2)
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       const int e = e0 + THREADS16 * k;
-      if (e < R16 * P) gz[r0 * P + e] = Gs[e] * expf(lv[k]);
+      if (e < R16 * P) {
+        const float gv = Gs[e] * expf(lv[k]);
+        gz[r0 * P + e] = gv;
+        if (STEP && !isfinite_f(gv)) row_ok[e / P] = 0;
+      }
     }
+  }
+  if (STEP) {
+    // the lock-step leapfrog's full kick and drift (k_ls_step): rows whose gradient is
+    // finite move, a chain with a non-finite entry is marked invalid, invalid chains stay
+    __syncthreads();
+    for (int e = tid; e < R16 * P; e += THREADS16) {
+      const int row = e / P, a = e - row * P;
+      const long long cr = r0 + row;
+      if (row_ok[row] && stp.valid[cr]) {
+        const float gv = Gs[e] * expf(prm[P + a]);
+        const float pv = stp.wp[cr * P + a] + (1.0f * stp.eps) * gv;
+        const float qv = stp.wq[cr * P + a] + (stp.eps * stp.im_f[a]) * pv;
+        stp.wq[cr * P + a] = qv;
+        stp.wp[cr * P + a] = pv;
+      }
+    }
+    if (tid < R16 && !row_ok[tid] && stp.valid[r0 + tid]) stp.valid[r0 + tid] = 0;
   }
 }
 
 typedef void (*Fwd16Fn)(const float*, const float*, const float*, float*, float*, int, const Flow16Layer*, int, float,
                         float*, float*);
-typedef void (*Bwd16Fn)(const float*, const float*, float*, int, const Flow16Layer*, int, float);
+typedef void (*Bwd16Fn)(const float*, const float*, float*, int, const Flow16Layer*, int, float, Flow16Step);
 Fwd16Fn fwd16_kernel(int L) {
   switch (L) {
     case 2: return k_flow16_fwd<2>;
@@ -520,13 +547,14 @@ Fwd16Fn fwd16_kernel(int L) {
     default: return k_flow16_fwd<0>;
   }
 }
+template <bool STEP>
 Bwd16Fn bwd16_kernel(int L) {
   switch (L) {
-    case 2: return k_flow16_bwd<2>;
-    case 4: return k_flow16_bwd<4>;
-    case 6: return k_flow16_bwd<6>;
-    case 8: return k_flow16_bwd<8>;
-    default: return k_flow16_bwd<0>;
+    case 2: return k_flow16_bwd<2, STEP>;
+    case 4: return k_flow16_bwd<4, STEP>;
+    case 6: return k_flow16_bwd<6, STEP>;
+    case 8: return k_flow16_bwd<8, STEP>;
+    default: return k_flow16_bwd<0, STEP>;
   }
 }
 size_t fwd16_lds(int P) { return (size_t)R16 * P * 4 + (size_t)R16 * XPS16 * 2 + (size_t)R16 * HS16 * 2 + NW16 * 16 * 4; }
@@ -561,8 +589,16 @@ hipError_t flow16_forward(Flow& f, const float* z, float* x, float* q, float* lo
 
 hipError_t flow16_backward(Flow& f, const float* g_q, float* g_z, int B, hipStream_t st) {
   if (B % R16) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(bwd16_kernel(f.L), dim3(B / R16), dim3(THREADS16), bwd16_lds(f.P), st, g_q, f.params, g_z, f.P,
-                     f.d16, f.L, f.clamp);
+  hipLaunchKernelGGL(bwd16_kernel<false>(f.L), dim3(B / R16), dim3(THREADS16), bwd16_lds(f.P), st, g_q, f.params, g_z,
+                     f.P, f.d16, f.L, f.clamp, Flow16Step{});
+  return hipGetLastError();
+}
+
+hipError_t flow16_backward_step(Flow& f, const float* g_q, float* g_z, int B, const Flow16Step& stp,
+                                hipStream_t st) {
+  if (B % R16 || !stp.wq || !stp.wp || !stp.im_f || !stp.valid) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(bwd16_kernel<true>(f.L), dim3(B / R16), dim3(THREADS16), bwd16_lds(f.P), st, g_q, f.params, g_z,
+                     f.P, f.d16, f.L, f.clamp, stp);
   return hipGetLastError();
 }
 
