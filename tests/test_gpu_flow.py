@@ -278,16 +278,19 @@ def test_flow_graph_replay_matches_eager(monkeypatch):
 @pytest.mark.parametrize("dim,layers", [(10, 4), (100, 6)])
 def test_fused_leapfrog_step_matches_separate_kernel(monkeypatch, dim, layers):
     """With a flow on the 16-row kernels the lock-step samplers take each leapfrog step
-    after the first in the flow backward's epilogue; with UT_FUSED_STEP=0 the step is
-    k_ls_step after the backward.  Same arithmetic: whole shared-HMC and ChEES runs must
-    agree bit for bit (the funnel at the benchmark's width too)."""
-    say("fused leapfrog step", dim, layers)
+    after the first in the flow backward's epilogue, and the funnel's gradient comes from
+    the flow forward's epilogue; with UT_FUSED_STEP=0 and UT_FUSED_MODEL=0 the step is
+    k_ls_step after the backward and the model the density kernel after the forward.  Same
+    arithmetic: whole shared-HMC and ChEES runs must agree bit for bit (the funnel at the
+    benchmark's width too)."""
+    say("fused leapfrog step and row model", dim, layers)
     model = M.NealsFunnel(dim, 3.0)
     flow = ut.AffineCouplingFlow(model.num_params, layers, 256, seed=5)
     flow.weights = _random_weights(flow, layers, scale=0.2)
     runs = {}
     for mode in ("0", "1"):
         monkeypatch.setenv("UT_FUSED_STEP", mode)
+        monkeypatch.setenv("UT_FUSED_MODEL", mode)
         h = ut.batched_hmc(model, num_chains=256, num_samples=20, num_warmup=30, num_leapfrog_steps=6,
                            per_chain_adaptation=False, flow=flow, seed=23)
         c = ut.batched_chees(model, num_chains=256, num_samples=20, num_warmup=30, flow=flow, seed=24)

@@ -177,6 +177,7 @@ struct ut_workspace {
   std::vector<GradGraph> graphs;
   bool use_graphs = true;
   bool fused_step = true;  // lock-step leapfrog kick and drift in the flow backward's epilogue
+  bool fused_model = true;  // a row model's gradient in the flow forward's epilogue
   void clear_graphs() {
     for (GradGraph& g : graphs) (void)hipGraphExecDestroy(g.exec);
     graphs.clear();
@@ -806,6 +807,9 @@ int ut_workspace_create(const ut_plan* plan, int32_t num_chains, int32_t precisi
     // UT_FUSED_STEP=0 keeps k_ls_step a kernel of its own (for comparison runs)
     const char* fs = std::getenv("UT_FUSED_STEP");
     ws->fused_step = !(fs && std::strcmp(fs, "0") == 0);
+    // UT_FUSED_MODEL=0 keeps the funnel's density kernel a launch of its own
+    const char* fm = std::getenv("UT_FUSED_MODEL");
+    ws->fused_model = !(fm && std::strcmp(fm, "0") == 0);
   }
   const size_t PC = (size_t)ws->P * num_chains;
   const size_t es = precision == UT_FLOAT32 ? 4 : 8;
@@ -859,10 +863,27 @@ static hipError_t model_grad(ut_workspace* ws, const float* q, float* lj, float*
 // z: q = f(z), the model's logjoint and gradient at q, then
 //   log p_z(z) = log p(q) + log |det df/dz|,  grad_z = (dq/dz)^T grad_q + grad_z logdet
 // (the NeuTra reparameterisation).
+// The flow forward and the model's gradient at its rows (into f.sc.g): Neal's funnel, whose
+// terms are row-local, is evaluated in the 16-row forward kernel's epilogue (the density
+// kernel's arithmetic); every other model by its own gradient engine after the forward.
+static hipError_t flow_forward_grad(ut_workspace* ws, const float* q, float* lj, bool need_lj) {
+  Flow& f = ws->flow->f;
+  if (ws->fused_model && ws->dev.kind == MODEL_FUNNEL && f.m16 && (ws->dev.K == 1 || ws->dev.K == 2)) {
+    FlowRowModel rm;
+    rm.K = ws->dev.K;
+    rm.dim = ws->dev.dim;
+    rm.v_scale = ws->dev.v_scale;
+    rm.total_const = ws->dev.total_const;
+    const hipError_t e = flow_forward_model(f, q, rm, f.sc.g, lj, ws->C, ws->stream);
+    if (e != hipErrorNotSupported) return e;
+  }
+  hipError_t e = flow_forward(f, q, nullptr, nullptr, ws->C, ws->stream);
+  return e == hipSuccess ? model_grad(ws, f.sc.x, lj, f.sc.g, need_lj) : e;
+}
+
 static hipError_t flow_grad_eager(ut_workspace* ws, const float* q, float* lj, float* grad, bool need_lj) {
   Flow& f = ws->flow->f;
-  hipError_t e = flow_forward(f, q, nullptr, nullptr, ws->C, ws->stream);
-  if (e == hipSuccess) e = model_grad(ws, f.sc.x, lj, f.sc.g, need_lj);
+  hipError_t e = flow_forward_grad(ws, q, lj, need_lj);
   if (e == hipSuccess) e = flow_backward(f, f.sc.g, grad, ws->C, ws->stream);
   if (e == hipSuccess && need_lj) e = flow_add_logdet(lj, f.sc.logdet, ws->C, ws->stream);
   return e;
@@ -913,10 +934,7 @@ static hipError_t batched_grad_step(ut_workspace* ws, const LockstepState& ls, i
   }
   Flow& f = ws->flow->f;
   hipError_t e = hipSuccess;
-  auto forward = [&]() -> hipError_t {
-    hipError_t r = flow_forward(f, ls.wq, nullptr, nullptr, ws->C, ws->stream);
-    return r == hipSuccess ? model_grad(ws, f.sc.x, ls.wlj, f.sc.g, false) : r;
-  };
+  auto forward = [&]() -> hipError_t { return flow_forward_grad(ws, ls.wq, ls.wlj, false); };
   if (!ws->use_graphs) {
     e = forward();
   } else {

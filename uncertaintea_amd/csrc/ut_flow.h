@@ -147,6 +147,23 @@ hipError_t flow16_forward(Flow& f, const float* z, float* x, float* q, float* lo
                           hipStream_t st);
 // g_z = (dq/dz)^T g_q + d logdet / dz for the rows of the last flow16_forward
 hipError_t flow16_backward(Flow& f, const float* g_q, float* g_z, int B, hipStream_t st);
+// A model whose log density and gradient one row's coordinates give by themselves (Neal's
+// funnel), evaluated in the flow forward's epilogue on the rows it produces, with the device
+// model's own arithmetic (FunnelModel<K>, one wavefront per row, as in the density kernel).
+struct FlowRowModel {
+  int K = 0;  // register slots per lane of the density kernel's instance (1 or 2)
+  int dim = 0;
+  double v_scale = 1.0, total_const = 0.0;
+};
+// flow16_forward followed by the row model's gradient (g, P x B chain-major as the density
+// kernel writes it) and log density (lj, without the flow's log-determinant)
+hipError_t flow16_forward_model(Flow& f, const float* z, float* x, float* logdet, const FlowRowModel& m, float* g,
+                                float* lj, int B, hipStream_t st);
+// The sampling-path forward with the row model fused; hipErrorNotSupported unless the
+// 16-row kernels run this flow.
+hipError_t flow_forward_model(Flow& f, const float* z, const FlowRowModel& m, float* g, float* lj, int B,
+                              hipStream_t st);
+
 // A lock-step leapfrog's full kick and drift applied to the rows of a backward pass (the
 // lock-step samplers' k_ls_step, same arithmetic): chain-major positions and momenta,
 // the shared inverse mass and the per-chain validity flags.

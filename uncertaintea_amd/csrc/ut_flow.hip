@@ -2036,6 +2036,18 @@ hipError_t flow_forward_train(Flow& f, const float* z, int B, hipStream_t st) {
   return forward_impl(f, z, nullptr, nullptr, B, false, st);
 }
 
+hipError_t flow_forward_model(Flow& f, const float* z, const FlowRowModel& m, float* g, float* lj, int B,
+                              hipStream_t st) {
+  if (B % FLOW_PAD || B > f.sc.B) return hipErrorInvalidValue;
+  FlowScratch& s = f.sc;
+  bool all = f.m16 && f.L > 0;
+  for (int l = 0; l < f.L && all; ++l) all = fused_layer(f, l);
+  if (!all) return hipErrorNotSupported;
+  for (int l = 0; l < f.L; ++l) s.fwd_fused[l] = true;
+  s.fwd16 = true;
+  return flow16_forward_model(f, z, s.x, s.logdet, m, g, lj, B, st);
+}
+
 hipError_t flow_backward(Flow& f, const float* g_q, float* g_z, int B, hipStream_t st) {
   if (B % FLOW_PAD || B > f.sc.B) return hipErrorInvalidValue;
   FlowScratch& s = f.sc;

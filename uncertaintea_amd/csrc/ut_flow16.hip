@@ -30,15 +30,20 @@
 // flow_tanh, the clamp c tanh(s / c)), summed in the orders of this kernel's tiles, so results
 // agree with the per-step path to fp32 rounding (and the occasional bf16 rounding flip of an
 // operand), not bit for bit.
+#include <type_traits>
+
 #include "ut_flow.h"
 #include "ut_math.h"
 #include "ut_mfma.h"
+#include "ut_models.h"
 
 namespace ut {
 
 namespace {
 
 #define UT_GLOBAL16 __attribute__((address_space(1)))
+
+struct NoRowModel {};
 template <typename T>
 __device__ inline UT_GLOBAL16 T* g16(T* p) {
   return (UT_GLOBAL16 T*)p;
@@ -161,11 +166,12 @@ __device__ inline void fwd16_load2(const Flow16Layer& ly, Fwd16Regs& g) {
   }
 }
 
-template <int LT>
+template <int LT, class RM>
 __global__ __launch_bounds__(THREADS16) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_flow16_fwd(
     const float* __restrict__ z, const float* __restrict__ prm, const float* __restrict__ ldbase,
     float* __restrict__ x, float* __restrict__ q, int P, const Flow16Layer* __restrict__ layers, int L_, float c,
-    float* __restrict__ logdet, float* __restrict__ logdet_out) {
+    float* __restrict__ logdet, float* __restrict__ logdet_out, RM rm, float* __restrict__ g_rm,
+    float* __restrict__ lj_rm) {
   const int L = LT > 0 ? LT : L_;
   extern __shared__ __attribute__((aligned(16))) float lds16[];
   float* Xs = lds16;                          // 16 x P fp32 rows
@@ -322,6 +328,28 @@ __global__ __launch_bounds__(THREADS16) __attribute__((amdgpu_waves_per_eu(2, 2)
     const float v = Xs[e];
     x[r0 * P + e] = v;
     if (q) q[r0 * P + e] = v;
+  }
+  if constexpr (!std::is_same<RM, NoRowModel>::value) {
+    // the row model on the rows just produced: one wavefront per row, the density
+    // kernel's loads, arithmetic and stores
+    typename RM::template Lane<float> ld;
+    rm.prepare(ld);
+    for (int rr = wu; rr < R16; rr += NW16) {
+      float qv[RM::K], gv[RM::K];
+#pragma unroll
+      for (int k = 0; k < RM::K; ++k) {
+        const int a = rm.api(lane, k);
+        qv[k] = a >= 0 ? Xs[rr * P + a] : 0.0f;
+      }
+      bool ok;
+      const float val = rm.eval(ld, qv, gv, ok);
+#pragma unroll
+      for (int k = 0; k < RM::K; ++k) {
+        const int a = rm.api(lane, k);
+        if (a >= 0) g_rm[(r0 + rr) * P + a] = gv[k];
+      }
+      if (lane == 0) lj_rm[r0 + rr] = val;
+    }
   }
 }
 
@@ -535,16 +563,18 @@ __global__ __launch_bounds__(THREADS16) __attribute__((amdgpu_waves_per_eu(2, 2)
   }
 }
 
-typedef void (*Fwd16Fn)(const float*, const float*, const float*, float*, float*, int, const Flow16Layer*, int, float,
-                        float*, float*);
+template <class RM>
+using Fwd16Fn = void (*)(const float*, const float*, const float*, float*, float*, int, const Flow16Layer*, int, float,
+                         float*, float*, RM, float*, float*);
 typedef void (*Bwd16Fn)(const float*, const float*, float*, int, const Flow16Layer*, int, float, Flow16Step);
-Fwd16Fn fwd16_kernel(int L) {
+template <class RM>
+Fwd16Fn<RM> fwd16_kernel(int L) {
   switch (L) {
-    case 2: return k_flow16_fwd<2>;
-    case 4: return k_flow16_fwd<4>;
-    case 6: return k_flow16_fwd<6>;
-    case 8: return k_flow16_fwd<8>;
-    default: return k_flow16_fwd<0>;
+    case 2: return k_flow16_fwd<2, RM>;
+    case 4: return k_flow16_fwd<4, RM>;
+    case 6: return k_flow16_fwd<6, RM>;
+    case 8: return k_flow16_fwd<8, RM>;
+    default: return k_flow16_fwd<0, RM>;
   }
 }
 template <bool STEP>
@@ -582,9 +612,26 @@ hipError_t flow16_sync_weights(Flow& f, hipStream_t st) {
 hipError_t flow16_forward(Flow& f, const float* z, float* x, float* q, float* logdet, float* logdet_out, int B,
                           hipStream_t st) {
   if (B % R16) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(fwd16_kernel(f.L), dim3(B / R16), dim3(THREADS16), fwd16_lds(f.P), st, z, f.params, f.ldbase, x,
-                     q, f.P, f.d16, f.L, f.clamp, logdet, logdet_out);
+  hipLaunchKernelGGL(fwd16_kernel<NoRowModel>(f.L), dim3(B / R16), dim3(THREADS16), fwd16_lds(f.P), st, z, f.params,
+                     f.ldbase, x, q, f.P, f.d16, f.L, f.clamp, logdet, logdet_out, NoRowModel{}, nullptr, nullptr);
   return hipGetLastError();
+}
+
+hipError_t flow16_forward_model(Flow& f, const float* z, float* x, float* logdet, const FlowRowModel& m, float* g,
+                                float* lj, int B, hipStream_t st) {
+  if (B % R16 || !g || !lj) return hipErrorInvalidValue;
+  auto launch = [&](auto fm) {
+    fm.dim = m.dim;
+    fm.v_scale = m.v_scale;
+    fm.total_const = m.total_const;
+    using FM = decltype(fm);
+    hipLaunchKernelGGL(fwd16_kernel<FM>(f.L), dim3(B / R16), dim3(THREADS16), fwd16_lds(f.P), st, z, f.params,
+                       f.ldbase, x, nullptr, f.P, f.d16, f.L, f.clamp, logdet, nullptr, fm, g, lj);
+    return hipGetLastError();
+  };
+  if (m.K == 1) return launch(FunnelModel<1>{});
+  if (m.K == 2) return launch(FunnelModel<2>{});
+  return hipErrorInvalidValue;
 }
 
 hipError_t flow16_backward(Flow& f, const float* g_q, float* g_z, int B, hipStream_t st) {
