@@ -933,6 +933,22 @@ static hipError_t batched_grad_step(ut_workspace* ws, const LockstepState& ls, i
     return e == hipSuccess ? launch_ls_step(ls, P, C, eps, ws->stream) : e;
   }
   Flow& f = ws->flow->f;
+  if (ws->fused_model && ws->dev.kind == MODEL_FUNNEL && (ws->dev.K == 1 || ws->dev.K == 2)) {
+    // the funnel: forward, model, backward and step in one kernel per 16-row tile
+    FlowRowModel rm;
+    rm.K = ws->dev.K;
+    rm.dim = ws->dev.dim;
+    rm.v_scale = ws->dev.v_scale;
+    rm.total_const = ws->dev.total_const;
+    Flow16Step stp;
+    stp.wq = ls.wq;
+    stp.wp = ls.wp;
+    stp.im_f = ls.im_f;
+    stp.valid = ls.valid;
+    stp.eps = eps;
+    const hipError_t e = flow16_grad_step(f, ls.wq, rm, ls.wlj, ls.wg, ws->C, stp, ws->stream);
+    if (e != hipErrorNotSupported) return e;
+  }
   hipError_t e = hipSuccess;
   auto forward = [&]() -> hipError_t { return flow_forward_grad(ws, ls.wq, ls.wlj, false); };
   if (!ws->use_graphs) {

@@ -166,18 +166,56 @@ __device__ inline void fwd16_load2(const Flow16Layer& ly, Fwd16Regs& g) {
   }
 }
 
-template <int LT, class RM>
-__global__ __launch_bounds__(THREADS16) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_flow16_fwd(
-    const float* __restrict__ z, const float* __restrict__ prm, const float* __restrict__ ldbase,
-    float* __restrict__ x, float* __restrict__ q, int P, const Flow16Layer* __restrict__ layers, int L_, float c,
-    float* __restrict__ logdet, float* __restrict__ logdet_out, RM rm, float* __restrict__ g_rm,
-    float* __restrict__ lj_rm) {
-  const int L = LT > 0 ? LT : L_;
-  extern __shared__ __attribute__((aligned(16))) float lds16[];
-  float* Xs = lds16;                          // 16 x P fp32 rows
-  bf16_t* Xp = (bf16_t*)(Xs + R16 * P);       // 16 x XPS16 passive inputs (bf16)
-  bf16_t* Hs = Xp + R16 * XPS16;              // 16 x HS16 hidden activations (bf16)
-  float* Red = (float*)(Hs + R16 * HS16);     // 8 x 16 log-determinant partials
+// Where the forward keeps what the backward reads again: per layer the clamped scales s and
+// the scaled active inputs x_a e^s of each lane's coordinates (row stride 64), and the
+// passive-input fragments.  GlobalSaved: the layer's HBM buffers (the separate forward and
+// backward kernels); LdsSaved: LDS of the fused gradient kernel (per layer 16 x 64 of each
+// and 2 x 64 x 8 bf16 fragments).
+struct GlobalSaved {
+  __device__ void put(const Flow16Layer& ly, int, long long r0, int lr, int j, float sv, float xs) const {
+    g16(ly.s)[(r0 + lr) * KP16 + j] = sv;
+    g16(ly.xs)[(r0 + lr) * KP16 + j] = xs;
+  }
+  __device__ void put_frag(const Flow16Layer& ly, int, int ks, int lane, const bf16x8& v) const {
+    *(UT_GLOBAL16 bf16x8*)(g16(ly.xpg) + (((long long)blockIdx.x * 2 + ks) * 64 + lane) * 8) = v;
+  }
+  __device__ void get(const Flow16Layer& ly, int, long long r0, int lr, int j, float& sv, float& xs) const {
+    sv = g16(ly.s)[(r0 + lr) * KP16 + j];
+    xs = g16(ly.xs)[(r0 + lr) * KP16 + j];
+  }
+  __device__ bf16x8 get_frag(const Flow16Layer& ly, int, int ks, int lane) const {
+    return *(const UT_GLOBAL16 bf16x8*)(g16(ly.xpg) + (((long long)blockIdx.x * 2 + ks) * 64 + lane) * 8);
+  }
+};
+constexpr int SAVED16_F32 = R16 * KP16;  // floats of s (and of x_a e^s) per layer
+constexpr int SAVED16_BF = 2 * 64 * 8;   // bf16 of the passive-input fragments per layer
+struct LdsSaved {
+  float* s;
+  float* xs;
+  bf16_t* xp;
+  __device__ void put(const Flow16Layer&, int l, long long, int lr, int j, float sv, float x) const {
+    s[l * SAVED16_F32 + lr * KP16 + j] = sv;
+    xs[l * SAVED16_F32 + lr * KP16 + j] = x;
+  }
+  __device__ void put_frag(const Flow16Layer&, int l, int ks, int lane, const bf16x8& v) const {
+    *(bf16x8*)(xp + l * SAVED16_BF + (ks * 64 + lane) * 8) = v;
+  }
+  __device__ void get(const Flow16Layer&, int l, long long, int lr, int j, float& sv, float& x) const {
+    sv = s[l * SAVED16_F32 + lr * KP16 + j];
+    x = xs[l * SAVED16_F32 + lr * KP16 + j];
+  }
+  __device__ bf16x8 get_frag(const Flow16Layer&, int l, int ks, int lane) const {
+    return *(const bf16x8*)(xp + l * SAVED16_BF + (ks * 64 + lane) * 8);
+  }
+};
+
+// The forward through every layer of one workgroup's 16 rows: the base x = mu + e^ls z into Xs,
+// then per layer U^T = W1 X_p^T, h = tanh(U + b1) (Hs), O^T = W2' h^T and the coupling
+// x_a <- x_a e^s + t; returns this lane's share of its row's coupling log-determinants.
+template <int LT, class SAVE>
+__device__ __forceinline__ float flow16_fwd_body(const float* __restrict__ z, const float* __restrict__ prm, int P,
+                                                 const Flow16Layer* __restrict__ layers, int L, float c, float* Xs,
+                                                 bf16_t* Xp, bf16_t* Hs, const SAVE& save) {
   const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63, lr = lane & 15, lg = lane >> 4;
   const long long r0 = (long long)blockIdx.x * R16;
   const int j0 = 8 * w + 2 * lg;  // this lane's coupling coordinates j0, j0 + 1 (row lr)
@@ -224,15 +262,13 @@ __global__ __launch_bounds__(THREADS16) __attribute__((amdgpu_waves_per_eu(2, 2)
     const Flow16Layer& nx = layers[next2 ? l + 2 : l];
     const int na = ly.na, par = ly.par;
     // ---- U^T = W1 X_p^T: B operand = 8 passive inputs of row lr.  Every wavefront holds the
-    // same fragments; wavefront 0 stores them for the backward's recomputation of U
+    // same fragments; wavefront 0 saves them for the backward's recomputation of U
     bf16x8 xb[2];
 #pragma unroll
     for (int ks = 0; ks < 2; ++ks) xb[ks] = *(const bf16x8*)(Xp + lr * XPS16 + 32 * ks + 8 * lg);
     if (wu == 0) {
-      UT_GLOBAL16 bf16_t* xpg = g16(ly.xpg);
 #pragma unroll
-      for (int ks = 0; ks < 2; ++ks)
-        *(UT_GLOBAL16 bf16x8*)(xpg + (((long long)blockIdx.x * 2 + ks) * 64 + lane) * 8) = xb[ks];
+      for (int ks = 0; ks < 2; ++ks) save.put_frag(ly, l, ks, lane, xb[ks]);
     }
     f32x4 u[2];
 #pragma unroll
@@ -276,8 +312,6 @@ __global__ __launch_bounds__(THREADS16) __attribute__((amdgpu_waves_per_eu(2, 2)
       const float cb2s[2] = {g.b2s[0], g.b2s[1]}, cb2t[2] = {g.b2t[0], g.b2t[1]};
       __builtin_amdgcn_sched_barrier(0);
       if (next2) fwd16_load2(nx, g);
-      UT_GLOBAL16 float* sg = g16(ly.s);
-      UT_GLOBAL16 float* xsg = g16(ly.xs);
 #pragma unroll
       for (int e = 0; e < 2; ++e) {
         const int j = j0 + e;
@@ -288,9 +322,7 @@ __global__ __launch_bounds__(THREADS16) __attribute__((amdgpu_waves_per_eu(2, 2)
           xn[e] = xs + (o[2 * e + 1] + cb2t[e]);
           Xs[lr * P + ak] = xn[e];
           ldp += sv;
-          // saved for the backward (row stride 64)
-          sg[(r0 + lr) * KP16 + j] = sv;
-          xsg[(r0 + lr) * KP16 + j] = xs;
+          save.put(ly, l, r0, lr, j, sv, xs);  // for the backward
         }
       }
     } else if (next2) {
@@ -312,6 +344,50 @@ __global__ __launch_bounds__(THREADS16) __attribute__((amdgpu_waves_per_eu(2, 2)
       if (l + 1 < L) layer(l + 1, gb);
     }
   }
+  return ldp;
+}
+
+// the row model on the 16 rows in Xs: one wavefront per row, the density kernel's loads,
+// arithmetic and stores (gradient rows to `g` with row stride P, log densities to lj)
+template <class RM>
+__device__ __forceinline__ void flow16_row_model(const RM& rm, const float* Xs, int P, float* g, float* lj) {
+  const int lane = threadIdx.x & 63;
+  const int wu = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  typename RM::template Lane<float> ld;
+  rm.prepare(ld);
+  for (int rr = wu; rr < R16; rr += NW16) {
+    float qv[RM::K], gv[RM::K];
+#pragma unroll
+    for (int k = 0; k < RM::K; ++k) {
+      const int a = rm.api(lane, k);
+      qv[k] = a >= 0 ? Xs[rr * P + a] : 0.0f;
+    }
+    bool ok;
+    const float val = rm.eval(ld, qv, gv, ok);
+#pragma unroll
+    for (int k = 0; k < RM::K; ++k) {
+      const int a = rm.api(lane, k);
+      if (a >= 0) g[rr * P + a] = gv[k];
+    }
+    if (lane == 0) lj[rr] = val;
+  }
+}
+
+template <int LT, class RM>
+__global__ __launch_bounds__(THREADS16) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_flow16_fwd(
+    const float* __restrict__ z, const float* __restrict__ prm, const float* __restrict__ ldbase,
+    float* __restrict__ x, float* __restrict__ q, int P, const Flow16Layer* __restrict__ layers, int L_, float c,
+    float* __restrict__ logdet, float* __restrict__ logdet_out, RM rm, float* __restrict__ g_rm,
+    float* __restrict__ lj_rm) {
+  const int L = LT > 0 ? LT : L_;
+  extern __shared__ __attribute__((aligned(16))) float lds16[];
+  float* Xs = lds16;                          // 16 x P fp32 rows
+  bf16_t* Xp = (bf16_t*)(Xs + R16 * P);       // 16 x XPS16 passive inputs (bf16)
+  bf16_t* Hs = Xp + R16 * XPS16;              // 16 x HS16 hidden activations (bf16)
+  float* Red = (float*)(Hs + R16 * HS16);     // 8 x 16 log-determinant partials
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63, lr = lane & 15;
+  const long long r0 = (long long)blockIdx.x * R16;
+  float ldp = flow16_fwd_body<LT>(z, prm, P, layers, L, c, Xs, Xp, Hs, GlobalSaved{});
   // log-determinant: the base's, plus the row sums of the lanes' shares
   ldp += __shfl_xor(ldp, 16);
   ldp += __shfl_xor(ldp, 32);
@@ -329,28 +405,7 @@ __global__ __launch_bounds__(THREADS16) __attribute__((amdgpu_waves_per_eu(2, 2)
     x[r0 * P + e] = v;
     if (q) q[r0 * P + e] = v;
   }
-  if constexpr (!std::is_same<RM, NoRowModel>::value) {
-    // the row model on the rows just produced: one wavefront per row, the density
-    // kernel's loads, arithmetic and stores
-    typename RM::template Lane<float> ld;
-    rm.prepare(ld);
-    for (int rr = wu; rr < R16; rr += NW16) {
-      float qv[RM::K], gv[RM::K];
-#pragma unroll
-      for (int k = 0; k < RM::K; ++k) {
-        const int a = rm.api(lane, k);
-        qv[k] = a >= 0 ? Xs[rr * P + a] : 0.0f;
-      }
-      bool ok;
-      const float val = rm.eval(ld, qv, gv, ok);
-#pragma unroll
-      for (int k = 0; k < RM::K; ++k) {
-        const int a = rm.api(lane, k);
-        if (a >= 0) g_rm[(r0 + rr) * P + a] = gv[k];
-      }
-      if (lane == 0) lj_rm[r0 + rr] = val;
-    }
-  }
+  if constexpr (!std::is_same<RM, NoRowModel>::value) flow16_row_model(rm, Xs, P, g_rm + r0 * P, lj_rm + r0);
 }
 
 struct Bwd16Regs {
@@ -383,46 +438,38 @@ __device__ inline void bwd16_load(const Flow16Layer& ly, Bwd16Regs& g) {
     for (int r = 0; r < 4; ++r) g.b1[t][r] = b1[32 * w + 16 * t + 4 * lg + r];
 }
 
-template <int LT, bool STEP>
-__global__ __launch_bounds__(THREADS16) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_flow16_bwd(
-    const float* g_q, const float* __restrict__ prm, float* gz, int P, const Flow16Layer* __restrict__ layers, int L_,
-    float c, Flow16Step stp) {
-  const int L = LT > 0 ? LT : L_;
-  extern __shared__ __attribute__((aligned(16))) float lds16[];
-  float* Gs = lds16;                      // 16 x P fp32 gradient rows
-  bf16_t* dO = (bf16_t*)(Gs + R16 * P);   // 16 x OS16 conditioner-output gradient (interleaved order)
-  bf16_t* DU = dO + R16 * OS16;           // 16 x HS16 hidden pre-activation gradient
+// The backward's registers carried from its prologue into the layers: the weights of the
+// last two layers and the last layer's saved values of this lane's coordinates.
+struct Bwd16Carry {
+  Bwd16Regs ga, gb;
+  float sv[2], xsv[2];
+  bf16x8 xb[2];
+};
+template <class SAVE>
+__device__ __forceinline__ void flow16_bwd_prologue(const Flow16Layer* __restrict__ layers, int L, const SAVE& save,
+                                                    Bwd16Carry& cy) {
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, lr = lane & 15, lg = lane >> 4;
+  const long long r0 = (long long)blockIdx.x * R16;
+  const int j0 = 8 * w + 2 * lg;
+  bwd16_load(layers[L - 1], cy.ga);
+  if (L > 1) bwd16_load(layers[L - 2], cy.gb);
+#pragma unroll
+  for (int e = 0; e < 2; ++e) save.get(layers[L - 1], L - 1, r0, lr, j0 + e, cy.sv[e], cy.xsv[e]);
+#pragma unroll
+  for (int ks = 0; ks < 2; ++ks) cy.xb[ks] = save.get_frag(layers[L - 1], L - 1, ks, lane);
+}
+
+// The backward through every layer (gradient rows in Gs, in place): coupling backward,
+// dh^T = W2'^T dO^T with U recomputed from the saved passive inputs, dU = dh (1 - h^2),
+// dX_p^T = W1^T dU^T.
+template <int LT, class SAVE>
+__device__ __forceinline__ void flow16_bwd_body(int P, const Flow16Layer* __restrict__ layers, int L, float c,
+                                                float* Gs, bf16_t* dO, bf16_t* DU, const SAVE& save, Bwd16Carry& cy) {
   const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63, lr = lane & 15, lg = lane >> 4;
   const long long r0 = (long long)blockIdx.x * R16;
   const int j0 = 8 * w + 2 * lg;
   const int wu = __builtin_amdgcn_readfirstlane(w);  // the wavefront index as a scalar (uniform branches)
-  // weights in two register sets: a layer computes with one while the next layer's are in flight
-  Bwd16Regs ga, gb;
-  bwd16_load(layers[L - 1], ga);
-  if (L > 1) bwd16_load(layers[L - 2], gb);
-  // the forward's saved values of this lane's coordinates (s, x_a e^s; row stride 64) and the
-  // passive-input fragments of a layer, loaded one layer ahead
-  auto saved = [&](const Flow16Layer& ly, float (&sv)[2], float (&xsv)[2]) {
-    const UT_GLOBAL16 float* sg = g16(ly.s);
-    const UT_GLOBAL16 float* xsg = g16(ly.xs);
-#pragma unroll
-    for (int e = 0; e < 2; ++e) {
-      sv[e] = sg[(r0 + lr) * KP16 + j0 + e];
-      xsv[e] = xsg[(r0 + lr) * KP16 + j0 + e];
-    }
-  };
-  auto xfrags = [&](const Flow16Layer& ly, bf16x8 (&xb)[2]) {
-    const UT_GLOBAL16 bf16_t* xpg = g16(ly.xpg);
-#pragma unroll
-    for (int ks = 0; ks < 2; ++ks)
-      xb[ks] = *(const UT_GLOBAL16 bf16x8*)(xpg + (((long long)blockIdx.x * 2 + ks) * 64 + lane) * 8);
-  };
-  float sv[2], xsv[2];
-  bf16x8 xb[2];
-  saved(layers[L - 1], sv, xsv);
-  xfrags(layers[L - 1], xb);
-  for (int e = tid; e < R16 * P; e += THREADS16) Gs[e] = g_q[r0 * P + e];
-  lds_barrier16();
+  (void)tid;
   // one layer (l = L - 1 - li) with weights `g`; afterwards `g` receives layer l - 2's weights
   auto layer = [&](int li, Bwd16Regs& g) {
     const int l = L - 1 - li;
@@ -439,10 +486,10 @@ __global__ __launch_bounds__(THREADS16) __attribute__((amdgpu_waves_per_eu(2, 2)
       const bool valid = j < na;
       const int ak = valid ? 2 * j + 1 - par : 0;
       const float ga = valid ? Gs[lr * P + ak] : 0.0f;
-      const float r = sv[e] * (1.0f / c);
-      d[2 * e] = valid ? (ga * xsv[e] + 1.0f) * (1.0f - r * r) : 0.0f;
+      const float r = cy.sv[e] * (1.0f / c);
+      d[2 * e] = valid ? (ga * cy.xsv[e] + 1.0f) * (1.0f - r * r) : 0.0f;
       d[2 * e + 1] = ga;
-      if (valid) Gs[lr * P + ak] = ga * expf(sv[e]);
+      if (valid) Gs[lr * P + ak] = ga * expf(cy.sv[e]);
     }
     {
       uint2 pk;
@@ -451,7 +498,10 @@ __global__ __launch_bounds__(THREADS16) __attribute__((amdgpu_waves_per_eu(2, 2)
       *(uint2*)(dO + lr * OS16 + 16 * w + 4 * lg) = pk;
     }
     float sv_n[2], xsv_n[2];
-    if (next) saved(nx, sv_n, xsv_n);
+    if (next) {
+#pragma unroll
+      for (int e = 0; e < 2; ++e) save.get(nx, l - 1, r0, lr, j0 + e, sv_n[e], xsv_n[e]);
+    }
     lds_barrier16();
     // ---- dh^T = W2'^T dO^T; U recomputed from the forward's passive inputs; dU = dh (1 - h^2)
     bf16x8 ob[4];
@@ -465,11 +515,14 @@ __global__ __launch_bounds__(THREADS16) __attribute__((amdgpu_waves_per_eu(2, 2)
 #pragma unroll
       for (int ks = 0; ks < 4; ++ks) dh[t] = mfma16(g.a2t[t][ks], ob[ks], dh[t]);
 #pragma unroll
-      for (int ks = 0; ks < 2; ++ks) u[t] = mfma16(g.a1[t][ks], xb[ks], u[t]);
+      for (int ks = 0; ks < 2; ++ks) u[t] = mfma16(g.a1[t][ks], cy.xb[ks], u[t]);
     }
     bf16x8 xb_n[2];
     __builtin_amdgcn_sched_barrier(0);
-    if (next) xfrags(nx, xb_n);
+    if (next) {
+#pragma unroll
+      for (int ks = 0; ks < 2; ++ks) xb_n[ks] = save.get_frag(nx, l - 1, ks, lane);
+    }
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
       float du[4];
@@ -506,25 +559,34 @@ __global__ __launch_bounds__(THREADS16) __attribute__((amdgpu_waves_per_eu(2, 2)
     if (next) {
 #pragma unroll
       for (int e = 0; e < 2; ++e) {
-        sv[e] = sv_n[e];
-        xsv[e] = xsv_n[e];
-        xb[e] = xb_n[e];
+        cy.sv[e] = sv_n[e];
+        cy.xsv[e] = xsv_n[e];
+        cy.xb[e] = xb_n[e];
       }
     }
   };
   if (LT > 0) {
 #pragma unroll
     for (int li = 0; li < LT; li += 2) {
-      layer(li, ga);
-      if (li + 1 < LT) layer(li + 1, gb);
+      layer(li, cy.ga);
+      if (li + 1 < LT) layer(li + 1, cy.gb);
     }
   } else {
     for (int li = 0; li < L; li += 2) {
-      layer(li, ga);
-      if (li + 1 < L) layer(li + 1, gb);
+      layer(li, cy.ga);
+      if (li + 1 < L) layer(li + 1, cy.gb);
     }
   }
-  // base backward: g_z = g_x e^ls
+}
+
+// base backward g_z = g_x e^ls of the workgroup's rows, and with STEP the lock-step
+// leapfrog's full kick and drift (k_ls_step): rows whose gradient is finite move, a chain
+// with a non-finite entry is marked invalid, invalid chains stay
+template <bool STEP>
+__device__ __forceinline__ void flow16_bwd_base(const float* __restrict__ prm, int P, const float* Gs, float* gz,
+                                                const Flow16Step& stp) {
+  const int tid = threadIdx.x;
+  const long long r0 = (long long)blockIdx.x * R16;
   __shared__ int row_ok[R16];
   if (STEP) {
     if (tid < R16) row_ok[tid] = 1;
@@ -545,8 +607,6 @@ __global__ __launch_bounds__(THREADS16) __attribute__((amdgpu_waves_per_eu(2, 2)
     }
   }
   if (STEP) {
-    // the lock-step leapfrog's full kick and drift (k_ls_step): rows whose gradient is
-    // finite move, a chain with a non-finite entry is marked invalid, invalid chains stay
     __syncthreads();
     for (int e = tid; e < R16 * P; e += THREADS16) {
       const int row = e / P, a = e - row * P;
@@ -561,6 +621,52 @@ __global__ __launch_bounds__(THREADS16) __attribute__((amdgpu_waves_per_eu(2, 2)
     }
     if (tid < R16 && !row_ok[tid] && stp.valid[r0 + tid]) stp.valid[r0 + tid] = 0;
   }
+}
+
+template <int LT, bool STEP>
+__global__ __launch_bounds__(THREADS16) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_flow16_bwd(
+    const float* g_q, const float* __restrict__ prm, float* gz, int P, const Flow16Layer* __restrict__ layers, int L_,
+    float c, Flow16Step stp) {
+  const int L = LT > 0 ? LT : L_;
+  extern __shared__ __attribute__((aligned(16))) float lds16[];
+  float* Gs = lds16;                      // 16 x P fp32 gradient rows
+  bf16_t* dO = (bf16_t*)(Gs + R16 * P);   // 16 x OS16 conditioner-output gradient (interleaved order)
+  bf16_t* DU = dO + R16 * OS16;           // 16 x HS16 hidden pre-activation gradient
+  const long long r0 = (long long)blockIdx.x * R16;
+  Bwd16Carry cy;
+  flow16_bwd_prologue(layers, L, GlobalSaved{}, cy);
+  for (int e = threadIdx.x; e < R16 * P; e += THREADS16) Gs[e] = g_q[r0 * P + e];
+  lds_barrier16();
+  flow16_bwd_body<LT>(P, layers, L, c, Gs, dO, DU, GlobalSaved{}, cy);
+  flow16_bwd_base<STEP>(prm, P, Gs, gz, stp);
+}
+
+// The flow-wrapped gradient of a row model with the lock-step leapfrog step, in one kernel
+// per 16-row tile: forward (saved values kept in LDS), the row model's gradient into the
+// backward's gradient rows, backward, base backward and the step.  The arithmetic of
+// k_flow16_fwd + the density kernel + k_flow16_bwd + k_ls_step, so the results are theirs.
+template <int LT, class RM>
+__global__ __launch_bounds__(THREADS16) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_flow16_grad(
+    const float* __restrict__ z, const float* __restrict__ prm, int P, const Flow16Layer* __restrict__ layers,
+    int L_, float c, RM rm, float* __restrict__ lj_rm, float* __restrict__ gz, Flow16Step stp) {
+  const int L = LT > 0 ? LT : L_;
+  extern __shared__ __attribute__((aligned(16))) float lds16[];
+  float* Xs = lds16;                          // 16 x P fp32 rows
+  bf16_t* Xp = (bf16_t*)(Xs + R16 * P);       // 16 x XPS16 passive inputs (bf16)
+  bf16_t* Hs = Xp + R16 * XPS16;              // 16 x HS16 hidden activations (bf16)
+  float* Gs = (float*)(Hs + R16 * HS16);      // 16 x P fp32 gradient rows
+  bf16_t* dO = (bf16_t*)(Gs + R16 * P);       // 16 x OS16
+  bf16_t* DU = dO + R16 * OS16;               // 16 x HS16
+  float* sav = (float*)(DU + R16 * HS16);     // L x (s, x_a e^s) 16 x 64 fp32, then L x fragments
+  const LdsSaved save{sav, sav + L * SAVED16_F32, (bf16_t*)(sav + 2 * L * SAVED16_F32)};
+  const long long r0 = (long long)blockIdx.x * R16;
+  (void)flow16_fwd_body<LT>(z, prm, P, layers, L, c, Xs, Xp, Hs, save);
+  Bwd16Carry cy;
+  flow16_bwd_prologue(layers, L, save, cy);  // the backward's first weights fly during the model
+  flow16_row_model(rm, Xs, P, Gs, lj_rm + r0);
+  lds_barrier16();
+  flow16_bwd_body<LT>(P, layers, L, c, Gs, dO, DU, save, cy);
+  flow16_bwd_base<true>(prm, P, Gs, gz, stp);
 }
 
 template <class RM>
@@ -586,6 +692,23 @@ Bwd16Fn bwd16_kernel(int L) {
     case 8: return k_flow16_bwd<8, STEP>;
     default: return k_flow16_bwd<0, STEP>;
   }
+}
+template <class RM>
+using Grad16Fn = void (*)(const float*, const float*, int, const Flow16Layer*, int, float, RM, float*, float*,
+                          Flow16Step);
+template <class RM>
+Grad16Fn<RM> grad16_kernel(int L) {
+  switch (L) {
+    case 2: return k_flow16_grad<2, RM>;
+    case 4: return k_flow16_grad<4, RM>;
+    case 6: return k_flow16_grad<6, RM>;
+    case 8: return k_flow16_grad<8, RM>;
+    default: return k_flow16_grad<0, RM>;
+  }
+}
+size_t grad16_lds(int P, int L) {
+  return (size_t)R16 * P * 4 + (size_t)R16 * XPS16 * 2 + (size_t)R16 * HS16 * 2 + (size_t)R16 * P * 4 +
+         (size_t)R16 * OS16 * 2 + (size_t)R16 * HS16 * 2 + (size_t)L * (2 * SAVED16_F32 * 4 + SAVED16_BF * 2);
 }
 size_t fwd16_lds(int P) { return (size_t)R16 * P * 4 + (size_t)R16 * XPS16 * 2 + (size_t)R16 * HS16 * 2 + NW16 * 16 * 4; }
 size_t bwd16_lds(int P) { return (size_t)R16 * P * 4 + (size_t)R16 * OS16 * 2 + (size_t)R16 * HS16 * 2; }
@@ -639,6 +762,27 @@ hipError_t flow16_backward(Flow& f, const float* g_q, float* g_z, int B, hipStre
   hipLaunchKernelGGL(bwd16_kernel<false>(f.L), dim3(B / R16), dim3(THREADS16), bwd16_lds(f.P), st, g_q, f.params, g_z,
                      f.P, f.d16, f.L, f.clamp, Flow16Step{});
   return hipGetLastError();
+}
+
+hipError_t flow16_grad_step(Flow& f, const float* z, const FlowRowModel& m, float* lj, float* g_z, int B,
+                           const Flow16Step& stp, hipStream_t st) {
+  if (B % R16 || !lj || !stp.wq || !stp.wp || !stp.im_f || !stp.valid) return hipErrorInvalidValue;
+  const size_t lds = grad16_lds(f.P, f.L);
+  if (lds > 160 * 1024) return hipErrorNotSupported;
+  auto launch = [&](auto fm) {
+    fm.dim = m.dim;
+    fm.v_scale = m.v_scale;
+    fm.total_const = m.total_const;
+    using FM = decltype(fm);
+    const auto kern = grad16_kernel<FM>(f.L);
+    if (lds > 65536) (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(kern, dim3(B / R16), dim3(THREADS16), lds, st, z, f.params, f.P, f.d16, f.L, f.clamp, fm, lj,
+                       g_z, stp);
+    return hipGetLastError();
+  };
+  if (m.K == 1) return launch(FunnelModel<1>{});
+  if (m.K == 2) return launch(FunnelModel<2>{});
+  return hipErrorInvalidValue;
 }
 
 hipError_t flow16_backward_step(Flow& f, const float* g_q, float* g_z, int B, const Flow16Step& stp,
