@@ -476,6 +476,7 @@ def bench_hier_nuts(args) -> int:
 
 
 FLOW16_PMC = "k_flow16_cfg4_pmc.json"
+FLOW16_GRAD_PMC = "k_flow16_grad_cfg4_pmc.json"
 
 
 def flow_roofline(flow, rows, reps=200) -> dict:
@@ -521,6 +522,15 @@ def bench_flow_kernels(args) -> int:
     flow.weights = w
     reps = 200 if args.steps is None else args.steps
     roof = flow_roofline(flow, rows, reps=reps)
+    # the fused gradient-and-step kernel on the funnel with this flow (step size 0)
+    from uncertaintea_amd import _lib as UL
+    from uncertaintea_amd.device import DeviceBatchedWorkspace
+    model = ut.NealsFunnel(100, 3.0)
+    ws = DeviceBatchedWorkspace(model, rows)
+    UL.check(UL.lib().ut_workspace_set_flow(ws.handle, flow.handle))
+    roof["fused_step_us"] = ws.gradient_step_kernel_time(np.zeros((model.num_params, rows), np.float32), reps=reps)
+    roof["fused_step_kernel"] = "k_flow16_grad (forward, funnel gradient, backward, leapfrog step)"
+    del ws
     line = {"metric": "flow kernel pair throughput (rows/s through forward + backward)",
             "value": rows / ((roof["forward_us"] + roof["backward_us"]) * 1e-6), "unit": "rows/s", "n_gpus": 1,
             "steps": reps, "warmup": 1, "ms_per_step": 1e-3 * (roof["forward_us"] + roof["backward_us"]),
@@ -628,11 +638,36 @@ def bench_funnel_flow(args) -> int:
     roofline = flow_roofline(flow, chains)
     ksec = fres.sampling_kernel_seconds
     evals = fres.sampling_gradient_evals
+    # the sampling path's dominant kernel: the flow-wrapped funnel gradient and leapfrog step
+    # in one launch per 16-row tile (k_flow16_grad), timed on its own with HIP events
+    from uncertaintea_amd import _lib as UL
+    from uncertaintea_amd.device import DeviceBatchedWorkspace
+    ws = DeviceBatchedWorkspace(model, chains)
+    UL.check(UL.lib().ut_workspace_set_flow(ws.handle, flow.handle))
+    step_us = ws.gradient_step_kernel_time(np.zeros((model.num_params, chains), np.float32), reps=200)
+    del ws
+    achieved = chains * roofline["flops_per_chain_gradient"] / (step_us * 1e-6) / 1e12
+    grad_pmc = ROOT / "profiles" / FLOW16_GRAD_PMC
+    grad_rec = json.loads(grad_pmc.read_text()) if grad_pmc.exists() else None
+    separate_traffic = roofline.get("traffic")
     roofline.update({
+        "achieved": achieved, "frac": achieved / PEAK_BF16_TFLOPS,
+        "kernel": "k_flow16_grad (forward, the funnel's gradient, backward and the leapfrog step of 16 rows per "
+                  "workgroup, one launch per leapfrog step after a trajectory's first)",
+        "timing": f"HIP events around 200 back-to-back leapfrog steps of {chains} device-resident rows on the "
+                  f"workspace stream (ut_gradient_step_kernel_time, step size 0)",
+        "us_per_launch": step_us,
+        "flops_per_launch": chains * roofline["flops_per_chain_gradient"],
+        "separate_kernels": {"forward_us": roofline.pop("forward_us"), "backward_us": roofline.pop("backward_us"),
+                             "traffic": separate_traffic,
+                             "note": "k_flow16_fwd + k_flow16_bwd alone (the path of a trajectory's last step)"},
+        "traffic": grad_rec["traffic_bytes_per_launch"] if grad_rec else None,
+        "traffic_note": grad_rec.get("note") if grad_rec else "no FETCH_SIZE / WRITE_SIZE record for the fused kernel",
         "us_per_batched_gradient_in_sampling": 1e6 * ksec / max(1, evals // chains),
-        "per_gradient_note": "the sampling run's device time per batched gradient (HIP events around the timed "
-                             "region on the workspace stream): flow pair + model gradient + lock-step leapfrog "
-                             "kernels"})
+        "per_gradient_note": "the sampling run's device time per leapfrog step of all chains (HIP events around the "
+                             "timed region on the workspace stream): the fused kernel, and per trajectory the "
+                             "separate forward, model and backward kernels of the last step and the lock-step "
+                             "begin / end / record kernels"})
     cpu = None
     if not args.no_cpu_baseline:
         from oracle import binding as B

@@ -312,6 +312,12 @@ int ut_logjoint_gradient(ut_workspace* ws, const void* params, void* values,
  * no counterpart in the reference (the benchmark's roofline of the gradient).    */
 int ut_gradient_kernel_time(ut_workspace* ws, const void* params, int32_t reps, int32_t need_logjoint,
                             double* us_per_call);
+/* Device time of one leapfrog step of the lock-step samplers after a trajectory's first: the
+ * batched gradient at params (P x C, float32) and the full kick and drift, with step size 0
+ * so that every call does the same work; `reps` back-to-back calls timed with HIP events on
+ * the workspace's stream (with a flow on the 16-row kernels and the funnel this is one
+ * fused kernel per call).  A measurement utility with no counterpart in the reference.     */
+int ut_gradient_step_kernel_time(ut_workspace* ws, const void* params, int32_t reps, double* us_per_call);
 
 /* Allocation-free density and gradient.  No counterpart in the reference: this is
  * for callers that cannot allocate device memory (code under HIP graph capture,

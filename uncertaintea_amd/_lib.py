@@ -45,6 +45,7 @@ EXPORTED = (
     "ut_plan_gp_poisson", "ut_gp_factor",
     "ut_plan_destroy", "ut_plan_num_params", "ut_plan_validate", "ut_plan_create", "ut_plan_engine",
     "ut_workspace_create", "ut_workspace_destroy", "ut_workspace_set_chain_waves", "ut_workspace_chain_waves",
+    "ut_gradient_step_kernel_time",
     "ut_logjoint", "ut_logjoint_gradient", "ut_gradient_kernel_time", "ut_noalloc_scratch_bytes",
     "ut_logjoint_gradient_noalloc",
     "ut_sampler_opts_default", "ut_nuts_run", "ut_nuts_masked_run", "ut_hmc_run", "ut_chees_opts_default", "ut_chees_run",
@@ -199,6 +200,7 @@ def lib() -> ct.CDLL:
         "ut_workspace_create": (ct.c_int, [vp, i32, i32, pp]),
         "ut_workspace_destroy": (ct.c_int, [vp]),
         "ut_workspace_set_chain_waves": (ct.c_int, [vp, i32]),
+        "ut_gradient_step_kernel_time": (ct.c_int, [vp, vp, i32, vp]),
         "ut_workspace_chain_waves": (ct.c_int, [vp, vp]),
         "ut_logjoint": (ct.c_int, [vp, vp, vp]),
         "ut_logjoint_gradient": (ct.c_int, [vp, vp, vp, vp]),

@@ -2983,3 +2983,45 @@ int ut_gp_factor(const ut_plan* plan, const double* hyper, int32_t num_chains, d
 }
 
 }  // extern "C"
+
+extern "C" {
+
+int ut_gradient_step_kernel_time(ut_workspace* ws, const void* params, int32_t reps, double* us_per_call) {
+  if (!ws || !params || !us_per_call) return fail(UT_E_ARG, "ut_gradient_step_kernel_time: null pointer");
+  if (reps < 1) return fail(UT_E_ARG, "ut_gradient_step_kernel_time: reps must be >= 1");
+  if (ws->precision != UT_FLOAT32)
+    return fail(UT_E_PRECISION, "ut_gradient_step_kernel_time: float32 workspaces only");
+  const int P = ws->P, C = ws->C;
+  const size_t PC = (size_t)P * C;
+  DeviceBuffers buf;
+  LockstepState ls;
+  float* im = nullptr;
+  UT_HIP(buf.alloc(&ls.wq, PC));
+  UT_HIP(buf.alloc(&ls.wp, PC));
+  UT_HIP(buf.alloc(&ls.wg, PC));
+  UT_HIP(buf.alloc(&ls.wlj, (size_t)C));
+  UT_HIP(buf.alloc(&ls.valid, (size_t)C));
+  UT_HIP(buf.alloc(&im, (size_t)P));
+  ls.im_f = im;
+  const std::vector<float> ones((size_t)P, 1.0f);
+  UT_HIP(hipMemcpyAsync(ls.wq, params, PC * 4, hipMemcpyHostToDevice, ws->stream));
+  UT_HIP(hipMemcpyAsync(im, ones.data(), (size_t)P * 4, hipMemcpyHostToDevice, ws->stream));
+  UT_HIP(hipMemsetAsync(ls.wp, 0, PC * 4, ws->stream));
+  UT_HIP(hipMemsetAsync(ls.valid, 1, (size_t)C, ws->stream));
+  hipEvent_t ev[2];
+  for (auto& e : ev) UT_HIP(hipEventCreate(&e));
+  // step size 0: the positions stay, so every call does the same work
+  UT_HIP(batched_grad_step(ws, ls, P, C, 0.0f));  // warm (and capture, when graphs are used)
+  UT_HIP(hipEventRecord(ev[0], ws->stream));
+  for (int r = 0; r < reps; ++r) UT_HIP(batched_grad_step(ws, ls, P, C, 0.0f));
+  UT_HIP(hipEventRecord(ev[1], ws->stream));
+  UT_HIP(hipEventSynchronize(ev[1]));
+  float ms = 0.0f;
+  UT_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  UT_HIP(hipStreamSynchronize(ws->stream));
+  *us_per_call = 1e3 * ms / reps;
+  return UT_OK;
+}
+
+}  // extern "C"

@@ -72,6 +72,17 @@ class DeviceBatchedWorkspace:
                                                 ct.byref(us)))
         return us.value
 
+    def gradient_step_kernel_time(self, params, reps: int = 20) -> float:
+        """Mean device microseconds per leapfrog step of the lock-step samplers after a
+        trajectory's first (batched gradient at `params` plus the kick and drift, step size
+        0), over `reps` back-to-back calls (HIP events on the workspace stream)."""
+        if self.dtype != np.float32:
+            raise ValueError("gradient_step_kernel_time: float32 workspaces only")
+        p = self._params(params)
+        us = ct.c_double()
+        L.check(L.lib().ut_gradient_step_kernel_time(self._h, L.ptr(p), int(reps), ct.byref(us)))
+        return us.value
+
     def _params(self, params) -> np.ndarray:
         a = np.asarray(params)
         if a.ndim != 2:
