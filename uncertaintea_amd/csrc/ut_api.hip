@@ -922,6 +922,23 @@ static hipError_t batched_grad(ut_workspace* ws, const float* q, float* lj, floa
   return hipGraphLaunch(exec, ws->stream);
 }
 
+// A trajectory's last gradient in the lock-step samplers (log density included): for Neal's
+// funnel under a flow on the 16-row kernels one fused kernel (forward, model, backward and the
+// log-determinant), else the batched gradient.
+static hipError_t batched_grad_final(ut_workspace* ws, const LockstepState& ls) {
+  if (ws->flow && ws->flow->f.m16 && ws->fused_model && ws->fused_step && ws->dev.kind == MODEL_FUNNEL &&
+      (ws->dev.K == 1 || ws->dev.K == 2)) {
+    FlowRowModel rm;
+    rm.K = ws->dev.K;
+    rm.dim = ws->dev.dim;
+    rm.v_scale = ws->dev.v_scale;
+    rm.total_const = ws->dev.total_const;
+    const hipError_t e = flow16_grad_step(ws->flow->f, ls.wq, rm, ls.wlj, ls.wg, ws->C, nullptr, ws->stream);
+    if (e != hipErrorNotSupported) return e;
+  }
+  return batched_grad(ws, ls.wq, ls.wlj, ls.wg, true);
+}
+
 // One leapfrog step of the lock-step samplers after the first: the batched gradient at
 // ls.wq, then the full kick and drift (k_ls_step).  With a flow on the 16-row kernels the
 // step runs in the flow backward's epilogue: the flow forward and the model gradient are
@@ -946,7 +963,7 @@ static hipError_t batched_grad_step(ut_workspace* ws, const LockstepState& ls, i
     stp.im_f = ls.im_f;
     stp.valid = ls.valid;
     stp.eps = eps;
-    const hipError_t e = flow16_grad_step(f, ls.wq, rm, ls.wlj, ls.wg, ws->C, stp, ws->stream);
+    const hipError_t e = flow16_grad_step(f, ls.wq, rm, ls.wlj, ls.wg, ws->C, &stp, ws->stream);
     if (e != hipErrorNotSupported) return e;
   }
   hipError_t e = hipSuccess;
@@ -1898,7 +1915,7 @@ int run_lockstep(ut_workspace* ws, const ut_chees_opts* co, ut_chain_out* out, d
     const float eps_f = (float)eps;
     UT_HIP(launch_ls_begin(ls, P, C, itc, STREAM_MOMENTUM, eps_f, (warm && adapt_traj) ? 1 : 0, stream));
     for (int l = 1; l < L; ++l) UT_HIP(batched_grad_step(ws, ls, P, C, eps_f));
-    UT_HIP(batched_grad(ws, ls.wq, ls.wlj, ls.wg, true));
+    UT_HIP(batched_grad_final(ws, ls));
     UT_HIP(launch_ls_end(ls, rec, P, C, itc, eps_f, o->divergence_threshold, 0, stream));
     UT_HIP(launch_ls_record(ls, rec, sp, it, L, stream));
     if (!warm) {

@@ -179,13 +179,14 @@ struct Flow16Step {
 // (all of the row's g_z finite) or the chain is marked invalid.
 hipError_t flow16_backward_step(Flow& f, const float* g_q, float* g_z, int B, const Flow16Step& stp,
                                 hipStream_t st);
-// The whole flow-wrapped gradient of a row model followed by the leapfrog step, one kernel per
-// 16-row tile (forward with its saved values in LDS, the row model, backward, step): the
-// arithmetic of flow16_forward_model + flow16_backward_step.  lj receives the model's log
-// density (without the log-determinant), g_z the gradient rows.  hipErrorNotSupported when the
-// layers' saved values do not fit in LDS.
+// The whole flow-wrapped gradient of a row model in one kernel per 16-row tile (forward with its
+// saved values in LDS, the row model, backward), g_z the gradient rows: with a step, followed by
+// the leapfrog step (the arithmetic of flow16_forward_model + flow16_backward_step; lj receives
+// the model's log density), without one (stp null) lj receives the model's log density plus
+// the flow's log-determinant (flow_forward_model + flow16_backward + flow_add_logdet).
+// hipErrorNotSupported when the layers' saved values do not fit in LDS.
 hipError_t flow16_grad_step(Flow& f, const float* z, const FlowRowModel& m, float* lj, float* g_z, int B,
-                           const Flow16Step& stp, hipStream_t st);
+                           const Flow16Step* stp, hipStream_t st);
 
 // factor = true: the full-rank guide (L must be 0), mu, ls and the strictly lower
 // factor as its weights

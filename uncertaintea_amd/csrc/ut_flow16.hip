@@ -641,14 +641,17 @@ __global__ __launch_bounds__(THREADS16) __attribute__((amdgpu_waves_per_eu(2, 2)
   flow16_bwd_base<STEP>(prm, P, Gs, gz, stp);
 }
 
-// The flow-wrapped gradient of a row model with the lock-step leapfrog step, in one kernel
-// per 16-row tile: forward (saved values kept in LDS), the row model's gradient into the
-// backward's gradient rows, backward, base backward and the step.  The arithmetic of
-// k_flow16_fwd + the density kernel + k_flow16_bwd + k_ls_step, so the results are theirs.
-template <int LT, class RM>
+// The flow-wrapped gradient of a row model, in one kernel per 16-row tile: forward (saved
+// values kept in LDS), the row model's gradient into the backward's gradient rows, backward
+// and base backward; then with STEP the lock-step leapfrog step (lj_rm: the model's log
+// density), without it the log density plus the flow's log-determinant (a trajectory's last
+// step).  The arithmetic of k_flow16_fwd + the density kernel + k_flow16_bwd + k_ls_step (or
+// k_add_logdet), so the results are theirs.
+template <int LT, class RM, bool STEP>
 __global__ __launch_bounds__(THREADS16) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_flow16_grad(
-    const float* __restrict__ z, const float* __restrict__ prm, int P, const Flow16Layer* __restrict__ layers,
-    int L_, float c, RM rm, float* __restrict__ lj_rm, float* __restrict__ gz, Flow16Step stp) {
+    const float* __restrict__ z, const float* __restrict__ prm, const float* __restrict__ ldbase, int P,
+    const Flow16Layer* __restrict__ layers, int L_, float c, RM rm, float* __restrict__ lj_rm,
+    float* __restrict__ gz, Flow16Step stp) {
   const int L = LT > 0 ? LT : L_;
   extern __shared__ __attribute__((aligned(16))) float lds16[];
   float* Xs = lds16;                          // 16 x P fp32 rows
@@ -657,16 +660,33 @@ __global__ __launch_bounds__(THREADS16) __attribute__((amdgpu_waves_per_eu(2, 2)
   float* Gs = (float*)(Hs + R16 * HS16);      // 16 x P fp32 gradient rows
   bf16_t* dO = (bf16_t*)(Gs + R16 * P);       // 16 x OS16
   bf16_t* DU = dO + R16 * OS16;               // 16 x HS16
-  float* sav = (float*)(DU + R16 * HS16);     // L x (s, x_a e^s) 16 x 64 fp32, then L x fragments
+  float* Red = (float*)(DU + R16 * HS16);     // 8 x 16 log-determinant partials
+  float* LJs = Red + NW16 * 16;               // 16 model log densities
+  float* sav = LJs + R16;                     // L x (s, x_a e^s) 16 x 64 fp32, then L x fragments
   const LdsSaved save{sav, sav + L * SAVED16_F32, (bf16_t*)(sav + 2 * L * SAVED16_F32)};
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63, lr = lane & 15;
   const long long r0 = (long long)blockIdx.x * R16;
-  (void)flow16_fwd_body<LT>(z, prm, P, layers, L, c, Xs, Xp, Hs, save);
+  float ldp = flow16_fwd_body<LT>(z, prm, P, layers, L, c, Xs, Xp, Hs, save);
   Bwd16Carry cy;
   flow16_bwd_prologue(layers, L, save, cy);  // the backward's first weights fly during the model
-  flow16_row_model(rm, Xs, P, Gs, lj_rm + r0);
+  if (STEP) {
+    flow16_row_model(rm, Xs, P, Gs, lj_rm + r0);
+  } else {
+    flow16_row_model(rm, Xs, P, Gs, LJs);
+    // log-determinant as k_flow16_fwd forms it, added to the model's log density
+    ldp += __shfl_xor(ldp, 16);
+    ldp += __shfl_xor(ldp, 32);
+    if (lane < 16) Red[w * 16 + lr] = ldp;
+  }
   lds_barrier16();
+  if (!STEP && tid < R16) {
+    float s = ldbase[0];
+#pragma unroll
+    for (int ww = 0; ww < NW16; ++ww) s += Red[ww * 16 + tid];
+    lj_rm[r0 + tid] = LJs[tid] + s;
+  }
   flow16_bwd_body<LT>(P, layers, L, c, Gs, dO, DU, save, cy);
-  flow16_bwd_base<true>(prm, P, Gs, gz, stp);
+  flow16_bwd_base<STEP>(prm, P, Gs, gz, stp);
 }
 
 template <class RM>
@@ -694,21 +714,22 @@ Bwd16Fn bwd16_kernel(int L) {
   }
 }
 template <class RM>
-using Grad16Fn = void (*)(const float*, const float*, int, const Flow16Layer*, int, float, RM, float*, float*,
-                          Flow16Step);
-template <class RM>
+using Grad16Fn = void (*)(const float*, const float*, const float*, int, const Flow16Layer*, int, float, RM, float*,
+                          float*, Flow16Step);
+template <class RM, bool STEP>
 Grad16Fn<RM> grad16_kernel(int L) {
   switch (L) {
-    case 2: return k_flow16_grad<2, RM>;
-    case 4: return k_flow16_grad<4, RM>;
-    case 6: return k_flow16_grad<6, RM>;
-    case 8: return k_flow16_grad<8, RM>;
-    default: return k_flow16_grad<0, RM>;
+    case 2: return k_flow16_grad<2, RM, STEP>;
+    case 4: return k_flow16_grad<4, RM, STEP>;
+    case 6: return k_flow16_grad<6, RM, STEP>;
+    case 8: return k_flow16_grad<8, RM, STEP>;
+    default: return k_flow16_grad<0, RM, STEP>;
   }
 }
 size_t grad16_lds(int P, int L) {
   return (size_t)R16 * P * 4 + (size_t)R16 * XPS16 * 2 + (size_t)R16 * HS16 * 2 + (size_t)R16 * P * 4 +
-         (size_t)R16 * OS16 * 2 + (size_t)R16 * HS16 * 2 + (size_t)L * (2 * SAVED16_F32 * 4 + SAVED16_BF * 2);
+         (size_t)R16 * OS16 * 2 + (size_t)R16 * HS16 * 2 + (size_t)(NW16 * 16 + R16) * 4 +
+         (size_t)L * (2 * SAVED16_F32 * 4 + SAVED16_BF * 2);
 }
 size_t fwd16_lds(int P) { return (size_t)R16 * P * 4 + (size_t)R16 * XPS16 * 2 + (size_t)R16 * HS16 * 2 + NW16 * 16 * 4; }
 size_t bwd16_lds(int P) { return (size_t)R16 * P * 4 + (size_t)R16 * OS16 * 2 + (size_t)R16 * HS16 * 2; }
@@ -765,23 +786,26 @@ hipError_t flow16_backward(Flow& f, const float* g_q, float* g_z, int B, hipStre
 }
 
 hipError_t flow16_grad_step(Flow& f, const float* z, const FlowRowModel& m, float* lj, float* g_z, int B,
-                           const Flow16Step& stp, hipStream_t st) {
-  if (B % R16 || !lj || !stp.wq || !stp.wp || !stp.im_f || !stp.valid) return hipErrorInvalidValue;
+                           const Flow16Step* stp, hipStream_t st) {
+  if (B % R16 || !lj) return hipErrorInvalidValue;
+  if (stp && (!stp->wq || !stp->wp || !stp->im_f || !stp->valid)) return hipErrorInvalidValue;
   const size_t lds = grad16_lds(f.P, f.L);
   if (lds > 160 * 1024) return hipErrorNotSupported;
-  auto launch = [&](auto fm) {
+  auto launch = [&](auto fm, auto step) {
     fm.dim = m.dim;
     fm.v_scale = m.v_scale;
     fm.total_const = m.total_const;
     using FM = decltype(fm);
-    const auto kern = grad16_kernel<FM>(f.L);
+    const auto kern = grad16_kernel<FM, decltype(step)::value>(f.L);
     if (lds > 65536) (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL(kern, dim3(B / R16), dim3(THREADS16), lds, st, z, f.params, f.P, f.d16, f.L, f.clamp, fm, lj,
-                       g_z, stp);
+    hipLaunchKernelGGL(kern, dim3(B / R16), dim3(THREADS16), lds, st, z, f.params, f.ldbase, f.P, f.d16, f.L, f.clamp,
+                       fm, lj, g_z, stp ? *stp : Flow16Step{});
     return hipGetLastError();
   };
-  if (m.K == 1) return launch(FunnelModel<1>{});
-  if (m.K == 2) return launch(FunnelModel<2>{});
+  using T = std::true_type;
+  using F = std::false_type;
+  if (m.K == 1) return stp ? launch(FunnelModel<1>{}, T{}) : launch(FunnelModel<1>{}, F{});
+  if (m.K == 2) return stp ? launch(FunnelModel<2>{}, T{}) : launch(FunnelModel<2>{}, F{});
   return hipErrorInvalidValue;
 }
 
