@@ -167,10 +167,12 @@ __device__ inline void fwd16_load2(const Flow16Layer& ly, Fwd16Regs& g) {
 }
 
 // Where the forward keeps what the backward reads again: per layer the clamped scales s and
-// the scaled active inputs x_a e^s of each lane's coordinates (row stride 64), and the
-// passive-input fragments.  GlobalSaved: the layer's HBM buffers (the separate forward and
-// backward kernels); LdsSaved: LDS of the fused gradient kernel (per layer 16 x 64 of each
-// and 2 x 64 x 8 bf16 fragments).
+// the scaled active inputs x_a e^s of each lane's coordinates, and the passive-input
+// fragments.  GlobalSaved: the layer's HBM buffers (row stride 64; the separate forward and
+// backward kernels); LdsSaved: LDS of the fused gradient kernel, per layer 1024 of each in
+// the order of the lanes that write and read them (coordinate j = 8 w + 2 lg + e of row lr
+// at e 512 + 64 w + 16 lg + lr: consecutive lanes, consecutive banks), and 2 x 64 x 8 bf16
+// fragments.
 struct GlobalSaved {
   __device__ void put(const Flow16Layer& ly, int, long long r0, int lr, int j, float sv, float xs) const {
     g16(ly.s)[(r0 + lr) * KP16 + j] = sv;
@@ -193,16 +195,19 @@ struct LdsSaved {
   float* s;
   float* xs;
   bf16_t* xp;
+  __device__ static int at(int l, int lr, int j) {
+    return l * SAVED16_F32 + (j & 1) * 512 + (j >> 3) * 64 + ((j >> 1) & 3) * 16 + lr;
+  }
   __device__ void put(const Flow16Layer&, int l, long long, int lr, int j, float sv, float x) const {
-    s[l * SAVED16_F32 + lr * KP16 + j] = sv;
-    xs[l * SAVED16_F32 + lr * KP16 + j] = x;
+    s[at(l, lr, j)] = sv;
+    xs[at(l, lr, j)] = x;
   }
   __device__ void put_frag(const Flow16Layer&, int l, int ks, int lane, const bf16x8& v) const {
     *(bf16x8*)(xp + l * SAVED16_BF + (ks * 64 + lane) * 8) = v;
   }
   __device__ void get(const Flow16Layer&, int l, long long, int lr, int j, float& sv, float& x) const {
-    sv = s[l * SAVED16_F32 + lr * KP16 + j];
-    x = xs[l * SAVED16_F32 + lr * KP16 + j];
+    sv = s[at(l, lr, j)];
+    x = xs[at(l, lr, j)];
   }
   __device__ bf16x8 get_frag(const Flow16Layer&, int l, int ks, int lane) const {
     return *(const bf16x8*)(xp + l * SAVED16_BF + (ks * 64 + lane) * 8);
