@@ -30,6 +30,7 @@
 // flow_tanh, the clamp c tanh(s / c)), summed in the orders of this kernel's tiles, so results
 // agree with the per-step path to fp32 rounding (and the occasional bf16 rounding flip of an
 // operand), not bit for bit.
+#include <algorithm>
 #include <type_traits>
 
 #include "ut_flow.h"
@@ -172,7 +173,9 @@ __device__ inline void fwd16_load2(const Flow16Layer& ly, Fwd16Regs& g) {
 // backward kernels); LdsSaved: LDS of the fused gradient kernel, per layer 1024 of each in
 // the order of the lanes that write and read them (coordinate j = 8 w + 2 lg + e of row lr
 // at e 512 + 64 w + 16 lg + lr: consecutive lanes, consecutive banks), and 2 x 64 x 8 bf16
-// fragments.
+// fragments.  LdsSaved also keeps the hidden activations h (fp32, exactly the forward's
+// values) of the layers from h_from on, in the order of the lanes that compute them, so the
+// backward reads them instead of recomputing U and tanh.
 struct GlobalSaved {
   __device__ void put(const Flow16Layer& ly, int, long long r0, int lr, int j, float sv, float xs) const {
     g16(ly.s)[(r0 + lr) * KP16 + j] = sv;
@@ -188,13 +191,19 @@ struct GlobalSaved {
   __device__ bf16x8 get_frag(const Flow16Layer& ly, int, int ks, int lane) const {
     return *(const UT_GLOBAL16 bf16x8*)(g16(ly.xpg) + (((long long)blockIdx.x * 2 + ks) * 64 + lane) * 8);
   }
+  __device__ bool keeps_h(int) const { return false; }
+  __device__ void put_h(int, int, int, int, const f32x4&) const {}
+  __device__ f32x4 get_h(int, int, int, int) const { return (f32x4){0.0f, 0.0f, 0.0f, 0.0f}; }
 };
 constexpr int SAVED16_F32 = R16 * KP16;  // floats of s (and of x_a e^s) per layer
 constexpr int SAVED16_BF = 2 * 64 * 8;   // bf16 of the passive-input fragments per layer
+constexpr int SAVED16_H = R16 * HID16;   // floats of h per kept layer
 struct LdsSaved {
   float* s;
   float* xs;
   bf16_t* xp;
+  float* h;
+  int h_from;
   __device__ static int at(int l, int lr, int j) {
     return l * SAVED16_F32 + (j & 1) * 512 + (j >> 3) * 64 + ((j >> 1) & 3) * 16 + lr;
   }
@@ -211,6 +220,14 @@ struct LdsSaved {
   }
   __device__ bf16x8 get_frag(const Flow16Layer&, int l, int ks, int lane) const {
     return *(const bf16x8*)(xp + l * SAVED16_BF + (ks * 64 + lane) * 8);
+  }
+  __device__ bool keeps_h(int l) const { return l >= h_from; }
+  // hidden units 32 w + 16 t + 4 lg .. + 3 of row lr
+  __device__ void put_h(int l, int w, int t, int lane, const f32x4& v) const {
+    *(f32x4*)(h + (l - h_from) * SAVED16_H + ((w * 2 + t) * 64 + lane) * 4) = v;
+  }
+  __device__ f32x4 get_h(int l, int w, int t, int lane) const {
+    return *(const f32x4*)(h + (l - h_from) * SAVED16_H + ((w * 2 + t) * 64 + lane) * 4);
   }
 };
 
@@ -295,6 +312,7 @@ __device__ __forceinline__ float flow16_fwd_body(const float* __restrict__ z, co
       float hv[4];
 #pragma unroll
       for (int r = 0; r < 4; ++r) hv[r] = flow_tanh(u[t][r] + cb1[t][r]);
+      if (save.keeps_h(l)) save.put_h(l, w, t, lane, (f32x4){hv[0], hv[1], hv[2], hv[3]});
       uint2 pk;
       pk.x = pack2(hv[0], hv[1]);
       pk.y = pack2(hv[2], hv[3]);
@@ -512,15 +530,27 @@ __device__ __forceinline__ void flow16_bwd_body(int P, const Flow16Layer* __rest
     bf16x8 ob[4];
 #pragma unroll
     for (int ks = 0; ks < 4; ++ks) ob[ks] = *(const bf16x8*)(dO + lr * OS16 + 32 * ks + 8 * lg);
-    f32x4 dh[2], u[2];
+    f32x4 dh[2];
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
       dh[t] = (f32x4){0.0f, 0.0f, 0.0f, 0.0f};
-      u[t] = (f32x4){0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
       for (int ks = 0; ks < 4; ++ks) dh[t] = mfma16(g.a2t[t][ks], ob[ks], dh[t]);
+    }
+    // h: the forward's values where kept, else recomputed with the forward's arithmetic
+    f32x4 hk[2];
+    if (save.keeps_h(l)) {
 #pragma unroll
-      for (int ks = 0; ks < 2; ++ks) u[t] = mfma16(g.a1[t][ks], cy.xb[ks], u[t]);
+      for (int t = 0; t < 2; ++t) hk[t] = save.get_h(l, w, t, lane);
+    } else {
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        f32x4 u = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks) u = mfma16(g.a1[t][ks], cy.xb[ks], u);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) hk[t][r] = flow_tanh(u[r] + g.b1[t][r]);
+      }
     }
     bf16x8 xb_n[2];
     __builtin_amdgcn_sched_barrier(0);
@@ -533,7 +563,7 @@ __device__ __forceinline__ void flow16_bwd_body(int P, const Flow16Layer* __rest
       float du[4];
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const float hv = flow_tanh(u[t][r] + g.b1[t][r]);
+        const float hv = hk[t][r];
         du[r] = dh[t][r] * (1.0f - hv * hv);
       }
       uint2 pk;
@@ -656,19 +686,21 @@ template <int LT, class RM, bool STEP>
 __global__ __launch_bounds__(THREADS16) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_flow16_grad(
     const float* __restrict__ z, const float* __restrict__ prm, const float* __restrict__ ldbase, int P,
     const Flow16Layer* __restrict__ layers, int L_, float c, RM rm, float* __restrict__ lj_rm,
-    float* __restrict__ gz, Flow16Step stp) {
+    float* __restrict__ gz, Flow16Step stp, int h_from) {
   const int L = LT > 0 ? LT : L_;
   extern __shared__ __attribute__((aligned(16))) float lds16[];
+  // the forward's bf16 tiles and the backward's share their space (dead once the forward ends)
   float* Xs = lds16;                          // 16 x P fp32 rows
-  bf16_t* Xp = (bf16_t*)(Xs + R16 * P);       // 16 x XPS16 passive inputs (bf16)
-  bf16_t* Hs = Xp + R16 * XPS16;              // 16 x HS16 hidden activations (bf16)
+  bf16_t* Xp = (bf16_t*)(Xs + R16 * P);       // forward: 16 x XPS16 passive inputs (bf16)
+  bf16_t* dO = Xp;                            // backward: 16 x OS16 (OS16 >= XPS16)
+  bf16_t* Hs = Xp + R16 * OS16;               // forward: 16 x HS16 hidden activations (bf16)
+  bf16_t* DU = Hs;                            // backward: 16 x HS16
   float* Gs = (float*)(Hs + R16 * HS16);      // 16 x P fp32 gradient rows
-  bf16_t* dO = (bf16_t*)(Gs + R16 * P);       // 16 x OS16
-  bf16_t* DU = dO + R16 * OS16;               // 16 x HS16
-  float* Red = (float*)(DU + R16 * HS16);     // 8 x 16 log-determinant partials
+  float* Red = Gs + R16 * P;                  // 8 x 16 log-determinant partials
   float* LJs = Red + NW16 * 16;               // 16 model log densities
-  float* sav = LJs + R16;                     // L x (s, x_a e^s) 16 x 64 fp32, then L x fragments
-  const LdsSaved save{sav, sav + L * SAVED16_F32, (bf16_t*)(sav + 2 * L * SAVED16_F32)};
+  float* sav = LJs + R16;                     // L x (s, x_a e^s) 16 x 64 fp32, L x fragments, kept h
+  bf16_t* xpf = (bf16_t*)(sav + 2 * L * SAVED16_F32);
+  const LdsSaved save{sav, sav + L * SAVED16_F32, xpf, (float*)(xpf + L * SAVED16_BF), h_from};
   const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63, lr = lane & 15;
   const long long r0 = (long long)blockIdx.x * R16;
   float ldp = flow16_fwd_body<LT>(z, prm, P, layers, L, c, Xs, Xp, Hs, save);
@@ -720,7 +752,7 @@ Bwd16Fn bwd16_kernel(int L) {
 }
 template <class RM>
 using Grad16Fn = void (*)(const float*, const float*, const float*, int, const Flow16Layer*, int, float, RM, float*,
-                          float*, Flow16Step);
+                          float*, Flow16Step, int);
 template <class RM, bool STEP>
 Grad16Fn<RM> grad16_kernel(int L) {
   switch (L) {
@@ -731,10 +763,12 @@ Grad16Fn<RM> grad16_kernel(int L) {
     default: return k_flow16_grad<0, RM, STEP>;
   }
 }
-size_t grad16_lds(int P, int L) {
-  return (size_t)R16 * P * 4 + (size_t)R16 * XPS16 * 2 + (size_t)R16 * HS16 * 2 + (size_t)R16 * P * 4 +
-         (size_t)R16 * OS16 * 2 + (size_t)R16 * HS16 * 2 + (size_t)(NW16 * 16 + R16) * 4 +
-         (size_t)L * (2 * SAVED16_F32 * 4 + SAVED16_BF * 2);
+static_assert(OS16 >= XPS16, "the backward's output-gradient tile holds the forward's passive-input tile");
+// the fused kernel's LDS with the hidden activations of nh layers kept
+size_t grad16_lds(int P, int L, int nh) {
+  return (size_t)R16 * P * 4 + (size_t)R16 * OS16 * 2 + (size_t)R16 * HS16 * 2 + (size_t)R16 * P * 4 +
+         (size_t)(NW16 * 16 + R16) * 4 + (size_t)L * (2 * SAVED16_F32 * 4 + SAVED16_BF * 2) +
+         (size_t)nh * SAVED16_H * 4;
 }
 size_t fwd16_lds(int P) { return (size_t)R16 * P * 4 + (size_t)R16 * XPS16 * 2 + (size_t)R16 * HS16 * 2 + NW16 * 16 * 4; }
 size_t bwd16_lds(int P) { return (size_t)R16 * P * 4 + (size_t)R16 * OS16 * 2 + (size_t)R16 * HS16 * 2; }
@@ -794,8 +828,12 @@ hipError_t flow16_grad_step(Flow& f, const float* z, const FlowRowModel& m, floa
                            const Flow16Step* stp, hipStream_t st) {
   if (B % R16 || !lj) return hipErrorInvalidValue;
   if (stp && (!stp->wq || !stp->wp || !stp->im_f || !stp->valid)) return hipErrorInvalidValue;
-  const size_t lds = grad16_lds(f.P, f.L);
-  if (lds > 160 * 1024) return hipErrorNotSupported;
+  constexpr size_t kLds = 160 * 1024;
+  const size_t base = grad16_lds(f.P, f.L, 0);
+  if (base > kLds) return hipErrorNotSupported;
+  // keep the hidden activations of as many of the last layers as LDS holds
+  const int nh = (int)std::min<size_t>((size_t)f.L, (kLds - base) / (SAVED16_H * 4));
+  const size_t lds = grad16_lds(f.P, f.L, nh);
   auto launch = [&](auto fm, auto step) {
     fm.dim = m.dim;
     fm.v_scale = m.v_scale;
@@ -804,7 +842,7 @@ hipError_t flow16_grad_step(Flow& f, const float* z, const FlowRowModel& m, floa
     const auto kern = grad16_kernel<FM, decltype(step)::value>(f.L);
     if (lds > 65536) (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     hipLaunchKernelGGL(kern, dim3(B / R16), dim3(THREADS16), lds, st, z, f.params, f.ldbase, f.P, f.d16, f.L, f.clamp,
-                       fm, lj, g_z, stp ? *stp : Flow16Step{});
+                       fm, lj, g_z, stp ? *stp : Flow16Step{}, f.L - nh);
     return hipGetLastError();
   };
   using T = std::true_type;
