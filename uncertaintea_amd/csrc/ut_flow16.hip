@@ -437,16 +437,20 @@ struct Bwd16Regs {
   bf16x8 a1t[8];     // W1^T k-steps (wavefront w & 3's tile)
   float b1[2][4];
 };
-__device__ inline void bwd16_load(const Flow16Layer& ly, Bwd16Regs& g) {
+// the weights of layer `ly`; W1 (for the recomputation of U) only where `need_a1`
+__device__ inline void bwd16_load(const Flow16Layer& ly, Bwd16Regs& g, bool need_a1) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, lg = lane >> 4;
   const UT_GLOBAL16 bf16_t* W1f = g16(ly.W1f);
   const UT_GLOBAL16 bf16_t* W2Tf = g16(ly.W2Tf);
   const UT_GLOBAL16 bf16_t* W1Tf = g16(ly.W1Tf);
+  if (need_a1)
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+      for (int ks = 0; ks < 2; ++ks)
+        g.a1[t][ks] = *(const UT_GLOBAL16 bf16x8*)(W1f + (((w * 2 + t) * 2 + ks) * 64 + lane) * 8);
 #pragma unroll
   for (int t = 0; t < 2; ++t) {
-#pragma unroll
-    for (int ks = 0; ks < 2; ++ks)
-      g.a1[t][ks] = *(const UT_GLOBAL16 bf16x8*)(W1f + (((w * 2 + t) * 2 + ks) * 64 + lane) * 8);
 #pragma unroll
     for (int ks = 0; ks < 4; ++ks)
       g.a2t[t][ks] = *(const UT_GLOBAL16 bf16x8*)(W2Tf + (((w * 2 + t) * 4 + ks) * 64 + lane) * 8);
@@ -474,8 +478,8 @@ __device__ __forceinline__ void flow16_bwd_prologue(const Flow16Layer* __restric
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, lr = lane & 15, lg = lane >> 4;
   const long long r0 = (long long)blockIdx.x * R16;
   const int j0 = 8 * w + 2 * lg;
-  bwd16_load(layers[L - 1], cy.ga);
-  if (L > 1) bwd16_load(layers[L - 2], cy.gb);
+  bwd16_load(layers[L - 1], cy.ga, !save.keeps_h(L - 1));
+  if (L > 1) bwd16_load(layers[L - 2], cy.gb, !save.keeps_h(L - 2));
 #pragma unroll
   for (int e = 0; e < 2; ++e) save.get(layers[L - 1], L - 1, r0, lr, j0 + e, cy.sv[e], cy.xsv[e]);
 #pragma unroll
@@ -589,7 +593,7 @@ __device__ __forceinline__ void flow16_bwd_body(int P, const Flow16Layer* __rest
         if (k < np) Gs[lr * P + 2 * k + par] += gp[r];
       }
     }
-    if (l > 1) bwd16_load(nx2, g);
+    if (l > 1) bwd16_load(nx2, g, !save.keeps_h(l - 2));
     lds_barrier16();
     if (next) {
 #pragma unroll
