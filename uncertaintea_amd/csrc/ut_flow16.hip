@@ -151,9 +151,11 @@ __device__ inline void fwd16_load1(const Flow16Layer& ly, Fwd16Regs& g) {
 #pragma unroll
     for (int r = 0; r < 4; ++r) g.b1[t][r] = b1[32 * w + 16 * t + 4 * lg + r];
 }
-// the second GEMM's weights and the coupling biases of this lane's coordinates
+// the second GEMM's weights and the coupling biases of this lane's coordinates (none on a
+// wavefront whose outputs are all padding)
 __device__ inline void fwd16_load2(const Flow16Layer& ly, Fwd16Regs& g) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, lg = lane >> 4;
+  if (16 * __builtin_amdgcn_readfirstlane(w) >= 2 * ly.na) return;
   const UT_GLOBAL16 bf16_t* W2f = g16(ly.W2f);
 #pragma unroll
   for (int ks = 0; ks < 8; ++ks) g.a2[ks] = *(const UT_GLOBAL16 bf16x8*)(W2f + ((w * 8 + ks) * 64 + lane) * 8);
