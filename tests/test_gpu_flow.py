@@ -275,14 +275,16 @@ def test_flow_graph_replay_matches_eager(monkeypatch):
         assert np.array_equal(a.integration_steps, b.integration_steps)
 
 
-@pytest.mark.parametrize("dim,layers", [(10, 4), (100, 6)])
+@pytest.mark.parametrize("dim,layers", [(10, 4), (100, 6), (127, 8)])
 def test_fused_leapfrog_step_matches_separate_kernel(monkeypatch, dim, layers):
-    """With a flow on the 16-row kernels the lock-step samplers take each leapfrog step
-    after the first in the flow backward's epilogue, and the funnel's gradient comes from
-    the flow forward's epilogue; with UT_FUSED_STEP=0 and UT_FUSED_MODEL=0 the step is
-    k_ls_step after the backward and the model the density kernel after the forward.  Same
-    arithmetic: whole shared-HMC and ChEES runs must agree bit for bit (the funnel at the
-    benchmark's width too)."""
+    """With a flow on the 16-row kernels the lock-step samplers run each leapfrog step of
+    Neal's funnel as one kernel (forward, model, backward, kick and drift; a trajectory's
+    last gradient with the log-determinant instead of the step); with UT_FUSED_STEP=0 and
+    UT_FUSED_MODEL=0 they run the separate forward, density, backward and step kernels.
+    Same arithmetic: whole shared-HMC and ChEES runs must agree bit for bit.  The fused
+    kernel keeps the hidden activations of as many of the last layers as LDS holds and
+    recomputes the others: all of them at dim 10 and at the benchmark's dim 100, four of
+    eight at dim 127 (P = 128, the widest)."""
     say("fused leapfrog step and row model", dim, layers)
     model = M.NealsFunnel(dim, 3.0)
     flow = ut.AffineCouplingFlow(model.num_params, layers, 256, seed=5)

@@ -177,7 +177,8 @@ __device__ inline void fwd16_load2(const Flow16Layer& ly, Fwd16Regs& g) {
 // at e 512 + 64 w + 16 lg + lr: consecutive lanes, consecutive banks), and 2 x 64 x 8 bf16
 // fragments.  LdsSaved also keeps the hidden activations h (fp32, exactly the forward's
 // values) of the layers from h_from on, in the order of the lanes that compute them, so the
-// backward reads them instead of recomputing U and tanh.
+// backward reads them instead of recomputing U and tanh (and those layers need no passive-input
+// fragments); its s and x_a e^s hold `slots` floats per layer, 128 per wavefront with outputs.
 struct GlobalSaved {
   __device__ void put(const Flow16Layer& ly, int, long long r0, int lr, int j, float sv, float xs) const {
     g16(ly.s)[(r0 + lr) * KP16 + j] = sv;
@@ -197,17 +198,17 @@ struct GlobalSaved {
   __device__ void put_h(int, int, int, int, const f32x4&) const {}
   __device__ f32x4 get_h(int, int, int, int) const { return (f32x4){0.0f, 0.0f, 0.0f, 0.0f}; }
 };
-constexpr int SAVED16_F32 = R16 * KP16;  // floats of s (and of x_a e^s) per layer
 constexpr int SAVED16_BF = 2 * 64 * 8;   // bf16 of the passive-input fragments per layer
 constexpr int SAVED16_H = R16 * HID16;   // floats of h per kept layer
 struct LdsSaved {
   float* s;
   float* xs;
-  bf16_t* xp;
-  float* h;
+  bf16_t* xp;  // layers below h_from
+  float* h;    // layers from h_from on
   int h_from;
-  __device__ static int at(int l, int lr, int j) {
-    return l * SAVED16_F32 + (j & 1) * 512 + (j >> 3) * 64 + ((j >> 1) & 3) * 16 + lr;
+  int slots;
+  __device__ int at(int l, int lr, int j) const {
+    return l * slots + (j & 1) * (slots >> 1) + (j >> 3) * 64 + ((j >> 1) & 3) * 16 + lr;
   }
   __device__ void put(const Flow16Layer&, int l, long long, int lr, int j, float sv, float x) const {
     s[at(l, lr, j)] = sv;
@@ -290,7 +291,7 @@ __device__ __forceinline__ float flow16_fwd_body(const float* __restrict__ z, co
     bf16x8 xb[2];
 #pragma unroll
     for (int ks = 0; ks < 2; ++ks) xb[ks] = *(const bf16x8*)(Xp + lr * XPS16 + 32 * ks + 8 * lg);
-    if (wu == 0) {
+    if (wu == 0 && !save.keeps_h(l)) {
 #pragma unroll
       for (int ks = 0; ks < 2; ++ks) save.put_frag(ly, l, ks, lane, xb[ks]);
     }
@@ -485,7 +486,8 @@ __device__ __forceinline__ void flow16_bwd_prologue(const Flow16Layer* __restric
 #pragma unroll
   for (int e = 0; e < 2; ++e) save.get(layers[L - 1], L - 1, r0, lr, j0 + e, cy.sv[e], cy.xsv[e]);
 #pragma unroll
-  for (int ks = 0; ks < 2; ++ks) cy.xb[ks] = save.get_frag(layers[L - 1], L - 1, ks, lane);
+  for (int ks = 0; ks < 2; ++ks)
+    cy.xb[ks] = save.keeps_h(L - 1) ? (bf16x8){} : save.get_frag(layers[L - 1], L - 1, ks, lane);
 }
 
 // The backward through every layer (gradient rows in Gs, in place): coupling backward,
@@ -558,9 +560,9 @@ __device__ __forceinline__ void flow16_bwd_body(int P, const Flow16Layer* __rest
         for (int r = 0; r < 4; ++r) hk[t][r] = flow_tanh(u[r] + g.b1[t][r]);
       }
     }
-    bf16x8 xb_n[2];
+    bf16x8 xb_n[2] = {};
     __builtin_amdgcn_sched_barrier(0);
-    if (next) {
+    if (next && !save.keeps_h(l - 1)) {
 #pragma unroll
       for (int ks = 0; ks < 2; ++ks) xb_n[ks] = save.get_frag(nx, l - 1, ks, lane);
     }
@@ -692,21 +694,23 @@ template <int LT, class RM, bool STEP>
 __global__ __launch_bounds__(THREADS16) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_flow16_grad(
     const float* __restrict__ z, const float* __restrict__ prm, const float* __restrict__ ldbase, int P,
     const Flow16Layer* __restrict__ layers, int L_, float c, RM rm, float* __restrict__ lj_rm,
-    float* __restrict__ gz, Flow16Step stp, int h_from) {
+    float* __restrict__ gz, Flow16Step stp, int h_from, int slots) {
   const int L = LT > 0 ? LT : L_;
   extern __shared__ __attribute__((aligned(16))) float lds16[];
-  // the forward's bf16 tiles and the backward's share their space (dead once the forward ends)
-  float* Xs = lds16;                          // 16 x P fp32 rows
+  // the forward's bf16 tiles and the backward's share their space (dead once the forward ends);
+  // the gradient rows replace the rows in place (the row model reads each element and writes
+  // its gradient from the same lane, and nothing reads the rows afterwards)
+  float* Xs = lds16;                          // 16 x P fp32 rows, then their gradient rows
+  float* Gs = Xs;
   bf16_t* Xp = (bf16_t*)(Xs + R16 * P);       // forward: 16 x XPS16 passive inputs (bf16)
   bf16_t* dO = Xp;                            // backward: 16 x OS16 (OS16 >= XPS16)
   bf16_t* Hs = Xp + R16 * OS16;               // forward: 16 x HS16 hidden activations (bf16)
   bf16_t* DU = Hs;                            // backward: 16 x HS16
-  float* Gs = (float*)(Hs + R16 * HS16);      // 16 x P fp32 gradient rows
-  float* Red = Gs + R16 * P;                  // 8 x 16 log-determinant partials
+  float* Red = (float*)(Hs + R16 * HS16);     // 8 x 16 log-determinant partials
   float* LJs = Red + NW16 * 16;               // 16 model log densities
-  float* sav = LJs + R16;                     // L x (s, x_a e^s) 16 x 64 fp32, L x fragments, kept h
-  bf16_t* xpf = (bf16_t*)(sav + 2 * L * SAVED16_F32);
-  const LdsSaved save{sav, sav + L * SAVED16_F32, xpf, (float*)(xpf + L * SAVED16_BF), h_from};
+  float* sav = LJs + R16;                     // L x (s, x_a e^s), fragments below h_from, kept h
+  bf16_t* xpf = (bf16_t*)(sav + 2 * L * slots);
+  const LdsSaved save{sav, sav + L * slots, xpf, (float*)(xpf + h_from * SAVED16_BF), h_from, slots};
   const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63, lr = lane & 15;
   const long long r0 = (long long)blockIdx.x * R16;
   float ldp = flow16_fwd_body<LT>(z, prm, P, layers, L, c, Xs, Xp, Hs, save);
@@ -758,7 +762,7 @@ Bwd16Fn bwd16_kernel(int L) {
 }
 template <class RM>
 using Grad16Fn = void (*)(const float*, const float*, const float*, int, const Flow16Layer*, int, float, RM, float*,
-                          float*, Flow16Step, int);
+                          float*, Flow16Step, int, int);
 template <class RM, bool STEP>
 Grad16Fn<RM> grad16_kernel(int L) {
   switch (L) {
@@ -770,11 +774,11 @@ Grad16Fn<RM> grad16_kernel(int L) {
   }
 }
 static_assert(OS16 >= XPS16, "the backward's output-gradient tile holds the forward's passive-input tile");
-// the fused kernel's LDS with the hidden activations of nh layers kept
-size_t grad16_lds(int P, int L, int nh) {
-  return (size_t)R16 * P * 4 + (size_t)R16 * OS16 * 2 + (size_t)R16 * HS16 * 2 + (size_t)R16 * P * 4 +
-         (size_t)(NW16 * 16 + R16) * 4 + (size_t)L * (2 * SAVED16_F32 * 4 + SAVED16_BF * 2) +
-         (size_t)nh * SAVED16_H * 4;
+// the fused kernel's LDS with the hidden activations of the last nh layers kept and `slots`
+// saved s (and x_a e^s) per layer
+size_t grad16_lds(int P, int L, int nh, int slots) {
+  return (size_t)R16 * P * 4 + (size_t)R16 * OS16 * 2 + (size_t)R16 * HS16 * 2 + (size_t)(NW16 * 16 + R16) * 4 +
+         (size_t)L * 2 * slots * 4 + (size_t)(L - nh) * SAVED16_BF * 2 + (size_t)nh * SAVED16_H * 4;
 }
 size_t fwd16_lds(int P) { return (size_t)R16 * P * 4 + (size_t)R16 * XPS16 * 2 + (size_t)R16 * HS16 * 2 + NW16 * 16 * 4; }
 size_t bwd16_lds(int P) { return (size_t)R16 * P * 4 + (size_t)R16 * OS16 * 2 + (size_t)R16 * HS16 * 2; }
@@ -835,11 +839,15 @@ hipError_t flow16_grad_step(Flow& f, const float* z, const FlowRowModel& m, floa
   if (B % R16 || !lj) return hipErrorInvalidValue;
   if (stp && (!stp->wq || !stp->wp || !stp->im_f || !stp->valid)) return hipErrorInvalidValue;
   constexpr size_t kLds = 160 * 1024;
-  const size_t base = grad16_lds(f.P, f.L, 0);
-  if (base > kLds) return hipErrorNotSupported;
+  // saved s and x_a e^s: 128 per wavefront that has outputs (16 w < 2 na) in some layer
+  int na_max = 0;
+  for (const FlowLayer& fl : f.layers) na_max = std::max(na_max, fl.na);
+  const int slots = 128 * ((2 * na_max + 15) / 16);
+  if (grad16_lds(f.P, f.L, 0, slots) > kLds) return hipErrorNotSupported;
   // keep the hidden activations of as many of the last layers as LDS holds
-  const int nh = (int)std::min<size_t>((size_t)f.L, (kLds - base) / (SAVED16_H * 4));
-  const size_t lds = grad16_lds(f.P, f.L, nh);
+  int nh = f.L;
+  while (nh > 0 && grad16_lds(f.P, f.L, nh, slots) > kLds) --nh;
+  const size_t lds = grad16_lds(f.P, f.L, nh, slots);
   auto launch = [&](auto fm, auto step) {
     fm.dim = m.dim;
     fm.v_scale = m.v_scale;
@@ -848,7 +856,7 @@ hipError_t flow16_grad_step(Flow& f, const float* z, const FlowRowModel& m, floa
     const auto kern = grad16_kernel<FM, decltype(step)::value>(f.L);
     if (lds > 65536) (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     hipLaunchKernelGGL(kern, dim3(B / R16), dim3(THREADS16), lds, st, z, f.params, f.ldbase, f.P, f.d16, f.L, f.clamp,
-                       fm, lj, g_z, stp ? *stp : Flow16Step{}, f.L - nh);
+                       fm, lj, g_z, stp ? *stp : Flow16Step{}, f.L - nh, slots);
     return hipGetLastError();
   };
   using T = std::true_type;
