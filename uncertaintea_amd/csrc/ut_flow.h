@@ -174,16 +174,18 @@ struct Flow16Step {
   uint8_t* valid = nullptr;
   float eps = 0.0f;
 };
-// flow16_backward with the leapfrog step in its epilogue: g_z is written as by
-// flow16_backward, then for every row of a still-valid chain either the step is taken
-// (all of the row's g_z finite) or the chain is marked invalid.
+// flow16_backward with the leapfrog step in its epilogue: for every row of a still-valid chain
+// either the step is taken with the gradient flow16_backward would write (all of the row's
+// entries finite) or the chain is marked invalid; g_z is not written (nothing reads a
+// mid-trajectory gradient after its kick).
 hipError_t flow16_backward_step(Flow& f, const float* g_q, float* g_z, int B, const Flow16Step& stp,
                                 hipStream_t st);
 // The whole flow-wrapped gradient of a row model in one kernel per 16-row tile (forward with its
-// saved values in LDS, the row model, backward), g_z the gradient rows: with a step, followed by
-// the leapfrog step (the arithmetic of flow16_forward_model + flow16_backward_step; lj receives
-// the model's log density), without one (stp null) lj receives the model's log density plus
-// the flow's log-determinant (flow_forward_model + flow16_backward + flow_add_logdet).
+// saved values in LDS, the row model, backward): with a step, followed by the leapfrog step
+// (the arithmetic of flow16_forward_model + flow16_backward_step; lj receives the model's log
+// density, g_z is not written), without one (stp null) lj receives the model's log density
+// plus the flow's log-determinant and g_z the gradient rows (flow_forward_model +
+// flow16_backward + flow_add_logdet).
 // hipErrorNotSupported when the layers' saved values do not fit in LDS.
 hipError_t flow16_grad_step(Flow& f, const float* z, const FlowRowModel& m, float* lj, float* g_z, int B,
                            const Flow16Step* stp, hipStream_t st);

@@ -624,7 +624,9 @@ __device__ __forceinline__ void flow16_bwd_body(int P, const Flow16Layer* __rest
 
 // base backward g_z = g_x e^ls of the workgroup's rows, and with STEP the lock-step
 // leapfrog's full kick and drift (k_ls_step): rows whose gradient is finite move, a chain
-// with a non-finite entry is marked invalid, invalid chains stay
+// with a non-finite entry is marked invalid, invalid chains stay.  With STEP the gradient is
+// used only by the kick, so g_z is not written (the trajectory's last gradient, without
+// STEP, is the one the samplers read).
 template <bool STEP>
 __device__ __forceinline__ void flow16_bwd_base(const float* __restrict__ prm, int P, const float* Gs, float* gz,
                                                 const Flow16Step& stp) {
@@ -644,8 +646,11 @@ __device__ __forceinline__ void flow16_bwd_base(const float* __restrict__ prm, i
       const int e = e0 + THREADS16 * k;
       if (e < R16 * P) {
         const float gv = Gs[e] * expf(lv[k]);
-        gz[r0 * P + e] = gv;
-        if (STEP && !isfinite_f(gv)) row_ok[e / P] = 0;
+        if (STEP) {
+          if (!isfinite_f(gv)) row_ok[e / P] = 0;
+        } else {
+          gz[r0 * P + e] = gv;
+        }
       }
     }
   }
