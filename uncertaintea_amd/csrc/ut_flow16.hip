@@ -633,9 +633,25 @@ __device__ __forceinline__ void flow16_bwd_base(const float* __restrict__ prm, i
   const int tid = threadIdx.x;
   const long long r0 = (long long)blockIdx.x * R16;
   __shared__ int row_ok[R16];
+  // with STEP, this thread's momenta, positions, inverse masses and chain flags (elements
+  // tid + 512 k, P <= 128) are loaded first, so their latency overlaps the base backward; the
+  // barriers below wait for LDS only
+  constexpr int EPT = R16 * 128 / THREADS16;
+  float pw[EPT], pq[EPT], pim[EPT];
+  bool pok[EPT];
   if (STEP) {
+#pragma unroll
+    for (int k = 0; k < EPT; ++k) {
+      const int e = min(tid + THREADS16 * k, R16 * P - 1);
+      const int row = e / P, a = e - row * P;
+      const long long cr = r0 + row;
+      pw[k] = stp.wp[cr * P + a];
+      pq[k] = stp.wq[cr * P + a];
+      pim[k] = stp.im_f[a];
+      pok[k] = stp.valid[cr] != 0;
+    }
     if (tid < R16) row_ok[tid] = 1;
-    __syncthreads();
+    lds_barrier16();
   }
   for (int e0 = tid; e0 < R16 * P; e0 += 4 * THREADS16) {
     float lv[4];
@@ -655,16 +671,20 @@ __device__ __forceinline__ void flow16_bwd_base(const float* __restrict__ prm, i
     }
   }
   if (STEP) {
-    __syncthreads();
-    for (int e = tid; e < R16 * P; e += THREADS16) {
-      const int row = e / P, a = e - row * P;
-      const long long cr = r0 + row;
-      if (row_ok[row] && stp.valid[cr]) {
-        const float gv = Gs[e] * expf(prm[P + a]);
-        const float pv = stp.wp[cr * P + a] + (1.0f * stp.eps) * gv;
-        const float qv = stp.wq[cr * P + a] + (stp.eps * stp.im_f[a]) * pv;
-        stp.wq[cr * P + a] = qv;
-        stp.wp[cr * P + a] = pv;
+    lds_barrier16();
+#pragma unroll
+    for (int k = 0; k < EPT; ++k) {
+      const int e = tid + THREADS16 * k;
+      if (e < R16 * P) {
+        const int row = e / P, a = e - row * P;
+        const long long cr = r0 + row;
+        if (row_ok[row] && pok[k]) {
+          const float gv = Gs[e] * expf(prm[P + a]);
+          const float pv = pw[k] + (1.0f * stp.eps) * gv;
+          const float qv = pq[k] + (stp.eps * pim[k]) * pv;
+          stp.wq[cr * P + a] = qv;
+          stp.wp[cr * P + a] = pv;
+        }
       }
     }
     if (tid < R16 && !row_ok[tid] && stp.valid[r0 + tid]) stp.valid[r0 + tid] = 0;
