@@ -628,7 +628,7 @@ __device__ __forceinline__ void flow16_bwd_body(int P, const Flow16Layer* __rest
 // used only by the kick, so g_z is not written (the trajectory's last gradient, without
 // STEP, is the one the samplers read).
 template <bool STEP>
-__device__ __forceinline__ void flow16_bwd_base(const float* __restrict__ prm, int P, const float* Gs, float* gz,
+__device__ __forceinline__ void flow16_bwd_base(const float* __restrict__ prm, int P, float* Gs, float* gz,
                                                 const Flow16Step& stp) {
   const int tid = threadIdx.x;
   const long long r0 = (long long)blockIdx.x * R16;
@@ -663,6 +663,7 @@ __device__ __forceinline__ void flow16_bwd_base(const float* __restrict__ prm, i
       if (e < R16 * P) {
         const float gv = Gs[e] * expf(lv[k]);
         if (STEP) {
+          Gs[e] = gv;  // the kick's gradient, read back by this thread below
           if (!isfinite_f(gv)) row_ok[e / P] = 0;
         } else {
           gz[r0 * P + e] = gv;
@@ -679,7 +680,7 @@ __device__ __forceinline__ void flow16_bwd_base(const float* __restrict__ prm, i
         const int row = e / P, a = e - row * P;
         const long long cr = r0 + row;
         if (row_ok[row] && pok[k]) {
-          const float gv = Gs[e] * expf(prm[P + a]);
+          const float gv = Gs[e];
           const float pv = pw[k] + (1.0f * stp.eps) * gv;
           const float qv = pq[k] + (stp.eps * pim[k]) * pv;
           stp.wq[cr * P + a] = qv;
